@@ -1,0 +1,359 @@
+"""Python bindings of the oracle (CPU restatement of the FEM_CODES solve path).
+
+TEST INFRASTRUCTURE: only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module.  The GPU product
+(``fem_codes_amd``) never does.
+
+``Oracle`` wraps ``oracle/liboracle.so`` (order-faithful restatement, built
+``-O2 -ffp-contract=off``).  ``Reference`` wraps ``oracle/_ref/libref_ss2d.so``,
+the reference's own SS-2D routines compiled from the FEM_CODES tree, when that
+library has been built (only where the tree is present).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+_dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i, _d, _vp = C.c_int, C.c_double, C.c_void_p
+
+P_CONST, P_ROTATING, P_VALIDACAO = 0, 1, 2
+H_2SQRTAREA, H_SQRTAREA = 0, 1
+
+
+class OrcProb2d(C.Structure):
+    _fields_ = [("kind", C.c_int), ("kappa", C.c_double), ("gamma", C.c_double),
+                ("f", C.c_double), ("bx", C.c_double), ("by", C.c_double),
+                ("scale", C.c_double), ("h_shock", C.c_int)]
+
+
+OP_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+class OrcOps(C.Structure):
+    _fields_ = [("matvec", C.c_void_p), ("matvec_ctx", C.c_void_p),
+                ("precond", C.c_void_p), ("precond_ctx", C.c_void_p),
+                ("precondR", C.c_void_p), ("precondR_ctx", C.c_void_p)]
+
+
+class OrcCsrCtx(C.Structure):
+    _fields_ = [("neq", C.c_int), ("AA", C.c_void_p), ("JA", C.c_void_p), ("IA", C.c_void_p)]
+
+
+class OrcEbeCtx(C.Structure):
+    _fields_ = [("neq", C.c_int), ("nel", C.c_int), ("k", C.c_int),
+                ("A", C.c_void_p), ("lm", C.c_void_p)]
+
+
+class OrcEdeCtx(C.Structure):
+    _fields_ = [("neq", C.c_int), ("nedge", C.c_int), ("A", C.c_void_p), ("lm2", C.c_void_p)]
+
+
+class OrcDiagCtx(C.Structure):
+    _fields_ = [("neq", C.c_int), ("invDiag", C.c_void_p)]
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so (and _ref when the FEM_CODES tree is present)."""
+    out = subprocess.run(["make", "-s", "-C", HERE, "all"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """Thin ctypes layer over liboracle.so; all arrays are numpy."""
+
+    def __init__(self, fast: bool = False):
+        name = "liboracle_fast.so" if fast else "liboracle.so"
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            build()
+        L = self.lib = C.CDLL(path)
+        L.orc_ddot.restype = _d
+        L.orc_ddot.argtypes = [_i, _dp, _dp]
+        L.orc_csrmv.argtypes = [_i, _dp, _ip, _ip, _dp, _dp]
+        L.orc_ebemv.argtypes = [_i, _i, _i, _dp, _ip, _dp, _dp]
+        L.orc_ebemv_f32.argtypes = [_i, _i, _i, _fp, _ip, _fp, _fp]
+        L.orc_edemv.argtypes = [_i, _i, _dp, _ip, _dp, _dp]
+        L.orc_diag_apply.argtypes = [_i, _dp, _dp, _dp]
+        L.orc_diag_setup_ebe.argtypes = [_i, _i, _i, _dp, _ip, _dp, _dp, _dp]
+        L.orc_diag_setup_ede.argtypes = [_i, _i, _dp, _ip, _dp, _dp, _dp]
+        L.orc_diag_setup_csr.argtypes = [_i, _dp, _ip, _ip, _dp, _dp, _dp]
+        L.orc_jacobi_ebe_setup.argtypes = [_i, _i, _dp, _ip, _dp, _dp]
+        L.orc_jacobi_ebe_apply.argtypes = [_i, _i, _dp, _ip, _dp, _dp]
+        L.orc_pgmres.restype = _i
+        L.orc_pgmres.argtypes = [_i, _i, _i, _d, _dp, _dp, C.POINTER(OrcOps),
+                                 _vp, _i, _vp, C.POINTER(C.c_int)]
+        L.orc_mesh2d_structured.argtypes = [_i, _dp, _dp, _ip, _ip]
+        L.orc_number_nodes.restype = _i
+        L.orc_number_nodes.argtypes = [_i, _ip, _ip]
+        L.orc_fill_lm.argtypes = [_i, _i, _i, _ip, _ip, _ip, _ip]
+        L.orc_csr_init.restype = _i
+        L.orc_csr_init.argtypes = [_i, _i, _ip, _vp, _vp, _vp]
+        L.orc_ede_init.restype = _i
+        L.orc_ede_init.argtypes = [_i, _i, _ip, _vp, _vp, _vp]
+        L.orc_build_supg2d.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _dp,
+                                       C.POINTER(OrcProb2d), _dp, _dp]
+        L.orc_ede_assembly.argtypes = [_i, _i, _dp, _ip, _ip, _dp]
+        L.orc_csr_assembly.argtypes = [_i, _i, _dp, _ip, _dp]
+        for nm in ("orc_op_csr", "orc_op_ebe", "orc_op_ede", "orc_op_diag"):
+            getattr(L, nm).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+
+    # ---- BLAS / matvec ----
+    def ddot(self, x, y):
+        return self.lib.orc_ddot(len(x), x, y)
+
+    def csrmv(self, neq, AA, JA, IA, P):
+        Q = np.zeros(neq + 1)
+        self.lib.orc_csrmv(neq, AA, JA, IA, P, Q)
+        return Q
+
+    def ebemv(self, neq, nel, k, A, lm, P):
+        Q = np.zeros(neq + 1)
+        self.lib.orc_ebemv(neq, nel, k, A, lm, P, Q)
+        return Q
+
+    def ebemv_f32(self, neq, nel, k, A, lm, P):
+        Q = np.zeros(neq + 1, np.float32)
+        self.lib.orc_ebemv_f32(neq, nel, k, A, lm, P, Q)
+        return Q
+
+    def edemv(self, neq, nedge, A, lm2, P):
+        Q = np.zeros(neq + 1)
+        self.lib.orc_edemv(neq, nedge, A, lm2, P, Q)
+        return Q
+
+    # ---- preconditioner setups: return (Diag, invDiag, F_scaled) ----
+    def diag_setup_ebe(self, neq, nel, k, A, lm, F):
+        D, iD, F2 = np.zeros(neq + 1), np.zeros(neq + 1), F.copy()
+        self.lib.orc_diag_setup_ebe(neq, nel, k, A, lm, D, iD, F2)
+        return D, iD, F2
+
+    def diag_setup_ede(self, neq, nedge, A, lm2, F):
+        D, iD, F2 = np.zeros(neq + 1), np.zeros(neq + 1), F.copy()
+        self.lib.orc_diag_setup_ede(neq, nedge, A, lm2, D, iD, F2)
+        return D, iD, F2
+
+    def diag_setup_csr(self, neq, AA, JA, IA, F):
+        D, iD, F2 = np.zeros(neq + 1), np.zeros(neq + 1), F.copy()
+        self.lib.orc_diag_setup_csr(neq, AA, JA, IA, D, iD, F2)
+        return D, iD, F2
+
+    def jacobi_ebe_setup(self, neq, nel, A, lm, F):
+        invDe, F2 = np.zeros(3 * nel), F.copy()
+        self.lib.orc_jacobi_ebe_setup(neq, nel, A, lm, invDe, F2)
+        return invDe, F2
+
+    def jacobi_ebe_apply(self, neq, nel, invDe, lm, p):
+        z = np.zeros(neq + 1)
+        self.lib.orc_jacobi_ebe_apply(neq, nel, invDe, lm, p, z)
+        return z
+
+    # ---- GMRES over a stored operator ----
+    def pgmres(self, neq, kmax, lmax, tol, B, scheme, mats, invDiag=None, hist_cap=100000):
+        """scheme: 'CSR' (mats=(AA,JA,IA)), 'EBE' (mats=(A,lm,k,nel)),
+        'EDE' (mats=(A,lm2,nedge)).  invDiag: Diag left preconditioner or None.
+        Returns (X, steps, cycles, rho_hist, e0_hist)."""
+        L = self.lib
+        keep = []
+        if scheme == "CSR":
+            AA, JA, IA = mats
+            ctx = OrcCsrCtx(neq, _ptr(AA), _ptr(JA), _ptr(IA)); fn = L.orc_op_csr
+        elif scheme == "EBE":
+            A, lm, k, nel = mats
+            ctx = OrcEbeCtx(neq, nel, k, _ptr(A), _ptr(lm)); fn = L.orc_op_ebe
+        else:
+            A, lm2, nedge = mats
+            ctx = OrcEdeCtx(neq, nedge, _ptr(A), _ptr(lm2)); fn = L.orc_op_ede
+        keep.append(ctx)
+        ops = OrcOps()
+        ops.matvec = C.cast(fn, C.c_void_p)
+        ops.matvec_ctx = C.cast(C.pointer(ctx), C.c_void_p)
+        if invDiag is not None:
+            dctx = OrcDiagCtx(neq, _ptr(invDiag)); keep.append(dctx)
+            ops.precond = C.cast(L.orc_op_diag, C.c_void_p)
+            ops.precond_ctx = C.cast(C.pointer(dctx), C.c_void_p)
+        X = np.zeros(neq + 1)
+        rho = np.zeros(hist_cap)
+        e0 = np.zeros(lmax + 1)
+        ncyc = C.c_int(0)
+        steps = L.orc_pgmres(neq, kmax, lmax, tol, np.ascontiguousarray(B), X, C.byref(ops),
+                             _ptr(rho), hist_cap, _ptr(e0), C.byref(ncyc))
+        return X, steps, ncyc.value, rho[:min(steps, hist_cap)].copy(), e0[:ncyc.value].copy()
+
+    # ---- mesh / structures / assembly ----
+    def mesh2d(self, n):
+        nn, nel = (n + 1) ** 2, 2 * n * n
+        x, y = np.zeros(nn), np.zeros(nn)
+        t, v = np.zeros(nn, np.int32), np.zeros(3 * nel, np.int32)
+        self.lib.orc_mesh2d_structured(n, x, y, t, v)
+        return x, y, t, v
+
+    def number_nodes(self, type_):
+        idv = np.zeros(len(type_), np.int32)
+        neq = self.lib.orc_number_nodes(len(type_), type_, idv)
+        return neq, idv
+
+    def fill_lm(self, neq, nel, vertex, type_, idv, nnoel=3):
+        lm = np.zeros(nnoel * nel, np.int32)
+        self.lib.orc_fill_lm(neq, nel, nnoel, vertex, type_, idv, lm)
+        return lm
+
+    def csr_init(self, neq, nel, lm):
+        nnz = self.lib.orc_csr_init(neq, nel, lm, None, None, None)
+        IA, JA = np.zeros(neq + 1, np.int32), np.zeros(nnz, np.int32)
+        cbe = np.zeros(9 * nel, np.int32)
+        self.lib.orc_csr_init(neq, nel, lm, _ptr(IA), _ptr(JA), _ptr(cbe))
+        return nnz, IA, JA, cbe
+
+    def ede_init(self, neq, nel, lm):
+        nedge = self.lib.orc_ede_init(neq, nel, lm, None, None, None)
+        order, ebe = np.zeros(3 * nel, np.int32), np.zeros(3 * nel, np.int32)
+        lm2 = np.zeros(2 * nedge, np.int32)
+        self.lib.orc_ede_init(neq, nel, lm, _ptr(order), _ptr(ebe), _ptr(lm2))
+        return nedge, order, ebe, lm2
+
+    def build_supg2d(self, x, y, type_, idv, vertex, neq, u, prob: OrcProb2d):
+        nel = len(vertex) // 3
+        ke, F = np.zeros(9 * nel), np.zeros(neq + 1)
+        self.lib.orc_build_supg2d(len(x), nel, neq, x, y, type_, idv, vertex, u,
+                                  C.byref(prob), ke, F)
+        return ke, F
+
+    def ede_assembly(self, nel, nedge, ke, order, ebe):
+        A = np.zeros(4 * (nedge + 1))
+        self.lib.orc_ede_assembly(nel, nedge, ke, order, ebe, A)
+        return A
+
+    def csr_assembly(self, nel, nnz, ke, cbe):
+        AA = np.zeros(nnz + 1)
+        self.lib.orc_csr_assembly(nel, nnz, ke, cbe, AA)
+        return AA
+
+
+def prob2d(kind=P_CONST, kappa=1e-4, gamma=0.0, f=1.0, bx=1.0, by=0.0, scale=1.0,
+           h_shock=H_2SQRTAREA) -> OrcProb2d:
+    return OrcProb2d(kind, kappa, gamma, f, bx, by, scale, h_shock)
+
+
+class Reference:
+    """The reference SS-2D routines compiled from their sources (oracle/_ref).
+
+    Available only where oracle/_ref/libref_ss2d.so was built."""
+
+    PATH = os.path.join(HERE, "_ref", "libref_ss2d.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.exists(cls.PATH)
+
+    def __init__(self, fast: bool = False):
+        path = self.PATH.replace(".so", "_fast.so") if fast else self.PATH
+        L = self.lib = C.CDLL(path)
+        L.ref_setup.restype = _i
+        L.ref_setup.argtypes = [_i, _i, _dp, _dp, _ip, _ip, C.c_char_p, C.c_char_p]
+        L.ref_sizes.argtypes = [C.POINTER(C.c_int)] * 3
+        L.ref_get_lm.argtypes = [_ip]
+        L.ref_get_ids.argtypes = [_ip]
+        L.ref_get_csr.argtypes = [_ip, _ip, _ip]
+        L.ref_get_ede.argtypes = [_ip, _ip, _ip]
+        L.ref_set_problem.argtypes = [_i, _d, _d, _d, _d, _d, _d, _i]
+        L.ref_build.argtypes = [_dp]
+        L.ref_get_F.argtypes = [_dp]
+        L.ref_set_F.argtypes = [_dp]
+        L.ref_get_matrix.argtypes = [_dp]
+        L.ref_set_matrix.argtypes = [_dp]
+        L.ref_matvec.argtypes = [_dp, _dp]
+        L.ref_precond_setup.argtypes = [C.c_char_p, _dp]
+        L.ref_get_invdiag.argtypes = [_dp, _dp]
+        L.ref_get_invde.argtypes = [_dp]
+        L.ref_precond_apply.argtypes = [_dp, _dp]
+        L.ref_pgmres.restype = _i
+        L.ref_pgmres.argtypes = [_i, _i, _d, _dp, _dp]
+        L.ref_ddot.restype = _d
+        L.ref_ddot.argtypes = [_i, _dp, _dp]
+        self.nel = 0
+
+    def setup(self, x, y, type_, vertex, scheme="EBE", reorder="NOT"):
+        self.nel = len(vertex) // 3
+        neq = self.lib.ref_setup(len(x), self.nel, x, y, type_, vertex,
+                                 scheme.encode(), reorder.encode())
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self.lib.ref_sizes(C.byref(a), C.byref(b), C.byref(c))
+        self.neq, self.nnz, self.nedge = a.value, b.value, c.value
+        self.scheme = scheme
+        return neq
+
+    def lm(self):
+        out = np.zeros(3 * self.nel, np.int32); self.lib.ref_get_lm(out); return out
+
+    def ids(self, nnodes):
+        out = np.zeros(nnodes, np.int32); self.lib.ref_get_ids(out); return out
+
+    def csr(self):
+        IA, JA = np.zeros(self.neq + 1, np.int32), np.zeros(self.nnz, np.int32)
+        cbe = np.zeros(9 * self.nel, np.int32)
+        self.lib.ref_get_csr(IA, JA, cbe)
+        return IA, JA, cbe
+
+    def ede(self):
+        o, ebe = np.zeros(3 * self.nel, np.int32), np.zeros(3 * self.nel, np.int32)
+        lm2 = np.zeros(2 * self.nedge, np.int32)
+        self.lib.ref_get_ede(o, ebe, lm2)
+        return o, ebe, lm2
+
+    def set_problem(self, p: OrcProb2d):
+        self.lib.ref_set_problem(p.kind, p.kappa, p.gamma, p.f, p.bx, p.by, p.scale, p.h_shock)
+
+    def matrix_size(self):
+        return {"EBE": 9 * self.nel, "EDE": 4 * (self.nedge + 1), "CSR": self.nnz + 1}[self.scheme]
+
+    def build(self, u):
+        self.lib.ref_build(np.ascontiguousarray(u, np.float64))
+        A, F = np.zeros(self.matrix_size()), np.zeros(self.neq + 1)
+        self.lib.ref_get_matrix(A); self.lib.ref_get_F(F)
+        return A, F
+
+    def set_matrix(self, A):
+        self.lib.ref_set_matrix(np.ascontiguousarray(A, np.float64))
+
+    def matvec(self, P):
+        P = P.copy(); Q = np.zeros(self.neq + 1)
+        self.lib.ref_matvec(P, Q)
+        return Q
+
+    def precond_setup(self, name, F):
+        F2 = F.copy(); self.lib.ref_precond_setup(name.encode(), F2)
+        return F2
+
+    def invdiag(self):
+        D, iD = np.zeros(self.neq + 1), np.zeros(self.neq + 1)
+        self.lib.ref_get_invdiag(D, iD)
+        return D, iD
+
+    def invde(self):
+        out = np.zeros(3 * self.nel); self.lib.ref_get_invde(out); return out
+
+    def precond_apply(self, p):
+        z = np.zeros(self.neq + 1); self.lib.ref_precond_apply(p.copy(), z); return z
+
+    def pgmres(self, kmax, lmax, tol, B):
+        X = np.zeros(self.neq + 1)
+        it = self.lib.ref_pgmres(kmax, lmax, tol, np.ascontiguousarray(B), X)
+        return X, it
+
+    def ddot(self, x, y):
+        return self.lib.ref_ddot(len(x), x.copy(), y.copy())

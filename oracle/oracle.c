@@ -1,0 +1,756 @@
+/*
+ * oracle.c -- CPU restatement of the FEM_CODES 2D solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libfemgpu, never the product.
+ * Build: oracle/Makefile, flags -O2 -ffp-contract=off (order-faithful: no FMA
+ * contraction, no reassociation), so that each routine reproduces the
+ * reference compiled with the same flags bit for bit.
+ *
+ * Pinning (how each function is checked; tests/test_oracle_*.py):
+ *   BLAS-1, csrmv, ebemv, edemv, Diag setups, Jacobi EBE, pgmres,
+ *   mesh numbering, Fill_LM, csr/ede initialisation, SUPG build,
+ *   ede/csr assembly:
+ *     - bitwise against oracle/_ref (the reference sources compiled from
+ *       the FEM_CODES tree by oracle/Makefile) on seeded meshes, and against
+ *       the fixtures in tests/golden/ that _ref wrote;
+ *     - independently of the reference: dense assembly of the three storage
+ *       schemes must agree, row sums of ke vanish for gamma=0 and u=0,
+ *       sum(Fe)=Area*f for constant f, GMRES against a dense LU solve, and
+ *       GMRES on diag(1..k) converges in k steps.
+ *   nnz / nedge closed forms for the structured mesh (SURVEY formulas).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ======================================================================== */
+/* BLAS-1                                                                    */
+/* ======================================================================== */
+
+/* ddot.c:4-19 : blocks of five products, summed left to right inside the
+ * block, then added to one running accumulator; scalar clean-up loop. */
+double orc_ddot(int n, const double *x, const double *y)
+{
+    double acc = 0.0;
+    long i = 0;
+    long last5 = (long)n - 4;
+    for (; i < last5; i += 5) {
+        double blk = x[i] * y[i];
+        blk = blk + x[i + 1] * y[i + 1];
+        blk = blk + x[i + 2] * y[i + 2];
+        blk = blk + x[i + 3] * y[i + 3];
+        blk = blk + x[i + 4] * y[i + 4];
+        acc = acc + blk;
+    }
+    for (; i < n; i++)
+        acc = acc + x[i] * y[i];
+    return acc;
+}
+
+/* daxpy.c:4-21 (the 4-way unroll does not change any result) */
+void orc_daxpy(int n, double a, const double *x, double *y)
+{
+    for (long i = 0; i < n; i++)
+        y[i] = y[i] + a * x[i];
+}
+
+/* dscal.c:4-21 */
+void orc_dscal(int n, double a, double *x)
+{
+    for (long i = 0; i < n; i++)
+        x[i] = a * x[i];
+}
+
+/* dcopy.c:4-19 */
+void orc_dcopy(int n, const double *x, double *y)
+{
+    if (x == y) return;
+    memmove(y, x, sizeof(double) * (size_t)(n > 0 ? n : 0));
+}
+
+/* dzero.c:3-18 */
+void orc_dzero(int n, double *x)
+{
+    for (long i = 0; i < n; i++)
+        x[i] = 0.0;
+}
+
+/* ======================================================================== */
+/* matrix-vector products                                                    */
+/* ======================================================================== */
+
+/* CSR/csrmv.c:3-24 : Q zeroed, then each row accumulated in slot order. */
+void orc_csrmv(int neq, const double *AA, const int *JA, const int *IA,
+               const double *P, double *Q)
+{
+    for (int r = 0; r < neq; r++)
+        Q[r] = 0.0;
+    for (int r = 0; r < neq; r++)
+        for (int j = IA[r]; j < IA[r + 1]; j++)
+            Q[r] = Q[r] + AA[j] * P[JA[j]];
+}
+
+/* EBE/ebemv.c:4-43 and its k=4/12/16 twins: for each element in order, row
+ * r of the element product is summed over columns left to right and added
+ * to Q at the row's equation; the sink slot is reset after every element. */
+void orc_ebemv(int neq, int nel, int k, const double *A, const int *lm,
+               double *P, double *Q)
+{
+    double p[16];
+    for (int r = 0; r < neq; r++)
+        Q[r] = 0.0;
+    P[neq] = 0.0;
+    Q[neq] = 0.0;
+    for (long e = 0; e < nel; e++) {
+        const int *le = lm + e * k;
+        const double *Ae = A + e * (long)k * k;
+        for (int c = 0; c < k; c++)
+            p[c] = P[le[c]];
+        for (int r = 0; r < k; r++) {
+            double row = Ae[r * k] * p[0];
+            for (int c = 1; c < k; c++)
+                row = row + Ae[r * k + c] * p[c];
+            Q[le[r]] = Q[le[r]] + row;
+        }
+        Q[neq] = 0.0;
+    }
+}
+
+void orc_ebemv_f32(int neq, int nel, int k, const float *A, const int *lm,
+                   float *P, float *Q)
+{
+    float p[16];
+    for (int r = 0; r < neq; r++)
+        Q[r] = 0.0f;
+    P[neq] = 0.0f;
+    Q[neq] = 0.0f;
+    for (long e = 0; e < nel; e++) {
+        const int *le = lm + e * k;
+        const float *Ae = A + e * (long)k * k;
+        for (int c = 0; c < k; c++)
+            p[c] = P[le[c]];
+        for (int r = 0; r < k; r++) {
+            float row = Ae[r * k] * p[0];
+            for (int c = 1; c < k; c++)
+                row = row + Ae[r * k + c] * p[c];
+            Q[le[r]] = Q[le[r]] + row;
+        }
+        Q[neq] = 0.0f;
+    }
+}
+
+/* EDE/edemv.c:4-39 */
+void orc_edemv(int neq, int nedge, const double *A, const int *lm2,
+               double *P, double *Q)
+{
+    for (int r = 0; r < neq; r++)
+        Q[r] = 0.0;
+    P[neq] = 0.0;
+    Q[neq] = 0.0;
+    for (long g = 0; g < nedge; g++) {
+        int a = lm2[2 * g], b = lm2[2 * g + 1];
+        double pa = P[a], pb = P[b];
+        const double *Ag = A + 4 * g;
+        Q[a] = Q[a] + (Ag[0] * pa + Ag[1] * pb);
+        Q[b] = Q[b] + (Ag[2] * pa + Ag[3] * pb);
+        Q[neq] = 0.0;
+    }
+}
+
+/* ======================================================================== */
+/* preconditioners                                                           */
+/* ======================================================================== */
+
+/* Diag_precond.c:3-14 */
+void orc_diag_apply(int neq, const double *invDiag, const double *p, double *z)
+{
+    for (int r = 0; r < neq; r++)
+        z[r] = p[r] * invDiag[r];
+}
+
+static void diag_finish(int neq, const double *Diag, double *invDiag, double *F)
+{
+    for (int r = 0; r < neq; r++)
+        invDiag[r] = 1.0 / Diag[r];
+    for (int r = 0; r < neq; r++)
+        F[r] = F[r] * invDiag[r];
+}
+
+/* EBE/Diag_precond_EBE_setup.c:3-35 : element diagonals summed in element
+ * order (stride k+1 walks the diagonal of the row-major element matrix). */
+void orc_diag_setup_ebe(int neq, int nel, int k, const double *A, const int *lm,
+                        double *Diag, double *invDiag, double *F)
+{
+    for (int r = 0; r < neq; r++)
+        Diag[r] = 0.0;
+    for (long e = 0; e < nel; e++) {
+        for (int j = 0; j < k; j++) {
+            int g = lm[e * k + j];
+            Diag[g] = Diag[g] + A[e * (long)k * k + (long)j * (k + 1)];
+        }
+        Diag[neq] = 0.0;
+    }
+    diag_finish(neq, Diag, invDiag, F);
+    invDiag[neq] = 1.0;                       /* EBE variant only, :29 */
+}
+
+/* EDE/Diag_precond_EDE_setup.c:3-33 (size = 2*NDOF = 2, stride 3) */
+void orc_diag_setup_ede(int neq, int nedge, const double *A, const int *lm2,
+                        double *Diag, double *invDiag, double *F)
+{
+    for (int r = 0; r < neq; r++)
+        Diag[r] = 0.0;
+    for (long g = 0; g < nedge; g++) {
+        Diag[lm2[2 * g]] = Diag[lm2[2 * g]] + A[4 * g];
+        Diag[lm2[2 * g + 1]] = Diag[lm2[2 * g + 1]] + A[4 * g + 3];
+        Diag[neq] = 0.0;
+    }
+    diag_finish(neq, Diag, invDiag, F);
+}
+
+/* CSR/Diag_precond_CSR_setup.c:3-29 : Diag is not cleared first; the last
+ * slot of the row whose column equals the row wins. */
+void orc_diag_setup_csr(int neq, const double *AA, const int *JA, const int *IA,
+                        double *Diag, double *invDiag, double *F)
+{
+    for (int r = 0; r < neq; r++)
+        for (int j = IA[r]; j < IA[r + 1]; j++)
+            if (JA[j] == r)
+                Diag[r] = AA[j];
+    diag_finish(neq, Diag, invDiag, F);
+}
+
+/* EBE/Jacobi_precond_EBE.c:8-47 */
+void orc_jacobi_ebe_apply(int neq, int nel, const double *invDe, const int *lm,
+                          const double *p, double *z)
+{
+    for (int r = 0; r < neq; r++)
+        z[r] = 0.0;
+    z[neq] = 0.0;
+    for (long e = 0; e < nel; e++) {
+        const int *le = lm + 3 * e;
+        double z0 = p[le[0]] * invDe[3 * e + 0];
+        double z1 = p[le[1]] * invDe[3 * e + 1];
+        double z2 = p[le[2]] * invDe[3 * e + 2];
+        z[le[0]] = z[le[0]] + z0;
+        z[le[1]] = z[le[1]] + z1;
+        z[le[2]] = z[le[2]] + z2;
+        z[neq] = 0.0;
+    }
+}
+
+/* EBE/Jacobi_precond_EBE_setup.c:4-43 */
+void orc_jacobi_ebe_setup(int neq, int nel, const double *A, const int *lm,
+                          double *invDe, double *F)
+{
+    for (long e = 0; e < nel; e++) {
+        invDe[3 * e + 0] = 1.0 / A[9 * e + 0];
+        invDe[3 * e + 1] = 1.0 / A[9 * e + 4];
+        invDe[3 * e + 2] = 1.0 / A[9 * e + 8];
+    }
+    double *faux = (double *)calloc((size_t)neq + 1, sizeof(double));
+    for (int r = 0; r < neq; r++)
+        faux[r] = F[r];
+    faux[neq] = 0.0;
+    F[neq] = 0.0;
+    orc_jacobi_ebe_apply(neq, nel, invDe, lm, faux, F);
+    free(faux);
+}
+
+/* ======================================================================== */
+/* operator callbacks                                                        */
+/* ======================================================================== */
+void orc_op_csr(void *ctx, double *x, double *y)
+{
+    orc_csr_ctx *c = (orc_csr_ctx *)ctx;
+    orc_csrmv(c->neq, c->AA, c->JA, c->IA, x, y);
+}
+void orc_op_ebe(void *ctx, double *x, double *y)
+{
+    orc_ebe_ctx *c = (orc_ebe_ctx *)ctx;
+    orc_ebemv(c->neq, c->nel, c->k, c->A, c->lm, x, y);
+}
+void orc_op_ede(void *ctx, double *x, double *y)
+{
+    orc_ede_ctx *c = (orc_ede_ctx *)ctx;
+    orc_edemv(c->neq, c->nedge, c->A, c->lm2, x, y);
+}
+void orc_op_diag(void *ctx, double *x, double *y)
+{
+    orc_diag_ctx *c = (orc_diag_ctx *)ctx;
+    orc_diag_apply(c->neq, c->invDiag, x, y);
+}
+
+/* ======================================================================== */
+/* restarted GMRES                                                           */
+/* ======================================================================== */
+
+/* pgmres.c:4-211.  Left preconditioner `precond`, right preconditioner
+ * `precondR`; modified Gram-Schmidt Arnoldi; Givens rotations on the
+ * Hessenberg column; back substitution; X update with the basis index as the
+ * outer loop.  A NULL callback is NO_precond (NO_precond.c:3-6). */
+int orc_pgmres(int neq, int kmax, int lmax, double tol,
+               const double *B, double *X, const orc_ops *ops,
+               double *rho_hist, int hist_cap, double *e0_hist, int *n_cycles)
+{
+    size_t len = (size_t)neq + 1;
+    double **u = (double **)calloc((size_t)kmax + 1, sizeof(double *));
+    double **h = (double **)calloc((size_t)kmax + 1, sizeof(double *));
+    for (int i = 0; i <= kmax; i++) {
+        u[i] = (double *)calloc(len, sizeof(double));
+        h[i] = (double *)calloc((size_t)kmax + 1, sizeof(double));
+    }
+    double *e = (double *)calloc((size_t)kmax + 1, sizeof(double));
+    double *c = (double *)calloc((size_t)kmax, sizeof(double));
+    double *s = (double *)calloc((size_t)kmax, sizeof(double));
+    double *y = (double *)calloc((size_t)kmax, sizeof(double));
+    double *v = (double *)calloc(len, sizeof(double));
+
+    orc_dzero(neq, X);
+    double normb = sqrt(orc_ddot(neq, B, B));
+    double eps = tol * normb;
+    int cycles = 0, steps = 0;
+    double rho;
+
+    do {
+        int i = 0;
+        for (int j = 0; j < kmax; j++)
+            orc_dzero(neq, u[j]);
+        orc_dcopy(neq, X, v);
+        if (ops->precondR) ops->precondR(ops->precondR_ctx, v, v);
+        ops->matvec(ops->matvec_ctx, v, u[0]);
+        if (ops->precond) ops->precond(ops->precond_ctx, u[0], u[0]);
+        orc_daxpy(neq, -1.0, B, u[0]);
+        orc_dscal(neq, -1.0, u[0]);
+        e[0] = sqrt(orc_ddot(neq, u[0], u[0]));
+        if (e0_hist) e0_hist[cycles] = e[0];
+        orc_dscal(neq, 1.0 / e[0], u[0]);
+        rho = e[0];
+
+        do {
+            steps++;
+            orc_dcopy(neq, u[i], v);
+            if (ops->precondR) ops->precondR(ops->precondR_ctx, v, v);
+            ops->matvec(ops->matvec_ctx, v, u[i + 1]);
+            if (ops->precond) ops->precond(ops->precond_ctx, u[i + 1], u[i + 1]);
+            for (int j = 0; j <= i; j++) {
+                h[j][i] = orc_ddot(neq, u[j], u[i + 1]);
+                orc_daxpy(neq, -h[j][i], u[j], u[i + 1]);
+            }
+            h[i + 1][i] = sqrt(orc_ddot(neq, u[i + 1], u[i + 1]));
+            orc_dscal(neq, 1.0 / h[i + 1][i], u[i + 1]);
+            for (int j = 0; j <= i - 1; j++) {
+                double h1 = c[j] * h[j][i] + s[j] * h[j + 1][i];
+                double h2 = -s[j] * h[j][i] + c[j] * h[j + 1][i];
+                h[j][i] = h1;
+                h[j + 1][i] = h2;
+            }
+            double r = sqrt(h[i][i] * h[i][i] + h[i + 1][i] * h[i + 1][i]);
+            c[i] = h[i][i] / r;
+            s[i] = h[i + 1][i] / r;
+            h[i][i] = r;
+            h[i + 1][i] = 0.0;
+            e[i + 1] = -s[i] * e[i];
+            e[i] = c[i] * e[i];
+            rho = fabs(e[i + 1]);
+            if (rho_hist && steps - 1 < hist_cap) rho_hist[steps - 1] = rho;
+            i++;
+        } while (rho > eps && i < kmax);
+
+        i--;
+        y[i] = e[i] / h[i][i];
+        for (int j = i - 1; j >= 0; j--) {
+            double acc = 0.0;
+            for (int q = j + 1; q <= i; q++)
+                acc = acc + h[j][q] * y[q];
+            y[j] = (e[j] - acc) / h[j][j];
+        }
+        for (int j = 0; j <= i; j++)
+            for (long q = 0; q < neq; q++)
+                X[q] = X[q] + u[j][q] * y[j];
+        cycles++;
+    } while (rho > eps && cycles < lmax);
+
+    if (ops->precondR) ops->precondR(ops->precondR_ctx, X, X);
+
+    for (int i = 0; i <= kmax; i++) { free(u[i]); free(h[i]); }
+    free(u); free(h); free(e); free(c); free(s); free(y); free(v);
+    if (n_cycles) *n_cycles = cycles;
+    return steps;
+}
+
+/* ======================================================================== */
+/* 2D mesh and data structures                                               */
+/* ======================================================================== */
+
+/* geraMalha.c:112-176 ("direita"): nodes row by row, x fastest; each cell
+ * (i,j) with lower-left node a gives the triangles (a, a+1, a+1+W) and
+ * (a, a+1+W, a+W), W = n+1 nodes per row; cells row by row. */
+void orc_mesh2d_structured(int n, double *x, double *y, int *type, int *vertex)
+{
+    int W = n + 1;
+    double hx = 1.0 / n;
+    for (int j = 0, k = 0; j < W; j++)
+        for (int i = 0; i < W; i++, k++) {
+            x[k] = 0.0 + i * hx;
+            y[k] = 0.0 + j * hx;
+            int bnd = fabs(x[k] - 1.0) <= 1e-15 || fabs(x[k]) <= 1e-15 ||
+                      fabs(y[k] - 1.0) <= 1e-15 || fabs(y[k]) <= 1e-15;
+            type[k] = bnd ? 0 : 1;
+        }
+    for (int j = 0, e = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            int a = j * W + i;
+            vertex[3 * e + 0] = a; vertex[3 * e + 1] = a + 1; vertex[3 * e + 2] = a + 1 + W; e++;
+            vertex[3 * e + 0] = a; vertex[3 * e + 1] = a + 1 + W; vertex[3 * e + 2] = a + W; e++;
+        }
+}
+
+/* Preprocess.c:73-81 */
+int orc_number_nodes(int nnodes, const int *type, int *id)
+{
+    int neq = 0;
+    for (int i = 0; i < nnodes; i++)
+        id[i] = type[i] >= 1 ? neq++ : -1;
+    return neq;
+}
+
+/* Fill_LM.c:3-24 */
+void orc_fill_lm(int neq, int nel, int nnoel, const int *vertex, const int *type,
+                 const int *id, int *lm)
+{
+    for (long e = 0; e < nel; e++)
+        for (int j = 0; j < nnoel; j++) {
+            int v = vertex[e * nnoel + j];
+            lm[e * nnoel + j] = type[v] >= 1 ? id[v] : neq;
+        }
+}
+
+/* (row, col) pair list sorted lexicographically and deduplicated: the same
+ * sets that the reference's sorted linked lists hold (csr_List_insertA,
+ * csr_Initialization.c:221-261; ede_List_insertA, ede_Initialization.c:172-212). */
+typedef struct { int r, c; } pair_t;
+static int pair_cmp(const void *a, const void *b)
+{
+    const pair_t *p = (const pair_t *)a, *q = (const pair_t *)b;
+    if (p->r != q->r) return p->r < q->r ? -1 : 1;
+    if (p->c != q->c) return p->c < q->c ? -1 : 1;
+    return 0;
+}
+static long pair_unique(pair_t *v, long n)
+{
+    if (n == 0) return 0;
+    qsort(v, (size_t)n, sizeof(pair_t), pair_cmp);
+    long m = 1;
+    for (long i = 1; i < n; i++)
+        if (v[i].r != v[m - 1].r || v[i].c != v[m - 1].c)
+            v[m++] = v[i];
+    return m;
+}
+/* rank of column c inside the sorted row [lo,hi) */
+static int pair_find(const pair_t *v, long lo, long hi, int c)
+{
+    while (lo < hi) {
+        long mid = (lo + hi) / 2;
+        if (v[mid].c < c) lo = mid + 1; else hi = mid;
+    }
+    return (int)lo;
+}
+
+/* csr_Initialization.c:5-218 with reordering NOT (reordering.c:8-10) */
+int orc_csr_init(int neq, int nel, const int *lm, int *IA, int *JA, int *csr_by_elem)
+{
+    pair_t *pr = (pair_t *)malloc(sizeof(pair_t) * (size_t)nel * 9 + 1);
+    long np = 0;
+    for (long e = 0; e < nel; e++)
+        for (int a = 0; a < 3; a++) {
+            int I = lm[3 * e + a];
+            if (I == neq) continue;
+            for (int b = 0; b < 3; b++) {
+                int J = lm[3 * e + b];
+                if (J == neq) continue;
+                pr[np].r = I; pr[np].c = J; np++;
+            }
+        }
+    long nnz = pair_unique(pr, np);
+    if (JA == NULL) { free(pr); return (int)nnz; }
+    for (int r = 0; r <= neq; r++) IA[r] = 0;
+    for (long q = 0; q < nnz; q++) { IA[pr[q].r + 1]++; JA[q] = pr[q].c; }
+    for (int r = 0; r < neq; r++) IA[r + 1] += IA[r];
+    for (long e = 0; e < nel; e++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                int I = lm[3 * e + a], J = lm[3 * e + b];
+                int slot = (int)nnz;
+                if (I != neq && J != neq)
+                    slot = pair_find(pr, IA[I], IA[I + 1], J);
+                csr_by_elem[9 * e + 3 * a + b] = slot;
+            }
+    free(pr);
+    return (int)nnz;
+}
+
+/* ede_Initialization.c:4-171.  The three local vertices of every element are
+ * ordered by equation number (ties broken exactly as the reference's
+ * comparison chain, :22-83); edges (I1,I2),(I1,I3),(I2,I3) are numbered row
+ * by row in ascending column order; the final count is decremented once
+ * (:103) so that the edge (neq,neq) becomes the sink slot. */
+static void ede_order(const int *l, int *o)
+{
+    if (l[0] < l[1] && l[0] < l[2]) {
+        if (l[1] < l[2]) { o[0] = 0; o[1] = 1; o[2] = 2; }
+        else             { o[0] = 0; o[1] = 2; o[2] = 1; }
+    } else if (l[1] < l[2]) {
+        if (l[0] < l[2]) { o[0] = 1; o[1] = 0; o[2] = 2; }
+        else             { o[0] = 1; o[1] = 2; o[2] = 0; }
+    } else {
+        if (l[0] < l[1]) { o[0] = 2; o[1] = 0; o[2] = 1; }
+        else             { o[0] = 2; o[1] = 1; o[2] = 0; }
+    }
+}
+
+int orc_ede_init(int neq, int nel, const int *lm, int *order, int *edge_by_elem, int *lm2)
+{
+    pair_t *pr = (pair_t *)malloc(sizeof(pair_t) * (size_t)nel * 3 + 1);
+    long np = 0;
+    for (long e = 0; e < nel; e++) {
+        int o[3];
+        ede_order(lm + 3 * e, o);
+        int I1 = lm[3 * e + o[0]], I2 = lm[3 * e + o[1]], I3 = lm[3 * e + o[2]];
+        pr[np].r = I1; pr[np].c = I2; np++;
+        pr[np].r = I1; pr[np].c = I3; np++;
+        pr[np].r = I2; pr[np].c = I3; np++;
+    }
+    long ntot = pair_unique(pr, np);
+    int nedge = (int)ntot - 1;
+    if (lm2 == NULL) { free(pr); return nedge; }
+    /* row offsets over rows 0..neq */
+    long *off = (long *)calloc((size_t)neq + 2, sizeof(long));
+    for (long q = 0; q < ntot; q++) off[pr[q].r + 1]++;
+    for (int r = 0; r <= neq; r++) off[r + 1] += off[r];
+    for (long e = 0; e < nel; e++) {
+        int *o = order + 3 * e;
+        ede_order(lm + 3 * e, o);
+        int I1 = lm[3 * e + o[0]], I2 = lm[3 * e + o[1]], I3 = lm[3 * e + o[2]];
+        edge_by_elem[3 * e + 0] = pair_find(pr, off[I1], off[I1 + 1], I2);
+        edge_by_elem[3 * e + 1] = pair_find(pr, off[I1], off[I1 + 1], I3);
+        edge_by_elem[3 * e + 2] = pair_find(pr, off[I2], off[I2 + 1], I3);
+    }
+    for (long q = 0; q < off[neq]; q++) {
+        lm2[2 * q] = pr[q].r;
+        lm2[2 * q + 1] = pr[q].c;
+    }
+    free(off);
+    free(pr);
+    return nedge;
+}
+
+/* ======================================================================== */
+/* problem callbacks                                                         */
+/* ======================================================================== */
+void orc_prob2d_velocity(const orc_prob2d *pb, double x, double y, double beta[2])
+{
+    switch (pb->kind) {
+    case ORC_P_ROTATING:
+        beta[0] = (-y + 0.5) * pb->scale;
+        beta[1] = (x - 0.5) * pb->scale;
+        break;
+    case ORC_P_VALIDACAO:                       /* VALIDACAO_Velocity.c:3-12 */
+        beta[0] = 1;
+        beta[1] = 20 * y;
+        break;
+    default:
+        beta[0] = pb->bx;
+        beta[1] = pb->by;
+    }
+}
+
+/* VALIDACAO_Font.c:4-17: forcing of u = 10xy(1-x)(1-y)exp(x^4.5) */
+static double validacao_font(double X, double Y, double k, double gamma,
+                             double Be_x, double Be_y)
+{
+    double E = exp(pow(X, 4.5));
+    double u = 10 * X * Y * (1 - X) * (1 - Y) * E;
+    double u_x = 10 * Y * (1 - Y) * (E - 2 * E * X + 4.5 * E * pow(X, 4.5) - 4.5 * E * pow(X, 5.5));
+    double u_x2 = 10 * Y * (1 - Y) * (-20.25 * E * pow(X, 9) + 20.25 * E * pow(X, 8)
+                  - 33.75 * E * pow(X, 4.5) + 24.75 * E * pow(X, 3.5) - 2 * E);
+    double u_y = 10 * E * X * (1 - X) * (1 - 2 * Y);
+    double u_y2 = -20 * E * X * (1 - X);
+    return -k * (u_x2 + u_y2) + Be_x * u_x + Be_y * u_y + gamma * u;
+}
+
+double orc_prob2d_font(const orc_prob2d *pb, double x, double y, double k, double gamma,
+                       double bx, double by)
+{
+    if (pb->kind == ORC_P_VALIDACAO)
+        return validacao_font(x, y, k, gamma, bx, by);
+    return pb->f;
+}
+
+/* CAU_ShockCapture.c:3-30 */
+static double cau_shock(double kx, double ky, double Be_x, double Be_y, double gamma,
+                        double ue1, double ue2, double ue3, double ueb, double feb,
+                        double y23, double y31, double y12, double x32, double x13,
+                        double x21, double invArea, double h)
+{
+    const double third = 1. / 3.;
+    double BetaGradu = 0.5 * invArea * (ue1 * (Be_x * y23 + Be_y * x32) +
+                                        ue2 * (Be_x * y31 + Be_y * x13) +
+                                        ue3 * (Be_x * y12 + Be_y * x21));
+    double Ru = BetaGradu + gamma * ueb - feb;
+    double g0 = 0.5 * invArea * (y23 * ue1 + y31 * ue2 + y12 * ue3);
+    double g1 = 0.5 * invArea * (x32 * ue1 + x13 * ue2 + x21 * ue3);
+    double ng = sqrt(g0 * g0 + g1 * g1);
+    double bxp = BetaGradu * g0 / (ng * ng);
+    double byp = BetaGradu * g1 / (ng * ng);
+    double kt = bxp * bxp * kx + byp * byp * ky;
+    double nbp = sqrt(bxp * bxp + byp * byp);
+    double Pee = 0.5 * h * nbp * nbp * nbp / kt;
+    if (ng > 0) {
+        if (third * Pee < 0.7)
+            return 0.5 * h * third * Pee * fabs(Ru) / ng;
+        return 0.5 * h * 0.7 * fabs(Ru) / ng;
+    }
+    return 0;
+}
+
+/* Build_K_F_SUPG.c:5-163.  Every expression keeps the reference's operand
+ * order so that the restatement is bitwise faithful at -ffp-contract=off. */
+void orc_build_supg2d(int nnodes, int nel, int neq, const double *x, const double *y,
+                      const int *type, const int *id, const int *vertex,
+                      const double *u, const orc_prob2d *pb, double *ke, double *F)
+{
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
+    double kx, ky, gamma;
+    if (pb->kind == ORC_P_VALIDACAO) { kx = 1.0; ky = 1.0; gamma = 1; }
+    else { kx = pb->kappa; ky = pb->kappa; gamma = pb->gamma; }
+
+    /* eval_U.c:3-25 : nodal field, Dirichlet nodes take u_D = 0 */
+    double *U = (double *)calloc((size_t)nnodes, sizeof(double));
+    for (int i = 0; i < nnodes; i++)
+        U[i] = id[i] >= 0 ? u[id[i]] : 0.0;
+    for (int r = 0; r < neq; r++)
+        F[r] = 0;
+
+    for (long E = 0; E < nel; E++) {
+        int J1 = vertex[3 * E], J2 = vertex[3 * E + 1], J3 = vertex[3 * E + 2];
+        double X0 = x[J1], X1 = x[J2], X2 = x[J3];
+        double Y0 = y[J1], Y1 = y[J2], Y2 = y[J3];
+        double y23 = Y1 - Y2, y31 = Y2 - Y0, y12 = Y0 - Y1;
+        double x32 = X2 - X1, x13 = X0 - X2, x21 = X1 - X0;
+        double TwoA = x21 * y31 - x13 * y12;
+        double Area = 0.5 * TwoA;
+        double invArea = 1.0 / Area;
+        double h = sqrt(fabs(TwoA));
+        double XB = third * (X0 + X1 + X2);
+        double YB = third * (Y0 + Y1 + Y2);
+        double Beta[2];
+        orc_prob2d_velocity(pb, XB, YB, Beta);
+        double Be_x = Beta[0], Be_y = Beta[1];
+        double norm = sqrt(Be_x * Be_x + Be_y * Be_y);
+        double tau = norm > 0 ? 0.5 * h / norm : 0;
+
+        double fe1 = orc_prob2d_font(pb, X0, Y0, kx, gamma, Be_x, Be_y);
+        double fe2 = orc_prob2d_font(pb, X1, Y1, kx, gamma, Be_x, Be_y);
+        double fe3 = orc_prob2d_font(pb, X2, Y2, kx, gamma, Be_x, Be_y);
+        double feb = third * (fe1 + fe2 + fe3);
+        double ue1 = U[J1], ue2 = U[J2], ue3 = U[J3];
+        double ueb = third * (ue1 + ue2 + ue3);
+
+        double D12 = 0.25 * invArea * ((kx * y23 * y31) + (ky * x32 * x13));
+        double D13 = 0.25 * invArea * ((kx * y23 * y12) + (ky * x32 * x21));
+        double D23 = 0.25 * invArea * ((kx * y31 * y12) + (ky * x13 * x21));
+        double C11 = sixth * (Be_x * y23 + Be_y * x32);
+        double C12 = sixth * (Be_x * y31 + Be_y * x13);
+        double C13 = sixth * (Be_x * y12 + Be_y * x21);
+        double R11 = sixth * (gamma * Area);
+        double R12 = twelfth * (gamma * Area);
+        double Cpg12 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y31 + Be_x * Be_y * x13) +
+                                               x32 * (Be_x * Be_y * y31 + Be_y * Be_y * x13));
+        double Cpg13 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) +
+                                               x32 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+        double Cpg23 = 0.25 * invArea * tau * (y31 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) +
+                                               x13 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+        double Rpg11 = sixth * tau * gamma * (Be_x * y23 + Be_y * x32);
+        double Rpg21 = sixth * tau * gamma * (Be_x * y31 + Be_y * x13);
+        double Rpg31 = sixth * tau * gamma * (Be_x * y12 + Be_y * x21);
+
+        double hs = pb->h_shock == ORC_H_SQRTAREA ? 2 * sqrt(fabs(Area))
+                                                  : sqrt(fabs(2 * Area));
+        double Eu = cau_shock(kx, ky, Be_x, Be_y, gamma, ue1, ue2, ue3, ueb, feb,
+                              y23, y31, y12, x32, x13, x21, invArea, hs);
+        double Csc12 = Eu * 0.25 * invArea * (y23 * y31 + x32 * x13);
+        double Csc13 = Eu * 0.25 * invArea * (y23 * y12 + x32 * x21);
+        double Csc23 = Eu * 0.25 * invArea * (y31 * y12 + x13 * x21);
+
+        double *k9 = ke + 9 * E;
+        k9[0] = -(D12 + D13) + C11 - (Cpg12 + Cpg13) + R11 + Rpg11 - (Csc12 + Csc13);
+        k9[1] = D12 + C12 + Cpg12 + R12 + Rpg11 + Csc12;
+        k9[2] = D13 + C13 + Cpg13 + R12 + Rpg11 + Csc13;
+        k9[3] = D12 + C11 + Cpg12 + R12 + Rpg21 + Csc12;
+        k9[4] = -(D12 + D23) + C12 - (Cpg12 + Cpg23) + R11 + Rpg21 - (Csc12 + Csc23);
+        k9[5] = D23 + C13 + Cpg23 + R12 + Rpg21 + Csc23;
+        k9[6] = D13 + C11 + Cpg13 + R12 + Rpg31 + Csc13;
+        k9[7] = D23 + C12 + Cpg23 + R12 + Rpg31 + Csc23;
+        k9[8] = -(D13 + D23) + C13 - (Cpg13 + Cpg23) + R11 + Rpg31 - (Csc13 + Csc23);
+
+        double Fe1 = Area * twelfth * (2 * fe1 + fe2 + fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y23 + Be_y * x32);
+        double Fe2 = Area * twelfth * (fe1 + 2 * fe2 + fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y31 + Be_y * x13);
+        double Fe3 = Area * twelfth * (fe1 + fe2 + 2 * fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y12 + Be_y * x21);
+
+        /* F_assembly.c:5-27 with u_D = 0 for every supported problem: the
+         * lifting terms subtract ke*0.  The Neumann term (:29-50) multiplies
+         * by sixth = 1/6 in integer arithmetic, i.e. 0, and needs Type 2
+         * nodes, which no supported mesh has. */
+        if (type[J1] >= 1) {
+            F[id[J1]] += Fe1;
+            if (type[J2] == 0) F[id[J1]] -= k9[1] * 0.0;
+            if (type[J3] == 0) F[id[J1]] -= k9[2] * 0.0;
+        }
+        if (type[J2] >= 1) {
+            F[id[J2]] += Fe2;
+            if (type[J1] == 0) F[id[J2]] -= k9[3] * 0.0;
+            if (type[J3] == 0) F[id[J2]] -= k9[5] * 0.0;
+        }
+        if (type[J3] >= 1) {
+            F[id[J3]] += Fe3;
+            if (type[J1] == 0) F[id[J3]] -= k9[6] * 0.0;
+            if (type[J2] == 0) F[id[J3]] -= k9[7] * 0.0;
+        }
+    }
+    free(U);
+}
+
+/* ede_assembly.c:3-35 after setzeros.c:8-11 */
+void orc_ede_assembly(int nel, int nedge, const double *ke, const int *order,
+                      const int *edge_by_elem, double *Aede)
+{
+    memset(Aede, 0, sizeof(double) * 4 * ((size_t)nedge + 1));
+    for (long E = 0; E < nel; E++) {
+        const double *k9 = ke + 9 * E;
+        int I1 = order[3 * E], I2 = order[3 * E + 1], I3 = order[3 * E + 2];
+        double *K1 = Aede + 4L * edge_by_elem[3 * E];
+        double *K2 = Aede + 4L * edge_by_elem[3 * E + 1];
+        double *K3 = Aede + 4L * edge_by_elem[3 * E + 2];
+        K1[0] += 0.5 * k9[3 * I1 + I1]; K1[1] += k9[3 * I1 + I2];
+        K1[2] += k9[3 * I2 + I1];       K1[3] += 0.5 * k9[3 * I2 + I2];
+        K2[0] += 0.5 * k9[3 * I1 + I1]; K2[1] += k9[3 * I1 + I3];
+        K2[2] += k9[3 * I3 + I1];       K2[3] += 0.5 * k9[3 * I3 + I3];
+        K3[0] += 0.5 * k9[3 * I2 + I2]; K3[1] += k9[3 * I2 + I3];
+        K3[2] += k9[3 * I3 + I2];       K3[3] += 0.5 * k9[3 * I3 + I3];
+        double *Ks = Aede + 4L * nedge;
+        Ks[0] = Ks[1] = Ks[2] = Ks[3] = 0;
+    }
+}
+
+/* csr_assembly.c:3-24 after setzeros.c:12-15 */
+void orc_csr_assembly(int nel, int nnz, const double *ke, const int *csr_by_elem, double *AA)
+{
+    memset(AA, 0, sizeof(double) * ((size_t)nnz + 1));
+    for (long E = 0; E < nel; E++) {
+        for (int q = 0; q < 9; q++)
+            AA[csr_by_elem[9 * E + q]] += ke[9 * E + q];
+        AA[nnz] = 0;
+    }
+}

@@ -1,0 +1,160 @@
+/*
+ * oracle.h -- CPU restatement of the FEM_CODES solve path (test infrastructure).
+ *
+ * THIS IS THE CHECKER, NOT THE PRODUCT.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load liboracle.so.  The GPU library
+ * (libfemgpu.so) never links or calls anything declared here.
+ *
+ * Every function restates one reference routine in the same loop and
+ * expression order so that, built with -O2 -ffp-contract=off, it reproduces
+ * the reference bit for bit.  The reference citation (path:line, relative to
+ * the FEM_CODES tree) is given on each declaration.  The interface is
+ * struct-agnostic: flat arrays plus sizes instead of the per-driver
+ * ParametersType / MatrixDataType / FemStructsType structs.
+ *
+ * Conventions kept from the reference:
+ *   - every equation vector has neq+1 slots; slot neq is the Dirichlet sink;
+ *   - lm[e*k + j] is the equation number of local dof j of element e, or neq;
+ *   - element matrices are row-major k x k per element (AoS), A[e*k*k + r*k + c].
+ *
+ * Pinning status of each function is recorded in the header comment of
+ * oracle.c and checked by tests/test_oracle_*.py.
+ */
+#ifndef FEM_ORACLE_H
+#define FEM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- BLAS-1 (00_CommonFiles/BLAS_Operations) ---------------- */
+double orc_ddot(int n, const double *x, const double *y);      /* ddot.c:4-19   */
+void   orc_daxpy(int n, double a, const double *x, double *y); /* daxpy.c:4-21  */
+void   orc_dscal(int n, double a, double *x);                  /* dscal.c:4-21  */
+void   orc_dcopy(int n, const double *x, double *y);           /* dcopy.c:4-19  */
+void   orc_dzero(int n, double *x);                            /* dzero.c:3-18  */
+
+/* ---------------- matvecs (00_CommonFiles/MatrixVector_Operations) -------- */
+/* csrmv  CSR/csrmv.c:3-24 */
+void orc_csrmv(int neq, const double *AA, const int *JA, const int *IA,
+               const double *P, double *Q);
+/* ebemv EBE/ebemv.c:4-43 (k=3), ebemv3Dndof1 EBE/ebemv3Dndof1.c:3-45 (k=4),
+ * ebemvNDOF4 EBE/ebemvNDOF4.c:5-112 (k=12) and its 3D branch (k=16).
+ * Writes P[neq]=0 as the reference does (ebemv.c:21). */
+void orc_ebemv(int neq, int nel, int k, const double *A, const int *lm,
+               double *P, double *Q);
+/* edemv EDE/edemv.c:4-39 ; A has (nedge+1)*4 entries, lm2 has nedge*2 */
+void orc_edemv(int neq, int nedge, const double *A, const int *lm2,
+               double *P, double *Q);
+/* single-precision twins used for the FP32 inner solve checks */
+void orc_ebemv_f32(int neq, int nel, int k, const float *A, const int *lm,
+                   float *P, float *Q);
+
+/* ---------------- preconditioners ----------------------------------------- */
+/* Diag_precond.c:3-14 : z = p*invDiag (p may alias z) */
+void orc_diag_apply(int neq, const double *invDiag, const double *p, double *z);
+/* EBE/Diag_precond_EBE_setup.c:3-35 ; Diag,invDiag,F have neq+1 slots */
+void orc_diag_setup_ebe(int neq, int nel, int k, const double *A, const int *lm,
+                        double *Diag, double *invDiag, double *F);
+/* EDE/Diag_precond_EDE_setup.c:3-33 */
+void orc_diag_setup_ede(int neq, int nedge, const double *A, const int *lm2,
+                        double *Diag, double *invDiag, double *F);
+/* CSR/Diag_precond_CSR_setup.c:3-29 */
+void orc_diag_setup_csr(int neq, const double *AA, const int *JA, const int *IA,
+                        double *Diag, double *invDiag, double *F);
+/* EBE/Jacobi_precond_EBE_setup.c:4-43 (k=3): invDe[e*3+j] = 1/A_e[j][j],
+ * then F <- Jacobi_precond_EBE(F) */
+void orc_jacobi_ebe_setup(int neq, int nel, const double *A, const int *lm,
+                          double *invDe, double *F);
+/* EBE/Jacobi_precond_EBE.c:8-47 ; p and z must not alias (the reference
+ * zeroes z first, so an aliased call returns zero) */
+void orc_jacobi_ebe_apply(int neq, int nel, const double *invDe, const int *lm,
+                          const double *p, double *z);
+
+/* ---------------- restarted GMRES ----------------------------------------- */
+/* operator callback: y = Op(x); x and y have neq+1 slots and may alias */
+typedef void (*orc_op_fn)(void *ctx, double *x, double *y);
+
+typedef struct {
+    orc_op_fn matvec;   void *matvec_ctx;    /* FemFunctions->ProductMatrixVector */
+    orc_op_fn precond;  void *precond_ctx;   /* FemFunctions->precond  (NULL = NO_precond) */
+    orc_op_fn precondR; void *precondR_ctx;  /* FemFunctions->precondR (NULL = NO_precond) */
+} orc_ops;
+
+/* pgmres  Solvers_and_Preconditioners/pgmres.c:4-211.
+ * kmax = KrylovBasisVectorsQuantity, lmax = SolverMaxIter, tol = SolverTolerance.
+ * B,X have neq+1 slots.  Returns the Arnoldi step count ("cont", :94).
+ * If rho_hist != NULL it receives rho after every Arnoldi step (at most
+ * hist_cap entries); e0_hist (optional) receives e[0] at each restart.
+ * *n_cycles receives the number of restart cycles l. */
+int orc_pgmres(int neq, int kmax, int lmax, double tol,
+               const double *B, double *X, const orc_ops *ops,
+               double *rho_hist, int hist_cap, double *e0_hist, int *n_cycles);
+
+/* ready-made operator contexts for the callbacks above */
+typedef struct { int neq; const double *AA; const int *JA; const int *IA; } orc_csr_ctx;
+typedef struct { int neq, nel, k; const double *A; const int *lm; } orc_ebe_ctx;
+typedef struct { int neq, nedge; const double *A; const int *lm2; } orc_ede_ctx;
+typedef struct { int neq; const double *invDiag; } orc_diag_ctx;
+void orc_op_csr(void *ctx, double *x, double *y);
+void orc_op_ebe(void *ctx, double *x, double *y);
+void orc_op_ede(void *ctx, double *x, double *y);
+void orc_op_diag(void *ctx, double *x, double *y);
+
+/* ---------------- 2D mesh, structures and SUPG assembly ------------------- */
+/* Structured unit-square mesh, n x n cells, each cell split along the
+ * "direita" diagonal exactly as Gerador_de_Malhas/Structured_Mesh/geraMalha.c:112-176
+ * numbers nodes and elements.  Boundary nodes get Type 0 (Dirichlet), inner
+ * nodes Type 1 (geraMalha.c:324-332 with print_bound, :17-20).
+ * x,y,type: (n+1)^2 ; vertex: 2n^2 * 3. */
+void orc_mesh2d_structured(int n, double *x, double *y, int *type, int *vertex);
+/* node equation numbering Preprocess.c:73-81 : id = neq++ for Type>=1 else -1.
+ * returns neq */
+int  orc_number_nodes(int nnodes, const int *type, int *id);
+/* Fill_LM.c:3-24 for NNOEL nodes per element and one dof per node */
+void orc_fill_lm(int neq, int nel, int nnoel, const int *vertex, const int *type,
+                 const int *id, int *lm);
+
+/* csr_Initialization.c:5-218 (reordering "NOT").  Two-pass: first call with
+ * JA==NULL returns nnz; second call fills IA[neq+1], JA[nnz] and
+ * csr_by_elem[nel*9] (slot nnz = sink). */
+int  orc_csr_init(int neq, int nel, const int *lm, int *IA, int *JA, int *csr_by_elem);
+/* ede_Initialization.c:4-171.  First call with lm2==NULL returns nedge (after
+ * the reference's nedge-- at :103).  Second call fills order[nel*3],
+ * edge_by_elem[nel*3], lm2[nedge*2]. */
+int  orc_ede_init(int neq, int nel, const int *lm, int *order, int *edge_by_elem, int *lm2);
+
+/* problem coefficients (FemFunctions callbacks of the SS-2D driver) */
+enum { ORC_P_CONST = 0,      /* beta=(bx,by), f const, u_D=0              */
+       ORC_P_ROTATING = 1,   /* beta=scale*(-y+0.5, x-0.5), f const, u_D=0 */
+       ORC_P_VALIDACAO = 2   /* VALIDACAO/VALIDACAO_*.c                    */ };
+enum { ORC_H_2SQRTAREA = 0,  /* h_shock_2sqrtArea.c:3-8                   */
+       ORC_H_SQRTAREA = 1    /* h_shock_sqrtArea.c:3-60 (every branch 2*sqrt|A|) */ };
+typedef struct {
+    int kind;
+    double kappa, gamma, f;
+    double bx, by, scale;
+    int h_shock;
+} orc_prob2d;
+
+void orc_prob2d_velocity(const orc_prob2d *pb, double x, double y, double beta[2]);
+double orc_prob2d_font(const orc_prob2d *pb, double x, double y, double k, double gamma,
+                       double bx, double by);
+
+/* Build_K_F_SUPG.c:5-163 with F_assembly.c:3-51, CAU_ShockCapture.c:3-30.
+ * Inputs: nodes (x,y,type,id), elements (vertex), u[neq+1] (current iterate,
+ * Picard), problem.  Outputs: ke[nel*9] row-major, F[neq+1]. */
+void orc_build_supg2d(int nnodes, int nel, int neq, const double *x, const double *y,
+                      const int *type, const int *id, const int *vertex,
+                      const double *u, const orc_prob2d *pb, double *ke, double *F);
+/* ebe_assembly.c:3-18 is the identity on ke (A[e][9] = ke row-major).
+ * ede_assembly.c:3-35 : Aede[(nedge+1)*4] from ke (zeroed first as setzeros.c). */
+void orc_ede_assembly(int nel, int nedge, const double *ke, const int *order,
+                      const int *edge_by_elem, double *Aede);
+/* csr_assembly.c:3-24 : AA[nnz+1] from ke (zeroed first as setzeros.c) */
+void orc_csr_assembly(int nel, int nnz, const double *ke, const int *csr_by_elem, double *AA);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
