@@ -1,0 +1,286 @@
+#!/usr/bin/env python3
+"""Benchmark of the FEM_CODES solve path on one MI355X.
+
+Workload (the 2D SUPG steady-transport system at its single-GPU size):
+  * unit square, 2048 x 2048 cells split into 8,388,608 P1 triangles
+    (4,198,401 nodes, 4,190,209 equations), SUPG element matrices built on
+    the GPU for a seeded rotating velocity (-y+0.5, x-0.5)*U[0.5,2] and
+    diffusivity 10^U[-6,-3];
+  * one "step" = one FP64 EBE matrix-vector product Q = A P over the whole
+    mesh (ebemv, MatrixVector_Operations/EBE/ebemv.c:4-43), inputs resident
+    in HBM; `value` = algorithmic bytes / time in GB/s;
+  * the same product in FP32 and with the EDE and CSR storage schemes;
+  * time to relative residual 1e-8 of the Diag-preconditioned GMRES(30),
+    FP64 vs FP32-IR, on the constant-velocity SUPG system (beta=(cos t,
+    sin t), k=1e-4, f=1) at 2048^2 and at 64^2.  The rotating system above
+    does not converge with GMRES(30)+Jacobi in a practical number of steps,
+    so it is used for the products only.
+  * cpu_baseline: the reference ebemv compiled from its own sources
+    (oracle/_ref, -Ofast) or, where that library is absent, the oracle
+    restatement, timed on one host core over the full mesh.
+
+Prints one JSON line.  --gpus N>1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("matvec HBM GB/s (% of peak) + time-to-1e-8 solve, FP32-IR vs FP64 GMRES")
+PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def setup_system(F, ctx, n, prob, schemes=("EBE", "EDE", "CSR")):
+    x, y, t, v = F.structured_mesh2d(n)
+    mesh = F.Mesh2D(ctx, x, y, t, v)
+    neq, nel = mesh.neq, mesh.nel
+    lm = mesh.lm()
+    ke = ctx.empty(9 * nel)
+    Fv = ctx.empty(neq + 1)
+    Fv.zero()
+    mesh.build_supg(prob, ke, Fv)
+    ops = {}
+    if "EBE" in schemes:
+        ops["EBE"] = F.Operator.ebe(ctx, neq, lm, 3)
+        ops["EBE"].set_values(ke)
+    if "CSR" in schemes:
+        nnz, IA, JA, cbe = F.csr_pattern(neq, lm)
+        ops["CSR"] = F.Operator.csr(ctx, neq, IA, JA)
+        F.assemble_csr(ops["CSR"], cbe, ke)
+        del IA, JA, cbe
+    if "EDE" in schemes:
+        nedge, order, ebe, lm2 = F.ede_pattern(neq, lm)
+        ops["EDE"] = F.Operator.ede(ctx, neq, lm2)
+        F.assemble_ede(ops["EDE"], order, ebe, ke)
+        del order, ebe, lm2
+    return dict(x=x, y=y, type=t, vertex=v, mesh=mesh, neq=neq, nel=nel, lm=lm, ke=ke, F=Fv,
+                ops=ops)
+
+
+def time_solve(F, ctx, sysd, precision, kmax=30, tol=1e-8, lmax=100000):
+    op = sysd["ops"]["EBE"]
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    B = ctx.empty(sysd["neq"] + 1)
+    F.lib.fg_memcpy_d2d(ctx.h, B.ptr, sysd["F"].ptr, B.nbytes)
+    pc.setup(B)
+    X = ctx.empty(sysd["neq"] + 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=kmax, lmax=lmax, tol=tol, precision=precision)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    return info, wall, X
+
+
+def cpu_baseline(sysd, ke_host, budget_s=12.0):
+    """reference ebemv (or the oracle port) on one core over the full mesh"""
+    import oracle as O
+    neq, nel = sysd["neq"], sysd["nel"]
+    P = np.random.default_rng(7).standard_normal(neq + 1)
+    kind = "port"
+    if O.Reference.available():
+        r = O.Reference(fast=True)
+        r.setup(sysd["x"], sysd["y"], sysd["type"], sysd["vertex"], "EBE")
+        r.set_matrix(ke_host)
+        run = lambda: r.matvec(P)  # noqa: E731
+        kind = "reference"
+    else:
+        o = O.Oracle(fast=True)
+        lm = sysd["lm"]
+        run = lambda: o.ebemv(neq, nel, 3, ke_host, lm, P.copy())  # noqa: E731
+    run()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        run()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or reps >= 200:
+            break
+    sec = el / reps
+    bytes_ = nel * (9 * 8 + 12) + 2 * neq * 8
+    return dict(value=bytes_ / sec / 1e9, unit="GB/s", cores=1, kind=kind,
+                sample=f"{reps} EBE FP64 matvecs of the full {nel}-element mesh "
+                       f"({sec * 1e3:.1f} ms each), -Ofast single thread",
+                ms_per_matvec=sec * 1e3, cpu=platform.processor() or platform.machine(),
+                nproc=os.cpu_count())
+
+
+def cpu_solve_small(n, prob_args):
+    """reference pgmres at the 64^2 constant-velocity system (Diag, GMRES(30))"""
+    import oracle as O
+    o = O.Oracle()
+    x, y, t, v = o.mesh2d(n)
+    neq, idv = o.number_nodes(t)
+    nel = len(v) // 3
+    lm = o.fill_lm(neq, nel, v, t, idv)
+    ke, Fv = o.build_supg2d(x, y, t, idv, v, neq, np.zeros(neq + 1), O.prob2d(*prob_args))
+    if O.Reference.available():
+        r = O.Reference(fast=True)
+        r.setup(x, y, t, v, "EBE")
+        r.set_problem(O.prob2d(*prob_args))
+        r.set_matrix(ke)
+        Fs = r.precond_setup("Diag", Fv)
+        t0 = time.perf_counter()
+        X, steps = r.pgmres(30, 100000, 1e-8, Fs)
+        sec = time.perf_counter() - t0
+        kind = "reference"
+    else:
+        fo = O.Oracle(fast=True)
+        D, iD, Fs = fo.diag_setup_ebe(neq, nel, 3, ke, lm, Fv)
+        t0 = time.perf_counter()
+        X, steps, _, _, _ = fo.pgmres(neq, 30, 100000, 1e-8, Fs, "EBE", (ke, lm, 3, nel), invDiag=iD)
+        sec = time.perf_counter() - t0
+        kind = "port"
+    return dict(seconds=sec, steps=int(steps), kind=kind), X, ke, Fv
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--seed", type=int, default=20260202)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--matvec-only", action="store_true",
+                    help="only the EBE FP64 product (for profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import fem_codes_amd as F
+    ctx = F.Context(0)
+    rng = np.random.default_rng(args.seed)
+    scale = float(rng.uniform(0.5, 2.0))
+    kappa = float(10.0 ** rng.uniform(-6.0, -3.0))
+    theta = float(rng.uniform(0.0, 2 * np.pi))
+    n = args.n
+    t0 = time.perf_counter()
+    rot = F.Supg2dProblem(F.P_ROTATING, kappa, 0.0, 1.0, 0.0, 0.0, scale, F.H_2SQRTAREA)
+    schemes = ("EBE",) if args.matvec_only else ("EBE", "EDE", "CSR")
+    sysd = setup_system(F, ctx, n, rot, schemes)
+    ctx.sync()
+    log(f"setup n={n} neq={sysd['neq']} nel={sysd['nel']} in {time.perf_counter() - t0:.1f}s")
+
+    neq = sysd["neq"]
+    P64 = ctx.to_device(rng.standard_normal(neq + 1))
+    Q64 = ctx.empty(neq + 1)
+    P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
+    Q32 = ctx.empty(neq + 1, np.float32)
+    ebe = sysd["ops"]["EBE"]
+    for _ in range(args.warmup):
+        ebe.apply(P64, Q64)
+    ctx.sync()
+    w0 = time.perf_counter()
+    ms = ebe.time_apply(P64, Q64, reps=args.steps)
+    ctx.sync()
+    wall = time.perf_counter() - w0
+    ms_main = ebe.time_apply(P64, Q64, reps=args.steps, main_only=True)
+    bytes64 = ebe.algorithmic_bytes(False)
+    value = bytes64 / (ms * 1e-3) / 1e9
+    achieved = bytes64 / (ms_main * 1e-3) / 1e9
+    log(f"EBE f64: {ms:.4f} ms/matvec (block kernel {ms_main:.4f} ms) -> {value:.1f} GB/s")
+
+    matvec = {}
+    for name, op in sysd["ops"].items():
+        r = {}
+        for fp32, (P, Q) in ((False, (P64, Q64)), (True, (P32, Q32))):
+            t = op.time_apply(P, Q, reps=args.steps)
+            b = op.algorithmic_bytes(fp32)
+            r["f32" if fp32 else "f64"] = dict(ms=t, GBs=b / (t * 1e-3) / 1e9,
+                                              pct_peak=100 * b / (t * 1e-3) / 1e9 / PEAK_GBS,
+                                              bytes=b)
+        matvec[name] = r
+        log(name, {k: round(v["GBs"], 1) for k, v in r.items()})
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ebe_f64_2048_traffic.json")
+    if os.path.exists(tpath) and n == 2048:
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    solve = {}
+    if not args.no_solve and not args.matvec_only:
+        const = F.Supg2dProblem(F.P_CONST, 1e-4, 0.0, 1.0, float(np.cos(theta)),
+                                float(np.sin(theta)), 1.0, F.H_2SQRTAREA)
+        del sysd["ops"]["EDE"], sysd["ops"]["CSR"]
+        sysd["mesh"].build_supg(const, sysd["ke"], sysd["F"])
+        ebe.set_values(sysd["ke"])
+        res = {}
+        X64 = X32 = None
+        for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+            info, wall_s, X = time_solve(F, ctx, sysd, prec)
+            res[tag] = dict(seconds=wall_s, gpu_ms=info.ms, steps=info.steps, cycles=info.cycles,
+                            true_rel_res=info.true_res / (info.eps / 1e-8), status=info.status)
+            if tag == "fp64":
+                X64 = X.get()
+            else:
+                X32 = X.get()
+            log(tag, res[tag])
+        res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
+        res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(X32[:neq] - X64[:neq]) /
+                                               np.linalg.norm(X64[:neq]))
+        solve[f"const_{n}"] = res
+        # 64^2 (the constant-velocity system at its reference size)
+        small = setup_system(F, ctx, 64, const, ("EBE",))
+        sres = {}
+        for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+            info, wall_s, X = time_solve(F, ctx, small, prec)
+            sres[tag] = dict(seconds=wall_s, steps=info.steps, cycles=info.cycles,
+                             X=X.get())
+        if not args.no_cpu:
+            cs, Xc, _, _ = cpu_solve_small(64, (0, 1e-4, 0.0, 1.0, float(np.cos(theta)),
+                                                float(np.sin(theta)), 1.0, 0))
+            sres["cpu"] = cs
+            for tag in ("fp64", "fp32_ir"):
+                Xg = sres[tag].pop("X")
+                sres[tag]["rel_err_vs_cpu"] = float(np.linalg.norm(Xg[:-1] - Xc[:-1]) /
+                                                    np.linalg.norm(Xc[:-1]))
+        else:
+            for tag in ("fp64", "fp32_ir"):
+                sres[tag].pop("X")
+        solve["const_64"] = sres
+        log("64^2", sres)
+
+    cpu = None
+    if not args.no_cpu and not args.matvec_only:
+        if "ke" in sysd:
+            sysd["mesh"].build_supg(rot, sysd["ke"], None)
+        cpu = cpu_baseline(sysd, sysd["ke"].get())
+        log("cpu", cpu)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2D SUPG transport, {n}x{n} grid of P1 triangles "
+                               f"({sysd['nel']} elements, {neq} equations), EBE 3x3 FP64 matvec",
+                   "velocity_scale": scale, "diffusivity": kappa, "pct_of_peak": 100 * value / PEAK_GBS,
+                   "wall_ms_per_step": wall * 1e3 / args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_ebe_apply<3,4,double>", "algorithmic_bytes": bytes64,
+                     "ms": ms_main},
+        "cpu_baseline": cpu,
+        "matvec": matvec,
+        "solve": solve,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,457 @@
+"""fem_codes_amd -- MI355X-native solve path of FEM_CODES.
+
+The product is ``libfemgpu.so`` (HIP kernels for gfx950 behind the C ABI in
+``include/femgpu.h``).  This module is the Python host layer over that ABI:
+ctypes bindings, device arrays and the operator / preconditioner / GMRES
+objects that mirror the reference's plugin table
+
+    FemFunctions->ProductMatrixVector = ebemv | edemv | csrmv
+    FemFunctions->precond_setup / precond = Diag_precond{_EBE,_EDE,_CSR}_setup / Diag_precond
+    FemOtherFunctions->solver = pgmres
+
+(01_SS_Transport_Equation/2D/01_CommonFiles/setMatrixVectorProductType.c:6-17,
+setPreconditioner.c:36-55, setSolver.c:8).  There is no CPU fallback: every
+call goes to the GPU library, and a missing library raises at import.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfemgpu.so")
+
+FG_OK, FG_ERR_HIP, FG_ERR_ARG, FG_ERR_BREAKDOWN, FG_ERR_NOCONV, FG_ERR_NOMEM = range(6)
+PREC_NONE, PREC_DIAG, PREC_JACOBI = 0, 1, 2
+GMRES_FP64, GMRES_FP32_IR = 0, 1
+
+
+def build(verbose: bool = False) -> None:
+    """Compile libfemgpu.so for gfx950 (hipcc cross-compiles without a GPU)."""
+    out = subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc"), "-j8"],
+                         capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("libfemgpu build failed:\n" + out.stdout + out.stderr)
+    if verbose:
+        print(out.stdout, out.stderr)
+
+
+class FemGpuError(RuntimeError):
+    pass
+
+
+class GmresOpts(C.Structure):
+    _fields_ = [("kmax", C.c_int), ("lmax", C.c_int), ("tol", C.c_double),
+                ("precision", C.c_int), ("inner_tol", C.c_double), ("hist_cap", C.c_int),
+                ("rho_hist", C.c_void_p), ("e0_hist", C.c_void_p)]
+
+
+class GmresInfo(C.Structure):
+    _fields_ = [("steps", C.c_int), ("cycles", C.c_int), ("rho", C.c_double),
+                ("eps", C.c_double), ("true_res", C.c_double), ("status", C.c_int),
+                ("ms", C.c_double)]
+
+
+_vp, _i, _d = C.c_void_p, C.c_int, C.c_double
+_ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+# name -> (restype, argtypes); every symbol declared in include/femgpu.h
+SIGNATURES = {
+    "fg_last_error": (C.c_char_p, []),
+    "fg_version": (C.c_char_p, []),
+    "fg_ctx_create": (_i, [_i, C.POINTER(_vp)]),
+    "fg_ctx_destroy": (_i, [_vp]),
+    "fg_ctx_sync": (_i, [_vp]),
+    "fg_malloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "fg_free": (_i, [_vp, _vp]),
+    "fg_memcpy_h2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "fg_memcpy_d2h": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "fg_memcpy_d2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
+    "fg_memset": (_i, [_vp, _vp, _i, C.c_size_t]),
+    "fg_cast_f64_to_f32": (_i, [_vp, _i, _vp, _vp]),
+    "fg_cast_f32_to_f64": (_i, [_vp, _i, _vp, _vp]),
+    "fg_op_ebe_create": (_i, [_vp, _i, _i, _i, _ip, _i, C.POINTER(_vp)]),
+    "fg_op_ede_create": (_i, [_vp, _i, _i, _ip, _i, C.POINTER(_vp)]),
+    "fg_op_csr_create": (_i, [_vp, _i, _i, _ip, _ip, C.POINTER(_vp)]),
+    "fg_op_destroy": (_i, [_vp]),
+    "fg_op_set_values_host": (_i, [_vp, _dp]),
+    "fg_op_set_values_dev": (_i, [_vp, _vp]),
+    "fg_op_apply_f64": (_i, [_vp, _vp, _vp]),
+    "fg_op_apply_f32": (_i, [_vp, _vp, _vp]),
+    "fg_op_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
+                        C.POINTER(C.c_longlong)]),
+    "fg_op_algorithmic_bytes": (_d, [_vp, _i]),
+    "fg_op_time_apply": (_i, [_vp, _i, _vp, _vp, _i, _i, C.POINTER(_d)]),
+    "fg_prec_create": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "fg_prec_setup": (_i, [_vp, _vp]),
+    "fg_prec_apply_f64": (_i, [_vp, _vp, _vp]),
+    "fg_prec_apply_f32": (_i, [_vp, _vp, _vp]),
+    "fg_prec_get_diag": (_i, [_vp, _vp, _vp]),
+    "fg_prec_destroy": (_i, [_vp]),
+    "fg_gmres": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(GmresOpts), C.POINTER(GmresInfo)]),
+    "fg_number_nodes": (_i, [_i, _ip, _ip]),
+    "fg_fill_lm": (_i, [_i, _i, _i, _ip, _ip, _ip, _ip]),
+    "fg_csr_pattern": (_i, [_i, _i, _ip, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
+    "fg_ede_pattern": (_i, [_i, _i, _ip, _vp, _vp, _vp, C.POINTER(_i)]),
+    "fg_mesh2d_create": (_i, [_vp, _i, _i, _dp, _dp, _ip, _ip, C.POINTER(_vp)]),
+    "fg_mesh2d_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "fg_mesh2d_get_lm": (_i, [_vp, _ip]),
+    "fg_mesh2d_destroy": (_i, [_vp]),
+    "fg_supg2d_build": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "fg_op_assemble_csr": (_i, [_vp, _i, _ip, _vp]),
+    "fg_op_assemble_ede": (_i, [_vp, _i, _ip, _ip, _vp]),
+}
+
+
+class Supg2dProblem(C.Structure):
+    """fg_supg2d_problem: the FemFunctions callbacks of one SS-2D problem."""
+    _fields_ = [("kind", C.c_int), ("kappa", C.c_double), ("gamma", C.c_double),
+                ("f", C.c_double), ("bx", C.c_double), ("by", C.c_double),
+                ("scale", C.c_double), ("h_shock", C.c_int)]
+
+
+P_CONST, P_ROTATING, P_VALIDACAO = 0, 1, 2
+H_2SQRTAREA, H_SQRTAREA = 0, 1
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise FemGpuError(f"{LIB_PATH} is missing: run fem_codes_amd.build() "
+                          "(or __graft_entry__.build()) first")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc: int, what: str, ok=(FG_OK,)) -> int:
+    if rc not in ok:
+        raise FemGpuError(f"{what}: status {rc}: {lib.fg_last_error().decode()}")
+    return rc
+
+
+# ---------------------------------------------------------------------------
+class Context:
+    """A device, its stream and reduction scratch (fg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib.fg_ctx_create(device, C.byref(h)), "fg_ctx_create")
+        self.h = h
+
+    def sync(self):
+        _check(lib.fg_ctx_sync(self.h), "fg_ctx_sync")
+
+    def close(self):
+        if self.h:
+            lib.fg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # device arrays
+    def empty(self, n: int, dtype=np.float64) -> "DeviceArray":
+        return DeviceArray(self, n, dtype)
+
+    def to_device(self, a: np.ndarray, dtype=None) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = DeviceArray(self, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+
+class DeviceArray:
+    """A flat device buffer owned by a Context (fg_malloc/fg_free)."""
+
+    def __init__(self, ctx: Context, n: int, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        p = _vp()
+        _check(lib.fg_malloc(ctx.h, max(1, self.n * self.dtype.itemsize), C.byref(p)),
+               "fg_malloc")
+        self.ptr = p.value
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _check(lib.fg_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes), "h2d")
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        _check(lib.fg_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes), "d2h")
+        return out
+
+    def zero(self):
+        _check(lib.fg_memset(self.ctx.h, self.ptr, 0, self.nbytes), "memset")
+
+    def free(self):
+        if self.ptr:
+            lib.fg_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+class Operator:
+    """A stored matrix on the device: EBE, EDE or CSR (fg_op)."""
+
+    def __init__(self, ctx: Context, h, kind: str):
+        self.ctx, self.h, self.kind = ctx, h, kind
+        neq, nit, k, nnz = _i(), _i(), _i(), C.c_longlong()
+        lib.fg_op_info(h, C.byref(neq), C.byref(nit), C.byref(k), C.byref(nnz))
+        self.neq, self.nitems, self.k, self.nnz = neq.value, nit.value, k.value, nnz.value
+
+    @classmethod
+    def ebe(cls, ctx, neq, lm, k, elems_per_block=0):
+        """ebemv family: lm (nel*k) with sink = neq."""
+        lm = np.ascontiguousarray(lm, np.int32)
+        h = _vp()
+        _check(lib.fg_op_ebe_create(ctx.h, neq, lm.size // k, k, lm, elems_per_block,
+                                    C.byref(h)), "fg_op_ebe_create")
+        return cls(ctx, h, "EBE")
+
+    @classmethod
+    def ede(cls, ctx, neq, lm2, edges_per_block=0):
+        """edemv: lm2 (nedge*2) with sink = neq."""
+        lm2 = np.ascontiguousarray(lm2, np.int32)
+        h = _vp()
+        _check(lib.fg_op_ede_create(ctx.h, neq, lm2.size // 2, lm2, edges_per_block,
+                                    C.byref(h)), "fg_op_ede_create")
+        return cls(ctx, h, "EDE")
+
+    @classmethod
+    def csr(cls, ctx, neq, IA, JA):
+        """csrmv: IA (neq+1), JA (nnz)."""
+        IA = np.ascontiguousarray(IA, np.int32)
+        JA = np.ascontiguousarray(JA, np.int32)
+        h = _vp()
+        _check(lib.fg_op_csr_create(ctx.h, neq, JA.size, IA, JA, C.byref(h)),
+               "fg_op_csr_create")
+        return cls(ctx, h, "CSR")
+
+    def set_values(self, vals):
+        if isinstance(vals, DeviceArray):
+            _check(lib.fg_op_set_values_dev(self.h, vals.ptr), "fg_op_set_values_dev")
+        else:
+            _check(lib.fg_op_set_values_host(self.h, np.ascontiguousarray(vals, np.float64)),
+                   "fg_op_set_values_host")
+
+    def apply(self, P: DeviceArray, Q: DeviceArray):
+        fn = lib.fg_op_apply_f32 if P.dtype == np.float32 else lib.fg_op_apply_f64
+        _check(fn(self.h, P.ptr, Q.ptr), "fg_op_apply")
+
+    def matvec(self, P: np.ndarray, dtype=np.float64) -> np.ndarray:
+        """host convenience: Q = A P (P has neq+1 entries)."""
+        dP = self.ctx.to_device(np.asarray(P), dtype)
+        dQ = self.ctx.empty(self.neq + 1, dtype)
+        self.apply(dP, dQ)
+        return dQ.get()
+
+    def algorithmic_bytes(self, fp32=False) -> float:
+        return lib.fg_op_algorithmic_bytes(self.h, int(fp32))
+
+    def time_apply(self, P: DeviceArray, Q: DeviceArray, reps=20, main_only=False) -> float:
+        """mean ms per apply, HIP events on the library stream"""
+        ms = _d()
+        _check(lib.fg_op_time_apply(self.h, int(P.dtype == np.float32), P.ptr, Q.ptr, reps,
+                                    int(main_only), C.byref(ms)), "fg_op_time_apply")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib.fg_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Preconditioner:
+    """fg_prec: NONE, DIAG (global diagonal) or JACOBI (element-local)."""
+
+    def __init__(self, op: Operator, kind: int):
+        h = _vp()
+        _check(lib.fg_prec_create(op.h, kind, C.byref(h)), "fg_prec_create")
+        self.op, self.h, self.kind = op, h, kind
+
+    def setup(self, F: DeviceArray | None):
+        _check(lib.fg_prec_setup(self.h, F.ptr if F is not None else None), "fg_prec_setup")
+
+    def apply(self, p: DeviceArray, z: DeviceArray):
+        fn = lib.fg_prec_apply_f32 if p.dtype == np.float32 else lib.fg_prec_apply_f64
+        _check(fn(self.h, p.ptr, z.ptr), "fg_prec_apply")
+
+    def diag(self):
+        n = self.op.neq + 1
+        D, iD = np.zeros(n), np.zeros(n)
+        _check(lib.fg_prec_get_diag(self.h, D.ctypes.data, iD.ctypes.data), "get_diag")
+        return D, iD
+
+    def close(self):
+        if self.h:
+            lib.fg_prec_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gmres(ctx: Context, op: Operator, prec: Preconditioner | None, B: DeviceArray,
+          X: DeviceArray, kmax=30, lmax=1000, tol=1e-8, precision=GMRES_FP64,
+          inner_tol=0.0, hist_cap=0, e0_cap=0):
+    """pgmres (Solvers_and_Preconditioners/pgmres.c:4-211) on the device.
+
+    Returns (info, rho_hist, e0_hist)."""
+    rho = np.zeros(max(1, hist_cap))
+    e0 = np.zeros(max(1, e0_cap if e0_cap else lmax + 1))
+    o = GmresOpts(kmax, lmax, tol, precision, inner_tol, hist_cap,
+                  rho.ctypes.data if hist_cap else None,
+                  e0.ctypes.data if e0_cap else None)
+    info = GmresInfo()
+    rc = lib.fg_gmres(ctx.h, op.h, prec.h if prec is not None else None, B.ptr, X.ptr,
+                      C.byref(o), C.byref(info))
+    _check(rc, "fg_gmres", ok=(FG_OK, FG_ERR_NOCONV))
+    return info, rho[:min(info.steps, hist_cap)].copy(), e0[:info.cycles + 1].copy()
+
+
+# ---------------------------------------------------------------------------
+# mesh structures (host; Preprocess.c / Fill_LM.c / csr_ / ede_Initialization.c)
+# ---------------------------------------------------------------------------
+def number_nodes(type_):
+    type_ = np.ascontiguousarray(type_, np.int32)
+    idv = np.zeros(type_.size, np.int32)
+    neq = lib.fg_number_nodes(type_.size, type_, idv)
+    return neq, idv
+
+
+def fill_lm(neq, vertex, type_, idv, nnoel=3):
+    vertex = np.ascontiguousarray(vertex, np.int32)
+    lm = np.zeros(vertex.size, np.int32)
+    _check(lib.fg_fill_lm(neq, vertex.size // nnoel, nnoel, vertex,
+                          np.ascontiguousarray(type_, np.int32), idv, lm), "fg_fill_lm")
+    return lm
+
+
+def csr_pattern(neq, lm):
+    """-> (nnz, IA, JA, csr_by_elem)"""
+    lm = np.ascontiguousarray(lm, np.int32)
+    nel = lm.size // 3
+    nnz = C.c_longlong()
+    _check(lib.fg_csr_pattern(neq, nel, lm, None, None, None, C.byref(nnz)), "csr_pattern")
+    IA, JA = np.zeros(neq + 1, np.int32), np.zeros(nnz.value, np.int32)
+    cbe = np.zeros(9 * nel, np.int32)
+    _check(lib.fg_csr_pattern(neq, nel, lm, IA.ctypes.data, JA.ctypes.data, cbe.ctypes.data,
+                              C.byref(nnz)), "csr_pattern")
+    return nnz.value, IA, JA, cbe
+
+
+def ede_pattern(neq, lm):
+    """-> (nedge, order, edge_by_elem, lm2)"""
+    lm = np.ascontiguousarray(lm, np.int32)
+    nel = lm.size // 3
+    ne = _i()
+    _check(lib.fg_ede_pattern(neq, nel, lm, None, None, None, C.byref(ne)), "ede_pattern")
+    order, ebe = np.zeros(3 * nel, np.int32), np.zeros(3 * nel, np.int32)
+    lm2 = np.zeros(2 * ne.value, np.int32)
+    _check(lib.fg_ede_pattern(neq, nel, lm, order.ctypes.data, ebe.ctypes.data,
+                              lm2.ctypes.data, C.byref(ne)), "ede_pattern")
+    return ne.value, order, ebe, lm2
+
+
+def structured_mesh2d(n: int):
+    """Unit square, n x n cells split along the "direita" diagonal with the
+    node and element numbering of Gerador_de_Malhas/Structured_Mesh/geraMalha.c:112-176;
+    boundary nodes Type 0 (Dirichlet), inner nodes Type 1.
+    -> (x, y, type, vertex[nel*3])"""
+    W = n + 1
+    i = np.arange(W, dtype=np.float64)
+    h = 1.0 / n
+    x = np.tile(0.0 + i * h, W)
+    y = np.repeat(0.0 + i * h, W)
+    bnd = (np.abs(x - 1.0) <= 1e-15) | (np.abs(x) <= 1e-15) | \
+          (np.abs(y - 1.0) <= 1e-15) | (np.abs(y) <= 1e-15)
+    type_ = np.where(bnd, 0, 1).astype(np.int32)
+    jj, ii = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    a = (jj * W + ii).ravel().astype(np.int64)
+    v = np.empty((n * n, 2, 3), np.int64)
+    v[:, 0, 0], v[:, 0, 1], v[:, 0, 2] = a, a + 1, a + 1 + W
+    v[:, 1, 0], v[:, 1, 1], v[:, 1, 2] = a, a + 1 + W, a + W
+    return x, y, type_, v.reshape(-1).astype(np.int32)
+
+
+class Mesh2D:
+    """fg_mesh2d: device copy of a P1 triangle mesh + SUPG assembly."""
+
+    def __init__(self, ctx: Context, x, y, type_, vertex):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        type_ = np.ascontiguousarray(type_, np.int32)
+        vertex = np.ascontiguousarray(vertex, np.int32)
+        h = _vp()
+        _check(lib.fg_mesh2d_create(ctx.h, x.size, vertex.size // 3, x, y, type_, vertex,
+                                    C.byref(h)), "fg_mesh2d_create")
+        self.ctx, self.h = ctx, h
+        neq, nel, nn = _i(), _i(), _i()
+        lib.fg_mesh2d_info(h, C.byref(neq), C.byref(nel), C.byref(nn))
+        self.neq, self.nel, self.nnodes = neq.value, nel.value, nn.value
+
+    def lm(self):
+        out = np.zeros(3 * self.nel, np.int32)
+        _check(lib.fg_mesh2d_get_lm(self.h, out), "get_lm")
+        return out
+
+    def build_supg(self, prob: Supg2dProblem, ke: DeviceArray, F: DeviceArray | None,
+                   u: DeviceArray | None = None):
+        """Build_K_F_SUPG on the device: ke (nel*9), F (neq+1)."""
+        _check(lib.fg_supg2d_build(self.h, C.byref(prob), u.ptr if u is not None else None,
+                                   ke.ptr, F.ptr if F is not None else None), "fg_supg2d_build")
+
+    def close(self):
+        if self.h:
+            lib.fg_mesh2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def assemble_csr(op: Operator, csr_by_elem, ke: DeviceArray):
+    cbe = np.ascontiguousarray(csr_by_elem, np.int32)
+    _check(lib.fg_op_assemble_csr(op.h, cbe.size // 9, cbe, ke.ptr), "assemble_csr")
+
+
+def assemble_ede(op: Operator, order, edge_by_elem, ke: DeviceArray):
+    order = np.ascontiguousarray(order, np.int32)
+    ebe = np.ascontiguousarray(edge_by_elem, np.int32)
+    _check(lib.fg_op_assemble_ede(op.h, order.size // 3, order, ebe, ke.ptr), "assemble_ede")

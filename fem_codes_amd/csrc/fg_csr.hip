@@ -1,0 +1,136 @@
+// fg_csr.hip -- CSR operator (MatrixVector_Operations/CSR/csrmv.c:3-24).
+//
+// Row-block ("CSR-stream") SpMV: the host cuts the rows into blocks of at
+// most 256 rows and FG_CSR_NNZB nonzeros.  A workgroup streams its block's
+// AA/JA range with coalesced loads, stores the products AA[j]*P[JA[j]] in
+// LDS, and then each thread sums one row's products from LDS in slot order.
+// The per-row summation order is the reference's (csrmv.c:15-16: Q[i]=0,
+// then Q[i] += AA[j]*P[JA[j]] for ascending j), so with FMA contraction
+// disabled the FP64 result is bitwise the reference's.  A row longer than a
+// block (never produced by P1 meshes) gets a block of its own and a tree sum.
+#include <vector>
+
+#include "fg_internal.h"
+
+constexpr int FG_CSR_NNZB = 2048;
+
+template <typename T>
+__global__ __launch_bounds__(FG_TPB) void k_csr_stream(
+    int nrb, const int *__restrict__ rb, const int *__restrict__ IA,
+    const int *__restrict__ JA, const T *__restrict__ AA, const T *__restrict__ P,
+    T *__restrict__ Q, int neq, const int *skip)
+{
+    if (skip && *skip) return;
+    __shared__ T s[FG_CSR_NNZB];
+    __shared__ T ws[FG_TPB / 64];
+    const int b = fg_xcd_remap(blockIdx.x, nrb);
+    const int r0 = rb[b], r1 = rb[b + 1];
+    const int j0 = IA[r0], j1 = IA[r1];
+    if (j1 - j0 <= FG_CSR_NNZB) {
+        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB) s[j - j0] = AA[j] * P[JA[j]];
+        __syncthreads();
+        for (int r = r0 + threadIdx.x; r < r1; r += FG_TPB) {
+            T acc = T(0);
+            const int a = IA[r] - j0, e = IA[r + 1] - j0;
+            for (int j = a; j < e; j++) acc = acc + s[j];
+            Q[r] = acc;
+        }
+    } else {
+        T acc = T(0);
+        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB) acc = acc + AA[j] * P[JA[j]];
+        acc = fg_block_sum(acc, ws);
+        if (threadIdx.x == 0) Q[r0] = acc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) Q[neq] = T(0);
+}
+
+int fg_csr_apply(fg_op *op, int fp32, const void *P, void *Q, const int *skip)
+{
+    fg_ctx *c = op->ctx;
+    if (op->nrb == 0) {
+        if (fp32) FG_HIP(hipMemsetAsync((float *)Q + op->neq, 0, sizeof(float), c->stream));
+        else FG_HIP(hipMemsetAsync((double *)Q + op->neq, 0, sizeof(double), c->stream));
+        return FG_OK;
+    }
+    if (fp32)
+        FG_LAUNCH(k_csr_stream<float>, op->nrb, FG_TPB, 0, c->stream, op->nrb, op->d_rb,
+                  op->d_IA, op->d_JA, (const float *)op->d_AA32, (const float *)P, (float *)Q,
+                  op->neq, skip);
+    else
+        FG_LAUNCH(k_csr_stream<double>, op->nrb, FG_TPB, 0, c->stream, op->nrb, op->d_rb,
+                  op->d_IA, op->d_JA, (const double *)op->d_AA, (const double *)P,
+                  (double *)Q, op->neq, skip);
+    return FG_OK;
+}
+
+extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, const int *JA,
+                                fg_op **out)
+{
+    FG_ARG(c && out && IA && (JA || nnz == 0), "ctx/out/IA/JA");
+    FG_ARG(neq >= 0 && nnz >= 0 && IA[0] == 0 && IA[neq] == nnz, "IA[0]=0 and IA[neq]=nnz");
+    for (int r = 0; r < neq; r++)
+        FG_ARG(IA[r + 1] >= IA[r], "IA must be non-decreasing");
+    for (int j = 0; j < nnz; j++)
+        FG_ARG(JA[j] >= 0 && JA[j] <= neq, "JA entries must lie in [0, neq]");
+    FG_HIP(hipSetDevice(c->device));
+    std::vector<int> rb{0};
+    int rows = 0, nz = 0;
+    for (int r = 0; r < neq; r++) {
+        int rn = IA[r + 1] - IA[r];
+        if (rows > 0 && (rows == FG_TPB || nz + rn > FG_CSR_NNZB)) {
+            rb.push_back(r);
+            rows = 0;
+            nz = 0;
+        }
+        rows++;
+        nz += rn;
+    }
+    if (rows > 0) rb.push_back(neq);
+    fg_op *op = new fg_op();
+    op->ctx = c;
+    op->kind = FG_OP_CSR;
+    op->neq = neq;
+    op->nitems = neq;
+    op->nnz = nnz;
+    op->nrb = (int)rb.size() - 1;
+    op->h_IA.assign(IA, IA + neq + 1);
+    op->h_JA.assign(JA, JA + nnz);
+    FG_HIP(hipMalloc(&op->d_rb, sizeof(int) * rb.size()));
+    FG_HIP(hipMemcpy(op->d_rb, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice));
+    FG_HIP(hipMalloc(&op->d_IA, sizeof(int) * (neq + 1)));
+    FG_HIP(hipMemcpy(op->d_IA, IA, sizeof(int) * (neq + 1), hipMemcpyHostToDevice));
+    FG_HIP(hipMalloc(&op->d_JA, sizeof(int) * (nnz ? nnz : 1)));
+    if (nnz) FG_HIP(hipMemcpy(op->d_JA, JA, sizeof(int) * nnz, hipMemcpyHostToDevice));
+    FG_HIP(hipMalloc(&op->d_AA, sizeof(double) * (nnz ? nnz : 1)));
+    FG_HIP(hipMalloc(&op->d_AA32, sizeof(float) * (nnz ? nnz : 1)));
+    FG_HIP(hipMemset(op->d_AA, 0, sizeof(double) * (nnz ? nnz : 1)));
+    FG_HIP(hipMemset(op->d_AA32, 0, sizeof(float) * (nnz ? nnz : 1)));
+    *out = op;
+    return FG_OK;
+}
+
+void fg_csr_free(fg_op *op)
+{
+    hipFree(op->d_rb); hipFree(op->d_IA); hipFree(op->d_JA); hipFree(op->d_AA); hipFree(op->d_AA32);
+}
+
+// CSR/Diag_precond_CSR_setup.c:10-14: the last slot of row i with column i
+__global__ void k_csr_diag(int neq, const int *__restrict__ IA, const int *__restrict__ JA,
+                           const double *__restrict__ AA, double *__restrict__ D)
+{
+    for (int r = blockIdx.x * FG_TPB + threadIdx.x; r < neq; r += gridDim.x * FG_TPB) {
+        double d = D[r];
+        for (int j = IA[r]; j < IA[r + 1]; j++)
+            if (JA[j] == r) d = AA[j];
+        D[r] = d;
+    }
+}
+
+int fg_csr_diag(fg_op *op, double *D)
+{
+    fg_ctx *c = op->ctx;
+    if (op->neq == 0) return FG_OK;
+    FG_LAUNCH(k_csr_diag, fg_vec_grid(c, op->neq), FG_TPB, 0, c->stream, op->neq, op->d_IA,
+              op->d_JA, op->d_AA, D);
+    return FG_OK;
+}

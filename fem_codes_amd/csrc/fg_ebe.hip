@@ -1,0 +1,640 @@
+// fg_ebe.hip -- element-by-element (and edge-by-edge) operator.
+//
+// Reference: MatrixVector_Operations/EBE/ebemv.c:4-43 (k=3),
+// ebemv3Dndof1.c:3-45 (k=4), ebemvNDOF4.c:5-253 (k=12,16) and
+// EDE/edemv.c:4-39 (edges, k=2).  The reference walks the elements in order
+// and scatters each element's k row sums into Q.
+//
+// MI355X design: the elements are cut into blocks of up to 256*EPT
+// mesh-adjacent elements (breadth-first growth over shared dofs).  One
+// workgroup owns one block:
+//   1. the block's distinct dofs (sorted) are gathered from P into LDS;
+//   2. every thread streams its elements' k*k values and u16 local dof
+//      indices (both stored per block as [entry][element], so each load is
+//      coalesced) and forms the k row sums in registers;
+//   3. the row sums are added into LDS accumulators colour by colour: the
+//      elements of a block are greedily coloured so that no two elements of
+//      one colour share a dof, which makes the LDS scatter conflict-free and
+//      the result independent of thread timing (bitwise reproducible);
+//   4. dofs touched only by this block are written to Q; dofs shared with
+//      other blocks write a partial sum to a halo slot, and a small second
+//      kernel adds the partials of each shared dof in block order.
+// No atomics anywhere; HBM traffic is the element values, the u16 indices,
+// one gather of P per block-local dof and one write of Q.
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "fg_internal.h"
+
+// ---------------------------------------------------------------------------
+// host: block plan
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Plan {
+    int nblk = 0, maxc = 0, max_nloc = 0, nkept = 0;
+    std::vector<int> eoff{0}, doff{0};
+    std::vector<uint16_t> coff;
+    std::vector<int> dofs, slots;
+    std::vector<uint16_t> lml;
+    std::vector<long long> kpos;   // [nel] blocked base index (value q at kpos + q*stride), -1 dropped
+    std::vector<int> kstride;      // [nel] elements in the element's block
+    std::vector<int> hdofs, hoff{0}, zdofs;
+    int nslots = 0;
+};
+
+int build_plan(int neq, int nel, int k, const int *lm, int eb, Plan &pl)
+{
+    // elements whose dofs are all Dirichlet add nothing to Q (their rows go
+    // to the sink slot) and are dropped from the operator
+    std::vector<int> state(nel, -1);
+    for (int e = 0; e < nel; e++) {
+        bool any = false;
+        for (int j = 0; j < k; j++) {
+            int g = lm[(size_t)e * k + j];
+            if (g < 0 || g > neq) {
+                fg_set_error("lm[%d][%d]=%d outside [0,%d]", e, j, g, neq);
+                return FG_ERR_ARG;
+            }
+            any |= (g != neq);
+        }
+        if (!any) state[e] = -2;
+    }
+    // dof -> element incidence, elements ascending
+    std::vector<int> ioff(neq + 1, 0);
+    for (int e = 0; e < nel; e++)
+        if (state[e] == -1)
+            for (int j = 0; j < k; j++) {
+                int g = lm[(size_t)e * k + j];
+                if (g != neq) ioff[g + 1]++;
+            }
+    for (int g = 0; g < neq; g++) ioff[g + 1] += ioff[g];
+    std::vector<int> inc(ioff[neq]);
+    {
+        std::vector<int> fill(ioff.begin(), ioff.end() - 1);
+        for (int e = 0; e < nel; e++)
+            if (state[e] == -1)
+                for (int j = 0; j < k; j++) {
+                    int g = lm[(size_t)e * k + j];
+                    if (g != neq) inc[fill[g]++] = e;
+                }
+    }
+    // breadth-first blocks
+    std::vector<int> order;
+    order.reserve(nel);
+    std::vector<int> mark(nel, -1), queue;
+    queue.reserve(8 * (size_t)eb);
+    int seed = 0;
+    for (;;) {
+        while (seed < nel && state[seed] != -1) seed++;
+        if (seed >= nel) break;
+        int b = pl.nblk++;
+        queue.clear();
+        size_t head = 0;
+        queue.push_back(seed);
+        mark[seed] = b;
+        int cnt = 0;
+        while (head < queue.size() && cnt < eb) {
+            int e = queue[head++];
+            state[e] = b;
+            order.push_back(e);
+            cnt++;
+            for (int j = 0; j < k; j++) {
+                int g = lm[(size_t)e * k + j];
+                if (g == neq) continue;
+                for (int q = ioff[g]; q < ioff[g + 1]; q++) {
+                    int e2 = inc[q];
+                    if (state[e2] == -1 && mark[e2] != b) {
+                        mark[e2] = b;
+                        queue.push_back(e2);
+                    }
+                }
+            }
+        }
+        pl.eoff.push_back((int)order.size());
+    }
+    pl.nkept = (int)order.size();
+    const int nb = pl.nblk;
+
+    // per block: local dofs, colouring, local indices
+    std::vector<int> locid(neq, -1);
+    std::vector<int> blk_of_dofcount(neq, 0);
+    std::vector<std::vector<uint16_t>> bcoff(nb);
+    std::vector<int> newpos(pl.nkept);
+    pl.lml.resize((size_t)k * pl.nkept);
+    pl.kpos.assign(nel, -1);
+    pl.kstride.assign(nel, 0);
+    std::vector<int> ld, col;
+    std::vector<uint64_t> mk;
+    int maxc = 0;
+    for (int b = 0; b < nb; b++) {
+        int e0 = pl.eoff[b], ne = pl.eoff[b + 1] - e0;
+        ld.clear();
+        for (int i = 0; i < ne; i++) {
+            int e = order[e0 + i];
+            for (int j = 0; j < k; j++) {
+                int g = lm[(size_t)e * k + j];
+                if (g != neq && locid[g] == -1) {
+                    locid[g] = -2;
+                    ld.push_back(g);
+                }
+            }
+        }
+        std::sort(ld.begin(), ld.end());
+        int nloc = (int)ld.size();
+        if (nloc >= 65535) {
+            fg_set_error("block with %d local dofs exceeds u16 indexing", nloc);
+            return FG_ERR_ARG;
+        }
+        for (int l = 0; l < nloc; l++) locid[ld[l]] = l;
+        pl.max_nloc = std::max(pl.max_nloc, nloc);
+        // greedy colouring in breadth-first order
+        mk.assign(nloc, 0);
+        col.assign(ne, 0);
+        int nc = 0;
+        for (int i = 0; i < ne; i++) {
+            int e = order[e0 + i];
+            uint64_t used = 0;
+            for (int j = 0; j < k; j++) {
+                int g = lm[(size_t)e * k + j];
+                if (g != neq) used |= mk[locid[g]];
+            }
+            if (used == ~0ull) {
+                fg_set_error("element colouring needs more than 64 colours");
+                return FG_ERR_ARG;
+            }
+            int c = __builtin_ctzll(~used);
+            col[i] = c;
+            nc = std::max(nc, c + 1);
+            for (int j = 0; j < k; j++) {
+                int g = lm[(size_t)e * k + j];
+                if (g != neq) mk[locid[g]] |= (1ull << c);
+            }
+        }
+        maxc = std::max(maxc, nc);
+        // stable counting sort of the block's elements by colour
+        std::vector<uint16_t> cnt(nc + 1, 0);
+        for (int i = 0; i < ne; i++) cnt[col[i] + 1]++;
+        for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
+        bcoff[b] = cnt;
+        std::vector<uint16_t> fillc(cnt.begin(), cnt.end() - 1);
+        std::vector<int> tmp(ne);
+        for (int i = 0; i < ne; i++) tmp[fillc[col[i]]++] = order[e0 + i];
+        for (int i = 0; i < ne; i++) {
+            int e = tmp[i];
+            order[e0 + i] = e;
+            pl.kpos[e] = (long long)k * k * e0 + i;
+            pl.kstride[e] = ne;
+            for (int j = 0; j < k; j++) {
+                int g = lm[(size_t)e * k + j];
+                pl.lml[(size_t)k * e0 + (size_t)j * ne + i] = (uint16_t)(g == neq ? nloc : locid[g]);
+            }
+        }
+        for (int l = 0; l < nloc; l++) {
+            pl.dofs.push_back(ld[l]);
+            blk_of_dofcount[ld[l]]++;
+            locid[ld[l]] = -1;
+        }
+        pl.doff.push_back((int)pl.dofs.size());
+    }
+    pl.maxc = maxc;
+    pl.coff.assign((size_t)nb * (maxc + 1), 0);
+    for (int b = 0; b < nb; b++) {
+        int ne = pl.eoff[b + 1] - pl.eoff[b];
+        for (int c = 0; c <= maxc; c++)
+            pl.coff[(size_t)b * (maxc + 1) + c] =
+                (uint16_t)(c < (int)bcoff[b].size() ? bcoff[b][c] : ne);
+    }
+    // shared dofs -> halo slots (blocks in ascending order per dof)
+    std::vector<int> hidx(neq, -1);
+    for (int g = 0; g < neq; g++) {
+        if (blk_of_dofcount[g] > 1) {
+            hidx[g] = (int)pl.hdofs.size();
+            pl.hdofs.push_back(g);
+            pl.hoff.push_back(pl.hoff.back() + blk_of_dofcount[g]);
+        } else if (blk_of_dofcount[g] == 0) {
+            pl.zdofs.push_back(g);
+        }
+    }
+    pl.nslots = pl.hoff.back();
+    std::vector<int> used(pl.hdofs.size(), 0);
+    pl.slots.resize(pl.dofs.size());
+    for (size_t l = 0; l < pl.dofs.size(); l++) {
+        int g = pl.dofs[l];
+        int h = hidx[g];
+        pl.slots[l] = h < 0 ? -1 : pl.hoff[h] + used[h]++;
+    }
+    return FG_OK;
+}
+
+template <typename T>
+int upload(const std::vector<T> &v, T **d)
+{
+    size_t n = v.size() ? v.size() : 1;
+    FG_HIP(hipMalloc(d, n * sizeof(T)));
+    if (!v.empty()) FG_HIP(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FG_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device: apply
+// ---------------------------------------------------------------------------
+template <int K, int EPT, typename T>
+__global__ __launch_bounds__(FG_TPB) void k_ebe_apply(
+    int nblk, int maxc, const int *__restrict__ eoff, const int *__restrict__ doff,
+    const uint16_t *__restrict__ coff, const int *__restrict__ dofs,
+    const int *__restrict__ slots, const uint16_t *__restrict__ lml,
+    const T *__restrict__ Kv, const T *__restrict__ P, T *__restrict__ Q,
+    T *__restrict__ hpart, const int *skip)
+{
+    if (skip && *skip) return;
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ uint16_t s_cb[65];
+    const int b = fg_xcd_remap(blockIdx.x, nblk);
+    const int e0 = eoff[b], ne = eoff[b + 1] - e0;
+    const int d0 = doff[b], nd = doff[b + 1] - d0;
+    T *sP = reinterpret_cast<T *>(smem_raw);
+    T *sA = sP + nd + 1;
+    for (int l = threadIdx.x; l < nd; l += FG_TPB) {
+        sP[l] = P[dofs[d0 + l]];
+        sA[l] = T(0);
+    }
+    if (threadIdx.x == 0) {
+        sP[nd] = T(0);
+        sA[nd] = T(0);
+    }
+    for (int c = threadIdx.x; c <= maxc; c += FG_TPB) s_cb[c] = coff[(size_t)b * (maxc + 1) + c];
+    __syncthreads();
+
+    const T *Kb = Kv + (size_t)K * K * e0;
+    const uint16_t *Lb = lml + (size_t)K * e0;
+    T y[EPT][K];
+    uint16_t li[EPT][K];
+    int col[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        const int el = threadIdx.x + i * FG_TPB;
+        col[i] = -1;
+        if (el < ne) {
+            T p[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                li[i][j] = Lb[(size_t)j * ne + el];
+                p[j] = sP[li[i][j]];
+            }
+#pragma unroll
+            for (int r = 0; r < K; r++) {
+                T acc = Kb[(size_t)(r * K) * ne + el] * p[0];
+#pragma unroll
+                for (int c = 1; c < K; c++) acc = acc + Kb[(size_t)(r * K + c) * ne + el] * p[c];
+                y[i][r] = acc;
+            }
+            int c = 0;
+            while (s_cb[c + 1] <= el) c++;
+            col[i] = c;
+        }
+    }
+    for (int c = 0; c < maxc; c++) {
+        if (s_cb[c] == s_cb[c + 1]) continue;   // empty colour: uniform over the block
+#pragma unroll
+        for (int i = 0; i < EPT; i++)
+            if (col[i] == c) {
+#pragma unroll
+                for (int r = 0; r < K; r++) sA[li[i][r]] = sA[li[i][r]] + y[i][r];
+            }
+        __syncthreads();
+    }
+    for (int l = threadIdx.x; l < nd; l += FG_TPB) {
+        const int s = slots[d0 + l];
+        if (s < 0)
+            Q[dofs[d0 + l]] = sA[l];
+        else
+            hpart[s] = sA[l];
+    }
+}
+
+// shared dofs: partials in block order; dofs no element touches: 0; sink: 0
+template <typename T>
+__global__ __launch_bounds__(FG_TPB) void k_ebe_halo(
+    int nh, const int *__restrict__ hdofs, const int *__restrict__ hoff,
+    const T *__restrict__ hpart, int nz, const int *__restrict__ zdofs, int neq,
+    T *__restrict__ P, T *__restrict__ Q, const int *skip)
+{
+    if (skip && *skip) return;
+    const int h = blockIdx.x * FG_TPB + threadIdx.x;
+    if (h < nh) {
+        int q0 = hoff[h], q1 = hoff[h + 1];
+        T s = hpart[q0];
+        for (int q = q0 + 1; q < q1; q++) s = s + hpart[q];
+        Q[hdofs[h]] = s;
+    } else if (h < nh + nz) {
+        Q[zdofs[h - nh]] = T(0);
+    }
+    if (h == 0) {
+        Q[neq] = T(0);
+        P[neq] = T(0);
+    }
+}
+
+// AoS (element-major) values -> blocked layout, FP64 and FP32 copies
+__global__ void k_ebe_scatter_values(long long n, int kk, const double *__restrict__ A,
+                                     const long long *__restrict__ kpos,
+                                     const int *__restrict__ kstride,
+                                     double *__restrict__ K64, float *__restrict__ K32)
+{
+    for (long long t = (long long)blockIdx.x * FG_TPB + threadIdx.x; t < n;
+         t += (long long)gridDim.x * FG_TPB) {
+        long long e = t / kk;
+        int q = (int)(t - e * kk);
+        long long p = kpos[e];
+        if (p < 0) continue;
+        long long dst = p + (long long)q * kstride[e];
+        double v = A[t];
+        K64[dst] = v;
+        K32[dst] = (float)v;
+    }
+}
+
+template <int K, int EPT, typename T>
+static int launch_apply_k(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *skip)
+{
+    fg_ctx *c = op->ctx;
+    size_t shm = sizeof(T) * 2 * (size_t)(op->max_nloc + 1);
+    FG_LAUNCH((k_ebe_apply<K, EPT, T>), op->nblk, FG_TPB, shm, c->stream, op->nblk,
+              op->maxc, op->d_eoff, op->d_doff, op->d_coff, op->d_dofs, op->d_slots,
+              op->d_lml, Kv, P, Q, hp, skip);
+    return FG_OK;
+}
+
+template <int K, typename T>
+static int launch_apply_ept(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *skip)
+{
+    switch (op->ept) {
+    case 1: return launch_apply_k<K, 1, T>(op, Kv, P, Q, hp, skip);
+    case 2: return launch_apply_k<K, 2, T>(op, Kv, P, Q, hp, skip);
+    case 4: return launch_apply_k<K, 4, T>(op, Kv, P, Q, hp, skip);
+    case 8: return launch_apply_k<K, 8, T>(op, Kv, P, Q, hp, skip);
+    }
+    fg_set_error("unsupported elements-per-thread %d", op->ept);
+    return FG_ERR_ARG;
+}
+
+template <typename T>
+static int ebe_apply(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *skip)
+{
+    fg_ctx *c = op->ctx;
+    if (op->nblk > 0) {
+        switch (op->k) {
+        case 2: FG_CHECK((launch_apply_ept<2, T>(op, Kv, P, Q, hp, skip))); break;
+        case 3: FG_CHECK((launch_apply_ept<3, T>(op, Kv, P, Q, hp, skip))); break;
+        case 4: FG_CHECK((launch_apply_ept<4, T>(op, Kv, P, Q, hp, skip))); break;
+        case 12: FG_CHECK((launch_apply_ept<12, T>(op, Kv, P, Q, hp, skip))); break;
+        case 16: FG_CHECK((launch_apply_ept<16, T>(op, Kv, P, Q, hp, skip))); break;
+        default: fg_set_error("unsupported k=%d", op->k); return FG_ERR_ARG;
+        }
+    }
+    int n = op->nhalo + op->nzero;
+    int g = (n + FG_TPB - 1) / FG_TPB;
+    if (g < 1) g = 1;
+    FG_LAUNCH((k_ebe_halo<T>), g, FG_TPB, 0, c->stream, op->nhalo, op->d_hdofs, op->d_hoff,
+              hp, op->nzero, op->d_zdofs, op->neq, P, Q, skip);
+    return FG_OK;
+}
+
+// the block kernel alone (timing of the dominant kernel)
+int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
+{
+    if (op->nblk == 0) return FG_OK;
+    if (fp32) {
+        switch (op->k) {
+        case 2: return launch_apply_ept<2, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
+        case 3: return launch_apply_ept<3, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
+        case 4: return launch_apply_ept<4, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
+        case 12: return launch_apply_ept<12, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
+        case 16: return launch_apply_ept<16, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
+        }
+    } else {
+        switch (op->k) {
+        case 2: return launch_apply_ept<2, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
+        case 3: return launch_apply_ept<3, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
+        case 4: return launch_apply_ept<4, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
+        case 12: return launch_apply_ept<12, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
+        case 16: return launch_apply_ept<16, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
+        }
+    }
+    fg_set_error("unsupported k=%d", op->k);
+    return FG_ERR_ARG;
+}
+
+int fg_ebe_apply_f64(fg_op *op, double *P, double *Q, const int *skip)
+{
+    return ebe_apply<double>(op, op->d_K64, P, Q, op->d_hpart64, skip);
+}
+int fg_ebe_apply_f32(fg_op *op, float *P, float *Q, const int *skip)
+{
+    return ebe_apply<float>(op, op->d_K32, P, Q, op->d_hpart32, skip);
+}
+
+// ---------------------------------------------------------------------------
+// host API
+// ---------------------------------------------------------------------------
+static int default_ept(int k)
+{
+    if (k <= 4) return 4;
+    return 1;
+}
+
+static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int per_block,
+                      int kind, fg_op **out)
+{
+    FG_ARG(c && out && lm, "ctx/out/lm");
+    FG_ARG(neq >= 0 && nitems >= 0, "sizes");
+    FG_ARG(k == 2 || k == 3 || k == 4 || k == 12 || k == 16, "k must be 2, 3, 4, 12 or 16");
+    int ept = per_block > 0 ? (per_block + FG_TPB - 1) / FG_TPB : default_ept(k);
+    if (ept > 4) ept = 8;
+    else if (ept > 2) ept = 4;
+    FG_HIP(hipSetDevice(c->device));
+    Plan pl;
+    FG_CHECK(build_plan(neq, nitems, k, lm, FG_TPB * ept, pl));
+    fg_op *op = new fg_op();
+    op->ctx = c;
+    op->kind = kind;
+    op->neq = neq;
+    op->nitems = nitems;
+    op->k = k;
+    op->ept = ept;
+    op->nblk = pl.nblk;
+    op->maxc = pl.maxc;
+    op->max_nloc = pl.max_nloc;
+    op->nkept = pl.nkept;
+    op->nnz = (long long)k * k * pl.nkept;
+    op->nhalo = (int)pl.hdofs.size();
+    op->nslots = pl.nslots;
+    op->nzero = (int)pl.zdofs.size();
+    size_t shm = sizeof(double) * 2 * (size_t)(pl.max_nloc + 1);
+    if (shm > 64 * 1024) {
+        delete op;
+        fg_set_error("block needs %zu bytes of LDS; use fewer elements per block", shm);
+        return FG_ERR_ARG;
+    }
+    FG_CHECK(upload(pl.eoff, &op->d_eoff));
+    FG_CHECK(upload(pl.doff, &op->d_doff));
+    FG_CHECK(upload(pl.coff, &op->d_coff));
+    FG_CHECK(upload(pl.dofs, &op->d_dofs));
+    FG_CHECK(upload(pl.slots, &op->d_slots));
+    FG_CHECK(upload(pl.lml, &op->d_lml));
+    FG_CHECK(upload(pl.kpos, &op->d_kpos));
+    FG_CHECK(upload(pl.hdofs, &op->d_hdofs));
+    FG_CHECK(upload(pl.hoff, &op->d_hoff));
+    FG_CHECK(upload(pl.zdofs, &op->d_zdofs));
+    FG_CHECK(upload(pl.kstride, &op->d_kstride));
+    size_t nv = (size_t)k * k * (pl.nkept ? pl.nkept : 1);
+    FG_HIP(hipMalloc(&op->d_K64, nv * sizeof(double)));
+    FG_HIP(hipMalloc(&op->d_K32, nv * sizeof(float)));
+    FG_HIP(hipMemset(op->d_K64, 0, nv * sizeof(double)));
+    FG_HIP(hipMemset(op->d_K32, 0, nv * sizeof(float)));
+    size_t ns = pl.nslots ? pl.nslots : 1;
+    FG_HIP(hipMalloc(&op->d_hpart64, ns * sizeof(double)));
+    FG_HIP(hipMalloc(&op->d_hpart32, ns * sizeof(float)));
+    // original lm (element-local preconditioners) and dof incidence with the
+    // blocked position of each diagonal entry, elements ascending
+    std::vector<int> lmv(lm, lm + (size_t)nitems * k);
+    FG_CHECK(upload(lmv, &op->d_lm));
+    std::vector<int> ioff(neq + 1, 0);
+    for (int e = 0; e < nitems; e++)
+        if (pl.kpos[e] >= 0)
+            for (int j = 0; j < k; j++) {
+                int g = lm[(size_t)e * k + j];
+                if (g != neq) ioff[g + 1]++;
+            }
+    for (int g = 0; g < neq; g++) ioff[g + 1] += ioff[g];
+    std::vector<long long> idiag(ioff[neq]);
+    {
+        std::vector<int> fill(ioff.begin(), ioff.end() - 1);
+        for (int e = 0; e < nitems; e++)
+            if (pl.kpos[e] >= 0)
+                for (int j = 0; j < k; j++) {
+                    int g = lm[(size_t)e * k + j];
+                    if (g != neq)
+                        idiag[fill[g]++] = pl.kpos[e] + (long long)(j * k + j) * pl.kstride[e];
+                }
+    }
+    FG_CHECK(upload(ioff, &op->d_inc_off));
+    FG_CHECK(upload(idiag, &op->d_inc_diag));
+    *out = op;
+    return FG_OK;
+}
+
+extern "C" int fg_op_ebe_create(fg_ctx *c, int neq, int nel, int k, const int *lm,
+                                int elems_per_block, fg_op **out)
+{
+    return ebe_create(c, neq, nel, k, lm, elems_per_block, FG_OP_EBE, out);
+}
+
+extern "C" int fg_op_ede_create(fg_ctx *c, int neq, int nedge, const int *lm2,
+                                int edges_per_block, fg_op **out)
+{
+    return ebe_create(c, neq, nedge, 2, lm2, edges_per_block, FG_OP_EDE, out);
+}
+
+int fg_ebe_set_values_dev(fg_op *op, const double *A)
+{
+    fg_ctx *c = op->ctx;
+    long long n = (long long)op->nitems * op->k * op->k;
+    if (n == 0) return FG_OK;
+    int g = fg_vec_grid(c, n);
+    FG_LAUNCH(k_ebe_scatter_values, g, FG_TPB, 0, c->stream, n, op->k * op->k, A, op->d_kpos,
+              op->d_kstride, op->d_K64, op->d_K32);
+    return FG_OK;
+}
+
+void fg_ebe_free(fg_op *op)
+{
+    hipFree(op->d_eoff); hipFree(op->d_doff); hipFree(op->d_coff); hipFree(op->d_dofs);
+    hipFree(op->d_slots); hipFree(op->d_lml); hipFree(op->d_K64); hipFree(op->d_K32);
+    hipFree(op->d_kpos); hipFree(op->d_hdofs); hipFree(op->d_hoff); hipFree(op->d_zdofs);
+    hipFree(op->d_hpart64); hipFree(op->d_hpart32); hipFree(op->d_inc_off);
+    hipFree(op->d_inc_diag); hipFree(op->d_lm); hipFree(op->d_kstride);
+}
+
+// ---- diagonal of an element/edge operator, element order per dof ---------
+// Diag_precond_EBE_setup.c:30-39 / Diag_precond_EDE_setup.c:30-39
+__global__ void k_ebe_diag(int neq, const int *__restrict__ ioff,
+                           const long long *__restrict__ idiag, const double *__restrict__ K64,
+                           double *__restrict__ D)
+{
+    for (int g = blockIdx.x * FG_TPB + threadIdx.x; g < neq; g += gridDim.x * FG_TPB) {
+        double s = 0.0;
+        for (int q = ioff[g]; q < ioff[g + 1]; q++) s = s + K64[idiag[q]];
+        D[g] = s;
+    }
+}
+
+int fg_ebe_diag(fg_op *op, double *D)
+{
+    fg_ctx *c = op->ctx;
+    if (op->neq == 0) return FG_OK;
+    FG_LAUNCH(k_ebe_diag, fg_vec_grid(c, op->neq), FG_TPB, 0, c->stream, op->neq,
+              op->d_inc_off, op->d_inc_diag, op->d_K64, D);
+    return FG_OK;
+}
+
+// ---- element-local Jacobi (EBE/Jacobi_precond_EBE.c:8-47) ------------------
+// z_g = sum over elements e containing g, in element order, of p_g / A^e_jj.
+// invde holds 1/A^e_jj per incidence (same order as the diagonal incidence).
+__global__ void k_ebe_jacobi_setup(int ninc, const long long *__restrict__ idiag,
+                                   const double *__restrict__ K64, double *__restrict__ invde,
+                                   float *__restrict__ invde32)
+{
+    for (int q = blockIdx.x * FG_TPB + threadIdx.x; q < ninc; q += gridDim.x * FG_TPB) {
+        double v = 1.0 / K64[idiag[q]];
+        invde[q] = v;
+        invde32[q] = (float)v;
+    }
+}
+
+template <typename T>
+__global__ void k_ebe_jacobi_apply(int neq, const int *__restrict__ ioff,
+                                   const T *__restrict__ invde, const T *__restrict__ p,
+                                   T *__restrict__ z, const int *skip)
+{
+    if (skip && *skip) return;
+    for (int g = blockIdx.x * FG_TPB + threadIdx.x; g < neq; g += gridDim.x * FG_TPB) {
+        T pg = p[g], s = T(0);
+        for (int q = ioff[g]; q < ioff[g + 1]; q++) s = s + pg * invde[q];
+        z[g] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) z[neq] = T(0);
+}
+
+int fg_ebe_jacobi_setup(fg_op *op, double **invde, float **invde32)
+{
+    fg_ctx *c = op->ctx;
+    int ninc = 0;
+    FG_HIP(hipMemcpy(&ninc, op->d_inc_off + op->neq, sizeof(int), hipMemcpyDeviceToHost));
+    if (!*invde) FG_HIP(hipMalloc(invde, sizeof(double) * (ninc ? ninc : 1)));
+    if (!*invde32) FG_HIP(hipMalloc(invde32, sizeof(float) * (ninc ? ninc : 1)));
+    if (ninc)
+        FG_LAUNCH(k_ebe_jacobi_setup, fg_vec_grid(c, ninc), FG_TPB, 0, c->stream, ninc,
+                  op->d_inc_diag, op->d_K64, *invde, *invde32);
+    return FG_OK;
+}
+
+int fg_ebe_jacobi_apply(fg_op *op, int fp32, const void *invde, const void *p, void *z,
+                        const int *skip)
+{
+    fg_ctx *c = op->ctx;
+    int g = fg_vec_grid(c, op->neq);
+    if (fp32)
+        FG_LAUNCH(k_ebe_jacobi_apply<float>, g, FG_TPB, 0, c->stream, op->neq, op->d_inc_off,
+                  (const float *)invde, (const float *)p, (float *)z, skip);
+    else
+        FG_LAUNCH(k_ebe_jacobi_apply<double>, g, FG_TPB, 0, c->stream, op->neq, op->d_inc_off,
+                  (const double *)invde, (const double *)p, (double *)z, skip);
+    return FG_OK;
+}

@@ -1,0 +1,200 @@
+// fg_internal.h -- shared internals of libfemgpu (HIP, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <vector>
+
+#include "femgpu.h"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+void fg_set_error(const char *fmt, ...);
+
+#define FG_HIP(call)                                                             \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            fg_set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,            \
+                         hipGetErrorString(e_));                                 \
+            return FG_ERR_HIP;                                                   \
+        }                                                                        \
+    } while (0)
+
+#define FG_CHECK(expr)                                                           \
+    do {                                                                         \
+        int r_ = (expr);                                                         \
+        if (r_ != FG_OK) return r_;                                              \
+    } while (0)
+
+#define FG_ARG(cond, msg)                                                        \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            fg_set_error("invalid argument: %s", msg);                           \
+            return FG_ERR_ARG;                                                   \
+        }                                                                        \
+    } while (0)
+
+// launch + immediate launch-error check
+#define FG_LAUNCH(kern, grid, block, shm, stream, ...)                           \
+    do {                                                                         \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shm, stream,           \
+                           __VA_ARGS__);                                         \
+        FG_HIP(hipGetLastError());                                               \
+    } while (0)
+
+constexpr int FG_TPB = 256;          // threads per block for every kernel
+constexpr int FG_RED_MAX = 4096;     // max blocks of a grid reduction
+constexpr int FG_KMAX = 64;          // max GMRES restart length
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct fg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    int red_blocks = 2048;           // grid of vector kernels / reductions
+    double *red_part = nullptr;      // [FG_RED_MAX] partial sums
+    unsigned *red_ticket = nullptr;  // [4] arrival tickets (self-resetting)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// operators
+// ---------------------------------------------------------------------------
+enum { FG_OP_EBE = 0, FG_OP_EDE = 1, FG_OP_CSR = 2 };
+
+struct fg_op {
+    fg_ctx *ctx = nullptr;
+    int kind = FG_OP_EBE;
+    int neq = 0;
+    int nitems = 0;        // elements, edges or rows
+    int k = 0;             // dofs per item (EBE/EDE)
+    long long nnz = 0;     // CSR nonzeros / EBE stored values
+
+    // ---- blocked element layout (EBE / EDE) ----
+    int nblk = 0, maxc = 0, ept = 0, max_nloc = 0, nkept = 0;
+    int *d_eoff = nullptr;       // [nblk+1] first element (blocked order) of each block
+    int *d_doff = nullptr;       // [nblk+1] first local dof of each block
+    uint16_t *d_coff = nullptr;  // [nblk*(maxc+1)] colour offsets inside each block
+    int *d_dofs = nullptr;       // [ndloc] global dof of each local dof
+    int *d_slots = nullptr;      // [ndloc] -1: complete in block; else halo slot
+    uint16_t *d_lml = nullptr;   // [k*nkept] local dof indices, [k][ne_b] per block
+    double *d_K64 = nullptr;     // [k*k*nkept] values, [k*k][ne_b] per block
+    float *d_K32 = nullptr;
+    long long *d_kpos = nullptr; // [nitems] position of element e in the blocked order, -1 if dropped
+    int *d_kstride = nullptr;    // [nitems] element count of element e's block
+    int nhalo = 0, nslots = 0, nzero = 0;
+    int *d_hdofs = nullptr, *d_hoff = nullptr, *d_zdofs = nullptr;
+    double *d_hpart64 = nullptr;
+    float *d_hpart32 = nullptr;
+    // incidence (dof -> element-local entries), element order: diagonal positions
+    int *d_inc_off = nullptr;    // [neq+1]
+    long long *d_inc_diag = nullptr;  // [ninc] index of K_jj in d_K64 for that incidence
+    int *d_lm = nullptr;         // [nitems*k] original lm (AoS) for element-local preconditioners
+
+    // ---- CSR ----
+    int *d_IA = nullptr, *d_JA = nullptr;
+    double *d_AA = nullptr;
+    float *d_AA32 = nullptr;
+    int nrb = 0;
+    int *d_rb = nullptr;         // [nrb+1] row-block boundaries
+    std::vector<int> h_IA, h_JA;
+};
+
+struct fg_prec {
+    fg_op *op = nullptr;
+    int kind = FG_PREC_NONE;
+    double *d_diag = nullptr, *d_invdiag = nullptr;
+    float *d_invdiag32 = nullptr;
+    double *d_invde = nullptr;   // [nel*3] element-local Jacobi
+    float *d_invde32 = nullptr;
+    double *d_tmp64 = nullptr;   // scratch vector for out-of-place element Jacobi
+    float *d_tmp32 = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+// XCD-aware block index: workgroups are dispatched round-robin over the 8
+// XCDs; remap so that each XCD receives a contiguous range of logical blocks
+// (neighbouring blocks share halo dofs and P entries in that XCD's L2).
+__device__ __forceinline__ int fg_xcd_remap(int bid, int nblk)
+{
+    const int NX = 8;
+    int per = nblk / NX, rem = nblk % NX;
+    int x = bid % NX, q = bid / NX;
+    // XCD x owns logical blocks [x*per + min(x,rem), ...) of size per + (x<rem)
+    int base = x * per + (x < rem ? x : rem);
+    int cnt = per + (x < rem ? 1 : 0);
+    if (q < cnt) return base + q;
+    return bid;  // unreachable when gridDim.x == nblk
+}
+
+template <typename T>
+__device__ __forceinline__ T fg_wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// deterministic block sum; result valid in thread 0
+template <typename T>
+__device__ __forceinline__ T fg_block_sum(T v, T *ws /* [FG_TPB/64] shared */)
+{
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = fg_wave_sum(v);
+    if (lane == 0) ws[wid] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0) {
+        r = ws[0];
+#pragma unroll
+        for (int w = 1; w < FG_TPB / 64; w++) r = r + ws[w];
+    }
+    __syncthreads();
+    return r;
+}
+
+// Grid reduction: every block contributes `v` (its block partial is computed
+// here); the last block to arrive sums the partials in block order and
+// returns true in all of its threads with the total in *total (thread 0).
+// The ticket resets itself, so consecutive kernels can reuse it.
+__device__ __forceinline__ bool fg_grid_sum(double v, double *part, unsigned *ticket,
+                                            double *total)
+{
+    __shared__ double ws[FG_TPB / 64];
+    __shared__ bool last;
+    double b = fg_block_sum(v, ws);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = b;
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += FG_TPB) acc = acc + part[i];
+    acc = fg_block_sum(acc, ws);
+    if (threadIdx.x == 0) {
+        *total = acc;
+        *ticket = 0u;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// internal host entry points shared between translation units
+// ---------------------------------------------------------------------------
+// skip: optional device flag; when *skip != 0 the launched kernels return at once
+int fg_op_apply_any(fg_op *op, int fp32, void *P, void *Q, const int *skip = nullptr);
+int fg_prec_apply_any(fg_prec *pc, int fp32, const void *p, void *z, const int *skip = nullptr);
+int fg_vec_grid(fg_ctx *ctx, long long n);

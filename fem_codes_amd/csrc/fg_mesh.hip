@@ -1,0 +1,519 @@
+// fg_mesh.hip -- mesh structures (host) and SUPG assembly (device).
+//
+// Host structure builders restate what the SS-2D driver does once per mesh:
+//   node numbering  Preprocess.c:73-81
+//   Fill_LM         Fill_LM.c:3-24
+//   CSR pattern     csr_Initialization.c:5-218 (reordering NOT)
+//   EDE pattern     ede_Initialization.c:4-171
+// with counting sorts instead of the reference's sorted linked lists (same
+// sets, same numbering; O(nnz) instead of O(nnz * row length)).
+//
+// Device assembly:
+//   k_supg2d     one thread per triangle: Build_K_F_SUPG.c:35-149 (ke, Fe,
+//                CAU shock capturing) -> ke[nel*9] row-major, Fe[nel*3]
+//   k_gather_F   one thread per equation: F_assembly.c:5-27 as a gather over
+//                the equation's (element, vertex) incidences in element order,
+//                so F is summed exactly in the reference's order
+//   k_gather_csr / k_gather_ede: csr_assembly.c / ede_assembly.c the same way.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "fg_internal.h"
+
+
+// ---------------------------------------------------------------------------
+// host structure builders
+// ---------------------------------------------------------------------------
+extern "C" int fg_number_nodes(int nnodes, const int *type, int *id)
+{
+    int neq = 0;
+    for (int i = 0; i < nnodes; i++) id[i] = type[i] >= 1 ? neq++ : -1;
+    return neq;
+}
+
+extern "C" int fg_fill_lm(int neq, int nel, int nnoel, const int *vertex, const int *type,
+                          const int *id, int *lm)
+{
+    FG_ARG(vertex && type && id && lm && nnoel > 0, "fill_lm arguments");
+    for (long e = 0; e < nel; e++)
+        for (int j = 0; j < nnoel; j++) {
+            int v = vertex[e * nnoel + j];
+            lm[e * nnoel + j] = type[v] >= 1 ? id[v] : neq;
+        }
+    return FG_OK;
+}
+
+namespace {
+// rows (0..nrows-1) of (row, col) pairs -> sorted unique columns per row
+// returns offsets (nrows+1) and columns; counting sort by row then sort each row
+void sorted_pattern(int nrows, const std::vector<int> &pr, const std::vector<int> &pc,
+                    std::vector<long long> &off, std::vector<int> &cols)
+{
+    off.assign((size_t)nrows + 1, 0);
+    for (size_t q = 0; q < pr.size(); q++) off[pr[q] + 1]++;
+    for (int r = 0; r < nrows; r++) off[r + 1] += off[r];
+    std::vector<int> tmp(pr.size());
+    {
+        std::vector<long long> fill(off.begin(), off.end() - 1);
+        for (size_t q = 0; q < pr.size(); q++) tmp[fill[pr[q]]++] = pc[q];
+    }
+    cols.clear();
+    cols.reserve(pr.size());
+    std::vector<long long> noff((size_t)nrows + 1, 0);
+    for (int r = 0; r < nrows; r++) {
+        auto b = tmp.begin() + off[r], e = tmp.begin() + off[r + 1];
+        std::sort(b, e);
+        auto u = std::unique(b, e);
+        cols.insert(cols.end(), b, u);
+        noff[r + 1] = (long long)cols.size();
+    }
+    off.swap(noff);
+}
+
+int find_col(const std::vector<int> &cols, long long lo, long long hi, int c)
+{
+    auto it = std::lower_bound(cols.begin() + lo, cols.begin() + hi, c);
+    return (int)(it - cols.begin());
+}
+
+void ede_order(const int *l, int *o)
+{
+    // ede_Initialization.c:22-83 comparison chain (ties resolved identically)
+    if (l[0] < l[1] && l[0] < l[2]) {
+        if (l[1] < l[2]) { o[0] = 0; o[1] = 1; o[2] = 2; }
+        else             { o[0] = 0; o[1] = 2; o[2] = 1; }
+    } else if (l[1] < l[2]) {
+        if (l[0] < l[2]) { o[0] = 1; o[1] = 0; o[2] = 2; }
+        else             { o[0] = 1; o[1] = 2; o[2] = 0; }
+    } else {
+        if (l[0] < l[1]) { o[0] = 2; o[1] = 0; o[2] = 1; }
+        else             { o[0] = 2; o[1] = 1; o[2] = 0; }
+    }
+}
+}  // namespace
+
+extern "C" int fg_csr_pattern(int neq, int nel, const int *lm, int *IA, int *JA,
+                              int *csr_by_elem, long long *nnz_out)
+{
+    FG_ARG(lm && nnz_out, "lm/nnz");
+    std::vector<int> pr, pc;
+    pr.reserve((size_t)nel * 9);
+    pc.reserve((size_t)nel * 9);
+    for (long e = 0; e < nel; e++)
+        for (int a = 0; a < 3; a++) {
+            int I = lm[3 * e + a];
+            if (I == neq) continue;
+            for (int b = 0; b < 3; b++) {
+                int J = lm[3 * e + b];
+                if (J != neq) { pr.push_back(I); pc.push_back(J); }
+            }
+        }
+    std::vector<long long> off;
+    std::vector<int> cols;
+    sorted_pattern(neq, pr, pc, off, cols);
+    long long nnz = (long long)cols.size();
+    *nnz_out = nnz;
+    if (!JA) return FG_OK;
+    FG_ARG(nnz < 2147483647LL, "nnz fits int32");
+    for (int r = 0; r <= neq; r++) IA[r] = (int)off[r];
+    memcpy(JA, cols.data(), sizeof(int) * cols.size());
+    if (csr_by_elem)
+        for (long e = 0; e < nel; e++)
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    int I = lm[3 * e + a], J = lm[3 * e + b];
+                    csr_by_elem[9 * e + 3 * a + b] =
+                        (I != neq && J != neq) ? find_col(cols, off[I], off[I + 1], J) : (int)nnz;
+                }
+    return FG_OK;
+}
+
+extern "C" int fg_ede_pattern(int neq, int nel, const int *lm, int *order, int *edge_by_elem,
+                              int *lm2, int *nedge_out)
+{
+    FG_ARG(lm && nedge_out, "lm/nedge");
+    std::vector<int> pr, pc;
+    pr.reserve((size_t)nel * 3);
+    pc.reserve((size_t)nel * 3);
+    std::vector<int> ord((size_t)nel * 3);
+    for (long e = 0; e < nel; e++) {
+        int *o = &ord[3 * e];
+        ede_order(lm + 3 * e, o);
+        int I1 = lm[3 * e + o[0]], I2 = lm[3 * e + o[1]], I3 = lm[3 * e + o[2]];
+        pr.push_back(I1); pc.push_back(I2);
+        pr.push_back(I1); pc.push_back(I3);
+        pr.push_back(I2); pc.push_back(I3);
+    }
+    std::vector<long long> off;
+    std::vector<int> cols;
+    sorted_pattern(neq + 1, pr, pc, off, cols);   // rows 0..neq (row neq: the sink edge)
+    int nedge = (int)cols.size() - 1;            // ede_Initialization.c:103
+    *nedge_out = nedge;
+    if (!lm2) return FG_OK;
+    if (order) memcpy(order, ord.data(), sizeof(int) * ord.size());
+    if (edge_by_elem)
+        for (long e = 0; e < nel; e++) {
+            const int *o = &ord[3 * e];
+            int I1 = lm[3 * e + o[0]], I2 = lm[3 * e + o[1]], I3 = lm[3 * e + o[2]];
+            edge_by_elem[3 * e + 0] = find_col(cols, off[I1], off[I1 + 1], I2);
+            edge_by_elem[3 * e + 1] = find_col(cols, off[I1], off[I1 + 1], I3);
+            edge_by_elem[3 * e + 2] = find_col(cols, off[I2], off[I2 + 1], I3);
+        }
+    for (int r = 0; r < neq; r++)
+        for (long long q = off[r]; q < off[r + 1]; q++) {
+            lm2[2 * q] = r;
+            lm2[2 * q + 1] = cols[q];
+        }
+    return FG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mesh object
+// ---------------------------------------------------------------------------
+struct fg_mesh2d {
+    fg_ctx *ctx = nullptr;
+    int nnodes = 0, nel = 0, neq = 0;
+    double *d_x = nullptr, *d_y = nullptr;
+    int *d_vertex = nullptr, *d_id = nullptr, *d_type = nullptr;
+    int *d_finc_off = nullptr, *d_finc = nullptr;   // equation -> e*3+r, element order
+    double *d_Fe = nullptr;                          // [nel*3]
+    std::vector<int> h_lm;
+};
+
+template <typename T>
+static int up(const T *h, size_t n, T **d)
+{
+    FG_HIP(hipMalloc(d, sizeof(T) * (n ? n : 1)));
+    if (n) FG_HIP(hipMemcpy(*d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+    return FG_OK;
+}
+
+extern "C" int fg_mesh2d_create(fg_ctx *c, int nnodes, int nel, const double *x,
+                                const double *y, const int *type, const int *vertex,
+                                fg_mesh2d **out)
+{
+    FG_ARG(c && out && x && y && type && vertex && nnodes > 0 && nel >= 0, "mesh2d args");
+    for (long q = 0; q < 3L * nel; q++)
+        FG_ARG(vertex[q] >= 0 && vertex[q] < nnodes, "vertex index out of range");
+    FG_HIP(hipSetDevice(c->device));
+    fg_mesh2d *m = new fg_mesh2d();
+    m->ctx = c;
+    m->nnodes = nnodes;
+    m->nel = nel;
+    std::vector<int> id(nnodes);
+    m->neq = fg_number_nodes(nnodes, type, id.data());
+    m->h_lm.resize((size_t)nel * 3);
+    fg_fill_lm(m->neq, nel, 3, vertex, type, id.data(), m->h_lm.data());
+    // F incidence: equation -> (element, vertex), element order
+    std::vector<int> off(m->neq + 1, 0), inc;
+    for (long q = 0; q < 3L * nel; q++)
+        if (m->h_lm[q] != m->neq) off[m->h_lm[q] + 1]++;
+    for (int g = 0; g < m->neq; g++) off[g + 1] += off[g];
+    inc.resize(off[m->neq]);
+    {
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (long q = 0; q < 3L * nel; q++)
+            if (m->h_lm[q] != m->neq) inc[fill[m->h_lm[q]]++] = (int)q;
+    }
+    FG_CHECK(up(x, nnodes, &m->d_x));
+    FG_CHECK(up(y, nnodes, &m->d_y));
+    FG_CHECK(up(vertex, (size_t)nel * 3, &m->d_vertex));
+    FG_CHECK(up(id.data(), nnodes, &m->d_id));
+    FG_CHECK(up(type, nnodes, &m->d_type));
+    FG_CHECK(up(off.data(), off.size(), &m->d_finc_off));
+    FG_CHECK(up(inc.data(), inc.size(), &m->d_finc));
+    FG_HIP(hipMalloc(&m->d_Fe, sizeof(double) * 3 * (nel ? nel : 1)));
+    *out = m;
+    return FG_OK;
+}
+
+extern "C" int fg_mesh2d_info(const fg_mesh2d *m, int *neq, int *nel, int *nnodes)
+{
+    FG_ARG(m, "mesh");
+    if (neq) *neq = m->neq;
+    if (nel) *nel = m->nel;
+    if (nnodes) *nnodes = m->nnodes;
+    return FG_OK;
+}
+
+extern "C" int fg_mesh2d_get_lm(const fg_mesh2d *m, int *lm)
+{
+    FG_ARG(m && lm, "mesh/lm");
+    memcpy(lm, m->h_lm.data(), sizeof(int) * m->h_lm.size());
+    return FG_OK;
+}
+
+extern "C" int fg_mesh2d_destroy(fg_mesh2d *m)
+{
+    if (!m) return FG_OK;
+    hipStreamSynchronize(m->ctx->stream);
+    hipFree(m->d_x); hipFree(m->d_y); hipFree(m->d_vertex); hipFree(m->d_id);
+    hipFree(m->d_type); hipFree(m->d_finc_off); hipFree(m->d_finc); hipFree(m->d_Fe);
+    delete m;
+    return FG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SUPG element kernel
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void prob_velocity(const fg_supg2d_problem &pb, double X, double Y,
+                                              double &bx, double &by)
+{
+    if (pb.kind == FG_P_ROTATING) {
+        bx = (-Y + 0.5) * pb.scale;
+        by = (X - 0.5) * pb.scale;
+    } else if (pb.kind == FG_P_VALIDACAO) {
+        bx = 1;
+        by = 20 * Y;
+    } else {
+        bx = pb.bx;
+        by = pb.by;
+    }
+}
+
+// VALIDACAO/VALIDACAO_Font.c:4-17
+__device__ double validacao_font(double X, double Y, double k, double gamma, double Be_x,
+                                 double Be_y)
+{
+    double E = exp(pow(X, 4.5));
+    double u = 10 * X * Y * (1 - X) * (1 - Y) * E;
+    double u_x = 10 * Y * (1 - Y) * (E - 2 * E * X + 4.5 * E * pow(X, 4.5) - 4.5 * E * pow(X, 5.5));
+    double u_x2 = 10 * Y * (1 - Y) * (-20.25 * E * pow(X, 9.0) + 20.25 * E * pow(X, 8.0)
+                  - 33.75 * E * pow(X, 4.5) + 24.75 * E * pow(X, 3.5) - 2 * E);
+    double u_y = 10 * E * X * (1 - X) * (1 - 2 * Y);
+    double u_y2 = -20 * E * X * (1 - X);
+    return -k * (u_x2 + u_y2) + Be_x * u_x + Be_y * u_y + gamma * u;
+}
+
+__device__ __forceinline__ double prob_font(const fg_supg2d_problem &pb, double X, double Y,
+                                            double k, double g, double bx, double by)
+{
+    if (pb.kind == FG_P_VALIDACAO) return validacao_font(X, Y, k, g, bx, by);
+    return pb.f;
+}
+
+__global__ __launch_bounds__(FG_TPB) void k_supg2d(
+    int nel, const double *__restrict__ x, const double *__restrict__ y,
+    const int *__restrict__ vertex, const int *__restrict__ id, const double *__restrict__ u,
+    fg_supg2d_problem pb, double *__restrict__ ke, double *__restrict__ Fe)
+{
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
+    double kx, ky, gamma;
+    if (pb.kind == FG_P_VALIDACAO) { kx = 1.0; ky = 1.0; gamma = 1; }
+    else { kx = pb.kappa; ky = pb.kappa; gamma = pb.gamma; }
+    for (int E = blockIdx.x * FG_TPB + threadIdx.x; E < nel; E += gridDim.x * FG_TPB) {
+        int J1 = vertex[3 * E], J2 = vertex[3 * E + 1], J3 = vertex[3 * E + 2];
+        double X0 = x[J1], X1 = x[J2], X2 = x[J3];
+        double Y0 = y[J1], Y1 = y[J2], Y2 = y[J3];
+        double y23 = Y1 - Y2, y31 = Y2 - Y0, y12 = Y0 - Y1;
+        double x32 = X2 - X1, x13 = X0 - X2, x21 = X1 - X0;
+        double TwoA = x21 * y31 - x13 * y12;
+        double Area = 0.5 * TwoA;
+        double invArea = 1.0 / Area;
+        double h = sqrt(fabs(TwoA));
+        double XB = third * (X0 + X1 + X2);
+        double YB = third * (Y0 + Y1 + Y2);
+        double Be_x, Be_y;
+        prob_velocity(pb, XB, YB, Be_x, Be_y);
+        double norm = sqrt(Be_x * Be_x + Be_y * Be_y);
+        double tau = norm > 0 ? 0.5 * h / norm : 0;
+        double fe1 = prob_font(pb, X0, Y0, kx, gamma, Be_x, Be_y);
+        double fe2 = prob_font(pb, X1, Y1, kx, gamma, Be_x, Be_y);
+        double fe3 = prob_font(pb, X2, Y2, kx, gamma, Be_x, Be_y);
+        double feb = third * (fe1 + fe2 + fe3);
+        int i1 = id[J1], i2 = id[J2], i3 = id[J3];
+        double ue1 = (u && i1 >= 0) ? u[i1] : 0.0;
+        double ue2 = (u && i2 >= 0) ? u[i2] : 0.0;
+        double ue3 = (u && i3 >= 0) ? u[i3] : 0.0;
+        double ueb = third * (ue1 + ue2 + ue3);
+
+        double D12 = 0.25 * invArea * ((kx * y23 * y31) + (ky * x32 * x13));
+        double D13 = 0.25 * invArea * ((kx * y23 * y12) + (ky * x32 * x21));
+        double D23 = 0.25 * invArea * ((kx * y31 * y12) + (ky * x13 * x21));
+        double C11 = sixth * (Be_x * y23 + Be_y * x32);
+        double C12 = sixth * (Be_x * y31 + Be_y * x13);
+        double C13 = sixth * (Be_x * y12 + Be_y * x21);
+        double R11 = sixth * (gamma * Area);
+        double R12 = twelfth * (gamma * Area);
+        double Cpg12 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y31 + Be_x * Be_y * x13) +
+                                               x32 * (Be_x * Be_y * y31 + Be_y * Be_y * x13));
+        double Cpg13 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) +
+                                               x32 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+        double Cpg23 = 0.25 * invArea * tau * (y31 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) +
+                                               x13 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+        double Rpg11 = sixth * tau * gamma * (Be_x * y23 + Be_y * x32);
+        double Rpg21 = sixth * tau * gamma * (Be_x * y31 + Be_y * x13);
+        double Rpg31 = sixth * tau * gamma * (Be_x * y12 + Be_y * x21);
+
+        // h_shock (h_shock_2sqrtArea.c / h_shock_sqrtArea.c) and CAU (CAU_ShockCapture.c:3-30)
+        double hs = pb.h_shock == FG_H_SQRTAREA ? 2 * sqrt(fabs(Area)) : sqrt(fabs(2 * Area));
+        double Eu = 0.0;
+        {
+            double BetaGradu = 0.5 * invArea * (ue1 * (Be_x * y23 + Be_y * x32) +
+                                                ue2 * (Be_x * y31 + Be_y * x13) +
+                                                ue3 * (Be_x * y12 + Be_y * x21));
+            double Ru = BetaGradu + gamma * ueb - feb;
+            double g0 = 0.5 * invArea * (y23 * ue1 + y31 * ue2 + y12 * ue3);
+            double g1 = 0.5 * invArea * (x32 * ue1 + x13 * ue2 + x21 * ue3);
+            double ng = sqrt(g0 * g0 + g1 * g1);
+            if (ng > 0) {
+                double bxp = BetaGradu * g0 / (ng * ng);
+                double byp = BetaGradu * g1 / (ng * ng);
+                double kt = bxp * bxp * kx + byp * byp * ky;
+                double nbp = sqrt(bxp * bxp + byp * byp);
+                double Pee = 0.5 * hs * nbp * nbp * nbp / kt;
+                if (third * Pee < 0.7) Eu = 0.5 * hs * third * Pee * fabs(Ru) / ng;
+                else Eu = 0.5 * hs * 0.7 * fabs(Ru) / ng;
+            }
+        }
+        double Csc12 = Eu * 0.25 * invArea * (y23 * y31 + x32 * x13);
+        double Csc13 = Eu * 0.25 * invArea * (y23 * y12 + x32 * x21);
+        double Csc23 = Eu * 0.25 * invArea * (y31 * y12 + x13 * x21);
+
+        double *k9 = ke + 9L * E;
+        k9[0] = -(D12 + D13) + C11 - (Cpg12 + Cpg13) + R11 + Rpg11 - (Csc12 + Csc13);
+        k9[1] = D12 + C12 + Cpg12 + R12 + Rpg11 + Csc12;
+        k9[2] = D13 + C13 + Cpg13 + R12 + Rpg11 + Csc13;
+        k9[3] = D12 + C11 + Cpg12 + R12 + Rpg21 + Csc12;
+        k9[4] = -(D12 + D23) + C12 - (Cpg12 + Cpg23) + R11 + Rpg21 - (Csc12 + Csc23);
+        k9[5] = D23 + C13 + Cpg23 + R12 + Rpg21 + Csc23;
+        k9[6] = D13 + C11 + Cpg13 + R12 + Rpg31 + Csc13;
+        k9[7] = D23 + C12 + Cpg23 + R12 + Rpg31 + Csc23;
+        k9[8] = -(D13 + D23) + C13 - (Cpg13 + Cpg23) + R11 + Rpg31 - (Csc13 + Csc23);
+        double *f3 = Fe + 3L * E;
+        f3[0] = Area * twelfth * (2 * fe1 + fe2 + fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y23 + Be_y * x32);
+        f3[1] = Area * twelfth * (fe1 + 2 * fe2 + fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y31 + Be_y * x13);
+        f3[2] = Area * twelfth * (fe1 + fe2 + 2 * fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y12 + Be_y * x21);
+    }
+}
+
+// F[g] = sum of Fe over g's incidences in element order (F_assembly.c:5-27, u_D = 0)
+__global__ void k_gather_F(int neq, const int *__restrict__ off, const int *__restrict__ inc,
+                           const double *__restrict__ Fe, double *__restrict__ F)
+{
+    for (int g = blockIdx.x * FG_TPB + threadIdx.x; g < neq; g += gridDim.x * FG_TPB) {
+        double s = 0;
+        for (int q = off[g]; q < off[g + 1]; q++) s += Fe[inc[q]];
+        F[g] = s;
+    }
+}
+
+extern "C" int fg_supg2d_build(fg_mesh2d *m, const fg_supg2d_problem *pb, const double *u,
+                               double *ke, double *F)
+{
+    FG_ARG(m && pb && ke, "mesh/problem/ke");
+    FG_ARG(pb->kind >= FG_P_CONST && pb->kind <= FG_P_VALIDACAO, "problem kind");
+    fg_ctx *c = m->ctx;
+    if (m->nel > 0)
+        FG_LAUNCH(k_supg2d, fg_vec_grid(c, m->nel), FG_TPB, 0, c->stream, m->nel, m->d_x, m->d_y,
+                  m->d_vertex, m->d_id, u, *pb, ke, m->d_Fe);
+    if (F && m->neq > 0)
+        FG_LAUNCH(k_gather_F, fg_vec_grid(c, m->neq), FG_TPB, 0, c->stream, m->neq,
+                  m->d_finc_off, m->d_finc, m->d_Fe, F);
+    return FG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CSR / EDE assembly from element matrices (gathers in element order)
+// ---------------------------------------------------------------------------
+// target t receives sum over its list of (half ? 0.5 : 1) * ke[idx]
+__global__ void k_gather_targets(int nt, const int *__restrict__ off,
+                                 const long long *__restrict__ src, const double *__restrict__ ke,
+                                 double *__restrict__ out)
+{
+    for (int t = blockIdx.x * FG_TPB + threadIdx.x; t < nt; t += gridDim.x * FG_TPB) {
+        double s = 0;
+        for (int q = off[t]; q < off[t + 1]; q++) {
+            long long v = src[q];
+            double a = ke[v >> 1];
+            s += (v & 1) ? 0.5 * a : a;
+        }
+        out[t] = s;
+    }
+}
+
+// contributions (target, ke index, half) in the reference's sequence order ->
+// per-target lists (stable counting sort)
+static int upload_targets(int nt, const std::vector<int> &tg, const std::vector<long long> &src,
+                          int **d_off, long long **d_src)
+{
+    std::vector<int> off((size_t)nt + 1, 0);
+    for (int t : tg) off[t + 1]++;
+    for (int t = 0; t < nt; t++) off[t + 1] += off[t];
+    std::vector<long long> lst(off[nt]);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (size_t q = 0; q < tg.size(); q++) lst[fill[tg[q]]++] = src[q];
+    FG_CHECK(up(off.data(), off.size(), d_off));
+    FG_CHECK(up(lst.data(), lst.size(), d_src));
+    return FG_OK;
+}
+
+extern "C" int fg_op_assemble_csr(fg_op *op, int nel, const int *cbe, const double *ke)
+{
+    FG_ARG(op && op->kind == FG_OP_CSR && cbe && ke, "CSR operator / map / ke");
+    fg_ctx *c = op->ctx;
+    int nnz = (int)op->nnz;
+    std::vector<int> tg;
+    std::vector<long long> src;
+    tg.reserve((size_t)nel * 9);
+    src.reserve((size_t)nel * 9);
+    for (long e = 0; e < nel; e++)
+        for (int q = 0; q < 9; q++) {
+            int s = cbe[9 * e + q];
+            FG_ARG(s >= 0 && s <= nnz, "csr_by_elem entry out of range");
+            if (s < nnz) { tg.push_back(s); src.push_back((9LL * e + q) << 1); }
+        }
+    int *d_off = nullptr;
+    long long *d_src = nullptr;
+    FG_CHECK(upload_targets(nnz, tg, src, &d_off, &d_src));
+    double *AA = nullptr;
+    FG_HIP(hipMalloc(&AA, sizeof(double) * (nnz ? nnz : 1)));
+    if (nnz) FG_LAUNCH(k_gather_targets, fg_vec_grid(c, nnz), FG_TPB, 0, c->stream, nnz, d_off,
+                       d_src, ke, AA);
+    int rc = fg_op_set_values_dev(op, AA);
+    FG_HIP(hipStreamSynchronize(c->stream));
+    hipFree(AA); hipFree(d_off); hipFree(d_src);
+    return rc;
+}
+
+extern "C" int fg_op_assemble_ede(fg_op *op, int nel, const int *order, const int *ebe,
+                                  const double *ke)
+{
+    FG_ARG(op && op->kind == FG_OP_EDE && order && ebe && ke, "EDE operator / maps / ke");
+    fg_ctx *c = op->ctx;
+    int nedge = op->nitems, nt = 4 * nedge;
+    std::vector<int> tg;
+    std::vector<long long> src;
+    tg.reserve((size_t)nel * 12);
+    src.reserve((size_t)nel * 12);
+    // ede_assembly.c:15-26: per element the 12 updates in this sequence
+    for (long e = 0; e < nel; e++) {
+        const int *o = order + 3 * e;
+        int I[3] = {o[0], o[1], o[2]};
+        int J[3] = {ebe[3 * e], ebe[3 * e + 1], ebe[3 * e + 2]};
+        const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};   // (I1,I2), (I1,I3), (I2,I3)
+        for (int m = 0; m < 3; m++) {
+            int a = I[pa[m]], b = I[pb[m]];
+            long long base = 9LL * e;
+            long long s4[4] = {((base + 3 * a + a) << 1) | 1, (base + 3 * a + b) << 1,
+                               (base + 3 * b + a) << 1, ((base + 3 * b + b) << 1) | 1};
+            FG_ARG(J[m] >= 0 && J[m] <= nedge, "edge_by_elem entry out of range");
+            if (J[m] == nedge) continue;   // the sink edge row, zeroed by the reference
+            for (int q = 0; q < 4; q++) { tg.push_back(4 * J[m] + q); src.push_back(s4[q]); }
+        }
+    }
+    int *d_off = nullptr;
+    long long *d_src = nullptr;
+    FG_CHECK(upload_targets(nt, tg, src, &d_off, &d_src));
+    double *A = nullptr;
+    FG_HIP(hipMalloc(&A, sizeof(double) * (nt ? nt : 1)));
+    if (nt) FG_LAUNCH(k_gather_targets, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, nt, d_off,
+                      d_src, ke, A);
+    int rc = fg_op_set_values_dev(op, A);
+    FG_HIP(hipStreamSynchronize(c->stream));
+    hipFree(A); hipFree(d_off); hipFree(d_src);
+    return rc;
+}

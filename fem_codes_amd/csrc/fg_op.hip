@@ -1,0 +1,141 @@
+// fg_op.hip -- operator-independent entry points: values, apply, timing.
+#include "fg_internal.h"
+
+int fg_ebe_apply_f64(fg_op *op, double *P, double *Q, const int *skip);
+int fg_ebe_apply_f32(fg_op *op, float *P, float *Q, const int *skip);
+int fg_ebe_set_values_dev(fg_op *op, const double *A);
+int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q);
+void fg_ebe_free(fg_op *op);
+int fg_csr_apply(fg_op *op, int fp32, const void *P, void *Q, const int *skip);
+void fg_csr_free(fg_op *op);
+
+__global__ void k_copy_cast(long long n, const double *__restrict__ a, double *__restrict__ d64,
+                            float *__restrict__ d32)
+{
+    for (long long i = (long long)blockIdx.x * FG_TPB + threadIdx.x; i < n;
+         i += (long long)gridDim.x * FG_TPB) {
+        double v = a[i];
+        d64[i] = v;
+        d32[i] = (float)v;
+    }
+}
+
+static long long op_nvals(const fg_op *op)
+{
+    if (op->kind == FG_OP_CSR) return op->nnz;
+    return (long long)op->nitems * op->k * op->k;
+}
+
+extern "C" int fg_op_set_values_dev(fg_op *op, const double *vals)
+{
+    FG_ARG(op, "op");
+    fg_ctx *c = op->ctx;
+    long long n = op_nvals(op);
+    if (n == 0) return FG_OK;
+    FG_ARG(vals, "vals");
+    if (op->kind == FG_OP_CSR) {
+        FG_LAUNCH(k_copy_cast, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, vals, op->d_AA,
+                  op->d_AA32);
+        return FG_OK;
+    }
+    return fg_ebe_set_values_dev(op, vals);
+}
+
+extern "C" int fg_op_set_values_host(fg_op *op, const double *vals)
+{
+    FG_ARG(op, "op");
+    fg_ctx *c = op->ctx;
+    long long n = op_nvals(op);
+    if (n == 0) return FG_OK;
+    FG_ARG(vals, "vals");
+    double *tmp = nullptr;
+    FG_HIP(hipMalloc(&tmp, sizeof(double) * n));
+    FG_HIP(hipMemcpyAsync(tmp, vals, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    int rc = fg_op_set_values_dev(op, tmp);
+    FG_HIP(hipStreamSynchronize(c->stream));
+    FG_HIP(hipFree(tmp));
+    return rc;
+}
+
+int fg_op_apply_any(fg_op *op, int fp32, void *P, void *Q, const int *skip)
+{
+    if (op->kind == FG_OP_CSR) return fg_csr_apply(op, fp32, P, Q, skip);
+    if (fp32) return fg_ebe_apply_f32(op, (float *)P, (float *)Q, skip);
+    return fg_ebe_apply_f64(op, (double *)P, (double *)Q, skip);
+}
+
+extern "C" int fg_op_apply_f64(fg_op *op, double *P, double *Q)
+{
+    FG_ARG(op && P && Q, "op/P/Q");
+    return fg_op_apply_any(op, 0, P, Q);
+}
+
+extern "C" int fg_op_apply_f32(fg_op *op, float *P, float *Q)
+{
+    FG_ARG(op && P && Q, "op/P/Q");
+    return fg_op_apply_any(op, 1, P, Q);
+}
+
+extern "C" int fg_op_info(const fg_op *op, int *neq, int *nitems, int *k, long long *nnz)
+{
+    FG_ARG(op, "op");
+    if (neq) *neq = op->neq;
+    if (nitems) *nitems = op->nitems;
+    if (k) *k = op->k;
+    if (nnz) *nnz = op->nnz;
+    return FG_OK;
+}
+
+extern "C" double fg_op_algorithmic_bytes(const fg_op *op, int fp32)
+{
+    if (!op) return 0.0;
+    double s = fp32 ? 4.0 : 8.0;
+    double neq = op->neq;
+    if (op->kind == FG_OP_CSR)
+        return (double)op->nnz * (s + 4.0) + 4.0 * (neq + 1.0) + 2.0 * neq * s;
+    double n = op->nitems, k = op->k;
+    if (op->kind == FG_OP_EDE) return n * (4.0 * s + 8.0) + 2.0 * neq * s;
+    return n * (k * k * s + 4.0 * k) + 2.0 * neq * s;
+}
+
+extern "C" int fg_op_time_apply(fg_op *op, int fp32, void *P, void *Q, int reps, int main_only,
+                                double *ms)
+{
+    FG_ARG(op && P && Q && reps > 0 && ms, "op/P/Q/reps/ms");
+    fg_ctx *c = op->ctx;
+    FG_CHECK(fg_op_apply_any(op, fp32, P, Q));   // warm
+    if (!main_only || op->kind == FG_OP_CSR) {
+        FG_HIP(hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < reps; r++) FG_CHECK(fg_op_apply_any(op, fp32, P, Q));
+        FG_HIP(hipEventRecord(c->ev1, c->stream));
+        FG_HIP(hipEventSynchronize(c->ev1));
+        float t = 0.f;
+        FG_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+        *ms = (double)t / reps;
+        return FG_OK;
+    }
+    double tot = 0.0;
+    for (int r = 0; r < reps; r++) {
+        FG_HIP(hipEventRecord(c->ev0, c->stream));
+        FG_CHECK(fg_ebe_apply_main(op, fp32, P, Q));
+        FG_HIP(hipEventRecord(c->ev1, c->stream));
+        FG_HIP(hipEventSynchronize(c->ev1));
+        float t = 0.f;
+        FG_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+        tot += t;
+    }
+    FG_CHECK(fg_op_apply_any(op, fp32, P, Q));   // leave Q complete
+    FG_HIP(hipStreamSynchronize(c->stream));
+    *ms = tot / reps;
+    return FG_OK;
+}
+
+extern "C" int fg_op_destroy(fg_op *op)
+{
+    if (!op) return FG_OK;
+    hipStreamSynchronize(op->ctx->stream);
+    if (op->kind == FG_OP_CSR) fg_csr_free(op);
+    else fg_ebe_free(op);
+    delete op;
+    return FG_OK;
+}

@@ -1,0 +1,213 @@
+/*
+ * femgpu.h -- C ABI of libfemgpu, the MI355X (gfx950) solve path of FEM_CODES.
+ *
+ * Plain C: opaque handles, raw pointers and sizes, int status codes.  No HIP,
+ * torch or C++ types cross this boundary.  Device buffers are obtained with
+ * fg_malloc (or any hipMalloc'd pointer on the context's device).
+ *
+ * Which reference routine each entry point replaces is cited on the
+ * declaration (paths relative to the FEM_CODES tree).  The reference's own
+ * struct-based signatures (csrmv/ebemv/edemv/pgmres/Diag_precond...) are
+ * exported by the per-driver adapters built on top of this ABI (see
+ * fem_codes_amd/adapters/), which map ParametersType / MatrixDataType /
+ * FemStructsType fields onto these calls.
+ *
+ * Conventions kept from the reference:
+ *   - equation vectors have neq+1 slots; slot neq is the Dirichlet sink
+ *     and reads as zero; matvecs write P[neq] = Q[neq] = 0
+ *     (MatrixVector_Operations/EBE/ebemv.c:21-22);
+ *   - lm[e*k + j] is the equation of local dof j of element e, or neq;
+ *   - element matrices are row-major k x k per element (AoS) on input,
+ *     A[e*k*k + r*k + c] (EBE/ebemv.c:34-36, ebe_assembly.c:9-17).
+ *
+ * Errors: every call returns FG_OK (0) or a positive code; fg_last_error()
+ * gives the message.  The reference prints and exits; its callers ignore the
+ * return value, so the adapters print the message and return nonzero.
+ * Threading: one context per host thread; calls on one context are
+ * serialised on the context's stream.
+ */
+#ifndef FEMGPU_H
+#define FEMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FG_OK             0
+#define FG_ERR_HIP        1   /* a HIP runtime call failed                 */
+#define FG_ERR_ARG        2   /* invalid argument or size                  */
+#define FG_ERR_BREAKDOWN  3   /* GMRES breakdown (zero residual or h)      */
+#define FG_ERR_NOCONV     4   /* solver hit its cycle limit (not fatal)    */
+#define FG_ERR_NOMEM      5
+
+typedef struct fg_ctx  fg_ctx;   /* device, stream, reduction scratch     */
+typedef struct fg_op   fg_op;    /* a stored operator: EBE, EDE or CSR     */
+typedef struct fg_prec fg_prec;  /* a preconditioner bound to an operator  */
+
+const char *fg_last_error(void);
+const char *fg_version(void);
+
+/* ---- context and device memory ------------------------------------------ */
+int fg_ctx_create(int device, fg_ctx **out);
+int fg_ctx_destroy(fg_ctx *ctx);
+int fg_ctx_sync(fg_ctx *ctx);
+int fg_malloc(fg_ctx *ctx, size_t bytes, void **dptr);
+int fg_free(fg_ctx *ctx, void *dptr);
+int fg_memcpy_h2d(fg_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fg_memcpy_d2h(fg_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fg_memcpy_d2d(fg_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fg_memset(fg_ctx *ctx, void *dst, int value, size_t bytes);
+/* element-wise precision casts on device vectors of n entries */
+int fg_cast_f64_to_f32(fg_ctx *ctx, int n, const double *x, float *y);
+int fg_cast_f32_to_f64(fg_ctx *ctx, int n, const float *x, double *y);
+
+/* ---- operators ------------------------------------------------------------ */
+/* Element-by-element operator, k = NNOEL*NDOF dofs per element (2..16).
+ * Replaces the storage set up by Preprocess.c:181-193 and the products
+ * ebemv (EBE/ebemv.c:4-43, k=3), ebemv3Dndof1 (EBE/ebemv3Dndof1.c:3-45, k=4),
+ * ebemvNDOF4 (EBE/ebemvNDOF4.c:5-112 k=12, :113-253 k=16).
+ * lm: host, nel*k.  elems_per_block: 0 = default (autotuned choice). */
+int fg_op_ebe_create(fg_ctx *ctx, int neq, int nel, int k, const int *lm,
+                     int elems_per_block, fg_op **out);
+/* Edge-by-edge operator (EDE/edemv.c:4-39): nedge edges, lm2 host nedge*2,
+ * 2x2 matrices; identical machinery to EBE with k=2. */
+int fg_op_ede_create(fg_ctx *ctx, int neq, int nedge, const int *lm2,
+                     int edges_per_block, fg_op **out);
+/* CSR operator (CSR/csrmv.c:3-24); IA[neq+1], JA[nnz] host. */
+int fg_op_csr_create(fg_ctx *ctx, int neq, int nnz, const int *IA, const int *JA, fg_op **out);
+int fg_op_destroy(fg_op *op);
+
+/* Matrix values.  EBE: nel*k*k row-major per element (MatrixData->A);
+ * EDE: nedge*4 (the sink row nedge of MatrixData->A is not read);
+ * CSR: nnz (MatrixData->AA without the sink slot).
+ * The _host variant copies from host memory, the _dev variant from a device
+ * buffer in the same AoS layout.  Both refresh the FP32 copy used by the
+ * mixed-precision solver. */
+int fg_op_set_values_host(fg_op *op, const double *vals);
+int fg_op_set_values_dev(fg_op *op, const double *vals_dev);
+
+/* Q = A P on device vectors of neq+1 entries; writes P[neq]=Q[neq]=0.
+ * FP64 and FP32 forms (the FP32 form uses the FP32 copy of the values). */
+int fg_op_apply_f64(fg_op *op, double *P, double *Q);
+int fg_op_apply_f32(fg_op *op, float *P, float *Q);
+/* shape queries */
+int fg_op_info(const fg_op *op, int *neq, int *nitems, int *k, long long *nnz);
+/* bytes one matvec must move (compulsory traffic, see metric definition):
+ * EBE nel(k^2 s + 4k) + 2 neq s ; EDE nedge(4s+8) + 2 neq s ;
+ * CSR nnz(s+4) + 4(neq+1) + 2 neq s ; s = 8 (f64) or 4 (f32) */
+double fg_op_algorithmic_bytes(const fg_op *op, int fp32);
+/* time `reps` back-to-back applies with HIP events on the context stream;
+ * returns the mean milliseconds per apply in *ms.  main_only = 1 times the
+ * dominant kernel alone (EBE/EDE: the block kernel without the halo pass),
+ * one event pair around each launch. */
+int fg_op_time_apply(fg_op *op, int fp32, void *P, void *Q, int reps, int main_only,
+                     double *ms);
+
+/* ---- preconditioners ------------------------------------------------------ */
+#define FG_PREC_NONE    0   /* NO_precond.c / NO_precond_setup.c            */
+#define FG_PREC_DIAG    1   /* Diag_precond.c + Diag_precond_{EBE,EDE,CSR}_setup.c */
+#define FG_PREC_JACOBI  2   /* EBE/Jacobi_precond_EBE{,_setup}.c (k=3 EBE only) */
+int fg_prec_create(fg_op *op, int kind, fg_prec **out);
+/* setup: build from the operator's current values and left-precondition F
+ * in place (device, neq+1 entries), as every reference *_setup does.
+ * Diag: Diag=sum of element/edge diagonals or the CSR diagonal entry,
+ * invDiag=1/Diag, F*=invDiag (Diag_precond_EBE_setup.c:18-33). */
+int fg_prec_setup(fg_prec *pc, double *F);
+/* z = M^{-1} p (p may alias z) */
+int fg_prec_apply_f64(fg_prec *pc, const double *p, double *z);
+int fg_prec_apply_f32(fg_prec *pc, const float *p, float *z);
+/* copy Diag / invDiag (neq+1) to host for checks (Diag kind only) */
+int fg_prec_get_diag(fg_prec *pc, double *Diag_host, double *invDiag_host);
+int fg_prec_destroy(fg_prec *pc);
+
+/* ---- mesh structures (host, once per mesh) -------------------------------- */
+/* Node numbering Preprocess.c:73-81 (id = neq++ for Type>=1, else -1) and
+ * Fill_LM.c:3-24 (lm[e*nnoel+j] = id of vertex j, or neq).  Returns neq. */
+int fg_number_nodes(int nnodes, const int *type, int *id);
+int fg_fill_lm(int neq, int nel, int nnoel, const int *vertex, const int *type,
+               const int *id, int *lm);
+/* csr_Initialization.c:5-218 (reordering NOT) for 3-dof elements: sorted
+ * column pattern of every row and the element -> CSR slot map.  Call with
+ * JA == NULL to get nnz; then IA[neq+1], JA[nnz], csr_by_elem[nel*9] (sink = nnz). */
+int fg_csr_pattern(int neq, int nel, const int *lm, int *IA, int *JA, int *csr_by_elem,
+                   long long *nnz);
+/* ede_Initialization.c:4-171: edges (I1<I2) of every element numbered row by
+ * row; order[nel*3], edge_by_elem[nel*3], lm2[nedge*2].  Call with lm2 == NULL
+ * to get nedge (after the reference's nedge-- at :103). */
+int fg_ede_pattern(int neq, int nel, const int *lm, int *order, int *edge_by_elem, int *lm2,
+                   int *nedge);
+
+/* ---- SUPG element assembly on the device ----------------------------------- */
+/* Build_K_F_SUPG.c:5-163 + F_assembly.c:3-51 + CAU_ShockCapture.c:3-30 for
+ * problems with u_D = 0.  Problem kinds mirror the FemFunctions callbacks:
+ *   FG_P_CONST     beta = (bx,by), f const, kappa, gamma
+ *   FG_P_ROTATING  beta = scale*(-y+0.5, x-0.5), f const, kappa, gamma
+ *   FG_P_VALIDACAO 01_SS_Transport_Equation/2D/VALIDACAO/VALIDACAO_*.c   */
+#define FG_P_CONST      0
+#define FG_P_ROTATING   1
+#define FG_P_VALIDACAO  2
+#define FG_H_2SQRTAREA  0   /* h_shock_2sqrtArea.c */
+#define FG_H_SQRTAREA   1   /* h_shock_sqrtArea.c  */
+typedef struct {
+    int kind;
+    double kappa, gamma, f, bx, by, scale;
+    int h_shock;
+} fg_supg2d_problem;
+
+typedef struct fg_mesh2d fg_mesh2d;   /* device copy of a P1 triangle mesh */
+/* x,y,type: nnodes ; vertex: nel*3 (host).  Numbers the nodes and fills lm. */
+int fg_mesh2d_create(fg_ctx *ctx, int nnodes, int nel, const double *x, const double *y,
+                     const int *type, const int *vertex, fg_mesh2d **out);
+int fg_mesh2d_info(const fg_mesh2d *m, int *neq, int *nel, int *nnodes);
+int fg_mesh2d_get_lm(const fg_mesh2d *m, int *lm_host);   /* nel*3 */
+int fg_mesh2d_destroy(fg_mesh2d *m);
+/* ke_dev: nel*9 element matrices, row-major (the EBE layout of
+ * ebe_assembly.c:9-17); F_dev: neq+1, assembled in element order.
+ * u_dev: current iterate (neq+1) for the shock-capturing term, or NULL = 0. */
+int fg_supg2d_build(fg_mesh2d *m, const fg_supg2d_problem *pb, const double *u_dev,
+                    double *ke_dev, double *F_dev);
+/* csr_assembly.c:3-24 / ede_assembly.c:3-35 on the device: the stored
+ * operator receives the sum of element contributions in element order. */
+int fg_op_assemble_csr(fg_op *csr, int nel, const int *csr_by_elem, const double *ke_dev);
+int fg_op_assemble_ede(fg_op *ede, int nel, const int *order, const int *edge_by_elem,
+                       const double *ke_dev);
+
+/* ---- restarted GMRES -------------------------------------------------------*/
+#define FG_GMRES_FP64     0   /* plain FP64: the drop-in for pgmres            */
+#define FG_GMRES_FP32_IR  1   /* FP32 Krylov cycle, FP64 residual/update/LSQ  */
+
+typedef struct {
+    int    kmax;        /* KrylovBasisVectorsQuantity (pgmres.c:14)           */
+    int    lmax;        /* SolverMaxIter: restart-cycle cap (pgmres.c:15,184) */
+    double tol;         /* SolverTolerance: rho <= tol*||B|| (pgmres.c:49-53)  */
+    int    precision;   /* FG_GMRES_FP64 or FG_GMRES_FP32_IR                   */
+    double inner_tol;   /* FP32-IR: a cycle stops at rho <= inner_tol*e0 (0 = 1e-5) */
+    int    hist_cap;    /* capacity of rho_hist below (0 = none)               */
+    double *rho_hist;   /* host: rho after every Arnoldi step                  */
+    double *e0_hist;    /* host, lmax+1 entries: ||r|| at every restart, or NULL */
+} fg_gmres_opts;
+
+typedef struct {
+    int    steps;       /* Arnoldi steps ("cont", pgmres.c:94)                 */
+    int    cycles;      /* restart cycles ("l", pgmres.c:182)                  */
+    double rho;         /* last residual estimate                              */
+    double eps;         /* tol*||B||                                           */
+    double true_res;    /* ||B - M^{-1}A X|| recomputed in FP64 at the end      */
+    int    status;      /* FG_OK, FG_ERR_NOCONV, FG_ERR_BREAKDOWN              */
+    double ms;          /* wall time of the solve on the stream (hipEvent)     */
+} fg_gmres_info;
+
+/* pgmres (Solvers_and_Preconditioners/pgmres.c:4-211): solve M^{-1}A X = B
+ * with B already left-preconditioned by fg_prec_setup; X (neq+1, device)
+ * is overwritten (the reference zeroes it first, :46).  pc may be NULL
+ * (NO_precond).  Returns FG_OK on convergence. */
+int fg_gmres(fg_ctx *ctx, fg_op *op, fg_prec *pc, const double *B, double *X,
+             const fg_gmres_opts *opts, fg_gmres_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+"""GPU parity: libfemgpu against the reference's golden outputs and the oracle.
+
+Tolerances (stated per check):
+  CSR matvec FP64            bitwise (same per-row order, no FMA)
+  EBE/EDE matvec FP64        1e-14 relative to max|Q| (per-dof sum order differs)
+  FP32 matvecs               2e-6 relative to max|Q| against the FP32 oracle
+  SUPG assembly (CONST/ROT)  bitwise; VALIDACAO 1e-13 relative (GPU exp/pow)
+  CSR/EDE assembly, F, Diag  bitwise on bitwise inputs
+  FP64 GMRES                 same step count; X within 1e-12 relative;
+                             first-cycle residual history within 1e-12 relative
+  FP32-IR GMRES              X within 1e-6 relative; true residual <= tol*||B||
+"""
+import os
+
+import numpy as np
+import pytest
+
+import fem_codes_amd as F
+import oracle as O
+from conftest import golden_files, load_golden
+
+pytestmark = pytest.mark.gpu
+FILES = golden_files()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = F.Context(0)
+    yield c
+    c.close()
+
+
+def _prob(d):
+    k = d["prob"]
+    return F.Supg2dProblem(int(k[0]), k[1], k[2], k[3], k[4], k[5], k[6], int(k[7]))
+
+
+def _op(ctx, d, vals=None, epb=0):
+    scheme, neq = str(d["scheme"]), int(d["neq"])
+    A = d["A_u0"] if vals is None else vals
+    if scheme == "EBE":
+        op = F.Operator.ebe(ctx, neq, d["lm"], 3, epb)
+        op.set_values(A)
+    elif scheme == "EDE":
+        op = F.Operator.ede(ctx, neq, d["lm2"], epb)
+        op.set_values(A[: 4 * (len(d["lm2"]) // 2)])
+    else:
+        op = F.Operator.csr(ctx, neq, d["IA"], d["JA"])
+        op.set_values(A[: len(d["JA"])])
+    return op
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("path", FILES, ids=os.path.basename)
+def test_matvec_f64(ctx, path):
+    d = load_golden(path)
+    neq = int(d["neq"])
+    for epb in (0, 256, 512):
+        op = _op(ctx, d, epb=epb)
+        Q = op.matvec(d["P"])
+        if str(d["scheme"]) == "CSR":
+            assert np.array_equal(Q[:neq], d["Q"][:neq])
+        else:
+            assert _rel(Q[:neq], d["Q"][:neq]) <= 1e-14
+        assert Q[neq] == 0.0
+
+
+@pytest.mark.parametrize("path", golden_files("ss2d_n16_*.npz"), ids=os.path.basename)
+def test_matvec_f32(ctx, path, orc):
+    """FP32 product against the FP64 reference product: the error is bounded
+    by (terms per row) * 2^-24 * (|A||P|), checked with the bound from the
+    element matrices taken in absolute value."""
+    d = load_golden(path)
+    neq, nel = int(d["neq"]), len(d["vertex"]) // 3
+    op = _op(ctx, d)
+    P32 = d["P"].astype(np.float32)
+    Q = op.matvec(P32, np.float32).astype(np.float64)
+    k = d["prob"]
+    ke, _ = orc.build_supg2d(d["x"], d["y"], d["type"], d["ids"], d["vertex"], neq,
+                             np.zeros(neq + 1),
+                             O.prob2d(int(k[0]), k[1], k[2], k[3], k[4], k[5], k[6], int(k[7])))
+    bound = orc.ebemv(neq, nel, 3, np.abs(ke), d["lm"], np.abs(P32.astype(np.float64)))
+    Q64 = orc.ebemv(neq, nel, 3, ke, d["lm"], P32.astype(np.float64))
+    assert np.all(np.abs(Q[:neq] - Q64[:neq]) <= 2e-6 * np.max(bound[:neq]))
+
+
+@pytest.mark.parametrize("path", FILES, ids=os.path.basename)
+def test_supg_assembly(ctx, path):
+    d = load_golden(path)
+    neq, nel = int(d["neq"]), len(d["vertex"]) // 3
+    m = F.Mesh2D(ctx, d["x"], d["y"], d["type"], d["vertex"])
+    assert m.neq == neq and np.array_equal(m.lm(), d["lm"])
+    pb = _prob(d)
+    exact = pb.kind != F.P_VALIDACAO
+    ke, Fd = ctx.empty(9 * nel), ctx.empty(neq + 1)
+    Fd.zero()
+    for tag in ("u0", "u1"):
+        u = ctx.to_device(np.zeros(neq + 1) if tag == "u0" else d["u1"])
+        m.build_supg(pb, ke, Fd, u)
+        Fh = Fd.get()
+        scheme = str(d["scheme"])
+        if exact:
+            assert np.array_equal(Fh[:neq], d["F_" + tag][:neq]), tag
+        else:
+            assert _rel(Fh[:neq], d["F_" + tag][:neq]) <= 1e-13
+        if scheme == "EBE":
+            if exact:
+                assert np.array_equal(ke.get(), d["A_" + tag]), tag
+            else:
+                assert _rel(ke.get(), d["A_" + tag]) <= 1e-13
+        elif scheme == "CSR":
+            op = F.Operator.csr(ctx, neq, d["IA"], d["JA"])
+            F.assemble_csr(op, d["csr_by_elem"], ke)
+            Q = op.matvec(d["P"])
+            ref = O.Oracle().csrmv(neq, d["A_" + tag], d["JA"], d["IA"], d["P"].copy())
+            if exact:
+                assert np.array_equal(Q[:neq], ref[:neq])
+            else:
+                assert _rel(Q[:neq], ref[:neq]) <= 1e-13
+        else:
+            op = F.Operator.ede(ctx, neq, d["lm2"])
+            F.assemble_ede(op, d["order"], d["edge_by_elem"], ke)
+            Q = op.matvec(d["P"])
+            ref = O.Oracle().edemv(neq, len(d["lm2"]) // 2, d["A_" + tag], d["lm2"],
+                                   d["P"].copy())
+            assert _rel(Q[:neq], ref[:neq]) <= 1e-14
+
+
+@pytest.mark.parametrize("path", FILES, ids=os.path.basename)
+def test_diag_setup(ctx, path):
+    d = load_golden(path)
+    neq = int(d["neq"])
+    op = _op(ctx, d)
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    Fd = ctx.to_device(d["F_u0"])
+    pc.setup(Fd)
+    D, iD = pc.diag()
+    assert np.array_equal(D[:neq], d["Diag"][:neq])
+    assert np.array_equal(iD, d["invDiag"])
+    assert np.array_equal(Fd.get()[:neq], d["F_diag"][:neq])
+
+
+@pytest.mark.parametrize("path", golden_files("ss2d_*_ebe.npz"), ids=os.path.basename)
+def test_jacobi_ebe(ctx, path):
+    d = load_golden(path)
+    neq = int(d["neq"])
+    op = _op(ctx, d)
+    pc = F.Preconditioner(op, F.PREC_JACOBI)
+    Fd = ctx.to_device(d["F_u0"])
+    pc.setup(Fd)
+    assert np.array_equal(Fd.get()[:neq], d["F_jac"][:neq])
+    p, z = ctx.to_device(d["P"]), ctx.empty(neq + 1)
+    pc.apply(p, z)
+    assert np.array_equal(z.get()[:neq], d["zjac"][:neq])
+    pc.apply(p, p)              # in place: same result (the reference's aliasing defect is not kept)
+    assert np.array_equal(p.get()[:neq], d["zjac"][:neq])
+
+
+@pytest.mark.parametrize("path", FILES, ids=os.path.basename)
+def test_gmres_fp64(ctx, path, orc):
+    d = load_golden(path)
+    neq = int(d["neq"])
+    op = _op(ctx, d)
+    B, X = ctx.to_device(d["F_u0"]), ctx.empty(neq + 1)
+    info, rho, _ = F.gmres(ctx, op, None, B, X, kmax=30, lmax=200, tol=1e-8, hist_cap=4096)
+    assert info.steps == int(d["it_not"])
+    assert _rel(X.get()[:neq], d["X_not"][:neq]) <= 1e-12
+    # residual history against the oracle's (same algorithm, same order)
+    scheme = str(d["scheme"])
+    mats = {"EBE": (d["A_u0"], d["lm"], 3, len(d["vertex"]) // 3),
+            "CSR": (d["A_u0"], d["JA"], d["IA"]),
+            "EDE": (d["A_u0"], d["lm2"], len(d["lm2"]) // 2)}[scheme]
+    Xo, so, _, rho_o, _ = orc.pgmres(neq, 30, 200, 1e-8, d["F_u0"], scheme, mats)
+    n1 = min(30, so)
+    assert np.max(np.abs(rho[:n1] - rho_o[:n1]) / rho_o[:n1]) <= 1e-12
+    # Diag-preconditioned solve
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    Bd = ctx.to_device(d["F_u0"])
+    pc.setup(Bd)
+    info, _, _ = F.gmres(ctx, op, pc, Bd, X, kmax=30, lmax=200, tol=1e-8)
+    assert info.steps == int(d["it_diag"])
+    assert _rel(X.get()[:neq], d["X_diag"][:neq]) <= 1e-12
+
+
+@pytest.mark.parametrize("path", golden_files("ss2d_n16_*.npz"), ids=os.path.basename)
+def test_gmres_fp32_ir(ctx, path):
+    d = load_golden(path)
+    neq = int(d["neq"])
+    op = _op(ctx, d)
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    Bd = ctx.to_device(d["F_u0"])
+    pc.setup(Bd)
+    X = ctx.empty(neq + 1)
+    info, _, _ = F.gmres(ctx, op, pc, Bd, X, kmax=30, lmax=200, tol=1e-8,
+                         precision=F.GMRES_FP32_IR)
+    assert info.status == F.FG_OK
+    assert info.true_res <= info.eps
+    assert _rel(X.get()[:neq], d["X_diag"][:neq]) <= 1e-6
+
+
+def test_gmres_zero_rhs(ctx):
+    d = load_golden(FILES[0])
+    neq = int(d["neq"])
+    op = _op(ctx, d)
+    B, X = ctx.to_device(np.zeros(neq + 1)), ctx.to_device(np.ones(neq + 1))
+    info, _, _ = F.gmres(ctx, op, None, B, X, kmax=10, lmax=5, tol=1e-8)
+    assert info.status == F.FG_OK and np.all(X.get()[:neq] == 0.0)
