@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("matvec HBM GB/s (% of peak) + time-to-1e-8 solve, FP32-IR vs FP64 GMRES")
 PEAK_GBS = 8000.0
+ENGINES = {"auto": 0, "kernels": 1, "fused": 2}
 
 
 def log(*a):
@@ -68,8 +69,8 @@ def setup_system(F, ctx, n, prob, schemes=("EBE", "EDE", "CSR")):
                 ops=ops)
 
 
-def time_solve(F, ctx, sysd, precision, kmax=30, tol=1e-8, lmax=100000):
-    op = sysd["ops"]["EBE"]
+def time_solve(F, ctx, sysd, precision, engine, kmax=30, tol=1e-8, lmax=100000):
+    op = sysd["ops"].get("EBE_default", sysd["ops"]["EBE"])
     pc = F.Preconditioner(op, F.PREC_DIAG)
     B = ctx.empty(sysd["neq"] + 1)
     F.lib.fg_memcpy_d2d(ctx.h, B.ptr, sysd["F"].ptr, B.nbytes)
@@ -77,7 +78,8 @@ def time_solve(F, ctx, sysd, precision, kmax=30, tol=1e-8, lmax=100000):
     X = ctx.empty(sysd["neq"] + 1)
     ctx.sync()
     t0 = time.perf_counter()
-    info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=kmax, lmax=lmax, tol=tol, precision=precision)
+    info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=kmax, lmax=lmax, tol=tol, precision=precision,
+                         engine=engine)
     ctx.sync()
     wall = time.perf_counter() - t0
     return info, wall, X
@@ -154,6 +156,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20260202)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--engine", default="kernels", choices=sorted(ENGINES),
+                    help="GMRES engine for the time-to-solve runs")
     ap.add_argument("--matvec-only", action="store_true",
                     help="only the EBE FP64 product (for profiling runs)")
     args = ap.parse_args()
@@ -181,7 +185,14 @@ def main():
     Q64 = ctx.empty(neq + 1)
     P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
     Q32 = ctx.empty(neq + 1, np.float32)
-    ebe = sysd["ops"]["EBE"]
+    # elements per block of the headline product: the autotuner's choice
+    from fem_codes_amd import autotune
+    best_epb, tune_ms, ebe_tuned = autotune.tune_ebe(ctx, neq, sysd["lm"], 3, sysd["ke"])
+    log("autotune EBE f64 ms by elements/block:", {k: round(v, 4) for k, v in tune_ms.items()},
+        "->", best_epb)
+    sysd["ops"]["EBE_default"] = sysd["ops"]["EBE"]
+    sysd["ops"]["EBE"] = ebe_tuned
+    ebe = ebe_tuned
     for _ in range(args.warmup):
         ebe.apply(P64, Q64)
     ctx.sync()
@@ -219,11 +230,11 @@ def main():
                                 float(np.sin(theta)), 1.0, F.H_2SQRTAREA)
         del sysd["ops"]["EDE"], sysd["ops"]["CSR"]
         sysd["mesh"].build_supg(const, sysd["ke"], sysd["F"])
-        ebe.set_values(sysd["ke"])
-        res = {}
+        sysd["ops"]["EBE_default"].set_values(sysd["ke"])
+        res = {"engine": args.engine}
         X64 = X32 = None
         for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
-            info, wall_s, X = time_solve(F, ctx, sysd, prec)
+            info, wall_s, X = time_solve(F, ctx, sysd, prec, ENGINES[args.engine])
             res[tag] = dict(seconds=wall_s, gpu_ms=info.ms, steps=info.steps, cycles=info.cycles,
                             true_rel_res=info.true_res / (info.eps / 1e-8), status=info.status)
             if tag == "fp64":
@@ -237,9 +248,9 @@ def main():
         solve[f"const_{n}"] = res
         # 64^2 (the constant-velocity system at its reference size)
         small = setup_system(F, ctx, 64, const, ("EBE",))
-        sres = {}
+        sres = {"engine": args.engine}
         for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
-            info, wall_s, X = time_solve(F, ctx, small, prec)
+            info, wall_s, X = time_solve(F, ctx, small, prec, ENGINES[args.engine])
             sres[tag] = dict(seconds=wall_s, steps=info.steps, cycles=info.cycles,
                              X=X.get())
         if not args.no_cpu:
@@ -273,10 +284,11 @@ def main():
                    "wall_ms_per_step": wall * 1e3 / args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_ebe_apply<3,4,double>", "algorithmic_bytes": bytes64,
-                     "ms": ms_main},
+                     "kernel": f"k_ebe_apply<3,{best_epb // 256},double>",
+                     "algorithmic_bytes": bytes64, "ms": ms_main},
         "cpu_baseline": cpu,
         "matvec": matvec,
+        "ebe_autotune_ms": {str(k): v for k, v in tune_ms.items()},
         "solve": solve,
     }
     print(json.dumps(out))

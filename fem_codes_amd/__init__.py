@@ -46,7 +46,10 @@ class FemGpuError(RuntimeError):
 class GmresOpts(C.Structure):
     _fields_ = [("kmax", C.c_int), ("lmax", C.c_int), ("tol", C.c_double),
                 ("precision", C.c_int), ("inner_tol", C.c_double), ("hist_cap", C.c_int),
-                ("rho_hist", C.c_void_p), ("e0_hist", C.c_void_p)]
+                ("rho_hist", C.c_void_p), ("e0_hist", C.c_void_p), ("engine", C.c_int)]
+
+
+ENGINE_AUTO, ENGINE_KERNELS, ENGINE_FUSED = 0, 1, 2
 
 
 class GmresInfo(C.Structure):
@@ -91,6 +94,7 @@ SIGNATURES = {
     "fg_prec_apply_f64": (_i, [_vp, _vp, _vp]),
     "fg_prec_apply_f32": (_i, [_vp, _vp, _vp]),
     "fg_prec_get_diag": (_i, [_vp, _vp, _vp]),
+    "fg_prec_set_invdiag": (_i, [_vp, _dp]),
     "fg_prec_destroy": (_i, [_vp]),
     "fg_gmres": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(GmresOpts), C.POINTER(GmresInfo)]),
     "fg_number_nodes": (_i, [_i, _ip, _ip]),
@@ -201,9 +205,9 @@ class DeviceArray:
         _check(lib.fg_memset(self.ctx.h, self.ptr, 0, self.nbytes), "memset")
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             lib.fg_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -280,9 +284,9 @@ class Operator:
         return ms.value
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib.fg_op_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -313,9 +317,9 @@ class Preconditioner:
         return D, iD
 
     def close(self):
-        if self.h:
+        if self.h and self.op.h and self.op.ctx.h:
             lib.fg_prec_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -326,15 +330,17 @@ class Preconditioner:
 
 def gmres(ctx: Context, op: Operator, prec: Preconditioner | None, B: DeviceArray,
           X: DeviceArray, kmax=30, lmax=1000, tol=1e-8, precision=GMRES_FP64,
-          inner_tol=0.0, hist_cap=0, e0_cap=0):
+          inner_tol=0.0, hist_cap=0, e0_cap=0, engine=ENGINE_KERNELS):
     """pgmres (Solvers_and_Preconditioners/pgmres.c:4-211) on the device.
 
+    engine: ENGINE_KERNELS (one kernel per vector operation), ENGINE_FUSED
+    (the whole solve in one persistent kernel) or ENGINE_AUTO.
     Returns (info, rho_hist, e0_hist)."""
     rho = np.zeros(max(1, hist_cap))
     e0 = np.zeros(max(1, e0_cap if e0_cap else lmax + 1))
     o = GmresOpts(kmax, lmax, tol, precision, inner_tol, hist_cap,
                   rho.ctypes.data if hist_cap else None,
-                  e0.ctypes.data if e0_cap else None)
+                  e0.ctypes.data if e0_cap else None, engine)
     info = GmresInfo()
     rc = lib.fg_gmres(ctx.h, op.h, prec.h if prec is not None else None, B.ptr, X.ptr,
                       C.byref(o), C.byref(info))
@@ -435,9 +441,9 @@ class Mesh2D:
                                    ke.ptr, F.ptr if F is not None else None), "fg_supg2d_build")
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib.fg_mesh2d_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -27,6 +27,7 @@
 #include <numeric>
 #include <vector>
 
+#include "fg_ebe_body.h"
 #include "fg_internal.h"
 
 // ---------------------------------------------------------------------------
@@ -256,66 +257,8 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_apply(
     extern __shared__ unsigned char smem_raw[];
     __shared__ uint16_t s_cb[65];
     const int b = fg_xcd_remap(blockIdx.x, nblk);
-    const int e0 = eoff[b], ne = eoff[b + 1] - e0;
-    const int d0 = doff[b], nd = doff[b + 1] - d0;
-    T *sP = reinterpret_cast<T *>(smem_raw);
-    T *sA = sP + nd + 1;
-    for (int l = threadIdx.x; l < nd; l += FG_TPB) {
-        sP[l] = P[dofs[d0 + l]];
-        sA[l] = T(0);
-    }
-    if (threadIdx.x == 0) {
-        sP[nd] = T(0);
-        sA[nd] = T(0);
-    }
-    for (int c = threadIdx.x; c <= maxc; c += FG_TPB) s_cb[c] = coff[(size_t)b * (maxc + 1) + c];
-    __syncthreads();
-
-    const T *Kb = Kv + (size_t)K * K * e0;
-    const uint16_t *Lb = lml + (size_t)K * e0;
-    T y[EPT][K];
-    uint16_t li[EPT][K];
-    int col[EPT];
-#pragma unroll
-    for (int i = 0; i < EPT; i++) {
-        const int el = threadIdx.x + i * FG_TPB;
-        col[i] = -1;
-        if (el < ne) {
-            T p[K];
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                li[i][j] = Lb[(size_t)j * ne + el];
-                p[j] = sP[li[i][j]];
-            }
-#pragma unroll
-            for (int r = 0; r < K; r++) {
-                T acc = Kb[(size_t)(r * K) * ne + el] * p[0];
-#pragma unroll
-                for (int c = 1; c < K; c++) acc = acc + Kb[(size_t)(r * K + c) * ne + el] * p[c];
-                y[i][r] = acc;
-            }
-            int c = 0;
-            while (s_cb[c + 1] <= el) c++;
-            col[i] = c;
-        }
-    }
-    for (int c = 0; c < maxc; c++) {
-        if (s_cb[c] == s_cb[c + 1]) continue;   // empty colour: uniform over the block
-#pragma unroll
-        for (int i = 0; i < EPT; i++)
-            if (col[i] == c) {
-#pragma unroll
-                for (int r = 0; r < K; r++) sA[li[i][r]] = sA[li[i][r]] + y[i][r];
-            }
-        __syncthreads();
-    }
-    for (int l = threadIdx.x; l < nd; l += FG_TPB) {
-        const int s = slots[d0 + l];
-        if (s < 0)
-            Q[dofs[d0 + l]] = sA[l];
-        else
-            hpart[s] = sA[l];
-    }
+    fg_ebe_block<K, EPT, T>(b, maxc, eoff, doff, coff, dofs, slots, lml, Kv, P, Q, hpart,
+                            smem_raw, s_cb);
 }
 
 // shared dofs: partials in block order; dofs no element touches: 0; sink: 0
@@ -443,11 +386,7 @@ int fg_ebe_apply_f32(fg_op *op, float *P, float *Q, const int *skip)
 // ---------------------------------------------------------------------------
 // host API
 // ---------------------------------------------------------------------------
-static int default_ept(int k)
-{
-    if (k <= 4) return 4;
-    return 1;
-}
+static int default_ept(int k) { return fg_ebe_default_ept(k); }
 
 static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int per_block,
                       int kind, fg_op **out)
@@ -492,6 +431,12 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
     FG_CHECK(upload(pl.hdofs, &op->d_hdofs));
     FG_CHECK(upload(pl.hoff, &op->d_hoff));
     FG_CHECK(upload(pl.zdofs, &op->d_zdofs));
+    {
+        std::vector<int> rowmap(neq, -1);
+        for (size_t h = 0; h < pl.hdofs.size(); h++) rowmap[pl.hdofs[h]] = (int)h;
+        for (int g : pl.zdofs) rowmap[g] = -2;
+        FG_CHECK(upload(rowmap, &op->d_rowmap));
+    }
     FG_CHECK(upload(pl.kstride, &op->d_kstride));
     size_t nv = (size_t)k * k * (pl.nkept ? pl.nkept : 1);
     FG_HIP(hipMalloc(&op->d_K64, nv * sizeof(double)));
@@ -559,7 +504,7 @@ void fg_ebe_free(fg_op *op)
     hipFree(op->d_slots); hipFree(op->d_lml); hipFree(op->d_K64); hipFree(op->d_K32);
     hipFree(op->d_kpos); hipFree(op->d_hdofs); hipFree(op->d_hoff); hipFree(op->d_zdofs);
     hipFree(op->d_hpart64); hipFree(op->d_hpart32); hipFree(op->d_inc_off);
-    hipFree(op->d_inc_diag); hipFree(op->d_lm); hipFree(op->d_kstride);
+    hipFree(op->d_inc_diag); hipFree(op->d_lm); hipFree(op->d_kstride); hipFree(op->d_rowmap);
 }
 
 // ---- diagonal of an element/edge operator, element order per dof ---------

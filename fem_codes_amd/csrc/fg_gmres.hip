@@ -297,6 +297,9 @@ int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState
 
 }  // namespace
 
+int fg_gmres_fused(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
+                   const fg_gmres_opts *o, double eps, fg_gmres_info *info, int *handled);
+
 extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
                         const fg_gmres_opts *o, fg_gmres_info *info)
 {
@@ -304,8 +307,48 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
     FG_ARG(o->kmax >= 1 && o->kmax <= FG_KMAX, "kmax in [1,64]");
     FG_ARG(o->lmax >= 1, "lmax >= 1");
     FG_ARG(o->precision == FG_GMRES_FP64 || o->precision == FG_GMRES_FP32_IR, "precision");
+    FG_ARG(o->engine >= FG_ENGINE_AUTO && o->engine <= FG_ENGINE_FUSED, "engine");
     if (pc && pc->kind == FG_PREC_NONE) pc = nullptr;
     FG_HIP(hipSetDevice(c->device));
+    if (o->engine != FG_ENGINE_KERNELS) {
+        // eps = tol * ||B|| (pgmres.c:49-53), then the persistent solver
+        const int n = op->neq;
+        double *nb2 = nullptr;
+        FG_HIP(hipMalloc(&nb2, sizeof(double)));
+        FG_LAUNCH(k_norm2, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, B, nb2, c->red_part,
+                  c->red_ticket);
+        double h = 0.0;
+        FG_HIP(hipMemcpyAsync(&h, nb2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        hipFree(nb2);
+        const double eps = o->tol * sqrt(h);
+        fg_gmres_info loc;
+        int handled = 0;
+        FG_CHECK(fg_gmres_fused(c, op, pc, B, X, o, eps, &loc, &handled));
+        if (handled) {
+            if (info) {
+                *info = loc;
+                // true preconditioned residual of the returned X (FP64)
+                GmState *st = nullptr;
+                double *W64 = nullptr;
+                FG_HIP(hipMalloc(&st, sizeof(GmState)));
+                FG_HIP(hipMalloc(&W64, sizeof(double) * ((size_t)n + 1)));
+                FG_LAUNCH(k_state_init, 1, 1, 0, c->stream, st, o->kmax, eps);
+                HostState h2{};
+                int rc = residual(c, op, pc, B, X, W64, st, 0.0);
+                if (rc == FG_OK) rc = read_state(c, st, h2);
+                hipFree(st);
+                hipFree(W64);
+                FG_CHECK(rc);
+                info->true_res = h2.e0;
+            }
+            return loc.status == FG_ERR_NOCONV ? FG_ERR_NOCONV : loc.status;
+        }
+        if (o->engine == FG_ENGINE_FUSED) {
+            fg_set_error("the fused GMRES engine does not apply to this operator/preconditioner/size");
+            return FG_ERR_ARG;
+        }
+    }
     const int n = op->neq, kmax = o->kmax;
     const bool ir = o->precision == FG_GMRES_FP32_IR;
     const long long ld = ((long long)n + 1 + 63) / 64 * 64;

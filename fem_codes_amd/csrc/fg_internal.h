@@ -90,6 +90,7 @@ struct fg_op {
     int *d_kstride = nullptr;    // [nitems] element count of element e's block
     int nhalo = 0, nslots = 0, nzero = 0;
     int *d_hdofs = nullptr, *d_hoff = nullptr, *d_zdofs = nullptr;
+    int *d_rowmap = nullptr;     // [neq] -1: written by its block, >= 0: halo index, -2: untouched
     double *d_hpart64 = nullptr;
     float *d_hpart32 = nullptr;
     // incidence (dof -> element-local entries), element order: diagonal positions

@@ -126,6 +126,25 @@ extern "C" int fg_prec_apply_f32(fg_prec *pc, const float *p, float *z)
     return fg_prec_apply_any(pc, 1, p, z);
 }
 
+__global__ void k_to_f32(int n, const double *__restrict__ a, float *__restrict__ b)
+{
+    for (int r = blockIdx.x * FG_TPB + threadIdx.x; r < n; r += gridDim.x * FG_TPB)
+        b[r] = (float)a[r];
+}
+
+extern "C" int fg_prec_set_invdiag(fg_prec *pc, const double *iDh)
+{
+    FG_ARG(pc && pc->kind == FG_PREC_DIAG && iDh, "Diag preconditioner / invDiag");
+    fg_ctx *c = pc->op->ctx;
+    int n = pc->op->neq + 1;
+    FG_HIP(hipMemcpyAsync(pc->d_invdiag, iDh, sizeof(double) * n, hipMemcpyHostToDevice,
+                          c->stream));
+    FG_LAUNCH(k_to_f32, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, pc->d_invdiag,
+              pc->d_invdiag32);
+    FG_HIP(hipStreamSynchronize(c->stream));
+    return FG_OK;
+}
+
 extern "C" int fg_prec_get_diag(fg_prec *pc, double *Dh, double *iDh)
 {
     FG_ARG(pc && pc->kind == FG_PREC_DIAG, "Diag preconditioner");

@@ -119,6 +119,10 @@ int fg_prec_setup(fg_prec *pc, double *F);
 /* z = M^{-1} p (p may alias z) */
 int fg_prec_apply_f64(fg_prec *pc, const double *p, double *z);
 int fg_prec_apply_f32(fg_prec *pc, const float *p, float *z);
+/* load an invDiag computed by the caller (host, neq+1 entries; Diag kind
+ * only): the drop-in path when the driver ran Diag_precond_*_setup itself
+ * and passes MatrixData->invDiag (Diag_precond.c:6). */
+int fg_prec_set_invdiag(fg_prec *pc, const double *invDiag_host);
 /* copy Diag / invDiag (neq+1) to host for checks (Diag kind only) */
 int fg_prec_get_diag(fg_prec *pc, double *Diag_host, double *invDiag_host);
 int fg_prec_destroy(fg_prec *pc);
@@ -187,8 +191,20 @@ typedef struct {
     double inner_tol;   /* FP32-IR: a cycle stops at rho <= inner_tol*e0 (0 = 1e-5) */
     int    hist_cap;    /* capacity of rho_hist below (0 = none)               */
     double *rho_hist;   /* host: rho after every Arnoldi step                  */
-    double *e0_hist;    /* host, lmax+1 entries: ||r|| at every restart, or NULL */
+    double *e0_hist;    /* host, lmax+1 entries: ||r|| at every restart, or NULL
+                           (kernel-per-operation engine only)                  */
+    int    engine;      /* FG_ENGINE_AUTO / _KERNELS / _FUSED                  */
 } fg_gmres_opts;
+
+/* engines: AUTO uses FUSED when it applies (EBE/EDE with k <= 4 or CSR,
+ * NONE or DIAG preconditioner, neq <= 32 rows per thread of a
+ * two-workgroups-per-CU grid) and KERNELS otherwise.
+ * KERNELS: one kernel per vector operation, host sync once per cycle.
+ * FUSED: the whole solve in one persistent cooperative kernel; returns
+ * FG_ERR_ARG when it does not apply. */
+#define FG_ENGINE_AUTO     0
+#define FG_ENGINE_KERNELS  1
+#define FG_ENGINE_FUSED    2
 
 typedef struct {
     int    steps;       /* Arnoldi steps ("cont", pgmres.c:94)                 */

@@ -106,6 +106,11 @@ int ref_setup(int nnodes, int nel, const double *x, const double *y, const int *
         return -1;
     }
     FS_.Node = Node; FS_.Element = El;
+    /* setMatrixVectorProductType.c:6-17 (ref_set_problem sets it again) */
+    if (strcasecmp(scheme, "EBE") == 0) FF_.ProductMatrixVector = ebemv;
+    else if (strcasecmp(scheme, "EDE") == 0) FF_.ProductMatrixVector = edemv;
+    else FF_.ProductMatrixVector = csrmv;
+    FF_.precond = NO_precond; FF_.precondR = NO_precond; FF_.precond_setup = NO_precond_setup;
     return neq;
 }
 

@@ -159,30 +159,64 @@ def test_jacobi_ebe(ctx, path):
     assert np.array_equal(p.get()[:neq], d["zjac"][:neq])
 
 
+def _scheme_spread(path, key):
+    """largest relative difference of `key` between the reference's three
+    storage schemes (EBE/EDE/CSR) for the same mesh and problem"""
+    base = path.rsplit("_", 1)[0]
+    Xs = [load_golden(f"{base}_{s}.npz")[key] for s in ("ebe", "ede", "csr")]
+    return max(_rel(Xs[i], Xs[j]) for i in range(3) for j in range(3) if i != j)
+
+
+def _check_solve(info, X, steps_ref, X_ref, neq, spread=0.0):
+    """Well-conditioned solves (<= 300 reference steps): same step count and
+    X within max(1e-12, 4 x the spread between the reference's own storage
+    schemes) relative -- the unpreconditioned n=8 rotating-flow system is
+    ill-conditioned enough that the reference's EBE/EDE/CSR solutions already
+    differ by 1.7e-12.  Stagnating solves (the n=16 rotating-flow system
+    needs 3179-3946 steps): rounding decides the step count, and the
+    reference's own schemes disagree by up to 20% in steps and 1.8e-9
+    relative in X; there the GPU must stay within 25% of the reference's
+    steps and 1e-8 relative in X."""
+    if steps_ref <= 300:
+        assert info.steps == steps_ref
+        assert _rel(X[:neq], X_ref[:neq]) <= max(1e-12, 4.0 * spread)
+    else:
+        assert abs(info.steps - steps_ref) <= 0.25 * steps_ref
+        assert _rel(X[:neq], X_ref[:neq]) <= 1e-8
+
+
 @pytest.mark.parametrize("path", FILES, ids=os.path.basename)
 def test_gmres_fp64(ctx, path, orc):
     d = load_golden(path)
     neq = int(d["neq"])
     op = _op(ctx, d)
     B, X = ctx.to_device(d["F_u0"]), ctx.empty(neq + 1)
-    info, rho, _ = F.gmres(ctx, op, None, B, X, kmax=30, lmax=200, tol=1e-8, hist_cap=4096)
-    assert info.steps == int(d["it_not"])
-    assert _rel(X.get()[:neq], d["X_not"][:neq]) <= 1e-12
+    info, rho, _ = F.gmres(ctx, op, None, B, X, kmax=30, lmax=500, tol=1e-8, hist_cap=8192)
+    _check_solve(info, X.get(), int(d["it_not"]), d["X_not"], neq, _scheme_spread(path, "X_not"))
     # residual history against the oracle's (same algorithm, same order)
     scheme = str(d["scheme"])
-    mats = {"EBE": (d["A_u0"], d["lm"], 3, len(d["vertex"]) // 3),
-            "CSR": (d["A_u0"], d["JA"], d["IA"]),
-            "EDE": (d["A_u0"], d["lm2"], len(d["lm2"]) // 2)}[scheme]
-    Xo, so, _, rho_o, _ = orc.pgmres(neq, 30, 200, 1e-8, d["F_u0"], scheme, mats)
+    if scheme == "EBE":
+        mats = (d["A_u0"], d["lm"], 3, len(d["vertex"]) // 3)
+    elif scheme == "CSR":
+        mats = (d["A_u0"], d["JA"], d["IA"])
+    else:
+        mats = (d["A_u0"], d["lm2"], len(d["lm2"]) // 2)
+    Xo, so, _, rho_o, _ = orc.pgmres(neq, 30, 500, 1e-8, d["F_u0"], scheme, mats)
+    # the first restart cycle: each rho_k within 1e-12 relative, plus an
+    # absolute floor of 1e-14 * rho_0: the GPU's reduction order perturbs each
+    # dot product at the 1e-16 level relative to rho_0, the perturbations add
+    # up over the cycle (measured up to 1.7e-15 rho_0), and the later rho_k
+    # fall 8+ decades below rho_0
     n1 = min(30, so)
-    assert np.max(np.abs(rho[:n1] - rho_o[:n1]) / rho_o[:n1]) <= 1e-12
+    err = np.abs(rho[:n1] - rho_o[:n1])
+    assert np.all(err <= 1e-12 * rho_o[:n1] + 1e-14 * rho_o[0])
     # Diag-preconditioned solve
     pc = F.Preconditioner(op, F.PREC_DIAG)
     Bd = ctx.to_device(d["F_u0"])
     pc.setup(Bd)
-    info, _, _ = F.gmres(ctx, op, pc, Bd, X, kmax=30, lmax=200, tol=1e-8)
-    assert info.steps == int(d["it_diag"])
-    assert _rel(X.get()[:neq], d["X_diag"][:neq]) <= 1e-12
+    info, _, _ = F.gmres(ctx, op, pc, Bd, X, kmax=30, lmax=500, tol=1e-8)
+    _check_solve(info, X.get(), int(d["it_diag"]), d["X_diag"], neq,
+                 _scheme_spread(path, "X_diag"))
 
 
 @pytest.mark.parametrize("path", golden_files("ss2d_n16_*.npz"), ids=os.path.basename)
