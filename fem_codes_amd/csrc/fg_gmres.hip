@@ -7,15 +7,21 @@
 // Device-resident design: the basis, the Hessenberg matrix, the rotations and
 // the residual estimate all live in HBM; the host only enqueues kernels and
 // reads a few scalars once per restart cycle.  Each Arnoldi step is
-//   matvec (+halo) -> [precondition + h_0i]        (fused)
-//                  -> [w -= h_ji u_j ; h_{j+1,i}] (fused, j = 0..i-1)
-//                  -> [w -= h_ii u_i ; ||w|| ; Givens update in the last block]
-//                  -> [u_{i+1} = w / h_{i+1,i}]
-// with deterministic grid reductions (block partials summed in block order
-// by the last block to arrive).  Every per-step kernel returns at once when
-// the cycle has already converged, so a whole cycle is enqueued without host
-// synchronisation.  Operation order inside each step follows pgmres.c so the
-// FP64 solver reproduces the reference's iterates up to dot-product rounding.
+//   matvec (+halo) -> [precondition ; partials of (u_0, w)]
+//                  -> [h_ji = sum ; w -= h_ji u_j ; partials of (u_{j+1}, w)]
+//                  -> [h_ii = sum ; w -= h_ii u_i ; partials of ||w||^2]
+//                  -> [h_{i+1,i} = sqrt(sum) ; u_{i+1} = w / h_{i+1,i} ; Givens]
+// Reductions are finished by the consumer: each kernel writes one partial
+// per block, and every block of the next kernel sums all partials in the
+// same fixed order (so every block holds the identical, deterministic
+// total).  No fences or atomics sit on the critical path -- the kernel
+// boundary orders the partials -- which matters on a multi-XCD device where
+// an agent-scope fence writes back and invalidates the L2.  Consecutive
+// reductions alternate between two partial buffers.  Every per-step kernel
+// returns at once when the cycle has already converged, so a whole cycle is
+// enqueued without host synchronisation.  Operation order inside each step
+// follows pgmres.c so the FP64 solver reproduces the reference's iterates up
+// to dot-product rounding.
 //
 // FP32-IR: the same cycle with an FP32 basis, FP32 matvec and FP32
 // preconditioner; the residual r = B - M^{-1}A X at every restart, the
@@ -49,20 +55,77 @@ struct GmState {
 
 __device__ __forceinline__ double &Hij(GmState *s, int j, int i) { return s->H[j * LDH + i]; }
 
-// ---- cycle start: r = B - W (W = M^{-1} A X), e0 = ||r|| -------------------
-__global__ __launch_bounds__(FG_TPB) void k_res_init(int n, const double *__restrict__ B,
-                                                     double *__restrict__ W, GmState *st,
-                                                     double *part, unsigned *ticket,
-                                                     double inner_tol)
+// ---- grid of the vector kernels and consumer-side reductions ---------------
+constexpr int VG_MAX = 1024;   // blocks of every vector kernel of a solve
+constexpr int VU = 4;          // entries per thread per sweep (loads in flight)
+
+int vgrid(long long n)
+{
+    long long g = (n + FG_TPB * VU - 1) / (FG_TPB * VU);
+    return (int)(g < 1 ? 1 : g > VG_MAX ? VG_MAX : g);
+}
+
+// this block's partial of a grid reduction -> part[blockIdx.x]
+__device__ __forceinline__ void put_part(double v, double *__restrict__ part)
+{
+    __shared__ double ws[FG_TPB / 64];
+    v = fg_block_sum(v, ws);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// total of the previous kernel's G partials, summed in the same order by
+// every block; returned in every thread
+__device__ __forceinline__ double sum_parts(const double *__restrict__ part, int G)
+{
+    __shared__ double ws[FG_TPB / 64];
+    __shared__ double tot;
+    double a = 0.0;
+    for (int q = threadIdx.x; q < G; q += FG_TPB) a = a + part[q];
+    a = fg_block_sum(a, ws);
+    if (threadIdx.x == 0) tot = a;
+    __syncthreads();
+    return tot;
+}
+
+// first entry of this thread's sweep; sweeps step by gridDim.x*FG_TPB*VU and
+// cover entries base + q*FG_TPB, q < VU
+#define VSWEEP(base, n)                                                          \
+    for (int base = blockIdx.x * FG_TPB * VU + threadIdx.x; base < (n);          \
+         base += gridDim.x * FG_TPB * VU)
+
+// ---- cycle start: r = B - W (W = M^{-1} A X), partials of ||r||^2 ----------
+__global__ __launch_bounds__(FG_TPB) void k_res_part(int n, const double *__restrict__ B,
+                                                     double *__restrict__ W,
+                                                     double *__restrict__ pout)
 {
     double acc = 0.0;
-    for (int k = blockIdx.x * FG_TPB + threadIdx.x; k < n; k += gridDim.x * FG_TPB) {
-        double r = B[k] - W[k];
-        W[k] = r;
-        acc = acc + r * r;
+    VSWEEP(base, n) {
+        double b[VU], w[VU];
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            b[q] = k < n ? B[k] : 0.0;
+            w[q] = k < n ? W[k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            if (k < n) {
+                const double r = b[q] - w[q];
+                W[k] = r;
+                acc = acc + r * r;
+            }
+        }
     }
-    double tot;
-    if (fg_grid_sum(acc, part, ticket, &tot) && threadIdx.x == 0) {
+    put_part(acc, pout);
+}
+
+// e0 = ||r|| and the state of a new cycle (one block)
+__global__ __launch_bounds__(FG_TPB) void k_cycle_start(GmState *st, const double *__restrict__ pin,
+                                                        int G, double inner_tol)
+{
+    const double tot = sum_parts(pin, G);
+    if (threadIdx.x == 0) {
         double e0 = sqrt(tot);
         st->e[0] = e0;
         st->e0 = e0;
@@ -94,68 +157,146 @@ __global__ __launch_bounds__(FG_TPB) void k_scale_to(int n, const double *__rest
     if (blockIdx.x == 0 && threadIdx.x == 0) U[n] = TB(0);
 }
 
-// w = M^{-1} w (Diag, optional) ; h_{0,i} = (u_0, w)   (pgmres.c:104, :110)
+// w = M^{-1} w (Diag, optional) ; partials of h_{0,i} = (u_0, w)
+// (pgmres.c:104, :110)
 template <typename TB>
 __global__ __launch_bounds__(FG_TPB) void k_prec_dot0(int n, const TB *__restrict__ invD,
                                                       TB *__restrict__ W,
-                                                      const TB *__restrict__ U0, GmState *st,
-                                                      double *part, unsigned *ticket)
+                                                      const TB *__restrict__ U0,
+                                                      const GmState *st, double *__restrict__ pout)
 {
     if (st->done) return;
     double acc = 0.0;
-    for (int k = blockIdx.x * FG_TPB + threadIdx.x; k < n; k += gridDim.x * FG_TPB) {
-        TB w = W[k];
-        if (invD) {
-            w = w * invD[k];
-            W[k] = w;
+    VSWEEP(base, n) {
+        TB w[VU], u[VU], d[VU];
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            w[q] = k < n ? W[k] : TB(0);
+            u[q] = k < n ? U0[k] : TB(0);
+            d[q] = (invD && k < n) ? invD[k] : TB(1);
         }
-        acc = acc + (double)U0[k] * (double)w;
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            if (k < n) {
+                TB x = w[q];
+                if (invD) {
+                    x = x * d[q];
+                    W[k] = x;
+                }
+                acc = acc + (double)u[q] * (double)x;
+            }
+        }
     }
-    double tot;
-    if (fg_grid_sum(acc, part, ticket, &tot) && threadIdx.x == 0) Hij(st, 0, st->i) = tot;
+    put_part(acc, pout);
 }
 
-// w -= h_{j,i} u_j ; h_{j+1,i} = (u_{j+1}, w)   (pgmres.c:107-114)
+// h_{j,i} = sum ; w -= h_{j,i} u_j ; partials of h_{j+1,i} = (u_{j+1}, w)
+// (pgmres.c:107-114)
 template <typename TB>
-__global__ __launch_bounds__(FG_TPB) void k_axpy_dot(int n, int j, TB *__restrict__ W,
+__global__ __launch_bounds__(FG_TPB) void k_axpy_dot(int n, int i, int j, TB *__restrict__ W,
                                                      const TB *__restrict__ Uj,
                                                      const TB *__restrict__ Uj1, GmState *st,
-                                                     double *part, unsigned *ticket)
-{
-    if (st->done || j >= st->i) return;
-    const int i = st->i;
-    const TB a = (TB)(-Hij(st, j, i));
-    double acc = 0.0;
-    for (int k = blockIdx.x * FG_TPB + threadIdx.x; k < n; k += gridDim.x * FG_TPB) {
-        TB w = W[k] + a * Uj[k];
-        W[k] = w;
-        acc = acc + (double)Uj1[k] * (double)w;
-    }
-    double tot;
-    if (fg_grid_sum(acc, part, ticket, &tot) && threadIdx.x == 0) Hij(st, j + 1, i) = tot;
-}
-
-// w -= h_{i,i} u_i ; h_{i+1,i} = ||w|| ; Givens (pgmres.c:113-163)
-template <typename TB>
-__global__ __launch_bounds__(FG_TPB) void k_axpy_norm_givens(int n, TB *__restrict__ W,
-                                                             const TB *__restrict__ Ui,
-                                                             GmState *st, double *part,
-                                                             unsigned *ticket)
+                                                     const double *__restrict__ pin, int G,
+                                                     double *__restrict__ pout)
 {
     if (st->done) return;
-    const int i = st->i;
-    const TB a = (TB)(-Hij(st, i, i));
+    const double h = sum_parts(pin, G);
+    if (blockIdx.x == 0 && threadIdx.x == 0) Hij(st, j, i) = h;
+    const TB a = (TB)(-h);
     double acc = 0.0;
-    for (int k = blockIdx.x * FG_TPB + threadIdx.x; k < n; k += gridDim.x * FG_TPB) {
-        TB w = W[k] + a * Ui[k];
-        W[k] = w;
-        acc = acc + (double)w * (double)w;
+    VSWEEP(base, n) {
+        TB w[VU], u[VU], v[VU];
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            w[q] = k < n ? W[k] : TB(0);
+            u[q] = k < n ? Uj[k] : TB(0);
+            v[q] = k < n ? Uj1[k] : TB(0);
+        }
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            if (k < n) {
+                const TB x = w[q] + a * u[q];
+                W[k] = x;
+                acc = acc + (double)v[q] * (double)x;
+            }
+        }
     }
-    double tot;
-    if (fg_grid_sum(acc, part, ticket, &tot) && threadIdx.x == 0) {
-        double hn = sqrt(tot);
+    put_part(acc, pout);
+}
+
+// h_{i,i} = sum ; w -= h_{i,i} u_i ; partials of ||w||^2   (pgmres.c:113-118)
+template <typename TB>
+__global__ __launch_bounds__(FG_TPB) void k_axpy_norm(int n, int i, TB *__restrict__ W,
+                                                      const TB *__restrict__ Ui, GmState *st,
+                                                      const double *__restrict__ pin, int G,
+                                                      double *__restrict__ pout)
+{
+    if (st->done) return;
+    const double h = sum_parts(pin, G);
+    if (blockIdx.x == 0 && threadIdx.x == 0) Hij(st, i, i) = h;
+    const TB a = (TB)(-h);
+    double acc = 0.0;
+    VSWEEP(base, n) {
+        TB w[VU], u[VU];
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            w[q] = k < n ? W[k] : TB(0);
+            u[q] = k < n ? Ui[k] : TB(0);
+        }
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            if (k < n) {
+                const TB x = w[q] + a * u[q];
+                W[k] = x;
+                acc = acc + (double)x * (double)x;
+            }
+        }
+    }
+    put_part(acc, pout);
+}
+
+// h_{i+1,i} = ||w|| ; u_{i+1} = w / h_{i+1,i} ; Givens rotation of column i
+// and the new residual estimate (pgmres.c:118-163).  Block 0 owns the state:
+// when it ends the cycle other blocks may see `done` late or early (each
+// block reads it once), which only affects u_{i+1}, a vector the finished
+// cycle never reads.
+template <typename TB>
+__global__ __launch_bounds__(FG_TPB) void k_scale_givens(int n, int i, const TB *__restrict__ W,
+                                                         TB *__restrict__ Un, GmState *st,
+                                                         const double *__restrict__ pin, int G)
+{
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = __hip_atomic_load(&st->done, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (s_done) return;
+    const double hn = sqrt(sum_parts(pin, G));
+    const double s = 1.0 / hn;
+    VSWEEP(base, n) {
+        TB w[VU];
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            w[q] = k < n ? W[k] : TB(0);
+        }
+#pragma unroll
+        for (int q = 0; q < VU; q++) {
+            const int k = base + q * FG_TPB;
+            if (k < n) Un[k] = sizeof(TB) == 8 ? (TB)(s * (double)w[q]) : (TB)((TB)s * w[q]);
+        }
+    }
+    if (blockIdx.x != 0) return;
+    __syncthreads();   // every thread of block 0 has read `done`
+    if (threadIdx.x == 0) {
+        Un[n] = TB(0);
         Hij(st, i + 1, i) = hn;
-        st->scale = 1.0 / hn;
+        st->scale = s;
         for (int j = 0; j <= i - 1; j++) {
             double h1 = st->cs[j] * Hij(st, j, i) + st->sn[j] * Hij(st, j + 1, i);
             double h2 = -st->sn[j] * Hij(st, j, i) + st->cs[j] * Hij(st, j + 1, i);
@@ -256,23 +397,37 @@ int read_state(fg_ctx *c, GmState *st, HostState &h)
 }
 
 // residual W64 = M^{-1} A X, then r = B - W64 and e0
+// the two partial buffers of the consumer-side reductions: a producer writes
+// out() and flips, the next kernel reads in()
+struct Parts {
+    double *buf[2];
+    int cur;
+    int G;
+    double *in() const { return buf[cur]; }
+    double *out() const { return buf[cur ^ 1]; }
+    void flip() { cur ^= 1; }
+};
+
 int residual(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X, double *W64,
-             GmState *st, double inner_tol)
+             GmState *st, double inner_tol, Parts &pp)
 {
     int n = op->neq;
     FG_CHECK(fg_op_apply_any(op, 0, X, W64));
     if (pc) FG_CHECK(fg_prec_apply_any(pc, 0, W64, W64));
-    FG_LAUNCH(k_res_init, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, B, W64, st,
-              c->red_part, c->red_ticket, inner_tol);
+    FG_LAUNCH(k_res_part, pp.G, FG_TPB, 0, c->stream, n, B, W64, pp.out());
+    pp.flip();
+    FG_LAUNCH(k_cycle_start, 1, FG_TPB, 0, c->stream, st, (const double *)pp.in(), pp.G,
+              inner_tol);
     return FG_OK;
 }
 
 // one Arnoldi cycle with basis type TB, enqueued without host sync
 template <typename TB>
-int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState *st, int kmax)
+int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState *st, int kmax,
+          Parts &pp)
 {
     int n = op->neq;
-    int g = fg_vec_grid(c, n);
+    const int g = pp.G;
     const int fp32 = sizeof(TB) == 4;
     const TB *invD = nullptr;
     bool diag = pc && pc->kind == FG_PREC_DIAG;
@@ -282,15 +437,20 @@ int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState
         TB *Ui = U + (size_t)i * ld;
         FG_CHECK(fg_op_apply_any(op, fp32, Ui, W, skip));
         if (pc && !diag) FG_CHECK(fg_prec_apply_any(pc, fp32, W, W, skip));
-        FG_LAUNCH(k_prec_dot0<TB>, g, FG_TPB, 0, c->stream, n, invD, W, U, st, c->red_part,
-                  c->red_ticket);
-        for (int j = 0; j < i; j++)
-            FG_LAUNCH(k_axpy_dot<TB>, g, FG_TPB, 0, c->stream, n, j, W, U + (size_t)j * ld,
-                      U + (size_t)(j + 1) * ld, st, c->red_part, c->red_ticket);
-        FG_LAUNCH(k_axpy_norm_givens<TB>, g, FG_TPB, 0, c->stream, n, W, Ui, st, c->red_part,
-                  c->red_ticket);
-        FG_LAUNCH(k_scale_to<TB>, g, FG_TPB, 0, c->stream, n, (const double *)nullptr,
-                  (const TB *)W, U + (size_t)(i + 1) * ld, st, 1);
+        FG_LAUNCH(k_prec_dot0<TB>, g, FG_TPB, 0, c->stream, n, invD, W, (const TB *)U,
+                  (const GmState *)st, pp.out());
+        pp.flip();
+        for (int j = 0; j < i; j++) {
+            FG_LAUNCH(k_axpy_dot<TB>, g, FG_TPB, 0, c->stream, n, i, j, W,
+                      (const TB *)(U + (size_t)j * ld), (const TB *)(U + (size_t)(j + 1) * ld),
+                      st, (const double *)pp.in(), g, pp.out());
+            pp.flip();
+        }
+        FG_LAUNCH(k_axpy_norm<TB>, g, FG_TPB, 0, c->stream, n, i, W, (const TB *)Ui, st,
+                  (const double *)pp.in(), g, pp.out());
+        pp.flip();
+        FG_LAUNCH(k_scale_givens<TB>, g, FG_TPB, 0, c->stream, n, i, (const TB *)W,
+                  U + (size_t)(i + 1) * ld, st, (const double *)pp.in(), g);
     }
     return FG_OK;
 }
@@ -310,61 +470,63 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
     FG_ARG(o->engine >= FG_ENGINE_AUTO && o->engine <= FG_ENGINE_FUSED, "engine");
     if (pc && pc->kind == FG_PREC_NONE) pc = nullptr;
     FG_HIP(hipSetDevice(c->device));
-    if (o->engine != FG_ENGINE_KERNELS) {
+    const int n = op->neq, kmax = o->kmax;
+    const bool ir = o->precision == FG_GMRES_FP32_IR;
+    const long long ld = ((long long)n + 1 + 63) / 64 * 64;
+    const size_t sb = ir ? sizeof(float) : sizeof(double);
+    const bool fused = o->engine == FG_ENGINE_FUSED;
+
+    // device workspace, released on every return path
+    struct Work {
+        GmState *st = nullptr;
+        void *U = nullptr, *W = nullptr;
+        double *W64 = nullptr, *nb2 = nullptr, *parts = nullptr;
+        ~Work()
+        {
+            hipFree(st); hipFree(U); hipFree(W); hipFree(W64); hipFree(nb2); hipFree(parts);
+        }
+    } wk;
+    FG_HIP(hipMalloc(&wk.st, sizeof(GmState)));
+    FG_HIP(hipMalloc(&wk.W64, sizeof(double) * ld));
+    FG_HIP(hipMalloc(&wk.nb2, sizeof(double)));
+    FG_HIP(hipMalloc(&wk.parts, sizeof(double) * 2 * VG_MAX));
+    FG_HIP(hipMemsetAsync(wk.W64, 0, sizeof(double) * ld, c->stream));
+    GmState *st = wk.st;
+    double *W64 = wk.W64, *nb2 = wk.nb2;
+    Parts pp{{wk.parts, wk.parts + VG_MAX}, 0, vgrid(n)};
+
+    if (fused) {
         // eps = tol * ||B|| (pgmres.c:49-53), then the persistent solver
-        const int n = op->neq;
-        double *nb2 = nullptr;
-        FG_HIP(hipMalloc(&nb2, sizeof(double)));
         FG_LAUNCH(k_norm2, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, B, nb2, c->red_part,
                   c->red_ticket);
         double h = 0.0;
         FG_HIP(hipMemcpyAsync(&h, nb2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         FG_HIP(hipStreamSynchronize(c->stream));
-        hipFree(nb2);
         const double eps = o->tol * sqrt(h);
         fg_gmres_info loc;
         int handled = 0;
         FG_CHECK(fg_gmres_fused(c, op, pc, B, X, o, eps, &loc, &handled));
-        if (handled) {
-            if (info) {
-                *info = loc;
-                // true preconditioned residual of the returned X (FP64)
-                GmState *st = nullptr;
-                double *W64 = nullptr;
-                FG_HIP(hipMalloc(&st, sizeof(GmState)));
-                FG_HIP(hipMalloc(&W64, sizeof(double) * ((size_t)n + 1)));
-                FG_LAUNCH(k_state_init, 1, 1, 0, c->stream, st, o->kmax, eps);
-                HostState h2{};
-                int rc = residual(c, op, pc, B, X, W64, st, 0.0);
-                if (rc == FG_OK) rc = read_state(c, st, h2);
-                hipFree(st);
-                hipFree(W64);
-                FG_CHECK(rc);
-                info->true_res = h2.e0;
-            }
-            return loc.status == FG_ERR_NOCONV ? FG_ERR_NOCONV : loc.status;
-        }
-        if (o->engine == FG_ENGINE_FUSED) {
+        if (!handled) {
             fg_set_error("the fused GMRES engine does not apply to this operator/preconditioner/size");
             return FG_ERR_ARG;
         }
+        if (info) {
+            *info = loc;
+            // true preconditioned residual of the returned X (FP64)
+            FG_LAUNCH(k_state_init, 1, 1, 0, c->stream, st, o->kmax, eps);
+            HostState h2{};
+            FG_CHECK(residual(c, op, pc, B, X, W64, st, 0.0, pp));
+            FG_CHECK(read_state(c, st, h2));
+            info->true_res = h2.e0;
+        }
+        return loc.status == FG_ERR_NOCONV ? FG_ERR_NOCONV : loc.status;
     }
-    const int n = op->neq, kmax = o->kmax;
-    const bool ir = o->precision == FG_GMRES_FP32_IR;
-    const long long ld = ((long long)n + 1 + 63) / 64 * 64;
-    const size_t sb = ir ? sizeof(float) : sizeof(double);
-
-    GmState *st = nullptr;
-    void *U = nullptr, *W = nullptr;
-    double *W64 = nullptr, *nb2 = nullptr;
-    FG_HIP(hipMalloc(&st, sizeof(GmState)));
-    FG_HIP(hipMalloc(&U, sb * ld * (kmax + 1)));
-    FG_HIP(hipMalloc(&W, sb * ld));
-    FG_HIP(hipMalloc(&W64, sizeof(double) * ld));
-    FG_HIP(hipMalloc(&nb2, sizeof(double)));
+    // the multi-kernel engine (FG_ENGINE_AUTO and FG_ENGINE_KERNELS)
+    FG_HIP(hipMalloc(&wk.U, sb * ld * (kmax + 1)));
+    FG_HIP(hipMalloc(&wk.W, sb * ld));
+    void *U = wk.U, *W = wk.W;
     FG_HIP(hipMemsetAsync(U, 0, sb * ld * (kmax + 1), c->stream));
     FG_HIP(hipMemsetAsync(W, 0, sb * ld, c->stream));
-    FG_HIP(hipMemsetAsync(W64, 0, sizeof(double) * ld, c->stream));
 
     FG_HIP(hipEventRecord(c->ev0, c->stream));
     // X = 0 (pgmres.c:46) ; eps = tol*||B|| (:49-53)
@@ -381,9 +543,9 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
 
     HostState hs{};
     int cycles = 0, rc = FG_OK;
-    int g = fg_vec_grid(c, n);
+    const int g = pp.G;
     for (;;) {
-        FG_CHECK(residual(c, op, pc, B, X, W64, st, inner));
+        FG_CHECK(residual(c, op, pc, B, X, W64, st, inner, pp));
         FG_CHECK(read_state(c, st, hs));
         if (o->e0_hist && cycles <= o->lmax) o->e0_hist[cycles] = hs.e0;
         // FP64: the reference tests rho only after a cycle (do-while, :184),
@@ -396,13 +558,13 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         if (ir) {
             FG_LAUNCH(k_scale_to<float>, g, FG_TPB, 0, c->stream, n, W64, (const float *)nullptr,
                       (float *)U, st, 0);
-            FG_CHECK(cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, kmax));
+            FG_CHECK(cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, kmax, pp));
             FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
             FG_LAUNCH(k_xupdate<float>, g, FG_TPB, 0, c->stream, n, X, (const float *)U, ld, st);
         } else {
             FG_LAUNCH(k_scale_to<double>, g, FG_TPB, 0, c->stream, n, W64,
                       (const double *)nullptr, (double *)U, st, 0);
-            FG_CHECK(cycle<double>(c, op, pc, (double *)U, ld, (double *)W, st, kmax));
+            FG_CHECK(cycle<double>(c, op, pc, (double *)U, ld, (double *)W, st, kmax, pp));
             FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
             FG_LAUNCH(k_xupdate<double>, g, FG_TPB, 0, c->stream, n, X, (const double *)U, ld,
                       st);
@@ -432,10 +594,9 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         info->status = rc;
         // true preconditioned residual of the returned X (FP64)
         HostState h2{};
-        FG_CHECK(residual(c, op, pc, B, X, W64, st, 0.0));
+        FG_CHECK(residual(c, op, pc, B, X, W64, st, 0.0, pp));
         FG_CHECK(read_state(c, st, h2));
         info->true_res = h2.e0;
     }
-    hipFree(st); hipFree(U); hipFree(W); hipFree(W64); hipFree(nb2);
     return rc == FG_ERR_NOCONV ? FG_ERR_NOCONV : rc;
 }

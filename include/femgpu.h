@@ -196,12 +196,13 @@ typedef struct {
     int    engine;      /* FG_ENGINE_AUTO / _KERNELS / _FUSED                  */
 } fg_gmres_opts;
 
-/* engines: AUTO uses FUSED when it applies (EBE/EDE with k <= 4 or CSR,
- * NONE or DIAG preconditioner, neq <= 32 rows per thread of a
- * two-workgroups-per-CU grid) and KERNELS otherwise.
+/* engines: AUTO selects KERNELS (the faster engine at every measured size).
  * KERNELS: one kernel per vector operation, host sync once per cycle.
- * FUSED: the whole solve in one persistent cooperative kernel; returns
- * FG_ERR_ARG when it does not apply. */
+ * FUSED: the whole solve in one persistent cooperative kernel (EBE/EDE with
+ * k <= 4 or CSR, NONE or DIAG preconditioner, neq <= 32 rows per thread of
+ * a two-workgroups-per-CU grid); returns FG_ERR_ARG when it does not apply.
+ * Its grid-wide barriers cost more than kernel boundaries on a multi-XCD
+ * device, so it is kept as an option, not a default. */
 #define FG_ENGINE_AUTO     0
 #define FG_ENGINE_KERNELS  1
 #define FG_ENGINE_FUSED    2

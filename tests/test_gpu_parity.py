@@ -167,6 +167,34 @@ def _scheme_spread(path, key):
     return max(_rel(Xs[i], Xs[j]) for i in range(3) for j in range(3) if i != j)
 
 
+def _oracle_rho(orc, d):
+    neq = int(d["neq"])
+    scheme = str(d["scheme"])
+    if scheme == "EBE":
+        mats = (d["A_u0"], d["lm"], 3, len(d["vertex"]) // 3)
+    elif scheme == "CSR":
+        mats = (d["A_u0"], d["JA"], d["IA"])
+    else:
+        mats = (d["A_u0"], d["lm2"], len(d["lm2"]) // 2)
+    _, so, _, rho_o, _ = orc.pgmres(neq, 30, 500, 1e-8, d["F_u0"], scheme, mats)
+    return rho_o[:so]
+
+
+def _rho_hists(orc, path):
+    """oracle's unpreconditioned residual history for this file's scheme, and
+    the largest difference, step by step over the first restart cycle,
+    between the histories of the three storage schemes"""
+    base = path.rsplit("_", 1)[0]
+    own = _oracle_rho(orc, load_golden(path))
+    hs = [_oracle_rho(orc, load_golden(f"{base}_{s}.npz"))[:30] for s in ("ebe", "ede", "csr")]
+    m = min(len(h) for h in hs)
+    spread = np.zeros(30)
+    for a in hs:
+        for b in hs:
+            spread[:m] = np.maximum(spread[:m], np.abs(a[:m] - b[:m]))
+    return own, spread
+
+
 def _check_solve(info, X, steps_ref, X_ref, neq, spread=0.0):
     """Well-conditioned solves (<= 300 reference steps): same step count and
     X within max(1e-12, 4 x the spread between the reference's own storage
@@ -194,22 +222,18 @@ def test_gmres_fp64(ctx, path, orc):
     info, rho, _ = F.gmres(ctx, op, None, B, X, kmax=30, lmax=500, tol=1e-8, hist_cap=8192)
     _check_solve(info, X.get(), int(d["it_not"]), d["X_not"], neq, _scheme_spread(path, "X_not"))
     # residual history against the oracle's (same algorithm, same order)
-    scheme = str(d["scheme"])
-    if scheme == "EBE":
-        mats = (d["A_u0"], d["lm"], 3, len(d["vertex"]) // 3)
-    elif scheme == "CSR":
-        mats = (d["A_u0"], d["JA"], d["IA"])
-    else:
-        mats = (d["A_u0"], d["lm2"], len(d["lm2"]) // 2)
-    Xo, so, _, rho_o, _ = orc.pgmres(neq, 30, 500, 1e-8, d["F_u0"], scheme, mats)
+    rho_o, spread = _rho_hists(orc, path)
     # the first restart cycle: each rho_k within 1e-12 relative, plus an
     # absolute floor of 1e-14 * rho_0: the GPU's reduction order perturbs each
     # dot product at the 1e-16 level relative to rho_0, the perturbations add
     # up over the cycle (measured up to 1.7e-15 rho_0), and the later rho_k
-    # fall 8+ decades below rho_0
-    n1 = min(30, so)
+    # fall 8+ decades below rho_0.  Where the Krylov space is close to
+    # invariant (h_{k+1,k} -> 0 on the unpreconditioned n=8 rotating-flow
+    # system) rounding is amplified by 1/h_{k+1,k}: there the reference's own
+    # three storage schemes disagree, and 4x that spread is allowed too.
+    n1 = min(30, len(rho_o))
     err = np.abs(rho[:n1] - rho_o[:n1])
-    assert np.all(err <= 1e-12 * rho_o[:n1] + 1e-14 * rho_o[0])
+    assert np.all(err <= 1e-12 * rho_o[:n1] + 1e-14 * rho_o[0] + 4.0 * spread[:n1])
     # Diag-preconditioned solve
     pc = F.Preconditioner(op, F.PREC_DIAG)
     Bd = ctx.to_device(d["F_u0"])
