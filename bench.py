@@ -69,9 +69,9 @@ def setup_system(F, ctx, n, prob, schemes=("EBE", "EDE", "CSR")):
                 ops=ops)
 
 
-def time_solve(F, ctx, sysd, precision, engine, kmax=30, tol=1e-8, lmax=100000):
+def time_solve(F, ctx, sysd, precision, engine, kmax=30, tol=1e-8, lmax=100000, prec="diag"):
     op = sysd["ops"].get("EBE_default", sysd["ops"]["EBE"])
-    pc = F.Preconditioner(op, F.PREC_DIAG)
+    pc = F.Preconditioner(op, F.PREC_JACOBI if prec == "jacobi" else F.PREC_DIAG)
     B = ctx.empty(sysd["neq"] + 1)
     F.lib.fg_memcpy_d2d(ctx.h, B.ptr, sysd["F"].ptr, B.nbytes)
     pc.setup(B)
@@ -158,6 +158,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--engine", default="kernels", choices=sorted(ENGINES),
                     help="GMRES engine for the time-to-solve runs")
+    ap.add_argument("--prec", default="diag", choices=("diag", "jacobi"),
+                    help="preconditioner of the time-to-solve runs")
     ap.add_argument("--matvec-only", action="store_true",
                     help="only the EBE FP64 product (for profiling runs)")
     args = ap.parse_args()
@@ -187,9 +189,10 @@ def main():
     Q32 = ctx.empty(neq + 1, np.float32)
     # elements per block of the headline product: the autotuner's choice
     from fem_codes_amd import autotune
-    best_epb, tune_ms, ebe_tuned = autotune.tune_ebe(ctx, neq, sysd["lm"], 3, sysd["ke"])
-    log("autotune EBE f64 ms by elements/block:", {k: round(v, 4) for k, v in tune_ms.items()},
-        "->", best_epb)
+    (best_epb, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(ctx, neq, sysd["lm"], 3,
+                                                                  sysd["ke"])
+    log("autotune EBE f64 ms by (elements/block, workgroups/CU):",
+        {k: round(v, 4) for k, v in tune_ms.items()}, "->", (best_epb, best_pipe))
     sysd["ops"]["EBE_default"] = sysd["ops"]["EBE"]
     sysd["ops"]["EBE"] = ebe_tuned
     ebe = ebe_tuned
@@ -230,11 +233,13 @@ def main():
                                 float(np.sin(theta)), 1.0, F.H_2SQRTAREA)
         del sysd["ops"]["EDE"], sysd["ops"]["CSR"]
         sysd["mesh"].build_supg(const, sysd["ke"], sysd["F"])
-        sysd["ops"]["EBE_default"].set_values(sysd["ke"])
-        res = {"engine": args.engine}
+        sysd["ops"]["EBE"].set_values(sysd["ke"])
+        del sysd["ops"]["EBE_default"]
+        res = {"engine": args.engine, "precond": args.prec}
         X64 = X32 = None
         for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
-            info, wall_s, X = time_solve(F, ctx, sysd, prec, ENGINES[args.engine])
+            info, wall_s, X = time_solve(F, ctx, sysd, prec, ENGINES[args.engine],
+                                         prec=args.prec)
             res[tag] = dict(seconds=wall_s, gpu_ms=info.ms, steps=info.steps, cycles=info.cycles,
                             true_rel_res=info.true_res / (info.eps / 1e-8), status=info.status)
             if tag == "fp64":
@@ -250,7 +255,8 @@ def main():
         small = setup_system(F, ctx, 64, const, ("EBE",))
         sres = {"engine": args.engine}
         for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
-            info, wall_s, X = time_solve(F, ctx, small, prec, ENGINES[args.engine])
+            info, wall_s, X = time_solve(F, ctx, small, prec, ENGINES[args.engine],
+                                         prec=args.prec)
             sres[tag] = dict(seconds=wall_s, steps=info.steps, cycles=info.cycles,
                              X=X.get())
         if not args.no_cpu:
@@ -284,11 +290,13 @@ def main():
                    "wall_ms_per_step": wall * 1e3 / args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_GBS, "traffic": traffic,
-                     "kernel": f"k_ebe_apply<3,{best_epb // 256},double>",
+                     "kernel": (f"k_ebe_apply_pipe<3,{best_epb // 256},double>" if best_pipe
+                                else f"k_ebe_apply<3,{best_epb // 256},double>"),
+                     "schedule": {"elems_per_block": best_epb, "wg_per_cu": best_pipe},
                      "algorithmic_bytes": bytes64, "ms": ms_main},
         "cpu_baseline": cpu,
         "matvec": matvec,
-        "ebe_autotune_ms": {str(k): v for k, v in tune_ms.items()},
+        "ebe_autotune_ms": {f"{k[0]}/{k[1]}": v for k, v in tune_ms.items()},
         "solve": solve,
     }
     print(json.dumps(out))

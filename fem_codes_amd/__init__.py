@@ -89,6 +89,7 @@ SIGNATURES = {
                         C.POINTER(C.c_longlong)]),
     "fg_op_algorithmic_bytes": (_d, [_vp, _i]),
     "fg_op_time_apply": (_i, [_vp, _i, _vp, _vp, _i, _i, C.POINTER(_d)]),
+    "fg_op_set_pipeline": (_i, [_vp, _i]),
     "fg_prec_create": (_i, [_vp, _i, C.POINTER(_vp)]),
     "fg_prec_setup": (_i, [_vp, _vp]),
     "fg_prec_apply_f64": (_i, [_vp, _vp, _vp]),
@@ -282,6 +283,12 @@ class Operator:
         _check(lib.fg_op_time_apply(self.h, int(P.dtype == np.float32), P.ptr, Q.ptr, reps,
                                     int(main_only), C.byref(ms)), "fg_op_time_apply")
         return ms.value
+
+    def set_pipeline(self, wg_per_cu: int):
+        """EBE/EDE schedule: 0 = a block per workgroup, > 0 = persistent grid of
+        wg_per_cu workgroups per CU with next-block prefetch"""
+        _check(lib.fg_op_set_pipeline(self.h, int(wg_per_cu)), "fg_op_set_pipeline")
+        self.pipeline = int(wg_per_cu)
 
     def close(self):
         if self.h and self.ctx.h:

@@ -12,10 +12,11 @@
 //   2. every thread streams its elements' k*k values and u16 local dof
 //      indices (both stored per block as [entry][element], so each load is
 //      coalesced) and forms the k row sums in registers;
-//   3. the row sums are added into LDS accumulators colour by colour: the
-//      elements of a block are greedily coloured so that no two elements of
-//      one colour share a dof, which makes the LDS scatter conflict-free and
-//      the result independent of thread timing (bitwise reproducible);
+//   3. the row sums go to LDS, and every local dof then adds the row sums
+//      that belong to it through a per-block incidence list (u16 LDS slots,
+//      elements ascending): a gather instead of a scatter, so there are no
+//      write conflicts, one barrier, and the result does not depend on
+//      thread timing (bitwise reproducible);
 //   4. dofs touched only by this block are written to Q; dofs shared with
 //      other blocks write a partial sum to a halo slot, and a small second
 //      kernel adds the partials of each shared dof in block order.
@@ -36,9 +37,9 @@
 namespace {
 
 struct Plan {
-    int nblk = 0, maxc = 0, max_nloc = 0, nkept = 0;
-    std::vector<int> eoff{0}, doff{0};
-    std::vector<uint16_t> coff;
+    int nblk = 0, max_nloc = 0, max_ne = 0, max_ninc = 0, nkept = 0;
+    std::vector<int> eoff{0}, doff{0}, ibase{0};
+    std::vector<uint16_t> ioff, inc;
     std::vector<int> dofs, slots;
     std::vector<uint16_t> lml;
     std::vector<long long> kpos;   // [nel] blocked base index (value q at kpos + q*stride), -1 dropped
@@ -120,17 +121,14 @@ int build_plan(int neq, int nel, int k, const int *lm, int eb, Plan &pl)
     pl.nkept = (int)order.size();
     const int nb = pl.nblk;
 
-    // per block: local dofs, colouring, local indices
+    // per block: local dofs, local indices, dof -> (row, element) incidences
     std::vector<int> locid(neq, -1);
     std::vector<int> blk_of_dofcount(neq, 0);
-    std::vector<std::vector<uint16_t>> bcoff(nb);
-    std::vector<int> newpos(pl.nkept);
     pl.lml.resize((size_t)k * pl.nkept);
     pl.kpos.assign(nel, -1);
     pl.kstride.assign(nel, 0);
-    std::vector<int> ld, col;
-    std::vector<uint64_t> mk;
-    int maxc = 0;
+    pl.inc.reserve((size_t)k * pl.nkept);
+    std::vector<int> ld, icnt;
     for (int b = 0; b < nb; b++) {
         int e0 = pl.eoff[b], ne = pl.eoff[b + 1] - e0;
         ld.clear();
@@ -152,41 +150,14 @@ int build_plan(int neq, int nel, int k, const int *lm, int eb, Plan &pl)
         }
         for (int l = 0; l < nloc; l++) locid[ld[l]] = l;
         pl.max_nloc = std::max(pl.max_nloc, nloc);
-        // greedy colouring in breadth-first order
-        mk.assign(nloc, 0);
-        col.assign(ne, 0);
-        int nc = 0;
+        pl.max_ne = std::max(pl.max_ne, ne);
+        if ((long long)k * ne >= 65535) {
+            fg_set_error("block with %d elements of %d dofs exceeds u16 indexing", ne, k);
+            return FG_ERR_ARG;
+        }
+        // local indices, elements in breadth-first order
         for (int i = 0; i < ne; i++) {
             int e = order[e0 + i];
-            uint64_t used = 0;
-            for (int j = 0; j < k; j++) {
-                int g = lm[(size_t)e * k + j];
-                if (g != neq) used |= mk[locid[g]];
-            }
-            if (used == ~0ull) {
-                fg_set_error("element colouring needs more than 64 colours");
-                return FG_ERR_ARG;
-            }
-            int c = __builtin_ctzll(~used);
-            col[i] = c;
-            nc = std::max(nc, c + 1);
-            for (int j = 0; j < k; j++) {
-                int g = lm[(size_t)e * k + j];
-                if (g != neq) mk[locid[g]] |= (1ull << c);
-            }
-        }
-        maxc = std::max(maxc, nc);
-        // stable counting sort of the block's elements by colour
-        std::vector<uint16_t> cnt(nc + 1, 0);
-        for (int i = 0; i < ne; i++) cnt[col[i] + 1]++;
-        for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
-        bcoff[b] = cnt;
-        std::vector<uint16_t> fillc(cnt.begin(), cnt.end() - 1);
-        std::vector<int> tmp(ne);
-        for (int i = 0; i < ne; i++) tmp[fillc[col[i]]++] = order[e0 + i];
-        for (int i = 0; i < ne; i++) {
-            int e = tmp[i];
-            order[e0 + i] = e;
             pl.kpos[e] = (long long)k * k * e0 + i;
             pl.kstride[e] = ne;
             for (int j = 0; j < k; j++) {
@@ -194,20 +165,34 @@ int build_plan(int neq, int nel, int k, const int *lm, int eb, Plan &pl)
                 pl.lml[(size_t)k * e0 + (size_t)j * ne + i] = (uint16_t)(g == neq ? nloc : locid[g]);
             }
         }
+        // incidences of each local dof as LDS slots j*ne + i, elements ascending
+        icnt.assign(nloc + 1, 0);
+        for (int i = 0; i < ne; i++)
+            for (int j = 0; j < k; j++) {
+                int g = lm[(size_t)order[e0 + i] * k + j];
+                if (g != neq) icnt[locid[g] + 1]++;
+            }
+        for (int l = 0; l < nloc; l++) icnt[l + 1] += icnt[l];
+        const int ninc = icnt[nloc];
+        pl.max_ninc = std::max(pl.max_ninc, ninc);
+        const size_t ib = pl.inc.size();
+        pl.inc.resize(ib + ninc);
+        {
+            std::vector<int> fp(icnt.begin(), icnt.end() - 1);
+            for (int i = 0; i < ne; i++)
+                for (int j = 0; j < k; j++) {
+                    int g = lm[(size_t)order[e0 + i] * k + j];
+                    if (g != neq) pl.inc[ib + fp[locid[g]]++] = (uint16_t)(j * ne + i);
+                }
+        }
+        for (int l = 0; l <= nloc; l++) pl.ioff.push_back((uint16_t)icnt[l]);
+        pl.ibase.push_back((int)pl.inc.size());
         for (int l = 0; l < nloc; l++) {
             pl.dofs.push_back(ld[l]);
             blk_of_dofcount[ld[l]]++;
             locid[ld[l]] = -1;
         }
         pl.doff.push_back((int)pl.dofs.size());
-    }
-    pl.maxc = maxc;
-    pl.coff.assign((size_t)nb * (maxc + 1), 0);
-    for (int b = 0; b < nb; b++) {
-        int ne = pl.eoff[b + 1] - pl.eoff[b];
-        for (int c = 0; c <= maxc; c++)
-            pl.coff[(size_t)b * (maxc + 1) + c] =
-                (uint16_t)(c < (int)bcoff[b].size() ? bcoff[b][c] : ne);
     }
     // shared dofs -> halo slots (blocks in ascending order per dof)
     std::vector<int> hidx(neq, -1);
@@ -246,19 +231,45 @@ int upload(const std::vector<T> &v, T **d)
 // device: apply
 // ---------------------------------------------------------------------------
 template <int K, int EPT, typename T>
-__global__ __launch_bounds__(FG_TPB) void k_ebe_apply(
-    int nblk, int maxc, const int *__restrict__ eoff, const int *__restrict__ doff,
-    const uint16_t *__restrict__ coff, const int *__restrict__ dofs,
-    const int *__restrict__ slots, const uint16_t *__restrict__ lml,
-    const T *__restrict__ Kv, const T *__restrict__ P, T *__restrict__ Q,
-    T *__restrict__ hpart, const int *skip)
+__global__ __launch_bounds__(FG_TPB) void k_ebe_apply(FgEbeLayout L, const T *__restrict__ Kv,
+                                                      const T *__restrict__ P, T *__restrict__ Q,
+                                                      T *__restrict__ hpart, const int *skip)
 {
     if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
-    __shared__ uint16_t s_cb[65];
-    const int b = fg_xcd_remap(blockIdx.x, nblk);
-    fg_ebe_block<K, EPT, T>(b, maxc, eoff, doff, coff, dofs, slots, lml, Kv, P, Q, hpart,
-                            smem_raw, s_cb);
+    const int b = fg_xcd_remap(blockIdx.x, L.nblk);
+    fg_ebe_block<K, EPT, T>(b, L, Kv, P, Q, hpart, smem_raw);
+}
+
+// persistent form: workgroup w walks logical blocks w, w+G, ... and issues the
+// loads of its next block's element matrices before it reduces the current
+// one, so they are in flight during the LDS phase.  G is a multiple of 8 so
+// the XCD-contiguous remap stays aligned with the hardware's round-robin.
+template <int K, int EPT, typename T>
+__global__ __launch_bounds__(FG_TPB) void k_ebe_apply_pipe(FgEbeLayout L,
+                                                           const T *__restrict__ Kv,
+                                                           const T *__restrict__ P,
+                                                           T *__restrict__ Q,
+                                                           T *__restrict__ hpart, const int *skip)
+{
+    if (skip && *skip) return;
+    extern __shared__ unsigned char smem_raw[];
+    const int nblk = L.nblk;
+    int s = blockIdx.x;
+    if (s >= nblk) return;
+    FgEbeRegs<K, EPT, T> cur, nxt;
+    fg_ebe_load<K, EPT, T>(fg_xcd_remap(s, nblk), L, Kv, cur);
+    for (;;) {
+        const int b = fg_xcd_remap(s, nblk);
+        const int sn = s + (int)gridDim.x;
+        fg_ebe_gather<K, T>(b, L, P, smem_raw);
+        if (sn < nblk) fg_ebe_load<K, EPT, T>(fg_xcd_remap(sn, nblk), L, Kv, nxt);
+        fg_ebe_finish<K, EPT, T>(b, L, Kv, Q, hpart, smem_raw, cur);
+        if (sn >= nblk) break;
+        cur = nxt;
+        s = sn;
+        __syncthreads();   // LDS is reused by the next gather
+    }
 }
 
 // shared dofs: partials in block order; dofs no element touches: 0; sink: 0
@@ -307,10 +318,19 @@ template <int K, int EPT, typename T>
 static int launch_apply_k(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *skip)
 {
     fg_ctx *c = op->ctx;
-    size_t shm = sizeof(T) * 2 * (size_t)(op->max_nloc + 1);
-    FG_LAUNCH((k_ebe_apply<K, EPT, T>), op->nblk, FG_TPB, shm, c->stream, op->nblk,
-              op->maxc, op->d_eoff, op->d_doff, op->d_coff, op->d_dofs, op->d_slots,
-              op->d_lml, Kv, P, Q, hp, skip);
+    const size_t shm = sizeof(T) == 8 ? op->smem64 : op->smem32;
+    const FgEbeLayout L = fg_ebe_layout(op);
+    if (op->pipe_wpc > 0) {
+        long long g = (long long)c->num_cu * op->pipe_wpc;
+        g = g / 8 * 8;
+        if (g < 8) g = 8;
+        if (g > op->nblk) g = op->nblk;
+        FG_LAUNCH((k_ebe_apply_pipe<K, EPT, T>), (int)g, FG_TPB, shm, c->stream, L, Kv, P, Q,
+                  hp, skip);
+        return FG_OK;
+    }
+    FG_LAUNCH((k_ebe_apply<K, EPT, T>), op->nblk, FG_TPB, shm, c->stream, L, Kv, P, Q, hp,
+              skip);
     return FG_OK;
 }
 
@@ -408,22 +428,25 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
     op->k = k;
     op->ept = ept;
     op->nblk = pl.nblk;
-    op->maxc = pl.maxc;
     op->max_nloc = pl.max_nloc;
     op->nkept = pl.nkept;
     op->nnz = (long long)k * k * pl.nkept;
     op->nhalo = (int)pl.hdofs.size();
     op->nslots = pl.nslots;
     op->nzero = (int)pl.zdofs.size();
-    size_t shm = sizeof(double) * 2 * (size_t)(pl.max_nloc + 1);
-    if (shm > 64 * 1024) {
+    op->smem64 = fg_ebe_smem_bytes<double>(k, pl.max_nloc, pl.max_ne, pl.max_ninc);
+    op->smem32 = fg_ebe_smem_bytes<float>(k, pl.max_nloc, pl.max_ne, pl.max_ninc);
+    if (op->smem64 > 64 * 1024) {
+        size_t shm = op->smem64;
         delete op;
         fg_set_error("block needs %zu bytes of LDS; use fewer elements per block", shm);
         return FG_ERR_ARG;
     }
     FG_CHECK(upload(pl.eoff, &op->d_eoff));
     FG_CHECK(upload(pl.doff, &op->d_doff));
-    FG_CHECK(upload(pl.coff, &op->d_coff));
+    FG_CHECK(upload(pl.ibase, &op->d_ibase));
+    FG_CHECK(upload(pl.ioff, &op->d_ioff));
+    FG_CHECK(upload(pl.inc, &op->d_inc));
     FG_CHECK(upload(pl.dofs, &op->d_dofs));
     FG_CHECK(upload(pl.slots, &op->d_slots));
     FG_CHECK(upload(pl.lml, &op->d_lml));
@@ -500,7 +523,8 @@ int fg_ebe_set_values_dev(fg_op *op, const double *A)
 
 void fg_ebe_free(fg_op *op)
 {
-    hipFree(op->d_eoff); hipFree(op->d_doff); hipFree(op->d_coff); hipFree(op->d_dofs);
+    hipFree(op->d_eoff); hipFree(op->d_doff); hipFree(op->d_dofs);
+    hipFree(op->d_ibase); hipFree(op->d_ioff); hipFree(op->d_inc);
     hipFree(op->d_slots); hipFree(op->d_lml); hipFree(op->d_K64); hipFree(op->d_K32);
     hipFree(op->d_kpos); hipFree(op->d_hdofs); hipFree(op->d_hoff); hipFree(op->d_zdofs);
     hipFree(op->d_hpart64); hipFree(op->d_hpart32); hipFree(op->d_inc_off);

@@ -1,18 +1,22 @@
 // fg_ebe_body.h -- one workgroup's share of an element-by-element product:
-// the block body used by the stand-alone EBE/EDE kernel (fg_ebe.hip) and by
+// the block body used by the stand-alone EBE/EDE kernels (fg_ebe.hip) and by
 // the matvec phase of the persistent GMRES kernel (fg_gmres_fused.hip).
 //
-// Block b holds ne elements (colour-sorted) and nd distinct dofs.  Values are
-// stored per block as [k*k][ne] and local dof indices as [k][ne], so every
-// global load is coalesced.  Steps:
-//   1. issue the loads of this thread's local indices (and, when they fit in
-//      registers, its element values) -- independent of P;
-//   2. gather the block's dofs of P into LDS, zero the LDS accumulators;
-//   3. row sums of each element (reference order: left to right over the
-//      columns, ebemv.c:34-36);
-//   4. add the row sums into the accumulators one colour at a time (no two
-//      elements of a colour share a dof, so no conflicts and a fixed order);
-//   5. dofs complete in this block -> Q, shared dofs -> their halo slot.
+// Block b holds ne elements (breadth-first order) and nd distinct dofs.
+// Values are stored per block as [k*k][ne] and local dof indices as [k][ne],
+// so every global load is coalesced.  For each local dof the block also
+// stores its incidences -- the (row r, element i) pairs that add into it, in
+// element order -- as LDS slots r*ne + i.  The body has three phases so that
+// a persistent kernel can overlap the loads of the next block with the work
+// on the current one:
+//   load   -- this thread's local indices (and, when they fit in registers,
+//             its element values); independent of P;
+//   gather -- the block's dofs of P, its incidence lists into LDS;
+//   finish -- row sums of each element (reference order: left to right over
+//             the columns, ebemv.c:34-36) into LDS; then every dof adds its
+//             incidences in their stored order (deterministic, no atomics),
+//             dofs complete in this block -> Q, shared dofs -> their halo
+//             slot.
 #pragma once
 
 #include "fg_internal.h"
@@ -21,91 +25,150 @@
 // values of a thread (EPT * k * k) in registers for k <= 4
 constexpr int fg_ebe_default_ept(int k) { return k <= 2 ? 8 : k == 3 ? 4 : k == 4 ? 2 : 1; }
 
-template <int K, int EPT, typename T>
-__device__ __forceinline__ void fg_ebe_block(
-    int b, int maxc, const int *__restrict__ eoff, const int *__restrict__ doff,
-    const uint16_t *__restrict__ coff, const int *__restrict__ dofs,
-    const int *__restrict__ slots, const uint16_t *__restrict__ lml, const T *__restrict__ Kv,
-    const T *__restrict__ P, T *__restrict__ Q, T *__restrict__ hpart, unsigned char *smem,
-    uint16_t *s_cb)
-{
-    constexpr bool PRE = EPT * K * K <= 72;   // element values held in registers
-    const int e0 = eoff[b], ne = eoff[b + 1] - e0;
-    const int d0 = doff[b], nd = doff[b + 1] - d0;
-    T *sP = reinterpret_cast<T *>(smem);
-    T *sA = sP + nd + 1;
-    const T *Kb = Kv + (size_t)K * K * e0;
-    const uint16_t *Lb = lml + (size_t)K * e0;
+// device view of the blocked layout
+struct FgEbeLayout {
+    int nblk;
+    const int *eoff;       // [nblk+1] first element of each block
+    const int *doff;       // [nblk+1] first local dof of each block
+    const int *dofs;       // global dof of each local dof
+    const int *slots;      // -1: dof complete in its block, else halo slot
+    const int *ibase;      // [nblk+1] first incidence of each block
+    const uint16_t *lml;   // local dof indices, [k][ne] per block (nd = sink)
+    const uint16_t *ioff;  // per block nd+1 incidence offsets, at doff[b] + b
+    const uint16_t *inc;   // incidence LDS slots r*ne + i
+};
 
+inline FgEbeLayout fg_ebe_layout(const fg_op *op)
+{
+    return FgEbeLayout{op->nblk, op->d_eoff, op->d_doff, op->d_dofs, op->d_slots,
+                       op->d_ibase, op->d_lml, op->d_ioff, op->d_inc};
+}
+
+// LDS bytes one block needs
+template <typename T>
+__host__ __device__ constexpr size_t fg_ebe_smem_bytes(int k, int nd, int ne, int ninc)
+{
+    return sizeof(T) * ((size_t)nd + 1 + (size_t)k * ne) + 2 * ((size_t)ninc + nd + 1);
+}
+
+template <int K, int EPT, typename T>
+struct FgEbeRegs {
+    static constexpr bool PRE = EPT * K * K <= 72;   // element values held in registers
     uint16_t li[EPT][K];
     T kv[PRE ? EPT : 1][PRE ? K * K : 1];
+};
+
+template <int K, int EPT, typename T>
+__device__ __forceinline__ void fg_ebe_load(int b, const FgEbeLayout &L, const T *__restrict__ Kv,
+                                            FgEbeRegs<K, EPT, T> &r)
+{
+    const int e0 = L.eoff[b], ne = L.eoff[b + 1] - e0;
+    const T *Kb = Kv + (size_t)K * K * e0;
+    const uint16_t *Lb = L.lml + (size_t)K * e0;
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
         const int el = threadIdx.x + i * FG_TPB;
         if (el < ne) {
 #pragma unroll
-            for (int j = 0; j < K; j++) li[i][j] = Lb[(size_t)j * ne + el];
-            if constexpr (PRE) {
+            for (int j = 0; j < K; j++) r.li[i][j] = Lb[(size_t)j * ne + el];
+            if constexpr (FgEbeRegs<K, EPT, T>::PRE) {
 #pragma unroll
-                for (int q = 0; q < K * K; q++) kv[i][q] = Kb[(size_t)q * ne + el];
+                for (int q = 0; q < K * K; q++) r.kv[i][q] = Kb[(size_t)q * ne + el];
             }
         }
     }
+}
+
+// LDS carve-up of a block: P values, element row sums, incidence lists
+template <int K, typename T>
+struct FgEbeSmem {
+    T *sP, *sY;
+    uint16_t *sInc, *sIoff;
+    __device__ __forceinline__ FgEbeSmem(unsigned char *smem, int nd, int ne, int ninc)
+    {
+        sP = reinterpret_cast<T *>(smem);
+        sY = sP + nd + 1;
+        sInc = reinterpret_cast<uint16_t *>(sY + (size_t)K * ne);
+        sIoff = sInc + ninc;
+    }
+};
+
+template <int K, typename T>
+__device__ __forceinline__ void fg_ebe_gather(int b, const FgEbeLayout &L,
+                                              const T *__restrict__ P, unsigned char *smem)
+{
+    const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
+    const int e0 = L.eoff[b], ne = L.eoff[b + 1] - e0;
+    const int i0 = L.ibase[b], ninc = L.ibase[b + 1] - i0;
+    FgEbeSmem<K, T> S(smem, nd, ne, ninc);
+    const uint16_t *io = L.ioff + d0 + b;
     for (int l = threadIdx.x; l < nd; l += FG_TPB) {
-        sP[l] = P[dofs[d0 + l]];
-        sA[l] = T(0);
+        S.sP[l] = P[L.dofs[d0 + l]];
+        S.sIoff[l] = io[l];
     }
     if (threadIdx.x == 0) {
-        sP[nd] = T(0);
-        sA[nd] = T(0);
+        S.sP[nd] = T(0);
+        S.sIoff[nd] = io[nd];
     }
-    for (int c = threadIdx.x; c <= maxc; c += FG_TPB) s_cb[c] = coff[(size_t)b * (maxc + 1) + c];
+    for (int q = threadIdx.x; q < ninc; q += FG_TPB) S.sInc[q] = L.inc[i0 + q];
     __syncthreads();
+}
 
-    T y[EPT][K];
-    int col[EPT];
+template <int K, int EPT, typename T>
+__device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
+                                              const T *__restrict__ Kv, T *__restrict__ Q,
+                                              T *__restrict__ hpart, unsigned char *smem,
+                                              const FgEbeRegs<K, EPT, T> &r)
+{
+    const int e0 = L.eoff[b], ne = L.eoff[b + 1] - e0;
+    const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
+    const int ninc = L.ibase[b + 1] - L.ibase[b];
+    FgEbeSmem<K, T> S(smem, nd, ne, ninc);
+    const T *Kb = Kv + (size_t)K * K * e0;
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
         const int el = threadIdx.x + i * FG_TPB;
-        col[i] = -1;
         if (el < ne) {
             T p[K];
 #pragma unroll
-            for (int j = 0; j < K; j++) p[j] = sP[li[i][j]];
+            for (int j = 0; j < K; j++) p[j] = S.sP[r.li[i][j]];
 #pragma unroll
-            for (int r = 0; r < K; r++) {
+            for (int rr = 0; rr < K; rr++) {
                 T acc;
-                if constexpr (PRE) {
-                    acc = kv[i][r * K] * p[0];
+                if constexpr (FgEbeRegs<K, EPT, T>::PRE) {
+                    acc = r.kv[i][rr * K] * p[0];
 #pragma unroll
-                    for (int c = 1; c < K; c++) acc = acc + kv[i][r * K + c] * p[c];
+                    for (int c = 1; c < K; c++) acc = acc + r.kv[i][rr * K + c] * p[c];
                 } else {
-                    acc = Kb[(size_t)(r * K) * ne + el] * p[0];
+                    acc = Kb[(size_t)(rr * K) * ne + el] * p[0];
 #pragma unroll
-                    for (int c = 1; c < K; c++) acc = acc + Kb[(size_t)(r * K + c) * ne + el] * p[c];
+                    for (int c = 1; c < K; c++) acc = acc + Kb[(size_t)(rr * K + c) * ne + el] * p[c];
                 }
-                y[i][r] = acc;
+                S.sY[rr * ne + el] = acc;
             }
-            int c = 0;
-            while (s_cb[c + 1] <= el) c++;
-            col[i] = c;
         }
     }
-    for (int c = 0; c < maxc; c++) {
-        if (s_cb[c] == s_cb[c + 1]) continue;   // empty colour: uniform over the block
-#pragma unroll
-        for (int i = 0; i < EPT; i++)
-            if (col[i] == c) {
-#pragma unroll
-                for (int r = 0; r < K; r++) sA[li[i][r]] = sA[li[i][r]] + y[i][r];
-            }
-        __syncthreads();
-    }
+    __syncthreads();
     for (int l = threadIdx.x; l < nd; l += FG_TPB) {
-        const int s = slots[d0 + l];
+        const int s0 = S.sIoff[l], s1 = S.sIoff[l + 1];
+        T acc = S.sY[S.sInc[s0]];
+        for (int q = s0 + 1; q < s1; q++) acc = acc + S.sY[S.sInc[q]];
+        const int s = L.slots[d0 + l];
         if (s < 0)
-            Q[dofs[d0 + l]] = sA[l];
+            Q[L.dofs[d0 + l]] = acc;
         else
-            hpart[s] = sA[l];
+            hpart[s] = acc;
     }
+}
+
+// the three phases back to back (one block per workgroup)
+template <int K, int EPT, typename T>
+__device__ __forceinline__ void fg_ebe_block(int b, const FgEbeLayout &L, const T *__restrict__ Kv,
+                                             const T *__restrict__ P, T *__restrict__ Q,
+                                             T *__restrict__ hpart, unsigned char *smem)
+{
+    FgEbeRegs<K, EPT, T> r;
+    fg_ebe_load<K, EPT, T>(b, L, Kv, r);
+    fg_ebe_gather<K, T>(b, L, P, smem);
+    fg_ebe_finish<K, EPT, T>(b, L, Kv, Q, hpart, smem, r);
 }

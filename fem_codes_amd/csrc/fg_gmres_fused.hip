@@ -51,9 +51,7 @@ struct Args {
     long long ld;
     double eps_final, inner_tol;
     // blocked element operator
-    int nblk, maxc;
-    const int *eoff, *doff, *dofs, *slots;
-    const uint16_t *coff, *lml;
+    FgEbeLayout L;
     const double *K64;
     const TB *KB;
     double *hp64;
@@ -152,22 +150,17 @@ __device__ bool grid_sync(Bar *bar, double *part, double *slot)
 
 // ---- matvec phase: blocked EBE body, workgroups stride over blocks ---------
 template <int K, int EPT, typename T>
-__device__ void ebe_phase(int nblk, int maxc, const int *__restrict__ eoff,
-                          const int *__restrict__ doff, const uint16_t *__restrict__ coff,
-                          const int *__restrict__ dofs, const int *__restrict__ slots,
-                          const uint16_t *__restrict__ lml, const T *__restrict__ Kv,
+__device__ void ebe_phase(const FgEbeLayout &L, const T *__restrict__ Kv,
                           const T *__restrict__ P, T *__restrict__ Q, T *__restrict__ hpart,
                           unsigned char *smem)
 {
-    __shared__ uint16_t s_cb[65];
     const int G = gridDim.x;
-    for (int t = blockIdx.x; t < nblk; t += G) {
+    for (int t = blockIdx.x; t < L.nblk; t += G) {
         // logical index t is congruent to blockIdx.x mod 8 (G is a multiple of 8):
         // the remap gives each XCD a contiguous range of blocks
-        const int b = fg_xcd_remap(t, nblk);
+        const int b = fg_xcd_remap(t, L.nblk);
         __syncthreads();   // LDS reuse across blocks
-        fg_ebe_block<K, EPT, T>(b, maxc, eoff, doff, coff, dofs, slots, lml, Kv, P, Q, hpart,
-                                smem, s_cb);
+        fg_ebe_block<K, EPT, T>(b, L, Kv, P, Q, hpart, smem);
     }
 }
 
@@ -211,8 +204,7 @@ __device__ __forceinline__ void matvec_phase(const Args<T> &a, const T *Kv, cons
     if constexpr (K == 0)
         csr_phase<T>(a.nrb, a.rb, a.IA, a.JA, AA, P, Q, smem);
     else
-        ebe_phase<K, fg_ebe_default_ept(K), T>(a.nblk, a.maxc, a.eoff, a.doff, a.coff, a.dofs,
-                                           a.slots, a.lml, Kv, P, Q, hpart, smem);
+        ebe_phase<K, fg_ebe_default_ept(K), T>(a.L, Kv, P, Q, hpart, smem);
 }
 
 // value of row `row` of a product computed by matvec_phase
@@ -252,8 +244,7 @@ __global__ __launch_bounds__(FG_TPB) void k_gmres_fused(Args<TB> A)
     const bool w0 = blockIdx.x == 0 && threadIdx.x == 0;
     double *R64 = A.W64;
     Args<double> A64;   // the FP64 view for the residual matvec
-    A64.nblk = A.nblk; A64.maxc = A.maxc; A64.eoff = A.eoff; A64.doff = A.doff;
-    A64.dofs = A.dofs; A64.slots = A.slots; A64.coff = A.coff; A64.lml = A.lml;
+    A64.L = A.L;
     A64.nrb = A.nrb; A64.rb = A.rb; A64.IA = A.IA; A64.JA = A.JA;
     Bar *bar = A.bar;
     bool ok = true;
@@ -477,10 +468,10 @@ int run_fused(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
     // the block body is compiled for the default block size of each k
     if (kk != 0 && op->ept != fg_ebe_default_ept(kk)) return FG_OK;
     if (pc && pc->kind != FG_PREC_DIAG) return FG_OK;
-    // LDS: the matvec block scratch (EBE 2*(max_nloc+1) doubles, CSR NNZB
-    // doubles), reused for the workgroup's slice of w (VPT*256 values)
-    const size_t shm_mv = kk == 0 ? sizeof(double) * NNZB
-                                  : sizeof(double) * 2 * (size_t)(op->max_nloc + 1);
+    // LDS: the matvec block scratch (EBE: P values, row sums and incidence
+    // lists of a block; CSR: NNZB doubles), reused for the workgroup's slice
+    // of w (VPT*256 values)
+    const size_t shm_mv = kk == 0 ? sizeof(double) * NNZB : op->smem64;
     // grid: 2 (else 1) workgroups per CU, a multiple of 8 (one range per XCD);
     // rows per thread: the power of two that covers neq
     int vpt = 0, nb = 0, G = 0;
@@ -533,8 +524,7 @@ int run_fused(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
         a.nrb = op->nrb; a.rb = op->d_rb; a.IA = op->d_IA; a.JA = op->d_JA;
         a.AA64 = op->d_AA; a.AAB = ir ? (const TB *)op->d_AA32 : (const TB *)op->d_AA;
     } else {
-        a.nblk = op->nblk; a.maxc = op->maxc; a.eoff = op->d_eoff; a.doff = op->d_doff;
-        a.dofs = op->d_dofs; a.slots = op->d_slots; a.coff = op->d_coff; a.lml = op->d_lml;
+        a.L = fg_ebe_layout(op);
         a.K64 = op->d_K64; a.KB = ir ? (const TB *)op->d_K32 : (const TB *)op->d_K64;
         a.hp64 = op->d_hpart64; a.hpB = ir ? (TB *)op->d_hpart32 : (TB *)op->d_hpart64;
         a.rowmap = op->d_rowmap; a.hoff = op->d_hoff;

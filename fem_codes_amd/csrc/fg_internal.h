@@ -77,10 +77,14 @@ struct fg_op {
     long long nnz = 0;     // CSR nonzeros / EBE stored values
 
     // ---- blocked element layout (EBE / EDE) ----
-    int nblk = 0, maxc = 0, ept = 0, max_nloc = 0, nkept = 0;
+    int nblk = 0, ept = 0, max_nloc = 0, nkept = 0;
+    size_t smem64 = 0, smem32 = 0;  // LDS bytes per block (FP64 / FP32 values)
+    int pipe_wpc = 0;            // 0: a block per workgroup; > 0: persistent, WGs per CU
     int *d_eoff = nullptr;       // [nblk+1] first element (blocked order) of each block
     int *d_doff = nullptr;       // [nblk+1] first local dof of each block
-    uint16_t *d_coff = nullptr;  // [nblk*(maxc+1)] colour offsets inside each block
+    int *d_ibase = nullptr;      // [nblk+1] first incidence of each block
+    uint16_t *d_ioff = nullptr;  // per block nd+1 incidence offsets (at doff[b] + b)
+    uint16_t *d_inc = nullptr;   // incidence LDS slots r*ne + i, per block
     int *d_dofs = nullptr;       // [ndloc] global dof of each local dof
     int *d_slots = nullptr;      // [ndloc] -1: complete in block; else halo slot
     uint16_t *d_lml = nullptr;   // [k*nkept] local dof indices, [k][ne_b] per block

@@ -86,6 +86,15 @@ extern "C" int fg_op_info(const fg_op *op, int *neq, int *nitems, int *k, long l
     return FG_OK;
 }
 
+extern "C" int fg_op_set_pipeline(fg_op *op, int wg_per_cu)
+{
+    FG_ARG(op, "op");
+    FG_ARG(op->kind == FG_OP_EBE || op->kind == FG_OP_EDE, "pipeline applies to EBE/EDE operators");
+    FG_ARG(wg_per_cu >= 0 && wg_per_cu <= 16, "wg_per_cu in [0,16]");
+    op->pipe_wpc = wg_per_cu;
+    return FG_OK;
+}
+
 extern "C" double fg_op_algorithmic_bytes(const fg_op *op, int fp32)
 {
     if (!op) return 0.0;

@@ -105,6 +105,12 @@ double fg_op_algorithmic_bytes(const fg_op *op, int fp32);
  * one event pair around each launch. */
 int fg_op_time_apply(fg_op *op, int fp32, void *P, void *Q, int reps, int main_only,
                      double *ms);
+/* EBE/EDE launch schedule (no reference counterpart: a tuning knob of the
+ * blocked layout).  wg_per_cu = 0: one element block per workgroup.
+ * wg_per_cu > 0: a persistent grid of wg_per_cu workgroups per CU, each
+ * walking its blocks and loading the next block's element matrices while it
+ * reduces the current one.  Results are identical either way. */
+int fg_op_set_pipeline(fg_op *op, int wg_per_cu);
 
 /* ---- preconditioners ------------------------------------------------------ */
 #define FG_PREC_NONE    0   /* NO_precond.c / NO_precond_setup.c            */

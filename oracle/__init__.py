@@ -34,6 +34,24 @@ class OrcProb2d(C.Structure):
                 ("scale", C.c_double), ("h_shock", C.c_int)]
 
 
+P3_DIFCONV, P3_CONST = 0, 1
+
+
+class OrcProb3d(C.Structure):
+    _fields_ = [("kind", C.c_int), ("A", C.c_double), ("kappa", C.c_double),
+                ("bx", C.c_double), ("by", C.c_double), ("bz", C.c_double),
+                ("sigma", C.c_double), ("f", C.c_double)]
+
+
+def prob3d(kind=P3_DIFCONV, A=1.0, kappa=0.0, bx=0.0, by=0.0, bz=0.0, sigma=0.0, f=0.0):
+    return OrcProb3d(kind, A, kappa, bx, by, bz, sigma, f)
+
+
+class OrcIluCtx(C.Structure):
+    _fields_ = [("neq", C.c_int), ("LU", C.c_void_p), ("JA", C.c_void_p), ("IA", C.c_void_p),
+                ("D", C.c_void_p)]
+
+
 OP_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 
@@ -109,8 +127,65 @@ class Oracle:
                                        C.POINTER(OrcProb2d), _dp, _dp]
         L.orc_ede_assembly.argtypes = [_i, _i, _dp, _ip, _ip, _dp]
         L.orc_csr_assembly.argtypes = [_i, _i, _dp, _ip, _dp]
-        for nm in ("orc_op_csr", "orc_op_ebe", "orc_op_ede", "orc_op_diag"):
+        for nm in ("orc_op_csr", "orc_op_ebe", "orc_op_ede", "orc_op_diag", "orc_op_ilu0"):
             getattr(L, nm).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_csr_init_k.restype = _i
+        L.orc_csr_init_k.argtypes = [_i, _i, _i, _ip, _vp, _vp, _vp]
+        L.orc_csr_assembly_k.argtypes = [_i, _i, _i, _dp, _ip, _dp]
+        L.orc_ilu0_factor.restype = _i
+        L.orc_ilu0_factor.argtypes = [_i, _dp, _ip, _ip, _dp, _dp]
+        L.orc_ilu0_apply.argtypes = [_i, _dp, _ip, _ip, _dp, _dp, _dp]
+        L.orc_ilu0_apply_f32.argtypes = [_i, _fp, _ip, _ip, _fp, _fp, _fp]
+        L.orc_mesh3d_structured.argtypes = [_i, _dp, _dp, _dp, _ip, _ip]
+        L.orc_prob3d_f.restype = _d
+        L.orc_prob3d_f.argtypes = [C.POINTER(OrcProb3d), _d, _d, _d]
+        L.orc_build_supg3d.argtypes = [_i, _i, _dp, _dp, _dp, _ip, _ip, _ip,
+                                       C.POINTER(OrcProb3d), _dp, _dp]
+
+    # ---- 3D / CSR(k) / ILU(0) ----
+    def mesh3d(self, n):
+        nn, nel = (n + 1) ** 3, 6 * n ** 3
+        x, y, z = np.zeros(nn), np.zeros(nn), np.zeros(nn)
+        t, v = np.zeros(nn, np.int32), np.zeros(4 * nel, np.int32)
+        self.lib.orc_mesh3d_structured(n, x, y, z, t, v)
+        return x, y, z, t, v
+
+    def csr_init_k(self, neq, nel, k, lm):
+        nnz = self.lib.orc_csr_init_k(neq, nel, k, lm, None, None, None)
+        IA = np.zeros(neq + 1, np.int32)
+        JA = np.zeros(nnz, np.int32)
+        cbe = np.zeros(k * k * nel, np.int32)
+        self.lib.orc_csr_init_k(neq, nel, k, lm, _ptr(IA), _ptr(JA), _ptr(cbe))
+        return nnz, IA, JA, cbe
+
+    def csr_assembly_k(self, nel, k, nnz, ke, cbe):
+        AA = np.zeros(nnz + 1)
+        self.lib.orc_csr_assembly_k(nel, k, nnz, ke, cbe, AA)
+        return AA
+
+    def ilu0_factor(self, neq, AA, JA, IA):
+        LU = np.zeros(len(JA))
+        D = np.zeros(max(neq, 1))
+        rc = self.lib.orc_ilu0_factor(neq, np.ascontiguousarray(AA[:len(JA)]), JA, IA, LU, D)
+        return rc, LU, D
+
+    def ilu0_apply(self, neq, LU, JA, IA, D, p):
+        z = np.zeros(neq + 1)
+        self.lib.orc_ilu0_apply(neq, LU, JA, IA, D, np.ascontiguousarray(p), z)
+        return z
+
+    def ilu0_apply_f32(self, neq, LU, JA, IA, D, p):
+        z = np.zeros(neq + 1, np.float32)
+        self.lib.orc_ilu0_apply_f32(neq, LU.astype(np.float32), JA, IA, D.astype(np.float32),
+                                    np.ascontiguousarray(p, np.float32), z)
+        return z
+
+    def build_supg3d(self, x, y, z, utype, vertex, lm, neq, pb):
+        nel = len(vertex) // 4
+        ke = np.zeros(16 * nel)
+        F = np.zeros(neq + 1)
+        self.lib.orc_build_supg3d(nel, neq, x, y, z, utype, vertex, lm, C.byref(pb), ke, F)
+        return ke, F
 
     # ---- BLAS / matvec ----
     def ddot(self, x, y):
@@ -163,9 +238,11 @@ class Oracle:
         return z
 
     # ---- GMRES over a stored operator ----
-    def pgmres(self, neq, kmax, lmax, tol, B, scheme, mats, invDiag=None, hist_cap=100000):
+    def pgmres(self, neq, kmax, lmax, tol, B, scheme, mats, invDiag=None, hist_cap=100000,
+               ilu=None):
         """scheme: 'CSR' (mats=(AA,JA,IA)), 'EBE' (mats=(A,lm,k,nel)),
-        'EDE' (mats=(A,lm2,nedge)).  invDiag: Diag left preconditioner or None.
+        'EDE' (mats=(A,lm2,nedge)).  invDiag: Diag left preconditioner or None;
+        ilu: (LU, JA, IA, D) ILU(0) left preconditioner or None.
         Returns (X, steps, cycles, rho_hist, e0_hist)."""
         L = self.lib
         keep = []
@@ -186,6 +263,11 @@ class Oracle:
             dctx = OrcDiagCtx(neq, _ptr(invDiag)); keep.append(dctx)
             ops.precond = C.cast(L.orc_op_diag, C.c_void_p)
             ops.precond_ctx = C.cast(C.pointer(dctx), C.c_void_p)
+        if ilu is not None:
+            LU, JAi, IAi, Di = ilu
+            ictx = OrcIluCtx(neq, _ptr(LU), _ptr(JAi), _ptr(IAi), _ptr(Di)); keep.append(ictx)
+            ops.precond = C.cast(L.orc_op_ilu0, C.c_void_p)
+            ops.precond_ctx = C.cast(C.pointer(ictx), C.c_void_p)
         X = np.zeros(neq + 1)
         rho = np.zeros(hist_cap)
         e0 = np.zeros(lmax + 1)
@@ -357,3 +439,57 @@ class Reference:
 
     def ddot(self, x, y):
         return self.lib.ref_ddot(len(x), x.copy(), y.copy())
+
+
+class Reference3D:
+    """The reference 3D steady-transport routines (Build_SUPG with the DIFCONV
+    problem, CSR ILU(0)) compiled from their sources (oracle/_ref).
+
+    Available only where oracle/_ref/libref_ss3d.so was built."""
+
+    PATH = os.path.join(HERE, "_ref", "libref_ss3d.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.exists(cls.PATH)
+
+    def __init__(self):
+        L = self.lib = C.CDLL(self.PATH)
+        L.ref3d_setup.restype = _i
+        L.ref3d_setup.argtypes = [_i, _i, _dp, _dp, _dp, _ip, _ip, _d]
+        L.ref3d_lm.argtypes = [_ip]
+        L.ref3d_build.argtypes = [_dp, _dp]
+        L.ref3d_ilu0_setup.argtypes = [_i, _i, _dp, _ip, _ip, _dp, _dp, _dp]
+        L.ref3d_ilu0_apply.argtypes = [_dp, _dp]
+
+    def setup(self, x, y, z, utype, vertex, cst=1.0):
+        self.nel = len(vertex) // 4
+        self.neq = self.lib.ref3d_setup(len(x), self.nel, x, y, z, utype, vertex, cst)
+        return self.neq
+
+    def lm(self):
+        lm = np.zeros(4 * self.nel, np.int32)
+        self.lib.ref3d_lm(lm)
+        return lm
+
+    def build(self):
+        ke = np.zeros(16 * self.nel)
+        F = np.zeros(self.neq + 1)
+        self.lib.ref3d_build(ke, F)
+        return ke, F
+
+    def ilu0_setup(self, neq, AA, JA, IA, F):
+        nnz = len(JA)
+        LU = np.zeros(nnz)
+        D = np.zeros(max(neq, 1))
+        Fp = np.ascontiguousarray(F, np.float64).copy()
+        self._keep = (np.ascontiguousarray(AA, np.float64).copy(), JA.copy(), IA.copy())
+        self.neq_ilu = neq
+        self.lib.ref3d_ilu0_setup(neq, nnz, self._keep[0], self._keep[1], self._keep[2], Fp,
+                                  LU, D)
+        return LU, D, Fp
+
+    def ilu0_apply(self, p):
+        z = np.zeros(self.neq_ilu + 1)
+        self.lib.ref3d_ilu0_apply(np.ascontiguousarray(p, np.float64).copy(), z)
+        return z

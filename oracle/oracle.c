@@ -282,6 +282,12 @@ void orc_op_diag(void *ctx, double *x, double *y)
     orc_diag_apply(c->neq, c->invDiag, x, y);
 }
 
+void orc_op_ilu0(void *ctx, double *x, double *y)
+{
+    orc_ilu0_ctx *c = (orc_ilu0_ctx *)ctx;
+    orc_ilu0_apply(c->neq, c->LU, c->JA, c->IA, c->D, x, y);
+}
+
 /* ======================================================================== */
 /* restarted GMRES                                                           */
 /* ======================================================================== */
@@ -459,16 +465,16 @@ static int pair_find(const pair_t *v, long lo, long hi, int c)
 }
 
 /* csr_Initialization.c:5-218 with reordering NOT (reordering.c:8-10) */
-int orc_csr_init(int neq, int nel, const int *lm, int *IA, int *JA, int *csr_by_elem)
+int orc_csr_init_k(int neq, int nel, int k, const int *lm, int *IA, int *JA, int *csr_by_elem)
 {
-    pair_t *pr = (pair_t *)malloc(sizeof(pair_t) * (size_t)nel * 9 + 1);
+    pair_t *pr = (pair_t *)malloc(sizeof(pair_t) * (size_t)nel * k * k + 1);
     long np = 0;
     for (long e = 0; e < nel; e++)
-        for (int a = 0; a < 3; a++) {
-            int I = lm[3 * e + a];
+        for (int a = 0; a < k; a++) {
+            int I = lm[k * e + a];
             if (I == neq) continue;
-            for (int b = 0; b < 3; b++) {
-                int J = lm[3 * e + b];
+            for (int b = 0; b < k; b++) {
+                int J = lm[k * e + b];
                 if (J == neq) continue;
                 pr[np].r = I; pr[np].c = J; np++;
             }
@@ -479,16 +485,21 @@ int orc_csr_init(int neq, int nel, const int *lm, int *IA, int *JA, int *csr_by_
     for (long q = 0; q < nnz; q++) { IA[pr[q].r + 1]++; JA[q] = pr[q].c; }
     for (int r = 0; r < neq; r++) IA[r + 1] += IA[r];
     for (long e = 0; e < nel; e++)
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) {
-                int I = lm[3 * e + a], J = lm[3 * e + b];
+        for (int a = 0; a < k; a++)
+            for (int b = 0; b < k; b++) {
+                int I = lm[k * e + a], J = lm[k * e + b];
                 int slot = (int)nnz;
                 if (I != neq && J != neq)
                     slot = pair_find(pr, IA[I], IA[I + 1], J);
-                csr_by_elem[9 * e + 3 * a + b] = slot;
+                csr_by_elem[(long)k * k * e + k * a + b] = slot;
             }
     free(pr);
     return (int)nnz;
+}
+
+int orc_csr_init(int neq, int nel, const int *lm, int *IA, int *JA, int *csr_by_elem)
+{
+    return orc_csr_init_k(neq, nel, 3, lm, IA, JA, csr_by_elem);
 }
 
 /* ede_Initialization.c:4-171.  The three local vertices of every element are
@@ -747,10 +758,272 @@ void orc_ede_assembly(int nel, int nedge, const double *ke, const int *order,
 /* csr_assembly.c:3-24 after setzeros.c:12-15 */
 void orc_csr_assembly(int nel, int nnz, const double *ke, const int *csr_by_elem, double *AA)
 {
+    orc_csr_assembly_k(nel, 3, nnz, ke, csr_by_elem, AA);
+}
+
+/* 2D csr_assembly.c:3-24 / 3D csr_assembly.c:3-30: slot by slot in row-major
+ * element order, sink reset after every element */
+void orc_csr_assembly_k(int nel, int k, int nnz, const double *ke, const int *csr_by_elem,
+                        double *AA)
+{
+    const int kk = k * k;
     memset(AA, 0, sizeof(double) * ((size_t)nnz + 1));
     for (long E = 0; E < nel; E++) {
-        for (int q = 0; q < 9; q++)
-            AA[csr_by_elem[9 * E + q]] += ke[9 * E + q];
+        for (int q = 0; q < kk; q++)
+            AA[csr_by_elem[kk * E + q]] += ke[kk * E + q];
         AA[nnz] = 0;
+    }
+}
+
+/* ======================================================================== */
+/* ILU(0)                                                                    */
+/* ======================================================================== */
+
+/* ILUp_precond_setup.c:8-60 with p = 0 and ILUp :66-167.  With p = 0 the
+ * symbolic phase (ilup.c LEVEL_OF_FILL_lu) adds no fill-in, so L and U have
+ * the pattern of A below and above the diagonal, each row in ascending
+ * column order (the CSR order).  Row i: L, D, U copied from A; for each L
+ * entry in order, the multiplier L_ij *= D_j (D already inverted) and row j
+ * of U is merged into row i where the patterns meet; finally D_i = 1/D_i.
+ * LU[nnz] holds L and U in A's slots (the diagonal slot unused), D[neq] the
+ * inverted pivots.  Returns 0, or 1 + i for a zero pivot at row i (the
+ * reference prints and exits, :154-163). */
+int orc_ilu0_factor(int neq, const double *AA, const int *JA, const int *IA, double *LU,
+                    double *D)
+{
+    int *jw = (int *)malloc(sizeof(int) * (size_t)(neq > 0 ? neq : 1));
+    for (int j = 0; j < neq; j++) jw[j] = -1;
+    int rc = 0;
+    for (int i = 0; i < neq && !rc; i++) {
+        for (int q = IA[i]; q < IA[i + 1]; q++) {
+            jw[JA[q]] = q;
+            LU[q] = 0;
+        }
+        D[i] = 0;
+        for (int q = IA[i]; q < IA[i + 1]; q++) {
+            if (JA[q] == i) D[i] = AA[q];
+            else LU[q] = AA[q];
+        }
+        for (int q = IA[i]; q < IA[i + 1] && JA[q] < i; q++) {
+            int jrow = JA[q];
+            LU[q] *= D[jrow];
+            for (int t = IA[jrow]; t < IA[jrow + 1]; t++) {
+                int col = JA[t];
+                if (col <= jrow) continue;            /* U part of row jrow */
+                int pos = jw[col];
+                if (pos == -1) continue;
+                if (col == i) D[i] -= LU[q] * LU[t];
+                else LU[pos] -= LU[q] * LU[t];
+            }
+        }
+        for (int q = IA[i]; q < IA[i + 1]; q++) jw[JA[q]] = -1;
+        if (D[i] == 0) rc = 1 + i;
+        else D[i] = 1.0 / D[i];
+    }
+    free(jw);
+    return rc;
+}
+
+/* ILUp_precond.c:26-46: forward solve with unit L, backward with U and the
+ * inverted pivots; z may alias p */
+void orc_ilu0_apply(int neq, const double *LU, const int *JA, const int *IA, const double *D,
+                    const double *p, double *z)
+{
+    for (int i = 0; i < neq; i++) {
+        double x = p[i];
+        for (int q = IA[i]; q < IA[i + 1] && JA[q] < i; q++) x -= z[JA[q]] * LU[q];
+        z[i] = x;
+    }
+    for (int i = neq - 1; i >= 0; i--) {
+        double x = z[i];
+        for (int q = IA[i]; q < IA[i + 1]; q++)
+            if (JA[q] > i) x -= z[JA[q]] * LU[q];
+        z[i] = x * D[i];
+    }
+}
+
+void orc_ilu0_apply_f32(int neq, const float *LU, const int *JA, const int *IA, const float *D,
+                        const float *p, float *z)
+{
+    for (int i = 0; i < neq; i++) {
+        float x = p[i];
+        for (int q = IA[i]; q < IA[i + 1] && JA[q] < i; q++) x -= z[JA[q]] * LU[q];
+        z[i] = x;
+    }
+    for (int i = neq - 1; i >= 0; i--) {
+        float x = z[i];
+        for (int q = IA[i]; q < IA[i + 1]; q++)
+            if (JA[q] > i) x -= z[JA[q]] * LU[q];
+        z[i] = x * D[i];
+    }
+}
+
+/* ======================================================================== */
+/* 3D steady transport (01_SS_Transport_Equation/3D)                         */
+/* ======================================================================== */
+
+/* Unit cube, n^3 cells, each split into the 6 tetrahedra around its main
+ * diagonal (Kuhn split, conforming across cells): for every permutation
+ * (a,b,c) of the axes the tetrahedron (v0, v0+e_a, v0+e_a+e_b, v0+e_a+e_b+e_c),
+ * with the last two vertices swapped for odd permutations so that every
+ * element has positive volume in Element_Configuration.c:58.
+ * Node (i,j,k) has number i + (n+1)(j + (n+1)k); boundary nodes get uType 0
+ * (prescribed), inner nodes uType 1 (unknown), as Fill_ID.c reads them.
+ * Elements cell by cell (i fastest), the 6 permutations in lexicographic
+ * order.  x,y,z,utype: (n+1)^3 ; vertex: 6 n^3 * 4.  This generator is the
+ * build's own (the reference reads meshes from files). */
+void orc_mesh3d_structured(int n, double *x, double *y, double *z, int *utype, int *vertex)
+{
+    const long N = n + 1;
+    for (long k = 0; k < N; k++)
+        for (long j = 0; j < N; j++)
+            for (long i = 0; i < N; i++) {
+                long v = i + N * (j + N * k);
+                x[v] = (double)i / n;
+                y[v] = (double)j / n;
+                z[v] = (double)k / n;
+                utype[v] = (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n) ? 1 : 0;
+            }
+    static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    const long stride[3] = {1, N, N * N};
+    long e = 0;
+    for (long k = 0; k < n; k++)
+        for (long j = 0; j < n; j++)
+            for (long i = 0; i < n; i++) {
+                long v0 = i + N * (j + N * k);
+                for (int p = 0; p < 6; p++) {
+                    long v = v0;
+                    vertex[4 * e] = (int)v;
+                    for (int s = 0; s < 3; s++) {
+                        v += stride[perm[p][s]];
+                        vertex[4 * e + 1 + s] = (int)v;
+                    }
+                    if (p == 1 || p == 2 || p == 5) {   /* odd permutation: keep volume > 0 */
+                        int tmp = vertex[4 * e + 2];
+                        vertex[4 * e + 2] = vertex[4 * e + 3];
+                        vertex[4 * e + 3] = tmp;
+                    }
+                    e++;
+                }
+            }
+}
+
+#define ORC_PI3 3.141592654   /* SSTransportEquation3D.h:15 */
+
+/* DIFCONV/DIFCONV_*.c (kappa = A, beta = (1,1,1), sigma = 0, manufactured f,
+ * u = 0 on the boundary) or constant coefficients */
+static void prob3d_kappa(const orc_prob3d *pb, double *kx, double *ky, double *kz)
+{
+    if (pb->kind == ORC_P3_DIFCONV) { *kx = pb->A; *ky = pb->A; *kz = pb->A; }
+    else { *kx = pb->kappa; *ky = pb->kappa; *kz = pb->kappa; }
+}
+static void prob3d_beta(const orc_prob3d *pb, double *bx, double *by, double *bz)
+{
+    if (pb->kind == ORC_P3_DIFCONV) { *bx = 1.0; *by = 1.0; *bz = 1.0; }
+    else { *bx = pb->bx; *by = pb->by; *bz = pb->bz; }
+}
+static double prob3d_sigma(const orc_prob3d *pb)
+{
+    return pb->kind == ORC_P3_DIFCONV ? 0.0 : pb->sigma;
+}
+/* DIFCONV_f.c */
+double orc_prob3d_f(const orc_prob3d *pb, double x, double y, double z)
+{
+    if (pb->kind != ORC_P3_DIFCONV) return pb->f;
+    const double A = pb->A, PI = ORC_PI3;
+    double f;
+    f = A * 3.0 * PI * PI * sin(PI * x) * sin(PI * y) * sin(PI * z) +
+        PI * cos(PI * x) * sin(PI * y) * sin(PI * z) + PI * sin(PI * x) * cos(PI * y) * sin(PI * z) +
+        PI * sin(PI * x) * sin(PI * y) * cos(PI * z);
+    return f;
+}
+
+/* Build_SUPG.c:22-196 with Element_Configuration.c:4-58 and
+ * font_assembly.c:3-57 (prescribed values are 0 for both problem kinds).
+ * ke[16e + 4r + c] = Ke[r][c] (ebe_assembly.c:11-15 stores row-major). */
+void orc_build_supg3d(int nel, int neq, const double *x, const double *y, const double *z,
+                      const int *utype, const int *vertex, const int *lm,
+                      const orc_prob3d *pb, double *ke, double *F)
+{
+    for (int i = 0; i <= neq; i++) F[i] = 0.0;
+    for (long e = 0; e < nel; e++) {
+        const int *V = vertex + 4 * e;
+        double X[4], Y[4], Z[4], a[4], b[4], c[4], d[4];
+        for (int i = 0; i < 4; i++) { X[i] = x[V[i]]; Y[i] = y[V[i]]; Z[i] = z[V[i]]; }
+        a[0] =  (X[1]*Y[2]*Z[3] + X[3]*Y[1]*Z[2] + X[2]*Y[3]*Z[1]) - (X[3]*Y[2]*Z[1] + X[1]*Y[3]*Z[2] + X[2]*Y[1]*Z[3]);
+        b[0] = -(Y[1]-Y[3])*(Z[2]-Z[3]) + (Y[2]-Y[3])*(Z[1]-Z[3]);
+        b[1] =  (Y[0]-Y[3])*(Z[2]-Z[3]) - (Y[2]-Y[3])*(Z[0]-Z[3]);
+        b[2] = -(Y[0]-Y[3])*(Z[1]-Z[3]) + (Y[1]-Y[3])*(Z[0]-Z[3]);
+        b[3] =  (Y[0]-Y[2])*(Z[1]-Z[2]) - (Y[1]-Y[2])*(Z[0]-Z[2]);
+        c[0] =  (X[1]-X[3])*(Z[2]-Z[3]) - (X[2]-X[3])*(Z[1]-Z[3]);
+        c[1] = -(X[0]-X[3])*(Z[2]-Z[3]) + (X[2]-X[3])*(Z[0]-Z[3]);
+        c[2] =  (X[0]-X[3])*(Z[1]-Z[3]) - (X[1]-X[3])*(Z[0]-Z[3]);
+        c[3] = -(X[0]-X[2])*(Z[1]-Z[2]) + (X[1]-X[2])*(Z[0]-Z[2]);
+        d[0] = -(X[1]-X[3])*(Y[2]-Y[3]) + (X[2]-X[3])*(Y[1]-Y[3]);
+        d[1] =  (X[0]-X[3])*(Y[2]-Y[3]) - (X[2]-X[3])*(Y[0]-Y[3]);
+        d[2] = -(X[0]-X[3])*(Y[1]-Y[3]) + (X[1]-X[3])*(Y[0]-Y[3]);
+        d[3] =  (X[0]-X[2])*(Y[1]-Y[2]) - (X[1]-X[2])*(Y[0]-Y[2]);
+        const double Volume = (a[0] + X[0]*b[0] + Y[0]*c[0] + Z[0]*d[0])/6.0;
+        const double h = pow(Volume, 1.0/3.0);
+        const double xb = (X[0] + X[1] + X[2] + X[3])/4.0;
+        const double yb = (Y[0] + Y[1] + Y[2] + Y[3])/4.0;
+        const double zb = (Z[0] + Z[1] + Z[2] + Z[3])/4.0;
+        (void)xb; (void)yb; (void)zb;   /* coefficients here are constant in space */
+        double bb[4][4], cc[4][4], dd[4][4];
+        for (int r = 0; r < 4; r++)
+            for (int s = 0; s < 4; s++) { bb[r][s] = b[r]*b[s]; cc[r][s] = c[r]*c[s]; dd[r][s] = d[r]*d[s]; }
+        const double ConstD = 1.0/(36.0*Volume);
+        double kX, kY, kZ, bX, bY, bZ;
+        prob3d_kappa(pb, &kX, &kY, &kZ);
+        const double normK = sqrt(kX*kX + kY*kY + kZ*kZ);
+        double E[4][4];
+        for (int r = 0; r < 4; r++)
+            for (int s = 0; s < 4; s++) E[r][s] = kX*bb[r][s] + kY*cc[r][s] + kZ*dd[r][s];
+        const double ConstC = 1.0/24.0;
+        prob3d_beta(pb, &bX, &bY, &bZ);
+        const double normB = sqrt(bX*bX + bY*bY + bZ*bZ);
+        double C[4];
+        for (int r = 0; r < 4; r++) C[r] = bX*b[r] + bY*c[r] + bZ*d[r];
+        const double sigma = prob3d_sigma(pb);
+        const double ConstR = (sigma*Volume)/20.0;
+        const double peclet = (normB*h)/(2.0*normK);
+        const double xi = fmax(0, 1-1/peclet);
+        const double tau = (h*xi)/(2.0*normB);
+        const double ConstCsupg = tau/(36.0*Volume);
+        double Cs[4][4];
+        for (int r = 0; r < 4; r++)
+            for (int s = r + 1; s < 4; s++)
+                Cs[r][s] = Cs[s][r] =
+                    bX*bX*b[r]*b[s] + bX*bY*c[r]*b[s] + bX*bZ*d[r]*b[s] + bX*bY*b[r]*c[s] +
+                    bY*bY*c[r]*c[s] + bY*bZ*d[r]*c[s] + bX*bZ*b[r]*d[s] + bY*bZ*c[r]*d[s] +
+                    bZ*bZ*d[r]*d[s];
+        Cs[0][0] = -(Cs[0][1] + Cs[0][2] + Cs[0][3]);
+        Cs[1][1] = -(Cs[0][1] + Cs[1][2] + Cs[1][3]);
+        Cs[2][2] = -(Cs[0][2] + Cs[1][2] + Cs[2][3]);
+        Cs[3][3] = -(Cs[0][3] + Cs[1][3] + Cs[2][3]);
+        const double ConstRsupg = (sigma*tau)/24.0;
+        double Ke[4][4];
+        for (int r = 0; r < 4; r++)
+            for (int s = 0; s < 4; s++)
+                Ke[r][s] = ConstD*E[r][s] + ConstC*C[s] + ConstCsupg*Cs[r][s] + ConstRsupg*C[r] +
+                           (r == s ? 2*ConstR : ConstR);
+        double f[4], Fe[4];
+        const double ConstF = Volume/20.0;
+        for (int i = 0; i < 4; i++) f[i] = orc_prob3d_f(pb, X[i], Y[i], Z[i]);
+        const double Faux = f[0] + f[1] + f[2] + f[3];
+        const double ConstFsupg = tau/24.0;
+        for (int i = 0; i < 4; i++) Fe[i] = ConstF*(Faux + f[i]) + ConstFsupg*C[i]*Faux;
+        for (int r = 0; r < 4; r++)
+            for (int s = 0; s < 4; s++) ke[16 * e + 4 * r + s] = Ke[r][s];
+        /* font_assembly.c:21-56: g = 0 at unknowns (uType 1) and upresc at
+         * prescribed nodes, which is 0 for both problem kinds here */
+        double g[4];
+        for (int i = 0; i < 4; i++) g[i] = 0.0;
+        (void)utype;
+        for (int r = 0; r < 4; r++)
+            Fe[r] = Fe[r] - (Ke[r][0]*g[0] + Ke[r][1]*g[1] + Ke[r][2]*g[2] + Ke[r][3]*g[3]);
+        const int *L = lm + 4 * e;
+        for (int r = 0; r < 4; r++) F[L[r]] = F[L[r]] + Fe[r];
+        F[neq] = 0.0;
     }
 }

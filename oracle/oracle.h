@@ -100,6 +100,8 @@ void orc_op_csr(void *ctx, double *x, double *y);
 void orc_op_ebe(void *ctx, double *x, double *y);
 void orc_op_ede(void *ctx, double *x, double *y);
 void orc_op_diag(void *ctx, double *x, double *y);
+typedef struct { int neq; const double *LU; const int *JA; const int *IA; const double *D; } orc_ilu0_ctx;
+void orc_op_ilu0(void *ctx, double *x, double *y);   /* ILUp_precond.c:4-49 */
 
 /* ---------------- 2D mesh, structures and SUPG assembly ------------------- */
 /* Structured unit-square mesh, n x n cells, each cell split along the
@@ -153,6 +155,41 @@ void orc_ede_assembly(int nel, int nedge, const double *ke, const int *order,
                       const int *edge_by_elem, double *Aede);
 /* csr_assembly.c:3-24 : AA[nnz+1] from ke (zeroed first as setzeros.c) */
 void orc_csr_assembly(int nel, int nnz, const double *ke, const int *csr_by_elem, double *AA);
+/* the same for k x k element matrices (3D csr_assembly.c:3-30 for k = 4) */
+void orc_csr_assembly_k(int nel, int k, int nnz, const double *ke, const int *csr_by_elem,
+                        double *AA);
+/* csr_Initialization.c for k local dofs per element (2D :5-218 with k=3,
+ * 3D :80-256 with k=4; reordering "NOT"); csr_by_elem[nel*k*k] */
+int  orc_csr_init_k(int neq, int nel, int k, const int *lm, int *IA, int *JA, int *csr_by_elem);
+
+/* ---------------- ILU(0) (Solvers_and_Preconditioners/CSR) ---------------- */
+/* ILUp_precond_setup.c:8-60 + ILUp :66-167 with p = 0: LU[nnz] in A's CSR
+ * slots (L below, U above the diagonal), D[neq] inverted pivots.  Returns 0
+ * or 1 + i for a zero pivot at row i. */
+int  orc_ilu0_factor(int neq, const double *AA, const int *JA, const int *IA, double *LU,
+                     double *D);
+/* ILUp_precond.c:4-49 : z = U^{-1} L^{-1} p (z may alias p) */
+void orc_ilu0_apply(int neq, const double *LU, const int *JA, const int *IA, const double *D,
+                    const double *p, double *z);
+void orc_ilu0_apply_f32(int neq, const float *LU, const int *JA, const int *IA, const float *D,
+                        const float *p, float *z);
+
+/* ---------------- 3D steady transport ------------------------------------ */
+/* Kuhn-split unit cube (6 tets per cell); x,y,z,utype: (n+1)^3, vertex: 24 n^3 */
+void orc_mesh3d_structured(int n, double *x, double *y, double *z, int *utype, int *vertex);
+enum { ORC_P3_DIFCONV = 0,   /* DIFCONV/DIFCONV_*.c with A = ConstApli          */
+       ORC_P3_CONST = 1 };   /* constant kappa, beta, sigma, f; u = 0 on boundary */
+typedef struct {
+    int kind;
+    double A;                 /* DIFCONV: kappa = A and the A of DIFCONV_f        */
+    double kappa, bx, by, bz, sigma, f;   /* ORC_P3_CONST                         */
+} orc_prob3d;
+double orc_prob3d_f(const orc_prob3d *pb, double x, double y, double z);
+/* Build_SUPG.c:22-196 + Element_Configuration.c:4-58 + font_assembly.c:3-57.
+ * ke[nel*16] row-major, F[neq+1] (zeroed first). */
+void orc_build_supg3d(int nel, int neq, const double *x, const double *y, const double *z,
+                      const int *utype, const int *vertex, const int *lm,
+                      const orc_prob3d *pb, double *ke, double *F);
 
 #ifdef __cplusplus
 }
