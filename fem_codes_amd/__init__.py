@@ -25,7 +25,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfemgpu.so")
 
 FG_OK, FG_ERR_HIP, FG_ERR_ARG, FG_ERR_BREAKDOWN, FG_ERR_NOCONV, FG_ERR_NOMEM = range(6)
-PREC_NONE, PREC_DIAG, PREC_JACOBI = 0, 1, 2
+PREC_NONE, PREC_DIAG, PREC_JACOBI, PREC_ILU0 = 0, 1, 2, 3
+P3_DIFCONV, P3_CONST = 0, 1
 GMRES_FP64, GMRES_FP32_IR = 0, 1
 
 
@@ -109,6 +110,18 @@ SIGNATURES = {
     "fg_supg2d_build": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "fg_op_assemble_csr": (_i, [_vp, _i, _ip, _vp]),
     "fg_op_assemble_ede": (_i, [_vp, _i, _ip, _ip, _vp]),
+    "fg_csr_pattern_k": (_i, [_i, _i, _i, _ip, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
+    "fg_op_assemble_csr_k": (_i, [_vp, _i, _i, _ip, _vp]),
+    "fg_mesh3d_create": (_i, [_vp, _i, _i, _dp, _dp, _dp, _ip, _ip, C.POINTER(_vp)]),
+    "fg_mesh3d_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "fg_mesh3d_get_lm": (_i, [_vp, _ip]),
+    "fg_mesh3d_destroy": (_i, [_vp]),
+    "fg_supg3d_build": (_i, [_vp, _vp, _vp, _vp]),
+    "fg_prec_ilu_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "fg_prec_ilu_get": (_i, [_vp, _vp, _vp]),
+    "fg_prec_ilu_schedule": (_i, [_vp, _i]),
+    "fg_rcm": (_i, [_i, _ip, _ip, _ip]),
+    "fg_csr_bandwidth": (_i, [_i, _ip, _ip, C.POINTER(_i)]),
 }
 
 
@@ -323,6 +336,23 @@ class Preconditioner:
         _check(lib.fg_prec_get_diag(self.h, D.ctypes.data, iD.ctypes.data), "get_diag")
         return D, iD
 
+    def ilu_levels(self):
+        """ILU(0): (lower levels, upper levels, kernel segments)"""
+        a, b, c = _i(), _i(), _i()
+        _check(lib.fg_prec_ilu_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "ilu_info")
+        return a.value, b.value, c.value
+
+    def ilu_schedule(self, wide_rows: int):
+        """ILU(0): levels wider than wide_rows get a launch each"""
+        _check(lib.fg_prec_ilu_schedule(self.h, int(wide_rows)), "ilu_schedule")
+
+    def ilu_factors(self):
+        """ILU(0): (LU[nnz] in the operator's CSR slots, D[neq] inverted pivots)"""
+        LU = np.zeros(max(int(self.op.nnz), 1))
+        D = np.zeros(max(self.op.neq, 1))
+        _check(lib.fg_prec_ilu_get(self.h, LU.ctypes.data, D.ctypes.data), "ilu_get")
+        return LU[:int(self.op.nnz)], D[:self.op.neq]
+
     def close(self):
         if self.h and self.op.h and self.op.ctx.h:
             lib.fg_prec_destroy(self.h)
@@ -459,9 +489,123 @@ class Mesh2D:
             pass
 
 
-def assemble_csr(op: Operator, csr_by_elem, ke: DeviceArray):
+def assemble_csr(op: Operator, csr_by_elem, ke: DeviceArray, k: int = 3):
     cbe = np.ascontiguousarray(csr_by_elem, np.int32)
-    _check(lib.fg_op_assemble_csr(op.h, cbe.size // 9, cbe, ke.ptr), "assemble_csr")
+    _check(lib.fg_op_assemble_csr_k(op.h, cbe.size // (k * k), k, cbe, ke.ptr), "assemble_csr")
+
+
+def csr_pattern_k(neq, lm, k):
+    """csr_Initialization for k dofs per element -> (nnz, IA, JA, csr_by_elem)"""
+    lm = np.ascontiguousarray(lm, np.int32)
+    nel = lm.size // k
+    nnz = C.c_longlong()
+    _check(lib.fg_csr_pattern_k(neq, nel, k, lm, None, None, None, C.byref(nnz)), "csr_pattern_k")
+    IA, JA = np.zeros(neq + 1, np.int32), np.zeros(nnz.value, np.int32)
+    cbe = np.zeros(k * k * nel, np.int32)
+    _check(lib.fg_csr_pattern_k(neq, nel, k, lm, IA.ctypes.data, JA.ctypes.data,
+                                cbe.ctypes.data, C.byref(nnz)), "csr_pattern_k")
+    return nnz.value, IA, JA, cbe
+
+
+def rcm(neq, IA, JA):
+    """reverse Cuthill-McKee (symrcm.c): perm[c] = old index of new equation c"""
+    IA = np.ascontiguousarray(IA, np.int32)
+    JA = np.ascontiguousarray(JA, np.int32)
+    perm = np.zeros(max(neq, 1), np.int32)
+    _check(lib.fg_rcm(neq, IA, JA, perm), "fg_rcm")
+    return perm[:neq]
+
+
+def csr_bandwidth(neq, IA, JA):
+    bw = _i()
+    _check(lib.fg_csr_bandwidth(neq, np.ascontiguousarray(IA, np.int32),
+                                np.ascontiguousarray(JA, np.int32), C.byref(bw)), "bandwidth")
+    return bw.value
+
+
+def renumber_lm(neq, lm, perm):
+    """lm with equations renumbered by a permutation perm[new] = old (the
+    sink neq stays the sink), as csr_Initialization.c renumbers Node[].id"""
+    inv = np.empty(neq + 1, np.int64)
+    inv[np.asarray(perm, np.int64)] = np.arange(neq)
+    inv[neq] = neq
+    return inv[np.asarray(lm, np.int64)].astype(np.int32)
+
+
+class Supg3dProblem(C.Structure):
+    """fg_supg3d_problem: DIFCONV (kappa = A, beta = (1,1,1), sigma = 0,
+    manufactured f) or constant coefficients."""
+    _fields_ = [("kind", C.c_int), ("A", C.c_double), ("kappa", C.c_double),
+                ("bx", C.c_double), ("by", C.c_double), ("bz", C.c_double),
+                ("sigma", C.c_double), ("f", C.c_double)]
+
+
+def structured_mesh3d(n: int):
+    """Unit cube, n^3 cells, each split into the 6 tetrahedra around its main
+    diagonal (conforming Kuhn split).  Node (i,j,k) -> i + (n+1)(j + (n+1)k);
+    boundary nodes uType 0 (prescribed), inner nodes uType 1 (unknown).
+    Elements cell by cell (i fastest), the axis permutations (0,1,2), (0,2,1),
+    (1,0,2), (1,2,0), (2,0,1), (2,1,0) in that order, the last two vertices
+    swapped for the odd ones so every volume is positive.
+    -> (x, y, z, utype, vertex[nel*4])"""
+    N = n + 1
+    g = np.arange(N, dtype=np.float64) / n
+    z, y, x = [a.ravel() for a in np.meshgrid(g, g, g, indexing="ij")]
+    ii = np.arange(N)
+    kk, jj, i3 = [a.ravel() for a in np.meshgrid(ii, ii, ii, indexing="ij")]
+    inner = (i3 > 0) & (i3 < n) & (jj > 0) & (jj < n) & (kk > 0) & (kk < n)
+    utype = inner.astype(np.int32)
+    ck, cj, ci = [a.ravel().astype(np.int64) for a in
+                  np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")]
+    v0 = ci + N * (cj + N * ck)
+    stride = np.array([1, N, N * N], np.int64)
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    V = np.empty((n ** 3, 6, 4), np.int64)
+    for p, (a, b, c) in enumerate(perms):
+        w1 = v0 + stride[a]
+        w2 = w1 + stride[b]
+        w3 = w2 + stride[c]
+        if p in (1, 2, 5):
+            w2, w3 = w3, w2
+        V[:, p, 0], V[:, p, 1], V[:, p, 2], V[:, p, 3] = v0, w1, w2, w3
+    return x.copy(), y.copy(), z.copy(), utype, V.reshape(-1).astype(np.int32)
+
+
+class Mesh3D:
+    """fg_mesh3d: device copy of a P1 tetrahedral mesh + 3D SUPG assembly."""
+
+    def __init__(self, ctx: Context, x, y, z, utype, vertex):
+        x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+        utype = np.ascontiguousarray(utype, np.int32)
+        vertex = np.ascontiguousarray(vertex, np.int32)
+        h = _vp()
+        _check(lib.fg_mesh3d_create(ctx.h, x.size, vertex.size // 4, x, y, z, utype, vertex,
+                                    C.byref(h)), "fg_mesh3d_create")
+        self.ctx, self.h = ctx, h
+        neq, nel, nn = _i(), _i(), _i()
+        lib.fg_mesh3d_info(h, C.byref(neq), C.byref(nel), C.byref(nn))
+        self.neq, self.nel, self.nnodes = neq.value, nel.value, nn.value
+
+    def lm(self):
+        out = np.zeros(4 * self.nel, np.int32)
+        _check(lib.fg_mesh3d_get_lm(self.h, out), "get_lm")
+        return out
+
+    def build_supg(self, prob: Supg3dProblem, ke: DeviceArray, F: DeviceArray | None):
+        """Build_SUPG (3D) on the device: ke (nel*16), F (neq+1)."""
+        _check(lib.fg_supg3d_build(self.h, C.byref(prob), ke.ptr,
+                                   F.ptr if F is not None else None), "fg_supg3d_build")
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib.fg_mesh3d_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def assemble_ede(op: Operator, order, edge_by_elem, ke: DeviceArray):

@@ -469,6 +469,8 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
     FG_ARG(o->precision == FG_GMRES_FP64 || o->precision == FG_GMRES_FP32_IR, "precision");
     FG_ARG(o->engine >= FG_ENGINE_AUTO && o->engine <= FG_ENGINE_FUSED, "engine");
     if (pc && pc->kind == FG_PREC_NONE) pc = nullptr;
+    FG_ARG(!pc || (pc->op->neq == op->neq && pc->op->ctx == c),
+           "preconditioner built on an operator of another size or context");
     FG_HIP(hipSetDevice(c->device));
     const int n = op->neq, kmax = o->kmax;
     const bool ir = o->precision == FG_GMRES_FP32_IR;

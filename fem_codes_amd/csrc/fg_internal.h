@@ -111,9 +111,19 @@ struct fg_op {
     std::vector<int> h_IA, h_JA;
 };
 
+struct fg_ilu;
+int fg_ilu_create(fg_op *op, fg_ilu **out);
+void fg_ilu_destroy(fg_ilu *u);
+int fg_ilu_factor(fg_ilu *u, fg_op *op);
+int fg_ilu_apply(fg_ilu *u, fg_ctx *c, int fp32, const void *p, void *z, const int *skip);
+int fg_ilu_get(const fg_ilu *u, fg_ctx *c, double *LU, double *D);
+void fg_ilu_levels(const fg_ilu *u, int *nlev_lower, int *nlev_upper, int *nseg);
+void fg_ilu_set_wide(fg_ilu *u, int wide_rows);
+
 struct fg_prec {
     fg_op *op = nullptr;
     int kind = FG_PREC_NONE;
+    fg_ilu *ilu = nullptr;       // FG_PREC_ILU0
     double *d_diag = nullptr, *d_invdiag = nullptr;
     float *d_invdiag32 = nullptr;
     double *d_invde = nullptr;   // [nel*3] element-local Jacobi

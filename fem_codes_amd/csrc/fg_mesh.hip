@@ -98,16 +98,23 @@ void ede_order(const int *l, int *o)
 extern "C" int fg_csr_pattern(int neq, int nel, const int *lm, int *IA, int *JA,
                               int *csr_by_elem, long long *nnz_out)
 {
-    FG_ARG(lm && nnz_out, "lm/nnz");
+    return fg_csr_pattern_k(neq, nel, 3, lm, IA, JA, csr_by_elem, nnz_out);
+}
+
+extern "C" int fg_csr_pattern_k(int neq, int nel, int k, const int *lm, int *IA, int *JA,
+                                int *csr_by_elem, long long *nnz_out)
+{
+    FG_ARG(lm && nnz_out && k >= 1 && k <= 16, "lm/nnz/k");
+    const int kk = k * k;
     std::vector<int> pr, pc;
-    pr.reserve((size_t)nel * 9);
-    pc.reserve((size_t)nel * 9);
+    pr.reserve((size_t)nel * kk);
+    pc.reserve((size_t)nel * kk);
     for (long e = 0; e < nel; e++)
-        for (int a = 0; a < 3; a++) {
-            int I = lm[3 * e + a];
+        for (int a = 0; a < k; a++) {
+            int I = lm[k * e + a];
             if (I == neq) continue;
-            for (int b = 0; b < 3; b++) {
-                int J = lm[3 * e + b];
+            for (int b = 0; b < k; b++) {
+                int J = lm[k * e + b];
                 if (J != neq) { pr.push_back(I); pc.push_back(J); }
             }
         }
@@ -122,10 +129,10 @@ extern "C" int fg_csr_pattern(int neq, int nel, const int *lm, int *IA, int *JA,
     memcpy(JA, cols.data(), sizeof(int) * cols.size());
     if (csr_by_elem)
         for (long e = 0; e < nel; e++)
-            for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++) {
-                    int I = lm[3 * e + a], J = lm[3 * e + b];
-                    csr_by_elem[9 * e + 3 * a + b] =
+            for (int a = 0; a < k; a++)
+                for (int b = 0; b < k; b++) {
+                    int I = lm[k * e + a], J = lm[k * e + b];
+                    csr_by_elem[kk * e + k * a + b] =
                         (I != neq && J != neq) ? find_col(cols, off[I], off[I + 1], J) : (int)nnz;
                 }
     return FG_OK;
@@ -453,18 +460,25 @@ static int upload_targets(int nt, const std::vector<int> &tg, const std::vector<
 
 extern "C" int fg_op_assemble_csr(fg_op *op, int nel, const int *cbe, const double *ke)
 {
-    FG_ARG(op && op->kind == FG_OP_CSR && cbe && ke, "CSR operator / map / ke");
+    return fg_op_assemble_csr_k(op, nel, 3, cbe, ke);
+}
+
+extern "C" int fg_op_assemble_csr_k(fg_op *op, int nel, int k, const int *cbe, const double *ke)
+{
+    FG_ARG(op && op->kind == FG_OP_CSR && cbe && ke && k >= 1 && k <= 16,
+           "CSR operator / map / ke / k");
     fg_ctx *c = op->ctx;
     int nnz = (int)op->nnz;
+    const int kk = k * k;
     std::vector<int> tg;
     std::vector<long long> src;
-    tg.reserve((size_t)nel * 9);
-    src.reserve((size_t)nel * 9);
+    tg.reserve((size_t)nel * kk);
+    src.reserve((size_t)nel * kk);
     for (long e = 0; e < nel; e++)
-        for (int q = 0; q < 9; q++) {
-            int s = cbe[9 * e + q];
+        for (int q = 0; q < kk; q++) {
+            int s = cbe[(long)kk * e + q];
             FG_ARG(s >= 0 && s <= nnz, "csr_by_elem entry out of range");
-            if (s < nnz) { tg.push_back(s); src.push_back((9LL * e + q) << 1); }
+            if (s < nnz) { tg.push_back(s); src.push_back(((long long)kk * e + q) << 1); }
         }
     int *d_off = nullptr;
     long long *d_src = nullptr;

@@ -40,12 +40,22 @@ __global__ void k_diag_apply(int neq, const T *__restrict__ iD, const T *p, T *z
 extern "C" int fg_prec_create(fg_op *op, int kind, fg_prec **out)
 {
     FG_ARG(op && out, "op/out");
-    FG_ARG(kind == FG_PREC_NONE || kind == FG_PREC_DIAG || kind == FG_PREC_JACOBI, "kind");
+    FG_ARG(kind == FG_PREC_NONE || kind == FG_PREC_DIAG || kind == FG_PREC_JACOBI ||
+               kind == FG_PREC_ILU0, "kind");
     FG_ARG(kind != FG_PREC_JACOBI || op->kind == FG_OP_EBE, "Jacobi needs an EBE operator");
+    FG_ARG(kind != FG_PREC_ILU0 || op->kind == FG_OP_CSR, "ILU(0) needs a CSR operator");
+    FG_HIP(hipSetDevice(op->ctx->device));
     fg_prec *pc = new fg_prec();
     pc->op = op;
     pc->kind = kind;
     size_t n = (size_t)op->neq + 1;
+    if (kind == FG_PREC_ILU0) {
+        int rc = fg_ilu_create(op, &pc->ilu);
+        if (rc != FG_OK) {
+            delete pc;
+            return rc;
+        }
+    }
     if (kind == FG_PREC_DIAG) {
         FG_HIP(hipMalloc(&pc->d_diag, n * sizeof(double)));
         FG_HIP(hipMalloc(&pc->d_invdiag, n * sizeof(double)));
@@ -69,6 +79,12 @@ extern "C" int fg_prec_setup(fg_prec *pc, double *F)
     fg_ctx *c = op->ctx;
     int neq = op->neq;
     if (pc->kind == FG_PREC_NONE) return FG_OK;
+    if (pc->kind == FG_PREC_ILU0) {
+        // ILUp_precond_setup.c:53-56: factorise, then F <- (LU)^{-1} F
+        FG_CHECK(fg_ilu_factor(pc->ilu, op));
+        if (F) FG_CHECK(fg_ilu_apply(pc->ilu, c, 0, F, F, nullptr));
+        return FG_OK;
+    }
     if (pc->kind == FG_PREC_DIAG) {
         if (op->kind == FG_OP_CSR) FG_CHECK(fg_csr_diag(op, pc->d_diag));
         else FG_CHECK(fg_ebe_diag(op, pc->d_diag));
@@ -109,6 +125,7 @@ int fg_prec_apply_any(fg_prec *pc, int fp32, const void *p, void *z, const int *
                       (const double *)p, (double *)z, skip);
         return FG_OK;
     }
+    if (pc->kind == FG_PREC_ILU0) return fg_ilu_apply(pc->ilu, c, fp32, p, z, skip);
     // element Jacobi reads p[g] and writes z[g] in the same thread: in place is safe
     return fg_ebe_jacobi_apply(op, fp32, fp32 ? (const void *)pc->d_invde32 : pc->d_invde, p, z,
                                skip);
@@ -162,6 +179,28 @@ extern "C" int fg_prec_destroy(fg_prec *pc)
     hipStreamSynchronize(pc->op->ctx->stream);
     hipFree(pc->d_diag); hipFree(pc->d_invdiag); hipFree(pc->d_invdiag32);
     hipFree(pc->d_invde); hipFree(pc->d_invde32); hipFree(pc->d_tmp64); hipFree(pc->d_tmp32);
+    fg_ilu_destroy(pc->ilu);
     delete pc;
     return FG_OK;
+}
+
+extern "C" int fg_prec_ilu_info(fg_prec *pc, int *nlev_lower, int *nlev_upper, int *nsegments)
+{
+    FG_ARG(pc && pc->kind == FG_PREC_ILU0, "ILU(0) preconditioner");
+    fg_ilu_levels(pc->ilu, nlev_lower, nlev_upper, nsegments);
+    return FG_OK;
+}
+
+extern "C" int fg_prec_ilu_schedule(fg_prec *pc, int wide_rows)
+{
+    FG_ARG(pc && pc->kind == FG_PREC_ILU0, "ILU(0) preconditioner");
+    FG_ARG(wide_rows >= 0, "wide_rows >= 0");
+    fg_ilu_set_wide(pc->ilu, wide_rows);
+    return FG_OK;
+}
+
+extern "C" int fg_prec_ilu_get(fg_prec *pc, double *LU_host, double *D_host)
+{
+    FG_ARG(pc && pc->kind == FG_PREC_ILU0, "ILU(0) preconditioner");
+    return fg_ilu_get(pc->ilu, pc->op->ctx, LU_host, D_host);
 }

@@ -133,6 +133,25 @@ int fg_prec_set_invdiag(fg_prec *pc, const double *invDiag_host);
 int fg_prec_get_diag(fg_prec *pc, double *Diag_host, double *invDiag_host);
 int fg_prec_destroy(fg_prec *pc);
 
+/* ILU(0) of a CSR operator: FG_PREC_ILU0 replaces ILUp_precond_setup.c:8-60
+ * ("ILU0": p = 0, no fill-in) and ILUp_precond.c:4-49.  Setup factorises
+ * the operator's current values (zero pivot: FG_ERR_BREAKDOWN, where the
+ * reference prints and exits, ILUp_precond_setup.c:154-163) and left-
+ * preconditions F; apply solves (LU) z = p.  The factors and solves are
+ * level-scheduled on the device; the FP32 apply uses an FP32 copy of the
+ * factors.  The preconditioner may serve a solve whose matvec operator is
+ * another storage scheme of the same matrix (EBE matvec + CSR ILU(0)). */
+#define FG_PREC_ILU0    3
+/* dependency levels of the forward / backward sweeps and kernel segments */
+int fg_prec_ilu_info(fg_prec *pc, int *nlev_lower, int *nlev_upper, int *nsegments);
+/* launch schedule (no reference counterpart; a tuning knob): a level with
+ * more than wide_rows rows gets a launch of its own, runs of narrower levels
+ * share one workgroup (default 8192).  Results are identical either way. */
+int fg_prec_ilu_schedule(fg_prec *pc, int wide_rows);
+/* factors to host: LU[nnz] in the operator's CSR slots (diagonal slot
+ * unused), D[neq] inverted pivots */
+int fg_prec_ilu_get(fg_prec *pc, double *LU_host, double *D_host);
+
 /* ---- mesh structures (host, once per mesh) -------------------------------- */
 /* Node numbering Preprocess.c:73-81 (id = neq++ for Type>=1, else -1) and
  * Fill_LM.c:3-24 (lm[e*nnoel+j] = id of vertex j, or neq).  Returns neq. */
@@ -184,6 +203,50 @@ int fg_supg2d_build(fg_mesh2d *m, const fg_supg2d_problem *pb, const double *u_d
 int fg_op_assemble_csr(fg_op *csr, int nel, const int *csr_by_elem, const double *ke_dev);
 int fg_op_assemble_ede(fg_op *ede, int nel, const int *order, const int *edge_by_elem,
                        const double *ke_dev);
+
+/* ---- reordering (host, once per mesh) -------------------------------------- */
+/* Reordering/symrcm.c:4-242 REORDERING_SYMRCM: reverse Cuthill-McKee
+ * numbering of a CSR pattern with the reference's tie-breaking.
+ * perm[c] = old index of the equation numbered c (the row order
+ * reordering.c MATRIX_ROW_permutation applies; renumber ids by the inverse,
+ * as csr_Initialization.c does for Node[].id). */
+int fg_rcm(int neq, const int *IA, const int *JA, int *perm);
+/* reordering.c MATRIX_bandwidth: max |i - j| over the pattern */
+int fg_csr_bandwidth(int neq, const int *IA, const int *JA, int *bandwidth);
+
+/* ---- 3D steady transport (01_SS_Transport_Equation/3D) -------------------- */
+/* csr_Initialization.c for k dofs per element (2D: k = 3, :5-218 of the 2D
+ * file; 3D: k = 4, 3D csr_Initialization.c:80-256, reordering NOT):
+ * csr_by_elem[nel*k*k], slot nnz = sink.  Two calls as fg_csr_pattern. */
+int fg_csr_pattern_k(int neq, int nel, int k, const int *lm, int *IA, int *JA,
+                     int *csr_by_elem, long long *nnz);
+/* csr_assembly.c for k x k element matrices (3D csr_assembly.c:3-30) */
+int fg_op_assemble_csr_k(fg_op *csr, int nel, int k, const int *csr_by_elem,
+                         const double *ke_dev);
+
+/* problem callbacks of the 3D driver:
+ *   FG_P3_DIFCONV  3D/DIFCONV/DIFCONV_*.c: kappa = A, beta = (1,1,1),
+ *                  sigma = 0, manufactured f, u = 0 on the boundary
+ *   FG_P3_CONST    constant kappa, beta, sigma, f; u = 0 on the boundary */
+#define FG_P3_DIFCONV 0
+#define FG_P3_CONST   1
+typedef struct {
+    int kind;
+    double A;                               /* DIFCONV: ConstApli          */
+    double kappa, bx, by, bz, sigma, f;     /* FG_P3_CONST                 */
+} fg_supg3d_problem;
+
+typedef struct fg_mesh3d fg_mesh3d;   /* device copy of a P1 tetrahedral mesh */
+/* x,y,z,utype: nnodes (uType 1 = unknown, Fill_ID.c) ; vertex: nel*4 (host) */
+int fg_mesh3d_create(fg_ctx *ctx, int nnodes, int nel, const double *x, const double *y,
+                     const double *z, const int *utype, const int *vertex, fg_mesh3d **out);
+int fg_mesh3d_info(const fg_mesh3d *m, int *neq, int *nel, int *nnodes);
+int fg_mesh3d_get_lm(const fg_mesh3d *m, int *lm_host);   /* nel*4, Fill_LM.c */
+int fg_mesh3d_destroy(fg_mesh3d *m);
+/* Build_SUPG.c:22-196 + Element_Configuration.c:4-58 + font_assembly.c:3-57:
+ * ke_dev nel*16 row-major (ebe_assembly.c:11-15), F_dev neq+1 in element
+ * order. */
+int fg_supg3d_build(fg_mesh3d *m, const fg_supg3d_problem *pb, double *ke_dev, double *F_dev);
 
 /* ---- restarted GMRES -------------------------------------------------------*/
 #define FG_GMRES_FP64     0   /* plain FP64: the drop-in for pgmres            */

@@ -141,6 +141,12 @@ class Oracle:
         L.orc_prob3d_f.argtypes = [C.POINTER(OrcProb3d), _d, _d, _d]
         L.orc_build_supg3d.argtypes = [_i, _i, _dp, _dp, _dp, _ip, _ip, _ip,
                                        C.POINTER(OrcProb3d), _dp, _dp]
+        L.orc_symrcm.argtypes = [_i, _ip, _ip, _ip]
+
+    def symrcm(self, neq, IA, JA):
+        P = np.zeros(max(neq, 1), np.int32)
+        self.lib.orc_symrcm(neq, IA, JA, P)
+        return P[:neq]
 
     # ---- 3D / CSR(k) / ILU(0) ----
     def mesh3d(self, n):

@@ -1027,3 +1027,190 @@ void orc_build_supg3d(int nel, int neq, const double *x, const double *y, const 
         F[neq] = 0.0;
     }
 }
+
+/* ======================================================================== */
+/* reverse Cuthill-McKee (Reordering/symrcm.c)                               */
+/* ======================================================================== */
+
+/* a BFS node: equation, degree, distance to the root (symrcm.h:8-16) */
+typedef struct { int id, deg, dist; } rcm_node;
+
+static void rcm_heap_down(rcm_node *A, int j, int size)      /* H_heapify_min */
+{
+    for (;;) {
+        int l = 2 * j + 1, r = 2 * j + 2, s = j;
+        if (l < size && A[l].deg < A[j].deg) s = l;
+        if (r < size && A[r].deg < A[s].deg) s = r;
+        if (s == j) break;
+        rcm_node t = A[j]; A[j] = A[s]; A[s] = t;
+        j = s;
+    }
+}
+
+static void rcm_heap_push(rcm_node *H, int *h, rcm_node o)    /* H_insert */
+{
+    int i = (*h)++;
+    H[i] = o;
+    while (i > 0) {
+        int p = (i - 1) >> 1;
+        if (!(H[i].deg < H[p].deg)) break;
+        rcm_node t = H[i]; H[i] = H[p]; H[p] = t;
+        i = p;
+    }
+}
+
+static rcm_node rcm_heap_pop(rcm_node *H, int *h)             /* H_remove_min */
+{
+    rcm_node r = H[0];
+    H[0] = H[--(*h)];
+    rcm_heap_down(H, 0, *h);
+    return r;
+}
+
+/* the neighbours of i in ascending order: column i of A (ridx) merged with
+ * column i of A^T (ridx2), a shared index visited once (symrcm.c:117-177) */
+typedef void (*rcm_visit_fn)(void *st, int r);
+static void rcm_neighbours(int i, const int *cidx, const int *ridx, const int *cidx2,
+                           const int *ridx2, rcm_visit_fn fn, void *st)
+{
+    int j1 = cidx[i], j2 = cidx2[i];
+    while (j1 < cidx[i + 1] || j2 < cidx2[i + 1]) {
+        if (j1 == cidx[i + 1]) fn(st, ridx2[j2++]);
+        else if (j2 == cidx2[i + 1]) fn(st, ridx[j1++]);
+        else {
+            int r1 = ridx[j1], r2 = ridx2[j2];
+            if (r1 <= r2) {
+                fn(st, r1);
+                j1++;
+                if (r1 == r2) j2++;
+            } else {
+                fn(st, r2);
+                j2++;
+            }
+        }
+    }
+}
+
+typedef struct {
+    rcm_node *Q;  int N, *qt;       /* queue (find_starting_node) or heap (main BFS) */
+    rcm_node *H;  int *h;
+    int *visit;   const int *D;  int dist;
+} rcm_state;
+
+static void rcm_enqueue_visit(void *s_, int r)
+{
+    rcm_state *s = (rcm_state *)s_;
+    if (s->visit[r]) return;
+    rcm_node w = {r, s->D[r], s->dist};
+    s->Q[*s->qt] = w;
+    *s->qt = (*s->qt + 1) % (s->N + 1);
+    s->visit[r] = 1;
+}
+
+static void rcm_heap_visit(void *s_, int r)
+{
+    rcm_state *s = (rcm_state *)s_;
+    if (s->visit[r]) return;
+    rcm_node w = {r, s->D[r], s->dist};
+    rcm_heap_push(s->H, s->h, w);
+    s->visit[r] = 1;
+}
+
+/* find_starting_node (symrcm.c:357-493): repeated BFS from the farthest node
+ * found, preferring larger distance, then (other node) larger degree */
+static int rcm_start(int N, const int *cidx, const int *ridx, const int *cidx2, const int *ridx2,
+                     const int *D, int start)
+{
+    rcm_node *Q = (rcm_node *)calloc((size_t)N + 1, sizeof(rcm_node));
+    int *visit = (int *)calloc((size_t)N, sizeof(int));
+    int qh = 0, qt = 0, max_dist = 0;
+    rcm_node x = {start, D[start], 0};
+    Q[qt] = x; qt = (qt + 1) % (N + 1);
+    visit[start] = 1;
+    for (;;) {
+        while (qh != qt) {
+            rcm_node v = Q[qh]; qh = (qh + 1) % (N + 1);
+            if (v.dist > x.dist || (v.id != x.id && v.deg > x.deg)) x = v;
+            rcm_state st = {Q, N, &qt, NULL, NULL, visit, D, v.dist + 1};
+            rcm_neighbours(v.id, cidx, ridx, cidx2, ridx2, rcm_enqueue_visit, &st);
+        }
+        if (max_dist < x.dist) {
+            max_dist = x.dist;
+            memset(visit, 0, sizeof(int) * (size_t)N);
+            visit[x.id] = 1;
+            x.dist = 0;
+            qt = qh = 0;
+            Q[qt] = x; qt = (qt + 1) % (N + 1);
+        } else
+            break;
+    }
+    free(Q);
+    free(visit);
+    return x.id;
+}
+
+void orc_symrcm(int N, const int *ia, const int *ja, int *P)
+{
+    const int *cidx = ia, *ridx = ja;   /* the pattern read as CCS (symrcm.c:23-27) */
+    const int nz = ia[N];
+    int *cidx2 = (int *)calloc((size_t)N + 1, sizeof(int));
+    int *ridx2 = (int *)calloc((size_t)(nz > 0 ? nz : 1), sizeof(int));
+    int *w = (int *)calloc((size_t)N + 1, sizeof(int));
+    /* transpose (symrcm.c:277-308) */
+    for (int q = 0; q < nz; q++) w[ridx[q]]++;
+    int acc = 0;
+    for (int i = 0; i < N; i++) { cidx2[i] = acc; acc += w[i]; w[i] = cidx2[i]; }
+    cidx2[N] = acc;
+    for (int j = 0; j < N; j++)
+        for (int k = cidx[j]; k < cidx[j + 1]; k++) ridx2[w[ridx[k]]++] = j;
+    free(w);
+    /* calc_degrees (symrcm.c:310-355) */
+    int *D = (int *)calloc((size_t)(N > 0 ? N : 1), sizeof(int));
+    int max_deg = 0;
+    for (int j = 0; j < N; j++)
+        for (int q = cidx[j]; q < cidx[j + 1]; q++) {
+            int k = ridx[q];
+            if (++D[k] > max_deg) max_deg = D[k];
+            if (k != j) {
+                int found = 0;
+                for (int l = cidx[k]; l < cidx[k + 1]; l++) {
+                    if (ridx[l] == j) { found = 1; break; }
+                    if (ridx[l] > j) break;
+                }
+                if (!found && ++D[j] > max_deg) max_deg = D[j];
+            }
+        }
+    if (max_deg == 0) {
+        for (int i = 0; i < N; i++) P[i] = i;
+    } else {
+        rcm_node *S = (rcm_node *)calloc((size_t)max_deg, sizeof(rcm_node));
+        rcm_node *Q = (rcm_node *)calloc((size_t)N + 1, sizeof(rcm_node));
+        int *visit = (int *)calloc((size_t)N, sizeof(int));
+        int c = -1, qh = 0, qt = 0;
+        do {
+            int i = 0;
+            while (i < N && visit[i]) i++;
+            rcm_node v = {0, 0, 0};
+            v.id = rcm_start(N, cidx, ridx, cidx2, ridx2, D, i);
+            v.deg = D[v.id];
+            v.dist = 0;
+            visit[v.id] = 1;
+            Q[qt] = v; qt = (qt + 1) % (N + 1);
+            while (qh != qt) {
+                v = Q[qh]; qh = (qh + 1) % (N + 1);
+                P[++c] = v.id;
+                int s = 0;
+                rcm_state st = {NULL, N, NULL, S, &s, visit, D, v.dist + 1};
+                rcm_neighbours(v.id, cidx, ridx, cidx2, ridx2, rcm_heap_visit, &st);
+                while (s > 0) {
+                    rcm_node u = rcm_heap_pop(S, &s);
+                    Q[qt] = u; qt = (qt + 1) % (N + 1);
+                }
+            }
+        } while (c + 1 < N);
+        free(S); free(Q); free(visit);
+        /* reverse (symrcm.c:232-235) */
+        for (int i = 0, j = N - 1; i <= N / 2 - 1; i++, j--) { int t = P[i]; P[i] = P[j]; P[j] = t; }
+    }
+    free(D); free(cidx2); free(ridx2);
+}

@@ -174,6 +174,13 @@ void orc_ilu0_apply(int neq, const double *LU, const int *JA, const int *IA, con
 void orc_ilu0_apply_f32(int neq, const float *LU, const int *JA, const int *IA, const float *D,
                         const float *p, float *z);
 
+/* ---------------- reordering --------------------------------------------- */
+/* Reordering/symrcm.c:4-242 REORDERING_SYMRCM: reverse Cuthill-McKee of a
+ * CSR pattern (read as CCS, as the reference does).  P[c] = old index of
+ * the equation numbered c; the reference then permutes rows and columns by
+ * P (reordering.c MATRIX_ROW/COL_permutation). */
+void orc_symrcm(int neq, const int *IA, const int *JA, int *P);
+
 /* ---------------- 3D steady transport ------------------------------------ */
 /* Kuhn-split unit cube (6 tets per cell); x,y,z,utype: (n+1)^3, vertex: 24 n^3 */
 void orc_mesh3d_structured(int n, double *x, double *y, double *z, int *utype, int *vertex);

@@ -240,3 +240,18 @@ int ref_pgmres(int kmax, int lmax, double tol, const double *B, double *X)
 
 /* BLAS-1 passthroughs */
 double ref_ddot(int n, double *x, double *y) { return ddot(n, x, y); }
+
+/* Reordering/symrcm.c REORDERING_SYMRCM on a caller CSR pattern: perm
+ * (neq) receives P, IA/JA are permuted in place (MATRIX_ROW/COL_permutation,
+ * reordering.c:238-308) and pT (nnz) receives the slot permutation */
+void REORDERING_SYMRCM(ParametersType *, int *, int *, int *, int *);
+void ref_symrcm(int neq, int nnz, int *IA, int *JA, int *perm, int *pT)
+{
+    ParametersType P;
+    memset(&P, 0, sizeof P);
+    P.neq = neq;
+    P.nnzero = nnz;
+    for (int i = 0; i < neq; i++) perm[i] = i;
+    for (int q = 0; q < nnz; q++) pT[q] = q;
+    REORDERING_SYMRCM(&P, JA, IA, perm, pT);
+}
