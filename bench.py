@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("matvec HBM GB/s (% of peak) + time-to-1e-8 solve, FP32-IR vs FP64 GMRES")
 PEAK_GBS = 8000.0
 ENGINES = {"auto": 0, "kernels": 1, "fused": 2}
+ORTHS = {"mgs": 0, "cgs": 1}
 
 
 def log(*a):
@@ -69,7 +70,8 @@ def setup_system(F, ctx, n, prob, schemes=("EBE", "EDE", "CSR")):
                 ops=ops)
 
 
-def time_solve(F, ctx, sysd, precision, engine, kmax=30, tol=1e-8, lmax=100000, prec="diag"):
+def time_solve(F, ctx, sysd, precision, engine, kmax=30, tol=1e-8, lmax=100000, prec="diag",
+               orth=0):
     op = sysd["ops"].get("EBE_default", sysd["ops"]["EBE"])
     pc = F.Preconditioner(op, F.PREC_JACOBI if prec == "jacobi" else F.PREC_DIAG)
     B = ctx.empty(sysd["neq"] + 1)
@@ -79,7 +81,7 @@ def time_solve(F, ctx, sysd, precision, engine, kmax=30, tol=1e-8, lmax=100000, 
     ctx.sync()
     t0 = time.perf_counter()
     info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=kmax, lmax=lmax, tol=tol, precision=precision,
-                         engine=engine)
+                         engine=engine, orth=orth if precision == F.GMRES_FP32_IR else 0)
     ctx.sync()
     wall = time.perf_counter() - t0
     return info, wall, X
@@ -160,6 +162,8 @@ def main():
                     help="GMRES engine for the time-to-solve runs")
     ap.add_argument("--prec", default="diag", choices=("diag", "jacobi"),
                     help="preconditioner of the time-to-solve runs")
+    ap.add_argument("--orth", default="cgs", choices=sorted(ORTHS),
+                    help="orthogonalisation of the FP32-IR inner cycle (FP64 is always MGS)")
     ap.add_argument("--matvec-only", action="store_true",
                     help="only the EBE FP64 product (for profiling runs)")
     args = ap.parse_args()
@@ -235,11 +239,11 @@ def main():
         sysd["mesh"].build_supg(const, sysd["ke"], sysd["F"])
         sysd["ops"]["EBE"].set_values(sysd["ke"])
         del sysd["ops"]["EBE_default"]
-        res = {"engine": args.engine, "precond": args.prec}
+        res = {"engine": args.engine, "precond": args.prec, "fp32_ir_orth": args.orth}
         X64 = X32 = None
         for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
             info, wall_s, X = time_solve(F, ctx, sysd, prec, ENGINES[args.engine],
-                                         prec=args.prec)
+                                         prec=args.prec, orth=ORTHS[args.orth])
             res[tag] = dict(seconds=wall_s, gpu_ms=info.ms, steps=info.steps, cycles=info.cycles,
                             true_rel_res=info.true_res / (info.eps / 1e-8), status=info.status)
             if tag == "fp64":
@@ -256,7 +260,7 @@ def main():
         sres = {"engine": args.engine}
         for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
             info, wall_s, X = time_solve(F, ctx, small, prec, ENGINES[args.engine],
-                                         prec=args.prec)
+                                         prec=args.prec, orth=ORTHS[args.orth])
             sres[tag] = dict(seconds=wall_s, steps=info.steps, cycles=info.cycles,
                              X=X.get())
         if not args.no_cpu:

@@ -47,10 +47,12 @@ class FemGpuError(RuntimeError):
 class GmresOpts(C.Structure):
     _fields_ = [("kmax", C.c_int), ("lmax", C.c_int), ("tol", C.c_double),
                 ("precision", C.c_int), ("inner_tol", C.c_double), ("hist_cap", C.c_int),
-                ("rho_hist", C.c_void_p), ("e0_hist", C.c_void_p), ("engine", C.c_int)]
+                ("rho_hist", C.c_void_p), ("e0_hist", C.c_void_p), ("engine", C.c_int),
+                ("orth", C.c_int)]
 
 
 ENGINE_AUTO, ENGINE_KERNELS, ENGINE_FUSED = 0, 1, 2
+ORTH_MGS, ORTH_CGS = 0, 1
 
 
 class GmresInfo(C.Structure):
@@ -120,6 +122,7 @@ SIGNATURES = {
     "fg_prec_ilu_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
     "fg_prec_ilu_get": (_i, [_vp, _vp, _vp]),
     "fg_prec_ilu_schedule": (_i, [_vp, _i]),
+    "fg_prec_ilu_sweeps": (_i, [_vp, _i]),
     "fg_rcm": (_i, [_i, _ip, _ip, _ip]),
     "fg_csr_bandwidth": (_i, [_i, _ip, _ip, C.POINTER(_i)]),
 }
@@ -346,6 +349,11 @@ class Preconditioner:
         """ILU(0): levels wider than wide_rows get a launch each"""
         _check(lib.fg_prec_ilu_schedule(self.h, int(wide_rows)), "ilu_schedule")
 
+    def ilu_sweeps(self, sweeps: int):
+        """ILU(0): 0 = exact triangular solves, s > 0 = s Jacobi sweeps each
+        (set before setup)"""
+        _check(lib.fg_prec_ilu_sweeps(self.h, int(sweeps)), "ilu_sweeps")
+
     def ilu_factors(self):
         """ILU(0): (LU[nnz] in the operator's CSR slots, D[neq] inverted pivots)"""
         LU = np.zeros(max(int(self.op.nnz), 1))
@@ -367,17 +375,18 @@ class Preconditioner:
 
 def gmres(ctx: Context, op: Operator, prec: Preconditioner | None, B: DeviceArray,
           X: DeviceArray, kmax=30, lmax=1000, tol=1e-8, precision=GMRES_FP64,
-          inner_tol=0.0, hist_cap=0, e0_cap=0, engine=ENGINE_KERNELS):
+          inner_tol=0.0, hist_cap=0, e0_cap=0, engine=ENGINE_KERNELS, orth=ORTH_MGS):
     """pgmres (Solvers_and_Preconditioners/pgmres.c:4-211) on the device.
 
     engine: ENGINE_KERNELS (one kernel per vector operation), ENGINE_FUSED
     (the whole solve in one persistent kernel) or ENGINE_AUTO.
+    orth: ORTH_MGS or ORTH_CGS for the FP32-IR inner cycle.
     Returns (info, rho_hist, e0_hist)."""
     rho = np.zeros(max(1, hist_cap))
     e0 = np.zeros(max(1, e0_cap if e0_cap else lmax + 1))
     o = GmresOpts(kmax, lmax, tol, precision, inner_tol, hist_cap,
                   rho.ctypes.data if hist_cap else None,
-                  e0.ctypes.data if e0_cap else None, engine)
+                  e0.ctypes.data if e0_cap else None, engine, orth)
     info = GmresInfo()
     rc = lib.fg_gmres(ctx.h, op.h, prec.h if prec is not None else None, B.ptr, X.ptr,
                       C.byref(o), C.byref(info))

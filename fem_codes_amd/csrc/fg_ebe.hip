@@ -594,6 +594,28 @@ int fg_ebe_jacobi_setup(fg_op *op, double **invde, float **invde32)
     return FG_OK;
 }
 
+// the element Jacobi operator is diagonal: z_g = p_g * sum_e 1/A^e_gg.  The
+// FP32 inner cycle of the mixed-precision solver applies it as that diagonal
+// (summed in FP64, rounded once), fused into its first vector pass.
+__global__ void k_ebe_jacobi_sum(int neq, const int *__restrict__ ioff,
+                                 const double *__restrict__ invde, float *__restrict__ d32)
+{
+    for (int g = blockIdx.x * FG_TPB + threadIdx.x; g <= neq; g += gridDim.x * FG_TPB) {
+        double s = 0.0;
+        if (g < neq)
+            for (int q = ioff[g]; q < ioff[g + 1]; q++) s = s + invde[q];
+        d32[g] = (float)s;
+    }
+}
+
+int fg_ebe_jacobi_diag32(fg_op *op, const double *invde, float *d32)
+{
+    fg_ctx *c = op->ctx;
+    FG_LAUNCH(k_ebe_jacobi_sum, fg_vec_grid(c, op->neq + 1), FG_TPB, 0, c->stream, op->neq,
+              op->d_inc_off, invde, d32);
+    return FG_OK;
+}
+
 int fg_ebe_jacobi_apply(fg_op *op, int fp32, const void *invde, const void *p, void *z,
                         const int *skip)
 {

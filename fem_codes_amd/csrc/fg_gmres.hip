@@ -40,6 +40,7 @@ struct GmState {
     double H[LDH * LDH];   // H[j*LDH + i]
     double cs[FG_KMAX], sn[FG_KMAX], e[FG_KMAX + 1], y[FG_KMAX];
     double rhoh[FG_KMAX];  // rho after step i of the current cycle
+    double hc[FG_KMAX + 1];  // CGS: the coefficients h_{j,i} of the current step
     double eps;            // convergence threshold of the current cycle
     double eps_final;      // tol * ||B||
     double rho;            // |e_{i+1}| after the last step
@@ -326,6 +327,130 @@ __global__ __launch_bounds__(FG_TPB) void k_scale_givens(int n, int i, const TB 
     }
 }
 
+// ---- classical Gram-Schmidt (FP32-IR inner cycle, FG_ORTH_CGS) ------------
+// All coefficients h_{j,i} = (u_j, w), j <= i, come from one pass over w and
+// the basis, then w -= sum_j h_{j,i} u_j in a second pass that also forms
+// ||w||^2.  Each thread keeps CG_NV entries of w in registers for both
+// passes; the grid therefore covers n in one sweep.
+constexpr int CG_NV = 16;
+constexpr int CG_GMAX = 4096;
+
+int cgs_grid(long long n)
+{
+    long long g = (n + (long long)FG_TPB * CG_NV - 1) / ((long long)FG_TPB * CG_NV);
+    if (g < 1) g = 1;
+    return g <= CG_GMAX ? (int)g : -1;
+}
+
+// [w = M^{-1} w (Diag)] ; partials of (u_j, w), j = 0..i -> pout[j*G + block]
+template <typename TB>
+__global__ __launch_bounds__(FG_TPB) void k_cgs_dots(int n, int i, const TB *__restrict__ invD,
+                                                     TB *__restrict__ W,
+                                                     const TB *__restrict__ U, long long ld,
+                                                     const GmState *st, double *__restrict__ pout)
+{
+    if (st->done) return;
+    __shared__ double red[FG_KMAX + 1][FG_TPB / 64];
+    const int G = gridDim.x;
+    const int base = blockIdx.x * FG_TPB * CG_NV + threadIdx.x;
+    TB w[CG_NV];
+#pragma unroll
+    for (int q = 0; q < CG_NV; q++) {
+        const int k = base + q * FG_TPB;
+        w[q] = k < n ? W[k] : TB(0);
+    }
+    if (invD) {
+#pragma unroll
+        for (int q = 0; q < CG_NV; q++) {
+            const int k = base + q * FG_TPB;
+            if (k < n) {
+                w[q] = w[q] * invD[k];
+                W[k] = w[q];
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int j = 0; j <= i; j++) {
+        const TB *Uj = U + (size_t)j * ld;
+        TB u[CG_NV];
+#pragma unroll
+        for (int q = 0; q < CG_NV; q++) {
+            const int k = base + q * FG_TPB;
+            u[q] = k < n ? Uj[k] : TB(0);
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < CG_NV; q++) acc = acc + (double)u[q] * (double)w[q];
+        acc = fg_wave_sum(acc);
+        if (lane == 0) red[j][wid] = acc;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j <= i; j += FG_TPB) {
+        double s = red[j][0];
+#pragma unroll
+        for (int v = 1; v < FG_TPB / 64; v++) s = s + red[j][v];
+        pout[(size_t)j * G + blockIdx.x] = s;
+    }
+}
+
+// h_{j,i} = sum over the G block partials of coefficient j (one block each)
+__global__ __launch_bounds__(FG_TPB) void k_cgs_reduce(int i, const double *__restrict__ pin,
+                                                       int G, GmState *st)
+{
+    if (st->done) return;
+    __shared__ double ws[FG_TPB / 64];
+    const int j = blockIdx.x;
+    double a = 0.0;
+    for (int b = threadIdx.x; b < G; b += FG_TPB) a = a + pin[(size_t)j * G + b];
+    a = fg_block_sum(a, ws);
+    if (threadIdx.x == 0) {
+        st->hc[j] = a;
+        Hij(st, j, i) = a;
+    }
+}
+
+// w -= sum_j h_{j,i} u_j (j ascending) ; partials of ||w||^2
+template <typename TB>
+__global__ __launch_bounds__(FG_TPB) void k_cgs_update(int n, int i, TB *__restrict__ W,
+                                                       const TB *__restrict__ U, long long ld,
+                                                       const GmState *st,
+                                                       double *__restrict__ pout)
+{
+    if (st->done) return;
+    __shared__ TB a[FG_KMAX + 1];
+    for (int j = threadIdx.x; j <= i; j += FG_TPB) a[j] = (TB)(-st->hc[j]);
+    __syncthreads();
+    const int base = blockIdx.x * FG_TPB * CG_NV + threadIdx.x;
+    TB w[CG_NV];
+#pragma unroll
+    for (int q = 0; q < CG_NV; q++) {
+        const int k = base + q * FG_TPB;
+        w[q] = k < n ? W[k] : TB(0);
+    }
+    for (int j = 0; j <= i; j++) {
+        const TB *Uj = U + (size_t)j * ld;
+        const TB aj = a[j];
+        TB u[CG_NV];
+#pragma unroll
+        for (int q = 0; q < CG_NV; q++) {
+            const int k = base + q * FG_TPB;
+            u[q] = k < n ? Uj[k] : TB(0);
+        }
+#pragma unroll
+        for (int q = 0; q < CG_NV; q++) w[q] = w[q] + aj * u[q];
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < CG_NV; q++) {
+        const int k = base + q * FG_TPB;
+        if (k < n) {
+            W[k] = w[q];
+            acc = acc + (double)w[q] * (double)w[q];
+        }
+    }
+    put_part(acc, pout);
+}
+
 // back substitution (pgmres.c:165-175)
 __global__ void k_backsub(GmState *st)
 {
@@ -424,7 +549,7 @@ int residual(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X, doub
 // one Arnoldi cycle with basis type TB, enqueued without host sync
 template <typename TB>
 int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState *st, int kmax,
-          Parts &pp)
+          Parts &pp, double *cgparts)
 {
     int n = op->neq;
     const int g = pp.G;
@@ -432,11 +557,31 @@ int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState
     const TB *invD = nullptr;
     bool diag = pc && pc->kind == FG_PREC_DIAG;
     if (diag) invD = fp32 ? (const TB *)pc->d_invdiag32 : (const TB *)pc->d_invdiag;
+    // the FP32 inner cycle applies the element Jacobi as its diagonal, fused
+    // into the first vector pass (the FP64 residual keeps the exact form)
+    if (fp32 && pc && pc->kind == FG_PREC_JACOBI && pc->d_jsum32) {
+        diag = true;
+        invD = (const TB *)pc->d_jsum32;
+    }
     const int *skip = &st->done;
+    const int gc = cgparts ? cgs_grid(n) : 0;
     for (int i = 0; i < kmax; i++) {
         TB *Ui = U + (size_t)i * ld;
         FG_CHECK(fg_op_apply_any(op, fp32, Ui, W, skip));
         if (pc && !diag) FG_CHECK(fg_prec_apply_any(pc, fp32, W, W, skip));
+        if (cgparts) {
+            // classical Gram-Schmidt: all h_{j,i} from one pass, one update pass
+            FG_LAUNCH(k_cgs_dots<TB>, gc, FG_TPB, 0, c->stream, n, i, invD, W, (const TB *)U,
+                      ld, (const GmState *)st, cgparts);
+            FG_LAUNCH(k_cgs_reduce, i + 1, FG_TPB, 0, c->stream, i, (const double *)cgparts, gc,
+                      st);
+            FG_LAUNCH(k_cgs_update<TB>, gc, FG_TPB, 0, c->stream, n, i, W, (const TB *)U, ld,
+                      (const GmState *)st, pp.out());
+            pp.flip();
+            FG_LAUNCH(k_scale_givens<TB>, g, FG_TPB, 0, c->stream, n, i, (const TB *)W,
+                      U + (size_t)(i + 1) * ld, st, (const double *)pp.in(), gc);
+            continue;
+        }
         FG_LAUNCH(k_prec_dot0<TB>, g, FG_TPB, 0, c->stream, n, invD, W, (const TB *)U,
                   (const GmState *)st, pp.out());
         pp.flip();
@@ -477,25 +622,32 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
     const long long ld = ((long long)n + 1 + 63) / 64 * 64;
     const size_t sb = ir ? sizeof(float) : sizeof(double);
     const bool fused = o->engine == FG_ENGINE_FUSED;
+    FG_ARG(o->orth == FG_ORTH_MGS || o->orth == FG_ORTH_CGS, "orth");
+    FG_ARG(o->orth == FG_ORTH_MGS || ir, "classical Gram-Schmidt applies to the FP32-IR cycle");
+    // CGS needs the grid of cgs_grid(n) blocks to cover n in one sweep
+    const bool cgs = ir && o->orth == FG_ORTH_CGS && !fused && cgs_grid(n) > 0;
 
     // device workspace, released on every return path
     struct Work {
         GmState *st = nullptr;
         void *U = nullptr, *W = nullptr;
-        double *W64 = nullptr, *nb2 = nullptr, *parts = nullptr;
+        double *W64 = nullptr, *nb2 = nullptr, *parts = nullptr, *cg = nullptr;
         ~Work()
         {
             hipFree(st); hipFree(U); hipFree(W); hipFree(W64); hipFree(nb2); hipFree(parts);
+            hipFree(cg);
         }
     } wk;
+    constexpr int PMAX = VG_MAX > CG_GMAX ? VG_MAX : CG_GMAX;   // partials per buffer
     FG_HIP(hipMalloc(&wk.st, sizeof(GmState)));
     FG_HIP(hipMalloc(&wk.W64, sizeof(double) * ld));
     FG_HIP(hipMalloc(&wk.nb2, sizeof(double)));
-    FG_HIP(hipMalloc(&wk.parts, sizeof(double) * 2 * VG_MAX));
+    FG_HIP(hipMalloc(&wk.parts, sizeof(double) * 2 * PMAX));
+    if (cgs) FG_HIP(hipMalloc(&wk.cg, sizeof(double) * (FG_KMAX + 1) * CG_GMAX));
     FG_HIP(hipMemsetAsync(wk.W64, 0, sizeof(double) * ld, c->stream));
     GmState *st = wk.st;
     double *W64 = wk.W64, *nb2 = wk.nb2;
-    Parts pp{{wk.parts, wk.parts + VG_MAX}, 0, vgrid(n)};
+    Parts pp{{wk.parts, wk.parts + PMAX}, 0, vgrid(n)};
 
     if (fused) {
         // eps = tol * ||B|| (pgmres.c:49-53), then the persistent solver
@@ -560,13 +712,15 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         if (ir) {
             FG_LAUNCH(k_scale_to<float>, g, FG_TPB, 0, c->stream, n, W64, (const float *)nullptr,
                       (float *)U, st, 0);
-            FG_CHECK(cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, kmax, pp));
+            FG_CHECK(cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, kmax, pp,
+                                  cgs ? wk.cg : nullptr));
             FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
             FG_LAUNCH(k_xupdate<float>, g, FG_TPB, 0, c->stream, n, X, (const float *)U, ld, st);
         } else {
             FG_LAUNCH(k_scale_to<double>, g, FG_TPB, 0, c->stream, n, W64,
                       (const double *)nullptr, (double *)U, st, 0);
-            FG_CHECK(cycle<double>(c, op, pc, (double *)U, ld, (double *)W, st, kmax, pp));
+            FG_CHECK(cycle<double>(c, op, pc, (double *)U, ld, (double *)W, st, kmax, pp,
+                                   nullptr));
             FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
             FG_LAUNCH(k_xupdate<double>, g, FG_TPB, 0, c->stream, n, X, (const double *)U, ld,
                       st);

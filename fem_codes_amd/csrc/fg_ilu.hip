@@ -76,6 +76,16 @@ struct fg_ilu {
     float *d_LU32 = nullptr, *d_D32 = nullptr;
     int *d_err = nullptr;      // first zero pivot row + 1
     Sched lo, up;
+    int sweeps = 0;            // 0: exact level-scheduled solves; s > 0: s Jacobi sweeps
+    double *d_s64[2] = {nullptr, nullptr};   // sweep scratch
+    float *d_s32[2] = {nullptr, nullptr};
+    // strictly lower / upper parts as compact CSR matrices for the sweeps
+    struct Tri {
+        int nrb = 0, nnz = 0;
+        int *d_rb = nullptr, *d_IA = nullptr, *d_JA = nullptr, *d_map = nullptr;
+        double *d_v64 = nullptr;
+        float *d_v32 = nullptr;
+    } lt, ut;
 };
 
 namespace {
@@ -241,6 +251,68 @@ int blocks_for(int rows) { return (rows + FG_TPB - 1) / FG_TPB; }
 
 }  // namespace
 
+// compact CSR of the strictly lower (lower = true) or upper part of A's
+// pattern, with row blocks of at most 256 rows / TRI_NNZB entries and the A
+// slot of each entry
+constexpr int TRI_NNZB = 2048;
+
+static int build_tri(int neq, const std::vector<int> &IA, const std::vector<int> &JA,
+                     const std::vector<int> &lend, const std::vector<int> &ubeg, bool lower,
+                     fg_ilu::Tri &t)
+{
+    std::vector<int> tIA(neq + 1, 0), tJA, map, rb{0};
+    int rows = 0, nz = 0;
+    for (int i = 0; i < neq; i++) {
+        const int q0 = lower ? IA[i] : ubeg[i], q1 = lower ? lend[i] : IA[i + 1];
+        for (int q = q0; q < q1; q++) {
+            tJA.push_back(JA[q]);
+            map.push_back(q);
+        }
+        tIA[i + 1] = (int)tJA.size();
+        const int rn = q1 - q0;
+        if (rows > 0 && (rows == FG_TPB || nz + rn > TRI_NNZB)) {
+            rb.push_back(i);
+            rows = 0;
+            nz = 0;
+        }
+        rows++;
+        nz += rn;
+    }
+    if (rows > 0) rb.push_back(neq);
+    t.nrb = (int)rb.size() - 1;
+    t.nnz = (int)tJA.size();
+    const size_t nzz = t.nnz ? t.nnz : 1;
+    FG_HIP(hipMalloc(&t.d_rb, sizeof(int) * rb.size()));
+    FG_HIP(hipMalloc(&t.d_IA, sizeof(int) * (neq + 1)));
+    FG_HIP(hipMalloc(&t.d_JA, sizeof(int) * nzz));
+    FG_HIP(hipMalloc(&t.d_map, sizeof(int) * nzz));
+    FG_HIP(hipMalloc(&t.d_v64, sizeof(double) * nzz));
+    FG_HIP(hipMalloc(&t.d_v32, sizeof(float) * nzz));
+    FG_HIP(hipMemcpy(t.d_rb, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice));
+    FG_HIP(hipMemcpy(t.d_IA, tIA.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice));
+    if (t.nnz) {
+        FG_HIP(hipMemcpy(t.d_JA, tJA.data(), sizeof(int) * t.nnz, hipMemcpyHostToDevice));
+        FG_HIP(hipMemcpy(t.d_map, map.data(), sizeof(int) * t.nnz, hipMemcpyHostToDevice));
+    }
+    return FG_OK;
+}
+
+static void free_tri(fg_ilu::Tri &t)
+{
+    hipFree(t.d_rb); hipFree(t.d_IA); hipFree(t.d_JA); hipFree(t.d_map);
+    hipFree(t.d_v64); hipFree(t.d_v32);
+}
+
+__global__ void k_tri_gather(int n, const int *__restrict__ map, const double *__restrict__ LU,
+                             double *__restrict__ v64, float *__restrict__ v32)
+{
+    for (int t = blockIdx.x * FG_TPB + threadIdx.x; t < n; t += gridDim.x * FG_TPB) {
+        const double a = LU[map[t]];
+        v64[t] = a;
+        v32[t] = (float)a;
+    }
+}
+
 int fg_ilu_create(fg_op *op, fg_ilu **out)
 {
     FG_ARG(op && op->kind == FG_OP_CSR, "ILU(0) needs a CSR operator");
@@ -277,6 +349,8 @@ int fg_ilu_create(fg_op *op, fg_ilu **out)
     FG_HIP(hipMemset(u->d_D, 0, sizeof(double) * nq));
     FG_CHECK(build_sched(neq, IA, JA, lend, ubeg, true, u->lo));
     FG_CHECK(build_sched(neq, IA, JA, lend, ubeg, false, u->up));
+    FG_CHECK(build_tri(neq, IA, JA, lend, ubeg, true, u->lt));
+    FG_CHECK(build_tri(neq, IA, JA, lend, ubeg, false, u->ut));
     *out = u;
     return FG_OK;
 }
@@ -287,6 +361,12 @@ void fg_ilu_destroy(fg_ilu *u)
     hipFree(u->d_lend); hipFree(u->d_ubeg); hipFree(u->d_dpos);
     hipFree(u->d_LU); hipFree(u->d_LU32); hipFree(u->d_D); hipFree(u->d_D32);
     hipFree(u->d_err);
+    for (int k = 0; k < 2; k++) {
+        hipFree(u->d_s64[k]);
+        hipFree(u->d_s32[k]);
+    }
+    free_tri(u->lt);
+    free_tri(u->ut);
     hipFree(u->lo.d_rows); hipFree(u->lo.d_loff); hipFree(u->up.d_rows); hipFree(u->up.d_loff);
     delete u;
 }
@@ -333,6 +413,10 @@ int fg_ilu_factor(fg_ilu *u, fg_op *op)
     if (u->neq)
         FG_LAUNCH(k_ilu_cast, fg_vec_grid(c, u->neq), FG_TPB, 0, c->stream, (long long)u->neq,
                   u->d_D, u->d_D32);
+    for (fg_ilu::Tri *t : {&u->lt, &u->ut})
+        if (t->nnz)
+            FG_LAUNCH(k_tri_gather, fg_vec_grid(c, t->nnz), FG_TPB, 0, c->stream, t->nnz,
+                      (const int *)t->d_map, (const double *)u->d_LU, t->d_v64, t->d_v32);
     return FG_OK;
 }
 
@@ -362,9 +446,122 @@ static int ilu_apply_t(fg_ilu *u, fg_ctx *c, const T *LU, const T *D, const T *p
     return FG_OK;
 }
 
+// ---- approximate triangular solves: fixed Jacobi sweeps ---------------------
+// forward  y_{m+1} = p - L y_m,          y_0 = p
+// backward x_{m+1} = D (y - U x_m),      x_0 = D y
+// A fixed sweep count is a fixed linear operator (a truncated Neumann series
+// of the exact solve, exact once it reaches the number of levels), so GMRES
+// can use it as a preconditioner; every sweep is one fully parallel pass.
+// out[r] = base[r] - (T x)[r]            (UPPER = false: a forward sweep)
+// out[r] = D[r] * (base[r] - (T x)[r])   (UPPER = true: a backward sweep)
+// T is the strictly lower or upper part as a compact CSR; row blocks are
+// streamed with coalesced loads through LDS as in the CSR product.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nrb, const int *__restrict__ rb,
+                                                      const int *__restrict__ IA,
+                                                      const int *__restrict__ JA,
+                                                      const T *__restrict__ V,
+                                                      const T *__restrict__ x,
+                                                      const T *__restrict__ base,
+                                                      const T *__restrict__ D,
+                                                      T *__restrict__ out, const int *skip)
+{
+    if (skip && *skip) return;
+    __shared__ T s[TRI_NNZB];
+    __shared__ T ws[FG_TPB / 64];
+    const int b = fg_xcd_remap(blockIdx.x, nrb);
+    const int r0 = rb[b], r1 = rb[b + 1];
+    const int j0 = IA[r0], j1 = IA[r1];
+    if (j1 - j0 <= TRI_NNZB) {
+        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB) s[j - j0] = V[j] * x[JA[j]];
+        __syncthreads();
+        for (int r = r0 + threadIdx.x; r < r1; r += FG_TPB) {
+            T acc = T(0);
+            const int a = IA[r] - j0, e = IA[r + 1] - j0;
+            for (int j = a; j < e; j++) acc = acc + s[j];
+            const T y = base[r] - acc;
+            out[r] = UPPER ? D[r] * y : y;
+        }
+    } else {   // one row longer than a block
+        T acc = T(0);
+        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB) acc = acc + V[j] * x[JA[j]];
+        acc = fg_block_sum(acc, ws);
+        if (threadIdx.x == 0) {
+            const T y = base[r0] - acc;
+            out[r0] = UPPER ? D[r0] * y : y;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(FG_TPB) void k_dscale(int n, const T *__restrict__ D,
+                                                   const T *__restrict__ y, T *__restrict__ out,
+                                                   const int *skip)
+{
+    if (skip && *skip) return;
+    for (int i = blockIdx.x * FG_TPB + threadIdx.x; i < n; i += gridDim.x * FG_TPB)
+        out[i] = D[i] * y[i];
+}
+
+template <typename T>
+static const T *tri_vals(const fg_ilu::Tri &t);
+template <>
+const double *tri_vals<double>(const fg_ilu::Tri &t) { return t.d_v64; }
+template <>
+const float *tri_vals<float>(const fg_ilu::Tri &t) { return t.d_v32; }
+
+template <typename T>
+static int ilu_sweeps_t(fg_ilu *u, fg_ctx *c, const T *D, const T *p, T *z, T *const s[2],
+                        const int *skip)
+{
+    const int n = u->neq, S = u->sweeps;
+    const fg_ilu::Tri &L = u->lt, &U = u->ut;
+    // forward: y_1..y_S alternate s[0], s[1] (y_0 = p)
+    const T *y = p;
+    for (int m = 0; m < S; m++) {
+        T *out = s[m & 1];
+        if (L.nrb)
+            FG_LAUNCH((k_tri_sweep<T, false>), L.nrb, FG_TPB, 0, c->stream, L.nrb,
+                      (const int *)L.d_rb, (const int *)L.d_IA, (const int *)L.d_JA,
+                      tri_vals<T>(L), y, p, D, out, skip);
+        y = out;
+    }
+    // backward: x_0..x_S between the free scratch and z, x_S in z (p is no
+    // longer read, so z may alias it)
+    T *other = (y == s[0]) ? s[1] : s[0];
+    T *buf[2] = {(S & 1) ? other : z, (S & 1) ? z : other};   // x_m in buf[m & 1]
+    FG_LAUNCH(k_dscale<T>, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, D, y, buf[0], skip);
+    for (int m = 1; m <= S; m++)
+        if (U.nrb)
+            FG_LAUNCH((k_tri_sweep<T, true>), U.nrb, FG_TPB, 0, c->stream, U.nrb,
+                      (const int *)U.d_rb, (const int *)U.d_IA, (const int *)U.d_JA,
+                      tri_vals<T>(U), (const T *)buf[(m - 1) & 1], y, D, buf[m & 1], skip);
+    return FG_OK;
+}
+
+int fg_ilu_set_sweeps(fg_ilu *u, int sweeps)
+{
+    u->sweeps = sweeps;
+    if (sweeps > 0 && !u->d_s64[0]) {
+        const size_t n = (size_t)u->neq + 1;
+        for (int k = 0; k < 2; k++) {
+            FG_HIP(hipMalloc(&u->d_s64[k], n * sizeof(double)));
+            FG_HIP(hipMalloc(&u->d_s32[k], n * sizeof(float)));
+        }
+    }
+    return FG_OK;
+}
+
 // z = (LU)^{-1} p on device vectors (neq+1); p may alias z
 int fg_ilu_apply(fg_ilu *u, fg_ctx *c, int fp32, const void *p, void *z, const int *skip)
 {
+    if (u->sweeps > 0) {
+        if (fp32)
+            return ilu_sweeps_t<float>(u, c, u->d_D32, (const float *)p, (float *)z, u->d_s32,
+                                       skip);
+        return ilu_sweeps_t<double>(u, c, u->d_D, (const double *)p, (double *)z, u->d_s64,
+                                    skip);
+    }
     if (fp32)
         return ilu_apply_t<float>(u, c, u->d_LU32, u->d_D32, (const float *)p, (float *)z, skip);
     return ilu_apply_t<double>(u, c, u->d_LU, u->d_D, (const double *)p, (double *)z, skip);

@@ -119,11 +119,13 @@ int fg_ilu_apply(fg_ilu *u, fg_ctx *c, int fp32, const void *p, void *z, const i
 int fg_ilu_get(const fg_ilu *u, fg_ctx *c, double *LU, double *D);
 void fg_ilu_levels(const fg_ilu *u, int *nlev_lower, int *nlev_upper, int *nseg);
 void fg_ilu_set_wide(fg_ilu *u, int wide_rows);
+int fg_ilu_set_sweeps(fg_ilu *u, int sweeps);
 
 struct fg_prec {
     fg_op *op = nullptr;
     int kind = FG_PREC_NONE;
     fg_ilu *ilu = nullptr;       // FG_PREC_ILU0
+    float *d_jsum32 = nullptr;   // element Jacobi as a diagonal (FP32 inner cycle)
     double *d_diag = nullptr, *d_invdiag = nullptr;
     float *d_invdiag32 = nullptr;
     double *d_invde = nullptr;   // [nel*3] element-local Jacobi

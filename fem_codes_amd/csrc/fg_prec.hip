@@ -10,6 +10,7 @@
 int fg_ebe_diag(fg_op *op, double *D);
 int fg_csr_diag(fg_op *op, double *D);
 int fg_ebe_jacobi_setup(fg_op *op, double **invde, float **invde32);
+int fg_ebe_jacobi_diag32(fg_op *op, const double *invde, float *d32);
 int fg_ebe_jacobi_apply(fg_op *op, int fp32, const void *invde, const void *p, void *z,
                         const int *skip);
 
@@ -95,6 +96,8 @@ extern "C" int fg_prec_setup(fg_prec *pc, double *F)
     }
     // Jacobi: per-incidence 1/A^e_jj, then F <- Jacobi(F) (setup :76-84)
     FG_CHECK(fg_ebe_jacobi_setup(op, &pc->d_invde, &pc->d_invde32));
+    if (!pc->d_jsum32) FG_HIP(hipMalloc(&pc->d_jsum32, sizeof(float) * (neq + 1)));
+    FG_CHECK(fg_ebe_jacobi_diag32(op, pc->d_invde, pc->d_jsum32));
     if (F) {
         FG_HIP(hipMemcpyAsync(pc->d_tmp64, F, sizeof(double) * (neq + 1),
                               hipMemcpyDeviceToDevice, c->stream));
@@ -179,6 +182,7 @@ extern "C" int fg_prec_destroy(fg_prec *pc)
     hipStreamSynchronize(pc->op->ctx->stream);
     hipFree(pc->d_diag); hipFree(pc->d_invdiag); hipFree(pc->d_invdiag32);
     hipFree(pc->d_invde); hipFree(pc->d_invde32); hipFree(pc->d_tmp64); hipFree(pc->d_tmp32);
+    hipFree(pc->d_jsum32);
     fg_ilu_destroy(pc->ilu);
     delete pc;
     return FG_OK;
@@ -197,6 +201,13 @@ extern "C" int fg_prec_ilu_schedule(fg_prec *pc, int wide_rows)
     FG_ARG(wide_rows >= 0, "wide_rows >= 0");
     fg_ilu_set_wide(pc->ilu, wide_rows);
     return FG_OK;
+}
+
+extern "C" int fg_prec_ilu_sweeps(fg_prec *pc, int sweeps)
+{
+    FG_ARG(pc && pc->kind == FG_PREC_ILU0, "ILU(0) preconditioner");
+    FG_ARG(sweeps >= 0 && sweeps <= 1000, "sweeps in [0,1000]");
+    return fg_ilu_set_sweeps(pc->ilu, sweeps);
 }
 
 extern "C" int fg_prec_ilu_get(fg_prec *pc, double *LU_host, double *D_host)

@@ -148,6 +148,15 @@ int fg_prec_ilu_info(fg_prec *pc, int *nlev_lower, int *nlev_upper, int *nsegmen
  * more than wide_rows rows gets a launch of its own, runs of narrower levels
  * share one workgroup (default 8192).  Results are identical either way. */
 int fg_prec_ilu_schedule(fg_prec *pc, int wide_rows);
+/* triangular solves of the apply (no reference counterpart: an option of
+ * this build): sweeps = 0 solves L and U exactly (level-scheduled, the
+ * reference's result bit for bit); sweeps = s > 0 replaces each solve by s
+ * Jacobi sweeps, y <- p - L y and x <- D (y - U x), every sweep one fully
+ * parallel pass.  s sweeps are a fixed linear operator, exact once s reaches
+ * the number of levels; orderings such as RCM make the exact solves
+ * sequential over thousands of levels, the sweeps do not.  The setup
+ * preconditions F with the same solves, so set this before fg_prec_setup. */
+int fg_prec_ilu_sweeps(fg_prec *pc, int sweeps);
 /* factors to host: LU[nnz] in the operator's CSR slots (diagonal slot
  * unused), D[neq] inverted pivots */
 int fg_prec_ilu_get(fg_prec *pc, double *LU_host, double *D_host);
@@ -263,7 +272,14 @@ typedef struct {
     double *e0_hist;    /* host, lmax+1 entries: ||r|| at every restart, or NULL
                            (kernel-per-operation engine only)                  */
     int    engine;      /* FG_ENGINE_AUTO / _KERNELS / _FUSED                  */
+    int    orth;        /* FP32-IR inner cycle: FG_ORTH_MGS (0, pgmres.c:107-114)
+                           or FG_ORTH_CGS (classical Gram-Schmidt: every h_{j,i}
+                           from one pass, one update pass; an option of this
+                           build).  The FP64 solve is always MGS.            */
 } fg_gmres_opts;
+
+#define FG_ORTH_MGS 0
+#define FG_ORTH_CGS 1
 
 /* engines: AUTO selects KERNELS (the faster engine at every measured size).
  * KERNELS: one kernel per vector operation, host sync once per cycle.

@@ -207,3 +207,60 @@ def test_gmres_ebe_k4_with_csr_ilu0(ctx, precision):
                     A[L[e, a], L[e, b]] += kh[e, a, b]
     Xd = np.linalg.solve(A, Fh[:neq])
     assert np.linalg.norm(X.get()[:neq] - Xd) <= 1e-8 * np.linalg.norm(Xd)
+
+
+def test_ilu0_sweeps_reach_the_exact_solve(ctx):
+    """s Jacobi sweeps per triangle equal the exact triangular solves once s
+    reaches the number of dependency levels (the sweep iteration matrix is
+    nilpotent of that order); fewer sweeps differ"""
+    m, lm, ke, Fv, csr, IA, JA = _system3d(ctx, 8, perm_seed=5)
+    neq = m.neq
+    pc = F.Preconditioner(csr, F.PREC_ILU0)
+    pc.setup(None)
+    nl, nu, _ = pc.ilu_levels()
+    P = np.random.default_rng(8).standard_normal(neq + 1)
+    z0 = ctx.empty(neq + 1)
+    pc.apply(ctx.to_device(P), z0)
+    exact = z0.get()[:neq]
+    pc.ilu_sweeps(max(nl, nu))
+    pc.setup(None)
+    z = ctx.to_device(P)
+    pc.apply(z, z)                      # in place
+    assert _rel(z.get()[:neq], exact) <= 1e-12
+    pc.ilu_sweeps(1)
+    z1 = ctx.empty(neq + 1)
+    pc.apply(ctx.to_device(P), z1)
+    assert _rel(z1.get()[:neq], exact) > 1e-6
+
+
+@pytest.mark.parametrize("sweeps", [2, 4])
+def test_gmres_ir_with_ilu0_sweeps(ctx, sweeps):
+    m, lm, ke, Fv, csr, IA, JA = _system3d(ctx, 8, perm_seed=11)
+    neq = m.neq
+    P = F.rcm(neq, IA, JA)                   # the 3D configuration's numbering
+    lm2 = F.renumber_lm(neq, lm, P)
+    nnz, IA2, JA2, cbe = F.csr_pattern_k(neq, lm2, 4)
+    csr2 = F.Operator.csr(ctx, neq, IA2, JA2)
+    F.assemble_csr(csr2, cbe, ke, k=4)
+    ebe = F.Operator.ebe(ctx, neq, lm2, 4)
+    ebe.set_values(ke)
+    Fh = np.zeros(neq + 1)
+    Fh[:neq] = Fv.get()[P]
+    pc = F.Preconditioner(csr2, F.PREC_ILU0)
+    pc.ilu_sweeps(sweeps)
+    B = ctx.to_device(Fh)
+    pc.setup(B)
+    X = ctx.empty(neq + 1)
+    info, _, _ = F.gmres(ctx, ebe, pc, B, X, kmax=50, lmax=100, tol=1e-10,
+                         precision=F.GMRES_FP32_IR)
+    assert info.status == F.FG_OK and info.true_res <= info.eps
+    A = np.zeros((neq, neq))
+    kh = ke.get().reshape(-1, 4, 4)
+    L = lm2.reshape(-1, 4)
+    for e in range(len(L)):
+        for a in range(4):
+            for b in range(4):
+                if L[e, a] < neq and L[e, b] < neq:
+                    A[L[e, a], L[e, b]] += kh[e, a, b]
+    Xd = np.linalg.solve(A, Fh[:neq])
+    assert np.linalg.norm(X.get()[:neq] - Xd) <= 1e-8 * np.linalg.norm(Xd)

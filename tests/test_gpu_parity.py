@@ -259,6 +259,54 @@ def test_gmres_fp32_ir(ctx, path):
     assert _rel(X.get()[:neq], d["X_diag"][:neq]) <= 1e-6
 
 
+@pytest.mark.parametrize("path", golden_files("ss2d_n16_*.npz"), ids=os.path.basename)
+def test_gmres_fp32_ir_cgs(ctx, path):
+    """FP32-IR with the classical Gram-Schmidt inner cycle: the FP64 outer
+    residual still reaches the tolerance and X matches the reference's FP64
+    solution to the same bound as the MGS variant"""
+    d = load_golden(path)
+    neq = int(d["neq"])
+    op = _op(ctx, d)
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    Bd = ctx.to_device(d["F_u0"])
+    pc.setup(Bd)
+    X = ctx.empty(neq + 1)
+    info, _, _ = F.gmres(ctx, op, pc, Bd, X, kmax=30, lmax=200, tol=1e-8,
+                         precision=F.GMRES_FP32_IR, orth=F.ORTH_CGS)
+    assert info.status == F.FG_OK
+    assert info.true_res <= info.eps
+    assert _rel(X.get()[:neq], d["X_diag"][:neq]) <= 1e-6
+
+
+@pytest.mark.parametrize("path", golden_files("ss2d_n16_*_ebe.npz"), ids=os.path.basename)
+@pytest.mark.parametrize("orth", [F.ORTH_MGS, F.ORTH_CGS])
+def test_gmres_fp32_ir_jacobi(ctx, path, orth):
+    """FP32-IR with the element Jacobi (applied as its diagonal inside the
+    FP32 cycle, exactly in the FP64 residual) reaches the tolerance and the
+    reference's solution"""
+    d = load_golden(path)
+    neq = int(d["neq"])
+    op = _op(ctx, d)
+    pc = F.Preconditioner(op, F.PREC_JACOBI)
+    Bd = ctx.to_device(d["F_u0"])
+    pc.setup(Bd)
+    X = ctx.empty(neq + 1)
+    info, _, _ = F.gmres(ctx, op, pc, Bd, X, kmax=30, lmax=200, tol=1e-8,
+                         precision=F.GMRES_FP32_IR, orth=orth)
+    assert info.status == F.FG_OK
+    assert info.true_res <= info.eps
+    assert _rel(X.get()[:neq], d["X_diag"][:neq]) <= 1e-6
+
+
+def test_gmres_cgs_rejected_for_fp64(ctx):
+    d = load_golden(FILES[0])
+    neq = int(d["neq"])
+    op = _op(ctx, d)
+    B, X = ctx.to_device(d["F_u0"]), ctx.empty(neq + 1)
+    with pytest.raises(F.FemGpuError):
+        F.gmres(ctx, op, None, B, X, kmax=10, lmax=5, tol=1e-8, orth=F.ORTH_CGS)
+
+
 def test_gmres_zero_rhs(ctx):
     d = load_golden(FILES[0])
     neq = int(d["neq"])
