@@ -25,7 +25,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfemgpu.so")
 
 FG_OK, FG_ERR_HIP, FG_ERR_ARG, FG_ERR_BREAKDOWN, FG_ERR_NOCONV, FG_ERR_NOMEM = range(6)
-PREC_NONE, PREC_DIAG, PREC_JACOBI, PREC_ILU0 = 0, 1, 2, 3
+PREC_NONE, PREC_DIAG, PREC_JACOBI, PREC_ILU0, PREC_BLOCKDIAG = 0, 1, 2, 3, 4
+EBE2_BLOCKDIAG, EBE2_NOBLOCKDIAG = 0, 1
 P3_DIFCONV, P3_CONST = 0, 1
 GMRES_FP64, GMRES_FP32_IR = 0, 1
 
@@ -125,6 +126,11 @@ SIGNATURES = {
     "fg_prec_ilu_sweeps": (_i, [_vp, _i]),
     "fg_rcm": (_i, [_i, _ip, _ip, _ip]),
     "fg_csr_bandwidth": (_i, [_i, _ip, _ip, C.POINTER(_i)]),
+    "fg_prec_blockdiag_create": (_i, [_vp, _i, _ip, _ip, C.POINTER(_vp)]),
+    "fg_prec_blockdiag_get": (_i, [_vp, _vp, _vp]),
+    "fg_block_scaling_ebe": (_i, [_vp, _vp]),
+    "fg_op_ebe2_create": (_i, [_vp, _vp, _i, C.POINTER(_vp)]),
+    "fg_op_get_values_host": (_i, [_vp, _dp]),
 }
 
 
@@ -272,6 +278,24 @@ class Operator:
                "fg_op_csr_create")
         return cls(ctx, h, "CSR")
 
+    @classmethod
+    def ebe2(cls, base: "Operator", blockdiag: "Preconditioner", form=EBE2_BLOCKDIAG):
+        """ebe2mvNDOF4: off-block product of a k = 12 operator followed by
+        BlockDiag2_precond (form EBE2_BLOCKDIAG) or NOBlockDiag2_precond
+        (EBE2_NOBLOCKDIAG) with the node blocks of `blockdiag`"""
+        h = _vp()
+        _check(lib.fg_op_ebe2_create(base.h, blockdiag.h, int(form), C.byref(h)),
+               "fg_op_ebe2_create")
+        op = cls(base.ctx, h, "EBE2")
+        op._keep = (base, blockdiag)
+        return op
+
+    def get_values(self) -> np.ndarray:
+        """element matrices (nitems*k*k, row-major per element) back to host"""
+        A = np.zeros(max(self.nitems * self.k * self.k, 1))
+        _check(lib.fg_op_get_values_host(self.h, A), "fg_op_get_values_host")
+        return A[:self.nitems * self.k * self.k]
+
     def set_values(self, vals):
         if isinstance(vals, DeviceArray):
             _check(lib.fg_op_set_values_dev(self.h, vals.ptr), "fg_op_set_values_dev")
@@ -321,10 +345,37 @@ class Operator:
 class Preconditioner:
     """fg_prec: NONE, DIAG (global diagonal) or JACOBI (element-local)."""
 
-    def __init__(self, op: Operator, kind: int):
-        h = _vp()
-        _check(lib.fg_prec_create(op.h, kind, C.byref(h)), "fg_prec_create")
+    def __init__(self, op: Operator, kind: int, _h=None):
+        h = _vp() if _h is None else _h
+        if _h is None:
+            _check(lib.fg_prec_create(op.h, kind, C.byref(h)), "fg_prec_create")
         self.op, self.h, self.kind = op, h, kind
+
+    @classmethod
+    def blockdiag(cls, op: Operator, nnodes: int, vertex, ids):
+        """node block-diagonal (BlockDiag_precond*) of a k = 12 EBE operator;
+        vertex (nel*3), ids (nnodes*4, < 0 = prescribed)"""
+        h = _vp()
+        _check(lib.fg_prec_blockdiag_create(op.h, int(nnodes),
+                                            np.ascontiguousarray(vertex, np.int32),
+                                            np.ascontiguousarray(ids, np.int32), C.byref(h)),
+               "fg_prec_blockdiag_create")
+        pc = cls(op, PREC_BLOCKDIAG, _h=h)
+        pc.nnodes = int(nnodes)
+        return pc
+
+    def blocks(self):
+        """block-diagonal: (BD, invBD), each nnodes*16 row-major node blocks"""
+        BD, iBD = np.zeros(16 * self.nnodes), np.zeros(16 * self.nnodes)
+        _check(lib.fg_prec_blockdiag_get(self.h, BD.ctypes.data, iBD.ctypes.data),
+               "fg_prec_blockdiag_get")
+        return BD, iBD
+
+    def block_scaling(self, F: DeviceArray | None):
+        """Block_scaling_EBE: setup (F preconditioned) and the operator's
+        element matrices scaled in place"""
+        _check(lib.fg_block_scaling_ebe(self.h, F.ptr if F is not None else None),
+               "fg_block_scaling_ebe")
 
     def setup(self, F: DeviceArray | None):
         _check(lib.fg_prec_setup(self.h, F.ptr if F is not None else None), "fg_prec_setup")

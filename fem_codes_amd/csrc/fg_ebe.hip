@@ -230,7 +230,7 @@ int upload(const std::vector<T> &v, T **d)
 // ---------------------------------------------------------------------------
 // device: apply
 // ---------------------------------------------------------------------------
-template <int K, int EPT, typename T>
+template <int K, int EPT, typename T, int BS>
 __global__ __launch_bounds__(FG_TPB) void k_ebe_apply(FgEbeLayout L, const T *__restrict__ Kv,
                                                       const T *__restrict__ P, T *__restrict__ Q,
                                                       T *__restrict__ hpart, const int *skip)
@@ -238,14 +238,14 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_apply(FgEbeLayout L, const T *__
     if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     const int b = fg_xcd_remap(blockIdx.x, L.nblk);
-    fg_ebe_block<K, EPT, T>(b, L, Kv, P, Q, hpart, smem_raw);
+    fg_ebe_block<K, EPT, T, BS>(b, L, Kv, P, Q, hpart, smem_raw);
 }
 
 // persistent form: workgroup w walks logical blocks w, w+G, ... and issues the
 // loads of its next block's element matrices before it reduces the current
 // one, so they are in flight during the LDS phase.  G is a multiple of 8 so
 // the XCD-contiguous remap stays aligned with the hardware's round-robin.
-template <int K, int EPT, typename T>
+template <int K, int EPT, typename T, int BS>
 __global__ __launch_bounds__(FG_TPB) void k_ebe_apply_pipe(FgEbeLayout L,
                                                            const T *__restrict__ Kv,
                                                            const T *__restrict__ P,
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_apply_pipe(FgEbeLayout L,
         const int sn = s + (int)gridDim.x;
         fg_ebe_gather<K, T>(b, L, P, smem_raw);
         if (sn < nblk) fg_ebe_load<K, EPT, T>(fg_xcd_remap(sn, nblk), L, Kv, nxt);
-        fg_ebe_finish<K, EPT, T>(b, L, Kv, Q, hpart, smem_raw, cur);
+        fg_ebe_finish<K, EPT, T, BS>(b, L, Kv, Q, hpart, smem_raw, cur);
         if (sn >= nblk) break;
         cur = nxt;
         s = sn;
@@ -314,7 +314,7 @@ __global__ void k_ebe_scatter_values(long long n, int kk, const double *__restri
     }
 }
 
-template <int K, int EPT, typename T>
+template <int K, int EPT, typename T, int BS>
 static int launch_apply_k(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *skip)
 {
     fg_ctx *c = op->ctx;
@@ -325,23 +325,23 @@ static int launch_apply_k(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *
         g = g / 8 * 8;
         if (g < 8) g = 8;
         if (g > op->nblk) g = op->nblk;
-        FG_LAUNCH((k_ebe_apply_pipe<K, EPT, T>), (int)g, FG_TPB, shm, c->stream, L, Kv, P, Q,
+        FG_LAUNCH((k_ebe_apply_pipe<K, EPT, T, BS>), (int)g, FG_TPB, shm, c->stream, L, Kv, P, Q,
                   hp, skip);
         return FG_OK;
     }
-    FG_LAUNCH((k_ebe_apply<K, EPT, T>), op->nblk, FG_TPB, shm, c->stream, L, Kv, P, Q, hp,
+    FG_LAUNCH((k_ebe_apply<K, EPT, T, BS>), op->nblk, FG_TPB, shm, c->stream, L, Kv, P, Q, hp,
               skip);
     return FG_OK;
 }
 
-template <int K, typename T>
+template <int K, typename T, int BS = 0>
 static int launch_apply_ept(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *skip)
 {
     switch (op->ept) {
-    case 1: return launch_apply_k<K, 1, T>(op, Kv, P, Q, hp, skip);
-    case 2: return launch_apply_k<K, 2, T>(op, Kv, P, Q, hp, skip);
-    case 4: return launch_apply_k<K, 4, T>(op, Kv, P, Q, hp, skip);
-    case 8: return launch_apply_k<K, 8, T>(op, Kv, P, Q, hp, skip);
+    case 1: return launch_apply_k<K, 1, T, BS>(op, Kv, P, Q, hp, skip);
+    case 2: return launch_apply_k<K, 2, T, BS>(op, Kv, P, Q, hp, skip);
+    case 4: return launch_apply_k<K, 4, T, BS>(op, Kv, P, Q, hp, skip);
+    case 8: return launch_apply_k<K, 8, T, BS>(op, Kv, P, Q, hp, skip);
     }
     fg_set_error("unsupported elements-per-thread %d", op->ept);
     return FG_ERR_ARG;
@@ -401,6 +401,59 @@ int fg_ebe_apply_f64(fg_op *op, double *P, double *Q, const int *skip)
 int fg_ebe_apply_f32(fg_op *op, float *P, float *Q, const int *skip)
 {
     return ebe_apply<float>(op, op->d_K32, P, Q, op->d_hpart32, skip);
+}
+
+// off-block product of a 12-dof (3 nodes x 4 dofs) operator: the row sums
+// skip the row node's own 4x4 block (ebe2mvNDOF4.c:24-70); Q complete, P and
+// Q sink slots 0
+int fg_ebe_apply_offblock(fg_op *op, int fp32, void *P, void *Q, const int *skip)
+{
+    FG_ARG(op->k == 12, "off-block product needs k = 12");
+    fg_ctx *c = op->ctx;
+    if (op->nblk > 0) {
+        if (fp32)
+            FG_CHECK((launch_apply_ept<12, float, 4>(op, op->d_K32, (float *)P, (float *)Q,
+                                                     op->d_hpart32, skip)));
+        else
+            FG_CHECK((launch_apply_ept<12, double, 4>(op, op->d_K64, (double *)P, (double *)Q,
+                                                      op->d_hpart64, skip)));
+    }
+    int n = op->nhalo + op->nzero;
+    int g = (n + FG_TPB - 1) / FG_TPB;
+    if (g < 1) g = 1;
+    if (fp32)
+        FG_LAUNCH((k_ebe_halo<float>), g, FG_TPB, 0, c->stream, op->nhalo, op->d_hdofs,
+                  op->d_hoff, op->d_hpart32, op->nzero, op->d_zdofs, op->neq, (float *)P,
+                  (float *)Q, skip);
+    else
+        FG_LAUNCH((k_ebe_halo<double>), g, FG_TPB, 0, c->stream, op->nhalo, op->d_hdofs,
+                  op->d_hoff, op->d_hpart64, op->nzero, op->d_zdofs, op->neq, (double *)P,
+                  (double *)Q, skip);
+    return FG_OK;
+}
+
+// blocked layout -> AoS (element-major) FP64 values; dropped elements read 0
+__global__ void k_ebe_gather_values(long long n, int kk, const long long *__restrict__ kpos,
+                                    const int *__restrict__ kstride,
+                                    const double *__restrict__ K64, double *__restrict__ A)
+{
+    for (long long t = (long long)blockIdx.x * FG_TPB + threadIdx.x; t < n;
+         t += (long long)gridDim.x * FG_TPB) {
+        long long e = t / kk;
+        int q = (int)(t - e * kk);
+        long long p = kpos[e];
+        A[t] = p < 0 ? 0.0 : K64[p + (long long)q * kstride[e]];
+    }
+}
+
+int fg_ebe_get_values_dev(fg_op *op, double *A)
+{
+    fg_ctx *c = op->ctx;
+    long long n = (long long)op->nitems * op->k * op->k;
+    if (n == 0) return FG_OK;
+    FG_LAUNCH(k_ebe_gather_values, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, op->k * op->k,
+              op->d_kpos, op->d_kstride, op->d_K64, A);
+    return FG_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -114,7 +114,13 @@ __device__ __forceinline__ void fg_ebe_gather(int b, const FgEbeLayout &L,
     __syncthreads();
 }
 
-template <int K, int EPT, typename T>
+// BS > 0: the row sums leave out the BS x BS diagonal block of the row's
+// node (the off-block product of ebe2mvNDOF4.c:52-67, BS = 4); those entries
+// are not read.  First summed column of row rr:
+template <int BS>
+__device__ __forceinline__ constexpr int fg_ebe_c0(int rr) { return BS > 0 && rr < BS ? BS : 0; }
+
+template <int K, int EPT, typename T, int BS = 0>
 __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
                                               const T *__restrict__ Kv, T *__restrict__ Q,
                                               T *__restrict__ hpart, unsigned char *smem,
@@ -134,15 +140,19 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
             for (int j = 0; j < K; j++) p[j] = S.sP[r.li[i][j]];
 #pragma unroll
             for (int rr = 0; rr < K; rr++) {
+                const int c0 = fg_ebe_c0<BS>(rr);
                 T acc;
                 if constexpr (FgEbeRegs<K, EPT, T>::PRE) {
-                    acc = r.kv[i][rr * K] * p[0];
+                    acc = r.kv[i][rr * K + c0] * p[c0];
 #pragma unroll
-                    for (int c = 1; c < K; c++) acc = acc + r.kv[i][rr * K + c] * p[c];
+                    for (int c = c0 + 1; c < K; c++)
+                        if (BS == 0 || c / BS != rr / BS) acc = acc + r.kv[i][rr * K + c] * p[c];
                 } else {
-                    acc = Kb[(size_t)(rr * K) * ne + el] * p[0];
+                    acc = Kb[(size_t)(rr * K + c0) * ne + el] * p[c0];
 #pragma unroll
-                    for (int c = 1; c < K; c++) acc = acc + Kb[(size_t)(rr * K + c) * ne + el] * p[c];
+                    for (int c = c0 + 1; c < K; c++)
+                        if (BS == 0 || c / BS != rr / BS)
+                            acc = acc + Kb[(size_t)(rr * K + c) * ne + el] * p[c];
                 }
                 S.sY[rr * ne + el] = acc;
             }
@@ -162,7 +172,7 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
 }
 
 // the three phases back to back (one block per workgroup)
-template <int K, int EPT, typename T>
+template <int K, int EPT, typename T, int BS = 0>
 __device__ __forceinline__ void fg_ebe_block(int b, const FgEbeLayout &L, const T *__restrict__ Kv,
                                              const T *__restrict__ P, T *__restrict__ Q,
                                              T *__restrict__ hpart, unsigned char *smem)
@@ -170,5 +180,5 @@ __device__ __forceinline__ void fg_ebe_block(int b, const FgEbeLayout &L, const 
     FgEbeRegs<K, EPT, T> r;
     fg_ebe_load<K, EPT, T>(b, L, Kv, r);
     fg_ebe_gather<K, T>(b, L, P, smem);
-    fg_ebe_finish<K, EPT, T>(b, L, Kv, Q, hpart, smem, r);
+    fg_ebe_finish<K, EPT, T, BS>(b, L, Kv, Q, hpart, smem, r);
 }

@@ -66,7 +66,8 @@ struct fg_ctx {
 // ---------------------------------------------------------------------------
 // operators
 // ---------------------------------------------------------------------------
-enum { FG_OP_EBE = 0, FG_OP_EDE = 1, FG_OP_CSR = 2 };
+// FG_OP_EBE2: off-block product of a k = 12 EBE operator + node pass
+enum { FG_OP_EBE = 0, FG_OP_EDE = 1, FG_OP_CSR = 2, FG_OP_EBE2 = 3 };
 
 struct fg_op {
     fg_ctx *ctx = nullptr;
@@ -109,6 +110,11 @@ struct fg_op {
     int nrb = 0;
     int *d_rb = nullptr;         // [nrb+1] row-block boundaries
     std::vector<int> h_IA, h_JA;
+
+    // ---- EBE2 (shares the storage of `base` and the blocks of `bd`) ----
+    fg_op *base = nullptr;
+    struct fg_prec *bd = nullptr;
+    int form = 0;
 };
 
 struct fg_ilu;
@@ -121,10 +127,18 @@ void fg_ilu_levels(const fg_ilu *u, int *nlev_lower, int *nlev_upper, int *nseg)
 void fg_ilu_set_wide(fg_ilu *u, int wide_rows);
 int fg_ilu_set_sweeps(fg_ilu *u, int sweeps);
 
+struct fg_nodeblk;
+void fg_nodeblk_destroy(fg_nodeblk *b);
+int fg_nodeblk_setup(struct fg_prec *pc, double *F);
+int fg_nodeblk_apply(struct fg_prec *pc, int fp32, const void *p, void *z, const int *skip);
+int fg_ebe2_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
+double fg_ebe2_bytes(const fg_op *op, int fp32);
+
 struct fg_prec {
     fg_op *op = nullptr;
     int kind = FG_PREC_NONE;
     fg_ilu *ilu = nullptr;       // FG_PREC_ILU0
+    fg_nodeblk *nb = nullptr;    // FG_PREC_BLOCKDIAG
     float *d_jsum32 = nullptr;   // element Jacobi as a diagonal (FP32 inner cycle)
     double *d_diag = nullptr, *d_invdiag = nullptr;
     float *d_invdiag32 = nullptr;

@@ -26,9 +26,35 @@ static long long op_nvals(const fg_op *op)
     return (long long)op->nitems * op->k * op->k;
 }
 
+int fg_ebe_get_values_dev(fg_op *op, double *A);
+
+extern "C" int fg_op_get_values_host(fg_op *op, double *vals)
+{
+    FG_ARG(op && (op->kind == FG_OP_EBE || op->kind == FG_OP_EDE), "EBE/EDE operator");
+    fg_ctx *c = op->ctx;
+    long long n = op_nvals(op);
+    if (n == 0) return FG_OK;
+    FG_ARG(vals, "vals");
+    double *tmp = nullptr;
+    FG_HIP(hipMalloc(&tmp, sizeof(double) * n));
+    int rc = fg_ebe_get_values_dev(op, tmp);
+    if (rc == FG_OK) {
+        hipError_t e = hipMemcpyAsync(vals, tmp, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                      c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            fg_set_error("fg_op_get_values_host: %s", hipGetErrorString(e));
+            rc = FG_ERR_HIP;
+        }
+    }
+    hipFree(tmp);
+    return rc;
+}
+
 extern "C" int fg_op_set_values_dev(fg_op *op, const double *vals)
 {
     FG_ARG(op, "op");
+    FG_ARG(op->kind != FG_OP_EBE2, "an ebe2 operator reads the values of its base operator");
     fg_ctx *c = op->ctx;
     long long n = op_nvals(op);
     if (n == 0) return FG_OK;
@@ -44,6 +70,7 @@ extern "C" int fg_op_set_values_dev(fg_op *op, const double *vals)
 extern "C" int fg_op_set_values_host(fg_op *op, const double *vals)
 {
     FG_ARG(op, "op");
+    FG_ARG(op->kind != FG_OP_EBE2, "an ebe2 operator reads the values of its base operator");
     fg_ctx *c = op->ctx;
     long long n = op_nvals(op);
     if (n == 0) return FG_OK;
@@ -60,6 +87,7 @@ extern "C" int fg_op_set_values_host(fg_op *op, const double *vals)
 int fg_op_apply_any(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 {
     if (op->kind == FG_OP_CSR) return fg_csr_apply(op, fp32, P, Q, skip);
+    if (op->kind == FG_OP_EBE2) return fg_ebe2_apply(op, fp32, P, Q, skip);
     if (fp32) return fg_ebe_apply_f32(op, (float *)P, (float *)Q, skip);
     return fg_ebe_apply_f64(op, (double *)P, (double *)Q, skip);
 }
@@ -100,6 +128,7 @@ extern "C" double fg_op_algorithmic_bytes(const fg_op *op, int fp32)
     if (!op) return 0.0;
     double s = fp32 ? 4.0 : 8.0;
     double neq = op->neq;
+    if (op->kind == FG_OP_EBE2) return fg_ebe2_bytes(op, fp32);
     if (op->kind == FG_OP_CSR)
         return (double)op->nnz * (s + 4.0) + 4.0 * (neq + 1.0) + 2.0 * neq * s;
     double n = op->nitems, k = op->k;
@@ -113,7 +142,7 @@ extern "C" int fg_op_time_apply(fg_op *op, int fp32, void *P, void *Q, int reps,
     FG_ARG(op && P && Q && reps > 0 && ms, "op/P/Q/reps/ms");
     fg_ctx *c = op->ctx;
     FG_CHECK(fg_op_apply_any(op, fp32, P, Q));   // warm
-    if (!main_only || op->kind == FG_OP_CSR) {
+    if (!main_only || op->kind == FG_OP_CSR || op->kind == FG_OP_EBE2) {
         FG_HIP(hipEventRecord(c->ev0, c->stream));
         for (int r = 0; r < reps; r++) FG_CHECK(fg_op_apply_any(op, fp32, P, Q));
         FG_HIP(hipEventRecord(c->ev1, c->stream));
@@ -144,7 +173,7 @@ extern "C" int fg_op_destroy(fg_op *op)
     if (!op) return FG_OK;
     hipStreamSynchronize(op->ctx->stream);
     if (op->kind == FG_OP_CSR) fg_csr_free(op);
-    else fg_ebe_free(op);
+    else if (op->kind != FG_OP_EBE2) fg_ebe_free(op);
     delete op;
     return FG_OK;
 }

@@ -5,6 +5,7 @@
 //         CSR/Diag_precond_CSR_setup.c:3-29.
 // Jacobi: EBE/Jacobi_precond_EBE.c:8-47 + EBE/Jacobi_precond_EBE_setup.c:4-43
 //         (element-local diagonal, k=3 element operators).
+// ILU(0): fg_ilu.hip.  Node block-diagonal (k = 12): fg_block.hip.
 #include "fg_internal.h"
 
 int fg_ebe_diag(fg_op *op, double *D);
@@ -45,6 +46,7 @@ extern "C" int fg_prec_create(fg_op *op, int kind, fg_prec **out)
                kind == FG_PREC_ILU0, "kind");
     FG_ARG(kind != FG_PREC_JACOBI || op->kind == FG_OP_EBE, "Jacobi needs an EBE operator");
     FG_ARG(kind != FG_PREC_ILU0 || op->kind == FG_OP_CSR, "ILU(0) needs a CSR operator");
+    FG_ARG(op->kind != FG_OP_EBE2, "bind preconditioners to the k = 12 EBE operator");
     FG_HIP(hipSetDevice(op->ctx->device));
     fg_prec *pc = new fg_prec();
     pc->op = op;
@@ -80,6 +82,7 @@ extern "C" int fg_prec_setup(fg_prec *pc, double *F)
     fg_ctx *c = op->ctx;
     int neq = op->neq;
     if (pc->kind == FG_PREC_NONE) return FG_OK;
+    if (pc->kind == FG_PREC_BLOCKDIAG) return fg_nodeblk_setup(pc, F);
     if (pc->kind == FG_PREC_ILU0) {
         // ILUp_precond_setup.c:53-56: factorise, then F <- (LU)^{-1} F
         FG_CHECK(fg_ilu_factor(pc->ilu, op));
@@ -129,6 +132,7 @@ int fg_prec_apply_any(fg_prec *pc, int fp32, const void *p, void *z, const int *
         return FG_OK;
     }
     if (pc->kind == FG_PREC_ILU0) return fg_ilu_apply(pc->ilu, c, fp32, p, z, skip);
+    if (pc->kind == FG_PREC_BLOCKDIAG) return fg_nodeblk_apply(pc, fp32, p, z, skip);
     // element Jacobi reads p[g] and writes z[g] in the same thread: in place is safe
     return fg_ebe_jacobi_apply(op, fp32, fp32 ? (const void *)pc->d_invde32 : pc->d_invde, p, z,
                                skip);
@@ -184,6 +188,7 @@ extern "C" int fg_prec_destroy(fg_prec *pc)
     hipFree(pc->d_invde); hipFree(pc->d_invde32); hipFree(pc->d_tmp64); hipFree(pc->d_tmp32);
     hipFree(pc->d_jsum32);
     fg_ilu_destroy(pc->ilu);
+    fg_nodeblk_destroy(pc->nb);
     delete pc;
     return FG_OK;
 }

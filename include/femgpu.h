@@ -161,6 +161,47 @@ int fg_prec_ilu_sweeps(fg_prec *pc, int sweeps);
  * unused), D[neq] inverted pivots */
 int fg_prec_ilu_get(fg_prec *pc, double *LU_host, double *D_host);
 
+/* Node block-diagonal of a 4-dof, 3-node element operator (k = 12, Euler
+ * 2D).  FG_PREC_BLOCKDIAG replaces BlockDiag_precond_EBE_setup.c:6-86 (setup)
+ * and BlockDiag_precond.c:3-27 (apply).  vertex[nel*3]: the element's nodes,
+ * in the element order of the operator's lm; ids[nnodes*4]: Node[].id of
+ * every node (< 0 or == neq: prescribed).  Setup sums the same-node 4x4
+ * blocks of the element matrices in element order (I_diag,
+ * BlockDiag_precond_EBE_setup.c:18-19), inverts every node block with the
+ * reference's closed-form cofactor expressions (matrix_Inverse, :88-145,
+ * entry 14 included as written there) and left-preconditions F when F is
+ * not NULL (F == NULL gives the block diagonal alone, as
+ * EBE/NOBlockDiag2_precond_EBE_setup.c does).  Apply: z_node = inv_node
+ * p_node; z may alias p; the sink slot reads as 0.  Nodes without a free
+ * dof are skipped (their blocks are left 0). */
+#define FG_PREC_BLOCKDIAG 4
+int fg_prec_blockdiag_create(fg_op *op, int nnodes, const int *vertex, const int *ids,
+                             fg_prec **out);
+/* BD, invBD to host, [nnodes*16] row-major node blocks (MatrixData->
+ * BlockDiagAux layout); either pointer may be NULL */
+int fg_prec_blockdiag_get(fg_prec *pc, double *BD_host, double *invBD_host);
+/* Block_scaling.c:10-59 Block_scaling_EBE: runs the block-diagonal setup
+ * (F left-preconditioned, F may be NULL), then rewrites the operator's
+ * element matrices in place: the prescribed dofs of each node removed
+ * (matrix_Bondary_Conditions, :73-92) and every node block row
+ * left-multiplied by that node's inverse block (matrix_Scaling, :95-118).
+ * The block rows of nodes without a free dof (they only feed the sink slot)
+ * become 0. */
+int fg_block_scaling_ebe(fg_prec *blockdiag, double *F);
+/* ebe2mvNDOF4 (EBE/ebe2mvNDOF4.c:4-79): a product operator over the
+ * element matrices of `ebe12` that skips each row node's own 4x4 block and
+ * then applies BlockDiag2_precond.c:3-27 (form FG_EBE2_BLOCKDIAG:
+ * Q = P + invBD Q, node by node) or NOBlockDiag2_precond.c:3-25 (form
+ * FG_EBE2_NOBLOCKDIAG: Q += BD P), with the blocks of `blockdiag` (set up
+ * first).  The new operator shares ebe12's storage and blockdiag's blocks;
+ * destroy it before them.  It can drive fg_gmres like any operator. */
+#define FG_EBE2_BLOCKDIAG   0
+#define FG_EBE2_NOBLOCKDIAG 1
+int fg_op_ebe2_create(fg_op *ebe12, fg_prec *blockdiag, int form, fg_op **out);
+/* element matrices back to host, the fg_op_set_values_host layout (EBE/EDE
+ * operators; elements whose dofs are all prescribed are not stored and read 0) */
+int fg_op_get_values_host(fg_op *op, double *vals);
+
 /* ---- mesh structures (host, once per mesh) -------------------------------- */
 /* Node numbering Preprocess.c:73-81 (id = neq++ for Type>=1, else -1) and
  * Fill_LM.c:3-24 (lm[e*nnoel+j] = id of vertex j, or neq).  Returns neq. */

@@ -142,6 +142,37 @@ class Oracle:
         L.orc_build_supg3d.argtypes = [_i, _i, _dp, _dp, _dp, _ip, _ip, _ip,
                                        C.POINTER(OrcProb3d), _dp, _dp]
         L.orc_symrcm.argtypes = [_i, _ip, _ip, _ip]
+        L.orc_ebe2mv_ndof4.argtypes = [_i, _i, _dp, _ip, _i, _ip, _vp, _vp, _dp, _dp]
+        L.orc_matrix_inverse4.argtypes = [_dp, _dp]
+        L.orc_blockdiag_apply.argtypes = [_i, _i, _ip, _dp, _dp, _dp]
+        L.orc_blockdiag_setup_ebe.argtypes = [_i, _i, _i, _ip, _dp, _ip, _dp, _dp, _vp]
+        L.orc_block_scaling_ebe.argtypes = [_i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
+
+    def block_scaling_ebe(self, neq, nel, nnodes, vertex, ids, Id, A, F):
+        A2 = np.ascontiguousarray(A, np.float64).copy()
+        F2 = np.ascontiguousarray(F, np.float64).copy()
+        BD, iBD = np.zeros(16 * nnodes), np.zeros(16 * nnodes)
+        self.lib.orc_block_scaling_ebe(neq, nel, nnodes, vertex, ids, Id, A2, BD, iBD, F2)
+        return A2, F2, BD, iBD
+
+    def ebe2mv_ndof4(self, neq, nel, A, lm, nnodes, Id, P, invBD=None, BD=None):
+        Q = np.zeros(neq + 1)
+        self.lib.orc_ebe2mv_ndof4(neq, nel, A, lm, nnodes, Id,
+                                  _ptr(invBD) if invBD is not None else None,
+                                  _ptr(BD) if BD is not None else None, P, Q)
+        return Q
+
+    def blockdiag_setup_ebe(self, neq, nel, nnodes, vertex, A, Id, F=None):
+        BD, iBD = np.zeros(16 * nnodes), np.zeros(16 * nnodes)
+        Fo = None if F is None else np.ascontiguousarray(F, np.float64).copy()
+        self.lib.orc_blockdiag_setup_ebe(neq, nel, nnodes, vertex, A, Id, BD, iBD,
+                                         _ptr(Fo) if Fo is not None else None)
+        return BD, iBD, Fo
+
+    def blockdiag_apply(self, neq, nnodes, Id, invBD, p):
+        z = np.zeros(neq + 1)
+        self.lib.orc_blockdiag_apply(neq, nnodes, Id, invBD, np.ascontiguousarray(p), z)
+        return z
 
     def symrcm(self, neq, IA, JA):
         P = np.zeros(max(neq, 1), np.int32)
@@ -499,3 +530,56 @@ class Reference3D:
         z = np.zeros(self.neq_ilu + 1)
         self.lib.ref3d_ilu0_apply(np.ascontiguousarray(p, np.float64).copy(), z)
         return z
+
+
+class ReferenceEuler:
+    """The reference's 4-dof (Euler 2D) element products and node block-
+    diagonal routines compiled from their sources (oracle/_ref)."""
+
+    PATH = os.path.join(HERE, "_ref", "libref_eu2d.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.exists(cls.PATH)
+
+    def __init__(self):
+        L = self.lib = C.CDLL(self.PATH)
+        L.refeu_setup.argtypes = [_i, _i, _i, _ip, _ip, _ip, _dp, C.c_char_p]
+        L.refeu_ebemv.argtypes = [_dp, _dp]
+        L.refeu_ebe2mv.argtypes = [_dp, _dp]
+        L.refeu_blockdiag_setup.argtypes = [_dp, _dp, _dp, _ip]
+        L.refeu_blockdiag_apply.argtypes = [_dp, _dp]
+        L.refeu_block_scaling.argtypes = [_dp, _dp]
+
+    def setup(self, nnodes, nel, neq, vertex, ids, lm, A, precond="BlockDiag"):
+        self.nnodes, self.nel, self.neq = nnodes, nel, neq
+        self.lib.refeu_setup(nnodes, nel, neq, vertex, ids, lm, np.ascontiguousarray(A),
+                             precond.encode())
+
+    def ebemv(self, P):
+        Q = np.zeros(self.neq + 1)
+        self.lib.refeu_ebemv(np.ascontiguousarray(P, np.float64).copy(), Q)
+        return Q
+
+    def ebe2mv(self, P):
+        Q = np.zeros(self.neq + 1)
+        self.lib.refeu_ebe2mv(np.ascontiguousarray(P, np.float64).copy(), Q)
+        return Q
+
+    def blockdiag_setup(self, F):
+        Fp = np.ascontiguousarray(F, np.float64).copy()
+        BD, iBD = np.zeros(16 * self.nnodes), np.zeros(16 * self.nnodes)
+        Id = np.zeros(4 * self.nnodes, np.int32)
+        self.lib.refeu_blockdiag_setup(Fp, BD, iBD, Id)
+        return Fp, BD, iBD, Id
+
+    def blockdiag_apply(self, p):
+        z = np.zeros(self.neq + 1)
+        self.lib.refeu_blockdiag_apply(np.ascontiguousarray(p, np.float64).copy(), z)
+        return z
+
+    def block_scaling(self, F):
+        Fp = np.ascontiguousarray(F, np.float64).copy()
+        A = np.zeros(144 * self.nel)
+        self.lib.refeu_block_scaling(Fp, A)
+        return A, Fp

@@ -1214,3 +1214,167 @@ void orc_symrcm(int N, const int *ia, const int *ja, int *P)
     }
     free(D); free(cidx2); free(ridx2);
 }
+
+/* ======================================================================== */
+/* 4-dof (Euler) element products and node block-diagonal preconditioners   */
+/* ======================================================================== */
+
+/* EBE/ebe2mvNDOF4.c:4-79 (2D): the 12x12 element product without the
+ * same-node 4x4 blocks, then BlockDiag2_precond.c (invBD != NULL) or
+ * NOBlockDiag2_precond.c (BD != NULL).  Id[nnodes*4]: equation of dof d of
+ * node I (sink = neq). */
+void orc_ebe2mv_ndof4(int neq, int nel, const double *A, const int *lm, int nnodes,
+                      const int *Id, const double *invBD, const double *BD, double *P,
+                      double *Q)
+{
+    for (int r = 0; r < neq; r++) Q[r] = 0;
+    P[neq] = 0;
+    Q[neq] = 0;
+    for (long e = 0; e < nel; e++) {
+        const int *le = lm + 12 * e;
+        const double *Ae = A + 144 * e;
+        double p[12];
+        for (int c = 0; c < 12; c++) p[c] = P[le[c]];
+        for (int r = 0; r < 12; r++) {
+            const int nb = r / 4;     /* node block of the row: skip its columns */
+            double s = 0;
+            int first = 1;
+            for (int c = 0; c < 12; c++) {
+                if (c / 4 == nb) continue;
+                double t = Ae[12 * r + c] * p[c];
+                if (first) { s = t; first = 0; }
+                else s = s + t;
+            }
+            Q[le[r]] += s;
+        }
+        Q[neq] = 0.0;
+    }
+    for (int I = 0; I < nnodes; I++) {
+        const int *id = Id + 4 * I;
+        if (invBD) {   /* BlockDiag2_precond.c:11-20 (z = Q here) */
+            const double *M = invBD + 16 * I;
+            double a = Q[id[0]], b = Q[id[1]], c = Q[id[2]], d = Q[id[3]];
+            for (int r = 0; r < 4; r++)
+                Q[id[r]] = P[id[r]] + M[4 * r] * a + M[4 * r + 1] * b + M[4 * r + 2] * c +
+                           M[4 * r + 3] * d;
+        } else if (BD) {   /* NOBlockDiag2_precond.c:11-20 */
+            const double *M = BD + 16 * I;
+            double a = P[id[0]], b = P[id[1]], c = P[id[2]], d = P[id[3]];
+            for (int r = 0; r < 4; r++)
+                Q[id[r]] += M[4 * r] * a + M[4 * r + 1] * b + M[4 * r + 2] * c + M[4 * r + 3] * d;
+        }
+        Q[neq] = 0;
+    }
+}
+
+/* BlockDiag_precond_EBE_setup.c:88-145 matrix_Inverse: cofactor inverse of a
+ * row-major 4x4 block, the reference's expression grouping */
+void orc_matrix_inverse4(const double *B, double *inv)
+{
+    const double a = B[0], b = B[1], c = B[2], d = B[3], e = B[4], f = B[5], g = B[6],
+                 h = B[7], i = B[8], j = B[9], k = B[10], l = B[11], m = B[12], n = B[13],
+                 o = B[14], p = B[15];
+    const double det = (a*f*k*p - a*f*l*o - a*g*j*p + a*g*l*n + a*h*j*o - a*h*k*n
+                        -b*e*k*p  + b*e*l*o + b*g*i*p - b*g*l*m - b*h*i*o + b*h*k*m +
+                        c*(e*j*p - e*l*n - f*i*p + f*l*m + h*i*n - h*j*m) +
+                        d*(-e*j*o + e*k*n + f*i*o - f*k*m - g*i*n + g*j*m));
+    const double invdet = 1.0/det;
+    inv[0] = invdet*(-h*k*n + g*l*n + h*j*o - f*l*o - g*j*p + f*k*p);
+    inv[1] = invdet*(d*k*n - c*l*n - d*j*o + b*l*o + c*j*p - b*k*p);
+    inv[2] = invdet*(-d*g*n + c*h*n + d*f*o - b*h*o - c*f*p + b*g*p);
+    inv[3] = invdet*(d*g*j - c*h*j - d*f*k + b*h*k + c*f*l - b*g*l);
+    inv[4] = invdet*(h*k*m - g*l*m - h*i*o + e*l*o + g*i*p - e*k*p);
+    inv[5] = invdet*(-d*k*m + c*l*m + d*i*o - a*l*o - c*i*p + a*k*p);
+    inv[6] = invdet*(d*g*m - c*h*m - d*e*o + a*h*o + c*e*p - a*g*p);
+    inv[7] = invdet*(-d*g*i + c*h*i + d*e*k - a*h*k - c*e*l + a*g*l);
+    inv[8] = invdet*(-h*j*m + f*l*m + h*i*n - e*l*n - f*i*p + e*j*p);
+    inv[9] = invdet*(d*j*m - b*l*m - d*i*n + a*l*n + b*i*p - a*j*p);
+    inv[10] = invdet*(-d*f*m + b*h*m + d*e*n - a*h*n - b*e*p + a*f*p);
+    inv[11] = invdet*(d*f*i - b*h*i - d*e*j + a*h*j + b*e*l - a*f*l);
+    inv[12] = invdet*(g*j*m - f*k*m - g*i*n + e*k*n + f*i*o - e*j*o);
+    inv[13] = invdet*(-c*j*m + b*k*m + c*i*n - a*k*n - b*i*o + a*j*o);
+    inv[14] = invdet*(c*j*m - b*g*m - c*e*n + a*g*n + b*e*o - a*f*o);
+    inv[15] = invdet*(-c*f*i + b*g*i + c*e*j - a*g*j - b*e*k + a*f*k);
+}
+
+/* BlockDiag_precond.c:3-27: z_node = invBD_node * p_node, z[neq] = 0; z may alias p */
+void orc_blockdiag_apply(int neq, int nnodes, const int *Id, const double *invBD,
+                         const double *p, double *z)
+{
+    for (int I = 0; I < nnodes; I++) {
+        const int *id = Id + 4 * I;
+        const double *M = invBD + 16 * I;
+        double a = p[id[0]], b = p[id[1]], c = p[id[2]], d = p[id[3]];
+        for (int r = 0; r < 4; r++)
+            z[id[r]] = M[4 * r] * a + M[4 * r + 1] * b + M[4 * r + 2] * c + M[4 * r + 3] * d;
+        z[neq] = 0;
+    }
+}
+
+/* BlockDiag_precond_EBE_setup.c:6-86 (2D, 3 nodes of 4 dofs): node blocks
+ * summed in element order from the same-node 4x4 blocks of the 12x12 element
+ * matrices (index table I_diag, :18-19), cofactor inverses, F <- BD^{-1} F.
+ * vertex[nel*3]; BD, invBD [nnodes*16] (BD zeroed first). */
+void orc_blockdiag_setup_ebe(int neq, int nel, int nnodes, const int *vertex, const double *A,
+                             const int *Id, double *BD, double *invBD, double *F)
+{
+    memset(BD, 0, sizeof(double) * 16 * (size_t)nnodes);
+    for (long E = 0; E < nel; E++)
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++)
+                for (int v = 0; v < 3; v++) {
+                    const int node = vertex[3 * E + v];
+                    /* I_diag[v][4r+c] = 52 v + 12 r + c */
+                    BD[16 * node + 4 * r + c] += A[144 * E + 52 * v + 12 * r + c];
+                }
+    for (int I = 0; I < nnodes; I++) orc_matrix_inverse4(BD + 16 * I, invBD + 16 * I);
+    if (F) orc_blockdiag_apply(neq, nnodes, Id, invBD, F, F);
+}
+
+/* Block_scaling.c:10-59 Block_scaling_EBE (2D, NDOF 4): the block-diagonal
+ * setup (F preconditioned), then per element the nine 4x4 blocks A_vw with
+ * the prescribed dofs of each node removed (matrix_Bondary_Conditions,
+ * :73-92: row and column zeroed, 1 on the diagonal of its own block) and
+ * every block row v left-multiplied by invBD of its node (matrix_Scaling,
+ * :95-118, sum over j ascending from 0).  ids[nnodes*4] = Node[].id (-1 =
+ * prescribed).  A is scaled in place; BD/invBD as orc_blockdiag_setup_ebe. */
+void orc_block_scaling_ebe(int neq, int nel, int nnodes, const int *vertex, const int *ids,
+                           const int *Id, double *A, double *BD, double *invBD, double *F)
+{
+    orc_blockdiag_setup_ebe(neq, nel, nnodes, vertex, A, Id, BD, invBD, F);
+    for (long E = 0; E < nel; E++) {
+        double *Ae = A + 144 * E;
+        const int *V = vertex + 3 * E;
+        double M[3][3][4][4];   /* M[v][w][i][j] = Ae[12(4v+i) + 4w+j] */
+        for (int v = 0; v < 3; v++)
+            for (int w = 0; w < 3; w++)
+                for (int i = 0; i < 4; i++)
+                    for (int j = 0; j < 4; j++) M[v][w][i][j] = Ae[12 * (4 * v + i) + 4 * w + j];
+        /* boundary conditions, Block_scaling.c:46-50 call order */
+        for (int I = 0; I < 4; I++)
+            for (int v = 0; v < 3; v++) {
+                if (ids[4 * V[v] + I] >= 0) continue;
+                for (int i = 0; i < 4; i++) {
+                    M[v][v][I][i] = 0;
+                    M[v][v][i][I] = 0;
+                    for (int w = 0; w < 3; w++)
+                        if (w != v) {
+                            M[v][w][I][i] = 0;   /* row I of v's off-diagonal blocks */
+                            M[w][v][i][I] = 0;   /* column I of the blocks into v */
+                        }
+                }
+                M[v][v][I][I] = 1;
+            }
+        /* scaling of block row v by invBD of its node */
+        for (int v = 0; v < 3; v++) {
+            const double *iD = invBD + 16 * V[v];
+            for (int i = 0; i < 4; i++)
+                for (int k = 0; k < 4; k++)
+                    for (int w = 0; w < 3; w++) {
+                        double s = 0;
+                        for (int j = 0; j < 4; j++) s += iD[4 * i + j] * M[v][w][j][k];
+                        Ae[12 * (4 * v + i) + 4 * w + k] = s;
+                    }
+        }
+    }
+}

@@ -174,6 +174,25 @@ void orc_ilu0_apply(int neq, const double *LU, const int *JA, const int *IA, con
 void orc_ilu0_apply_f32(int neq, const float *LU, const int *JA, const int *IA, const float *D,
                         const float *p, float *z);
 
+/* ---------------- 4-dof (Euler) products and node block preconditioners --- */
+/* EBE/ebe2mvNDOF4.c:4-79 with BlockDiag2_precond.c (invBD != NULL) or
+ * NOBlockDiag2_precond.c (BD != NULL); Id[nnodes*4] = equations of a node */
+void orc_ebe2mv_ndof4(int neq, int nel, const double *A, const int *lm, int nnodes,
+                      const int *Id, const double *invBD, const double *BD, double *P,
+                      double *Q);
+/* BlockDiag_precond_EBE_setup.c:88-145 matrix_Inverse (row-major 4x4) */
+void orc_matrix_inverse4(const double *B, double *inv);
+/* BlockDiag_precond.c:3-27 */
+void orc_blockdiag_apply(int neq, int nnodes, const int *Id, const double *invBD,
+                         const double *p, double *z);
+/* BlockDiag_precond_EBE_setup.c:6-86: BD/invBD [nnodes*16], F <- BD^{-1} F */
+void orc_blockdiag_setup_ebe(int neq, int nel, int nnodes, const int *vertex, const double *A,
+                             const int *Id, double *BD, double *invBD, double *F);
+/* Block_scaling.c:10-59 Block_scaling_EBE: A (nel*144) scaled in place,
+ * ids[nnodes*4] = Node[].id (-1 prescribed) */
+void orc_block_scaling_ebe(int neq, int nel, int nnodes, const int *vertex, const int *ids,
+                           const int *Id, double *A, double *BD, double *invBD, double *F);
+
 /* ---------------- reordering --------------------------------------------- */
 /* Reordering/symrcm.c:4-242 REORDERING_SYMRCM: reverse Cuthill-McKee of a
  * CSR pattern (read as CCS, as the reference does).  P[c] = old index of

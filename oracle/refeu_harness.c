@@ -1,0 +1,82 @@
+/*
+ * refeu_harness.c -- drives the FEM_CODES 4-dof (Euler, 2D) element
+ * product and node block-diagonal routines of 00_CommonFiles, compiled from
+ * their own sources with the Euler driver's header (-DEulerEquations2D) by
+ * oracle/Makefile, through a flat C interface.  TEST INFRASTRUCTURE ONLY:
+ * built into oracle/_ref/, never linked into the product.
+ *
+ * The caller supplies the mesh (3 vertices per element), the equation number
+ * of every dof of every node (Node[].id[4], -1 = prescribed), lm and the
+ * 12x12 element matrices; the harness lays them out as the Euler driver does
+ * (MatrixData->A row pointers into one Aaux block).
+ */
+#include "EulerEquations.h"
+#include "../../../00_CommonFiles/Solvers_and_Preconditioners/preconditioners.h"
+
+int ebemvNDOF4(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
+int ebe2mvNDOF4(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
+int Block_scaling_EBE(ParametersType *, MatrixDataType *, FemStructsType *);
+
+static ParametersType PE;
+static MatrixDataType ME;
+static FemStructsType FE;
+
+void refeu_setup(int nnodes, int nel, int neq, const int *vertex, const int *ids,
+                 const int *lm, const double *A, const char *precond)
+{
+    memset(&PE, 0, sizeof PE); memset(&ME, 0, sizeof ME); memset(&FE, 0, sizeof FE);
+    PE.nnodes = nnodes;
+    PE.nel = nel;
+    PE.neq = neq;
+    strcpy(PE.Preconditioner, precond);
+    NodeType *Node = (NodeType *)calloc(nnodes, sizeof(NodeType));
+    for (int i = 0; i < nnodes; i++)
+        for (int d = 0; d < 4; d++) Node[i].id[d] = ids[4 * i + d];
+    ElementType *El = (ElementType *)calloc(nel, sizeof(ElementType));
+    for (int e = 0; e < nel; e++)
+        for (int v = 0; v < 3; v++) El[e].Vertex[v] = vertex[3 * e + v];
+    int **L = (int **)calloc(nel, sizeof(int *));
+    int *Laux = (int *)calloc((size_t)nel * 12, sizeof(int));
+    memcpy(Laux, lm, sizeof(int) * 12 * (size_t)nel);
+    for (int e = 0; e < nel; e++) L[e] = &Laux[12 * e];
+    double **Ar = (double **)calloc(nel, sizeof(double *));
+    double *Aaux = (double *)calloc((size_t)nel * 144, sizeof(double));
+    memcpy(Aaux, A, sizeof(double) * 144 * (size_t)nel);
+    for (int e = 0; e < nel; e++) Ar[e] = &Aaux[144 * e];
+    FE.Node = Node;
+    FE.Element = El;
+    FE.lm = L;
+    FE.lmaux = Laux;
+    FE.F = (double *)calloc(neq + 1, sizeof(double));
+    ME.A = Ar;
+    ME.Aaux = Aaux;
+}
+
+void refeu_ebemv(double *P, double *Q) { ebemvNDOF4(&PE, &ME, &FE, P, Q); }
+void refeu_ebe2mv(double *P, double *Q) { ebe2mvNDOF4(&PE, &ME, &FE, P, Q); }
+
+/* BlockDiag_precond_EBE_setup.c (tag 1): BD, invBD [nnodes*16], Id
+ * [nnodes*4], F (neq+1) preconditioned in place */
+void refeu_blockdiag_setup(double *F, double *BD, double *invBD, int *Id)
+{
+    BlockDiag_precond_EBE_setup(&PE, &ME, &FE, 1, F);
+    memcpy(BD, ME.BlockDiagAux, sizeof(double) * 16 * (size_t)PE.nnodes);
+    /* the setup keeps only the row pointers of the inverse blocks */
+    for (int i = 0; i < PE.nnodes; i++)
+        memcpy(invBD + 16 * i, ME.invBlockDiag[i], sizeof(double) * 16);
+    for (int i = 0; i < PE.nnodes; i++)
+        for (int d = 0; d < 4; d++) Id[4 * i + d] = ME.Id[i][d];
+}
+
+void refeu_blockdiag_apply(double *p, double *z) { BlockDiag_precond(&PE, &ME, &FE, p, z); }
+
+/* Block_scaling.c:10-59 Block_scaling_EBE (iterations = 0: the setup runs
+ * with tag 1); F (neq+1) in/out, A (nel*144) out */
+void refeu_block_scaling(double *F, double *A)
+{
+    PE.iterations = 0;
+    memcpy(FE.F, F, sizeof(double) * (PE.neq + 1));
+    Block_scaling_EBE(&PE, &ME, &FE);
+    memcpy(F, FE.F, sizeof(double) * (PE.neq + 1));
+    memcpy(A, ME.Aaux, sizeof(double) * 144 * (size_t)PE.nel);
+}
