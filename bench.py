@@ -10,11 +10,12 @@ Workload (the 2D SUPG steady-transport system at its single-GPU size):
     mesh (ebemv, MatrixVector_Operations/EBE/ebemv.c:4-43), inputs resident
     in HBM; `value` = algorithmic bytes / time in GB/s;
   * the same product in FP32 and with the EDE and CSR storage schemes;
-  * time to relative residual 1e-8 of the Diag-preconditioned GMRES(30),
-    FP64 vs FP32-IR, on the constant-velocity SUPG system (beta=(cos t,
-    sin t), k=1e-4, f=1) at 2048^2 and at 64^2.  The rotating system above
-    does not converge with GMRES(30)+Jacobi in a practical number of steps,
-    so it is used for the products only.
+  * time to relative residual 1e-8 of the element-Jacobi-preconditioned
+    GMRES(30) (--prec diag: the global diagonal), FP64 vs FP32-IR, on the
+    constant-velocity SUPG system (beta=(cos t, sin t), k=1e-4, f=1) at
+    2048^2 and at 64^2.  The rotating system above does not converge with
+    GMRES(30)+Jacobi in a practical number of steps (the element Jacobi
+    stagnates on it already at 16^2), so it is used for the products only.
   * cpu_baseline: the reference ebemv compiled from its own sources
     (oracle/_ref, -Ofast) or, where that library is absent, the oracle
     restatement, timed on one host core over the full mesh.
@@ -160,8 +161,12 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--engine", default="kernels", choices=sorted(ENGINES),
                     help="GMRES engine for the time-to-solve runs")
-    ap.add_argument("--prec", default="diag", choices=("diag", "jacobi"),
+    ap.add_argument("--prec", default="jacobi", choices=("diag", "jacobi"),
                     help="preconditioner of the time-to-solve runs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the products, run a cast kernel of known traffic (64Mi "
+                         "doubles read, 64Mi floats written) three times, the scale of "
+                         "the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--orth", default="cgs", choices=sorted(ORTHS),
                     help="orthogonalisation of the FP32-IR inner cycle (FP64 is always MGS)")
     ap.add_argument("--matvec-only", action="store_true",
@@ -224,6 +229,21 @@ def main():
                                               bytes=b)
         matvec[name] = r
         log(name, {k: round(v["GBs"], 1) for k, v in r.items()})
+
+    copy = None
+    if args.calibrate_copy:
+        m = 64 << 20
+        X = ctx.to_device(np.ones(m))
+        Y = ctx.empty(m, np.float32)
+        ctx.sync()
+        c0 = time.perf_counter()
+        for _ in range(3):
+            F.lib.fg_cast_f64_to_f32(ctx.h, m, X.ptr, Y.ptr)
+        ctx.sync()
+        cs = (time.perf_counter() - c0) / 3
+        copy = dict(bytes=12 * m, ms=cs * 1e3, GBs=12 * m / cs / 1e9)
+        log("copy calibration", copy)
+        del X, Y
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ebe_f64_2048_traffic.json")
@@ -294,12 +314,13 @@ def main():
                    "wall_ms_per_step": wall * 1e3 / args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_GBS, "traffic": traffic,
-                     "kernel": (f"k_ebe_apply_pipe<3,{best_epb // 256},double>" if best_pipe
-                                else f"k_ebe_apply<3,{best_epb // 256},double>"),
+                     "kernel": (f"k_ebe_apply_pipe<3, {best_epb // 256}, double, 0>"
+                                if best_pipe else f"k_ebe_apply<3, {best_epb // 256}, double, 0>"),
                      "schedule": {"elems_per_block": best_epb, "wg_per_cu": best_pipe},
                      "algorithmic_bytes": bytes64, "ms": ms_main},
         "cpu_baseline": cpu,
         "matvec": matvec,
+        "copy_calibration": copy,
         "ebe_autotune_ms": {f"{k[0]}/{k[1]}": v for k, v in tune_ms.items()},
         "solve": solve,
     }

@@ -174,9 +174,12 @@ int build_plan(int neq, int nel, int k, const int *lm, int eb, Plan &pl)
             }
         for (int l = 0; l < nloc; l++) icnt[l + 1] += icnt[l];
         const int ninc = icnt[nloc];
-        pl.max_ninc = std::max(pl.max_ninc, ninc);
+        // each block's list starts on a 16-byte boundary (copied to LDS with
+        // 16-byte loads); the padding entries are never read as slots
+        const int ninc_pad = (ninc + 7) / 8 * 8;
+        pl.max_ninc = std::max(pl.max_ninc, ninc_pad);
         const size_t ib = pl.inc.size();
-        pl.inc.resize(ib + ninc);
+        pl.inc.resize(ib + ninc_pad, 0);
         {
             std::vector<int> fp(icnt.begin(), icnt.end() - 1);
             for (int i = 0; i < ne; i++)
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_apply_pipe(FgEbeLayout L,
     for (;;) {
         const int b = fg_xcd_remap(s, nblk);
         const int sn = s + (int)gridDim.x;
-        fg_ebe_gather<K, T>(b, L, P, smem_raw);
+        fg_ebe_gather<K, EPT, T>(b, L, P, smem_raw, cur);
         if (sn < nblk) fg_ebe_load<K, EPT, T>(fg_xcd_remap(sn, nblk), L, Kv, nxt);
         fg_ebe_finish<K, EPT, T, BS>(b, L, Kv, Q, hpart, smem_raw, cur);
         if (sn >= nblk) break;

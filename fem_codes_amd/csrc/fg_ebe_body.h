@@ -44,36 +44,63 @@ inline FgEbeLayout fg_ebe_layout(const fg_op *op)
                        op->d_ibase, op->d_lml, op->d_ioff, op->d_inc};
 }
 
-// LDS bytes one block needs
+// LDS bytes one block needs (ninc: the block's incidence count padded to a
+// multiple of 8): incidence lists and offsets first (16-byte aligned), then
+// P values and element row sums
+__host__ __device__ constexpr size_t fg_ebe_smem_idx_bytes(int nd, int ninc)
+{
+    return (2 * ((size_t)ninc + nd + 1) + 15) / 16 * 16;
+}
 template <typename T>
 __host__ __device__ constexpr size_t fg_ebe_smem_bytes(int k, int nd, int ne, int ninc)
 {
-    return sizeof(T) * ((size_t)nd + 1 + (size_t)k * ne) + 2 * ((size_t)ninc + nd + 1);
+    return fg_ebe_smem_idx_bytes(nd, ninc) + sizeof(T) * ((size_t)nd + 1 + (size_t)k * ne);
+}
+
+// x[i] through a block-uniform base (scalar registers) and a 32-bit byte
+// offset, so each load or store needs one VGPR of address rather than a
+// 64-bit pair.  Requires i * sizeof(T) < 2^32: block-local arrays, and the
+// global vectors of neq + 1 < 2^28 entries (checked at operator creation).
+template <typename T>
+__device__ __forceinline__ const T &fg_at(const T *x, unsigned i)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(x) +
+                                        i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T &fg_at(T *x, unsigned i)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(x) + i * (unsigned)sizeof(T));
 }
 
 template <int K, int EPT, typename T>
 struct FgEbeRegs {
     static constexpr bool PRE = EPT * K * K <= 72;   // element values held in registers
+    // block dofs per thread whose global id and halo slot are kept in
+    // registers from the gather to the end of the block (more are reloaded)
+    static constexpr int ND = 2;
     uint16_t li[EPT][K];
     T kv[PRE ? EPT : 1][PRE ? K * K : 1];
+    int gd[ND], sd[ND];
 };
 
 template <int K, int EPT, typename T>
 __device__ __forceinline__ void fg_ebe_load(int b, const FgEbeLayout &L, const T *__restrict__ Kv,
                                             FgEbeRegs<K, EPT, T> &r)
 {
-    const int e0 = L.eoff[b], ne = L.eoff[b + 1] - e0;
+    const int e0 = L.eoff[b];
+    const unsigned ne = (unsigned)(L.eoff[b + 1] - e0);
     const T *Kb = Kv + (size_t)K * K * e0;
     const uint16_t *Lb = L.lml + (size_t)K * e0;
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
-        const int el = threadIdx.x + i * FG_TPB;
+        const unsigned el = threadIdx.x + i * FG_TPB;
         if (el < ne) {
 #pragma unroll
-            for (int j = 0; j < K; j++) r.li[i][j] = Lb[(size_t)j * ne + el];
+            for (int j = 0; j < K; j++) r.li[i][j] = fg_at(Lb, j * ne + el);
             if constexpr (FgEbeRegs<K, EPT, T>::PRE) {
 #pragma unroll
-                for (int q = 0; q < K * K; q++) r.kv[i][q] = Kb[(size_t)q * ne + el];
+                for (int q = 0; q < K * K; q++) r.kv[i][q] = fg_at(Kb, q * ne + el);
             }
         }
     }
@@ -86,31 +113,49 @@ struct FgEbeSmem {
     uint16_t *sInc, *sIoff;
     __device__ __forceinline__ FgEbeSmem(unsigned char *smem, int nd, int ne, int ninc)
     {
-        sP = reinterpret_cast<T *>(smem);
-        sY = sP + nd + 1;
-        sInc = reinterpret_cast<uint16_t *>(sY + (size_t)K * ne);
+        sInc = reinterpret_cast<uint16_t *>(smem);
         sIoff = sInc + ninc;
+        sP = reinterpret_cast<T *>(smem + fg_ebe_smem_idx_bytes(nd, ninc));
+        sY = sP + nd + 1;
+        (void)ne;
     }
 };
 
-template <int K, typename T>
+template <int K, int EPT, typename T>
 __device__ __forceinline__ void fg_ebe_gather(int b, const FgEbeLayout &L,
-                                              const T *__restrict__ P, unsigned char *smem)
+                                              const T *__restrict__ P, unsigned char *smem,
+                                              FgEbeRegs<K, EPT, T> &r)
 {
     const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
     const int e0 = L.eoff[b], ne = L.eoff[b + 1] - e0;
     const int i0 = L.ibase[b], ninc = L.ibase[b + 1] - i0;
     FgEbeSmem<K, T> S(smem, nd, ne, ninc);
     const uint16_t *io = L.ioff + d0 + b;
-    for (int l = threadIdx.x; l < nd; l += FG_TPB) {
-        S.sP[l] = P[L.dofs[d0 + l]];
-        S.sIoff[l] = io[l];
+    const int *Db = L.dofs + d0, *Sb = L.slots + d0;
+    // incidence lists: 16-byte aligned, padded to 8 entries (see the plan)
+    const uint4 *Ib = reinterpret_cast<const uint4 *>(L.inc + i0);
+    uint4 *sInc4 = reinterpret_cast<uint4 *>(S.sInc);
+    for (unsigned q = threadIdx.x; (int)q < (ninc >> 3); q += FG_TPB) sInc4[q] = fg_at(Ib, q);
+    constexpr int ND = FgEbeRegs<K, EPT, T>::ND;
+#pragma unroll
+    for (int u = 0; u < ND; u++) {
+        const unsigned l = threadIdx.x + u * FG_TPB;
+        if ((int)l < nd) {
+            const int g = fg_at(Db, l);
+            r.gd[u] = g;
+            r.sd[u] = fg_at(Sb, l);
+            S.sP[l] = fg_at(P, (unsigned)g);
+            S.sIoff[l] = fg_at(io, l);
+        }
+    }
+    for (unsigned l = threadIdx.x + ND * FG_TPB; (int)l < nd; l += FG_TPB) {
+        S.sP[l] = fg_at(P, (unsigned)fg_at(Db, l));
+        S.sIoff[l] = fg_at(io, l);
     }
     if (threadIdx.x == 0) {
         S.sP[nd] = T(0);
         S.sIoff[nd] = io[nd];
     }
-    for (int q = threadIdx.x; q < ninc; q += FG_TPB) S.sInc[q] = L.inc[i0 + q];
     __syncthreads();
 }
 
@@ -126,14 +171,16 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
                                               T *__restrict__ hpart, unsigned char *smem,
                                               const FgEbeRegs<K, EPT, T> &r)
 {
-    const int e0 = L.eoff[b], ne = L.eoff[b + 1] - e0;
+    const int e0 = L.eoff[b];
+    const unsigned ne = (unsigned)(L.eoff[b + 1] - e0);
     const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
     const int ninc = L.ibase[b + 1] - L.ibase[b];
     FgEbeSmem<K, T> S(smem, nd, ne, ninc);
     const T *Kb = Kv + (size_t)K * K * e0;
+    const int *Db = L.dofs + d0, *Sb = L.slots + d0;
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
-        const int el = threadIdx.x + i * FG_TPB;
+        const unsigned el = threadIdx.x + i * FG_TPB;
         if (el < ne) {
             T p[K];
 #pragma unroll
@@ -148,26 +195,40 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
                     for (int c = c0 + 1; c < K; c++)
                         if (BS == 0 || c / BS != rr / BS) acc = acc + r.kv[i][rr * K + c] * p[c];
                 } else {
-                    acc = Kb[(size_t)(rr * K + c0) * ne + el] * p[c0];
+                    acc = fg_at(Kb, (rr * K + c0) * ne + el) * p[c0];
 #pragma unroll
                     for (int c = c0 + 1; c < K; c++)
                         if (BS == 0 || c / BS != rr / BS)
-                            acc = acc + Kb[(size_t)(rr * K + c) * ne + el] * p[c];
+                            acc = acc + fg_at(Kb, (rr * K + c) * ne + el) * p[c];
                 }
                 S.sY[rr * ne + el] = acc;
             }
         }
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < nd; l += FG_TPB) {
+    constexpr int ND = FgEbeRegs<K, EPT, T>::ND;
+#pragma unroll
+    for (int u = 0; u < ND; u++) {
+        const int l = threadIdx.x + u * FG_TPB;
+        if (l < nd) {
+            const int s0 = S.sIoff[l], s1 = S.sIoff[l + 1];
+            T acc = S.sY[S.sInc[s0]];
+            for (int q = s0 + 1; q < s1; q++) acc = acc + S.sY[S.sInc[q]];
+            if (r.sd[u] < 0)
+                fg_at(Q, (unsigned)r.gd[u]) = acc;
+            else
+                fg_at(hpart, (unsigned)r.sd[u]) = acc;
+        }
+    }
+    for (unsigned l = threadIdx.x + ND * FG_TPB; (int)l < nd; l += FG_TPB) {
         const int s0 = S.sIoff[l], s1 = S.sIoff[l + 1];
         T acc = S.sY[S.sInc[s0]];
         for (int q = s0 + 1; q < s1; q++) acc = acc + S.sY[S.sInc[q]];
-        const int s = L.slots[d0 + l];
+        const int s = fg_at(Sb, l);
         if (s < 0)
-            Q[L.dofs[d0 + l]] = acc;
+            fg_at(Q, (unsigned)fg_at(Db, l)) = acc;
         else
-            hpart[s] = acc;
+            fg_at(hpart, (unsigned)s) = acc;
     }
 }
 
@@ -179,6 +240,6 @@ __device__ __forceinline__ void fg_ebe_block(int b, const FgEbeLayout &L, const 
 {
     FgEbeRegs<K, EPT, T> r;
     fg_ebe_load<K, EPT, T>(b, L, Kv, r);
-    fg_ebe_gather<K, T>(b, L, P, smem);
+    fg_ebe_gather<K, EPT, T>(b, L, P, smem, r);
     fg_ebe_finish<K, EPT, T, BS>(b, L, Kv, Q, hpart, smem, r);
 }
