@@ -102,6 +102,8 @@ SIGNATURES = {
     "fg_prec_set_invdiag": (_i, [_vp, _dp]),
     "fg_prec_destroy": (_i, [_vp]),
     "fg_gmres": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(GmresOpts), C.POINTER(GmresInfo)]),
+    "fg_gmres_batched": (_i, [_vp, _vp, _vp, _i, _ip, _vp, _vp, C.POINTER(GmresOpts), _i,
+                              _vp]),
     "fg_number_nodes": (_i, [_i, _ip, _ip]),
     "fg_fill_lm": (_i, [_i, _i, _i, _ip, _ip, _ip, _ip]),
     "fg_csr_pattern": (_i, [_i, _i, _ip, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
@@ -422,6 +424,39 @@ class Preconditioner:
             self.close()
         except Exception:
             pass
+
+
+def concat_lm(parts):
+    """Block-diagonal layout of independent problems for gmres_batched.
+
+    parts: sequence of (neq_p, lm_p) with lm_p's sink = neq_p.  Returns
+    (lm, seg): problem p's equations are shifted by seg[p]; every sink maps
+    to the shared sink slot seg[-1] = sum of neq_p."""
+    seg = np.zeros(len(parts) + 1, np.int32)
+    for p, (neq_p, _) in enumerate(parts):
+        seg[p + 1] = seg[p] + int(neq_p)
+    total = int(seg[-1])
+    out = []
+    for p, (neq_p, lm_p) in enumerate(parts):
+        lm_p = np.asarray(lm_p, np.int64)
+        out.append(np.where(lm_p == neq_p, total, lm_p + int(seg[p])).astype(np.int32))
+    return np.concatenate(out) if out else np.zeros(0, np.int32), seg
+
+
+def gmres_batched(ctx: Context, op: Operator, prec: Preconditioner | None, seg,
+                  B: DeviceArray, X: DeviceArray, kmax=30, lmax=1000, tol=1e-8,
+                  precision=GMRES_FP64, inner_tol=0.0, graph=False):
+    """fg_gmres_batched: independent problems of a block-diagonal operator
+    (problem p = equations [seg[p], seg[p+1])) solved together.
+    Returns (status, [GmresInfo per problem])."""
+    seg = np.ascontiguousarray(seg, np.int32)
+    np_ = len(seg) - 1
+    infos = (GmresInfo * np_)()
+    o = GmresOpts(kmax, lmax, tol, precision, inner_tol, 0, None, None, 0, ORTH_MGS)
+    rc = lib.fg_gmres_batched(ctx.h, op.h, prec.h if prec is not None else None, np_, seg,
+                              B.ptr, X.ptr, C.byref(o), int(bool(graph)), C.cast(infos, _vp))
+    _check(rc, "fg_gmres_batched", ok=(FG_OK, FG_ERR_NOCONV, FG_ERR_BREAKDOWN))
+    return rc, list(infos)
 
 
 def gmres(ctx: Context, op: Operator, prec: Preconditioner | None, B: DeviceArray,

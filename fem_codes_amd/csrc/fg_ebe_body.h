@@ -21,9 +21,10 @@
 
 #include "fg_internal.h"
 
-// default elements per thread (block = 256 * EPT elements): keeps the element
-// values of a thread (EPT * k * k) in registers for k <= 4
-constexpr int fg_ebe_default_ept(int k) { return k <= 2 ? 8 : k == 3 ? 4 : k == 4 ? 2 : 1; }
+// default elements per thread (block = 256 * EPT elements), the measured
+// optimum on MI355X: 1024 edges (k = 2), 512 triangles (k = 3) or
+// tetrahedra (k = 4), 256 elements of 12 or 16 dofs
+constexpr int fg_ebe_default_ept(int k) { return k <= 2 ? 4 : k <= 4 ? 2 : 1; }
 
 // device view of the blocked layout
 struct FgEbeLayout {

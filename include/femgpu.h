@@ -350,6 +350,25 @@ typedef struct {
 int fg_gmres(fg_ctx *ctx, fg_op *op, fg_prec *pc, const double *B, double *X,
              const fg_gmres_opts *opts, fg_gmres_info *info);
 
+/* Batched restarted GMRES: nprob independent systems solved at once (no
+ * reference counterpart as one call; each problem follows pgmres.c:4-211 as
+ * fg_gmres does).  The systems are one block-diagonal operator `op` (and its
+ * preconditioner): problem p owns equations [seg[p], seg[p+1]) (host array,
+ * seg[0] = 0, seg[nprob] = neq); element blocks must not couple problems
+ * (concatenate the per-problem lm with equation offsets, one shared sink
+ * slot neq).  B, X: device vectors of neq+1 entries, B already
+ * left-preconditioned.  Every problem has its own Krylov basis, Hessenberg
+ * matrix, rotations and stopping test tol * ||B_p|| and its own cycle count
+ * (opts->lmax each); the problems advance in lockstep and a finished
+ * problem drops out of the kernels.  Precision FP64 or FP32-IR; MGS only;
+ * opts->rho_hist / e0_hist and engine are ignored.  graph = 1 captures one
+ * restart cycle into a hipGraph and replays it every cycle (preconditioner
+ * NONE, Diag or element Jacobi).  info[nprob] (may be NULL).  Returns FG_OK
+ * when every problem converged, else FG_ERR_NOCONV or FG_ERR_BREAKDOWN. */
+int fg_gmres_batched(fg_ctx *ctx, fg_op *op, fg_prec *pc, int nprob, const int *seg,
+                     const double *B, double *X, const fg_gmres_opts *opts, int graph,
+                     fg_gmres_info *info);
+
 #ifdef __cplusplus
 }
 #endif
