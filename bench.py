@@ -150,6 +150,103 @@ def cpu_solve_small(n, prob_args):
     return dict(seconds=sec, steps=int(steps), kind=kind), X, ke, Fv
 
 
+def batched_leg(F, ctx, nprob, seed, prec="jacobi", kmax=30, tol=1e-8, sequential=True):
+    """nprob independent constant-velocity SUPG problems, n ~ U{32..512} cells
+    per side, angle t ~ U[0, 2 pi), diffusivity 10^U[-5,-2], assembled on the
+    GPU, solved (a) together by the batched GMRES (FP64 and FP32-IR, graph
+    replay of each cycle) and (b) one after another by the single-system
+    GMRES, from the same left-preconditioned right-hand sides."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(32, 513, nprob)
+    t0 = time.perf_counter()
+    parts, kes, Fs, probs = [], [], [], []
+    for n in sizes:
+        th = float(rng.uniform(0.0, 2 * np.pi))
+        kappa = float(10.0 ** rng.uniform(-5.0, -2.0))
+        prob = F.Supg2dProblem(F.P_CONST, kappa, 0.0, 1.0, float(np.cos(th)), float(np.sin(th)),
+                               1.0, F.H_2SQRTAREA)
+        x, y, t, v = F.structured_mesh2d(int(n))
+        mesh = F.Mesh2D(ctx, x, y, t, v)
+        ke = ctx.empty(9 * mesh.nel)
+        Fv = ctx.empty(mesh.neq + 1)
+        Fv.zero()
+        mesh.build_supg(prob, ke, Fv)
+        parts.append((mesh.neq, mesh.lm()))
+        kes.append(ke)
+        Fs.append(Fv)
+        probs.append((int(n), th, kappa))
+        mesh.close()
+    lm, seg = F.concat_lm(parts)
+    N = int(seg[-1])
+    nel = sum(len(p[1]) // 3 for p in parts)
+    ke_all = ctx.empty(9 * nel)
+    B = ctx.empty(N + 1)
+    B.zero()
+    off = 0
+    for p, (ke, Fv) in enumerate(zip(kes, Fs)):
+        F.lib.fg_memcpy_d2d(ctx.h, ke_all.ptr + 8 * off, ke.ptr, ke.nbytes)
+        F.lib.fg_memcpy_d2d(ctx.h, B.ptr + 8 * int(seg[p]), Fv.ptr, 8 * parts[p][0])
+        off += ke.n
+    op = F.Operator.ebe(ctx, N, lm, 3)
+    op.set_values(ke_all)
+    kind = F.PREC_JACOBI if prec == "jacobi" else F.PREC_DIAG
+    pc = F.Preconditioner(op, kind)
+    pc.setup(B)
+    ctx.sync()
+    log(f"batched: {nprob} problems, {nel} elements, {N} equations, setup "
+        f"{time.perf_counter() - t0:.1f}s")
+    res = {"problems": int(nprob), "elements": int(nel), "equations": N,
+           "sizes": {"min": int(sizes.min()), "max": int(sizes.max()),
+                     "mean_cells_per_side": float(sizes.mean())},
+           "precond": prec, "kmax": kmax, "tol": tol}
+    Xs = {}
+    for precision, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+        X = ctx.empty(N + 1)
+        ctx.sync()
+        w0 = time.perf_counter()
+        rc, infos = F.gmres_batched(ctx, op, pc, seg, B, X, kmax=kmax, lmax=100000, tol=tol,
+                                    precision=precision, graph=True)
+        ctx.sync()
+        wall = time.perf_counter() - w0
+        Xs[tag] = X.get()
+        res[f"batched_{tag}"] = dict(seconds=wall, status=int(rc),
+                                     converged=int(sum(f.status == F.FG_OK for f in infos)),
+                                     max_steps=int(max(f.steps for f in infos)),
+                                     total_steps=int(sum(f.steps for f in infos)))
+        log(f"batched {tag}", res[f"batched_{tag}"])
+    d = Xs["fp32_ir"][:N] - Xs["fp64"][:N]
+    errs = [float(np.linalg.norm(d[seg[p]:seg[p + 1]]) /
+                  max(np.linalg.norm(Xs["fp64"][seg[p]:seg[p + 1]]), 1e-300))
+            for p in range(nprob) if seg[p + 1] > seg[p]]
+    res["fp32_ir_max_rel_err_vs_fp64"] = max(errs)
+    res["speedup_fp32_ir"] = res["batched_fp64"]["seconds"] / res["batched_fp32_ir"]["seconds"]
+    del ke_all, op, pc
+    if sequential:
+        # the same problems, one fg_gmres call each (operators built first)
+        Bh = B.get()
+        tot = {"fp64": 0.0, "fp32_ir": 0.0}
+        for p, (neq_p, lm_p) in enumerate(parts):
+            opp = F.Operator.ebe(ctx, neq_p, lm_p, 3)
+            opp.set_values(kes[p])
+            pcp = F.Preconditioner(opp, kind)
+            Bp = ctx.to_device(np.append(Bh[seg[p]:seg[p + 1]], 0.0))
+            pcp.setup(None)
+            for precision, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+                Xp = ctx.empty(neq_p + 1)
+                ctx.sync()
+                w0 = time.perf_counter()
+                F.gmres(ctx, opp, pcp, Bp, Xp, kmax=kmax, lmax=100000, tol=tol,
+                        precision=precision)
+                ctx.sync()
+                tot[tag] += time.perf_counter() - w0
+            opp.close()
+        res["sequential_seconds"] = tot
+        res["batched_speedup_vs_sequential"] = {
+            k: tot[k] / res[f"batched_{k}"]["seconds"] for k in tot}
+        log("sequential", tot)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +266,9 @@ def main():
                          "the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--orth", default="cgs", choices=sorted(ORTHS),
                     help="orthogonalisation of the FP32-IR inner cycle (FP64 is always MGS)")
+    ap.add_argument("--batched", type=int, default=0, metavar="NPROB",
+                    help="also run the batched leg: NPROB independent problems of "
+                         "32..512 cells per side (256 = the batched workload)")
     ap.add_argument("--matvec-only", action="store_true",
                     help="only the EBE FP64 product (for profiling runs)")
     args = ap.parse_args()
@@ -297,6 +397,10 @@ def main():
         solve["const_64"] = sres
         log("64^2", sres)
 
+    batched = None
+    if args.batched > 0:
+        batched = batched_leg(F, ctx, args.batched, args.seed + 4, prec=args.prec)
+
     cpu = None
     if not args.no_cpu and not args.matvec_only:
         if "ke" in sysd:
@@ -323,6 +427,7 @@ def main():
         "copy_calibration": copy,
         "ebe_autotune_ms": {f"{k[0]}/{k[1]}": v for k, v in tune_ms.items()},
         "solve": solve,
+        "batched": batched,
     }
     print(json.dumps(out))
 
