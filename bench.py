@@ -345,11 +345,17 @@ def main():
         log("copy calibration", copy)
         del X, Y
 
+    # HBM bytes per launch from the committed counter runs, when they measured
+    # the kernel this run chose
+    kernel = (f"k_ebe_apply_pipe<3, {best_epb // 256}, double, 0>" if best_pipe
+              else f"k_ebe_apply<3, {best_epb // 256}, double, 0>")
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ebe_f64_2048_traffic.json")
     if os.path.exists(tpath) and n == 2048:
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("kernel") == kernel:
+            traffic = tj.get("hbm_bytes_per_launch")
 
     solve = {}
     if not args.no_solve and not args.matvec_only:
@@ -418,8 +424,7 @@ def main():
                    "wall_ms_per_step": wall * 1e3 / args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_GBS, "traffic": traffic,
-                     "kernel": (f"k_ebe_apply_pipe<3, {best_epb // 256}, double, 0>"
-                                if best_pipe else f"k_ebe_apply<3, {best_epb // 256}, double, 0>"),
+                     "kernel": kernel,
                      "schedule": {"elems_per_block": best_epb, "wg_per_cu": best_pipe},
                      "algorithmic_bytes": bytes64, "ms": ms_main},
         "cpu_baseline": cpu,
