@@ -113,6 +113,7 @@ SIGNATURES = {
     "fg_mesh2d_get_lm": (_i, [_vp, _ip]),
     "fg_mesh2d_destroy": (_i, [_vp]),
     "fg_supg2d_build": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "fg_supg2d_transient_build": (_i, [_vp, _vp, _d, _d, _vp, _vp, _vp, _vp]),
     "fg_op_assemble_csr": (_i, [_vp, _i, _ip, _vp]),
     "fg_op_assemble_ede": (_i, [_vp, _i, _ip, _ip, _vp]),
     "fg_csr_pattern_k": (_i, [_i, _i, _i, _ip, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
@@ -143,7 +144,7 @@ class Supg2dProblem(C.Structure):
                 ("scale", C.c_double), ("h_shock", C.c_int)]
 
 
-P_CONST, P_ROTATING, P_VALIDACAO = 0, 1, 2
+P_CONST, P_ROTATING, P_VALIDACAO, P_ROTATE_ABOUT = 0, 1, 2, 3
 H_2SQRTAREA, H_SQRTAREA = 0, 1
 
 
@@ -571,6 +572,14 @@ class Mesh2D:
         """Build_K_F_SUPG on the device: ke (nel*9), F (neq+1)."""
         _check(lib.fg_supg2d_build(self.h, C.byref(prob), u.ptr if u is not None else None,
                                    ke.ptr, F.ptr if F is not None else None), "fg_supg2d_build")
+
+    def build_transient(self, prob: Supg2dProblem, alpha: float, dt: float, U: DeviceArray,
+                        dU: DeviceArray, A: DeviceArray, R: DeviceArray):
+        """Build_M_K_R_SUPG on the device: A (nel*9) = me + alpha dt ke,
+        R (neq+1) = F - M dU - K U; U, dU nodal (nnodes)."""
+        _check(lib.fg_supg2d_transient_build(self.h, C.byref(prob), float(alpha), float(dt),
+                                             U.ptr, dU.ptr, A.ptr, R.ptr),
+               "fg_supg2d_transient_build")
 
     def close(self):
         if self.h and self.ctx.h:

@@ -187,6 +187,7 @@ struct fg_mesh2d {
     int *d_vertex = nullptr, *d_id = nullptr, *d_type = nullptr;
     int *d_finc_off = nullptr, *d_finc = nullptr;   // equation -> e*3+r, element order
     double *d_Fe = nullptr;                          // [nel*3]
+    double *d_Tr = nullptr;                          // [nel*12] transient residual terms
     std::vector<int> h_lm;
 };
 
@@ -259,6 +260,7 @@ extern "C" int fg_mesh2d_destroy(fg_mesh2d *m)
     hipStreamSynchronize(m->ctx->stream);
     hipFree(m->d_x); hipFree(m->d_y); hipFree(m->d_vertex); hipFree(m->d_id);
     hipFree(m->d_type); hipFree(m->d_finc_off); hipFree(m->d_finc); hipFree(m->d_Fe);
+    hipFree(m->d_Tr);
     delete m;
     return FG_OK;
 }
@@ -272,6 +274,9 @@ __device__ __forceinline__ void prob_velocity(const fg_supg2d_problem &pb, doubl
     if (pb.kind == FG_P_ROTATING) {
         bx = (-Y + 0.5) * pb.scale;
         by = (X - 0.5) * pb.scale;
+    } else if (pb.kind == FG_P_ROTATE_ABOUT) {   // 02 CONE/CONE_Velocity.c:3-12
+        bx = -(Y - pb.by) * pb.scale;
+        by = (X - pb.bx) * pb.scale;
     } else if (pb.kind == FG_P_VALIDACAO) {
         bx = 1;
         by = 20 * Y;
@@ -412,7 +417,7 @@ extern "C" int fg_supg2d_build(fg_mesh2d *m, const fg_supg2d_problem *pb, const 
                                double *ke, double *F)
 {
     FG_ARG(m && pb && ke, "mesh/problem/ke");
-    FG_ARG(pb->kind >= FG_P_CONST && pb->kind <= FG_P_VALIDACAO, "problem kind");
+    FG_ARG(pb->kind >= FG_P_CONST && pb->kind <= FG_P_ROTATE_ABOUT, "problem kind");
     fg_ctx *c = m->ctx;
     if (m->nel > 0)
         FG_LAUNCH(k_supg2d, fg_vec_grid(c, m->nel), FG_TPB, 0, c->stream, m->nel, m->d_x, m->d_y,
@@ -420,6 +425,165 @@ extern "C" int fg_supg2d_build(fg_mesh2d *m, const fg_supg2d_problem *pb, const 
     if (F && m->neq > 0)
         FG_LAUNCH(k_gather_F, fg_vec_grid(c, m->neq), FG_TPB, 0, c->stream, m->neq,
                   m->d_finc_off, m->d_finc, m->d_Fe, F);
+    return FG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// transient SUPG (02_Transport_Equation/2D/01_CommonFiles/Build_M_K_R_SUPG.c)
+// ---------------------------------------------------------------------------
+// One thread per element: Ae = me + alpha dt ke (:157-159) into A[9E..], and
+// for each row i the four terms the reference applies to R[lm[E][i]] in
+// order -- the load (:162-164), then me[i][j] du_j + ke[i][j] u_j for
+// j = 0, 1, 2 (:166-171) -- into Tr[12E + 4i ..]; a per-equation gather then
+// replays them in element order.  Shock capture: CAU with the residual's time
+// derivative (02 CAU_ShockCapture.c:3-30), h = |sqrt(2 Area)|
+// (02 h_shock_2sqrtArea.c:3-6).
+__global__ __launch_bounds__(FG_TPB) void k_supg2d_tr(
+    int nel, const double *__restrict__ x, const double *__restrict__ y,
+    const int *__restrict__ vertex, const double *__restrict__ U, const double *__restrict__ dU,
+    fg_supg2d_problem pb, double alpha, double dt, double *__restrict__ A,
+    double *__restrict__ Tr)
+{
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
+    double kx, ky, gamma;
+    if (pb.kind == FG_P_VALIDACAO) { kx = 1.0; ky = 1.0; gamma = 1; }
+    else { kx = pb.kappa; ky = pb.kappa; gamma = pb.gamma; }
+    for (int E = blockIdx.x * FG_TPB + threadIdx.x; E < nel; E += gridDim.x * FG_TPB) {
+        const int J1 = vertex[3 * E], J2 = vertex[3 * E + 1], J3 = vertex[3 * E + 2];
+        const double X0 = x[J1], X1 = x[J2], X2 = x[J3];
+        const double Y0 = y[J1], Y1 = y[J2], Y2 = y[J3];
+        const double y23 = Y1 - Y2, y31 = Y2 - Y0, y12 = Y0 - Y1;
+        const double x32 = X2 - X1, x13 = X0 - X2, x21 = X1 - X0;
+        const double TwoA = x21 * y31 - x13 * y12;
+        const double Area = 0.5 * TwoA;
+        const double invArea = 1.0 / Area;
+        const double h = sqrt(fabs(TwoA));
+        const double XB = third * (X0 + X1 + X2);
+        const double YB = third * (Y0 + Y1 + Y2);
+        double Be_x, Be_y;
+        prob_velocity(pb, XB, YB, Be_x, Be_y);
+        const double norm = sqrt(Be_x * Be_x + Be_y * Be_y);
+        const double tau = norm > 0 ? 0.5 * h / norm : 0;
+        const double fe1 = prob_font(pb, X0, Y0, kx, gamma, Be_x, Be_y);
+        const double fe2 = prob_font(pb, X1, Y1, kx, gamma, Be_x, Be_y);
+        const double fe3 = prob_font(pb, X2, Y2, kx, gamma, Be_x, Be_y);
+        const double feb = third * (fe1 + fe2 + fe3);
+        const double ue[3] = {U[J1], U[J2], U[J3]};
+        const double due[3] = {dU[J1], dU[J2], dU[J3]};
+        const double ueb = third * (ue[0] + ue[1] + ue[2]);
+        const double dueb = third * (due[0] + due[1] + due[2]);
+
+        const double D12 = 0.25 * invArea * ((kx * y23 * y31) + (ky * x32 * x13));
+        const double D13 = 0.25 * invArea * ((kx * y23 * y12) + (ky * x32 * x21));
+        const double D23 = 0.25 * invArea * ((kx * y31 * y12) + (ky * x13 * x21));
+        const double C11 = sixth * (Be_x * y23 + Be_y * x32);
+        const double C12 = sixth * (Be_x * y31 + Be_y * x13);
+        const double C13 = sixth * (Be_x * y12 + Be_y * x21);
+        const double R11 = sixth * (gamma * Area);
+        const double R12 = twelfth * (gamma * Area);
+        const double Cpg12 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y31 + Be_x * Be_y * x13) +
+                                                     x32 * (Be_x * Be_y * y31 + Be_y * Be_y * x13));
+        const double Cpg13 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) +
+                                                     x32 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+        const double Cpg23 = 0.25 * invArea * tau * (y31 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) +
+                                                     x13 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+        const double Rpg11 = sixth * tau * gamma * (Be_x * y23 + Be_y * x32);
+        const double Rpg21 = sixth * tau * gamma * (Be_x * y31 + Be_y * x13);
+        const double Rpg31 = sixth * tau * gamma * (Be_x * y12 + Be_y * x21);
+
+        const double hs = fabs(sqrt(2 * Area));
+        double Eu = 0.0;
+        {
+            const double BetaGradu = 0.5 * invArea * (ue[0] * (Be_x * y23 + Be_y * x32) +
+                                                      ue[1] * (Be_x * y31 + Be_y * x13) +
+                                                      ue[2] * (Be_x * y12 + Be_y * x21));
+            const double Ru = dueb + BetaGradu + gamma * ueb - feb;
+            const double g0 = 0.5 * invArea * (y23 * ue[0] + y31 * ue[1] + y12 * ue[2]);
+            const double g1 = 0.5 * invArea * (x32 * ue[0] + x13 * ue[1] + x21 * ue[2]);
+            const double ng = sqrt(g0 * g0 + g1 * g1);
+            if (ng > 0) {
+                const double bxp = BetaGradu * g0 / (ng * ng);
+                const double byp = BetaGradu * g1 / (ng * ng);
+                const double kt = bxp * bxp * kx + byp * byp * ky;
+                const double nbp = sqrt(bxp * bxp + byp * byp);
+                const double Pee = 0.5 * hs * nbp * nbp * nbp / kt;
+                if (third * Pee < 0.7) Eu = 0.5 * hs * third * Pee * fabs(Ru) / ng;
+                else Eu = 0.5 * hs * 0.7 * fabs(Ru) / ng;
+            }
+        }
+        const double Csc12 = Eu * 0.25 * invArea * (y23 * y31 + x32 * x13);
+        const double Csc13 = Eu * 0.25 * invArea * (y23 * y12 + x32 * x21);
+        const double Csc23 = Eu * 0.25 * invArea * (y31 * y12 + x13 * x21);
+
+        double me[9], ke[9];
+        me[0] = sixth * Area + tau * C11;
+        me[1] = twelfth * Area + tau * C11;
+        me[2] = twelfth * Area + tau * C11;
+        me[3] = twelfth * Area + tau * C12;
+        me[4] = sixth * Area + tau * C12;
+        me[5] = twelfth * Area + tau * C12;
+        me[6] = twelfth * Area + tau * C13;
+        me[7] = twelfth * Area + tau * C13;
+        me[8] = sixth * Area + tau * C13;
+        ke[0] = -(D12 + D13) + C11 - (Cpg12 + Cpg13) + R11 + Rpg11 - (Csc12 + Csc13);
+        ke[1] = D12 + C12 + Cpg12 + R12 + Rpg11 + Csc12;
+        ke[2] = D13 + C13 + Cpg13 + R12 + Rpg11 + Csc13;
+        ke[3] = D12 + C11 + Cpg12 + R12 + Rpg21 + Csc12;
+        ke[4] = -(D12 + D23) + C12 - (Cpg12 + Cpg23) + R11 + Rpg21 - (Csc12 + Csc23);
+        ke[5] = D23 + C13 + Cpg23 + R12 + Rpg21 + Csc23;
+        ke[6] = D13 + C11 + Cpg13 + R12 + Rpg31 + Csc13;
+        ke[7] = D23 + C12 + Cpg23 + R12 + Rpg31 + Csc23;
+        ke[8] = -(D13 + D23) + C13 - (Cpg13 + Cpg23) + R11 + Rpg31 - (Csc13 + Csc23);
+        double *a = A + 9L * E;
+#pragma unroll
+        for (int q = 0; q < 9; q++) a[q] = me[q] + alpha * dt * ke[q];
+        const double fs = fe1 + fe2 + fe3;
+        double load[3];
+        load[0] = Area * twelfth * (2 * fe1 + fe2 + fe3) + sixth * tau * fs * (Be_x * y23 + Be_y * x32);
+        load[1] = Area * twelfth * (fe1 + 2 * fe2 + fe3) + sixth * tau * fs * (Be_x * y31 + Be_y * x13);
+        load[2] = Area * twelfth * (fe1 + fe2 + 2 * fe3) + sixth * tau * fs * (Be_x * y12 + Be_y * x21);
+        double *t = Tr + 12L * E;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            t[4 * i] = load[i];
+#pragma unroll
+            for (int j = 0; j < 3; j++) t[4 * i + 1 + j] = me[3 * i + j] * due[j] + ke[3 * i + j] * ue[j];
+        }
+    }
+}
+
+// R[g] = sum over g's incidences (e, i) in element order of
+// ((+load) - t0) - t1 - t2, from zero (Build_M_K_R_SUPG.c:39, :162-171)
+__global__ void k_gather_R(int neq, const int *__restrict__ off, const int *__restrict__ inc,
+                           const double *__restrict__ Tr, double *__restrict__ R)
+{
+    for (int g = blockIdx.x * FG_TPB + threadIdx.x; g <= neq; g += gridDim.x * FG_TPB) {
+        double s = 0.0;
+        if (g < neq)
+            for (int q = off[g]; q < off[g + 1]; q++) {
+                const double *t = Tr + 4L * inc[q];
+                s += t[0];
+                s -= t[1];
+                s -= t[2];
+                s -= t[3];
+            }
+        R[g] = s;
+    }
+}
+
+extern "C" int fg_supg2d_transient_build(fg_mesh2d *m, const fg_supg2d_problem *pb,
+                                         double alpha, double dt, const double *U,
+                                         const double *dU, double *A, double *R)
+{
+    FG_ARG(m && pb && U && dU && A && R, "mesh/problem/U/dU/A/R");
+    FG_ARG(pb->kind >= FG_P_CONST && pb->kind <= FG_P_ROTATE_ABOUT, "problem kind");
+    fg_ctx *c = m->ctx;
+    if (!m->d_Tr) FG_HIP(hipMalloc(&m->d_Tr, sizeof(double) * 12 * (size_t)(m->nel ? m->nel : 1)));
+    if (m->nel > 0)
+        FG_LAUNCH(k_supg2d_tr, fg_vec_grid(c, m->nel), FG_TPB, 0, c->stream, m->nel, m->d_x,
+                  m->d_y, m->d_vertex, U, dU, *pb, alpha, dt, A, m->d_Tr);
+    FG_LAUNCH(k_gather_R, fg_vec_grid(c, m->neq + 1), FG_TPB, 0, c->stream, m->neq,
+              m->d_finc_off, m->d_finc, m->d_Tr, R);
     return FG_OK;
 }
 

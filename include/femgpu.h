@@ -224,10 +224,13 @@ int fg_ede_pattern(int neq, int nel, const int *lm, int *order, int *edge_by_ele
  * problems with u_D = 0.  Problem kinds mirror the FemFunctions callbacks:
  *   FG_P_CONST     beta = (bx,by), f const, kappa, gamma
  *   FG_P_ROTATING  beta = scale*(-y+0.5, x-0.5), f const, kappa, gamma
- *   FG_P_VALIDACAO 01_SS_Transport_Equation/2D/VALIDACAO/VALIDACAO_*.c   */
+ *   FG_P_VALIDACAO 01_SS_Transport_Equation/2D/VALIDACAO/VALIDACAO_*.c
+ *   FG_P_ROTATE_ABOUT beta = scale*(-(y-by), x-bx), a rotation about (bx,by)
+ *                  as 02_Transport_Equation/2D/CONE/CONE_Velocity.c:3-12   */
 #define FG_P_CONST      0
 #define FG_P_ROTATING   1
 #define FG_P_VALIDACAO  2
+#define FG_P_ROTATE_ABOUT 3
 #define FG_H_2SQRTAREA  0   /* h_shock_2sqrtArea.c */
 #define FG_H_SQRTAREA   1   /* h_shock_sqrtArea.c  */
 typedef struct {
@@ -248,6 +251,16 @@ int fg_mesh2d_destroy(fg_mesh2d *m);
  * u_dev: current iterate (neq+1) for the shock-capturing term, or NULL = 0. */
 int fg_supg2d_build(fg_mesh2d *m, const fg_supg2d_problem *pb, const double *u_dev,
                     double *ke_dev, double *F_dev);
+/* Transient SUPG: replaces 02_Transport_Equation/2D/01_CommonFiles/
+ * Build_M_K_R_SUPG.c:6-184 with that driver's CAU shock capture
+ * (CAU_ShockCapture.c:3-30) and h_shock_2sqrtArea.c.  U_dev, dU_dev: nodal
+ * values [nnodes] of u and du/dt (eval_U_dU.c: free nodes from u/du,
+ * prescribed nodes upresc and 0).  A_dev: nel*9 element matrices
+ * me + alpha*dt*ke, row-major (ebe_assembly.c); R_dev: neq+1, the residual
+ * F - M dU - K U assembled in element order, R[neq] = 0. */
+int fg_supg2d_transient_build(fg_mesh2d *m, const fg_supg2d_problem *pb, double alpha,
+                              double dt, const double *U_dev, const double *dU_dev,
+                              double *A_dev, double *R_dev);
 /* csr_assembly.c:3-24 / ede_assembly.c:3-35 on the device: the stored
  * operator receives the sum of element contributions in element order. */
 int fg_op_assemble_csr(fg_op *csr, int nel, const int *csr_by_elem, const double *ke_dev);

@@ -352,6 +352,19 @@ class Oracle:
                                   C.byref(prob), ke, F)
         return ke, F
 
+    def build_supg2d_transient(self, x, y, vertex, lm, neq, U, dU, prob: OrcProb2d, alpha, dt):
+        """Build_M_K_R_SUPG: (A[nel*9] = me + alpha dt ke, R[neq+1])"""
+        nel = len(vertex) // 3
+        A, R = np.zeros(9 * nel), np.zeros(neq + 1)
+        L = self.lib
+        L.orc_build_supg2d_transient.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _dp, _dp,
+                                                 C.POINTER(OrcProb2d), _d, _d, _dp, _dp]
+        L.orc_build_supg2d_transient(len(x), nel, neq, x, y, vertex, lm,
+                                     np.ascontiguousarray(U, np.float64),
+                                     np.ascontiguousarray(dU, np.float64), C.byref(prob),
+                                     alpha, dt, A, R)
+        return A, R
+
     def ede_assembly(self, nel, nedge, ke, order, ebe):
         A = np.zeros(4 * (nedge + 1))
         self.lib.orc_ede_assembly(nel, nedge, ke, order, ebe, A)
@@ -583,3 +596,32 @@ class ReferenceEuler:
         A = np.zeros(144 * self.nel)
         self.lib.refeu_block_scaling(Fp, A)
         return A, Fp
+
+
+class ReferenceTransient:
+    """The reference's transient 2D SUPG element build (Build_M_K_R_SUPG.c
+    of 02_Transport_Equation) compiled from its sources (oracle/_ref)."""
+
+    PATH = os.path.join(HERE, "_ref", "libref_tr2d.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.exists(cls.PATH)
+
+    def __init__(self):
+        L = self.lib = C.CDLL(self.PATH)
+        L.reftr_setup.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _ip, _i, _d, _d, _d, _d,
+                                  _d, _d, _d, _d, _d]
+        L.reftr_build.argtypes = [_dp, _dp, _dp, _dp]
+
+    def setup(self, x, y, type_, idv, vertex, lm, neq, kind, kappa, gamma, f, bx, by, scale,
+              ud, alpha, dt):
+        self.neq, self.nel = neq, len(vertex) // 3
+        self.lib.reftr_setup(len(x), self.nel, neq, x, y, type_, idv, vertex, lm, kind, kappa,
+                             gamma, f, bx, by, scale, ud, alpha, dt)
+
+    def build(self, u, du):
+        A, R = np.zeros(9 * self.nel), np.zeros(self.neq + 1)
+        self.lib.reftr_build(np.ascontiguousarray(u, np.float64),
+                             np.ascontiguousarray(du, np.float64), A, R)
+        return A, R

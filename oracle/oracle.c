@@ -571,6 +571,10 @@ void orc_prob2d_velocity(const orc_prob2d *pb, double x, double y, double beta[2
         beta[0] = 1;
         beta[1] = 20 * y;
         break;
+    case ORC_P_ROTATE_ABOUT:                    /* 02 CONE_Velocity.c:3-12 */
+        beta[0] = -(y - pb->by) * pb->scale;
+        beta[1] = (x - pb->bx) * pb->scale;
+        break;
     default:
         beta[0] = pb->bx;
         beta[1] = pb->by;
@@ -1376,5 +1380,132 @@ void orc_block_scaling_ebe(int neq, int nel, int nnodes, const int *vertex, cons
                         Ae[12 * (4 * v + i) + 4 * w + k] = s;
                     }
         }
+    }
+}
+
+/* ======================================================================== */
+/* transient 2D SUPG (02_Transport_Equation)                                 */
+/* ======================================================================== */
+/* 02 CAU_ShockCapture.c:3-30: the element residual includes the time
+ * derivative (Ru = du_b + beta.grad u + gamma u_b - f_b, :10) */
+static double cau_shock_tr(double kx, double ky, double Be_x, double Be_y, double gamma,
+                           double ue1, double ue2, double ue3, double ueb, double dueb,
+                           double feb, double y23, double y31, double y12, double x32,
+                           double x13, double x21, double invArea, double h)
+{
+    const double third = 1. / 3.;
+    double BetaGradu = 0.5 * invArea * (ue1 * (Be_x * y23 + Be_y * x32) +
+                                        ue2 * (Be_x * y31 + Be_y * x13) +
+                                        ue3 * (Be_x * y12 + Be_y * x21));
+    double Ru = dueb + BetaGradu + gamma * ueb - feb;
+    double g0 = 0.5 * invArea * (y23 * ue1 + y31 * ue2 + y12 * ue3);
+    double g1 = 0.5 * invArea * (x32 * ue1 + x13 * ue2 + x21 * ue3);
+    double ng = sqrt(g0 * g0 + g1 * g1);
+    double bxp = BetaGradu * g0 / (ng * ng);
+    double byp = BetaGradu * g1 / (ng * ng);
+    double kt = bxp * bxp * kx + byp * byp * ky;
+    double nbp = sqrt(bxp * bxp + byp * byp);
+    double Pee = 0.5 * h * nbp * nbp * nbp / kt;
+    if (ng > 0) {
+        if (third * Pee < 0.7)
+            return 0.5 * h * third * Pee * fabs(Ru) / ng;
+        return 0.5 * h * 0.7 * fabs(Ru) / ng;
+    }
+    return 0;
+}
+
+/* Build_M_K_R_SUPG.c:6-184 (operand order kept: bitwise at -ffp-contract=off) */
+void orc_build_supg2d_transient(int nnodes, int nel, int neq, const double *x, const double *y,
+                                const int *vertex, const int *lm, const double *U,
+                                const double *dU, const orc_prob2d *pb, double alpha, double dt,
+                                double *A, double *R)
+{
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
+    double kx, ky, gamma;
+    (void)nnodes;
+    if (pb->kind == ORC_P_VALIDACAO) { kx = 1.0; ky = 1.0; gamma = 1; }
+    else { kx = pb->kappa; ky = pb->kappa; gamma = pb->gamma; }
+    for (int r = 0; r <= neq; r++) R[r] = 0.0;
+    for (long E = 0; E < nel; E++) {
+        const int J1 = vertex[3 * E], J2 = vertex[3 * E + 1], J3 = vertex[3 * E + 2];
+        const double X0 = x[J1], X1 = x[J2], X2 = x[J3];
+        const double Y0 = y[J1], Y1 = y[J2], Y2 = y[J3];
+        double y23 = Y1 - Y2, y31 = Y2 - Y0, y12 = Y0 - Y1;
+        double x32 = X2 - X1, x13 = X0 - X2, x21 = X1 - X0;
+        double TwoA = x21 * y31 - x13 * y12;
+        double Area = 0.5 * TwoA;
+        double invArea = 1.0 / Area;
+        double h = sqrt(fabs(TwoA));
+        double XB = third * (X0 + X1 + X2);
+        double YB = third * (Y0 + Y1 + Y2);
+        double Beta[2];
+        orc_prob2d_velocity(pb, XB, YB, Beta);
+        double Be_x = Beta[0], Be_y = Beta[1];
+        double norm = sqrt(Be_x * Be_x + Be_y * Be_y);
+        double tau = norm > 0 ? 0.5 * h / norm : 0;
+        double fe1 = orc_prob2d_font(pb, X0, Y0, kx, gamma, Be_x, Be_y);
+        double fe2 = orc_prob2d_font(pb, X1, Y1, kx, gamma, Be_x, Be_y);
+        double fe3 = orc_prob2d_font(pb, X2, Y2, kx, gamma, Be_x, Be_y);
+        double feb = third * (fe1 + fe2 + fe3);
+        const double ue[3] = {U[J1], U[J2], U[J3]};
+        const double due[3] = {dU[J1], dU[J2], dU[J3]};
+        double ueb = third * (ue[0] + ue[1] + ue[2]);
+        double dueb = third * (due[0] + due[1] + due[2]);
+
+        double D12 = 0.25 * invArea * ((kx * y23 * y31) + (ky * x32 * x13));
+        double D13 = 0.25 * invArea * ((kx * y23 * y12) + (ky * x32 * x21));
+        double D23 = 0.25 * invArea * ((kx * y31 * y12) + (ky * x13 * x21));
+        double C11 = sixth * (Be_x * y23 + Be_y * x32);
+        double C12 = sixth * (Be_x * y31 + Be_y * x13);
+        double C13 = sixth * (Be_x * y12 + Be_y * x21);
+        double R11 = sixth * (gamma * Area);
+        double R12 = twelfth * (gamma * Area);
+        double Cpg12 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y31 + Be_x * Be_y * x13) +
+                                               x32 * (Be_x * Be_y * y31 + Be_y * Be_y * x13));
+        double Cpg13 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) +
+                                               x32 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+        double Cpg23 = 0.25 * invArea * tau * (y31 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) +
+                                               x13 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+        double Rpg11 = sixth * tau * gamma * (Be_x * y23 + Be_y * x32);
+        double Rpg21 = sixth * tau * gamma * (Be_x * y31 + Be_y * x13);
+        double Rpg31 = sixth * tau * gamma * (Be_x * y12 + Be_y * x21);
+        /* 02 h_shock_2sqrtArea.c:3-6 */
+        double hs = fabs(sqrt(2 * Area));
+        double Eu = cau_shock_tr(kx, ky, Be_x, Be_y, gamma, ue[0], ue[1], ue[2], ueb, dueb, feb,
+                                 y23, y31, y12, x32, x13, x21, invArea, hs);
+        double Csc12 = Eu * 0.25 * invArea * (y23 * y31 + x32 * x13);
+        double Csc13 = Eu * 0.25 * invArea * (y23 * y12 + x32 * x21);
+        double Csc23 = Eu * 0.25 * invArea * (y31 * y12 + x13 * x21);
+
+        double me[3][3], ke[3][3];
+        me[0][0] = sixth * Area + tau * C11;
+        me[0][1] = twelfth * Area + tau * C11;
+        me[0][2] = twelfth * Area + tau * C11;
+        me[1][0] = twelfth * Area + tau * C12;
+        me[1][1] = sixth * Area + tau * C12;
+        me[1][2] = twelfth * Area + tau * C12;
+        me[2][0] = twelfth * Area + tau * C13;
+        me[2][1] = twelfth * Area + tau * C13;
+        me[2][2] = sixth * Area + tau * C13;
+        ke[0][0] = -(D12 + D13) + C11 - (Cpg12 + Cpg13) + R11 + Rpg11 - (Csc12 + Csc13);
+        ke[0][1] = D12 + C12 + Cpg12 + R12 + Rpg11 + Csc12;
+        ke[0][2] = D13 + C13 + Cpg13 + R12 + Rpg11 + Csc13;
+        ke[1][0] = D12 + C11 + Cpg12 + R12 + Rpg21 + Csc12;
+        ke[1][1] = -(D12 + D23) + C12 - (Cpg12 + Cpg23) + R11 + Rpg21 - (Csc12 + Csc23);
+        ke[1][2] = D23 + C13 + Cpg23 + R12 + Rpg21 + Csc23;
+        ke[2][0] = D13 + C11 + Cpg13 + R12 + Rpg31 + Csc13;
+        ke[2][1] = D23 + C12 + Cpg23 + R12 + Rpg31 + Csc23;
+        ke[2][2] = -(D13 + D23) + C13 - (Cpg13 + Cpg23) + R11 + Rpg31 - (Csc13 + Csc23);
+        double *a = A + 9 * E;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) a[3 * i + j] = me[i][j] + alpha * dt * ke[i][j];
+
+        const int *le = lm + 3 * E;
+        R[le[0]] += Area * twelfth * (2 * fe1 + fe2 + fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y23 + Be_y * x32);
+        R[le[1]] += Area * twelfth * (fe1 + 2 * fe2 + fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y31 + Be_y * x13);
+        R[le[2]] += Area * twelfth * (fe1 + fe2 + 2 * fe3) + sixth * tau * (fe1 + fe2 + fe3) * (Be_x * y12 + Be_y * x21);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) R[le[i]] -= me[i][j] * due[j] + ke[i][j] * ue[j];
+        R[neq] = 0;
     }
 }

@@ -129,7 +129,9 @@ int  orc_ede_init(int neq, int nel, const int *lm, int *order, int *edge_by_elem
 /* problem coefficients (FemFunctions callbacks of the SS-2D driver) */
 enum { ORC_P_CONST = 0,      /* beta=(bx,by), f const, u_D=0              */
        ORC_P_ROTATING = 1,   /* beta=scale*(-y+0.5, x-0.5), f const, u_D=0 */
-       ORC_P_VALIDACAO = 2   /* VALIDACAO/VALIDACAO_*.c                    */ };
+       ORC_P_VALIDACAO = 2,  /* VALIDACAO/VALIDACAO_*.c                    */
+       ORC_P_ROTATE_ABOUT = 3 /* beta=scale*(-(y-by), x-bx): rotation about
+                                (bx,by), as 02 CONE/CONE_Velocity.c:3-12  */ };
 enum { ORC_H_2SQRTAREA = 0,  /* h_shock_2sqrtArea.c:3-8                   */
        ORC_H_SQRTAREA = 1    /* h_shock_sqrtArea.c:3-60 (every branch 2*sqrt|A|) */ };
 typedef struct {
@@ -149,6 +151,17 @@ double orc_prob2d_font(const orc_prob2d *pb, double x, double y, double k, doubl
 void orc_build_supg2d(int nnodes, int nel, int neq, const double *x, const double *y,
                       const int *type, const int *id, const int *vertex,
                       const double *u, const orc_prob2d *pb, double *ke, double *F);
+/* 02_Transport_Equation/2D/01_CommonFiles/Build_M_K_R_SUPG.c:6-184 with the
+ * CAU shock capture of that driver (CAU_ShockCapture.c:3-30, residual with
+ * du) and h_shock_2sqrtArea.c:3-6: per element Ae = me + alpha dt ke
+ * (A[nel*9], row-major, the EBE assembly) and the residual
+ * R = F - M dU - K U assembled in element order (R[neq+1], zeroed first,
+ * R[neq] = 0).  U, dU: nodal values [nnodes] (eval_U_dU.c: free nodes from
+ * u/du, prescribed nodes upresc and 0). */
+void orc_build_supg2d_transient(int nnodes, int nel, int neq, const double *x, const double *y,
+                                const int *vertex, const int *lm, const double *U,
+                                const double *dU, const orc_prob2d *pb, double alpha, double dt,
+                                double *A, double *R);
 /* ebe_assembly.c:3-18 is the identity on ke (A[e][9] = ke row-major).
  * ede_assembly.c:3-35 : Aede[(nedge+1)*4] from ke (zeroed first as setzeros.c). */
 void orc_ede_assembly(int nel, int nedge, const double *ke, const int *order,
