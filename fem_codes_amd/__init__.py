@@ -134,7 +134,16 @@ SIGNATURES = {
     "fg_block_scaling_ebe": (_i, [_vp, _vp]),
     "fg_op_ebe2_create": (_i, [_vp, _vp, _i, C.POINTER(_vp)]),
     "fg_op_get_values_host": (_i, [_vp, _dp]),
+    "fg_euler2d_create": (_i, [_vp, _i, _i, _i, _dp, _dp, _ip, _ip, C.POINTER(_vp)]),
+    "fg_euler2d_destroy": (_i, [_vp]),
+    "fg_euler2d_build": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
+
+
+class EulerParams(C.Structure):
+    """fg_euler2d_params"""
+    _fields_ = [("gamma", C.c_double), ("mach", C.c_double), ("dimensionless", C.c_int),
+                ("alpha", C.c_double), ("dt", C.c_double), ("tolerance", C.c_double)]
 
 
 class Supg2dProblem(C.Structure):
@@ -716,3 +725,37 @@ def assemble_ede(op: Operator, order, edge_by_elem, ke: DeviceArray):
     order = np.ascontiguousarray(order, np.int32)
     ebe = np.ascontiguousarray(edge_by_elem, np.int32)
     _check(lib.fg_op_assemble_ede(op.h, order.size // 3, order, ebe, ke.ptr), "assemble_ede")
+
+
+class Euler2D:
+    """fg_euler2d: device copy of a 4-dof triangle mesh + the Euler SUPG
+    element build (Build_M_K_F_SUPG, CAU)."""
+
+    def __init__(self, ctx: Context, x, y, vertex, lm, neq):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        vertex = np.ascontiguousarray(vertex, np.int32)
+        lm = np.ascontiguousarray(lm, np.int32)
+        h = _vp()
+        _check(lib.fg_euler2d_create(ctx.h, x.size, vertex.size // 3, int(neq), x, y, vertex, lm,
+                                     C.byref(h)), "fg_euler2d_create")
+        self.ctx, self.h = ctx, h
+        self.nnodes, self.nel, self.neq = x.size, vertex.size // 3, int(neq)
+
+    def build(self, params: EulerParams, U: DeviceArray, dU: DeviceArray, A: DeviceArray,
+              R: DeviceArray, rot: DeviceArray | None = None, theta: DeviceArray | None = None):
+        _check(lib.fg_euler2d_build(self.h, C.byref(params), U.ptr, dU.ptr,
+                                    rot.ptr if rot is not None else None,
+                                    theta.ptr if theta is not None else None, A.ptr, R.ptr),
+               "fg_euler2d_build")
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib.fg_euler2d_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -267,6 +267,34 @@ int fg_op_assemble_csr(fg_op *csr, int nel, const int *csr_by_elem, const double
 int fg_op_assemble_ede(fg_op *ede, int nel, const int *order, const int *edge_by_elem,
                        const double *ke_dev);
 
+/* ---- Euler 2D element assembly on the device ------------------------------- */
+/* Replaces 03_Euler_Equations/2D/01_CommonFiles/Build_M_K_F_SUPG.c:6-1073
+ * (SUPG + the CAU operator of Delta_CAU.c:4-78, flux Jacobians of
+ * Ax_Ay_calculations.c, slip-wall rotation.c:3-45).  The mesh object holds
+ * the triangles (vertex[nel*3]), the 12 equation numbers of every element
+ * (lm[nel*12], Fill_LM.c of that driver: DOF fastest per node, prescribed =
+ * neq) and the equation -> element-entry incidence used to assemble R. */
+typedef struct fg_euler2d fg_euler2d;
+typedef struct {
+    double gamma;         /* FemFunctions->gamma (constant over the mesh)          */
+    double mach;          /* Parameters->Mach (reference Mach number)               */
+    int dimensionless;    /* 1: dimensionless_Ax_Ay_calculations, 0: dimensional    */
+    double alpha, dt;     /* Parameters->Alpha_Build, DeltaT_Build                  */
+    double tolerance;     /* FemStructs->AuxBuild->tolerance (CAU switch-off)       */
+} fg_euler2d_params;
+int fg_euler2d_create(fg_ctx *ctx, int nnodes, int nel, int neq, const double *x,
+                      const double *y, const int *vertex, const int *lm, fg_euler2d **out);
+int fg_euler2d_destroy(fg_euler2d *m);
+/* U_dev, dU_dev: nodal conserved variables and their time derivatives
+ * [nnodes*4] (eval_U_dU.c); rot_dev [nnodes] (nonzero: Node[].v1Type == -1)
+ * and theta_dev [nnodes] (BC_theta at the node), both may be NULL.
+ * A_dev: nel*144 = Me + alpha*dt*Ke per element, row-major
+ * (ebe_assembly.c); R_dev: neq+1, -(Me dU + Ke U) assembled in element
+ * order, R[neq] = 0.  pow(v,2) is evaluated as v*v. */
+int fg_euler2d_build(fg_euler2d *m, const fg_euler2d_params *p, const double *U_dev,
+                     const double *dU_dev, const int *rot_dev, const double *theta_dev,
+                     double *A_dev, double *R_dev);
+
 /* ---- reordering (host, once per mesh) -------------------------------------- */
 /* Reordering/symrcm.c:4-242 REORDERING_SYMRCM: reverse Cuthill-McKee
  * numbering of a CSR pattern with the reference's tie-breaking.

@@ -365,6 +365,24 @@ class Oracle:
                                      alpha, dt, A, R)
         return A, R
 
+    def build_euler2d(self, x, y, vertex, lm, neq, U, dU, rot, theta, gamma, mach, dimless,
+                      alpha, dt, tol):
+        """Build_M_K_F_SUPG (CAU): (A[nel*144] = Me + alpha dt Ke, R[neq+1])"""
+        nel = len(vertex) // 3
+        A, R = np.zeros(144 * nel), np.zeros(neq + 1)
+        L = self.lib
+        L.orc_build_euler2d.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _dp, _dp, _vp, _vp, _d,
+                                        _d, _i, _d, _d, _d, _dp, _dp]
+        rot_a = None if rot is None else np.ascontiguousarray(rot, np.int32)
+        th_a = None if theta is None else np.ascontiguousarray(theta, np.float64)
+        L.orc_build_euler2d(len(x), nel, neq, x, y, vertex, lm,
+                            np.ascontiguousarray(U, np.float64),
+                            np.ascontiguousarray(dU, np.float64),
+                            _ptr(rot_a) if rot_a is not None else None,
+                            _ptr(th_a) if th_a is not None else None, gamma, mach, int(dimless),
+                            alpha, dt, tol, A, R)
+        return A, R
+
     def ede_assembly(self, nel, nedge, ke, order, ebe):
         A = np.zeros(4 * (nedge + 1))
         self.lib.orc_ede_assembly(nel, nedge, ke, order, ebe, A)
@@ -596,6 +614,23 @@ class ReferenceEuler:
         A = np.zeros(144 * self.nel)
         self.lib.refeu_block_scaling(Fp, A)
         return A, Fp
+
+    def build_setup(self, x, y, v1type, gamma, mach, dimless, alpha, dt, tol, presc, th):
+        L = self.lib
+        L.refeu_build_setup.argtypes = [_dp, _dp, _ip, _d, _d, _i, _d, _d, _d, _dp, _dp]
+        L.refeu_build_setup(np.ascontiguousarray(x, np.float64),
+                            np.ascontiguousarray(y, np.float64),
+                            np.ascontiguousarray(v1type, np.int32), gamma, mach, int(dimless),
+                            alpha, dt, tol, np.ascontiguousarray(presc, np.float64),
+                            np.ascontiguousarray(th, np.float64))
+
+    def build(self, u, du):
+        L = self.lib
+        L.refeu_build.argtypes = [_dp, _dp, _dp, _dp]
+        A, R = np.zeros(144 * self.nel), np.zeros(self.neq + 1)
+        L.refeu_build(np.ascontiguousarray(u, np.float64), np.ascontiguousarray(du, np.float64),
+                      A, R)
+        return A, R
 
 
 class ReferenceTransient:

@@ -1509,3 +1509,284 @@ void orc_build_supg2d_transient(int nnodes, int nel, int neq, const double *x, c
         R[neq] = 0;
     }
 }
+
+/* ======================================================================== */
+/* Euler 2D element build (03_Euler_Equations)                               */
+/* ======================================================================== */
+/* Ax_Ay_calculations.c:3-49 (dimensionless, reference Mach number) and
+ * :51-95 (dimensional): the flux Jacobians at the barycentre.  Ay[0][3],
+ * which the reference leaves unset and never reads, is 0 here. */
+static void eu_jacobians(double gamma, double Mach, int dimless, const double Ub[4],
+                         double Ax[4][4], double Ay[4][4])
+{
+    double v1 = Ub[1] / Ub[0], v2 = Ub[2] / Ub[0], E = Ub[3] / Ub[0];
+    memset(Ax, 0, 16 * sizeof(double));
+    memset(Ay, 0, 16 * sizeof(double));
+    Ax[0][1] = 1.0;
+    Ay[0][2] = 1.0;
+    if (dimless) {
+        double nv = v1 * v1 + v2 * v2, e2 = Mach * Mach;
+        Ax[1][0] = -(v1 * v1) + ((gamma - 1) * nv * 0.5);
+        Ax[1][1] = (3.0 - gamma) * v1;
+        Ax[1][2] = -(gamma - 1.0) * v2;
+        Ax[1][3] = (gamma - 1.0) / e2;
+        Ax[2][0] = -v1 * v2;
+        Ax[2][1] = v2;
+        Ax[2][2] = v1;
+        Ax[3][0] = -v1 * (gamma * E - ((gamma - 1.0) * e2 * nv));
+        Ax[3][1] = gamma * E - (gamma - 1.0) * e2 * (0.5 * nv + v1 * v1);
+        Ax[3][2] = -(gamma - 1.0) * e2 * v1 * v2;
+        Ax[3][3] = gamma * v1;
+        Ay[2][0] = -(v2 * v2) + (((gamma - 1.0) * nv * 0.5));
+        Ay[3][0] = -v2 * (gamma * E - (gamma - 1.0) * e2 * nv);
+        Ay[3][2] = gamma * E - (gamma - 1.0) * e2 * (0.5 * nv + v2 * v2);
+    } else {
+        double rho = Ub[0], nU = (Ub[1] * Ub[1]) + (Ub[2] * Ub[2]);
+        Ax[1][0] = -(v1 * v1) + (((gamma - 1) * nU * 0.5) / (rho * rho));
+        Ax[1][1] = (3.0 - gamma) * v1;
+        Ax[1][2] = -(gamma - 1.0) * v2;
+        Ax[1][3] = gamma - 1.0;
+        Ax[2][0] = -v1 * v2;
+        Ax[2][1] = v2;
+        Ax[2][2] = v1;
+        Ax[3][0] = (-gamma * v1 * E) + ((gamma - 1.0) * v1 * (nU / (rho * rho)));
+        Ax[3][1] = (gamma * E) - ((gamma - 1.0) * 0.5 * ((nU / (rho * rho)) + (2.0 * v1 * v1)));
+        Ax[3][2] = -((gamma - 1.0) * v1 * v2);
+        Ax[3][3] = gamma * v1;
+        Ay[2][0] = -(v2 * v2) + (((gamma - 1.0) * nU * 0.5) / (rho * rho));
+        Ay[3][0] = (-gamma * v2 * E) + (((gamma - 1.0) * v2 * nU) / (rho * rho));
+        Ay[3][2] = (gamma * E) - ((gamma - 1.0) * 0.5 * ((nU / (rho * rho)) + (2.0 * v2 * v2)));
+    }
+    Ay[1][0] = Ax[2][0];
+    Ay[1][1] = v2;
+    Ay[1][2] = v1;
+    Ay[2][1] = -(gamma - 1.0) * v1;
+    Ay[2][2] = (3.0 - gamma) * v2;
+    Ay[2][3] = Ax[1][3];
+    Ay[3][1] = Ax[3][2];
+    Ay[3][3] = gamma * v2;
+}
+
+/* x^T A0 x in the grouping of Delta_CAU.c:35-44 (upper triangle, off-diagonal
+ * terms doubled) */
+static double eu_quad(const double A0[4][4], const double v[4])
+{
+    return v[0] * (A0[0][0] * v[0] + 2.0 * A0[0][1] * v[1] + 2.0 * A0[0][2] * v[2] + 2.0 * A0[0][3] * v[3]) +
+           v[1] * (A0[1][1] * v[1] + 2.0 * A0[1][2] * v[2] + 2.0 * A0[1][3] * v[3]) +
+           v[2] * (A0[2][2] * v[2] + 2.0 * A0[2][3] * v[3]) +
+           v[3] * A0[3][3] * v[3];
+}
+
+/* Delta_CAU.c:4-78: the CAU discontinuity-capturing coefficient.  The 4x4
+ * products run over k ascending; the terms the reference leaves out are
+ * exact zeros of Ax/Ay (rows 0, Ax[2][3], Ay[1][3]), so the sums agree. */
+static double eu_delta_cau(double tol, const double gx[4], const double gy[4],
+                           const double Ax[4][4], const double Ay[4][4], const double A0[4][4],
+                           const double dUb[4], double y23, double y31, double x32, double x13,
+                           double twoArea)
+{
+    double Agx[4], Agy[4], kx[4], ky[4], L[4];
+    for (int i = 0; i < 4; i++) {
+        double sx = 0, sy = 0;
+        int first = 1;
+        for (int k = 0; k < 4; k++) {
+            if (first) { sx = Ax[i][k] * gx[k]; sy = Ay[i][k] * gy[k]; first = 0; }
+            else { sx = sx + Ax[i][k] * gx[k]; sy = sy + Ay[i][k] * gy[k]; }
+        }
+        Agx[i] = sx;
+        Agy[i] = sy;
+    }
+    for (int i = 0; i < 4; i++) {
+        kx[i] = (y23 * gx[i] + x32 * gy[i]) / twoArea;
+        ky[i] = (y31 * gx[i] + x13 * gy[i]) / twoArea;
+    }
+    double ngk = sqrt(eu_quad(A0, kx) + eu_quad(A0, ky));
+    for (int i = 0; i < 4; i++) L[i] = dUb[i] + Agx[i] + Agy[i];
+    double nL = sqrt(eu_quad(A0, L));
+    double ngu = sqrt(eu_quad(A0, gx) + eu_quad(A0, gy));
+    return ngu >= tol ? nL / ngk : 0.0;
+}
+
+/* rotation.c:3-45: the slip-wall rotation of node `a`'s block row (every
+ * column block) and of its load entries */
+static void eu_rotate(int a, double theta, double M[12][12], double F[12])
+{
+    const double cs = cos(theta), sn = sin(theta);
+    const int I = 4 * a;
+    for (int K = 0; K < 3; K++) {
+        const int J = 4 * K;
+        double b = M[I][J + 1], c = M[I][J + 2], e = M[I + 1][J], f = M[I + 1][J + 1];
+        double g = M[I + 1][J + 2], h = M[I + 1][J + 3], i = M[I + 2][J], j = M[I + 2][J + 1];
+        double k = M[I + 2][J + 2], l = M[I + 2][J + 3], n = M[I + 3][J + 1], o = M[I + 3][J + 2];
+        M[I][J + 1] = b * cs - c * sn;
+        M[I][J + 2] = b * sn + c * cs;
+        M[I + 1][J] = e * cs - i * sn;
+        M[I + 1][J + 1] = cs * (f * cs - j * sn) - sn * (g * cs - k * sn);
+        M[I + 1][J + 2] = cs * (g * cs - k * sn) + sn * (f * cs - j * sn);
+        M[I + 1][J + 3] = h * cs - l * sn;
+        M[I + 2][J] = e * sn + i * cs;
+        M[I + 2][J + 1] = cs * (f * sn + j * cs) - sn * (g * sn + k * cs);
+        M[I + 2][J + 2] = cs * (g * sn + k * cs) + sn * (f * sn + j * cs);
+        M[I + 2][J + 3] = h * sn + l * cs;
+        M[I + 3][J + 1] = n * cs - o * sn;
+        M[I + 3][J + 2] = n * sn + o * cs;
+    }
+    double b = F[I + 1], c = F[I + 2];
+    F[I + 1] = b * cs - c * sn;
+    F[I + 2] = b * sn + c * cs;
+}
+
+/* Build_M_K_F_SUPG.c:6-1073 with Delta_CAU (see header) */
+void orc_build_euler2d(int nnodes, int nel, int neq, const double *x, const double *y,
+                       const int *vertex, const int *lm, const double *U, const double *dU,
+                       const int *rot, const double *theta, double gamma, double Mach,
+                       int dimless, double alpha, double dt, double tol, double *A, double *R)
+{
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0;
+    (void)nnodes;
+    for (int r = 0; r <= neq; r++) R[r] = 0.0;
+    for (long e = 0; e < nel; e++) {
+        const int Jn[3] = {vertex[3 * e], vertex[3 * e + 1], vertex[3 * e + 2]};
+        const double x1 = x[Jn[0]], x2 = x[Jn[1]], x3 = x[Jn[2]];
+        const double y1 = y[Jn[0]], y2 = y[Jn[1]], y3 = y[Jn[2]];
+        const double y23 = y2 - y3, y31 = y3 - y1, y12 = y1 - y2;
+        const double x32 = x3 - x2, x13 = x1 - x3, x21 = x2 - x1;
+        const double twoArea = fabs(x21 * y31 - x13 * y12);
+        const double Area = 0.5 * twoArea;
+        /* node a's geometric factors (y_a, x_a): (y23, x32), (y31, x13), (y12, x21) */
+        const double ya[3] = {y23, y31, y12}, xa[3] = {x32, x13, x21};
+        double Ue[12], dUe[12], Ub[4], dUb[4], gx[4], gy[4];
+        for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 4; d++) {
+                Ue[4 * a + d] = U[4 * Jn[a] + d];
+                dUe[4 * a + d] = dU[4 * Jn[a] + d];
+            }
+        for (int i = 0; i < 4; i++) {
+            Ub[i] = (Ue[i] + Ue[i + 4] + Ue[i + 8]) * third;
+            dUb[i] = (dUe[i] + dUe[i + 4] + dUe[i + 8]) * third;
+        }
+        for (int i = 0; i < 4; i++) {
+            gx[i] = (Ue[i] * y23 + Ue[i + 4] * y31 + Ue[i + 8] * y12) / twoArea;
+            gy[i] = (Ue[i] * x32 + Ue[i + 4] * x13 + Ue[i + 8] * x21) / twoArea;
+        }
+        const double v1 = Ub[1] / Ub[0], v2 = Ub[2] / Ub[0], E = Ub[3] / Ub[0];
+        const double nU23 = (Ub[1] * Ub[1]) + (Ub[2] * Ub[2]);
+        double Ax[4][4], Ay[4][4], A0[4][4];
+        eu_jacobians(gamma, Mach, dimless, Ub, Ax, Ay);
+        /* :131-159 inverse Riemannian metric A0^{-1} from entropy variables */
+        const double rho_i = Ub[3] - ((nU23 * 0.5) / Ub[0]);
+        const double V2 = Ub[1] / rho_i, V3 = Ub[2] / rho_i, V4 = (-Ub[0]) / rho_i;
+        const double re = -1.0 / (rho_i * V4);
+        const double k1 = (V2 * V2 + V3 * V3) * 0.5 / V4;
+        A0[0][0] = re * (k1 * k1 + gamma);
+        A0[0][1] = re * (k1 * V2);
+        A0[0][2] = re * (k1 * V3);
+        A0[0][3] = re * ((k1 + 1.0) * V4);
+        A0[1][1] = re * (V2 * V2 - V4);
+        A0[1][2] = re * (V2 * V3);
+        A0[1][3] = re * (V2 * V4);
+        A0[2][2] = re * (V3 * V3 - V4);
+        A0[2][3] = re * (V3 * V4);
+        A0[3][3] = re * V4 * V4;
+        A0[1][0] = A0[0][1]; A0[2][0] = A0[0][2]; A0[2][1] = A0[1][2];
+        A0[3][0] = A0[0][3]; A0[3][1] = A0[1][3]; A0[3][2] = A0[2][3];
+        /* :161-237 AxAx, AxAy, AyAx, AyAy (k ascending, see eu_delta_cau) */
+        double P[4][4][4];   /* P[0]=AxAx P[1]=AxAy P[2]=AyAx P[3]=AyAy */
+        const double (*Lm[4])[4] = {Ax, Ax, Ay, Ay}, (*Rm[4])[4] = {Ax, Ay, Ax, Ay};
+        for (int q = 0; q < 4; q++)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    double s = Lm[q][i][0] * Rm[q][0][j];
+                    for (int k = 1; k < 4; k++) s = s + Lm[q][i][k] * Rm[q][k][j];
+                    P[q][i][j] = s;
+                }
+        const double delta = eu_delta_cau(tol, gx, gy, Ax, Ay, A0, dUb, y23, y31, x32, x13, twoArea);
+        /* :243-269 SUPG tau */
+        double betax = 2.0 * (gx[0] * Ub[0] + gx[1] * Ub[1] + gx[2] * Ub[2] + gx[3] * Ub[3]);
+        double betay = 2.0 * (gy[0] * Ub[0] + gy[1] * Ub[1] + gy[2] * Ub[2] + gy[3] * Ub[3]);
+        const double betaxy = sqrt(betax * betax + betay * betay);
+        if (betaxy != 0.0) { betax = betax / betaxy; betay = betay / betaxy; }
+        const double h = sqrt(twoArea);
+        const double avb = fabs(v1 * betax + v2 * betay);
+        const double c = sqrt(gamma * (gamma - 1.0) * (E - 0.5 * (pow(v1, 2) + pow(v2, 2))));
+        const double CFL = ((c + avb) * dt) / h;
+        const double psi = (2.0 * alpha * CFL) * (1.0 / (1.0 + 2.0 * alpha * CFL));
+        const double tau_d = delta / pow((c + avb), 2);
+        const double tau_a = h / (2.0 * (c + avb));
+        const double tau_t = (2.0 / (3.0 * (1.0 + 2.0 * alpha * CFL))) * tau_a;
+        double tau = tau_t + psi * (tau_a - tau_d);
+        if (tau < 0.0) tau = 0.0;
+        /* :273-332 C_a = (y_a Ax + x_a Ay)/6 (k-sums as above, Ca14 etc. = 0) */
+        double Cm[3][4][4];
+        for (int a = 0; a < 3; a++)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++)
+                    Cm[a][i][j] = (ya[a] * Ax[i][j] + xa[a] * Ay[i][j]) * sixth;
+        /* the reference writes row 0 and the two structural zeros directly */
+        for (int a = 0; a < 3; a++) {
+            Cm[a][0][0] = 0.0; Cm[a][0][1] = ya[a] * sixth; Cm[a][0][2] = xa[a] * sixth;
+            Cm[a][0][3] = 0.0;
+            Cm[a][1][3] = ya[a] * Ax[1][3] * sixth;
+            Cm[a][2][3] = xa[a] * Ay[2][3] * sixth;
+        }
+        const double Mg1 = Area * sixth, Mg2 = Area / 12.0;
+        /* :618-737 a_a = y_a [AxAx|AxAy] + x_a [AyAx|AyAy]; :739-855 Kpg */
+        double ax[3][4][4], ay[3][4][4];
+        for (int a = 0; a < 3; a++)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    ax[a][i][j] = ya[a] * P[0][i][j] + xa[a] * P[2][i][j];
+                    ay[a][i][j] = ya[a] * P[1][i][j] + xa[a] * P[3][i][j];
+                }
+        const double t4a = tau / (4 * Area);
+        const double d4a = delta / (4.0 * Area);
+        double Kc[3][3];   /* :858-861 */
+        Kc[0][1] = Kc[1][0] = d4a * (y23 * y31 + x32 * x13);
+        Kc[0][2] = Kc[2][0] = d4a * (y23 * y12 + x32 * x21);
+        Kc[1][2] = Kc[2][1] = d4a * (y31 * y12 + x13 * x21);
+        double Me[12][12], Ke[12][12];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                for (int i = 0; i < 4; i++)
+                    for (int j = 0; j < 4; j++) {
+                        const double mpg = Cm[a][i][j] * tau;
+                        Me[4 * a + i][4 * b + j] = i == j ? (a == b ? Mg1 : Mg2) + mpg : mpg;
+                        if (a != b) {
+                            const double kpg = t4a * (ya[b] * ax[a][i][j] + xa[b] * ay[a][i][j]);
+                            double v = Cm[b][i][j] + kpg;
+                            if (i == j) v = v + Kc[a][b];
+                            Ke[4 * a + i][4 * b + j] = v;
+                        }
+                    }
+        for (int a = 0; a < 3; a++) {
+            const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    const double k1p = t4a * (ya[b1] * ax[a][i][j] + xa[b1] * ay[a][i][j]);
+                    const double k2p = t4a * (ya[b2] * ax[a][i][j] + xa[b2] * ay[a][i][j]);
+                    double v = Cm[a][i][j] - (k1p + k2p);
+                    if (i == j) v = v - (Kc[a][b1] + Kc[a][b2]);
+                    Ke[4 * a + i][4 * a + j] = v;
+                }
+        }
+        /* :1034-1044 Ae, Re */
+        double Ae[12][12], Re[12];
+        for (int i = 0; i < 12; i++) {
+            double md = 0, ku = 0;
+            for (int j = 0; j < 12; j++) {
+                md += Me[i][j] * dUe[j];
+                ku += Ke[i][j] * Ue[j];
+                Ae[i][j] = Me[i][j] + alpha * dt * Ke[i][j];
+            }
+            Re[i] = -md - ku;
+        }
+        /* :1046-1057 slip walls */
+        if (rot)
+            for (int a = 0; a < 3; a++)
+                if (rot[Jn[a]]) eu_rotate(a, theta[Jn[a]], Ae, Re);
+        const int *le = lm + 12 * e;
+        for (int i = 0; i < 12; i++) R[le[i]] += Re[i];
+        R[neq] = 0;
+        memcpy(A + 144 * e, Ae, sizeof Ae);
+    }
+}

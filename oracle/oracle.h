@@ -201,6 +201,19 @@ void orc_blockdiag_apply(int neq, int nnodes, const int *Id, const double *invBD
 /* BlockDiag_precond_EBE_setup.c:6-86: BD/invBD [nnodes*16], F <- BD^{-1} F */
 void orc_blockdiag_setup_ebe(int neq, int nel, int nnodes, const int *vertex, const double *A,
                              const int *Id, double *BD, double *invBD, double *F);
+/* 03_Euler_Equations/2D/01_CommonFiles/Build_M_K_F_SUPG.c:6-1073 with the
+ * CAU operator (Delta_CAU.c:4-78), the flux Jacobians of
+ * Ax_Ay_calculations.c (dimless = 1: the dimensionless form with reference
+ * Mach number Mach) and the slip-wall rotation (rotation.c:3-45) of the
+ * nodes with rot[node] != 0 by theta[node] (rot may be NULL).  Inputs: nodal
+ * U, dU [nnodes*4] (eval_U_dU.c), lm [nel*12], constant gamma, alpha =
+ * Alpha_Build, dt = DeltaT_Build, tol = AuxBuild->tolerance.  Outputs:
+ * A [nel*144] = Me + alpha dt Ke per element (row-major, ebe_assembly.c) and
+ * R [neq+1] = -(Me dU + Ke U) assembled in element order, R[neq] = 0. */
+void orc_build_euler2d(int nnodes, int nel, int neq, const double *x, const double *y,
+                       const int *vertex, const int *lm, const double *U, const double *dU,
+                       const int *rot, const double *theta, double gamma, double Mach,
+                       int dimless, double alpha, double dt, double tol, double *A, double *R);
 /* Block_scaling.c:10-59 Block_scaling_EBE: A (nel*144) scaled in place,
  * ids[nnodes*4] = Node[].id (-1 prescribed) */
 void orc_block_scaling_ebe(int neq, int nel, int nnodes, const int *vertex, const int *ids,

@@ -80,3 +80,67 @@ void refeu_block_scaling(double *F, double *A)
     memcpy(F, FE.F, sizeof(double) * (PE.neq + 1));
     memcpy(A, ME.Aaux, sizeof(double) * 144 * (size_t)PE.nel);
 }
+
+/* ---- Build_M_K_F_SUPG.c (with Delta_CAU, Ax_Ay_calculations, rotation,
+ * eval_U_dU, ebe_assembly, setzeros of the Euler driver) ------------------
+ * Problem callbacks: constant gamma; prescribed values rho_p, v1_p, v2_p,
+ * e_p; slip-wall angle theta(x, y) = th0 + th1 x - th2 y at the nodes with
+ * v1Type == -1. */
+int Build_M_K_F_SUPG(ParametersType *, MatrixDataType *, FemStructsType *, FemFunctionsType *);
+double Delta_CAU(double, double *, double *, double *, double (*)[4], double (*)[4], double (*)[4],
+                 double *, double, double, double, double, double, double, double, int, double *,
+                 double *);
+void ebe_assembly(ParametersType *, MatrixDataType *, FemStructsType *, int, double (*)[12]);
+
+static FemFunctionsType FB;
+static AuxBuildStructuresType AB;
+static double b_gamma, b_p[4], b_th[3];
+static double cb_gamma(double x, double y) { (void)x; (void)y; return b_gamma; }
+static double cb_rho(double x, double y) { (void)x; (void)y; return b_p[0]; }
+static double cb_v1(double x, double y) { (void)x; (void)y; return b_p[1]; }
+static double cb_v2(double x, double y) { (void)x; (void)y; return b_p[2]; }
+static double cb_e(double x, double y) { (void)x; (void)y; return b_p[3]; }
+static double cb_theta(double x, double y) { return b_th[0] + b_th[1] * x - b_th[2] * y; }
+
+/* call after refeu_setup (mesh, ids, lm); x, y, v1type [nnodes] */
+void refeu_build_setup(const double *x, const double *y, const int *v1type, double gamma,
+                       double mach, int dimless, double alpha, double dt, double tol,
+                       const double *presc, const double *th)
+{
+    for (int i = 0; i < PE.nnodes; i++) {
+        FE.Node[i].x = x[i];
+        FE.Node[i].y = y[i];
+        FE.Node[i].v1Type = v1type[i];
+    }
+    PE.Mach = mach;
+    PE.Alpha_Build = alpha;
+    PE.DeltaT_Build = dt;
+    strcpy(PE.MatrixVectorProductScheme, "EBE");
+    memset(&FB, 0, sizeof FB);
+    memset(&AB, 0, sizeof AB);
+    AB.tolerance = tol;
+    FE.AuxBuild = &AB;
+    FE.delta_old = (double *)calloc(PE.nel, sizeof(double));
+    FE.u = (double *)calloc(PE.neq + 1, sizeof(double));
+    FE.du = (double *)calloc(PE.neq + 1, sizeof(double));
+    b_gamma = gamma;
+    memcpy(b_p, presc, sizeof b_p);
+    memcpy(b_th, th, sizeof b_th);
+    FB.gamma = cb_gamma;
+    FB.rhopresc = cb_rho; FB.v1presc = cb_v1; FB.v2presc = cb_v2; FB.epresc = cb_e;
+    FB.BC_theta = cb_theta;
+    FB.Ax_Ay_calculations = dimless ? dimensionless_Ax_Ay_calculations
+                                    : dimensional_Ax_Ay_calculations;
+    FB.ShockCapture = Delta_CAU;
+    FB.assembly = ebe_assembly;
+}
+
+/* u, du [neq+1] ; A out [nel*144] ; R out [neq+1] */
+void refeu_build(const double *u, const double *du, double *A, double *R)
+{
+    memcpy(FE.u, u, sizeof(double) * (PE.neq + 1));
+    memcpy(FE.du, du, sizeof(double) * (PE.neq + 1));
+    Build_M_K_F_SUPG(&PE, &ME, &FE, &FB);
+    memcpy(A, ME.Aaux, sizeof(double) * 144 * (size_t)PE.nel);
+    memcpy(R, FE.F, sizeof(double) * (PE.neq + 1));
+}
