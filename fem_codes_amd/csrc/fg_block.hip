@@ -18,6 +18,7 @@
 // the blocks in.  Reads of the sink slot neq return 0.
 #include <vector>
 
+#include "fg_ebe_body.h"
 #include "fg_internal.h"
 
 int fg_ebe_apply_offblock(fg_op *op, int fp32, void *P, void *Q, const int *skip);
@@ -104,7 +105,7 @@ __global__ void k_bd_setup(int nn, int neq, const int *__restrict__ Id,
                 const long long ks = kstride[e];
 #pragma unroll
                 for (int q = 0; q < 16; q++)
-                    s[q] = s[q] + K64[p + (long long)(52 * v + 12 * (q >> 2) + (q & 3)) * ks];
+                    s[q] = s[q] + K64[fg_ebe_vidx(p, ks, 52 * v + 12 * (q >> 2) + (q & 3), 4)];
             }
             inverse4(s, r);
         } else {
@@ -215,7 +216,7 @@ __global__ void k_block_scale(int nel, int nn, int neq, const int *__restrict__ 
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                double a = K64[p + (long long)(12 * (4 * v + i) + 4 * w + j) * ks];
+                double a = K64[fg_ebe_vidx(p, ks, 12 * (4 * v + i) + 4 * w + j, 4)];
                 if (pv[i] || pw[j]) a = (v == w && i == j && pv[i]) ? 1.0 : 0.0;
                 M[4 * i + j] = a;
             }
@@ -229,7 +230,7 @@ __global__ void k_block_scale(int nel, int nn, int neq, const int *__restrict__ 
                 double s = 0.0;
 #pragma unroll
                 for (int j = 0; j < 4; j++) s = s + m[j] * M[4 * j + k];
-                const long long dst = p + (long long)(12 * (4 * v + i) + 4 * w + k) * ks;
+                const long long dst = fg_ebe_vidx(p, ks, 12 * (4 * v + i) + 4 * w + k, 4);
                 K64[dst] = s;
                 K32[dst] = (float)s;
             }

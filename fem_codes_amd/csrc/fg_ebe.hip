@@ -10,8 +10,9 @@
 // workgroup owns one block:
 //   1. the block's distinct dofs (sorted) are gathered from P into LDS;
 //   2. every thread streams its elements' k*k values and u16 local dof
-//      indices (both stored per block as [entry][element], so each load is
-//      coalesced) and forms the k row sums in registers;
+//      indices (both stored per block as [entry][element], values in groups
+//      of four entries when k*k allows, so each load is coalesced and wide)
+//      and forms the k row sums in registers;
 //   3. the row sums go to LDS, and every local dof then adds the row sums
 //      that belong to it through a per-block incidence list (u16 LDS slots,
 //      elements ascending): a gather instead of a scatter, so there are no
@@ -42,8 +43,8 @@ struct Plan {
     std::vector<uint16_t> ioff, inc;
     std::vector<int> dofs, slots;
     std::vector<uint16_t> lml;
-    std::vector<long long> kpos;   // [nel] blocked base index (value q at kpos + q*stride), -1 dropped
-    std::vector<int> kstride;      // [nel] elements in the element's block
+    std::vector<long long> kpos;   // [nel] blocked base index (value q at fg_ebe_vidx), -1 dropped
+    std::vector<int> kstride;      // [nel] group stride: elements in the block x group size
     std::vector<int> hdofs, hoff{0}, zdofs;
     int nslots = 0;
 };
@@ -158,8 +159,8 @@ int build_plan(int neq, int nel, int k, const int *lm, int eb, Plan &pl)
         // local indices, elements in breadth-first order
         for (int i = 0; i < ne; i++) {
             int e = order[e0 + i];
-            pl.kpos[e] = (long long)k * k * e0 + i;
-            pl.kstride[e] = ne;
+            pl.kpos[e] = (long long)k * k * e0 + (long long)i * fg_ebe_vec(k);
+            pl.kstride[e] = ne * fg_ebe_vec(k);
             for (int j = 0; j < k; j++) {
                 int g = lm[(size_t)e * k + j];
                 pl.lml[(size_t)k * e0 + (size_t)j * ne + i] = (uint16_t)(g == neq ? nloc : locid[g]);
@@ -310,7 +311,7 @@ __global__ void k_ebe_scatter_values(long long n, int kk, const double *__restri
         int q = (int)(t - e * kk);
         long long p = kpos[e];
         if (p < 0) continue;
-        long long dst = p + (long long)q * kstride[e];
+        long long dst = fg_ebe_vidx(p, kstride[e], q, kk % 4 == 0 ? 4 : 1);
         double v = A[t];
         K64[dst] = v;
         K32[dst] = (float)v;
@@ -445,7 +446,7 @@ __global__ void k_ebe_gather_values(long long n, int kk, const long long *__rest
         long long e = t / kk;
         int q = (int)(t - e * kk);
         long long p = kpos[e];
-        A[t] = p < 0 ? 0.0 : K64[p + (long long)q * kstride[e]];
+        A[t] = p < 0 ? 0.0 : K64[fg_ebe_vidx(p, kstride[e], q, kk % 4 == 0 ? 4 : 1)];
     }
 }
 
@@ -545,7 +546,8 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
                 for (int j = 0; j < k; j++) {
                     int g = lm[(size_t)e * k + j];
                     if (g != neq)
-                        idiag[fill[g]++] = pl.kpos[e] + (long long)(j * k + j) * pl.kstride[e];
+                        idiag[fill[g]++] = fg_ebe_vidx(pl.kpos[e], pl.kstride[e], j * k + j,
+                                                       fg_ebe_vec(k));
                 }
     }
     FG_CHECK(upload(ioff, &op->d_inc_off));

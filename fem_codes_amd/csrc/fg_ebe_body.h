@@ -26,6 +26,20 @@
 // tetrahedra (k = 4), 256 elements of 12 or 16 dofs
 constexpr int fg_ebe_default_ept(int k) { return k <= 2 ? 4 : k <= 4 ? 2 : 1; }
 
+// entries per group of the blocked value layout.  When k*k is a multiple of
+// 4 (k = 2, 4, 12, 16) a block stores its values as [k*k/4][ne][4]: the four
+// consecutive entries q = 4g..4g+3 of an element sit together, so one 16-byte
+// (FP32) or 32-byte (FP64) load fetches them, and a wavefront still reads
+// one contiguous run per group.  Otherwise (k = 3) the layout is [k*k][ne].
+__host__ __device__ constexpr int fg_ebe_vec(int k) { return (k * k) % 4 == 0 ? 4 : 1; }
+
+// position of entry q of an element in the blocked values; kpos / kstride
+// are the element's base (first entry) and group stride
+__host__ __device__ inline long long fg_ebe_vidx(long long kpos, int kstride, int q, int V)
+{
+    return kpos + (long long)(q / V) * kstride + (q % V);
+}
+
 // device view of the blocked layout
 struct FgEbeLayout {
     int nblk;
@@ -74,6 +88,22 @@ __device__ __forceinline__ T &fg_at(T *x, unsigned i)
     return *reinterpret_cast<T *>(reinterpret_cast<char *>(x) + i * (unsigned)sizeof(T));
 }
 
+// group g (4 consecutive values) of a grouped layout: one vector load
+template <typename T> struct FgVec4;
+template <> struct FgVec4<float> { using type = float4; };
+template <> struct FgVec4<double> { using type = double4; };
+template <typename T>
+__device__ __forceinline__ void fg_ld4(const T *x, unsigned g, T out[4])
+{
+    using V = typename FgVec4<T>::type;
+    const V v = *reinterpret_cast<const V *>(reinterpret_cast<const char *>(x) +
+                                             g * (unsigned)(4 * sizeof(T)));
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
+    out[3] = v.w;
+}
+
 template <int K, int EPT, typename T>
 struct FgEbeRegs {
     static constexpr bool PRE = EPT * K * K <= 72;   // element values held in registers
@@ -100,8 +130,13 @@ __device__ __forceinline__ void fg_ebe_load(int b, const FgEbeLayout &L, const T
 #pragma unroll
             for (int j = 0; j < K; j++) r.li[i][j] = fg_at(Lb, j * ne + el);
             if constexpr (FgEbeRegs<K, EPT, T>::PRE) {
+                if constexpr (fg_ebe_vec(K) == 4) {
 #pragma unroll
-                for (int q = 0; q < K * K; q++) r.kv[i][q] = fg_at(Kb, q * ne + el);
+                    for (int g = 0; g < K * K / 4; g++) fg_ld4(Kb, g * ne + el, &r.kv[i][4 * g]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < K * K; q++) r.kv[i][q] = fg_at(Kb, q * ne + el);
+                }
             }
         }
     }
@@ -195,6 +230,22 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
 #pragma unroll
                     for (int c = c0 + 1; c < K; c++)
                         if (BS == 0 || c / BS != rr / BS) acc = acc + r.kv[i][rr * K + c] * p[c];
+                } else if constexpr (fg_ebe_vec(K) == 4) {
+                    // row rr = groups rr*K/4 .. (rr+1)*K/4 - 1, four columns each;
+                    // with BS = 4 a group is one node block and is skipped whole
+                    static_assert(K % 4 == 0 && (BS == 0 || BS == 4), "grouped rows");
+                    acc = T(0);
+#pragma unroll
+                    for (int gq = 0; gq < K / 4; gq++) {
+                        if (BS != 0 && gq == rr / 4) continue;
+                        T t[4];
+                        fg_ld4(Kb, (rr * (K / 4) + gq) * ne + el, t);
+#pragma unroll
+                        for (int m = 0; m < 4; m++) {
+                            const int c = 4 * gq + m;
+                            acc = c == c0 ? t[m] * p[c] : acc + t[m] * p[c];
+                        }
+                    }
                 } else {
                     acc = fg_at(Kb, (rr * K + c0) * ne + el) * p[c0];
 #pragma unroll
