@@ -247,6 +247,155 @@ def batched_leg(F, ctx, nprob, seed, prec="jacobi", kmax=30, tol=1e-8, sequentia
     return res
 
 
+def ss3d_leg(F, ctx, n=128, sweeps=4, seed=20260303):
+    """3D SUPG steady transport (DIFCONV) on the n^3 cube cut into 6 n^3
+    tetrahedra, a seeded random node numbering then reverse Cuthill-McKee;
+    FP32 EBE 4x4 products + ILU(0) of the CSR matrix (its triangular solves
+    as `sweeps` Jacobi sweeps) inside FP64 refinement (FP32-IR), GMRES(50)
+    to 1e-10, against FP64 GMRES with the same operator and preconditioner."""
+    t0 = time.perf_counter()
+    x, y, z, t, v = F.structured_mesh3d(n)
+    nn = len(x)
+    perm = np.random.default_rng(seed).permutation(nn)
+    inv = np.empty(nn, np.int64)
+    inv[perm] = np.arange(nn)
+    x, y, z, t, v = x[perm], y[perm], z[perm], t[perm], inv[v].astype(np.int32)
+    m = F.Mesh3D(ctx, x, y, z, t, v)
+    neq, nel = m.neq, m.nel
+    lm = m.lm()
+    nnz, IA, JA, cbe = F.csr_pattern_k(neq, lm, 4)
+    bw0 = F.csr_bandwidth(neq, IA, JA)
+    P = F.rcm(neq, IA, JA)
+    lm = F.renumber_lm(neq, lm, P)
+    nnz, IA, JA, cbe = F.csr_pattern_k(neq, lm, 4)
+    bw1 = F.csr_bandwidth(neq, IA, JA)
+    ke = ctx.empty(16 * nel)
+    F0 = ctx.empty(neq + 1)
+    F0.zero()
+    m.build_supg(F.Supg3dProblem(F.P3_DIFCONV, 1.0, 0, 0, 0, 0, 0, 0), ke, F0)
+    Fh = F0.get()
+    Fn = np.zeros(neq + 1)
+    Fn[:neq] = Fh[P]
+    csr = F.Operator.csr(ctx, neq, IA, JA)
+    F.assemble_csr(csr, cbe, ke, k=4)
+    ebe = F.Operator.ebe(ctx, neq, lm, 4)
+    ebe.set_values(ke)
+    del cbe, IA, JA
+    pc = F.Preconditioner(csr, F.PREC_ILU0)
+    levels = pc.ilu_levels()
+    pc.ilu_sweeps(sweeps)
+    B = ctx.to_device(Fn)
+    pc.setup(B)
+    ctx.sync()
+    log(f"ss3d: n={n} nel={nel} neq={neq} nnz={nnz} bandwidth {bw0} -> {bw1} (RCM), ILU levels "
+        f"{levels[0]}, setup {time.perf_counter() - t0:.1f}s")
+    rng = np.random.default_rng(1)
+    res = {"n": n, "elements": nel, "equations": neq, "nnz": int(nnz),
+           "bandwidth_random": int(bw0), "bandwidth_rcm": int(bw1), "ilu_levels": levels[0],
+           "ilu_sweeps": sweeps, "kmax": 50, "tol": 1e-10}
+    mv = {}
+    for nm, op in (("EBE", ebe), ("CSR", csr)):
+        for dt in (np.float64, np.float32):
+            Pd = ctx.to_device(rng.standard_normal(neq + 1).astype(dt))
+            Qd = ctx.empty(neq + 1, dt)
+            ms = op.time_apply(Pd, Qd, reps=10)
+            b = op.algorithmic_bytes(dt == np.float32)
+            mv[f"{nm}_{'f32' if dt == np.float32 else 'f64'}"] = dict(ms=ms, GBs=b / ms / 1e6)
+    res["matvec"] = mv
+    Xs = {}
+    for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+        X = ctx.empty(neq + 1)
+        ctx.sync()
+        w0 = time.perf_counter()
+        info, _, _ = F.gmres(ctx, ebe, pc, B, X, kmax=50, lmax=400, tol=1e-10, precision=prec)
+        ctx.sync()
+        Xs[tag] = X.get()
+        res[tag] = dict(seconds=time.perf_counter() - w0, steps=info.steps, cycles=info.cycles,
+                        status=info.status, true_rel_res=info.true_res / (info.eps / 1e-10))
+        log("ss3d", tag, res[tag])
+    res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
+    res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(Xs["fp32_ir"][:neq] - Xs["fp64"][:neq]) /
+                                           np.linalg.norm(Xs["fp64"][:neq]))
+    return res
+
+
+def euler_leg(F, ctx, n=1024, seed=20260305):
+    """2D Euler: the n x n grid of P1 triangles, 4 conserved variables per
+    node, a uniform Mach-3 inflow (rho 1 with a seeded 1% perturbation, u 1,
+    v 0, c 1/3), every variable prescribed on the inflow side x = 0; one
+    implicit backward-Euler step (alpha 1, dt 1e-3): the SUPG + CAU element
+    build, then (Me + dt Ke) x = R with the node block-diagonal left
+    preconditioner and GMRES(30) to 1e-8, FP64 and FP32-IR (12x12 block EBE
+    products in FP32)."""
+    t0 = time.perf_counter()
+    x, y, t, v = F.structured_mesh2d(n)
+    nn, nel = len(x), len(v) // 3
+    inflow = np.abs(x) <= 1e-15
+    ids = np.full(4 * nn, -1, np.int64)
+    free = np.repeat(~inflow, 4)
+    neq = int(free.sum())
+    ids[free] = np.arange(neq)
+    lm = ids[(4 * v[:, None].astype(np.int64) + np.arange(4)).reshape(-1)]
+    lm = np.where(lm < 0, neq, lm).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    gam = 1.4
+    rho = 1.0 + 0.01 * rng.standard_normal(nn)
+    u1 = np.ones(nn)
+    u2 = np.zeros(nn)
+    p = 1.0 / (9.0 * gam)
+    E = p / ((gam - 1.0) * rho) + 0.5 * (u1 ** 2 + u2 ** 2)
+    U = np.stack([rho, rho * u1, rho * u2, rho * E], 1).reshape(-1)
+    m = F.Euler2D(ctx, x, y, v, lm, neq)
+    A = ctx.empty(144 * nel)
+    R = ctx.empty(neq + 1)
+    Ud, dUd = ctx.to_device(U), ctx.to_device(np.zeros(4 * nn))
+    prm = F.EulerParams(gam, 3.0, 0, 1.0, 1e-3, 1e-10)
+    m.build(prm, Ud, dUd, A, R)
+    ctx.sync()
+    bt = []
+    for _ in range(3):
+        w0 = time.perf_counter()
+        m.build(prm, Ud, dUd, A, R)
+        ctx.sync()
+        bt.append(time.perf_counter() - w0)
+    op = F.Operator.ebe(ctx, neq, lm, 12)
+    op.set_values(A)
+    ids32 = np.where(ids < 0, -1, ids).astype(np.int32)
+    pc = F.Preconditioner.blockdiag(op, nn, v, ids32)
+    Rh = R.get()
+    log(f"euler: n={n} nel={nel} neq={neq} setup {time.perf_counter() - t0:.1f}s")
+    res = {"n": n, "elements": nel, "equations": neq, "element_build_ms": min(bt) * 1e3,
+           "element_matrix_bytes": 144 * 8 * nel, "precond": "BlockDiag", "kmax": 30,
+           "tol": 1e-8}
+    mv = {}
+    for dt in (np.float64, np.float32):
+        Pd = ctx.to_device(rng.standard_normal(neq + 1).astype(dt))
+        Qd = ctx.empty(neq + 1, dt)
+        ms = op.time_apply(Pd, Qd, reps=20)
+        b = op.algorithmic_bytes(dt == np.float32)
+        mv["f32" if dt == np.float32 else "f64"] = dict(ms=ms, GBs=b / ms / 1e6,
+                                                        pct_peak=100 * b / ms / 1e6 / PEAK_GBS,
+                                                        bytes=b)
+    res["matvec_ebe12"] = mv
+    Xs = {}
+    for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+        B = ctx.to_device(Rh)
+        pc.setup(B)
+        X = ctx.empty(neq + 1)
+        ctx.sync()
+        w0 = time.perf_counter()
+        info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=30, lmax=2000, tol=1e-8, precision=prec)
+        ctx.sync()
+        Xs[tag] = X.get()
+        res[tag] = dict(seconds=time.perf_counter() - w0, steps=info.steps, cycles=info.cycles,
+                        status=info.status, true_rel_res=info.true_res / (info.eps / 1e-8))
+        log("euler", tag, res[tag])
+    res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
+    res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(Xs["fp32_ir"][:neq] - Xs["fp64"][:neq]) /
+                                           max(np.linalg.norm(Xs["fp64"][:neq]), 1e-300))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +418,10 @@ def main():
     ap.add_argument("--batched", type=int, default=0, metavar="NPROB",
                     help="also run the batched leg: NPROB independent problems of "
                          "32..512 cells per side (256 = the batched workload)")
+    ap.add_argument("--ss3d", type=int, default=0, metavar="N",
+                    help="also run the 3D leg on the N^3 Kuhn cube (128 = the 3D workload)")
+    ap.add_argument("--euler", type=int, default=0, metavar="N",
+                    help="also run the Euler leg on the N x N grid (1024 = the Euler workload)")
     ap.add_argument("--matvec-only", action="store_true",
                     help="only the EBE FP64 product (for profiling runs)")
     args = ap.parse_args()
@@ -406,6 +559,8 @@ def main():
     batched = None
     if args.batched > 0:
         batched = batched_leg(F, ctx, args.batched, args.seed + 4, prec=args.prec)
+    ss3d = ss3d_leg(F, ctx, args.ss3d) if args.ss3d > 0 else None
+    euler = euler_leg(F, ctx, args.euler) if args.euler > 0 else None
 
     cpu = None
     if not args.no_cpu and not args.matvec_only:
@@ -433,6 +588,8 @@ def main():
         "ebe_autotune_ms": {f"{k[0]}/{k[1]}": v for k, v in tune_ms.items()},
         "solve": solve,
         "batched": batched,
+        "ss3d": ss3d,
+        "euler": euler,
     }
     print(json.dumps(out))
 
