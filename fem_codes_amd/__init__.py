@@ -52,7 +52,7 @@ class GmresOpts(C.Structure):
                 ("orth", C.c_int)]
 
 
-ENGINE_AUTO, ENGINE_KERNELS, ENGINE_FUSED = 0, 1, 2
+ENGINE_AUTO, ENGINE_KERNELS, ENGINE_FUSED, ENGINE_SMALL = 0, 1, 2, 3
 ORTH_MGS, ORTH_CGS = 0, 1
 
 
@@ -475,11 +475,13 @@ def gmres(ctx: Context, op: Operator, prec: Preconditioner | None, B: DeviceArra
     """pgmres (Solvers_and_Preconditioners/pgmres.c:4-211) on the device.
 
     engine: ENGINE_KERNELS (one kernel per vector operation), ENGINE_FUSED
-    (the whole solve in one persistent kernel) or ENGINE_AUTO.
+    (the whole solve in one persistent kernel), ENGINE_SMALL (the whole FP64
+    solve in one workgroup, systems of at most 4096 equations) or
+    ENGINE_AUTO (SMALL where it applies, else KERNELS).
     orth: ORTH_MGS or ORTH_CGS for the FP32-IR inner cycle.
     Returns (info, rho_hist, e0_hist)."""
     rho = np.zeros(max(1, hist_cap))
-    e0 = np.zeros(max(1, e0_cap if e0_cap else lmax + 1))
+    e0 = np.zeros(max(1, e0_cap, lmax + 1))   # the library writes up to lmax+1 entries
     o = GmresOpts(kmax, lmax, tol, precision, inner_tol, hist_cap,
                   rho.ctypes.data if hist_cap else None,
                   e0.ctypes.data if e0_cap else None, engine, orth)

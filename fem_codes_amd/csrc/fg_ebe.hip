@@ -539,19 +539,23 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
             }
     for (int g = 0; g < neq; g++) ioff[g + 1] += ioff[g];
     std::vector<long long> idiag(ioff[neq]);
+    const bool small = neq <= FG_SMALL_MAX;
+    std::vector<int> iea(small ? ioff[neq] : 0);
     {
         std::vector<int> fill(ioff.begin(), ioff.end() - 1);
         for (int e = 0; e < nitems; e++)
             if (pl.kpos[e] >= 0)
                 for (int j = 0; j < k; j++) {
                     int g = lm[(size_t)e * k + j];
-                    if (g != neq)
-                        idiag[fill[g]++] = fg_ebe_vidx(pl.kpos[e], pl.kstride[e], j * k + j,
-                                                       fg_ebe_vec(k));
+                    if (g == neq) continue;
+                    if (small) iea[fill[g]] = e * k + j;
+                    idiag[fill[g]++] = fg_ebe_vidx(pl.kpos[e], pl.kstride[e], j * k + j,
+                                                   fg_ebe_vec(k));
                 }
     }
     FG_CHECK(upload(ioff, &op->d_inc_off));
     FG_CHECK(upload(idiag, &op->d_inc_diag));
+    if (small) FG_CHECK(upload(iea, &op->d_inc_ea));
     *out = op;
     return FG_OK;
 }
@@ -587,6 +591,7 @@ void fg_ebe_free(fg_op *op)
     hipFree(op->d_kpos); hipFree(op->d_hdofs); hipFree(op->d_hoff); hipFree(op->d_zdofs);
     hipFree(op->d_hpart64); hipFree(op->d_hpart32); hipFree(op->d_inc_off);
     hipFree(op->d_inc_diag); hipFree(op->d_lm); hipFree(op->d_kstride); hipFree(op->d_rowmap);
+    hipFree(op->d_inc_ea);
 }
 
 // ---- diagonal of an element/edge operator, element order per dof ---------

@@ -604,6 +604,9 @@ int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState
 
 int fg_gmres_fused(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
                    const fg_gmres_opts *o, double eps, fg_gmres_info *info, int *handled);
+bool fg_gmres_small_applies(const fg_op *op, const fg_prec *pc, const fg_gmres_opts *o);
+int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
+                   const fg_gmres_opts *o, fg_gmres_info *info);
 
 extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
                         const fg_gmres_opts *o, fg_gmres_info *info)
@@ -612,11 +615,21 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
     FG_ARG(o->kmax >= 1 && o->kmax <= FG_KMAX, "kmax in [1,64]");
     FG_ARG(o->lmax >= 1, "lmax >= 1");
     FG_ARG(o->precision == FG_GMRES_FP64 || o->precision == FG_GMRES_FP32_IR, "precision");
-    FG_ARG(o->engine >= FG_ENGINE_AUTO && o->engine <= FG_ENGINE_FUSED, "engine");
+    FG_ARG(o->engine >= FG_ENGINE_AUTO && o->engine <= FG_ENGINE_SMALL, "engine");
     if (pc && pc->kind == FG_PREC_NONE) pc = nullptr;
     FG_ARG(!pc || (pc->op->neq == op->neq && pc->op->ctx == c),
            "preconditioner built on an operator of another size or context");
     FG_HIP(hipSetDevice(c->device));
+    // systems of at most FG_SMALL_MAX equations: the one-workgroup engine
+    if (o->engine == FG_ENGINE_SMALL || o->engine == FG_ENGINE_AUTO) {
+        const bool ok = fg_gmres_small_applies(op, pc, o);
+        if (ok) return fg_gmres_small(c, op, pc, B, X, o, info);
+        if (o->engine == FG_ENGINE_SMALL) {
+            fg_set_error("the one-workgroup GMRES engine does not apply to this "
+                         "operator/preconditioner/size/precision");
+            return FG_ERR_ARG;
+        }
+    }
     const int n = op->neq, kmax = o->kmax;
     const bool ir = o->precision == FG_GMRES_FP32_IR;
     const long long ld = ((long long)n + 1 + 63) / 64 * 64;
@@ -706,6 +719,7 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         // except that a zero residual (B = 0) returns X = 0 instead of the
         // reference's 0/0 (pgmres.c:85);
         // FP32-IR: converged when the true FP64 residual is below eps.
+        if (hs.e0 != hs.e0) { rc = FG_ERR_BREAKDOWN; break; }   // non-finite system
         if (ir && hs.conv) break;
         if (!ir && hs.e0 == 0.0) { hs.rho = 0.0; break; }
         int steps_before = hs.steps;

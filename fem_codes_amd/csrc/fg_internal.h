@@ -49,6 +49,7 @@ void fg_set_error(const char *fmt, ...);
 constexpr int FG_TPB = 256;          // threads per block for every kernel
 constexpr int FG_RED_MAX = 4096;     // max blocks of a grid reduction
 constexpr int FG_KMAX = 64;          // max GMRES restart length
+constexpr int FG_SMALL_MAX = 4096;   // equations of a system the one-workgroup solver takes
 
 // ---------------------------------------------------------------------------
 // context
@@ -101,6 +102,8 @@ struct fg_op {
     // incidence (dof -> element-local entries), element order: diagonal positions
     int *d_inc_off = nullptr;    // [neq+1]
     long long *d_inc_diag = nullptr;  // [ninc] index of K_jj in d_K64 for that incidence
+    int *d_inc_ea = nullptr;     // [ninc] e*k + j of that incidence (systems of at most
+                                 // FG_SMALL_MAX equations: the one-workgroup solver)
     int *d_lm = nullptr;         // [nitems*k] original lm (AoS) for element-local preconditioners
 
     // ---- CSR ----

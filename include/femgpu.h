@@ -353,7 +353,7 @@ typedef struct {
     double *rho_hist;   /* host: rho after every Arnoldi step                  */
     double *e0_hist;    /* host, lmax+1 entries: ||r|| at every restart, or NULL
                            (kernel-per-operation engine only)                  */
-    int    engine;      /* FG_ENGINE_AUTO / _KERNELS / _FUSED                  */
+    int    engine;      /* FG_ENGINE_AUTO / _KERNELS / _FUSED / _SMALL         */
     int    orth;        /* FP32-IR inner cycle: FG_ORTH_MGS (0, pgmres.c:107-114)
                            or FG_ORTH_CGS (classical Gram-Schmidt: every h_{j,i}
                            from one pass, one update pass; an option of this
@@ -363,7 +363,12 @@ typedef struct {
 #define FG_ORTH_MGS 0
 #define FG_ORTH_CGS 1
 
-/* engines: AUTO selects KERNELS (the faster engine at every measured size).
+/* engines: AUTO selects SMALL where it applies, else KERNELS.
+ * SMALL: the whole FP64 solve in one 1024-thread workgroup (systems of at
+ * most 4096 equations; CSR, or EBE/EDE with k = 2, 3, 4; NONE, DIAG or
+ * element JACOBI); same steps and stopping rules as KERNELS, products
+ * accumulated in the reference's element/row order.  FG_ERR_ARG when it
+ * does not apply.
  * KERNELS: one kernel per vector operation, host sync once per cycle.
  * FUSED: the whole solve in one persistent cooperative kernel (EBE/EDE with
  * k <= 4 or CSR, NONE or DIAG preconditioner, neq <= 32 rows per thread of
@@ -373,6 +378,7 @@ typedef struct {
 #define FG_ENGINE_AUTO     0
 #define FG_ENGINE_KERNELS  1
 #define FG_ENGINE_FUSED    2
+#define FG_ENGINE_SMALL    3
 
 typedef struct {
     int    steps;       /* Arnoldi steps ("cont", pgmres.c:94)                 */
