@@ -8,6 +8,7 @@
 // then Q[i] += AA[j]*P[JA[j]] for ascending j), so with FMA contraction
 // disabled the FP64 result is bitwise the reference's.  A row longer than a
 // block (never produced by P1 meshes) gets a block of its own and a tree sum.
+#include <algorithm>
 #include <vector>
 
 #include "fg_internal.h"
@@ -105,6 +106,19 @@ extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, cons
     FG_HIP(hipMalloc(&op->d_AA32, sizeof(float) * (nnz ? nnz : 1)));
     FG_HIP(hipMemset(op->d_AA, 0, sizeof(double) * (nnz ? nnz : 1)));
     FG_HIP(hipMemset(op->d_AA32, 0, sizeof(float) * (nnz ? nnz : 1)));
+    if (neq <= FG_SMALL_MAX) {
+        // the one-workgroup solver's transposed column lists
+        int mi = 0;
+        for (int r = 0; r < neq; r++) mi = std::max(mi, IA[r + 1] - IA[r]);
+        std::vector<int> iT((size_t)mi * neq, -1);
+        for (int r = 0; r < neq; r++)
+            for (int j = IA[r]; j < IA[r + 1]; j++) iT[(size_t)(j - IA[r]) * neq + r] = JA[j];
+        op->sm_mi = mi;
+        FG_HIP(hipMalloc(&op->d_sm_iT, sizeof(int) * (iT.empty() ? 1 : iT.size())));
+        if (!iT.empty())
+            FG_HIP(hipMemcpy(op->d_sm_iT, iT.data(), sizeof(int) * iT.size(),
+                             hipMemcpyHostToDevice));
+    }
     *out = op;
     return FG_OK;
 }
@@ -112,6 +126,7 @@ extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, cons
 void fg_csr_free(fg_op *op)
 {
     hipFree(op->d_rb); hipFree(op->d_IA); hipFree(op->d_JA); hipFree(op->d_AA); hipFree(op->d_AA32);
+    hipFree(op->d_sm_iT);
 }
 
 // CSR/Diag_precond_CSR_setup.c:10-14: the last slot of row i with column i

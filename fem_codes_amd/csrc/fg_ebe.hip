@@ -540,7 +540,9 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
     for (int g = 0; g < neq; g++) ioff[g + 1] += ioff[g];
     std::vector<long long> idiag(ioff[neq]);
     const bool small = neq <= FG_SMALL_MAX;
-    std::vector<int> iea(small ? ioff[neq] : 0);
+    int mi = 0;
+    for (int g = 0; g < neq; g++) mi = std::max(mi, ioff[g + 1] - ioff[g]);
+    std::vector<int> iT(small ? (size_t)mi * neq : 0, -1);
     {
         std::vector<int> fill(ioff.begin(), ioff.end() - 1);
         for (int e = 0; e < nitems; e++)
@@ -548,14 +550,21 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
                 for (int j = 0; j < k; j++) {
                     int g = lm[(size_t)e * k + j];
                     if (g == neq) continue;
-                    if (small) iea[fill[g]] = e * k + j;
+                    if (small) iT[(size_t)(fill[g] - ioff[g]) * neq + g] = j * nitems + e;
                     idiag[fill[g]++] = fg_ebe_vidx(pl.kpos[e], pl.kstride[e], j * k + j,
                                                    fg_ebe_vec(k));
                 }
     }
     FG_CHECK(upload(ioff, &op->d_inc_off));
     FG_CHECK(upload(idiag, &op->d_inc_diag));
-    if (small) FG_CHECK(upload(iea, &op->d_inc_ea));
+    if (small) {
+        std::vector<int> lmT((size_t)k * nitems);
+        for (int e = 0; e < nitems; e++)
+            for (int j = 0; j < k; j++) lmT[(size_t)j * nitems + e] = lm[(size_t)e * k + j];
+        op->sm_mi = mi;
+        FG_CHECK(upload(iT, &op->d_sm_iT));
+        FG_CHECK(upload(lmT, &op->d_sm_lmT));
+    }
     *out = op;
     return FG_OK;
 }
@@ -591,7 +600,7 @@ void fg_ebe_free(fg_op *op)
     hipFree(op->d_kpos); hipFree(op->d_hdofs); hipFree(op->d_hoff); hipFree(op->d_zdofs);
     hipFree(op->d_hpart64); hipFree(op->d_hpart32); hipFree(op->d_inc_off);
     hipFree(op->d_inc_diag); hipFree(op->d_lm); hipFree(op->d_kstride); hipFree(op->d_rowmap);
-    hipFree(op->d_inc_ea);
+    hipFree(op->d_sm_iT); hipFree(op->d_sm_lmT);
 }
 
 // ---- diagonal of an element/edge operator, element order per dof ---------

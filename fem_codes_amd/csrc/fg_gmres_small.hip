@@ -13,8 +13,9 @@
 //    the Arnoldi vector w, of u_j and of u_{j+1} in registers for the whole
 //    Gram-Schmidt sweep (VPT = 4 takes 122-128 VGPRs without spilling, the
 //    budget of a 1024-thread workgroup);
-//  * a dot product is a wavefront reduction plus one LDS exchange and one
-//    workgroup barrier; every thread sums the 16 wavefront partials in the
+//  * a dot product is a DPP wavefront reduction (register crossbar; the
+//    shuffle form goes through LDS six times), one LDS exchange and one
+//    workgroup barrier; every thread sums the 16 wavefront totals in the
 //    same order, so all threads hold the same value and branch uniformly;
 //  * the slice of u_{j+1} is loaded before the reduction of h_{j,i}, so the
 //    L2 latency of the next basis vector hides behind the barrier;
@@ -23,6 +24,11 @@
 //    element into an L2-resident scratch array, a barrier, then each row
 //    adds its elements' row sums in element order (no atomics); CSR rows
 //    (csrmv.c:18-21) in column order;
+//  * one CU's vector L1 serves every access, so its address throughput, not
+//    bandwidth, limits the product: every list is stored transposed (entry
+//    c of row g at c*n + g, entry q of element e at q*nel + e) so that the
+//    64 lanes of a wavefront read one contiguous run.  The element and CSR
+//    values are copied into that order once per solve;
 //  * Diag (Diag_precond.c) and element Jacobi (Jacobi_precond_EBE.c) are
 //    row-local and are applied to each product row in registers.
 #include <math.h>
@@ -44,20 +50,24 @@ struct SRes {
 };
 
 struct SArgs {
-    int n, nel, kmax, lmax, hist_cap, pc;
+    int n, nel, mi, kmax, lmax, hist_cap, pc;
     long long ld;
     double tol;
+    const int *iT;          // [mi][n] transposed entry lists (EBE/EDE: rs slots; CSR: JA)
     // CSR
-    const int *IA, *JA;
+    const int *IA;
     const double *AA;
-    // EBE / EDE (blocked values, element-order incidences)
-    const int *ioff, *iea, *lm, *kstride;
+    double *AAT;            // [mi][n] values in iT order (per solve)
+    // EBE / EDE
+    const int *ioff, *lmT, *kstride;
     const long long *kpos;
-    const double *Kv;
-    double *rs;             // [nel*K] element row sums of the current product
+    const double *Kv;       // blocked values
+    double *KeT;            // [K*K][nel] values, entry-major (per solve)
+    double *rs;             // [K][nel] element row sums of the current product
     // preconditioners
     const double *invD;     // Diag: [n]
     const double *invde;    // element Jacobi: per incidence, ioff order
+    double *invdeT;         // [mi][n] the same in iT order (per solve)
     // vectors
     const double *B;
     double *X, *U;
@@ -65,57 +75,116 @@ struct SArgs {
     SRes *res;
 };
 
-// deterministic workgroup sum, returned in every thread.  Two LDS buffers
-// used alternately: a thread writes buffer b again only after the barrier of
-// the following reduction, which every thread reaches after reading b.
+// x from another lane by a DPP pattern (register crossbar, no LDS); lanes
+// outside the row mask or without a source lane read 0
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_from(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// sum over the wavefront, valid in lane 63: partial sums of 2, 4, 8, 16
+// consecutive lanes within each row of 16 (row_shr 1, 2, 4, 8), then row 1
+// and 3 add the last lane of the row below (row_bcast:15), rows 2 and 3 add
+// lane 31 (row_bcast:31).  A fixed order, six dependent steps.
+__device__ __forceinline__ double wave_total(double v)
+{
+    v = v + dpp_from<0x111, 0xF>(v);
+    v = v + dpp_from<0x112, 0xF>(v);
+    v = v + dpp_from<0x114, 0xF>(v);
+    v = v + dpp_from<0x118, 0xF>(v);
+    v = v + dpp_from<0x142, 0xA>(v);
+    v = v + dpp_from<0x143, 0xC>(v);
+    return v;
+}
+
+// deterministic workgroup sum, returned in every thread: the 16 wavefront
+// totals summed in a fixed order (four chains of four, then pairs).  Two LDS buffers used alternately: a
+// thread writes buffer b again only after the barrier of the following
+// reduction, which every thread reaches after reading b.
 __device__ __forceinline__ double wg_sum(double v, double (*ws)[NWAVE], int &par)
 {
-    v = fg_wave_sum(v);
+    v = wave_total(v);
     double *b = ws[par];
     par ^= 1;
-    if ((threadIdx.x & 63) == 0) b[threadIdx.x >> 6] = v;
+    if ((threadIdx.x & 63) == 63) b[threadIdx.x >> 6] = v;
     __syncthreads();
-    double t = b[0];
+    double t[4];
 #pragma unroll
-    for (int w = 1; w < NWAVE; w++) t = t + b[w];
-    return t;
+    for (int g = 0; g < 4; g++) t[g] = ((b[4 * g] + b[4 * g + 1]) + b[4 * g + 2]) + b[4 * g + 3];
+    return (t[0] + t[1]) + (t[2] + t[3]);
+}
+
+// per-solve copies in lane-coalesced order (entry c of row g at c*n + g,
+// entry q of element e at q*nel + e): element values from the blocked
+// layout, CSR values, element-Jacobi factors; then a barrier
+template <int K>
+__device__ __forceinline__ void layout_phase(const SArgs &a)
+{
+    const int n = a.n, mi = a.mi;
+    if constexpr (K == 0) {
+        for (int r = threadIdx.x; r < n; r += STPB) {
+            const int j0 = a.IA[r], len = a.IA[r + 1] - j0;
+            for (int c = 0; c < mi; c++) fg_at(a.AAT, (unsigned)(c * n + r)) = c < len ? a.AA[j0 + c] : 0.0;
+        }
+    } else {
+        constexpr int V = fg_ebe_vec(K);
+        for (int e = threadIdx.x; e < a.nel; e += STPB) {
+            const long long kp = a.kpos[e];
+            const int ks = a.kstride[e];
+#pragma unroll
+            for (int q = 0; q < K * K; q++)
+                fg_at(a.KeT, (unsigned)(q * a.nel + e)) = kp >= 0 ? a.Kv[fg_ebe_vidx(kp, ks, q, V)] : 0.0;
+        }
+        if (a.pc == PC_JACOBI)
+            for (int g = threadIdx.x; g < n; g += STPB) {
+                const int t0 = a.ioff[g], len = a.ioff[g + 1] - t0;
+                for (int c = 0; c < mi; c++)
+                    fg_at(a.invdeT, (unsigned)(c * n + g)) = c < len ? a.invde[t0 + c] : 0.0;
+            }
+    }
+    __syncthreads();
 }
 
 // element row sums of A*P (ebemv.c:34-36, left to right over the columns)
-// into rs[e*K + r], every element of the system; EU elements per thread in
-// flight
+// into rs[r*nel + e], every element; EU elements per thread in flight, the
+// values and dof numbers of all of them loaded before the gathers of P
 template <int K, int EU>
 __device__ __forceinline__ void element_rows(const SArgs &a, const double *__restrict__ P)
 {
-    constexpr int V = fg_ebe_vec(K);
-    for (int e0 = threadIdx.x; e0 < a.nel; e0 += EU * STPB) {
-        long long kp[EU];
-        int ks[EU], d[EU][K];
+    const int nel = a.nel;
+    for (int e0 = threadIdx.x; e0 < nel; e0 += EU * STPB) {
+        int d[EU][K];
+        double kv[EU][K * K];
 #pragma unroll
         for (int u = 0; u < EU; u++) {
             const int e = e0 + u * STPB;
-            kp[u] = e < a.nel ? a.kpos[e] : -1;   // -1: every dof prescribed, not stored
-            ks[u] = e < a.nel ? a.kstride[e] : 0;
+            const bool in = e < nel;
 #pragma unroll
-            for (int b = 0; b < K; b++) d[u][b] = e < a.nel ? a.lm[e * K + b] : a.n;
+            for (int b = 0; b < K; b++) d[u][b] = in ? fg_at(a.lmT, (unsigned)(b * nel + e)) : a.n;
+#pragma unroll
+            for (int q = 0; q < K * K; q++)
+                kv[u][q] = in ? fg_at(a.KeT, (unsigned)(q * nel + e)) : 0.0;
         }
         double p[EU][K];
 #pragma unroll
         for (int u = 0; u < EU; u++)
 #pragma unroll
-            for (int b = 0; b < K; b++)
-                p[u][b] = d[u][b] == a.n ? 0.0 : P[d[u][b]];   // P[neq] = 0 (ebemv.c:21)
+            for (int b = 0; b < K; b++)   // P[neq] = 0 (ebemv.c:21)
+                p[u][b] = d[u][b] == a.n ? 0.0 : fg_at(P, (unsigned)d[u][b]);
 #pragma unroll
         for (int u = 0; u < EU; u++) {
-            if (kp[u] < 0) continue;
             const int e = e0 + u * STPB;
+            if (e >= nel) continue;
 #pragma unroll
             for (int r = 0; r < K; r++) {
-                double sr = a.Kv[fg_ebe_vidx(kp[u], ks[u], r * K, V)] * p[u][0];
+                double sr = kv[u][r * K] * p[u][0];
 #pragma unroll
-                for (int b = 1; b < K; b++)
-                    sr = sr + a.Kv[fg_ebe_vidx(kp[u], ks[u], r * K + b, V)] * p[u][b];
-                a.rs[e * K + r] = sr;
+                for (int b = 1; b < K; b++) sr = sr + kv[u][r * K + b] * p[u][b];
+                fg_at(a.rs, (unsigned)(r * nel + e)) = sr;
             }
         }
     }
@@ -123,54 +192,52 @@ __device__ __forceinline__ void element_rows(const SArgs &a, const double *__res
 
 // rows tid + v*STPB of M^{-1} A P.  EBE/EDE: each row adds the row sums of
 // its elements in element order (after element_rows and a barrier); CSR:
-// AA[j]*P[JA[j]] in column order (csrmv.c:18-21).  The entry loop runs in
-// chunks of CU with one trip count for all VPT rows, so the loads of every
-// row and chunk are in flight together; each row still adds its terms in
-// its own order.
+// AA[j]*P[JA[j]] in column order (csrmv.c:18-21).  Entry c of every row
+// comes from the transposed lists, so a wavefront's loads are contiguous;
+// CU entries per row are in flight at once, and each row still adds its
+// terms in its own order.
 template <int K, int VPT>
 __device__ __forceinline__ void apply_rows(const SArgs &a, const double *__restrict__ P,
                                            double (&q)[VPT])
 {
-    constexpr int CU = VPT >= 4 ? (K >= 3 ? 1 : 2) : 8 / VPT;   // terms in flight per row
-    const int n = a.n;
+    constexpr int CU = VPT >= 4 ? 2 : 8 / VPT;
+    const int n = a.n, mi = a.mi;
     if constexpr (K != 0) {
-        element_rows<K, (VPT >= 4 ? 1 : 2)>(a, P);   // VPT = 4: no spill
+        element_rows<K, (K == 4 ? 1 : 2)>(a, P);
         __syncthreads();
     }
-    const int *off_arr = K == 0 ? a.IA : a.ioff;
-    int off[VPT], cnt[VPT], cmax = 0;
 #pragma unroll
-    for (int v = 0; v < VPT; v++) {
-        const int row = threadIdx.x + v * STPB;
-        off[v] = row < n ? off_arr[row] : 0;
-        cnt[v] = row < n ? off_arr[row + 1] - off[v] : 0;
-        cmax = cnt[v] > cmax ? cnt[v] : cmax;
-        q[v] = 0.0;
-    }
-    for (int c = 0; c < cmax; c += CU) {
+    for (int v = 0; v < VPT; v++) q[v] = 0.0;
+    for (int c = 0; c < mi; c += CU) {
         double t[VPT][CU];
+        bool ok[VPT][CU];
 #pragma unroll
         for (int v = 0; v < VPT; v++)
 #pragma unroll
             for (int u = 0; u < CU; u++) {
-                const int j = off[v] + c + u;
+                const int row = threadIdx.x + v * STPB;
+                const unsigned at = (unsigned)((c + u) * n + row);
+                const int idx = (row < n && c + u < mi) ? fg_at(a.iT, at) : -1;
+                ok[v][u] = idx >= 0;
                 t[v][u] = 0.0;
-                if (c + u < cnt[v]) {
-                    if constexpr (K == 0) t[v][u] = a.AA[j] * P[a.JA[j]];
-                    else t[v][u] = a.rs[a.iea[j]];
+                if (idx >= 0) {
+                    if constexpr (K == 0)
+                        t[v][u] = fg_at(a.AAT, at) * (idx == n ? 0.0 : fg_at(P, (unsigned)idx));
+                    else
+                        t[v][u] = fg_at(a.rs, (unsigned)idx);
                 }
             }
 #pragma unroll
         for (int v = 0; v < VPT; v++)
 #pragma unroll
             for (int u = 0; u < CU; u++)
-                if (c + u < cnt[v]) q[v] = q[v] + t[v][u];
+                if (ok[v][u]) q[v] = q[v] + t[v][u];
     }
     if (a.pc == PC_DIAG) {
 #pragma unroll
         for (int v = 0; v < VPT; v++) {
             const int row = threadIdx.x + v * STPB;
-            if (row < n) q[v] = q[v] * a.invD[row];
+            if (row < n) q[v] = q[v] * fg_at(a.invD, (unsigned)row);
         }
     } else if (a.pc == PC_JACOBI) {
         // Jacobi_precond_EBE.c: z_g = sum over the elements of g, in element
@@ -180,7 +247,10 @@ __device__ __forceinline__ void apply_rows(const SArgs &a, const double *__restr
             const int row = threadIdx.x + v * STPB;
             if (row >= n) continue;
             double z = 0.0;
-            for (int t = a.ioff[row]; t < a.ioff[row + 1]; t++) z = z + q[v] * a.invde[t];
+            for (int c = 0; c < mi; c++) {
+                const unsigned at = (unsigned)(c * n + row);
+                if (fg_at(a.iT, at) >= 0) z = z + q[v] * fg_at(a.invdeT, at);
+            }
             q[v] = z;
         }
     }
@@ -197,15 +267,17 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
     const long long ld = a.ld;
     int par = 0;
 
-    // eps = tol * ||B|| (pgmres.c:49-53) ; X = 0 (:46)
+    layout_phase<K>(a);
+    // eps = tol * ||B|| (pgmres.c:49-53) ; X = 0 (:46), the sink slot too
+    if (tid == 0) a.X[n] = 0.0;
     double acc = 0.0;
 #pragma unroll
     for (int v = 0; v < VPT; v++) {
         const int row = tid + v * STPB;
         if (row < n) {
-            const double b = a.B[row];
+            const double b = fg_at(a.B, (unsigned)row);
             acc = acc + b * b;
-            a.X[row] = 0.0;
+            fg_at(a.X, (unsigned)row) = 0.0;
         }
     }
     const double eps = a.tol * sqrt(wg_sum(acc, ws, par));
@@ -221,7 +293,7 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
 #pragma unroll
         for (int v = 0; v < VPT; v++) {
             const int row = tid + v * STPB;
-            r[v] = row < n ? a.B[row] - r[v] : 0.0;
+            r[v] = row < n ? fg_at(a.B, (unsigned)row) - r[v] : 0.0;
             acc = acc + r[v] * r[v];
         }
         e0 = sqrt(wg_sum(acc, ws, par));
@@ -243,7 +315,7 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
 #pragma unroll
             for (int v = 0; v < VPT; v++) {
                 const int row = tid + v * STPB;
-                if (row < n) a.U[row] = sc * r[v];
+                if (row < n) fg_at(a.U, (unsigned)row) = sc * r[v];
             }
             if (tid == 0) {
                 a.U[n] = 0.0;
@@ -252,7 +324,7 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
             __syncthreads();
             int m = 0;
             for (int i = 0; i < a.kmax; i++) {
-                const double *Ui = a.U + (size_t)i * ld;
+                const double *Ui = a.U + (size_t)i * ld;   // uniform
                 // w = M^{-1} A u_i ; h_{0,i} = (u_0, w)
                 double w[VPT], cur[VPT], nxt[VPT];
                 apply_rows<K, VPT>(a, Ui, w);
@@ -260,13 +332,13 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
 #pragma unroll
                 for (int v = 0; v < VPT; v++) {
                     const int row = tid + v * STPB;
-                    cur[v] = row < n ? a.U[row] : 0.0;
+                    cur[v] = row < n ? fg_at(a.U, (unsigned)row) : 0.0;
                     acc = acc + cur[v] * w[v];
                 }
 #pragma unroll
                 for (int v = 0; v < VPT; v++) {
                     const int row = tid + v * STPB;
-                    nxt[v] = (i >= 1 && row < n) ? a.U[ld + row] : 0.0;
+                    nxt[v] = (i >= 1 && row < n) ? fg_at(a.U + ld, (unsigned)row) : 0.0;
                 }
                 double h = wg_sum(acc, ws, par);
                 if (tid == 0) sH[i] = h;
@@ -285,7 +357,7 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
 #pragma unroll
                         for (int v = 0; v < VPT; v++) {
                             const int row = tid + v * STPB;
-                            nxt[v] = row < n ? Un[row] : 0.0;
+                            nxt[v] = row < n ? fg_at(Un, (unsigned)row) : 0.0;
                         }
                     }
                     h = wg_sum(acc, ws, par);
@@ -305,7 +377,7 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
 #pragma unroll
                 for (int v = 0; v < VPT; v++) {
                     const int row = tid + v * STPB;
-                    if (row < n) U1[row] = s * w[v];
+                    if (row < n) fg_at(U1, (unsigned)row) = s * w[v];
                 }
                 // Givens rotations (:124-159)
                 if (tid == 0) {
@@ -355,9 +427,10 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
             for (int v = 0; v < VPT; v++) {
                 const int row = tid + v * STPB;
                 if (row < n) {
-                    double x = a.X[row];
-                    for (int j = 0; j < m; j++) x = x + a.U[(size_t)j * ld + row] * s_y[j];
-                    a.X[row] = x;
+                    double x = fg_at(a.X, (unsigned)row);
+                    for (int j = 0; j < m; j++)
+                        x = x + fg_at(a.U + (size_t)j * ld, (unsigned)row) * s_y[j];
+                    fg_at(a.X, (unsigned)row) = x;
                 }
             }
             __syncthreads();   // X of every row before the next residual
@@ -383,7 +456,7 @@ __global__ __launch_bounds__(STPB) void k_gmres_small(SArgs a)
         for (int v = 0; v < VPT; v++) {
             const int row = tid + v * STPB;
             if (row < n) {
-                const double d = a.B[row] - t[v];
+                const double d = fg_at(a.B, (unsigned)row) - t[v];
                 acc = acc + d * d;
             }
         }
@@ -421,10 +494,11 @@ bool fg_gmres_small_applies(const fg_op *op, const fg_prec *pc, const fg_gmres_o
     if (o->precision != FG_GMRES_FP64) return false;
     const int n = op->neq;
     if (n < 1 || n > FG_SMALL_MAX || o->kmax > KM) return false;
+    if (!op->d_sm_iT) return false;   // built at creation for systems of this size
     if (op->kind == FG_OP_CSR) {
         if (!op->d_IA) return false;
     } else if (op->kind == FG_OP_EBE || op->kind == FG_OP_EDE) {
-        if (!(op->k == 2 || op->k == 3 || op->k == 4) || !op->d_inc_ea) return false;
+        if (!(op->k == 2 || op->k == 3 || op->k == 4)) return false;
     } else {
         return false;
     }
@@ -445,11 +519,13 @@ int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X
     memset(&a, 0, sizeof a);
     a.n = n; a.kmax = o->kmax; a.lmax = o->lmax; a.ld = ld; a.tol = o->tol;
     a.hist_cap = o->rho_hist ? o->hist_cap : 0;
+    a.mi = op->sm_mi;
+    a.iT = op->d_sm_iT;
     if (op->kind == FG_OP_CSR) {
-        a.IA = op->d_IA; a.JA = op->d_JA; a.AA = op->d_AA;
+        a.IA = op->d_IA; a.AA = op->d_AA;
     } else {
         a.nel = op->nitems;
-        a.ioff = op->d_inc_off; a.iea = op->d_inc_ea; a.lm = op->d_lm;
+        a.ioff = op->d_inc_off; a.lmT = op->d_sm_lmT;
         a.kpos = op->d_kpos; a.kstride = op->d_kstride; a.Kv = op->d_K64;
     }
     a.pc = PC_NONE;
@@ -459,7 +535,6 @@ int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X
     } else if (pc && pc->kind == FG_PREC_JACOBI) {
         a.pc = PC_JACOBI;
         a.invde = pc->d_invde;
-        a.ioff = op->d_inc_off;
     }
     a.B = B;
     a.X = X;
@@ -469,11 +544,18 @@ int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X
     const size_t szU = sizeof(double) * ld * (o->kmax + 1);
     const size_t szH = sizeof(double) * (a.hist_cap > 0 ? a.hist_cap : 0);
     const size_t szE = sizeof(double) * ne0;
-    const size_t szS = op->kind == FG_OP_CSR ? 0 : sizeof(double) * op->nitems * op->k;
-    const size_t off_R = szU, off_H = off_R + 256, off_E = off_H + (szH + 255) / 256 * 256;
-    const size_t off_S = off_E + (szE + 255) / 256 * 256;
+    const bool csr = op->kind == FG_OP_CSR;
+    const size_t nmi = (size_t)op->sm_mi * n;
+    const size_t szS = csr ? 0 : sizeof(double) * op->nitems * op->k;             // rs
+    const size_t szK = csr ? 0 : sizeof(double) * op->nitems * op->k * op->k;     // KeT
+    const size_t szA = csr ? sizeof(double) * nmi : 0;                              // AAT
+    const size_t szJ = a.pc == PC_JACOBI ? sizeof(double) * nmi : 0;                // invdeT
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t off_R = up(szU), off_H = off_R + 256, off_E = off_H + up(szH);
+    const size_t off_S = off_E + up(szE), off_K = off_S + up(szS), off_A = off_K + up(szK);
+    const size_t off_J = off_A + up(szA);
     void *buf = nullptr;
-    FG_HIP(hipMalloc(&buf, off_S + szS));
+    FG_HIP(hipMalloc(&buf, off_J + szJ + 256));
     struct Free {
         void *p;
         ~Free() { hipFree(p); }
@@ -484,6 +566,9 @@ int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X
     a.rh = szH ? (double *)(base + off_H) : nullptr;
     a.e0h = szE ? (double *)(base + off_E) : nullptr;
     a.rs = szS ? (double *)(base + off_S) : nullptr;
+    a.KeT = szK ? (double *)(base + off_K) : nullptr;
+    a.AAT = szA ? (double *)(base + off_A) : nullptr;
+    a.invdeT = szJ ? (double *)(base + off_J) : nullptr;
 
     FG_HIP(hipEventRecord(c->ev0, c->stream));
     if (op->kind == FG_OP_CSR) FG_CHECK(launch_k<0>(c, a, vpt));

@@ -102,8 +102,6 @@ struct fg_op {
     // incidence (dof -> element-local entries), element order: diagonal positions
     int *d_inc_off = nullptr;    // [neq+1]
     long long *d_inc_diag = nullptr;  // [ninc] index of K_jj in d_K64 for that incidence
-    int *d_inc_ea = nullptr;     // [ninc] e*k + j of that incidence (systems of at most
-                                 // FG_SMALL_MAX equations: the one-workgroup solver)
     int *d_lm = nullptr;         // [nitems*k] original lm (AoS) for element-local preconditioners
 
     // ---- CSR ----
@@ -113,6 +111,13 @@ struct fg_op {
     int nrb = 0;
     int *d_rb = nullptr;         // [nrb+1] row-block boundaries
     std::vector<int> h_IA, h_JA;
+
+    // ---- systems of at most FG_SMALL_MAX equations: lane-coalesced lists for
+    // the one-workgroup solver, entry c of row g at c*neq + g (-1: none) ----
+    int sm_mi = 0;               // entries per row (max over the rows)
+    int *d_sm_iT = nullptr;      // EBE/EDE: r*nitems + e of the c-th element of row g
+                                 // (element order); CSR: JA of the c-th entry of row g
+    int *d_sm_lmT = nullptr;     // EBE/EDE: lm transposed, [k][nitems]
 
     // ---- EBE2 (shares the storage of `base` and the blocks of `bd`) ----
     fg_op *base = nullptr;
