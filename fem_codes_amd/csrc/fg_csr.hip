@@ -28,7 +28,10 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_stream(
     const int r0 = rb[b], r1 = rb[b + 1];
     const int j0 = IA[r0], j1 = IA[r1];
     if (j1 - j0 <= FG_CSR_NNZB) {
-        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB) s[j - j0] = AA[j] * P[JA[j]];
+        // AA and JA are read once per product: non-temporal, so that they do
+        // not evict the P entries other row blocks gather again
+        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB)
+            s[j - j0] = __builtin_nontemporal_load(&AA[j]) * P[__builtin_nontemporal_load(&JA[j])];
         __syncthreads();
         for (int r = r0 + threadIdx.x; r < r1; r += FG_TPB) {
             T acc = T(0);

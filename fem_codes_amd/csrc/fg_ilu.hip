@@ -473,7 +473,9 @@ __global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nrb, const int *__rest
     const int r0 = rb[b], r1 = rb[b + 1];
     const int j0 = IA[r0], j1 = IA[r1];
     if (j1 - j0 <= TRI_NNZB) {
-        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB) s[j - j0] = V[j] * x[JA[j]];
+        // the factor's values and columns are read once per sweep: non-temporal
+        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB)
+            s[j - j0] = __builtin_nontemporal_load(&V[j]) * x[__builtin_nontemporal_load(&JA[j])];
         __syncthreads();
         for (int r = r0 + threadIdx.x; r < r1; r += FG_TPB) {
             T acc = T(0);

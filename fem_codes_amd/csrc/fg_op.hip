@@ -152,19 +152,18 @@ extern "C" int fg_op_time_apply(fg_op *op, int fp32, void *P, void *Q, int reps,
         *ms = (double)t / reps;
         return FG_OK;
     }
-    double tot = 0.0;
-    for (int r = 0; r < reps; r++) {
-        FG_HIP(hipEventRecord(c->ev0, c->stream));
-        FG_CHECK(fg_ebe_apply_main(op, fp32, P, Q));
-        FG_HIP(hipEventRecord(c->ev1, c->stream));
-        FG_HIP(hipEventSynchronize(c->ev1));
-        float t = 0.f;
-        FG_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
-        tot += t;
-    }
-    FG_CHECK(fg_op_apply_any(op, fp32, P, Q));   // leave Q complete
+    // the block kernel alone, launched back to back between two events (each
+    // launch rewrites the same Q entries and halo partials, so repeats are
+    // idempotent), then one full product so that Q is complete
+    FG_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; r++) FG_CHECK(fg_ebe_apply_main(op, fp32, P, Q));
+    FG_HIP(hipEventRecord(c->ev1, c->stream));
+    FG_HIP(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    FG_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    FG_CHECK(fg_op_apply_any(op, fp32, P, Q));
     FG_HIP(hipStreamSynchronize(c->stream));
-    *ms = tot / reps;
+    *ms = (double)t / reps;
     return FG_OK;
 }
 

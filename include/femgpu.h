@@ -102,7 +102,8 @@ double fg_op_algorithmic_bytes(const fg_op *op, int fp32);
 /* time `reps` back-to-back applies with HIP events on the context stream;
  * returns the mean milliseconds per apply in *ms.  main_only = 1 times the
  * dominant kernel alone (EBE/EDE: the block kernel without the halo pass),
- * one event pair around each launch. */
+ * its launches back to back between one event pair, then one full product
+ * so that Q is complete. */
 int fg_op_time_apply(fg_op *op, int fp32, void *P, void *Q, int reps, int main_only,
                      double *ms);
 /* EBE/EDE launch schedule (no reference counterpart: a tuning knob of the
