@@ -320,6 +320,55 @@ def ss3d_leg(F, ctx, n=128, sweeps=4, seed=20260303):
     return res
 
 
+def transient_leg(F, ctx, n=1024, steps=10):
+    """2D transient transport, the reference's CONE problem (02_Transport_
+    Equation/2D/CONE: kappa 1e-8, no source or reaction, boundary value 0,
+    velocity the rotation about (5,5), initial state exp(-r^2/2) about
+    (5,7.5)) on the n x n grid of P1 triangles of [0,10]^2; Predictor_Old
+    with alpha 1/2, dt 1e-3 (Courant number below 1 near the cone at
+    n = 1024), corrections stopped by ||Da|| < 1e-3 ||a|| (at most 10),
+    `steps` time steps; each correction is the transient SUPG + CAU build on
+    the device and a Diag-preconditioned GMRES(30) solve to 1e-8 (at most 200
+    restarts), FP64 and FP32-IR.  u, a, Da, R and the element matrices stay
+    in HBM for the whole run."""
+    x, y, t, v = F.structured_mesh2d(n)
+    x, y = 10.0 * x, 10.0 * y
+    mesh = F.Mesh2D(ctx, x, y, t, v)
+    neq = mesh.neq
+    op = F.Operator.ebe(ctx, neq, mesh.lm(), 3)
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    pb = F.Supg2dProblem(F.P_ROTATE_ABOUT, 1e-8, 0.0, 0.0, 5.0, 5.0, 1.0, F.H_2SQRTAREA)
+    up = ctx.to_device(np.zeros(len(x)))
+    _, idv = F.number_nodes(t)
+    u0 = np.zeros(neq + 1)
+    free = t > 0
+    u0[idv[free]] = np.exp(-0.5 * ((x[free] - 5.0) ** 2 + (y[free] - 7.5) ** 2))
+    dt = 1e-3
+    res = {"n": n, "equations": neq, "elements": mesh.nel, "steps": steps, "dt": dt,
+           "problem": "CONE", "precond": "Diag", "kmax": 30, "tol": 1e-8}
+    U = {}
+    for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+        u = ctx.to_device(u0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        info, hist = F.predictor_old_transport(
+            mesh, pb, op, pc, up, u, alpha=0.5, dt=dt, final_time=steps * dt + 0.5 * dt,
+            nl_tol=1e-3, nl_maxiter=10, stop_by_norm=True, steady=False, max_steps=steps,
+            kmax=30, lmax=200, tol=1e-8, precision=prec, engine=F.ENGINE_KERNELS,
+            orth=F.ORTH_CGS if prec == F.GMRES_FP32_IR else F.ORTH_MGS)
+        wall = time.perf_counter() - t0
+        U[tag] = u.get()
+        res[tag] = dict(seconds=wall, ms_per_step=1e3 * wall / max(1, info.steps),
+                        time_steps=info.steps, corrections=info.corrections,
+                        corrections_per_step=hist.tolist(),
+                        krylov_steps=int(info.krylov_steps), unconverged=info.unconverged)
+        log(f"transient {tag}", res[tag])
+    res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
+    res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(U["fp32_ir"][:neq] - U["fp64"][:neq]) /
+                                           np.linalg.norm(U["fp64"][:neq]))
+    return res
+
+
 def euler_leg(F, ctx, n=1024, seed=20260305):
     """2D Euler: the n x n grid of P1 triangles, 4 conserved variables per
     node, a uniform Mach-3 inflow (rho 1 with a seeded 1% perturbation, u 1,
@@ -421,6 +470,8 @@ def main():
                          "32..512 cells per side (256 = the batched workload)")
     ap.add_argument("--ss3d", type=int, default=0, metavar="N",
                     help="also run the 3D leg on the N^3 Kuhn cube (128 = the 3D workload)")
+    ap.add_argument("--transient", type=int, default=0, metavar="N",
+                    help="also run the transient transport leg (Predictor_Old) on an N^2 grid")
     ap.add_argument("--euler", type=int, default=0, metavar="N",
                     help="also run the Euler leg on the N x N grid (1024 = the Euler workload)")
     ap.add_argument("--matvec-only", action="store_true",
@@ -573,6 +624,7 @@ def main():
         batched = batched_leg(F, ctx, args.batched, args.seed + 4, prec=args.prec)
     ss3d = ss3d_leg(F, ctx, args.ss3d) if args.ss3d > 0 else None
     euler = euler_leg(F, ctx, args.euler) if args.euler > 0 else None
+    transient = transient_leg(F, ctx, args.transient) if args.transient > 0 else None
 
     cpu = None
     if not args.no_cpu and not args.matvec_only:
@@ -601,7 +653,7 @@ def main():
         "solve": solve,
         "batched": batched,
         "ss3d": ss3d,
-        "euler": euler,
+        "euler": euler, "transient": transient,
     }
     print(json.dumps(out))
 

@@ -62,6 +62,20 @@ class GmresInfo(C.Structure):
                 ("ms", C.c_double)]
 
 
+class PredictorParams(C.Structure):
+    """fg_predictor_params: Predictor_Old / Criteria.c parameters"""
+    _fields_ = [("alpha", C.c_double), ("dt", C.c_double), ("final_time", C.c_double),
+                ("nl_tol", C.c_double), ("nl_maxiter", C.c_int), ("stop_by_norm", C.c_int),
+                ("steady", C.c_int), ("time_tol", C.c_double), ("max_steps", C.c_int)]
+
+
+class PredictorInfo(C.Structure):
+    """fg_predictor_info"""
+    _fields_ = [("steps", C.c_int), ("corrections", C.c_int), ("krylov_steps", C.c_longlong),
+                ("unconverged", C.c_int), ("t", C.c_double), ("ms", C.c_double),
+                ("status", C.c_int)]
+
+
 _vp, _i, _d = C.c_void_p, C.c_int, C.c_double
 _ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -114,6 +128,9 @@ SIGNATURES = {
     "fg_mesh2d_destroy": (_i, [_vp]),
     "fg_supg2d_build": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "fg_supg2d_transient_build": (_i, [_vp, _vp, _d, _d, _vp, _vp, _vp, _vp]),
+    "fg_predictor_old_transport": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(PredictorParams),
+                                        C.POINTER(GmresOpts), _vp, _vp, _i,
+                                        C.POINTER(PredictorInfo)]),
     "fg_op_assemble_csr": (_i, [_vp, _i, _ip, _vp]),
     "fg_op_assemble_ede": (_i, [_vp, _i, _ip, _ip, _vp]),
     "fg_csr_pattern_k": (_i, [_i, _i, _i, _ip, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
@@ -490,6 +507,28 @@ def gmres(ctx: Context, op: Operator, prec: Preconditioner | None, B: DeviceArra
                       C.byref(o), C.byref(info))
     _check(rc, "fg_gmres", ok=(FG_OK, FG_ERR_NOCONV))
     return info, rho[:min(info.steps, hist_cap)].copy(), e0[:info.cycles + 1].copy()
+
+
+def predictor_old_transport(mesh: "Mesh2D", prob: "Supg2dProblem", op: Operator,
+                            prec: Preconditioner | None, upresc: DeviceArray, u: DeviceArray,
+                            alpha, dt, final_time, nl_tol, nl_maxiter, stop_by_norm=True,
+                            steady=False, time_tol=0.0, max_steps=0, kmax=30, lmax=1000,
+                            tol=1e-8, precision=GMRES_FP64, engine=ENGINE_AUTO,
+                            orth=ORTH_MGS, hist_cap=4096):
+    """Predictor_Old (Time_Integration/Predictor_Old.c:4-145) over the
+    transient build, on the device; u (neq+1) holds the initial solution and
+    receives the result.  Returns (info, corrections per step)."""
+    pp = PredictorParams(alpha, dt, final_time, nl_tol, nl_maxiter, int(stop_by_norm),
+                         int(steady), time_tol, max_steps)
+    o = GmresOpts(kmax, lmax, tol, precision, 0.0, 0, None, None, engine, orth)
+    hist = np.zeros(max(1, hist_cap), np.int32)
+    info = PredictorInfo()
+    rc = lib.fg_predictor_old_transport(mesh.h, C.byref(prob), op.h,
+                                        prec.h if prec is not None else None, upresc.ptr,
+                                        C.byref(pp), C.byref(o), u.ptr, hist.ctypes.data,
+                                        hist_cap, C.byref(info))
+    _check(rc, "fg_predictor_old_transport", ok=(FG_OK, FG_ERR_NOCONV))
+    return info, hist[:min(info.steps, hist_cap)].copy()
 
 
 # ---------------------------------------------------------------------------

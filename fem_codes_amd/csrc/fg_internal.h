@@ -140,6 +140,9 @@ void fg_nodeblk_destroy(fg_nodeblk *b);
 int fg_nodeblk_setup(struct fg_prec *pc, double *F);
 int fg_nodeblk_apply(struct fg_prec *pc, int fp32, const void *p, void *z, const int *skip);
 int fg_ebe2_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
+int fg_mesh2d_eval_U_dU(const fg_mesh2d *m, const double *u, const double *du,
+                        const double *upresc, double *U, double *dU);
+fg_ctx *fg_mesh2d_ctx(const fg_mesh2d *m);
 double fg_ebe2_bytes(const fg_op *op, int fp32);
 
 struct fg_prec {

@@ -238,6 +238,35 @@ extern "C" int fg_mesh2d_create(fg_ctx *c, int nnodes, int nel, const double *x,
     return FG_OK;
 }
 
+// eval_U_dU.c of the 02 driver: nodal U, dU from u and du on the free nodes
+// (Type > 0), the prescribed value and 0 on the others
+__global__ void k_eval_U_dU_tr(int nn, const int *__restrict__ type, const int *__restrict__ id,
+                               const double *__restrict__ u, const double *__restrict__ du,
+                               const double *__restrict__ upresc, double *__restrict__ U,
+                               double *__restrict__ dU)
+{
+    for (int i = blockIdx.x * FG_TPB + threadIdx.x; i < nn; i += gridDim.x * FG_TPB) {
+        if (type[i] > 0) {
+            U[i] = u[id[i]];
+            dU[i] = du[id[i]];
+        } else {
+            U[i] = upresc[i];
+            dU[i] = 0.0;
+        }
+    }
+}
+
+int fg_mesh2d_eval_U_dU(const fg_mesh2d *m, const double *u, const double *du,
+                        const double *upresc, double *U, double *dU)
+{
+    fg_ctx *c = m->ctx;
+    FG_LAUNCH(k_eval_U_dU_tr, fg_vec_grid(c, m->nnodes), FG_TPB, 0, c->stream, m->nnodes,
+              m->d_type, m->d_id, u, du, upresc, U, dU);
+    return FG_OK;
+}
+
+fg_ctx *fg_mesh2d_ctx(const fg_mesh2d *m) { return m->ctx; }
+
 extern "C" int fg_mesh2d_info(const fg_mesh2d *m, int *neq, int *nel, int *nnodes)
 {
     FG_ARG(m, "mesh");

@@ -1,0 +1,199 @@
+// fg_time.hip -- the predictor/multicorrector time integrator on the device.
+//
+// Time_Integration/Predictor_Old.c:4-145 with Criteria.c:25-72.  The loop
+// control stays on the host, as in the reference; the state it updates (u,
+// a = du, Da, u_old, R, the element matrices, nodal U/dU) lives in HBM for
+// the whole run, and each correction moves two doubles to the host: ||a||^2
+// and ||Da||^2 for the stopping test (two more per step for the steady-state
+// test).  Every vector update is one pass over neq entries, fused with the
+// partial sums of the norms that follow it; the partial sums are added in
+// block order by one workgroup, so a run is deterministic.
+#include <math.h>
+#include <string.h>
+
+#include <chrono>
+
+#include "fg_internal.h"
+
+namespace {
+
+constexpr int PG = 1024;   // blocks of the vector kernels (partial sums)
+
+// predictor (:57-62): u_old = u ; u += ((1 - alpha) dt) a ; a = 0
+__global__ __launch_bounds__(FG_TPB) void k_predict(int n, double c, double *__restrict__ u,
+                                                    double *__restrict__ u_old,
+                                                    double *__restrict__ a)
+{
+    for (int i = blockIdx.x * FG_TPB + threadIdx.x; i < n; i += gridDim.x * FG_TPB) {
+        const double ui = u[i];
+        u_old[i] = ui;
+        u[i] = ui + c * a[i];
+        a[i] = 0.0;
+    }
+}
+
+// corrector (:115-119): a += 1 Da ; u += (alpha dt) Da (daxpy.c:4-21) ;
+// partial sums of a.a and Da.Da
+__global__ __launch_bounds__(FG_TPB) void k_correct(int n, double s, const double *__restrict__ Da,
+                                                    double *__restrict__ a,
+                                                    double *__restrict__ u,
+                                                    double *__restrict__ part)
+{
+    __shared__ double ws[FG_TPB / 64];
+    double sa = 0.0, sd = 0.0;
+    for (int i = blockIdx.x * FG_TPB + threadIdx.x; i < n; i += gridDim.x * FG_TPB) {
+        const double d = Da[i];
+        const double ai = a[i] + 1.0 * d;
+        a[i] = ai;
+        u[i] = u[i] + s * d;
+        sa = sa + ai * ai;
+        sd = sd + d * d;
+    }
+    sa = fg_block_sum(sa, ws);
+    sd = fg_block_sum(sd, ws);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sa;
+        part[gridDim.x + blockIdx.x] = sd;
+    }
+}
+
+// StopBySteadyState (:44-63): partial sums of (u - u_old).(u - u_old), u.u
+__global__ __launch_bounds__(FG_TPB) void k_steady(int n, const double *__restrict__ u,
+                                                   const double *__restrict__ u_old,
+                                                   double *__restrict__ part)
+{
+    __shared__ double ws[FG_TPB / 64];
+    double sdiff = 0.0, su = 0.0;
+    for (int i = blockIdx.x * FG_TPB + threadIdx.x; i < n; i += gridDim.x * FG_TPB) {
+        const double ui = u[i], di = ui - u_old[i];
+        sdiff = sdiff + di * di;
+        su = su + ui * ui;
+    }
+    sdiff = fg_block_sum(sdiff, ws);
+    su = fg_block_sum(su, ws);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sdiff;
+        part[gridDim.x + blockIdx.x] = su;
+    }
+}
+
+// the two totals of G block partials each, in block order (one workgroup)
+__global__ __launch_bounds__(FG_TPB) void k_sum2(const double *__restrict__ part, int G,
+                                                 double *__restrict__ out)
+{
+    __shared__ double ws[FG_TPB / 64];
+    for (int k = 0; k < 2; k++) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < G; b += FG_TPB) acc = acc + part[k * G + b];
+        acc = fg_block_sum(acc, ws);
+        if (threadIdx.x == 0) out[k] = acc;
+    }
+}
+
+struct Work {
+    void *buf = nullptr;
+    ~Work() { hipFree(buf); }
+};
+
+int sums(fg_ctx *c, const double *part, int G, double *dev_out, double host_out[2])
+{
+    FG_LAUNCH(k_sum2, 1, FG_TPB, 0, c->stream, part, G, dev_out);
+    FG_HIP(hipMemcpyAsync(host_out, dev_out, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    return FG_OK;
+}
+
+}  // namespace
+
+extern "C" int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem *pb, fg_op *A,
+                                          fg_prec *pc, const double *upresc,
+                                          const fg_predictor_params *p,
+                                          const fg_gmres_opts *gm, double *u,
+                                          int *corr_hist, int hist_cap,
+                                          fg_predictor_info *info)
+{
+    FG_ARG(m && pb && A && upresc && p && gm && u, "mesh/problem/operator/upresc/params/opts/u");
+    int neq = 0, nel = 0, nn = 0;
+    FG_CHECK(fg_mesh2d_info(m, &neq, &nel, &nn));
+    fg_ctx *c = fg_mesh2d_ctx(m);
+    FG_ARG(A->ctx == c && A->kind == FG_OP_EBE && A->k == 3 && A->neq == neq && A->nitems == nel,
+           "A must be the k = 3 EBE operator of the mesh's lm");
+    FG_ARG(!pc || pc->op == A, "the preconditioner must be built on A");
+    FG_ARG(p->dt > 0.0 && p->nl_maxiter >= 1, "dt > 0 and nl_maxiter >= 1");
+    FG_HIP(hipSetDevice(c->device));
+
+    // workspace: a, Da, u_old, R (neq+1), U, dU (nnodes), element matrices,
+    // partial sums, two totals
+    const size_t nv = (size_t)neq + 1, ne = (size_t)nel * 9;
+    Work w;
+    const size_t bytes = sizeof(double) * (4 * nv + 2 * (size_t)nn + (ne ? ne : 1) + 2 * PG + 2);
+    FG_HIP(hipMalloc(&w.buf, bytes));
+    double *a = (double *)w.buf, *Da = a + nv, *u_old = Da + nv, *R = u_old + nv;
+    double *U = R + nv, *dU = U + nn, *ke = dU + nn, *part = ke + (ne ? ne : 1);
+    double *tot = part + 2 * PG;
+    FG_HIP(hipMemsetAsync(w.buf, 0, bytes, c->stream));
+
+    const int G = fg_vec_grid(c, neq) < PG ? fg_vec_grid(c, neq) : PG;
+    const double alpha = p->alpha, dt = p->dt;
+    const double cpred = (1.0 - alpha) * dt, scorr = alpha * dt;
+    fg_predictor_info inf;
+    memset(&inf, 0, sizeof inf);
+    // host clock: the loop synchronises at every correction anyway, and the
+    // context's event pair is taken by fg_gmres
+    FG_HIP(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    double t = 0.0;
+    int passo = 0;
+    for (;;) {
+        passo++;
+        t += dt;
+        if (neq > 0) FG_LAUNCH(k_predict, G, FG_TPB, 0, c->stream, neq, cpred, u, u_old, a);
+        int i = 0;
+        for (;;) {
+            i++;
+            FG_CHECK(fg_mesh2d_eval_U_dU(m, u, a, upresc, U, dU));
+            FG_CHECK(fg_supg2d_transient_build(m, pb, alpha, dt, U, dU, ke, R));
+            FG_CHECK(fg_op_set_values_dev(A, ke));
+            if (pc) FG_CHECK(fg_prec_setup(pc, R));
+            fg_gmres_info gi;
+            const int rc = fg_gmres(c, A, pc, R, Da, gm, &gi);
+            if (rc == FG_ERR_NOCONV) inf.unconverged++;
+            else if (rc != FG_OK) return rc;
+            inf.krylov_steps += gi.steps;
+            double s2[2] = {0.0, 0.0};
+            if (neq > 0) {
+                FG_LAUNCH(k_correct, G, FG_TPB, 0, c->stream, neq, scorr, Da, a, u, part);
+                FG_CHECK(sums(c, part, G, tot, s2));
+            }
+            const double norm_a = sqrt(s2[0]), norm_Da = sqrt(s2[1]);
+            const bool stop = p->stop_by_norm ? (norm_Da < p->nl_tol * norm_a || i >= p->nl_maxiter)
+                                              : (i >= p->nl_maxiter);
+            if (stop) break;
+        }
+        inf.corrections += i;
+        if (corr_hist && passo - 1 < hist_cap) corr_hist[passo - 1] = i;
+        bool stop_time;
+        if (p->steady) {
+            double s2[2] = {0.0, 0.0};
+            if (neq > 0) {
+                FG_LAUNCH(k_steady, G, FG_TPB, 0, c->stream, neq, u, u_old, part);
+                FG_CHECK(sums(c, part, G, tot, s2));
+            }
+            stop_time = p->time_tol * sqrt(s2[1]) > sqrt(s2[0]) || fabs(p->final_time - t) < dt;
+        } else {
+            stop_time = fabs(p->final_time - t) < 1e-15;
+        }
+        if (stop_time) break;
+        if (p->max_steps > 0 && passo >= p->max_steps) {
+            inf.status = FG_ERR_NOCONV;
+            break;
+        }
+    }
+    FG_HIP(hipStreamSynchronize(c->stream));
+    inf.steps = passo;
+    inf.t = t;
+    inf.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (info) *info = inf;
+    return inf.status;
+}

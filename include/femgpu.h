@@ -417,6 +417,54 @@ int fg_gmres_batched(fg_ctx *ctx, fg_op *op, fg_prec *pc, int nprob, const int *
                      const double *B, double *X, const fg_gmres_opts *opts, int graph,
                      fg_gmres_info *info);
 
+/* ---- time integration ------------------------------------------------------ */
+/* Time_Integration/Predictor_Old.c:4-145 with the stopping rules of
+ * Criteria.c:25-72, on the device: u, a (du), Da, R and the element
+ * matrices stay in HBM; only the norms that steer the loops reach the host.
+ * Per time step: u_old = u, u += (1 - alpha) dt a, a = 0 (:57-62); then
+ * corrections (:68-125) Build, precond_setup, solver, a += Da,
+ * u += alpha dt Da until StopByNorm (||Da|| < nl_tol ||a|| or
+ * i >= nl_maxiter) or StopByIterations (i >= nl_maxiter); steps until
+ * StopBySteadyState (time_tol ||u|| > ||u - u_old|| or |T - t| < dt) or
+ * StopByTime (|T - t| < 1e-15).  The reference declares the 6-argument
+ * solver(P, M, FS, FF, B, X) but has no implementation of it; here it is
+ * fg_gmres with `gmres` options (a non-converged solve is reported in
+ * `status` and the loop goes on, as a solver returning without convergence
+ * would). */
+typedef struct {
+    double alpha, dt;        /* Parameters->Alpha, DeltaT                          */
+    double final_time;       /* FinalTime                                          */
+    double nl_tol;           /* NonLinearTolerance                                 */
+    int    nl_maxiter;       /* NonLinearMaxIter                                   */
+    int    stop_by_norm;     /* StopMulticorrection: 1 "NORM", 0 "ITERATION"       */
+    int    steady;           /* StopAtSteadyState: 1 "YES", 0 "NOT"                */
+    double time_tol;         /* TimeIntegrationTolerance                           */
+    int    max_steps;        /* > 0: end after this many steps with FG_ERR_NOCONV  */
+                             /* (no reference counterpart: StopByTime never ends   */
+                             /* when t misses T by more than 1e-15)                */
+} fg_predictor_params;
+typedef struct {
+    int    steps;            /* time steps ("passo")                               */
+    int    corrections;      /* correction iterations over all steps              */
+    long long krylov_steps;  /* Arnoldi steps over all solves                      */
+    int    unconverged;      /* solves that returned FG_ERR_NOCONV                 */
+    double t;                /* final time (Parameters->CurrentTime)               */
+    double ms;               /* wall time                                          */
+    int    status;           /* FG_OK, FG_ERR_NOCONV (max_steps) or an error       */
+} fg_predictor_info;
+/* The transient 2D transport driver (02_Transport_Equation): Build =
+ * fg_supg2d_transient_build on `m` (Build_M_K_R_SUPG.c, nodal U, dU from
+ * eval_U_dU.c with the prescribed values upresc_dev[nnodes]); A: the EBE
+ * (k = 3) operator on m's lm that receives the element matrices; pc: NULL
+ * (NO_precond_setup) or a preconditioner of A (Diag_precond_EBE_setup,
+ * element Jacobi), set up on R before every solve; no scaling (NO_scaling,
+ * NO_unscaling).  u_dev: neq+1, the initial solution in, u out.
+ * corr_hist[step] (host, may be NULL): corrections of each step. */
+int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem *pb, fg_op *A, fg_prec *pc,
+                               const double *upresc_dev, const fg_predictor_params *p,
+                               const fg_gmres_opts *gmres, double *u_dev, int *corr_hist,
+                               int hist_cap, fg_predictor_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -383,6 +383,27 @@ class Oracle:
                             alpha, dt, tol, A, R)
         return A, R
 
+    def predictor_old_tr(self, x, y, type_, idv, vertex, lm, neq, prob: OrcProb2d, ud, u0,
+                         alpha, dt, final_time, nl_tol, nl_maxiter, stop_by_norm, steady,
+                         time_tol, diag, cap=0, hist_cap=4096):
+        """Predictor_Old over the transient build: (u, corrections per step,
+        steps, t_final, status)"""
+        L = self.lib
+        L.orc_predictor_old_tr.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _ip,
+                                           C.POINTER(OrcProb2d), _d, C.POINTER(OrcPredParams),
+                                           _i, _dp, _dp, _ip, _i, C.POINTER(_i), C.POINTER(_d)]
+        nel = len(vertex) // 3
+        pp = OrcPredParams(alpha, dt, final_time, nl_tol, time_tol, nl_maxiter,
+                           int(stop_by_norm), int(steady), cap)
+        u = np.zeros(neq + 1)
+        hist = np.zeros(hist_cap, np.int32)
+        ns, tf = _i(0), _d(0.0)
+        rc = L.orc_predictor_old_tr(len(x), nel, neq, x, y, type_, idv, vertex, lm,
+                                    C.byref(prob), ud, C.byref(pp), int(diag),
+                                    np.ascontiguousarray(u0, np.float64), u, hist, hist_cap,
+                                    C.byref(ns), C.byref(tf))
+        return u, hist[:min(ns.value, hist_cap)].copy(), ns.value, tf.value, rc
+
     def ede_assembly(self, nel, nedge, ke, order, ebe):
         A = np.zeros(4 * (nedge + 1))
         self.lib.orc_ede_assembly(nel, nedge, ke, order, ebe, A)
@@ -392,6 +413,13 @@ class Oracle:
         AA = np.zeros(nnz + 1)
         self.lib.orc_csr_assembly(nel, nnz, ke, cbe, AA)
         return AA
+
+
+class OrcPredParams(C.Structure):
+    """orc_pred_params: Predictor_Old / Criteria.c parameters"""
+    _fields_ = [("alpha", C.c_double), ("dt", C.c_double), ("final_time", C.c_double),
+                ("nl_tol", C.c_double), ("time_tol", C.c_double), ("nl_maxiter", C.c_int),
+                ("stop_by_norm", C.c_int), ("steady", C.c_int), ("cap", C.c_int)]
 
 
 def prob2d(kind=P_CONST, kappa=1e-4, gamma=0.0, f=1.0, bx=1.0, by=0.0, scale=1.0,
@@ -660,3 +688,39 @@ class ReferenceTransient:
         self.lib.reftr_build(np.ascontiguousarray(u, np.float64),
                              np.ascontiguousarray(du, np.float64), A, R)
         return A, R
+
+
+class ReferencePredictor:
+    """The reference's time integrator (Predictor_Old.c, Criteria.c) over the
+    transient build of 02_Transport_Equation, compiled from its sources
+    (oracle/_ref), with a direct dense solver as the 6-argument solver."""
+
+    PATH = os.path.join(HERE, "_ref", "libref_pred2d.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.exists(cls.PATH)
+
+    def __init__(self):
+        L = self.lib = C.CDLL(self.PATH)
+        L.refpred_setup.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _ip, _i, _d, _d, _d,
+                                    _d, _d, _d, _d]
+        L.refpred_run.argtypes = [_dp, _d, _d, _d, _d, _i, _i, _i, _d, _d, _i, _i, _dp, _ip,
+                                  _i, C.POINTER(_i), C.POINTER(_d)]
+
+    def setup(self, x, y, type_, idv, vertex, lm, neq, kind, kappa, gamma, f, bx, by, scale,
+              ud):
+        self.neq = neq
+        self.lib.refpred_setup(len(x), len(vertex) // 3, neq, x, y, type_, idv, vertex, lm,
+                               kind, kappa, gamma, f, bx, by, scale, ud)
+
+    def run(self, u0, alpha, dt, final_time, nl_tol, nl_maxiter, stop_by_norm, steady,
+            time_tol, diag, cap=0, hist_cap=4096):
+        u = np.zeros(self.neq + 1)
+        hist = np.zeros(hist_cap, np.int32)
+        ns, tf = _i(0), _d(0.0)
+        rc = self.lib.refpred_run(np.ascontiguousarray(u0, np.float64), alpha, dt, final_time,
+                                  nl_tol, nl_maxiter, int(stop_by_norm), int(steady), time_tol,
+                                  0.0, int(diag), cap, u, hist, hist_cap, C.byref(ns),
+                                  C.byref(tf))
+        return u, hist[:min(ns.value, hist_cap)].copy(), ns.value, tf.value, rc

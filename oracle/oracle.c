@@ -20,6 +20,7 @@
  *   nnz / nedge closed forms for the structured mesh (SURVEY formulas).
  */
 #include "oracle.h"
+#include "dense_solve.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1789,4 +1790,76 @@ void orc_build_euler2d(int nnodes, int nel, int neq, const double *x, const doub
         R[neq] = 0;
         memcpy(A + 144 * e, Ae, sizeof Ae);
     }
+}
+
+/* ---- time integration: Predictor_Old.c:4-145, Criteria.c:25-67 ---------- */
+int orc_predictor_old_tr(int nnodes, int nel, int neq, const double *x, const double *y,
+                         const int *type, const int *id, const int *vertex, const int *lm,
+                         const orc_prob2d *pb, double ud, const orc_pred_params *pp, int diag,
+                         const double *u0, double *u, int *hist, int hist_cap, int *nsteps,
+                         double *t_final)
+{
+    const double alpha = pp->alpha, dt = pp->dt;
+    double *a = calloc(neq + 1, sizeof(double)), *Da = calloc(neq + 1, sizeof(double));
+    double *u_old = calloc(neq + 1, sizeof(double)), *R = calloc(neq + 1, sizeof(double));
+    double *A = calloc((size_t)nel * 9, sizeof(double));
+    double *U = calloc(nnodes, sizeof(double)), *dU = calloc(nnodes, sizeof(double));
+    double *Dg = calloc(neq + 1, sizeof(double)), *iD = calloc(neq + 1, sizeof(double));
+    double *diff = calloc(neq + 1, sizeof(double));
+    int rc = 0, passo = 0;
+    memcpy(u, u0, sizeof(double) * (neq + 1));   /* InitialSolution */
+    double t = 0.0;
+    int stop_time;
+    do {
+        passo++;
+        t += dt;
+        int i = 0;
+        for (int I = 0; I < neq; I++) {   /* predictor (:57-62) */
+            u_old[I] = u[I];
+            u[I] += (1.0 - alpha) * dt * a[I];
+            a[I] = 0.0;
+        }
+        double norm_a = sqrt(orc_ddot(neq, a, a));
+        double norm_Da = norm_a + 1.0;
+        int stop;
+        do {   /* multicorrection (:68-125) */
+            i++;
+            for (int I = 0; I < nnodes; I++) {   /* eval_U_dU.c */
+                if (type[I] > 0) {
+                    U[I] = u[id[I]];
+                    dU[I] = a[id[I]];
+                } else {
+                    U[I] = ud;
+                    dU[I] = 0.0;
+                }
+            }
+            orc_build_supg2d_transient(nnodes, nel, neq, x, y, vertex, lm, U, dU, pb, alpha, dt,
+                                       A, R);
+            if (diag) orc_diag_setup_ebe(neq, nel, 3, A, lm, Dg, iD, R);
+            if (orc_dense_solve_ebe(neq, nel, 3, A, lm, diag ? iD : NULL, R, Da) && !rc) rc = 1;
+            orc_daxpy(neq, 1, Da, a);
+            orc_daxpy(neq, alpha * dt, Da, u);
+            norm_a = sqrt(orc_ddot(neq, a, a));
+            norm_Da = sqrt(orc_ddot(neq, Da, Da));
+            if (pp->stop_by_norm)   /* StopByNorm (Criteria.c:35-41) */
+                stop = norm_Da < pp->nl_tol * norm_a || i >= pp->nl_maxiter;
+            else   /* StopByIterations (:26-32) */
+                stop = i >= pp->nl_maxiter;
+        } while (!stop);
+        if (passo - 1 < hist_cap) hist[passo - 1] = i;
+        if (pp->steady) {   /* StopBySteadyState (:44-63) */
+            for (int I = 0; I < neq; I++) diff[I] = u[I] - u_old[I];
+            const double norm_diff = sqrt(orc_ddot(neq, diff, diff));
+            const double norm_u = sqrt(orc_ddot(neq, u, u));
+            stop_time = pp->time_tol * norm_u > norm_diff || fabs(pp->final_time - t) < dt;
+        } else {   /* StopByTime (:65-72) */
+            stop_time = fabs(pp->final_time - t) < 1e-15;
+        }
+        if (!stop_time && pp->cap > 0 && passo >= pp->cap) stop_time = 1;
+    } while (!stop_time);
+    *nsteps = passo;
+    *t_final = t;
+    free(a); free(Da); free(u_old); free(R); free(A); free(U); free(dU); free(Dg); free(iD);
+    free(diff);
+    return rc;
 }

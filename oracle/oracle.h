@@ -243,6 +243,25 @@ void orc_build_supg3d(int nel, int neq, const double *x, const double *y, const 
                       const int *utype, const int *vertex, const int *lm,
                       const orc_prob3d *pb, double *ke, double *F);
 
+/* Time_Integration/Predictor_Old.c:4-145 (predictor, multicorrection) with
+ * Criteria.c:25-67 (StopByIterations / StopByNorm, StopBySteadyState /
+ * StopByTime) over the transient build above: eval_U_dU (free nodes from u
+ * and a, prescribed nodes ud and 0), Build_M_K_R_SUPG, NO_scaling,
+ * NO_precond_setup or Diag_precond_EBE_setup (diag = 1), the 6-argument
+ * solver as the direct solve of dense_solve.h, NO_unscaling, a += Da,
+ * u += alpha dt Da.  cap > 0 ends the run after cap time steps (the
+ * harness's safety limit).  u0, u: neq+1; hist[step] = corrections of each
+ * step.  Returns 0, or 1 if a solve met a zero pivot. */
+typedef struct {
+    double alpha, dt, final_time, nl_tol, time_tol;
+    int nl_maxiter, stop_by_norm, steady, cap;
+} orc_pred_params;
+int orc_predictor_old_tr(int nnodes, int nel, int neq, const double *x, const double *y,
+                         const int *type, const int *id, const int *vertex, const int *lm,
+                         const orc_prob2d *pb, double ud, const orc_pred_params *pp, int diag,
+                         const double *u0, double *u, int *hist, int hist_cap, int *nsteps,
+                         double *t_final);
+
 #ifdef __cplusplus
 }
 #endif

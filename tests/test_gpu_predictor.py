@@ -1,0 +1,93 @@
+"""The device time integrator (fg_predictor_old_transport: Predictor_Old.c
+with Criteria.c over the transient 2D SUPG build) against the reference's
+loop (tests/golden/pred_*, written by the reference's Predictor_Old with a
+direct dense solve) and the oracle.
+
+The device solves each correction with GMRES to 1e-13 relative, the
+reference fixtures with a direct solve: the step counts, final times and
+corrections per step must be identical, u within 1e-9 relative after the
+whole run (the solves differ by ~1e-13 per correction)."""
+import os
+
+import numpy as np
+import pytest
+
+import fem_codes_amd as F
+from conftest import golden_files, load_golden
+
+pytestmark = pytest.mark.gpu
+CASES = golden_files("pred_*.npz")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return F.Context(0)
+
+
+def _setup(ctx, d, pc_kind=None):
+    mesh = F.Mesh2D(ctx, d["x"], d["y"], d["type"], d["vertex"])
+    assert np.array_equal(mesh.lm(), d["lm"])
+    neq = mesh.neq
+    op = F.Operator.ebe(ctx, neq, mesh.lm(), 3)
+    pr = d["params"]
+    kind = pc_kind if pc_kind is not None else (F.PREC_DIAG if int(pr[8]) else None)
+    pc = F.Preconditioner(op, kind) if kind is not None else None
+    p = d["prob"]
+    pb = F.Supg2dProblem(int(p[0]), p[1], p[2], p[3], p[4], p[5], p[6], int(p[7]))
+    up = ctx.to_device(np.full(len(d["x"]), float(d["ud"])))
+    return mesh, op, pc, pb, up, neq
+
+
+def _params(d):
+    pr = d["params"]
+    return dict(alpha=pr[0], dt=pr[1], final_time=pr[2], nl_tol=pr[3], nl_maxiter=int(pr[4]),
+                stop_by_norm=bool(pr[5]), steady=bool(pr[6]), time_tol=pr[7])
+
+
+@pytest.mark.parametrize("engine", [F.ENGINE_AUTO, F.ENGINE_KERNELS])
+@pytest.mark.parametrize("path", CASES, ids=os.path.basename)
+def test_predictor_matches_reference(ctx, path, engine):
+    d = load_golden(path)
+    mesh, op, pc, pb, up, neq = _setup(ctx, d)
+    u = ctx.to_device(d["u0"])
+    info, hist = F.predictor_old_transport(mesh, pb, op, pc, up, u, tol=1e-13, kmax=30,
+                                           lmax=2000, engine=engine, max_steps=400,
+                                           **_params(d))
+    assert info.status == F.FG_OK and info.unconverged == 0
+    assert info.steps == int(d["nsteps"]) and info.t == float(d["t_final"])
+    assert np.array_equal(hist, d["hist"])
+    assert info.corrections == int(d["hist"].sum())
+    ug = u.get()
+    assert np.max(np.abs(ug[:neq] - d["u"][:neq])) <= 1e-9 * np.max(np.abs(d["u"][:neq]))
+    print(f"\n{os.path.basename(path)} engine {engine}: {info.steps} steps, "
+          f"{info.corrections} corrections, {info.krylov_steps} Arnoldi steps, {info.ms:.1f} ms")
+
+
+def test_predictor_preconditioner_does_not_change_the_answer(ctx):
+    """element Jacobi instead of Diag: the same corrections, u within 1e-9"""
+    d = load_golden(CASES[1])
+    mesh, op, pc, pb, up, neq = _setup(ctx, d, pc_kind=F.PREC_JACOBI)
+    u = ctx.to_device(d["u0"])
+    info, hist = F.predictor_old_transport(mesh, pb, op, pc, up, u, tol=1e-13, lmax=2000,
+                                           max_steps=400, **_params(d))
+    assert info.status == F.FG_OK and np.array_equal(hist, d["hist"])
+    assert np.max(np.abs(u.get()[:neq] - d["u"][:neq])) <= 1e-9 * np.max(np.abs(d["u"][:neq]))
+
+
+def test_predictor_max_steps(ctx):
+    d = load_golden(CASES[0])
+    mesh, op, pc, pb, up, neq = _setup(ctx, d)
+    u = ctx.to_device(d["u0"])
+    prm = _params(d)
+    prm["final_time"] = 1e9
+    info, hist = F.predictor_old_transport(mesh, pb, op, pc, up, u, max_steps=3, **prm)
+    assert info.status == F.FG_ERR_NOCONV and info.steps == 3 and len(hist) == 3
+
+
+def test_predictor_rejects_a_foreign_operator(ctx):
+    d = load_golden(CASES[0])
+    mesh, op, pc, pb, up, neq = _setup(ctx, d)
+    other = F.Operator.ebe(ctx, neq, np.full(3 * (mesh.nel - 1), neq, np.int32), 3)
+    u = ctx.to_device(d["u0"])
+    with pytest.raises(F.FemGpuError, match="status 2"):
+        F.predictor_old_transport(mesh, pb, other, None, up, u, **_params(d))

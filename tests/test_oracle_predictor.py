@@ -1,0 +1,108 @@
+"""Pins the oracle's time integrator (Predictor_Old.c with Criteria.c over the
+transient 2D SUPG build of 02_Transport_Equation) -- CPU only.
+
+* against the reference's own loop: tests/golden/pred_* were written by the
+  reference's Predictor_Old, StopBy* criteria, ddot/daxpy, precond setups
+  and transient build compiled from their sources, with a direct dense solve
+  as the 6-argument solver (tests/golden/make_golden_pred.py); bitwise: the
+  final u, the corrections of every step, the step count and final time;
+* against the mathematics of the scheme, independently of the oracle's loop
+  and of its dense solver: with one correction per step the method is
+  u_pred = u_n + (1 - alpha) dt a_n, a_{n+1} = Da = A^{-1} R(u_pred, 0),
+  u_{n+1} = u_pred + alpha dt Da, with A and R from the (separately pinned)
+  transient build and Da from numpy's solve; checked over two steps
+  (alpha = 1: backward Euler; alpha = 1/2: the predictor term of step 2).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden_files, load_golden
+
+CASES = golden_files("pred_*.npz")
+
+
+@pytest.fixture(scope="module")
+def o():
+    return O.Oracle()
+
+
+def _prob(p):
+    return O.OrcProb2d(int(p[0]), p[1], p[2], p[3], p[4], p[5], p[6], int(p[7]))
+
+
+def _run(o, d, **over):
+    pr = d["params"]
+    args = dict(alpha=pr[0], dt=pr[1], final_time=pr[2], nl_tol=pr[3], nl_maxiter=int(pr[4]),
+                stop_by_norm=int(pr[5]), steady=int(pr[6]), time_tol=pr[7], diag=int(pr[8]),
+                cap=400)
+    args.update(over)
+    return o.predictor_old_tr(d["x"], d["y"], d["type"], d["idv"], d["vertex"], d["lm"],
+                              int(d["neq"]), _prob(d["prob"]), float(d["ud"]), d["u0"], **args)
+
+
+@pytest.mark.parametrize("path", CASES, ids=os.path.basename)
+def test_predictor_golden(o, path):
+    d = load_golden(path)
+    u, hist, ns, tf, rc = _run(o, d)
+    assert rc == 0
+    assert ns == int(d["nsteps"]) and tf == float(d["t_final"])
+    assert np.array_equal(hist, d["hist"])
+    assert np.array_equal(u, d["u"])
+
+
+def _assemble(A, lm, neq):
+    M = np.zeros((neq, neq))
+    Ae = A.reshape(-1, 3, 3)
+    L = lm.reshape(-1, 3)
+    for e in range(len(L)):
+        for a in range(3):
+            if L[e, a] == neq:
+                continue
+            for b in range(3):
+                if L[e, b] != neq:
+                    M[L[e, a], L[e, b]] += Ae[e, a, b]
+    return M
+
+
+@pytest.mark.parametrize("alpha", [1.0, 0.5])
+def test_predictor_one_correction_two_steps(o, alpha):
+    d = load_golden(CASES[0])
+    neq = int(d["neq"])
+    dt = 0.05
+    u, hist, ns, tf, rc = _run(o, d, alpha=alpha, dt=dt, final_time=2 * dt, nl_maxiter=1,
+                               stop_by_norm=0, steady=0, diag=0)
+    assert rc == 0 and ns == 2 and list(hist) == [1, 1]
+    pb = _prob(d["prob"])
+    t, idv, ud = d["type"], d["idv"], float(d["ud"])
+    un = d["u0"][:neq].copy()
+    an = np.zeros(neq)
+    for _ in range(2):
+        upred = un + (1.0 - alpha) * dt * an
+        uf = np.append(upred, 0.0)
+        U = np.where(t > 0, uf[np.maximum(idv, 0)], ud)
+        A, R = o.build_supg2d_transient(d["x"], d["y"], d["vertex"], d["lm"], neq, U,
+                                        np.zeros_like(U), pb, alpha, dt)
+        Da = np.linalg.solve(_assemble(A, d["lm"], neq), R[:neq])
+        an = Da
+        un = upred + alpha * dt * Da
+    assert np.max(np.abs(u[:neq] - un)) <= 1e-12 * np.max(np.abs(un))
+
+
+@pytest.mark.skipif(not O.ReferencePredictor.available(), reason="reference build absent")
+def test_predictor_vs_reference_live(o):
+    """a case the fixtures do not hold: StopByIterations with NO_precond and
+    the steady-state test, against the reference run here and now"""
+    d = load_golden(CASES[1])
+    R = O.ReferencePredictor()
+    p = d["prob"]
+    R.setup(d["x"], d["y"], d["type"], d["idv"], d["vertex"], d["lm"], int(d["neq"]),
+            int(p[0]), p[1], p[2], p[3], p[4], p[5], p[6], float(d["ud"]))
+    args = dict(alpha=0.75, dt=0.2, final_time=20.0, nl_tol=0.0, nl_maxiter=2, stop_by_norm=0,
+                steady=1, time_tol=0.05, diag=0, cap=200)
+    ur, hr, nr, tr, rcr = R.run(d["u0"], **args)
+    uo, ho, no, to, rco = _run(o, d, **args)
+    assert rcr == rco == 0 and nr == no and tr == to
+    assert np.array_equal(hr, ho) and np.array_equal(ur, uo)
