@@ -131,6 +131,9 @@ SIGNATURES = {
     "fg_predictor_old_transport": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(PredictorParams),
                                         C.POINTER(GmresOpts), _vp, _vp, _i,
                                         C.POINTER(PredictorInfo)]),
+    "fg_predictor_old_euler": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                    C.POINTER(PredictorParams), C.POINTER(GmresOpts), _vp, _vp,
+                                    _i, C.POINTER(PredictorInfo)]),
     "fg_op_assemble_csr": (_i, [_vp, _i, _ip, _vp]),
     "fg_op_assemble_ede": (_i, [_vp, _i, _ip, _ip, _vp]),
     "fg_csr_pattern_k": (_i, [_i, _i, _i, _ip, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
@@ -528,6 +531,33 @@ def predictor_old_transport(mesh: "Mesh2D", prob: "Supg2dProblem", op: Operator,
                                         C.byref(pp), C.byref(o), u.ptr, hist.ctypes.data,
                                         hist_cap, C.byref(info))
     _check(rc, "fg_predictor_old_transport", ok=(FG_OK, FG_ERR_NOCONV))
+    return info, hist[:min(info.steps, hist_cap)].copy()
+
+
+PRED_EU_NOT, PRED_EU_BLOCK_SCALING, PRED_EU_BLOCKDIAG = 0, 1, 2
+
+
+def predictor_old_euler(mesh: "Euler2D", ep: "EulerParams", op: Operator,
+                        blocks: Preconditioner | None, mode: int, presc: DeviceArray,
+                        rot: DeviceArray | None, theta: DeviceArray | None, u: DeviceArray,
+                        alpha, dt, final_time, nl_tol, nl_maxiter, stop_by_norm=True,
+                        steady=False, time_tol=0.0, max_steps=0, kmax=30, lmax=1000, tol=1e-8,
+                        precision=GMRES_FP64, engine=ENGINE_AUTO, orth=ORTH_MGS,
+                        hist_cap=4096):
+    """Predictor_Old over the Euler build, on the device (mode: PRED_EU_NOT,
+    PRED_EU_BLOCK_SCALING, PRED_EU_BLOCKDIAG); u (neq+1) holds the initial
+    solution and receives the result.  Returns (info, corrections per step)."""
+    pp = PredictorParams(alpha, dt, final_time, nl_tol, nl_maxiter, int(stop_by_norm),
+                         int(steady), time_tol, max_steps)
+    o = GmresOpts(kmax, lmax, tol, precision, 0.0, 0, None, None, engine, orth)
+    hist = np.zeros(max(1, hist_cap), np.int32)
+    info = PredictorInfo()
+    rc = lib.fg_predictor_old_euler(mesh.h, C.byref(ep), op.h,
+                                    blocks.h if blocks is not None else None, int(mode),
+                                    presc.ptr, rot.ptr if rot is not None else None,
+                                    theta.ptr if theta is not None else None, C.byref(pp),
+                                    C.byref(o), u.ptr, hist.ctypes.data, hist_cap, C.byref(info))
+    _check(rc, "fg_predictor_old_euler", ok=(FG_OK, FG_ERR_NOCONV))
     return info, hist[:min(info.steps, hist_cap)].copy()
 
 

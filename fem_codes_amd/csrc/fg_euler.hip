@@ -30,6 +30,7 @@ struct fg_euler2d {
     int nnodes = 0, nel = 0, neq = 0;
     double *d_x = nullptr, *d_y = nullptr;
     int *d_vertex = nullptr, *d_roff = nullptr, *d_rinc = nullptr;
+    int *d_id4 = nullptr;     // [nnodes*4] equation of each dof, neq: prescribed
     double *d_Re = nullptr;   // [nel*12]
 };
 
@@ -325,6 +326,38 @@ __global__ void k_euler_gather(int neq, const int *__restrict__ off, const int *
 
 }  // namespace
 
+// eval_U_dU.c of the 03 driver: U, dU [nnodes*4] from u and du on the free
+// dofs, the prescribed values and 0 on the others
+__global__ void k_eval_U_dU_eu(int n4, int neq, const int *__restrict__ id4,
+                               const double *__restrict__ u, const double *__restrict__ du,
+                               const double *__restrict__ presc, double *__restrict__ U,
+                               double *__restrict__ dU)
+{
+    for (int q = blockIdx.x * FG_TPB + threadIdx.x; q < n4; q += gridDim.x * FG_TPB) {
+        const int id = id4[q];
+        U[q] = id < neq ? u[id] : presc[q];
+        dU[q] = id < neq ? du[id] : 0.0;
+    }
+}
+
+int fg_euler2d_eval_U_dU(const fg_euler2d *m, const double *u, const double *du,
+                         const double *presc, double *U, double *dU)
+{
+    fg_ctx *c = m->ctx;
+    const int n4 = 4 * m->nnodes;
+    FG_LAUNCH(k_eval_U_dU_eu, fg_vec_grid(c, n4), FG_TPB, 0, c->stream, n4, m->neq, m->d_id4,
+              u, du, presc, U, dU);
+    return FG_OK;
+}
+
+void fg_euler2d_sizes(const fg_euler2d *m, fg_ctx **c, int *nnodes, int *nel, int *neq)
+{
+    *c = m->ctx;
+    *nnodes = m->nnodes;
+    *nel = m->nel;
+    *neq = m->neq;
+}
+
 extern "C" int fg_euler2d_create(fg_ctx *c, int nnodes, int nel, int neq, const double *x,
                                  const double *y, const int *vertex, const int *lm,
                                  fg_euler2d **out)
@@ -346,6 +379,12 @@ extern "C" int fg_euler2d_create(fg_ctx *c, int nnodes, int nel, int neq, const 
         for (long long t = 0; t < 12LL * nel; t++)
             if (lm[t] < neq) inc[fill[lm[t]]++] = (int)t;
     }
+    // equation of every node dof, from the elements (Node[].id; a node in no
+    // element, or a prescribed dof, reads neq)
+    std::vector<int> id4(4 * (size_t)nnodes, neq);
+    for (long long e = 0; e < nel; e++)
+        for (int a = 0; a < 3; a++)
+            for (int k = 0; k < 4; k++) id4[4 * (size_t)vertex[3 * e + a] + k] = lm[12 * e + 4 * a + k];
     FG_HIP(hipSetDevice(c->device));
     fg_euler2d *m = new fg_euler2d();
     m->ctx = c;
@@ -356,7 +395,8 @@ extern "C" int fg_euler2d_create(fg_ctx *c, int nnodes, int nel, int neq, const 
     if ((rc = up(x, nnodes, &m->d_x)) || (rc = up(y, nnodes, &m->d_y)) ||
         (rc = up(vertex, 3 * (size_t)nel, &m->d_vertex)) ||
         (rc = up(off.data(), off.size(), &m->d_roff)) ||
-        (rc = up(inc.data(), inc.size(), &m->d_rinc))) {
+        (rc = up(inc.data(), inc.size(), &m->d_rinc)) ||
+        (rc = up(id4.data(), id4.size(), &m->d_id4))) {
         fg_euler2d_destroy(m);
         return rc;
     }
@@ -374,7 +414,7 @@ extern "C" int fg_euler2d_destroy(fg_euler2d *m)
     if (!m) return FG_OK;
     hipStreamSynchronize(m->ctx->stream);
     hipFree(m->d_x); hipFree(m->d_y); hipFree(m->d_vertex); hipFree(m->d_roff);
-    hipFree(m->d_rinc); hipFree(m->d_Re);
+    hipFree(m->d_rinc); hipFree(m->d_Re); hipFree(m->d_id4);
     delete m;
     return FG_OK;
 }

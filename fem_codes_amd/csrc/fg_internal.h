@@ -143,6 +143,9 @@ int fg_ebe2_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
 int fg_mesh2d_eval_U_dU(const fg_mesh2d *m, const double *u, const double *du,
                         const double *upresc, double *U, double *dU);
 fg_ctx *fg_mesh2d_ctx(const fg_mesh2d *m);
+int fg_euler2d_eval_U_dU(const fg_euler2d *m, const double *u, const double *du,
+                         const double *presc, double *U, double *dU);
+void fg_euler2d_sizes(const fg_euler2d *m, fg_ctx **c, int *nnodes, int *nel, int *neq);
 double fg_ebe2_bytes(const fg_op *op, int fp32);
 
 struct fg_prec {

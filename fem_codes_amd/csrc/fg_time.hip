@@ -104,34 +104,26 @@ int sums(fg_ctx *c, const double *part, int G, double *dev_out, double host_out[
     return FG_OK;
 }
 
-}  // namespace
-
-extern "C" int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem *pb, fg_op *A,
-                                          fg_prec *pc, const double *upresc,
-                                          const fg_predictor_params *p,
-                                          const fg_gmres_opts *gm, double *u,
-                                          int *corr_hist, int hist_cap,
-                                          fg_predictor_info *info)
+// Predictor_Old's two loops around one correction `build(u, a, R)` that
+// leaves the element matrices in A (and R, possibly scaled) on the stream
+template <typename Build>
+int predictor_loop(fg_ctx *c, int neq, size_t work_doubles, fg_op *A, fg_prec *pc,
+                   const fg_predictor_params *p, const fg_gmres_opts *gm, double *u,
+                   int *corr_hist, int hist_cap, fg_predictor_info *info, double **work,
+                   Build build)
 {
-    FG_ARG(m && pb && A && upresc && p && gm && u, "mesh/problem/operator/upresc/params/opts/u");
-    int neq = 0, nel = 0, nn = 0;
-    FG_CHECK(fg_mesh2d_info(m, &neq, &nel, &nn));
-    fg_ctx *c = fg_mesh2d_ctx(m);
-    FG_ARG(A->ctx == c && A->kind == FG_OP_EBE && A->k == 3 && A->neq == neq && A->nitems == nel,
-           "A must be the k = 3 EBE operator of the mesh's lm");
     FG_ARG(!pc || pc->op == A, "the preconditioner must be built on A");
     FG_ARG(p->dt > 0.0 && p->nl_maxiter >= 1, "dt > 0 and nl_maxiter >= 1");
     FG_HIP(hipSetDevice(c->device));
-
-    // workspace: a, Da, u_old, R (neq+1), U, dU (nnodes), element matrices,
-    // partial sums, two totals
-    const size_t nv = (size_t)neq + 1, ne = (size_t)nel * 9;
+    // workspace: a, Da, u_old, R (neq+1), partial sums, two totals, then the
+    // caller's share (nodal U/dU, element matrices)
+    const size_t nv = (size_t)neq + 1;
     Work w;
-    const size_t bytes = sizeof(double) * (4 * nv + 2 * (size_t)nn + (ne ? ne : 1) + 2 * PG + 2);
+    const size_t bytes = sizeof(double) * (4 * nv + 2 * PG + 2 + work_doubles);
     FG_HIP(hipMalloc(&w.buf, bytes));
     double *a = (double *)w.buf, *Da = a + nv, *u_old = Da + nv, *R = u_old + nv;
-    double *U = R + nv, *dU = U + nn, *ke = dU + nn, *part = ke + (ne ? ne : 1);
-    double *tot = part + 2 * PG;
+    double *part = R + nv, *tot = part + 2 * PG;
+    *work = tot + 2;
     FG_HIP(hipMemsetAsync(w.buf, 0, bytes, c->stream));
 
     const int G = fg_vec_grid(c, neq) < PG ? fg_vec_grid(c, neq) : PG;
@@ -152,9 +144,7 @@ extern "C" int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem 
         int i = 0;
         for (;;) {
             i++;
-            FG_CHECK(fg_mesh2d_eval_U_dU(m, u, a, upresc, U, dU));
-            FG_CHECK(fg_supg2d_transient_build(m, pb, alpha, dt, U, dU, ke, R));
-            FG_CHECK(fg_op_set_values_dev(A, ke));
+            FG_CHECK(build(u, a, R));
             if (pc) FG_CHECK(fg_prec_setup(pc, R));
             fg_gmres_info gi;
             const int rc = fg_gmres(c, A, pc, R, Da, gm, &gi);
@@ -196,4 +186,73 @@ extern "C" int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem 
     inf.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (info) *info = inf;
     return inf.status;
+}
+
+}  // namespace
+
+extern "C" int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem *pb, fg_op *A,
+                                          fg_prec *pc, const double *upresc,
+                                          const fg_predictor_params *p,
+                                          const fg_gmres_opts *gm, double *u,
+                                          int *corr_hist, int hist_cap,
+                                          fg_predictor_info *info)
+{
+    FG_ARG(m && pb && A && upresc && p && gm && u, "mesh/problem/operator/upresc/params/opts/u");
+    int neq = 0, nel = 0, nn = 0;
+    FG_CHECK(fg_mesh2d_info(m, &neq, &nel, &nn));
+    fg_ctx *c = fg_mesh2d_ctx(m);
+    FG_ARG(A->ctx == c && A->kind == FG_OP_EBE && A->k == 3 && A->neq == neq && A->nitems == nel,
+           "A must be the k = 3 EBE operator of the mesh's lm");
+    const size_t ne = (size_t)nel * 9;
+    double *work = nullptr;
+    double *const *wp = &work;
+    const double alpha = p->alpha, dt = p->dt;
+    // Build_M_K_R_SUPG.c with eval_U_dU.c of the 02 driver; NO_scaling
+    auto build = [&](const double *uu, const double *aa, double *R) -> int {
+        double *U = *wp, *dU = U + nn, *ke = dU + nn;
+        FG_CHECK(fg_mesh2d_eval_U_dU(m, uu, aa, upresc, U, dU));
+        FG_CHECK(fg_supg2d_transient_build(m, pb, alpha, dt, U, dU, ke, R));
+        return fg_op_set_values_dev(A, ke);
+    };
+    return predictor_loop(c, neq, 2 * (size_t)nn + (ne ? ne : 1), A, pc, p, gm, u, corr_hist,
+                          hist_cap, info, &work, build);
+}
+
+extern "C" int fg_predictor_old_euler(fg_euler2d *m, const fg_euler2d_params *ep, fg_op *A,
+                                      fg_prec *blocks, int mode, const double *presc,
+                                      const int *rot, const double *theta,
+                                      const fg_predictor_params *p, const fg_gmres_opts *gm,
+                                      double *u, int *corr_hist, int hist_cap,
+                                      fg_predictor_info *info)
+{
+    FG_ARG(m && ep && A && presc && p && gm && u, "mesh/params/operator/presc/params/opts/u");
+    FG_ARG(mode == FG_PRED_EU_NOT || mode == FG_PRED_EU_BLOCK_SCALING ||
+               mode == FG_PRED_EU_BLOCKDIAG, "mode");
+    FG_ARG(mode == FG_PRED_EU_NOT || (blocks && blocks->kind == FG_PREC_BLOCKDIAG && blocks->op == A),
+           "block scaling and the block-diagonal preconditioner need a FG_PREC_BLOCKDIAG of A");
+    fg_ctx *c = nullptr;
+    int nn = 0, nel = 0, neq = 0;
+    fg_euler2d_sizes(m, &c, &nn, &nel, &neq);
+    FG_ARG(A->ctx == c && A->kind == FG_OP_EBE && A->k == 12 && A->neq == neq && A->nitems == nel,
+           "A must be the k = 12 EBE operator of the Euler mesh's lm");
+    const size_t ne = (size_t)nel * 144;
+    double *work = nullptr;
+    double *const *wp = &work;
+    // Build_M_K_F_SUPG with Alpha_Build / DeltaT_Build set by the predictor
+    // (Predictor_Old.c:27-28); the CAU tolerance and the rest from ep
+    fg_euler2d_params bp = *ep;
+    bp.alpha = p->alpha;
+    bp.dt = p->dt;
+    auto build = [&](const double *uu, const double *aa, double *R) -> int {
+        double *U = *wp, *dU = U + 4 * (size_t)nn, *ke = dU + 4 * (size_t)nn;
+        FG_CHECK(fg_euler2d_eval_U_dU(m, uu, aa, presc, U, dU));
+        FG_CHECK(fg_euler2d_build(m, &bp, U, dU, rot, theta, ke, R));
+        FG_CHECK(fg_op_set_values_dev(A, ke));
+        // Block_scaling_EBE (Block_scaling.c:10-59): A and R scaled in place
+        if (mode == FG_PRED_EU_BLOCK_SCALING) FG_CHECK(fg_block_scaling_ebe(blocks, R));
+        return FG_OK;
+    };
+    return predictor_loop(c, neq, 8 * (size_t)nn + (ne ? ne : 1), A,
+                          mode == FG_PRED_EU_BLOCKDIAG ? blocks : nullptr, p, gm, u, corr_hist,
+                          hist_cap, info, &work, build);
 }

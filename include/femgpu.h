@@ -464,6 +464,27 @@ int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem *pb, fg_op 
                                const double *upresc_dev, const fg_predictor_params *p,
                                const fg_gmres_opts *gmres, double *u_dev, int *corr_hist,
                                int hist_cap, fg_predictor_info *info);
+/* The Euler driver (03_Euler_Equations): Build = fg_euler2d_build on `m`
+ * (Build_M_K_F_SUPG.c; alpha and dt of `ep` replaced by those of `p`, as
+ * Predictor_Old.c:27-28 sets Alpha_Build / DeltaT_Build; nodal U, dU from
+ * eval_U_dU.c with the prescribed values presc_dev[nnodes*4]); A: the EBE
+ * (k = 12) operator on m's lm; `mode` (setScaling.c / setPreconditioner.c
+ * of that driver):
+ *   FG_PRED_EU_NOT            NO_scaling, NO_precond_setup;
+ *   FG_PRED_EU_BLOCK_SCALING  Block_scaling_EBE (A and R scaled in place,
+ *                             Block_scaling.c:10-59), no preconditioner;
+ *   FG_PRED_EU_BLOCKDIAG      BlockDiag_precond_EBE_setup on R and
+ *                             BlockDiag_precond inside GMRES;
+ * `blocks`: a FG_PREC_BLOCKDIAG of A (NULL for FG_PRED_EU_NOT).  rot_dev,
+ * theta_dev: slip-wall nodes and angles as for fg_euler2d_build. */
+#define FG_PRED_EU_NOT           0
+#define FG_PRED_EU_BLOCK_SCALING 1
+#define FG_PRED_EU_BLOCKDIAG     2
+int fg_predictor_old_euler(fg_euler2d *m, const fg_euler2d_params *ep, fg_op *A, fg_prec *blocks,
+                           int mode, const double *presc_dev, const int *rot_dev,
+                           const double *theta_dev, const fg_predictor_params *p,
+                           const fg_gmres_opts *gmres, double *u_dev, int *corr_hist,
+                           int hist_cap, fg_predictor_info *info);
 
 #ifdef __cplusplus
 }

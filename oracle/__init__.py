@@ -404,6 +404,33 @@ class Oracle:
                                     C.byref(ns), C.byref(tf))
         return u, hist[:min(ns.value, hist_cap)].copy(), ns.value, tf.value, rc
 
+    def predictor_old_eu(self, x, y, vertex, lm, neq, ids, presc, rot, theta, gamma, mach,
+                         dimless, stab_tol, u0, alpha, dt, final_time, nl_tol, nl_maxiter,
+                         stop_by_norm, steady, time_tol, block_scaling, cap=0, hist_cap=4096):
+        """Predictor_Old over the Euler build: (u, corrections per step, steps,
+        t_final, status)"""
+        L = self.lib
+        L.orc_predictor_old_eu.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _dp, _vp, _vp,
+                                           _d, _d, _i, _d, C.POINTER(OrcPredParams), _i, _dp,
+                                           _dp, _ip, _i, C.POINTER(_i), C.POINTER(_d)]
+        nel = len(vertex) // 3
+        pp = OrcPredParams(alpha, dt, final_time, nl_tol, time_tol, nl_maxiter,
+                           int(stop_by_norm), int(steady), cap)
+        rot_a = None if rot is None else np.ascontiguousarray(rot, np.int32)
+        th_a = None if theta is None else np.ascontiguousarray(theta, np.float64)
+        u = np.zeros(neq + 1)
+        hist = np.zeros(hist_cap, np.int32)
+        ns, tf = _i(0), _d(0.0)
+        rc = L.orc_predictor_old_eu(len(x), nel, neq, x, y, vertex, lm,
+                                    np.ascontiguousarray(ids, np.int32),
+                                    np.ascontiguousarray(presc, np.float64),
+                                    _ptr(rot_a) if rot_a is not None else None,
+                                    _ptr(th_a) if th_a is not None else None, gamma, mach,
+                                    int(dimless), stab_tol, C.byref(pp), int(block_scaling),
+                                    np.ascontiguousarray(u0, np.float64), u, hist, hist_cap,
+                                    C.byref(ns), C.byref(tf))
+        return u, hist[:min(ns.value, hist_cap)].copy(), ns.value, tf.value, rc
+
     def ede_assembly(self, nel, nedge, ke, order, ebe):
         A = np.zeros(4 * (nedge + 1))
         self.lib.orc_ede_assembly(nel, nedge, ke, order, ebe, A)
@@ -636,6 +663,22 @@ class ReferenceEuler:
         z = np.zeros(self.neq + 1)
         self.lib.refeu_blockdiag_apply(np.ascontiguousarray(p, np.float64).copy(), z)
         return z
+
+    def pred_run(self, u0, alpha, dt, final_time, nl_tol, nl_maxiter, stop_by_norm, steady,
+                 time_tol, stab_tol, block_scaling, cap=0, hist_cap=4096):
+        """the reference's Predictor_Old over its Euler build (after setup and
+        build_setup): (u, corrections per step, steps, t_final, status)"""
+        L = self.lib
+        L.refeu_pred_run.argtypes = [_dp, _d, _d, _d, _d, _i, _i, _i, _d, _d, _i, _i, _dp, _ip,
+                                     _i, C.POINTER(_i), C.POINTER(_d)]
+        u = np.zeros(self.neq + 1)
+        hist = np.zeros(hist_cap, np.int32)
+        ns, tf = _i(0), _d(0.0)
+        rc = L.refeu_pred_run(np.ascontiguousarray(u0, np.float64), alpha, dt, final_time,
+                              nl_tol, nl_maxiter, int(stop_by_norm), int(steady), time_tol,
+                              stab_tol, int(block_scaling), cap, u, hist, hist_cap, C.byref(ns),
+                              C.byref(tf))
+        return u, hist[:min(ns.value, hist_cap)].copy(), ns.value, tf.value, rc
 
     def block_scaling(self, F):
         Fp = np.ascontiguousarray(F, np.float64).copy()

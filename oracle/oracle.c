@@ -1863,3 +1863,76 @@ int orc_predictor_old_tr(int nnodes, int nel, int neq, const double *x, const do
     free(diff);
     return rc;
 }
+
+int orc_predictor_old_eu(int nnodes, int nel, int neq, const double *x, const double *y,
+                         const int *vertex, const int *lm, const int *ids, const double *presc,
+                         const int *rot, const double *theta, double gamma, double mach,
+                         int dimless, double stab_tol, const orc_pred_params *pp,
+                         int block_scaling, const double *u0, double *u, int *hist, int hist_cap,
+                         int *nsteps, double *t_final)
+{
+    const double alpha = pp->alpha, dt = pp->dt;
+    double *a = calloc(neq + 1, sizeof(double)), *Da = calloc(neq + 1, sizeof(double));
+    double *u_old = calloc(neq + 1, sizeof(double)), *R = calloc(neq + 1, sizeof(double));
+    double *A = calloc((size_t)nel * 144, sizeof(double));
+    double *U = calloc(4 * (size_t)nnodes, sizeof(double));
+    double *dU = calloc(4 * (size_t)nnodes, sizeof(double));
+    double *BD = calloc(16 * (size_t)nnodes, sizeof(double));
+    double *iBD = calloc(16 * (size_t)nnodes, sizeof(double));
+    int *Id = calloc(4 * (size_t)nnodes, sizeof(int));
+    double *diff = calloc(neq + 1, sizeof(double));
+    for (long q = 0; q < 4L * nnodes; q++) Id[q] = ids[q] < 0 ? neq : ids[q];
+    int rc = 0, passo = 0, stop_time;
+    memcpy(u, u0, sizeof(double) * (neq + 1));
+    double t = 0.0;
+    do {
+        passo++;
+        t += dt;
+        int i = 0;
+        for (int I = 0; I < neq; I++) {
+            u_old[I] = u[I];
+            u[I] += (1.0 - alpha) * dt * a[I];
+            a[I] = 0.0;
+        }
+        double norm_a = sqrt(orc_ddot(neq, a, a));
+        double norm_Da = norm_a + 1.0;
+        int stop;
+        do {
+            i++;
+            for (long I = 0; I < nnodes; I++)   /* eval_U_dU.c of the 03 driver */
+                for (int k = 0; k < 4; k++) {
+                    const int id = ids[4 * I + k];
+                    U[4 * I + k] = id >= 0 ? u[id] : presc[k];
+                    dU[4 * I + k] = id >= 0 ? a[id] : 0.0;
+                }
+            orc_build_euler2d(nnodes, nel, neq, x, y, vertex, lm, U, dU, rot, theta, gamma, mach,
+                              dimless, alpha, dt, stab_tol, A, R);
+            if (block_scaling)
+                orc_block_scaling_ebe(neq, nel, nnodes, vertex, ids, Id, A, BD, iBD, R);
+            if (orc_dense_solve_ebe(neq, nel, 12, A, lm, NULL, R, Da) && !rc) rc = 1;
+            orc_daxpy(neq, 1, Da, a);
+            orc_daxpy(neq, alpha * dt, Da, u);
+            norm_a = sqrt(orc_ddot(neq, a, a));
+            norm_Da = sqrt(orc_ddot(neq, Da, Da));
+            if (pp->stop_by_norm)
+                stop = norm_Da < pp->nl_tol * norm_a || i >= pp->nl_maxiter;
+            else
+                stop = i >= pp->nl_maxiter;
+        } while (!stop);
+        if (passo - 1 < hist_cap) hist[passo - 1] = i;
+        if (pp->steady) {
+            for (int I = 0; I < neq; I++) diff[I] = u[I] - u_old[I];
+            const double norm_diff = sqrt(orc_ddot(neq, diff, diff));
+            const double norm_u = sqrt(orc_ddot(neq, u, u));
+            stop_time = pp->time_tol * norm_u > norm_diff || fabs(pp->final_time - t) < dt;
+        } else {
+            stop_time = fabs(pp->final_time - t) < 1e-15;
+        }
+        if (!stop_time && pp->cap > 0 && passo >= pp->cap) stop_time = 1;
+    } while (!stop_time);
+    *nsteps = passo;
+    *t_final = t;
+    free(a); free(Da); free(u_old); free(R); free(A); free(U); free(dU); free(BD); free(iBD);
+    free(Id); free(diff);
+    return rc;
+}

@@ -261,6 +261,17 @@ int orc_predictor_old_tr(int nnodes, int nel, int neq, const double *x, const do
                          const orc_prob2d *pb, double ud, const orc_pred_params *pp, int diag,
                          const double *u0, double *u, int *hist, int hist_cap, int *nsteps,
                          double *t_final);
+/* the same loop over the Euler build (orc_build_euler2d; eval_U_dU.c of the
+ * 03 driver: free dofs from u and a, prescribed dofs presc[0..3] and 0),
+ * NO_scaling or Block_scaling_EBE (block_scaling = 1), NO_precond_setup and
+ * the direct solve.  ids[nnodes*4] = Node[].id (-1 prescribed); rot, theta
+ * per node (may be NULL); stab_tol = StabilizationTolerance. */
+int orc_predictor_old_eu(int nnodes, int nel, int neq, const double *x, const double *y,
+                         const int *vertex, const int *lm, const int *ids, const double *presc,
+                         const int *rot, const double *theta, double gamma, double mach,
+                         int dimless, double stab_tol, const orc_pred_params *pp,
+                         int block_scaling, const double *u0, double *u, int *hist, int hist_cap,
+                         int *nsteps, double *t_final);
 
 #ifdef __cplusplus
 }

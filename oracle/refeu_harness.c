@@ -144,3 +144,84 @@ void refeu_build(const double *u, const double *du, double *A, double *R)
     memcpy(A, ME.Aaux, sizeof(double) * 144 * (size_t)PE.nel);
     memcpy(R, FE.F, sizeof(double) * (PE.neq + 1));
 }
+
+/* ---- Time_Integration/Predictor_Old.c with Criteria.c over the build above
+ * (call after refeu_setup and refeu_build_setup).  Supplied here: the
+ * initial solution (u = u0), the 6-argument solver the reference leaves
+ * unimplemented as a direct dense solve of the assembled (possibly
+ * block-scaled) system, and wrappers of the reference's StopBy* criteria
+ * that record each step's corrections and cap the steps.  Scaling: NO_scaling
+ * or Block_scaling_EBE; precond_setup: NO_precond_setup. */
+#include "dense_solve.h"
+int Predictor_Old(ParametersType *, MatrixDataType *, FemStructsType *, FemFunctionsType *,
+                  FemOtherFunctionsType *);
+int StopByIterations(ParametersType *, double, double, int);
+int StopByNorm(ParametersType *, double, double, int);
+int StopBySteadyState(ParametersType *, double *, double *, double);
+int StopByTime(ParametersType *, double *, double *, double);
+int NO_precond_setup(ParametersType *, MatrixDataType *, FemStructsType *, int, double *);
+int NO_scaling(ParametersType *, MatrixDataType *, FemStructsType *);
+int NO_unscaling(ParametersType *, MatrixDataType *, FemStructsType *, double *);
+
+static FemOtherFunctionsType FO;
+static const double *p_u0;
+static int p_norm, p_steady, p_cap, p_steps, p_rc, p_ncorr, *p_hist, p_hist_cap;
+
+static int cbp_initial(ParametersType *P, NodeType *N, double *u)
+{
+    (void)N;
+    memcpy(u, p_u0, sizeof(double) * (P->neq + 1));
+    return 0;
+}
+static int cbp_solver(ParametersType *P, MatrixDataType *M, FemStructsType *S, FemFunctionsType *F,
+                      double *B, double *X)
+{
+    (void)F;
+    int rc = orc_dense_solve_ebe(P->neq, P->nel, 12, M->Aaux, S->lmaux, NULL, B, X);
+    if (rc && !p_rc) p_rc = rc;
+    return rc;
+}
+static int cbp_stop_corr(ParametersType *P, double na, double nda, int i)
+{
+    int s = p_norm ? StopByNorm(P, na, nda, i) : StopByIterations(P, na, nda, i);
+    p_ncorr = i;
+    return s;
+}
+static int cbp_stop_time(ParametersType *P, double *u, double *u_old, double t)
+{
+    if (p_steps < p_hist_cap) p_hist[p_steps] = p_ncorr;
+    p_steps++;
+    int s = p_steady ? StopBySteadyState(P, u, u_old, t) : StopByTime(P, u, u_old, t);
+    if (!s && p_cap > 0 && p_steps >= p_cap) {
+        P->CurrentTime = t;
+        return 1;
+    }
+    return s;
+}
+
+int refeu_pred_run(const double *u0, double alpha, double dt, double final_time, double nl_tol,
+                   int nl_maxiter, int stop_by_norm, int steady, double time_tol,
+                   double stab_tol, int block_scaling, int cap, double *u, int *hist,
+                   int hist_cap, int *nsteps, double *t_final)
+{
+    PE.Alpha = alpha; PE.DeltaT = dt; PE.FinalTime = final_time;
+    PE.NonLinearTolerance = nl_tol; PE.NonLinearMaxIter = nl_maxiter;
+    PE.TimeIntegrationTolerance = time_tol; PE.StabilizationTolerance = stab_tol;
+    PE.CurrentTime = 0.0; PE.iterations = 0;
+    p_u0 = u0; p_norm = stop_by_norm; p_steady = steady; p_cap = cap;
+    p_steps = 0; p_rc = 0; p_ncorr = 0; p_hist = hist; p_hist_cap = hist_cap;
+    FB.InitialSolution = cbp_initial;
+    FB.StopCriteria = cbp_stop_corr;
+    FB.StopTimeIntegration = cbp_stop_time;
+    FB.scaling = block_scaling ? Block_scaling_EBE : NO_scaling;
+    FB.unscaling = NO_unscaling;
+    FB.precond_setup = NO_precond_setup;
+    memset(&FO, 0, sizeof FO);
+    FO.Build = Build_M_K_F_SUPG;
+    FO.solver = cbp_solver;
+    Predictor_Old(&PE, &ME, &FE, &FB, &FO);
+    memcpy(u, FE.u, sizeof(double) * (PE.neq + 1));
+    *nsteps = p_steps;
+    *t_final = PE.CurrentTime;
+    return p_rc;
+}

@@ -91,3 +91,57 @@ def test_predictor_rejects_a_foreign_operator(ctx):
     u = ctx.to_device(d["u0"])
     with pytest.raises(F.FemGpuError, match="status 2"):
         F.predictor_old_transport(mesh, pb, other, None, up, u, **_params(d))
+
+
+# ---- the Euler driver (03): Predictor_Old over the Euler build -------------
+EU_CASES = golden_files("predeu_*.npz")
+
+
+def _setup_eu(ctx, d):
+    neq, nn = int(d["neq"]), int(d["nnodes"])
+    m = F.Euler2D(ctx, d["x"], d["y"], d["vertex"], d["lm"], neq)
+    op = F.Operator.ebe(ctx, neq, d["lm"], 12)
+    blocks = F.Preconditioner.blockdiag(op, nn, d["vertex"], d["ids"])
+    pr = d["params"]
+    ep = F.EulerParams(1.4, 3.0, int(pr[9]), pr[0], pr[1], 1e-10)
+    presc = ctx.to_device(np.tile(d["presc"], nn))
+    rot = ctx.to_device((d["v1type"] == -1).astype(np.int32))
+    theta = ctx.to_device(d["theta"])
+    return m, op, blocks, ep, presc, rot, theta, neq
+
+
+def _params_eu(d):
+    pr = d["params"]
+    return dict(alpha=pr[0], dt=pr[1], final_time=pr[2], nl_tol=pr[3], nl_maxiter=int(pr[4]),
+                stop_by_norm=bool(pr[5]), steady=bool(pr[6]), time_tol=pr[7])
+
+
+@pytest.mark.parametrize("path", EU_CASES, ids=os.path.basename)
+def test_predictor_euler_matches_reference(ctx, path):
+    """the reference's scaling (NO_scaling or Block_scaling_EBE, as the file
+    says), and the block-diagonal preconditioner inside GMRES instead: the
+    same steps and corrections, u within 1e-9"""
+    d = load_golden(path)
+    m, op, blocks, ep, presc, rot, theta, neq = _setup_eu(ctx, d)
+    ref_mode = F.PRED_EU_BLOCK_SCALING if int(d["params"][8]) else F.PRED_EU_NOT
+    for mode in (ref_mode, F.PRED_EU_BLOCKDIAG):
+        u = ctx.to_device(d["u0"])
+        info, hist = F.predictor_old_euler(m, ep, op, blocks if mode else None, mode, presc,
+                                           rot, theta, u, tol=1e-13, lmax=2000, max_steps=200,
+                                           **_params_eu(d))
+        assert info.status == F.FG_OK and info.unconverged == 0
+        assert info.steps == int(d["nsteps"]) and info.t == float(d["t_final"])
+        assert np.array_equal(hist, d["hist"])
+        ug = u.get()
+        assert np.max(np.abs(ug[:neq] - d["u"][:neq])) <= 1e-9 * np.max(np.abs(d["u"][:neq]))
+        print(f"\n{os.path.basename(path)} mode {mode}: {info.steps} steps, "
+              f"{info.corrections} corrections, {info.krylov_steps} Arnoldi steps")
+
+
+def test_predictor_euler_rejects_missing_blocks(ctx):
+    d = load_golden(EU_CASES[0])
+    m, op, blocks, ep, presc, rot, theta, neq = _setup_eu(ctx, d)
+    u = ctx.to_device(d["u0"])
+    with pytest.raises(F.FemGpuError, match="status 2"):
+        F.predictor_old_euler(m, ep, op, None, F.PRED_EU_BLOCK_SCALING, presc, rot, theta, u,
+                              **_params_eu(d))

@@ -106,3 +106,42 @@ def test_predictor_vs_reference_live(o):
     uo, ho, no, to, rco = _run(o, d, **args)
     assert rcr == rco == 0 and nr == no and tr == to
     assert np.array_equal(hr, ho) and np.array_equal(ur, uo)
+
+
+# ---- the same loop over the Euler build (03_Euler_Equations) ---------------
+EU_CASES = golden_files("predeu_*.npz")
+
+
+def _run_eu(o, d, **over):
+    pr = d["params"]
+    args = dict(alpha=pr[0], dt=pr[1], final_time=pr[2], nl_tol=pr[3], nl_maxiter=int(pr[4]),
+                stop_by_norm=int(pr[5]), steady=int(pr[6]), time_tol=pr[7],
+                block_scaling=int(pr[8]), cap=200)
+    args.update(over)
+    rot = (d["v1type"] == -1).astype(np.int32)
+    return o.predictor_old_eu(d["x"], d["y"], d["vertex"], d["lm"], int(d["neq"]), d["ids"],
+                              d["presc"], rot, d["theta"], 1.4, 3.0, int(pr[9]), 1e-10,
+                              d["u0"], **args)
+
+
+@pytest.mark.parametrize("path", EU_CASES, ids=os.path.basename)
+def test_predictor_euler_golden(o, path):
+    """bitwise against the reference's Predictor_Old over its Euler build
+    (tests/golden/make_golden_pred_eu.py), with and without Block_scaling_EBE"""
+    d = load_golden(path)
+    u, hist, ns, tf, rc = _run_eu(o, d)
+    assert rc == 0
+    assert ns == int(d["nsteps"]) and tf == float(d["t_final"])
+    assert np.array_equal(hist, d["hist"])
+    assert np.array_equal(u, d["u"])
+
+
+def test_predictor_euler_block_scaling_keeps_the_solution(o):
+    """Block_scaling_EBE left-multiplies each node's rows by its inverse block
+    (and fixes the prescribed dofs): the scaled system has the same solution,
+    so the runs with and without it agree to rounding"""
+    d0 = load_golden([p for p in EU_CASES if p.endswith("n6_dimless.npz")][0])
+    d1 = load_golden([p for p in EU_CASES if p.endswith("n6_dimless_scaled.npz")][0])
+    assert np.array_equal(d0["hist"], d1["hist"])
+    neq = int(d0["neq"])
+    assert np.max(np.abs(d0["u"][:neq] - d1["u"][:neq])) <= 1e-12 * np.max(np.abs(d0["u"]))
