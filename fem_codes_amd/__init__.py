@@ -131,9 +131,17 @@ SIGNATURES = {
     "fg_predictor_old_transport": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(PredictorParams),
                                         C.POINTER(GmresOpts), _vp, _vp, _i,
                                         C.POINTER(PredictorInfo)]),
+    "fg_ede4_create": (_i, [_vp, _i, _i, _i, _ip, _ip, C.POINTER(_vp)]),
+    "fg_ede4_info": (_i, [_vp, C.POINTER(_i)]),
+    "fg_ede4_get": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "fg_ede4_assemble": (_i, [_vp, _vp, _vp]),
+    "fg_ede4_destroy": (_i, [_vp]),
     "fg_predictor_old_euler": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp,
                                     C.POINTER(PredictorParams), C.POINTER(GmresOpts), _vp, _vp,
                                     _i, C.POINTER(PredictorInfo)]),
+    "fg_predictor_old_euler_ede": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                        C.POINTER(PredictorParams), C.POINTER(GmresOpts), _vp,
+                                        _vp, _i, C.POINTER(PredictorInfo)]),
     "fg_op_assemble_csr": (_i, [_vp, _i, _ip, _vp]),
     "fg_op_assemble_ede": (_i, [_vp, _i, _ip, _ip, _vp]),
     "fg_csr_pattern_k": (_i, [_i, _i, _i, _ip, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
@@ -385,8 +393,9 @@ class Preconditioner:
 
     @classmethod
     def blockdiag(cls, op: Operator, nnodes: int, vertex, ids):
-        """node block-diagonal (BlockDiag_precond*) of a k = 12 EBE operator;
-        vertex (nel*3), ids (nnodes*4, < 0 = prescribed)"""
+        """node block-diagonal (BlockDiag_precond*) of a k = 12 EBE operator
+        (vertex: nel*3) or of a k = 8 edge operator (vertex: the edge nodes,
+        nedge*2); ids (nnodes*4, < 0 = prescribed)"""
         h = _vp()
         _check(lib.fg_prec_blockdiag_create(op.h, int(nnodes),
                                             np.ascontiguousarray(vertex, np.int32),
@@ -534,7 +543,45 @@ def predictor_old_transport(mesh: "Mesh2D", prob: "Supg2dProblem", op: Operator,
     return info, hist[:min(info.steps, hist_cap)].copy()
 
 
-PRED_EU_NOT, PRED_EU_BLOCK_SCALING, PRED_EU_BLOCKDIAG = 0, 1, 2
+class Ede4:
+    """fg_ede4: the edge structure of the 4-dof (Euler) driver
+    (ede_Initialization.c and ede_assembly.c of 03_Euler_Equations)."""
+
+    def __init__(self, ctx: Context, nnodes, vertex, ids, neq):
+        vertex = np.ascontiguousarray(vertex, np.int32)
+        ids = np.ascontiguousarray(ids, np.int32)
+        h = _vp()
+        _check(lib.fg_ede4_create(ctx.h, int(nnodes), vertex.size // 3, int(neq), vertex, ids,
+                                  C.byref(h)), "fg_ede4_create")
+        self.ctx, self.h, self.nel, self.neq = ctx, h, vertex.size // 3, int(neq)
+        n = _i()
+        lib.fg_ede4_info(h, C.byref(n))
+        self.nedge = n.value
+
+    def arrays(self):
+        """(order[nel*3], edge_by_elem[nel*3], lm2[nedge*8], edge_nodes[nedge*2])"""
+        o, e = np.zeros(3 * self.nel, np.int32), np.zeros(3 * self.nel, np.int32)
+        l2, en = np.zeros(8 * self.nedge, np.int32), np.zeros(2 * self.nedge, np.int32)
+        _check(lib.fg_ede4_get(self.h, o.ctypes.data, e.ctypes.data, l2.ctypes.data,
+                               en.ctypes.data), "fg_ede4_get")
+        return o, e, l2, en
+
+    def assemble(self, A12: DeviceArray, Aede: DeviceArray):
+        _check(lib.fg_ede4_assemble(self.h, A12.ptr, Aede.ptr), "fg_ede4_assemble")
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib.fg_ede4_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+PRED_EU_NOT, PRED_EU_BLOCK_SCALING, PRED_EU_BLOCKDIAG, PRED_EU_TWO = 0, 1, 2, 3
 
 
 def predictor_old_euler(mesh: "Euler2D", ep: "EulerParams", op: Operator,
@@ -543,20 +590,28 @@ def predictor_old_euler(mesh: "Euler2D", ep: "EulerParams", op: Operator,
                         alpha, dt, final_time, nl_tol, nl_maxiter, stop_by_norm=True,
                         steady=False, time_tol=0.0, max_steps=0, kmax=30, lmax=1000, tol=1e-8,
                         precision=GMRES_FP64, engine=ENGINE_AUTO, orth=ORTH_MGS,
-                        hist_cap=4096):
+                        hist_cap=4096, edges: "Ede4 | None" = None):
     """Predictor_Old over the Euler build, on the device (mode: PRED_EU_NOT,
-    PRED_EU_BLOCK_SCALING, PRED_EU_BLOCKDIAG); u (neq+1) holds the initial
+    PRED_EU_BLOCK_SCALING, PRED_EU_BLOCKDIAG, PRED_EU_TWO with op the ebe2
+    operator); with `edges` the edge-by-edge scheme (op: the k = 8 operator
+    on the edges' lm2, or its ede2 operator).  u (neq+1) holds the initial
     solution and receives the result.  Returns (info, corrections per step)."""
     pp = PredictorParams(alpha, dt, final_time, nl_tol, nl_maxiter, int(stop_by_norm),
                          int(steady), time_tol, max_steps)
     o = GmresOpts(kmax, lmax, tol, precision, 0.0, 0, None, None, engine, orth)
     hist = np.zeros(max(1, hist_cap), np.int32)
     info = PredictorInfo()
-    rc = lib.fg_predictor_old_euler(mesh.h, C.byref(ep), op.h,
-                                    blocks.h if blocks is not None else None, int(mode),
-                                    presc.ptr, rot.ptr if rot is not None else None,
-                                    theta.ptr if theta is not None else None, C.byref(pp),
-                                    C.byref(o), u.ptr, hist.ctypes.data, hist_cap, C.byref(info))
+    bh = blocks.h if blocks is not None else None
+    rp = rot.ptr if rot is not None else None
+    tp = theta.ptr if theta is not None else None
+    if edges is None:
+        rc = lib.fg_predictor_old_euler(mesh.h, C.byref(ep), op.h, bh, int(mode), presc.ptr, rp,
+                                        tp, C.byref(pp), C.byref(o), u.ptr, hist.ctypes.data,
+                                        hist_cap, C.byref(info))
+    else:
+        rc = lib.fg_predictor_old_euler_ede(mesh.h, C.byref(ep), edges.h, op.h, bh, int(mode),
+                                            presc.ptr, rp, tp, C.byref(pp), C.byref(o), u.ptr,
+                                            hist.ctypes.data, hist_cap, C.byref(info))
     _check(rc, "fg_predictor_old_euler", ok=(FG_OK, FG_ERR_NOCONV))
     return info, hist[:min(info.steps, hist_cap)].copy()
 

@@ -25,8 +25,9 @@ int fg_ebe_apply_offblock(fg_op *op, int fp32, void *P, void *Q, const int *skip
 
 struct fg_nodeblk {
     int nnodes = 0, nel = 0;
+    int k = 12, nv = 3;       // dofs and nodes per item: 12 / 3 (elements), 8 / 2 (edges)
     int *d_Id = nullptr;      // [4][nnodes] equation of dof d of node I (neq: prescribed)
-    int *d_vert = nullptr;    // [3][nel] nodes of each element
+    int *d_vert = nullptr;    // [nv][nel] nodes of each item
     int *d_noff = nullptr;    // [nnodes+1] incidence offsets
     int *d_ninc = nullptr;    // element*4 + vertex slot, element ascending
     double *d_BD = nullptr, *d_iBD = nullptr;   // [16][nnodes]
@@ -83,10 +84,12 @@ __device__ __forceinline__ bool node_free(const int *Id, int nn, int I, int neq)
     return Id[I] != neq || Id[nn + I] != neq || Id[2 * nn + I] != neq || Id[3 * nn + I] != neq;
 }
 
-// node blocks: BD_I = sum over the elements e containing I (ascending) of
-// the 4x4 block of A_e at (4v, 4v); entry (r, c) of that block is element
-// entry 52 v + 12 r + c (I_diag).  Then the inverse, and FP32 copies.
-__global__ void k_bd_setup(int nn, int neq, const int *__restrict__ Id,
+// node blocks: BD_I = sum over the items e containing I (ascending) of the
+// 4x4 block of A_e at (4v, 4v); entry (r, c) of that block is item entry
+// 4(k+1) v + k r + c (I_diag: 52 v + 12 r + c for the 12-dof element,
+// BlockDiag_precond_EBE_setup.c:18-19; 36 v + 8 r + c for the 8-dof edge,
+// BlockDiag_precond_EDE_setup.c:22).  Then the inverse, and FP32 copies.
+__global__ void k_bd_setup(int nn, int neq, int k, const int *__restrict__ Id,
                            const int *__restrict__ noff, const int *__restrict__ ninc,
                            const long long *__restrict__ kpos, const int *__restrict__ kstride,
                            const double *__restrict__ K64, double *__restrict__ BD,
@@ -105,7 +108,8 @@ __global__ void k_bd_setup(int nn, int neq, const int *__restrict__ Id,
                 const long long ks = kstride[e];
 #pragma unroll
                 for (int q = 0; q < 16; q++)
-                    s[q] = s[q] + K64[fg_ebe_vidx(p, ks, 52 * v + 12 * (q >> 2) + (q & 3), 4)];
+                    s[q] = s[q] + K64[fg_ebe_vidx(p, ks, 4 * (k + 1) * v + k * (q >> 2) + (q & 3),
+                                                  4)];
             }
             inverse4(s, r);
         } else {
@@ -267,10 +271,11 @@ extern "C" int fg_prec_blockdiag_create(fg_op *op, int nnodes, const int *vertex
                                         fg_prec **out)
 {
     FG_ARG(op && vertex && ids && out, "op/vertex/ids/out");
-    FG_ARG(op->kind == FG_OP_EBE && op->k == 12, "the node block-diagonal needs a k = 12 EBE operator");
+    FG_ARG(op->kind == FG_OP_EBE && (op->k == 12 || op->k == 8),
+           "the node block-diagonal needs a k = 12 (element) or k = 8 (edge) operator");
     FG_ARG(nnodes > 0, "nnodes > 0");
-    const int nel = op->nitems, neq = op->neq;
-    for (long long t = 0; t < 3LL * nel; t++)
+    const int nel = op->nitems, neq = op->neq, K = op->k, NV = K / 4;
+    for (long long t = 0; t < (long long)NV * nel; t++)
         FG_ARG(vertex[t] >= 0 && vertex[t] < nnodes, "vertex out of range");
     std::vector<int> Id(4 * (size_t)nnodes), seen(neq, 0);
     for (int I = 0; I < nnodes; I++)
@@ -284,35 +289,38 @@ extern "C" int fg_prec_blockdiag_create(fg_op *op, int nnodes, const int *vertex
             }
             Id[(size_t)d * nnodes + I] = g;
         }
-    // the operator's lm must hold these node dofs (Fill_LM.c:7-24: DOF fastest per node)
+    // the operator's lm must hold these node dofs (Fill_LM.c:7-24 / the edge
+    // lm of ede_Initialization.c: DOF fastest per node)
     {
-        std::vector<int> lm(12 * (size_t)nel);
+        std::vector<int> lm((size_t)K * nel);
         if (nel) FG_HIP(hipMemcpy(lm.data(), op->d_lm, lm.size() * sizeof(int), hipMemcpyDeviceToHost));
         for (int e = 0; e < nel; e++)
-            for (int v = 0; v < 3; v++)
+            for (int v = 0; v < NV; v++)
                 for (int d = 0; d < 4; d++)
-                    FG_ARG(lm[12 * (size_t)e + 4 * v + d] ==
-                               Id[(size_t)d * nnodes + vertex[3 * (size_t)e + v]],
+                    FG_ARG(lm[(size_t)K * e + 4 * v + d] ==
+                               Id[(size_t)d * nnodes + vertex[(size_t)NV * e + v]],
                            "lm does not match vertex/ids (node dofs, DOF fastest)");
     }
-    std::vector<int> vert(3 * (size_t)nel), noff(nnodes + 1, 0);
+    std::vector<int> vert((size_t)NV * nel), noff(nnodes + 1, 0);
     for (int e = 0; e < nel; e++)
-        for (int v = 0; v < 3; v++) {
-            vert[(size_t)v * nel + e] = vertex[3 * (size_t)e + v];
-            noff[vertex[3 * (size_t)e + v] + 1]++;
+        for (int v = 0; v < NV; v++) {
+            vert[(size_t)v * nel + e] = vertex[(size_t)NV * e + v];
+            noff[vertex[(size_t)NV * e + v] + 1]++;
         }
     for (int I = 0; I < nnodes; I++) noff[I + 1] += noff[I];
     std::vector<int> ninc(noff[nnodes]);
     {
         std::vector<int> fill(noff.begin(), noff.end() - 1);
         for (int e = 0; e < nel; e++)
-            for (int v = 0; v < 3; v++) ninc[fill[vertex[3 * (size_t)e + v]]++] = 4 * e + v;
+            for (int v = 0; v < NV; v++) ninc[fill[vertex[(size_t)NV * e + v]]++] = 4 * e + v;
     }
     FG_ARG(nel < (1 << 29), "too many elements");
     FG_HIP(hipSetDevice(op->ctx->device));
     fg_nodeblk *b = new fg_nodeblk();
     b->nnodes = nnodes;
     b->nel = nel;
+    b->k = K;
+    b->nv = NV;
     int rc = FG_OK;
     if ((rc = upload(Id, &b->d_Id)) != FG_OK || (rc = upload(vert, &b->d_vert)) != FG_OK ||
         (rc = upload(noff, &b->d_noff)) != FG_OK || (rc = upload(ninc, &b->d_ninc)) != FG_OK) {
@@ -346,7 +354,7 @@ int fg_nodeblk_setup(fg_prec *pc, double *F)
     fg_ctx *c = op->ctx;
     fg_nodeblk *b = pc->nb;
     FG_LAUNCH(k_bd_setup, fg_vec_grid(c, b->nnodes), FG_TPB, 0, c->stream, b->nnodes, op->neq,
-              b->d_Id, b->d_noff, b->d_ninc, op->d_kpos, op->d_kstride, op->d_K64, b->d_BD,
+              b->k, b->d_Id, b->d_noff, b->d_ninc, op->d_kpos, op->d_kstride, op->d_K64, b->d_BD,
               b->d_iBD, b->d_BD32, b->d_iBD32);
     b->setup_done = 1;
     if (F)
@@ -400,6 +408,8 @@ extern "C" int fg_prec_blockdiag_get(fg_prec *pc, double *BD_host, double *invBD
 extern "C" int fg_block_scaling_ebe(fg_prec *pc, double *F)
 {
     FG_ARG(pc && pc->kind == FG_PREC_BLOCKDIAG, "block-diagonal preconditioner");
+    FG_ARG(pc->nb->k == 12, "Block_scaling_EBE needs the 12-dof element operator "
+                            "(the reference scales only the EBE scheme, setScaling.c:11-18)");
     fg_op *op = pc->op;
     fg_ctx *c = op->ctx;
     fg_nodeblk *b = pc->nb;
@@ -416,7 +426,8 @@ extern "C" int fg_block_scaling_ebe(fg_prec *pc, double *F)
 extern "C" int fg_op_ebe2_create(fg_op *base, fg_prec *bd, int form, fg_op **out)
 {
     FG_ARG(base && bd && out, "op/blockdiag/out");
-    FG_ARG(base->kind == FG_OP_EBE && base->k == 12, "ebe2 needs a k = 12 EBE operator");
+    FG_ARG(base->kind == FG_OP_EBE && (base->k == 12 || base->k == 8),
+           "ebe2 needs a k = 12 (element) or k = 8 (edge) operator");
     FG_ARG(bd->kind == FG_PREC_BLOCKDIAG && bd->op == base,
            "ebe2 needs the block-diagonal preconditioner of the same operator");
     FG_ARG(form == FG_EBE2_BLOCKDIAG || form == FG_EBE2_NOBLOCKDIAG, "form");
@@ -425,7 +436,7 @@ extern "C" int fg_op_ebe2_create(fg_op *base, fg_prec *bd, int form, fg_op **out
     op->kind = FG_OP_EBE2;
     op->neq = base->neq;
     op->nitems = base->nitems;
-    op->k = 12;
+    op->k = base->k;
     op->nnz = base->nnz;
     op->base = base;
     op->bd = bd;
@@ -464,12 +475,15 @@ int fg_ebe2_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
     return FG_OK;
 }
 
-// compulsory bytes of one ebe2 product: the off-block element product
-// (96 of 144 values, 12 u16 + incidence per element, P read, Q written) and
-// the node pass (16 block values + 4 dof ids per node, P and Q read, Q written)
+// compulsory bytes of one ebe2 product: the off-block item product (k*k -
+// 16 nv values and k equation numbers per item: 96 of 144 for an element, 32
+// of 64 for an edge; P read, Q written) and the node pass (16 block values +
+// 4 dof ids per node, P and Q read, Q written)
 double fg_ebe2_bytes(const fg_op *op, int fp32)
 {
     const double s = fp32 ? 4.0 : 8.0;
     const double nel = op->nitems, neq = op->neq, nn = op->bd->nb->nnodes;
-    return nel * (96.0 * s + 48.0) + 2.0 * neq * s + nn * (16.0 * s + 16.0) + 3.0 * neq * s;
+    const double k = op->k, nv = op->k / 4;
+    return nel * ((k * k - 16.0 * nv) * s + 4.0 * k) + 2.0 * neq * s + nn * (16.0 * s + 16.0) +
+           3.0 * neq * s;
 }

@@ -360,6 +360,7 @@ static int ebe_apply(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *skip)
         case 2: FG_CHECK((launch_apply_ept<2, T>(op, Kv, P, Q, hp, skip))); break;
         case 3: FG_CHECK((launch_apply_ept<3, T>(op, Kv, P, Q, hp, skip))); break;
         case 4: FG_CHECK((launch_apply_ept<4, T>(op, Kv, P, Q, hp, skip))); break;
+        case 8: FG_CHECK((launch_apply_ept<8, T>(op, Kv, P, Q, hp, skip))); break;
         case 12: FG_CHECK((launch_apply_ept<12, T>(op, Kv, P, Q, hp, skip))); break;
         case 16: FG_CHECK((launch_apply_ept<16, T>(op, Kv, P, Q, hp, skip))); break;
         default: fg_set_error("unsupported k=%d", op->k); return FG_ERR_ARG;
@@ -382,6 +383,7 @@ int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
         case 2: return launch_apply_ept<2, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
         case 3: return launch_apply_ept<3, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
         case 4: return launch_apply_ept<4, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
+        case 8: return launch_apply_ept<8, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
         case 12: return launch_apply_ept<12, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
         case 16: return launch_apply_ept<16, float>(op, op->d_K32, (float *)P, (float *)Q, op->d_hpart32, nullptr);
         }
@@ -390,6 +392,7 @@ int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
         case 2: return launch_apply_ept<2, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
         case 3: return launch_apply_ept<3, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
         case 4: return launch_apply_ept<4, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
+        case 8: return launch_apply_ept<8, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
         case 12: return launch_apply_ept<12, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
         case 16: return launch_apply_ept<16, double>(op, op->d_K64, (double *)P, (double *)Q, op->d_hpart64, nullptr);
         }
@@ -407,20 +410,29 @@ int fg_ebe_apply_f32(fg_op *op, float *P, float *Q, const int *skip)
     return ebe_apply<float>(op, op->d_K32, P, Q, op->d_hpart32, skip);
 }
 
-// off-block product of a 12-dof (3 nodes x 4 dofs) operator: the row sums
-// skip the row node's own 4x4 block (ebe2mvNDOF4.c:24-70); Q complete, P and
-// Q sink slots 0
+// off-block product of a 4-dof-per-node operator (12 dofs: 3 nodes, 8 dofs:
+// an edge's 2 nodes): the row sums skip the row node's own 4x4 block
+// (ebe2mvNDOF4.c:24-70, ede2mvNDOF4.c:42-51); Q complete, P and Q sink slots 0
 int fg_ebe_apply_offblock(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 {
-    FG_ARG(op->k == 12, "off-block product needs k = 12");
+    FG_ARG(op->k == 12 || op->k == 8, "off-block product needs k = 8 or 12");
     fg_ctx *c = op->ctx;
     if (op->nblk > 0) {
-        if (fp32)
-            FG_CHECK((launch_apply_ept<12, float, 4>(op, op->d_K32, (float *)P, (float *)Q,
-                                                     op->d_hpart32, skip)));
-        else
-            FG_CHECK((launch_apply_ept<12, double, 4>(op, op->d_K64, (double *)P, (double *)Q,
-                                                      op->d_hpart64, skip)));
+        if (op->k == 12) {
+            if (fp32)
+                FG_CHECK((launch_apply_ept<12, float, 4>(op, op->d_K32, (float *)P, (float *)Q,
+                                                         op->d_hpart32, skip)));
+            else
+                FG_CHECK((launch_apply_ept<12, double, 4>(op, op->d_K64, (double *)P,
+                                                          (double *)Q, op->d_hpart64, skip)));
+        } else {
+            if (fp32)
+                FG_CHECK((launch_apply_ept<8, float, 4>(op, op->d_K32, (float *)P, (float *)Q,
+                                                        op->d_hpart32, skip)));
+            else
+                FG_CHECK((launch_apply_ept<8, double, 4>(op, op->d_K64, (double *)P,
+                                                         (double *)Q, op->d_hpart64, skip)));
+        }
     }
     int n = op->nhalo + op->nzero;
     int g = (n + FG_TPB - 1) / FG_TPB;
@@ -470,7 +482,8 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
 {
     FG_ARG(c && out && lm, "ctx/out/lm");
     FG_ARG(neq >= 0 && nitems >= 0, "sizes");
-    FG_ARG(k == 2 || k == 3 || k == 4 || k == 12 || k == 16, "k must be 2, 3, 4, 12 or 16");
+    FG_ARG(k == 2 || k == 3 || k == 4 || k == 8 || k == 12 || k == 16,
+           "k must be 2, 3, 4, 8, 12 or 16");
     int ept = per_block > 0 ? (per_block + FG_TPB - 1) / FG_TPB : default_ept(k);
     if (ept > 4) ept = 8;
     else if (ept > 2) ept = 4;

@@ -122,7 +122,8 @@ __device__ __forceinline__ void fg_ld4(const T *x, unsigned g, T out[4])
 
 template <int K, int EPT, typename T>
 struct FgEbeRegs {
-    static constexpr bool PRE = EPT * K * K <= 72;   // element values held in registers
+    // element values held in registers: at most 36 FP64 / 72 FP32 values
+    static constexpr bool PRE = EPT * K * K <= (sizeof(T) == 8 ? 36 : 72);
     // block dofs per thread whose global id and halo slot are kept in
     // registers from the gather to the end of the block (more are reloaded)
     static constexpr int ND = 2;

@@ -146,6 +146,7 @@ fg_ctx *fg_mesh2d_ctx(const fg_mesh2d *m);
 int fg_euler2d_eval_U_dU(const fg_euler2d *m, const double *u, const double *du,
                          const double *presc, double *U, double *dU);
 void fg_euler2d_sizes(const fg_euler2d *m, fg_ctx **c, int *nnodes, int *nel, int *neq);
+void fg_ede4_sizes(const fg_ede4 *g, fg_ctx **c, int *nel, int *nedge);
 double fg_ebe2_bytes(const fg_op *op, int fp32);
 
 struct fg_prec {

@@ -218,24 +218,45 @@ extern "C" int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem 
                           hist_cap, info, &work, build);
 }
 
-extern "C" int fg_predictor_old_euler(fg_euler2d *m, const fg_euler2d_params *ep, fg_op *A,
-                                      fg_prec *blocks, int mode, const double *presc,
-                                      const int *rot, const double *theta,
-                                      const fg_predictor_params *p, const fg_gmres_opts *gm,
-                                      double *u, int *corr_hist, int hist_cap,
-                                      fg_predictor_info *info)
+// the Euler driver's loop over its element (k = 12) or edge (k = 8, edges !=
+// NULL) scheme; A is the product operator: the EBE/EDE operator itself or,
+// for FG_PRED_EU_TWO, the ebe2/ede2 operator over it
+static int euler_predictor(fg_euler2d *m, const fg_euler2d_params *ep, fg_ede4 *edges, fg_op *A,
+                           fg_prec *blocks, int mode, const double *presc, const int *rot,
+                           const double *theta, const fg_predictor_params *p,
+                           const fg_gmres_opts *gm, double *u, int *corr_hist, int hist_cap,
+                           fg_predictor_info *info)
 {
     FG_ARG(m && ep && A && presc && p && gm && u, "mesh/params/operator/presc/params/opts/u");
     FG_ARG(mode == FG_PRED_EU_NOT || mode == FG_PRED_EU_BLOCK_SCALING ||
-               mode == FG_PRED_EU_BLOCKDIAG, "mode");
-    FG_ARG(mode == FG_PRED_EU_NOT || (blocks && blocks->kind == FG_PREC_BLOCKDIAG && blocks->op == A),
+               mode == FG_PRED_EU_BLOCKDIAG || mode == FG_PRED_EU_TWO, "mode");
+    FG_ARG(!(edges && mode == FG_PRED_EU_BLOCK_SCALING),
+           "block scaling is defined for the element scheme only (setScaling.c)");
+    fg_op *V = A;   // the operator that holds the matrices
+    if (mode == FG_PRED_EU_TWO) {
+        FG_ARG(A->kind == FG_OP_EBE2 && A->form == FG_EBE2_BLOCKDIAG && A->bd == blocks,
+               "FG_PRED_EU_TWO needs A = fg_op_ebe2_create(base, blocks, FG_EBE2_BLOCKDIAG)");
+        V = A->base;
+    }
+    FG_ARG(mode == FG_PRED_EU_NOT || (blocks && blocks->kind == FG_PREC_BLOCKDIAG && blocks->op == V),
            "block scaling and the block-diagonal preconditioner need a FG_PREC_BLOCKDIAG of A");
     fg_ctx *c = nullptr;
-    int nn = 0, nel = 0, neq = 0;
+    int nn = 0, nel = 0, neq = 0, nedge = 0;
     fg_euler2d_sizes(m, &c, &nn, &nel, &neq);
-    FG_ARG(A->ctx == c && A->kind == FG_OP_EBE && A->k == 12 && A->neq == neq && A->nitems == nel,
-           "A must be the k = 12 EBE operator of the Euler mesh's lm");
-    const size_t ne = (size_t)nel * 144;
+    if (edges) {
+        fg_ctx *ce = nullptr;
+        int nel_e = 0;
+        fg_ede4_sizes(edges, &ce, &nel_e, &nedge);
+        FG_ARG(ce == c && nel_e == nel, "the edges must be those of the Euler mesh");
+        FG_ARG(V->ctx == c && V->kind == FG_OP_EBE && V->k == 8 && V->neq == neq &&
+                   V->nitems == nedge,
+               "A must be the k = 8 operator of the edges' lm2");
+    } else {
+        FG_ARG(V->ctx == c && V->kind == FG_OP_EBE && V->k == 12 && V->neq == neq &&
+                   V->nitems == nel,
+               "A must be the k = 12 EBE operator of the Euler mesh's lm");
+    }
+    const size_t ne = (size_t)nel * 144, nd = (size_t)nedge * 64;
     double *work = nullptr;
     double *const *wp = &work;
     // Build_M_K_F_SUPG with Alpha_Build / DeltaT_Build set by the predictor
@@ -244,15 +265,47 @@ extern "C" int fg_predictor_old_euler(fg_euler2d *m, const fg_euler2d_params *ep
     bp.alpha = p->alpha;
     bp.dt = p->dt;
     auto build = [&](const double *uu, const double *aa, double *R) -> int {
-        double *U = *wp, *dU = U + 4 * (size_t)nn, *ke = dU + 4 * (size_t)nn;
+        double *U = *wp, *dU = U + 4 * (size_t)nn, *ke = dU + 4 * (size_t)nn, *kd = ke + ne;
         FG_CHECK(fg_euler2d_eval_U_dU(m, uu, aa, presc, U, dU));
         FG_CHECK(fg_euler2d_build(m, &bp, U, dU, rot, theta, ke, R));
-        FG_CHECK(fg_op_set_values_dev(A, ke));
+        if (edges) {
+            // ede_assembly.c: the element matrices into the edge matrices
+            FG_CHECK(fg_ede4_assemble(edges, ke, kd));
+            FG_CHECK(fg_op_set_values_dev(V, kd));
+        } else {
+            FG_CHECK(fg_op_set_values_dev(V, ke));
+        }
         // Block_scaling_EBE (Block_scaling.c:10-59): A and R scaled in place
         if (mode == FG_PRED_EU_BLOCK_SCALING) FG_CHECK(fg_block_scaling_ebe(blocks, R));
+        // EBE2/EDE2 with BlockDiag: the setup scales R, the solve has no
+        // preconditioner (setPreconditioner.c:27-34)
+        if (mode == FG_PRED_EU_TWO) FG_CHECK(fg_prec_setup(blocks, R));
         return FG_OK;
     };
-    return predictor_loop(c, neq, 8 * (size_t)nn + (ne ? ne : 1), A,
-                          mode == FG_PRED_EU_BLOCKDIAG ? blocks : nullptr, p, gm, u, corr_hist,
-                          hist_cap, info, &work, build);
+    const size_t wd = 8 * (size_t)nn + ne + nd;
+    return predictor_loop(c, neq, wd ? wd : 1, A, mode == FG_PRED_EU_BLOCKDIAG ? blocks : nullptr,
+                          p, gm, u, corr_hist, hist_cap, info, &work, build);
+}
+
+extern "C" int fg_predictor_old_euler(fg_euler2d *m, const fg_euler2d_params *ep, fg_op *A,
+                                      fg_prec *blocks, int mode, const double *presc,
+                                      const int *rot, const double *theta,
+                                      const fg_predictor_params *p, const fg_gmres_opts *gm,
+                                      double *u, int *corr_hist, int hist_cap,
+                                      fg_predictor_info *info)
+{
+    return euler_predictor(m, ep, nullptr, A, blocks, mode, presc, rot, theta, p, gm, u,
+                           corr_hist, hist_cap, info);
+}
+
+extern "C" int fg_predictor_old_euler_ede(fg_euler2d *m, const fg_euler2d_params *ep,
+                                          fg_ede4 *edges, fg_op *A, fg_prec *blocks, int mode,
+                                          const double *presc, const int *rot,
+                                          const double *theta, const fg_predictor_params *p,
+                                          const fg_gmres_opts *gm, double *u, int *corr_hist,
+                                          int hist_cap, fg_predictor_info *info)
+{
+    FG_ARG(edges, "edges");
+    return euler_predictor(m, ep, edges, A, blocks, mode, presc, rot, theta, p, gm, u,
+                           corr_hist, hist_cap, info);
 }

@@ -163,7 +163,7 @@ int fg_prec_ilu_sweeps(fg_prec *pc, int sweeps);
 int fg_prec_ilu_get(fg_prec *pc, double *LU_host, double *D_host);
 
 /* Node block-diagonal of a 4-dof, 3-node element operator (k = 12, Euler
- * 2D).  FG_PREC_BLOCKDIAG replaces BlockDiag_precond_EBE_setup.c:6-86 (setup)
+ * 2D; or of a 2-node edge operator, k = 8, with `vertex` = the edge nodes).  FG_PREC_BLOCKDIAG replaces BlockDiag_precond_EBE_setup.c:6-86 (setup)
  * and BlockDiag_precond.c:3-27 (apply).  vertex[nel*3]: the element's nodes,
  * in the element order of the operator's lm; ids[nnodes*4]: Node[].id of
  * every node (< 0 or == neq: prescribed).  Setup sums the same-node 4x4
@@ -202,6 +202,32 @@ int fg_op_ebe2_create(fg_op *ebe12, fg_prec *blockdiag, int form, fg_op **out);
 /* element matrices back to host, the fg_op_set_values_host layout (EBE/EDE
  * operators; elements whose dofs are all prescribed are not stored and read 0) */
 int fg_op_get_values_host(fg_op *op, double *vals);
+
+/* ---- edge-by-edge structure of the 4-dof (Euler) driver ------------------- */
+/* 03_Euler_Equations/2D/01_CommonFiles/ede_Initialization.c:4-190: edges
+ * numbered in order of first appearance (elements ascending; within an
+ * element the pairs (I1,I2), (I1,I3), (I2,I3) of its sorted vertices),
+ * order[nel*3] (local index of the smallest, middle, largest vertex),
+ * edge_by_elem[nel*3] (EDGE_by_Element), lm2[nedge*8] (the smaller node's 4
+ * dofs, then the other's; id -1 -> neq), edge_nodes[nedge*2].
+ * ids[nnodes*4]: Node[].id.  The edges are the items of a k = 8 operator
+ * (fg_op_ebe_create(ctx, neq, nedge, 8, lm2, ...)): edemvNDOF4.c:4-69; its
+ * node block-diagonal is fg_prec_blockdiag_create(op8, nnodes, edge_nodes,
+ * ids) (BlockDiag_precond_EDE_setup.c, EDE_matrix_Inverse as the EBE
+ * inverse, entry 14 included) and ede2mvNDOF4.c is fg_op_ebe2_create on
+ * them.  FG_ERR_ARG when an edge has more than two elements or an id lies
+ * outside [-1, neq). */
+typedef struct fg_ede4 fg_ede4;
+int fg_ede4_create(fg_ctx *ctx, int nnodes, int nel, int neq, const int *vertex, const int *ids,
+                   fg_ede4 **out);
+int fg_ede4_info(const fg_ede4 *g, int *nedge);
+int fg_ede4_get(const fg_ede4 *g, int *order, int *edge_by_elem, int *lm2, int *edge_nodes);
+/* ede_assembly.c:3-113 of the 03 driver: Aede_dev[nedge*64] (row-major 8x8
+ * edge matrices) from the element matrices A12_dev[nel*144]: the off-diagonal
+ * node blocks whole, the node-diagonal blocks times 1/2, added in element
+ * order into zeroed edge matrices (bit for bit). */
+int fg_ede4_assemble(fg_ede4 *g, const double *A12_dev, double *Aede_dev);
+int fg_ede4_destroy(fg_ede4 *g);
 
 /* ---- mesh structures (host, once per mesh) -------------------------------- */
 /* Node numbering Preprocess.c:73-81 (id = neq++ for Type>=1, else -1) and
@@ -475,16 +501,36 @@ int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem *pb, fg_op 
  *                             Block_scaling.c:10-59), no preconditioner;
  *   FG_PRED_EU_BLOCKDIAG      BlockDiag_precond_EBE_setup on R and
  *                             BlockDiag_precond inside GMRES;
- * `blocks`: a FG_PREC_BLOCKDIAG of A (NULL for FG_PRED_EU_NOT).  rot_dev,
- * theta_dev: slip-wall nodes and angles as for fg_euler2d_build. */
+ *   FG_PRED_EU_TWO            the EBE2 scheme: A = fg_op_ebe2_create(op12,
+ *                             blocks, FG_EBE2_BLOCKDIAG) (ebe2mvNDOF4),
+ *                             BlockDiag_precond_EBE_setup on R, NO_precond;
+ * `blocks`: a FG_PREC_BLOCKDIAG of the k = 12 operator (NULL for
+ * FG_PRED_EU_NOT).  rot_dev, theta_dev: slip-wall nodes and angles as for
+ * fg_euler2d_build. */
 #define FG_PRED_EU_NOT           0
 #define FG_PRED_EU_BLOCK_SCALING 1
 #define FG_PRED_EU_BLOCKDIAG     2
+#define FG_PRED_EU_TWO           3
 int fg_predictor_old_euler(fg_euler2d *m, const fg_euler2d_params *ep, fg_op *A, fg_prec *blocks,
                            int mode, const double *presc_dev, const int *rot_dev,
                            const double *theta_dev, const fg_predictor_params *p,
                            const fg_gmres_opts *gmres, double *u_dev, int *corr_hist,
                            int hist_cap, fg_predictor_info *info);
+/* The same loop over the driver's edge-by-edge scheme (Preprocess.c:163-192,
+ * setMatrixVectorProductType.c:17-22 of 03_Euler_Equations): each build's
+ * element matrices are assembled into the edges (fg_ede4_assemble) and A is
+ * the k = 8 operator on the edges' lm2 (EDE, edemvNDOF4) or, with
+ * FG_PRED_EU_TWO, fg_op_ebe2_create(op8, blocks, FG_EBE2_BLOCKDIAG) (EDE2,
+ * ede2mvNDOF4).  Modes NOT, BLOCKDIAG (BlockDiag_precond_EDE_setup on R,
+ * BlockDiag_precond in GMRES) and TWO; block scaling is EBE only
+ * (setScaling.c:10-16).  `blocks`: fg_prec_blockdiag_create(op8, nnodes,
+ * edge_nodes, ids). */
+int fg_predictor_old_euler_ede(fg_euler2d *m, const fg_euler2d_params *ep, fg_ede4 *edges,
+                               fg_op *A, fg_prec *blocks, int mode, const double *presc_dev,
+                               const int *rot_dev, const double *theta_dev,
+                               const fg_predictor_params *p, const fg_gmres_opts *gmres,
+                               double *u_dev, int *corr_hist, int hist_cap,
+                               fg_predictor_info *info);
 
 #ifdef __cplusplus
 }

@@ -147,6 +147,49 @@ class Oracle:
         L.orc_blockdiag_apply.argtypes = [_i, _i, _ip, _dp, _dp, _dp]
         L.orc_blockdiag_setup_ebe.argtypes = [_i, _i, _i, _ip, _dp, _ip, _dp, _dp, _vp]
         L.orc_block_scaling_ebe.argtypes = [_i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
+        L.orc_ebe2mv_items.argtypes = [_i, _i, _i, _dp, _ip, _i, _ip, _vp, _vp, _dp, _dp]
+        L.orc_blockdiag_setup_items.argtypes = [_i, _i, _i, _i, _ip, _dp, _ip, _dp, _dp, _vp]
+        L.orc_ede4_init.restype = _i
+        L.orc_ede4_init.argtypes = [_i, _i, _i, _ip, _ip, _ip, _ip, _vp]
+        L.orc_ede4_edge_nodes.argtypes = [_i, _ip, _ip, _ip, _ip]
+        L.orc_ede4_assembly.argtypes = [_i, _i, _dp, _ip, _ip, _dp]
+
+    def ede4_init(self, nnodes, vertex, ids, neq):
+        """ede_Initialization.c of the Euler driver: (order, edge_by_elem,
+        lm2, edge_nodes)"""
+        v = np.ascontiguousarray(vertex, np.int32)
+        ids = np.ascontiguousarray(ids, np.int32)
+        nel = v.size // 3
+        o, e = np.zeros(3 * nel, np.int32), np.zeros(3 * nel, np.int32)
+        ne = self.lib.orc_ede4_init(nnodes, nel, neq, v, ids, o, e, None)
+        lm2 = np.zeros(8 * ne, np.int32)
+        self.lib.orc_ede4_init(nnodes, nel, neq, v, ids, o, e, _ptr(lm2))
+        en = np.zeros(2 * ne, np.int32)
+        self.lib.orc_ede4_edge_nodes(nel, v, o, e, en)
+        return o, e, lm2, en
+
+    def ede4_assembly(self, nedge, A12, order, edge_by_elem):
+        Aede = np.zeros(64 * nedge)
+        self.lib.orc_ede4_assembly(len(order) // 3, nedge, np.ascontiguousarray(A12, np.float64),
+                                   order, edge_by_elem, Aede)
+        return Aede
+
+    def ebe2mv_items(self, neq, k, A, lm, nnodes, Id, P, invBD=None, BD=None):
+        Q = np.zeros(neq + 1)
+        self.lib.orc_ebe2mv_items(neq, len(lm) // k, k, A, lm, nnodes, Id,
+                                  _ptr(invBD) if invBD is not None else None,
+                                  _ptr(BD) if BD is not None else None,
+                                  np.ascontiguousarray(P, np.float64).copy(), Q)
+        return Q
+
+    def blockdiag_setup_items(self, neq, nv, nnodes, nodes, A, Id, F=None):
+        BD, iBD = np.zeros(16 * nnodes), np.zeros(16 * nnodes)
+        Fo = None if F is None else np.ascontiguousarray(F, np.float64).copy()
+        nodes = np.ascontiguousarray(nodes, np.int32)
+        self.lib.orc_blockdiag_setup_items(neq, nodes.size // nv, nv, nnodes, nodes,
+                                           np.ascontiguousarray(A, np.float64), Id, BD, iBD,
+                                           _ptr(Fo) if Fo is not None else None)
+        return BD, iBD, Fo
 
     def block_scaling_ebe(self, neq, nel, nnodes, vertex, ids, Id, A, F):
         A2 = np.ascontiguousarray(A, np.float64).copy()
@@ -406,13 +449,15 @@ class Oracle:
 
     def predictor_old_eu(self, x, y, vertex, lm, neq, ids, presc, rot, theta, gamma, mach,
                          dimless, stab_tol, u0, alpha, dt, final_time, nl_tol, nl_maxiter,
-                         stop_by_norm, steady, time_tol, block_scaling, cap=0, hist_cap=4096):
+                         stop_by_norm, steady, time_tol, block_scaling, cap=0, hist_cap=4096,
+                         scheme=0):
         """Predictor_Old over the Euler build: (u, corrections per step, steps,
-        t_final, status)"""
+        t_final, status); scheme 0 EBE, 1 EBE2, 2 EDE, 3 EDE2"""
         L = self.lib
-        L.orc_predictor_old_eu.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _dp, _vp, _vp,
-                                           _d, _d, _i, _d, C.POINTER(OrcPredParams), _i, _dp,
-                                           _dp, _ip, _i, C.POINTER(_i), C.POINTER(_d)]
+        L.orc_predictor_old_eu_scheme.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _dp, _vp,
+                                                  _vp, _d, _d, _i, _d, C.POINTER(OrcPredParams),
+                                                  _i, _i, _dp, _dp, _ip, _i, C.POINTER(_i),
+                                                  C.POINTER(_d)]
         nel = len(vertex) // 3
         pp = OrcPredParams(alpha, dt, final_time, nl_tol, time_tol, nl_maxiter,
                            int(stop_by_norm), int(steady), cap)
@@ -421,14 +466,15 @@ class Oracle:
         u = np.zeros(neq + 1)
         hist = np.zeros(hist_cap, np.int32)
         ns, tf = _i(0), _d(0.0)
-        rc = L.orc_predictor_old_eu(len(x), nel, neq, x, y, vertex, lm,
-                                    np.ascontiguousarray(ids, np.int32),
-                                    np.ascontiguousarray(presc, np.float64),
-                                    _ptr(rot_a) if rot_a is not None else None,
-                                    _ptr(th_a) if th_a is not None else None, gamma, mach,
-                                    int(dimless), stab_tol, C.byref(pp), int(block_scaling),
-                                    np.ascontiguousarray(u0, np.float64), u, hist, hist_cap,
-                                    C.byref(ns), C.byref(tf))
+        rc = L.orc_predictor_old_eu_scheme(len(x), nel, neq, x, y, vertex, lm,
+                                           np.ascontiguousarray(ids, np.int32),
+                                           np.ascontiguousarray(presc, np.float64),
+                                           _ptr(rot_a) if rot_a is not None else None,
+                                           _ptr(th_a) if th_a is not None else None, gamma, mach,
+                                           int(dimless), stab_tol, C.byref(pp),
+                                           int(block_scaling), int(scheme),
+                                           np.ascontiguousarray(u0, np.float64), u, hist,
+                                           hist_cap, C.byref(ns), C.byref(tf))
         return u, hist[:min(ns.value, hist_cap)].copy(), ns.value, tf.value, rc
 
     def ede_assembly(self, nel, nedge, ke, order, ebe):
@@ -664,11 +710,59 @@ class ReferenceEuler:
         self.lib.refeu_blockdiag_apply(np.ascontiguousarray(p, np.float64).copy(), z)
         return z
 
-    def pred_run(self, u0, alpha, dt, final_time, nl_tol, nl_maxiter, stop_by_norm, steady,
-                 time_tol, stab_tol, block_scaling, cap=0, hist_cap=4096):
-        """the reference's Predictor_Old over its Euler build (after setup and
-        build_setup): (u, corrections per step, steps, t_final, status)"""
+    # ---- the edge-by-edge scheme (after setup) ----
+    def ede_setup(self):
+        """ede_Initialization: (order, edge_by_elem, lm2)"""
         L = self.lib
+        L.refeu_ede_setup.restype = _i
+        ne = self.nedge = L.refeu_ede_setup()
+        o, e = np.zeros(3 * self.nel, np.int32), np.zeros(3 * self.nel, np.int32)
+        lm2 = np.zeros(8 * ne, np.int32)
+        L.refeu_ede_get.argtypes = [_ip, _ip, _ip]
+        L.refeu_ede_get(o, e, lm2)
+        return o, e, lm2
+
+    def ede_assemble(self, A12):
+        """setzeros + ede_assembly per element: the edge matrices [nedge*64]"""
+        L = self.lib
+        L.refeu_ede_assemble.argtypes = [_dp, _dp]
+        Aede = np.zeros(64 * self.nedge)
+        L.refeu_ede_assemble(np.ascontiguousarray(A12, np.float64), Aede)
+        return Aede
+
+    def ede_set_matrix(self, Aede):
+        self.lib.refeu_ede_set_matrix.argtypes = [_dp]
+        self.lib.refeu_ede_set_matrix(np.ascontiguousarray(Aede, np.float64))
+
+    def edemv(self, P):
+        self.lib.refeu_edemv.argtypes = [_dp, _dp]
+        Q = np.zeros(self.neq + 1)
+        self.lib.refeu_edemv(np.ascontiguousarray(P, np.float64).copy(), Q)
+        return Q
+
+    def ede_blockdiag_setup(self, F):
+        self.lib.refeu_ede_blockdiag_setup.argtypes = [_dp, _dp, _dp, _ip]
+        Fp = np.ascontiguousarray(F, np.float64).copy()
+        BD, iBD = np.zeros(16 * self.nnodes), np.zeros(16 * self.nnodes)
+        Id = np.zeros(4 * self.nnodes, np.int32)
+        self.lib.refeu_ede_blockdiag_setup(Fp, BD, iBD, Id)
+        return Fp, BD, iBD, Id
+
+    def ede2mv(self, P):
+        self.lib.refeu_ede2mv.argtypes = [_dp, _dp]
+        Q = np.zeros(self.neq + 1)
+        self.lib.refeu_ede2mv(np.ascontiguousarray(P, np.float64).copy(), Q)
+        return Q
+
+    def pred_run(self, u0, alpha, dt, final_time, nl_tol, nl_maxiter, stop_by_norm, steady,
+                 time_tol, stab_tol, block_scaling, cap=0, hist_cap=4096, scheme=0):
+        """the reference's Predictor_Old over its Euler build (after setup and
+        build_setup): (u, corrections per step, steps, t_final, status).
+        scheme: 0 EBE, 1 EBE2 (BlockDiag), 2 EDE (NOT), 3 EDE2 (BlockDiag);
+        2 and 3 after ede_setup"""
+        L = self.lib
+        L.refeu_pred_scheme.argtypes = [_i]
+        L.refeu_pred_scheme(int(scheme))
         L.refeu_pred_run.argtypes = [_dp, _d, _d, _d, _d, _i, _i, _i, _d, _d, _i, _i, _dp, _ip,
                                      _i, C.POINTER(_i), C.POINTER(_d)]
         u = np.zeros(self.neq + 1)

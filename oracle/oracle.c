@@ -1232,21 +1232,31 @@ void orc_ebe2mv_ndof4(int neq, int nel, const double *A, const int *lm, int nnod
                       const int *Id, const double *invBD, const double *BD, double *P,
                       double *Q)
 {
+    orc_ebe2mv_items(neq, nel, 12, A, lm, nnodes, Id, invBD, BD, P, Q);
+}
+
+/* the off-block product of 4-dof items of k = 8 (edges, ede2mvNDOF4.c:23-53:
+ * each node's rows against the other node's columns) or k = 12 (elements),
+ * then the node pass below */
+void orc_ebe2mv_items(int neq, int nitems, int k, const double *A, const int *lm, int nnodes,
+                      const int *Id, const double *invBD, const double *BD, double *P,
+                      double *Q)
+{
     for (int r = 0; r < neq; r++) Q[r] = 0;
     P[neq] = 0;
     Q[neq] = 0;
-    for (long e = 0; e < nel; e++) {
-        const int *le = lm + 12 * e;
-        const double *Ae = A + 144 * e;
+    for (long e = 0; e < nitems; e++) {
+        const int *le = lm + (long)k * e;
+        const double *Ae = A + (long)k * k * e;
         double p[12];
-        for (int c = 0; c < 12; c++) p[c] = P[le[c]];
-        for (int r = 0; r < 12; r++) {
+        for (int c = 0; c < k; c++) p[c] = P[le[c]];
+        for (int r = 0; r < k; r++) {
             const int nb = r / 4;     /* node block of the row: skip its columns */
             double s = 0;
             int first = 1;
-            for (int c = 0; c < 12; c++) {
+            for (int c = 0; c < k; c++) {
                 if (c / 4 == nb) continue;
-                double t = Ae[12 * r + c] * p[c];
+                double t = Ae[k * r + c] * p[c];
                 if (first) { s = t; first = 0; }
                 else s = s + t;
             }
@@ -1323,17 +1333,136 @@ void orc_blockdiag_apply(int neq, int nnodes, const int *Id, const double *invBD
 void orc_blockdiag_setup_ebe(int neq, int nel, int nnodes, const int *vertex, const double *A,
                              const int *Id, double *BD, double *invBD, double *F)
 {
+    orc_blockdiag_setup_items(neq, nel, 3, nnodes, vertex, A, Id, BD, invBD, F);
+}
+
+/* the node blocks of items of nv = 3 nodes (elements, k = 12) or nv = 2
+ * (edges, k = 8: BlockDiag_precond_EDE_setup.c:105-116, I_diag[v][4r+c] =
+ * 36 v + 8 r + c); items ascending, as both setups loop */
+void orc_blockdiag_setup_items(int neq, int nitems, int nv, int nnodes, const int *nodes,
+                               const double *A, const int *Id, double *BD, double *invBD,
+                               double *F)
+{
+    const int k = 4 * nv;
     memset(BD, 0, sizeof(double) * 16 * (size_t)nnodes);
-    for (long E = 0; E < nel; E++)
+    for (long E = 0; E < nitems; E++)
         for (int r = 0; r < 4; r++)
             for (int c = 0; c < 4; c++)
-                for (int v = 0; v < 3; v++) {
-                    const int node = vertex[3 * E + v];
-                    /* I_diag[v][4r+c] = 52 v + 12 r + c */
-                    BD[16 * node + 4 * r + c] += A[144 * E + 52 * v + 12 * r + c];
+                for (int v = 0; v < nv; v++) {
+                    const int node = nodes[(long)nv * E + v];
+                    /* I_diag[v][4r+c] = 4 (k + 1) v + k r + c */
+                    BD[16 * node + 4 * r + c] += A[(long)k * k * E + 4 * (k + 1) * v + k * r + c];
                 }
     for (int I = 0; I < nnodes; I++) orc_matrix_inverse4(BD + 16 * I, invBD + 16 * I);
     if (F) orc_blockdiag_apply(neq, nnodes, Id, invBD, F, F);
+}
+
+/* ---- the edge structure of the Euler driver ------------------------------
+ * 03_Euler_Equations/2D/01_CommonFiles/ede_Initialization.c:23-88: each
+ * element's vertices sorted (order[] = local index of the smallest, middle,
+ * largest), its edges (I1,I2), (I1,I3), (I2,I3) entered in a per-node list of
+ * the larger end, sorted ascending (ede_List_insertA, :205-241), a new entry
+ * taking the next edge number.  Here the lists are arrays of (J, number)
+ * linked by index. */
+typedef struct { int *head, *next, *J, *num, used; } orc_elist;
+
+static int orc_elist_insert(orc_elist *L, int I, int J, int *nedge)
+{
+    int prev = -1, cur = L->head[I];
+    while (cur >= 0 && L->J[cur] < J) { prev = cur; cur = L->next[cur]; }
+    if (cur >= 0 && L->J[cur] == J) return L->num[cur];
+    const int n = L->used++;
+    L->J[n] = J;
+    L->num[n] = (*nedge)++;
+    L->next[n] = cur;
+    if (prev < 0) L->head[I] = n; else L->next[prev] = n;
+    return L->num[n];
+}
+
+int orc_ede4_init(int nnodes, int nel, int neq, const int *vertex, const int *ids, int *order,
+                  int *edge_by_elem, int *lm2)
+{
+    orc_elist L;
+    const long cap = 3L * nel + 1;
+    L.head = (int *)malloc(sizeof(int) * (size_t)(nnodes > 0 ? nnodes : 1));
+    L.next = (int *)malloc(sizeof(int) * cap);
+    L.J = (int *)malloc(sizeof(int) * cap);
+    L.num = (int *)malloc(sizeof(int) * cap);
+    L.used = 0;
+    for (int i = 0; i < nnodes; i++) L.head[i] = -1;
+    int nedge = 0;
+    for (long K = 0; K < nel; K++) {
+        const int *V = vertex + 3 * K;
+        int o0, o1, o2;
+        if (V[0] < V[1] && V[0] < V[2]) { o0 = 0; o1 = V[1] < V[2] ? 1 : 2; }
+        else if (V[1] < V[2]) { o0 = 1; o1 = V[0] < V[2] ? 0 : 2; }
+        else { o0 = 2; o1 = V[0] < V[1] ? 0 : 1; }
+        o2 = 3 - o0 - o1;
+        int *o = order + 3 * K;
+        o[0] = o0; o[1] = o1; o[2] = o2;
+        const int I1 = V[o0], I2 = V[o1], I3 = V[o2];
+        /* EDGE_by_Element (:96-129): the numbers of (I1,I2), (I1,I3), (I2,I3) */
+        edge_by_elem[3 * K + 0] = orc_elist_insert(&L, I1, I2, &nedge);
+        edge_by_elem[3 * K + 1] = orc_elist_insert(&L, I1, I3, &nedge);
+        edge_by_elem[3 * K + 2] = orc_elist_insert(&L, I2, I3, &nedge);
+    }
+    /* lm2 (:131-183): the smaller node's 4 dofs, then the larger's; id -1 ->
+     * neq, written once per element that holds the edge */
+    if (lm2)
+        for (long K = 0; K < nel; K++) {
+            const int *V = vertex + 3 * K, *o = order + 3 * K, *eb = edge_by_elem + 3 * K;
+            const int Io[3] = {V[o[0]], V[o[1]], V[o[2]]};
+            static const int ends[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+            for (int s = 0; s < 3; s++)
+                for (int i = 0; i < 4; i++)
+                    for (int h = 0; h < 2; h++) {
+                        const int id = ids[4 * Io[ends[s][h]] + i];
+                        lm2[8L * eb[s] + 4 * h + i] = id == -1 ? neq : id;
+                    }
+        }
+    free(L.head); free(L.next); free(L.J); free(L.num);
+    return nedge;
+}
+
+/* BlockDiag_precond_EDE_setup.c:82-103: the two nodes of every edge
+ * (smaller first) from order and EDGE_by_Element */
+void orc_ede4_edge_nodes(int nel, const int *vertex, const int *order, const int *edge_by_elem,
+                         int *enodes)
+{
+    static const int ends[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (long E = 0; E < nel; E++) {
+        const int *V = vertex + 3 * E, *o = order + 3 * E;
+        const int Io[3] = {V[o[0]], V[o[1]], V[o[2]]};
+        for (int s = 0; s < 3; s++) {
+            enodes[2L * edge_by_elem[3 * E + s]] = Io[ends[s][0]];
+            enodes[2L * edge_by_elem[3 * E + s] + 1] = Io[ends[s][1]];
+        }
+    }
+}
+
+/* ede_assembly.c:3-113 of the Euler driver over zeroed edge matrices
+ * (setzeros.c:11-13): per element, edges (I1,I2), (I1,I3), (I2,I3) in turn,
+ * entry (r, c) of the 8x8 edge matrix += Me[row][col] of the element, times
+ * 0.5 when row and column belong to the same node */
+void orc_ede4_assembly(int nel, int nedge, const double *A12, const int *order,
+                       const int *edge_by_elem, double *Aede)
+{
+    static const int ends[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    memset(Aede, 0, sizeof(double) * 64 * (size_t)nedge);
+    for (long E = 0; E < nel; E++) {
+        const double *Me = A12 + 144 * E;
+        const int *o = order + 3 * E;
+        for (int s = 0; s < 3; s++) {
+            double *M = Aede + 64L * edge_by_elem[3 * E + s];
+            const int node[2] = {o[ends[s][0]], o[ends[s][1]]};
+            for (int r = 0; r < 8; r++)
+                for (int c = 0; c < 8; c++) {
+                    const int rn = r / 4, cn = c / 4;
+                    const double v = Me[12 * (4 * node[rn] + r % 4) + 4 * node[cn] + c % 4];
+                    M[8 * r + c] += rn == cn ? 0.5 * v : v;
+                }
+        }
+    }
 }
 
 /* Block_scaling.c:10-59 Block_scaling_EBE (2D, NDOF 4): the block-diagonal
@@ -1871,7 +2000,60 @@ int orc_predictor_old_eu(int nnodes, int nel, int neq, const double *x, const do
                          int block_scaling, const double *u0, double *u, int *hist, int hist_cap,
                          int *nsteps, double *t_final)
 {
+    return orc_predictor_old_eu_scheme(nnodes, nel, neq, x, y, vertex, lm, ids, presc, rot,
+                                       theta, gamma, mach, dimless, stab_tol, pp, block_scaling,
+                                       0, u0, u, hist, hist_cap, nsteps, t_final);
+}
+
+/* the dense matrix of a scheme's product, column j = product of e_j, then
+ * the direct solve (X = M^{-1} B, X[neq] = 0) */
+static int dense_solve_scheme(int scheme, int neq, int nel, int nedge, int nnodes,
+                              const double *A, const int *lm, const double *Aede, const int *lm2,
+                              const int *Id, const double *iBD, const double *B, double *X)
+{
+    double *M = calloc((size_t)neq * neq, sizeof(double));
+    double *e = calloc(neq + 1, sizeof(double)), *q = calloc(neq + 1, sizeof(double));
+    for (int j = 0; j < neq; j++) {
+        memset(e, 0, sizeof(double) * (neq + 1));
+        e[j] = 1.0;
+        if (scheme == 1) orc_ebe2mv_items(neq, nel, 12, A, lm, nnodes, Id, iBD, NULL, e, q);
+        else if (scheme == 2) orc_ebemv(neq, nedge, 8, Aede, lm2, e, q);
+        else orc_ebe2mv_items(neq, nedge, 8, Aede, lm2, nnodes, Id, iBD, NULL, e, q);
+        for (int i = 0; i < neq; i++) M[(size_t)i * neq + j] = q[i];
+    }
+    memcpy(X, B, sizeof(double) * neq);
+    const int rc = orc_dense_solve(neq, M, X);
+    X[neq] = 0.0;
+    free(M); free(e); free(q);
+    return rc;
+}
+
+/* scheme (setMatrixVectorProductType.c / setPreconditioner.c of the Euler
+ * driver): 0 EBE (ebemvNDOF4, NO_precond_setup; block_scaling: NO_scaling or
+ * Block_scaling_EBE), 1 EBE2 (ebe2mvNDOF4, BlockDiag_precond_EBE_setup on R),
+ * 2 EDE (ede_assembly, edemvNDOF4, NO_precond_setup), 3 EDE2 (ede_assembly,
+ * ede2mvNDOF4, BlockDiag_precond_EDE_setup on R); the solve is direct */
+int orc_predictor_old_eu_scheme(int nnodes, int nel, int neq, const double *x, const double *y,
+                                const int *vertex, const int *lm, const int *ids,
+                                const double *presc, const int *rot, const double *theta,
+                                double gamma, double mach, int dimless, double stab_tol,
+                                const orc_pred_params *pp, int block_scaling, int scheme,
+                                const double *u0, double *u, int *hist, int hist_cap,
+                                int *nsteps, double *t_final)
+{
     const double alpha = pp->alpha, dt = pp->dt;
+    int nedge = 0, *order = NULL, *ebe = NULL, *lm2 = NULL, *enodes = NULL;
+    double *Aede = NULL;
+    if (scheme >= 2) {
+        order = calloc(3 * (size_t)nel, sizeof(int));
+        ebe = calloc(3 * (size_t)nel, sizeof(int));
+        nedge = orc_ede4_init(nnodes, nel, neq, vertex, ids, order, ebe, NULL);
+        lm2 = calloc(8 * (size_t)nedge + 1, sizeof(int));
+        enodes = calloc(2 * (size_t)nedge + 1, sizeof(int));
+        Aede = calloc(64 * (size_t)nedge + 1, sizeof(double));
+        orc_ede4_init(nnodes, nel, neq, vertex, ids, order, ebe, lm2);
+        orc_ede4_edge_nodes(nel, vertex, order, ebe, enodes);
+    }
     double *a = calloc(neq + 1, sizeof(double)), *Da = calloc(neq + 1, sizeof(double));
     double *u_old = calloc(neq + 1, sizeof(double)), *R = calloc(neq + 1, sizeof(double));
     double *A = calloc((size_t)nel * 144, sizeof(double));
@@ -1907,9 +2089,17 @@ int orc_predictor_old_eu(int nnodes, int nel, int neq, const double *x, const do
                 }
             orc_build_euler2d(nnodes, nel, neq, x, y, vertex, lm, U, dU, rot, theta, gamma, mach,
                               dimless, alpha, dt, stab_tol, A, R);
-            if (block_scaling)
+            if (scheme >= 2) orc_ede4_assembly(nel, nedge, A, order, ebe, Aede);
+            if (scheme == 0 && block_scaling)
                 orc_block_scaling_ebe(neq, nel, nnodes, vertex, ids, Id, A, BD, iBD, R);
-            if (orc_dense_solve_ebe(neq, nel, 12, A, lm, NULL, R, Da) && !rc) rc = 1;
+            if (scheme == 1) orc_blockdiag_setup_ebe(neq, nel, nnodes, vertex, A, Id, BD, iBD, R);
+            if (scheme == 3)
+                orc_blockdiag_setup_items(neq, nedge, 2, nnodes, enodes, Aede, Id, BD, iBD, R);
+            const int src = scheme == 0
+                                ? orc_dense_solve_ebe(neq, nel, 12, A, lm, NULL, R, Da)
+                                : dense_solve_scheme(scheme, neq, nel, nedge, nnodes, A, lm, Aede,
+                                                     lm2, Id, iBD, R, Da);
+            if (src && !rc) rc = 1;
             orc_daxpy(neq, 1, Da, a);
             orc_daxpy(neq, alpha * dt, Da, u);
             norm_a = sqrt(orc_ddot(neq, a, a));
@@ -1934,5 +2124,6 @@ int orc_predictor_old_eu(int nnodes, int nel, int neq, const double *x, const do
     *t_final = t;
     free(a); free(Da); free(u_old); free(R); free(A); free(U); free(dU); free(BD); free(iBD);
     free(Id); free(diff);
+    free(order); free(ebe); free(lm2); free(enodes); free(Aede);
     return rc;
 }

@@ -193,6 +193,10 @@ void orc_ilu0_apply_f32(int neq, const float *LU, const int *JA, const int *IA, 
 void orc_ebe2mv_ndof4(int neq, int nel, const double *A, const int *lm, int nnodes,
                       const int *Id, const double *invBD, const double *BD, double *P,
                       double *Q);
+/* the same over items of k = 8 (edges: EDE/ede2mvNDOF4.c:4-61) or 12 */
+void orc_ebe2mv_items(int neq, int nitems, int k, const double *A, const int *lm, int nnodes,
+                      const int *Id, const double *invBD, const double *BD, double *P,
+                      double *Q);
 /* BlockDiag_precond_EBE_setup.c:88-145 matrix_Inverse (row-major 4x4) */
 void orc_matrix_inverse4(const double *B, double *inv);
 /* BlockDiag_precond.c:3-27 */
@@ -201,6 +205,21 @@ void orc_blockdiag_apply(int neq, int nnodes, const int *Id, const double *invBD
 /* BlockDiag_precond_EBE_setup.c:6-86: BD/invBD [nnodes*16], F <- BD^{-1} F */
 void orc_blockdiag_setup_ebe(int neq, int nel, int nnodes, const int *vertex, const double *A,
                              const int *Id, double *BD, double *invBD, double *F);
+/* node blocks of items of nv nodes (nv = 3: the above; nv = 2: the edges,
+ * BlockDiag_precond_EDE_setup.c:6-128, EDE_matrix_Inverse = matrix_Inverse) */
+void orc_blockdiag_setup_items(int neq, int nitems, int nv, int nnodes, const int *nodes,
+                               const double *A, const int *Id, double *BD, double *invBD,
+                               double *F);
+/* 03_Euler_Equations/2D/01_CommonFiles/ede_Initialization.c:4-241: returns
+ * nedge; order, edge_by_elem [nel*3]; lm2 [nedge*8] (NULL: count only) */
+int orc_ede4_init(int nnodes, int nel, int neq, const int *vertex, const int *ids, int *order,
+                  int *edge_by_elem, int *lm2);
+/* BlockDiag_precond_EDE_setup.c:82-103: enodes [nedge*2] */
+void orc_ede4_edge_nodes(int nel, const int *vertex, const int *order, const int *edge_by_elem,
+                         int *enodes);
+/* 03_Euler_Equations/2D/01_CommonFiles/ede_assembly.c:3-113: Aede [nedge*64] */
+void orc_ede4_assembly(int nel, int nedge, const double *A12, const int *order,
+                       const int *edge_by_elem, double *Aede);
 /* 03_Euler_Equations/2D/01_CommonFiles/Build_M_K_F_SUPG.c:6-1073 with the
  * CAU operator (Delta_CAU.c:4-78), the flux Jacobians of
  * Ax_Ay_calculations.c (dimless = 1: the dimensionless form with reference
@@ -272,6 +291,15 @@ int orc_predictor_old_eu(int nnodes, int nel, int neq, const double *x, const do
                          int dimless, double stab_tol, const orc_pred_params *pp,
                          int block_scaling, const double *u0, double *u, int *hist, int hist_cap,
                          int *nsteps, double *t_final);
+/* the same over the driver's other schemes: 0 EBE, 1 EBE2, 2 EDE, 3 EDE2
+ * (the solve: the direct solve of the scheme's product matrix) */
+int orc_predictor_old_eu_scheme(int nnodes, int nel, int neq, const double *x, const double *y,
+                                const int *vertex, const int *lm, const int *ids,
+                                const double *presc, const int *rot, const double *theta,
+                                double gamma, double mach, int dimless, double stab_tol,
+                                const orc_pred_params *pp, int block_scaling, int scheme,
+                                const double *u0, double *u, int *hist, int hist_cap,
+                                int *nsteps, double *t_final);
 
 #ifdef __cplusplus
 }

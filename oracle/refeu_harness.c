@@ -81,6 +81,80 @@ void refeu_block_scaling(double *F, double *A)
     memcpy(A, ME.Aaux, sizeof(double) * 144 * (size_t)PE.nel);
 }
 
+/* ---- the edge-by-edge scheme of the Euler driver: ede_Initialization.c,
+ * ede_assembly.c, EDE/edemvNDOF4.c, EDE/ede2mvNDOF4.c and
+ * BlockDiag_precond_EDE_setup.c, laid out as Preprocess.c:163-192 does
+ * (A: nedge+1 rows of 64).  Call after refeu_setup (mesh and ids). */
+int ede2mvNDOF4(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
+int edemvNDOF4(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
+int BlockDiag_precond_EDE_setup(ParametersType *, MatrixDataType *, FemStructsType *, int,
+                                double *);
+
+static ParametersType PD;
+static MatrixDataType MD;
+static FemStructsType FD;
+
+int refeu_ede_setup(void)
+{
+    PD = PE;
+    memset(&MD, 0, sizeof MD);
+    FD = FE;
+    int **order = (int **)calloc(PE.nel, sizeof(int *));
+    for (int e = 0; e < PE.nel; e++) order[e] = (int *)calloc(3, sizeof(int));
+    int **lm2 = NULL, *lm2aux = NULL, **ebe = NULL;
+    ede_Initialization(&PD, FE.Node, FE.Element, order, &lm2, &lm2aux, &ebe);
+    double **M = (double **)calloc(PD.nedge + 1, sizeof(double *));
+    double *Maux = (double *)calloc((size_t)(PD.nedge + 1) * 64, sizeof(double));
+    for (int i = 0; i < PD.nedge + 1; i++) M[i] = &Maux[64 * i];
+    FD.lm2 = lm2;
+    FD.lm2aux = lm2aux;
+    MD.A = M;
+    MD.Aaux = Maux;
+    MD.Scheme_by_Element = ebe;
+    MD.order = order;
+    return PD.nedge;
+}
+
+void refeu_ede_get(int *order, int *edge_by_elem, int *lm2)
+{
+    for (int e = 0; e < PD.nel; e++)
+        for (int j = 0; j < 3; j++) {
+            order[3 * e + j] = MD.order[e][j];
+            edge_by_elem[3 * e + j] = MD.Scheme_by_Element[e][j];
+        }
+    memcpy(lm2, FD.lm2aux, sizeof(int) * 8 * (size_t)PD.nedge);
+}
+
+/* setzeros.c:11-13, then ede_assembly per element; A12 [nel*144] */
+void refeu_ede_assemble(const double *A12, double *Aede)
+{
+    memset(MD.Aaux, 0, sizeof(double) * 64 * (size_t)(PD.nedge + 1));
+    for (int e = 0; e < PD.nel; e++)
+        ede_assembly(&PD, &MD, &FD, e, (double(*)[12])(A12 + 144 * (size_t)e));
+    memcpy(Aede, MD.Aaux, sizeof(double) * 64 * (size_t)PD.nedge);
+}
+
+void refeu_ede_set_matrix(const double *Aede)
+{
+    memcpy(MD.Aaux, Aede, sizeof(double) * 64 * (size_t)PD.nedge);
+}
+
+void refeu_edemv(double *P, double *Q) { edemvNDOF4(&PD, &MD, &FD, P, Q); }
+
+/* BlockDiag_precond_EDE_setup (tag 1): BD, invBD [nnodes*16], Id, F in place */
+void refeu_ede_blockdiag_setup(double *F, double *BD, double *invBD, int *Id)
+{
+    BlockDiag_precond_EDE_setup(&PD, &MD, &FD, 1, F);
+    memcpy(BD, MD.BlockDiagAux, sizeof(double) * 16 * (size_t)PD.nnodes);
+    for (int i = 0; i < PD.nnodes; i++) {
+        memcpy(invBD + 16 * i, MD.invBlockDiag[i], sizeof(double) * 16);
+        for (int d = 0; d < 4; d++) Id[4 * i + d] = MD.Id[i][d];
+    }
+}
+
+/* ede2mvNDOF4 (after refeu_ede_blockdiag_setup) */
+void refeu_ede2mv(double *P, double *Q) { ede2mvNDOF4(&PD, &MD, &FD, P, Q); }
+
 /* ---- Build_M_K_F_SUPG.c (with Delta_CAU, Ax_Ay_calculations, rotation,
  * eval_U_dU, ebe_assembly, setzeros of the Euler driver) ------------------
  * Problem callbacks: constant gamma; prescribed values rho_p, v1_p, v2_p,
@@ -173,14 +247,40 @@ static int cbp_initial(ParametersType *P, NodeType *N, double *u)
     memcpy(u, p_u0, sizeof(double) * (P->neq + 1));
     return 0;
 }
+static int p_scheme;   /* 0 EBE, 1 EBE2, 2 EDE, 3 EDE2 */
+/* schemes other than EBE: the dense matrix of the scheme's own product
+ * (FemFunctions->mv) column by column, then the same direct solve */
+static int dense_solve_mv(ParametersType *P, MatrixDataType *M, FemStructsType *S,
+                          FemFunctionsType *F, const double *B, double *X)
+{
+    const int n = P->neq;
+    double *A = (double *)calloc((size_t)n * n, sizeof(double));
+    double *e = (double *)calloc(n + 1, sizeof(double)), *q = (double *)calloc(n + 1, sizeof(double));
+    for (int j = 0; j < n; j++) {
+        memset(e, 0, sizeof(double) * (n + 1));
+        e[j] = 1.0;
+        F->mv(P, M, S, e, q);
+        for (int i = 0; i < n; i++) A[(size_t)i * n + j] = q[i];
+    }
+    memcpy(X, B, sizeof(double) * n);
+    const int rc = orc_dense_solve(n, A, X);
+    X[n] = 0.0;
+    free(A); free(e); free(q);
+    return rc;
+}
 static int cbp_solver(ParametersType *P, MatrixDataType *M, FemStructsType *S, FemFunctionsType *F,
                       double *B, double *X)
 {
-    (void)F;
-    int rc = orc_dense_solve_ebe(P->neq, P->nel, 12, M->Aaux, S->lmaux, NULL, B, X);
+    int rc = p_scheme ? dense_solve_mv(P, M, S, F, B, X)
+                      : orc_dense_solve_ebe(P->neq, P->nel, 12, M->Aaux, S->lmaux, NULL, B, X);
     if (rc && !p_rc) p_rc = rc;
     return rc;
 }
+
+/* the scheme of the next refeu_pred_run (setMatrixVectorProductType.c /
+ * setPreconditioner.c of the Euler driver with Preconditioner BlockDiag for
+ * EBE2 and EDE2, NOT for EDE); EDE schemes need refeu_ede_setup first */
+void refeu_pred_scheme(int scheme) { p_scheme = scheme; }
 static int cbp_stop_corr(ParametersType *P, double na, double nda, int i)
 {
     int s = p_norm ? StopByNorm(P, na, nda, i) : StopByIterations(P, na, nda, i);
@@ -216,12 +316,36 @@ int refeu_pred_run(const double *u0, double alpha, double dt, double final_time,
     FB.scaling = block_scaling ? Block_scaling_EBE : NO_scaling;
     FB.unscaling = NO_unscaling;
     FB.precond_setup = NO_precond_setup;
+    ParametersType *Pp = &PE;
+    MatrixDataType *Mp = &ME;
+    FemStructsType *Sp = &FE;
+    if (p_scheme == 1) {
+        strcpy(PE.MatrixVectorProductScheme, "EBE2");
+        FB.mv = ebe2mvNDOF4;
+        FB.precond_setup = BlockDiag_precond_EBE_setup;
+    } else if (p_scheme >= 2) {
+        /* the edge state of refeu_ede_setup, with the build's fields */
+        const int ne = PD.nedge;
+        PD = PE;
+        PD.nedge = ne;
+        strcpy(PD.MatrixVectorProductScheme, p_scheme == 2 ? "EDE" : "EDE2");
+        FD.AuxBuild = FE.AuxBuild; FD.delta_old = FE.delta_old;
+        FD.u = FE.u; FD.du = FE.du; FD.F = FE.F;
+        FB.assembly = ede_assembly;
+        FB.mv = p_scheme == 2 ? edemvNDOF4 : ede2mvNDOF4;
+        if (p_scheme == 3) FB.precond_setup = BlockDiag_precond_EDE_setup;
+        Pp = &PD; Mp = &MD; Sp = &FD;
+    }
     memset(&FO, 0, sizeof FO);
     FO.Build = Build_M_K_F_SUPG;
     FO.solver = cbp_solver;
-    Predictor_Old(&PE, &ME, &FE, &FB, &FO);
-    memcpy(u, FE.u, sizeof(double) * (PE.neq + 1));
+    Predictor_Old(Pp, Mp, Sp, &FB, &FO);
+    memcpy(u, Sp->u, sizeof(double) * (PE.neq + 1));
     *nsteps = p_steps;
-    *t_final = PE.CurrentTime;
+    *t_final = Pp->CurrentTime;
+    /* back to the element scheme */
+    strcpy(PE.MatrixVectorProductScheme, "EBE");
+    FB.assembly = ebe_assembly;
+    p_scheme = 0;
     return p_rc;
 }

@@ -145,3 +145,57 @@ def test_predictor_euler_rejects_missing_blocks(ctx):
     with pytest.raises(F.FemGpuError, match="status 2"):
         F.predictor_old_euler(m, ep, op, None, F.PRED_EU_BLOCK_SCALING, presc, rot, theta, u,
                               **_params_eu(d))
+
+
+SCH_CASES = golden_files("predsch_*.npz")
+
+
+@pytest.mark.parametrize("path", SCH_CASES, ids=os.path.basename)
+def test_predictor_euler_other_schemes(ctx, path):
+    """the driver's EBE2 scheme (ebe2mvNDOF4 with the block setup on R) and
+    its edge schemes EDE (edemvNDOF4; NO_precond, and BlockDiag, which does
+    not change the answer) and EDE2 (ede2mvNDOF4) against the reference's
+    runs of each (tests/golden/predsch_*): the same steps, corrections per
+    step and final time, u within 1e-9"""
+    d = load_golden(path)
+    sch = int(d["scheme"])
+    m, op, blocks, ep, presc, rot, theta, neq = _setup_eu(ctx, d)
+    nn = int(d["nnodes"])
+    g = F.Ede4(ctx, nn, d["vertex"], d["ids"], neq)
+    _, _, lm2, en = g.arrays()
+    op8 = F.Operator.ebe(ctx, neq, lm2, 8)
+    bd8 = F.Preconditioner.blockdiag(op8, nn, en, d["ids"])
+    runs = {1: [("EBE2", F.Operator.ebe2(op, blocks, F.EBE2_BLOCKDIAG), blocks, F.PRED_EU_TWO,
+                 None)],
+            2: [("EDE", op8, None, F.PRED_EU_NOT, g),
+                ("EDE+BlockDiag", op8, bd8, F.PRED_EU_BLOCKDIAG, g)],
+            3: [("EDE2", F.Operator.ebe2(op8, bd8, F.EBE2_BLOCKDIAG), bd8, F.PRED_EU_TWO, g)]}
+    for name, A, pc, mode, edges in runs[sch]:
+        u = ctx.to_device(d["u0"])
+        info, hist = F.predictor_old_euler(m, ep, A, pc, mode, presc, rot, theta, u, tol=1e-13,
+                                           lmax=2000, max_steps=200, edges=edges,
+                                           **_params_eu(d))
+        assert info.status == F.FG_OK and info.unconverged == 0, name
+        assert info.steps == int(d["nsteps"]) and info.t == float(d["t_final"]), name
+        assert np.array_equal(hist, d["hist"]), name
+        ug = u.get()
+        assert np.max(np.abs(ug[:neq] - d["u"][:neq])) <= 1e-9 * np.max(np.abs(d["u"][:neq])), name
+        print(f"\n{os.path.basename(path)} {name}: {info.steps} steps, "
+              f"{info.corrections} corrections, {info.krylov_steps} Arnoldi steps")
+
+
+def test_predictor_euler_edge_scheme_rejects_bad_input(ctx):
+    d = load_golden(EU_CASES[0])
+    m, op, blocks, ep, presc, rot, theta, neq = _setup_eu(ctx, d)
+    nn = int(d["nnodes"])
+    g = F.Ede4(ctx, nn, d["vertex"], d["ids"], neq)
+    _, _, lm2, en = g.arrays()
+    op8 = F.Operator.ebe(ctx, neq, lm2, 8)
+    bd8 = F.Preconditioner.blockdiag(op8, nn, en, d["ids"])
+    u = ctx.to_device(d["u0"])
+    for A, pc, mode in ((op8, bd8, F.PRED_EU_BLOCK_SCALING),   # EBE only
+                        (op, blocks, F.PRED_EU_BLOCKDIAG),      # the element operator
+                        (op8, bd8, F.PRED_EU_TWO)):             # TWO needs the ede2 operator
+        with pytest.raises(F.FemGpuError, match="status 2"):
+            F.predictor_old_euler(m, ep, A, pc, mode, presc, rot, theta, u, edges=g,
+                                  **_params_eu(d))

@@ -145,3 +145,36 @@ def test_predictor_euler_block_scaling_keeps_the_solution(o):
     assert np.array_equal(d0["hist"], d1["hist"])
     neq = int(d0["neq"])
     assert np.max(np.abs(d0["u"][:neq] - d1["u"][:neq])) <= 1e-12 * np.max(np.abs(d0["u"]))
+
+
+# ---- the Euler driver's other schemes: EBE2, EDE, EDE2 ----------------------
+SCH_CASES = golden_files("predsch_*.npz")
+
+
+@pytest.mark.parametrize("path", SCH_CASES, ids=os.path.basename)
+def test_predictor_euler_schemes_golden(o, path):
+    """bitwise against the reference's Predictor_Old over its EBE2 / EDE /
+    EDE2 schemes (tests/golden/make_golden_pred_eus.py)"""
+    d = load_golden(path)
+    u, hist, ns, tf, rc = _run_eu(o, d, scheme=int(d["scheme"]))
+    assert rc == 0
+    assert ns == int(d["nsteps"]) and tf == float(d["t_final"])
+    assert np.array_equal(hist, d["hist"])
+    assert np.array_equal(u, d["u"])
+
+
+def test_predictor_euler_edge_scheme_is_the_element_system(o):
+    """the edge matrices sum to the element matrices: EDE reproduces the EBE
+    run to rounding; EBE2 and EDE2 solve the same (block-preconditioned)
+    system as each other, and it is not the EBE system where a node has both
+    free and prescribed dofs (the whole node block is inverted)"""
+    ebe = load_golden([p for p in EU_CASES if p.endswith("predeu_n6_dimless.npz")][0])
+    get = {os.path.basename(p): load_golden(p) for p in SCH_CASES}
+    neq = int(ebe["neq"])
+    ede, e2, d2 = (get[f"predsch_n6_dimless_{s}.npz"] for s in ("ede", "ebe2", "ede2"))
+    scale = np.max(np.abs(ebe["u"][:neq]))
+    assert np.max(np.abs(ede["u"][:neq] - ebe["u"][:neq])) <= 1e-12 * scale
+    assert np.max(np.abs(e2["u"][:neq] - d2["u"][:neq])) <= 1e-12 * scale
+    mixed = ((ebe["ids"].reshape(-1, 4) < 0).any(1) & (ebe["ids"].reshape(-1, 4) >= 0).any(1))
+    assert mixed.any()
+    assert np.max(np.abs(e2["u"][:neq] - ebe["u"][:neq])) > 1e-8 * scale
