@@ -443,6 +443,48 @@ def euler_leg(F, ctx, n=1024, seed=20260305):
     res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
     res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(Xs["fp32_ir"][:neq] - Xs["fp64"][:neq]) /
                                            max(np.linalg.norm(Xs["fp64"][:neq]), 1e-300))
+    # the edge-by-edge scheme (EDE): the same system as 8x8 edge matrices
+    t1 = time.perf_counter()
+    g = F.Ede4(ctx, nn, v, ids32, neq)
+    _, _, lm2, en = g.arrays()
+    Aede = ctx.empty(64 * g.nedge)
+    op8 = F.Operator.ebe(ctx, neq, lm2, 8)
+    pc8 = F.Preconditioner.blockdiag(op8, nn, en, ids32)
+    ede = {"edges": g.nedge, "setup_s": time.perf_counter() - t1,
+           "edge_matrix_bytes": 64 * 8 * g.nedge}
+    g.assemble(A, Aede)
+    ctx.sync()
+    at = []
+    for _ in range(3):
+        w0 = time.perf_counter()
+        g.assemble(A, Aede)
+        ctx.sync()
+        at.append(time.perf_counter() - w0)
+    ede["edge_assembly_ms"] = min(at) * 1e3
+    op8.set_values(Aede)
+    mv8 = {}
+    for dt in (np.float64, np.float32):
+        Pd = ctx.to_device(rng.standard_normal(neq + 1).astype(dt))
+        Qd = ctx.empty(neq + 1, dt)
+        ms = op8.time_apply(Pd, Qd, reps=20)
+        b = op8.algorithmic_bytes(dt == np.float32)
+        mv8["f32" if dt == np.float32 else "f64"] = dict(ms=ms, GBs=b / ms / 1e6,
+                                                         pct_peak=100 * b / ms / 1e6 / PEAK_GBS,
+                                                         bytes=b)
+    ede["matvec_ede8"] = mv8
+    B = ctx.to_device(Rh)
+    pc8.setup(B)
+    X = ctx.empty(neq + 1)
+    ctx.sync()
+    w0 = time.perf_counter()
+    info, _, _ = F.gmres(ctx, op8, pc8, B, X, kmax=30, lmax=2000, tol=1e-8)
+    ctx.sync()
+    ede["fp64"] = dict(seconds=time.perf_counter() - w0, steps=info.steps, status=info.status)
+    ede["rel_diff_vs_ebe_fp64"] = float(np.linalg.norm(X.get()[:neq] - Xs["fp64"][:neq]) /
+                                        max(np.linalg.norm(Xs["fp64"][:neq]), 1e-300))
+    log("euler ede", ede)
+    res["ede"] = ede
+    g.close()
     return res
 
 
