@@ -131,6 +131,8 @@ SIGNATURES = {
     "fg_predictor_old_transport": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(PredictorParams),
                                         C.POINTER(GmresOpts), _vp, _vp, _i,
                                         C.POINTER(PredictorInfo)]),
+    "fg_scaling_diag": (_i, [_vp, _i, _vp]),
+    "fg_unscaling_left": (_i, [_vp, _vp]),
     "fg_ede4_create": (_i, [_vp, _i, _i, _i, _ip, _ip, C.POINTER(_vp)]),
     "fg_ede4_info": (_i, [_vp, C.POINTER(_i)]),
     "fg_ede4_get": (_i, [_vp, _vp, _vp, _vp, _vp]),
@@ -421,6 +423,15 @@ class Preconditioner:
     def setup(self, F: DeviceArray | None):
         _check(lib.fg_prec_setup(self.h, F.ptr if F is not None else None), "fg_prec_setup")
 
+    def scaling(self, mode: int, F: DeviceArray):
+        """Left_scaling_* (SCALING_LEFT) or LeftRight_scaling_* (SCALING_LEFTRIGHT)
+        of this Diag preconditioner's operator; F scaled in place"""
+        _check(lib.fg_scaling_diag(self.h, int(mode), F.ptr), "fg_scaling_diag")
+
+    def unscaling_left(self, x: DeviceArray):
+        """Left_unscaling: x[i] *= sqrt(invDiag[i])"""
+        _check(lib.fg_unscaling_left(self.h, x.ptr), "fg_unscaling_left")
+
     def apply(self, p: DeviceArray, z: DeviceArray):
         fn = lib.fg_prec_apply_f32 if p.dtype == np.float32 else lib.fg_prec_apply_f64
         _check(fn(self.h, p.ptr, z.ptr), "fg_prec_apply")
@@ -582,6 +593,7 @@ class Ede4:
 
 
 PRED_EU_NOT, PRED_EU_BLOCK_SCALING, PRED_EU_BLOCKDIAG, PRED_EU_TWO = 0, 1, 2, 3
+SCALING_LEFT, SCALING_LEFTRIGHT = 1, 2
 
 
 def predictor_old_euler(mesh: "Euler2D", ep: "EulerParams", op: Operator,

@@ -203,6 +203,19 @@ int fg_op_ebe2_create(fg_op *ebe12, fg_prec *blockdiag, int form, fg_op **out);
  * operators; elements whose dofs are all prescribed are not stored and read 0) */
 int fg_op_get_values_host(fg_op *op, double *vals);
 
+/* ---- diagonal scalings (Solvers_and_Preconditioners/scaling.c) ------------ */
+/* Left_scaling_EBE/EDE/CSR (scaling.c:10-73) and LeftRight_scaling_EBE/EDE/
+ * CSR (:76-162), with `diag` the FG_PREC_DIAG of the operator: its setup
+ * (Diag_precond_*_setup, F *= invDiag) runs first, then the stored values
+ * are scaled in place (FP32 copy refreshed): Left: row i times invDiag[i];
+ * LeftRight: entry (i, j) times sqrt(invDiag[i] invDiag[j]) and F[i] times
+ * sqrt(Diag[i]).  F: device, neq+1.  setScaling.c pairs LeftRight with
+ * Left_unscaling (:170-180): x[i] *= sqrt(invDiag[i]), i < neq. */
+#define FG_SCALING_LEFT      1
+#define FG_SCALING_LEFTRIGHT 2
+int fg_scaling_diag(fg_prec *diag, int mode, double *F_dev);
+int fg_unscaling_left(fg_prec *diag, double *x_dev);
+
 /* ---- edge-by-edge structure of the 4-dof (Euler) driver ------------------- */
 /* 03_Euler_Equations/2D/01_CommonFiles/ede_Initialization.c:4-190: edges
  * numbered in order of first appearance (elements ascending; within an

@@ -149,10 +149,29 @@ class Oracle:
         L.orc_block_scaling_ebe.argtypes = [_i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
         L.orc_ebe2mv_items.argtypes = [_i, _i, _i, _dp, _ip, _i, _ip, _vp, _vp, _dp, _dp]
         L.orc_blockdiag_setup_items.argtypes = [_i, _i, _i, _i, _ip, _dp, _ip, _dp, _dp, _vp]
+        L.orc_scaling_items.argtypes = [_i, _i, _i, _dp, _ip, _dp, _dp, _i, _dp]
+        L.orc_scaling_csr.argtypes = [_i, _dp, _ip, _ip, _dp, _dp, _i, _dp]
+        L.orc_left_unscaling.argtypes = [_i, _dp, _dp]
         L.orc_ede4_init.restype = _i
         L.orc_ede4_init.argtypes = [_i, _i, _i, _ip, _ip, _ip, _ip, _vp]
         L.orc_ede4_edge_nodes.argtypes = [_i, _ip, _ip, _ip, _ip]
         L.orc_ede4_assembly.argtypes = [_i, _i, _dp, _ip, _ip, _dp]
+
+    def scaling_items(self, neq, k, A, lm, Diag, invDiag, leftright, F):
+        A2, F2 = np.ascontiguousarray(A, np.float64).copy(), np.ascontiguousarray(F, np.float64).copy()
+        lm = np.ascontiguousarray(lm, np.int32)
+        self.lib.orc_scaling_items(neq, lm.size // k, k, A2, lm, Diag, invDiag, int(leftright), F2)
+        return A2, F2
+
+    def scaling_csr(self, neq, AA, JA, IA, Diag, invDiag, leftright, F):
+        A2, F2 = np.ascontiguousarray(AA, np.float64).copy(), np.ascontiguousarray(F, np.float64).copy()
+        self.lib.orc_scaling_csr(neq, A2, JA, IA, Diag, invDiag, int(leftright), F2)
+        return A2, F2
+
+    def left_unscaling(self, neq, invDiag, x):
+        x = np.ascontiguousarray(x, np.float64).copy()
+        self.lib.orc_left_unscaling(neq, invDiag, x)
+        return x
 
     def ede4_init(self, nnodes, vertex, ids, neq):
         """ede_Initialization.c of the Euler driver: (order, edge_by_elem,
@@ -597,6 +616,23 @@ class Reference:
 
     def invde(self):
         out = np.zeros(3 * self.nel); self.lib.ref_get_invde(out); return out
+
+    def scaling(self, name, A, F):
+        """scaling.c Left / LeftRight on A, F: (A, F, Diag, invDiag) after"""
+        self.set_matrix(A)
+        self.lib.ref_set_F(np.ascontiguousarray(F, np.float64))
+        self.lib.ref_scaling.argtypes = [C.c_char_p]
+        self.lib.ref_scaling(name.encode())
+        A2, F2 = np.zeros(self.matrix_size()), np.zeros(self.neq + 1)
+        self.lib.ref_get_matrix(A2); self.lib.ref_get_F(F2)
+        D, iD = self.invdiag()
+        return A2, F2, D, iD
+
+    def unscaling(self, x):
+        x = np.ascontiguousarray(x, np.float64).copy()
+        self.lib.ref_unscaling.argtypes = [_dp]
+        self.lib.ref_unscaling(x)
+        return x
 
     def precond_apply(self, p):
         z = np.zeros(self.neq + 1); self.lib.ref_precond_apply(p.copy(), z); return z

@@ -2127,3 +2127,42 @@ int orc_predictor_old_eu_scheme(int nnodes, int nel, int neq, const double *x, c
     free(order); free(ebe); free(lm2); free(enodes); free(Aede);
     return rc;
 }
+
+/* ---- scaling.c:10-180: Left / LeftRight diagonal scalings -----------------
+ * Diag, invDiag [neq+1] from the Diag setup of the same matrix (which also
+ * did F *= invDiag); invDiag[neq] reads as invDiag[neq-1] (the reference
+ * writes it so, :19).  Items: A [nitems*k*k] row-major, lm [nitems*k]. */
+static double orc_inv_at(const double *iD, int g, int neq) { return iD[g < neq ? g : neq - 1]; }
+
+void orc_scaling_items(int neq, int nitems, int k, double *A, const int *lm, const double *Diag,
+                       const double *invDiag, int leftright, double *F)
+{
+    for (long E = 0; E < nitems; E++)
+        for (int I = 0; I < k; I++)
+            for (int J = 0; J < k; J++) {
+                double *a = &A[(long)k * k * E + k * I + J];
+                const double di = orc_inv_at(invDiag, lm[(long)k * E + I], neq);
+                if (leftright) *a *= sqrt(di * orc_inv_at(invDiag, lm[(long)k * E + J], neq));
+                else *a *= di;
+            }
+    if (leftright)
+        for (int I = 0; I < neq; I++) F[I] *= sqrt(Diag[I]);
+}
+
+void orc_scaling_csr(int neq, double *AA, const int *JA, const int *IA, const double *Diag,
+                     const double *invDiag, int leftright, double *F)
+{
+    for (int I = 0; I < neq; I++)
+        for (int J = IA[I]; J < IA[I + 1]; J++) {
+            if (leftright) AA[J] *= sqrt(invDiag[I] * orc_inv_at(invDiag, JA[J], neq));
+            else AA[J] *= invDiag[I];
+        }
+    if (leftright)
+        for (int I = 0; I < neq; I++) F[I] *= sqrt(Diag[I]);
+}
+
+/* Left_unscaling (:170-180) */
+void orc_left_unscaling(int neq, const double *invDiag, double *x)
+{
+    for (int I = 0; I < neq; I++) x[I] *= sqrt(invDiag[I]);
+}

@@ -304,4 +304,14 @@ int orc_predictor_old_eu_scheme(int nnodes, int nel, int neq, const double *x, c
 #ifdef __cplusplus
 }
 #endif
+/* Solvers_and_Preconditioners/scaling.c:10-180: Left (leftright = 0) and
+ * LeftRight scalings of item matrices (EBE k = 3, EDE k = 2) or CSR, after
+ * the Diag setup of the same matrix (Diag, invDiag; F already times
+ * invDiag); Left_unscaling */
+void orc_scaling_items(int neq, int nitems, int k, double *A, const int *lm, const double *Diag,
+                       const double *invDiag, int leftright, double *F);
+void orc_scaling_csr(int neq, double *AA, const int *JA, const int *IA, const double *Diag,
+                     const double *invDiag, int leftright, double *F);
+void orc_left_unscaling(int neq, const double *invDiag, double *x);
+
 #endif

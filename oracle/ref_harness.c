@@ -255,3 +255,24 @@ void ref_symrcm(int neq, int nnz, int *IA, int *JA, int *perm, int *pT)
     for (int q = 0; q < nnz; q++) pT[q] = q;
     REORDERING_SYMRCM(&P, JA, IA, perm, pT);
 }
+
+/* scaling.c:10-180 on the current matrix and F (after ref_set_matrix /
+ * ref_set_F): name "Left" or "LeftRight"; the scaled matrix is read back
+ * with ref_get_matrix, F with ref_get_F, Diag/invDiag with ref_get_invdiag */
+int Left_scaling_EBE(ParametersType *, MatrixDataType *, FemStructsType *);
+int Left_scaling_EDE(ParametersType *, MatrixDataType *, FemStructsType *);
+int Left_scaling_CSR(ParametersType *, MatrixDataType *, FemStructsType *);
+int LeftRight_scaling_EBE(ParametersType *, MatrixDataType *, FemStructsType *);
+int LeftRight_scaling_EDE(ParametersType *, MatrixDataType *, FemStructsType *);
+int LeftRight_scaling_CSR(ParametersType *, MatrixDataType *, FemStructsType *);
+int Left_unscaling(ParametersType *, MatrixDataType *, FemStructsType *, double *);
+
+void ref_scaling(const char *name)
+{
+    const char *s = P_.MatrixVectorProductScheme;
+    const int lr = strcasecmp(name, "LeftRight") == 0;
+    if (strcasecmp(s, "EBE") == 0) lr ? LeftRight_scaling_EBE(&P_, &M_, &FS_) : Left_scaling_EBE(&P_, &M_, &FS_);
+    else if (strcasecmp(s, "EDE") == 0) lr ? LeftRight_scaling_EDE(&P_, &M_, &FS_) : Left_scaling_EDE(&P_, &M_, &FS_);
+    else lr ? LeftRight_scaling_CSR(&P_, &M_, &FS_) : Left_scaling_CSR(&P_, &M_, &FS_);
+}
+void ref_unscaling(double *x) { Left_unscaling(&P_, &M_, &FS_, x); }
