@@ -131,6 +131,9 @@ SIGNATURES = {
     "fg_predictor_old_transport": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(PredictorParams),
                                         C.POINTER(GmresOpts), _vp, _vp, _i,
                                         C.POINTER(PredictorInfo)]),
+    "fg_param_file_read": (_i, [C.c_char_p, C.c_char_p, C.c_char_p, _i, C.POINTER(_i)]),
+    "fg_mesh_file_read": (_i, [C.c_char_p, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_i), _vp,
+                               _vp, _vp, _vp]),
     "fg_scaling_diag": (_i, [_vp, _i, _vp]),
     "fg_unscaling_left": (_i, [_vp, _vp]),
     "fg_ede4_create": (_i, [_vp, _i, _i, _i, _ip, _ip, C.POINTER(_vp)]),
@@ -897,3 +900,86 @@ class Euler2D:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# on-disk formats (host): parameter and mesh files of the drivers
+# ---------------------------------------------------------------------------
+# field order and kind of each driver's parameter file (Preprocess.c)
+PARAM_FIELDS = {
+    # 01_SS_Transport_Equation/2D/01_CommonFiles/Preprocess.c:44-61
+    "ss2d": [("Experiments", "s"), ("ProblemTitle", "s"), ("SolverTolerance", "d"),
+             ("NonLinearTolerance", "d"), ("SolverMaxIter", "i"),
+             ("KrylovBasisVectorsQuantity", "i"), ("Solver", "s"), ("Preconditioner", "s"),
+             ("Scaling", "s"), ("reordering", "s"), ("MatrixVectorProductScheme", "s"),
+             ("StabilizationForm", "s"), ("ShockCapture", "s"), ("h_Shock", "s"),
+             ("OutputFlow", "s"), ("nnodes", "i"), ("nel", "i")],
+    # 03_Euler_Equations/2D/01_CommonFiles/Preprocess.c:45-74
+    "euler2d": [("Experiments", "s"), ("ProblemTitle", "s"), ("SolverTolerance", "d"),
+                ("NonLinearTolerance", "d"), ("TimeIntegrationTolerance", "d"),
+                ("StabilizationTolerance", "d"), ("LinearMaxIter", "i"),
+                ("KrylovBasisVectorsQuantity", "i"), ("Solver", "s"), ("Preconditioner", "p"),
+                ("Scaling", "s"), ("reordering", "s"), ("TimeIntegration", "s"),
+                ("StopAtSteadyState", "s"), ("Alpha", "d"), ("DeltaT", "d"),
+                ("FinalTime", "d"), ("Dimensionless", "s"), ("Mach", "d"),
+                ("NonLinearMaxIter", "i"), ("StopMulticorrection", "s"),
+                ("MatrixVectorProductScheme", "s"), ("StabilizationForm", "s"),
+                ("ShockCapture", "s"), ("invY0", "d"), ("invY1", "d"), ("invY2", "d"),
+                ("invY3", "d"), ("nnodes", "i"), ("nel", "i")],
+    # 01_SS_Transport_Equation/3D/01_CommonFiles/Preprocess.c:45-83
+    "ss3d": [("Experiments", "s"), ("ProblemTitle", "s"), ("NumberMesh", "i"),
+             ("OriginMesh", "s"), ("TypeMesh", "s"), ("ConstApli", "d"),
+             ("ExactSolution", "s"), ("InitialSolution", "s"), ("Solver", "s"),
+             ("KrylovBasisVectorsQuantity", "i"), ("SolverMaxIter", "i"),
+             ("SolverTolerance", "d"), ("Preconditioner", "s"), ("Reordering", "s"),
+             ("MatrixVectorProductScheme", "s"), ("StabilizationForm", "s"),
+             ("UseDamping", "s"), ("ComputeResidual", "s"), ("UsePeclet", "s"),
+             ("TauMacro", "s"), ("TauMicro", "s"), ("tolGradU", "d"), ("wMacro", "d"),
+             ("wMicro", "d"), ("tetha", "d"), ("ErrorType", "s"), ("NonLinearTolerance", "d"),
+             ("NonLinearMaxIter", "i"), ("TipoH", "i"), ("OutputFlow", "s"), ("Vmax", "d"),
+             ("xInf", "d"), ("xSup", "d"), ("yInf", "d"), ("ySup", "d"), ("zInf", "d"),
+             ("zSup", "d"), ("nnodes", "i"), ("nel", "i")],
+}
+
+# (ndim, node integers, vertices per element, element integers) of each mesh format
+MESH_FORMATS = {"ss2d": (2, 1, 3, 1), "euler2d": (2, 4, 3, 1), "ss3d": (3, 1, 4, 1)}
+
+
+def read_param_file(path, driver):
+    """the driver's parameter file as {field: value} (typed as Preprocess.c reads it)"""
+    fields = PARAM_FIELDS[driver]
+    kinds = "".join(k for _, k in fields)
+    buf = C.create_string_buffer(1 << 16)
+    n = _i()
+    _check(lib.fg_param_file_read(os.fsencode(path), kinds.encode(), buf, len(buf), C.byref(n)),
+           "fg_param_file_read")
+    raw = buf.raw.split(b"\0")[:n.value]
+    out = {}
+    for (name, kind), v in zip(fields, raw):
+        v = v.decode()
+        out[name] = float(v) if kind == "d" else int(v) if kind == "i" else v
+    return out
+
+
+def read_mesh_file(path, driver, one_based=False):
+    """(coords [nnodes, ndim], node integers [nnodes, nint], vertex [nel*nvert],
+    element integers [nel, ncols]) of a driver's mesh .dat file; one_based:
+    the vertices are numbered from 1 (the 3D driver's GMSH meshes,
+    Preprocess.c:189-193) and are returned from 0"""
+    ndim, nint, nvert, ecols = MESH_FORMATS[driver]
+    nn, ne = _i(), _i()
+    p = os.fsencode(path)
+    _check(lib.fg_mesh_file_read(p, ndim, nint, nvert, ecols, C.byref(nn), C.byref(ne), None,
+                                 None, None, None), "fg_mesh_file_read")
+    xyz = np.zeros((nn.value, ndim))
+    ni = np.zeros((nn.value, nint), np.int32)
+    v = np.zeros(ne.value * nvert, np.int32)
+    ei = np.zeros((ne.value, ecols), np.int32)
+    _check(lib.fg_mesh_file_read(p, ndim, nint, nvert, ecols, C.byref(nn), C.byref(ne),
+                                 xyz.ctypes.data, ni.ctypes.data, v.ctypes.data,
+                                 ei.ctypes.data), "fg_mesh_file_read")
+    if one_based:
+        v -= 1
+    if v.size and (v.min() < 0 or v.max() >= nn.value):
+        raise ValueError(f"{path}: element vertex outside [0, {nn.value})")
+    return xyz, ni, v, ei

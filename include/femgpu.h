@@ -203,6 +203,24 @@ int fg_op_ebe2_create(fg_op *ebe12, fg_prec *blockdiag, int form, fg_op **out);
  * operators; elements whose dofs are all prescribed are not stored and read 0) */
 int fg_op_get_values_host(fg_op *op, double *vals);
 
+/* ---- on-disk formats (host) ------------------------------------------------ */
+/* Parameter file: the value of each of the first strlen(kinds) fields, in
+ * order, as NUL-separated strings in values[cap]; kinds: one char per field,
+ * 's', 'd', 'i' (the "%s|%lf|%d\t:%[^\n]" lines of each driver's
+ * Preprocess.c, e.g. 01_SS_Transport_Equation/2D/01_CommonFiles/
+ * Preprocess.c:44-61) or 'p' (value up to the first tab: the Euler
+ * driver's Preconditioner line, 03 Preprocess.c:54).  FG_ERR_ARG on a short
+ * or unreadable file. */
+int fg_param_file_read(const char *path, const char *kinds, char *values, int cap, int *count);
+/* Mesh file <title>_<nnodes>_<nel>.dat (Preprocess.c:71-99 of the 2D
+ * scalar drivers; 03 Preprocess.c:80-100; 3D Preprocess.c:132-150): nnodes;
+ * per node ndim doubles and nint integers; nel; per element nvert vertices
+ * and etype_cols integers.  Call with xyz == NULL for the sizes, then with
+ * arrays of those sizes (node_ints / vertex / elem_ints may be NULL). */
+int fg_mesh_file_read(const char *path, int ndim, int nint, int nvert, int etype_cols,
+                      int *nnodes, int *nel, double *xyz, int *node_ints, int *vertex,
+                      int *elem_ints);
+
 /* ---- diagonal scalings (Solvers_and_Preconditioners/scaling.c) ------------ */
 /* Left_scaling_EBE/EDE/CSR (scaling.c:10-73) and LeftRight_scaling_EBE/EDE/
  * CSR (:76-162), with `diag` the FG_PREC_DIAG of the operator: its setup

@@ -157,6 +157,34 @@ class Oracle:
         L.orc_ede4_edge_nodes.argtypes = [_i, _ip, _ip, _ip, _ip]
         L.orc_ede4_assembly.argtypes = [_i, _i, _dp, _ip, _ip, _dp]
 
+    def read_params(self, path, kinds):
+        """values of a parameter file read with the drivers' fscanf formats
+        ('d' as %.17g text)"""
+        L = self.lib
+        L.orc_read_params.restype = _i
+        L.orc_read_params.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, _i]
+        buf = C.create_string_buffer(1 << 16)
+        n = L.orc_read_params(os.fsencode(path), kinds.encode(), buf, len(buf))
+        return [v.decode() for v in buf.raw.split(b"\0")[:max(n, 0)]]
+
+    def read_mesh(self, path, ndim, nint, nvert, ecols):
+        L = self.lib
+        L.orc_read_mesh.restype = _i
+        L.orc_read_mesh.argtypes = [C.c_char_p, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_i),
+                                    _vp, _vp, _vp, _vp]
+        nn, ne = _i(), _i()
+        p = os.fsencode(path)
+        assert L.orc_read_mesh(p, ndim, nint, nvert, ecols, C.byref(nn), C.byref(ne), None, None,
+                               None, None) == 0
+        xyz = np.zeros((nn.value, ndim))
+        ni = np.zeros((nn.value, nint), np.int32)
+        v = np.zeros(ne.value * nvert, np.int32)
+        ei = np.zeros((ne.value, ecols), np.int32)
+        assert L.orc_read_mesh(p, ndim, nint, nvert, ecols, C.byref(nn), C.byref(ne),
+                               xyz.ctypes.data, ni.ctypes.data, v.ctypes.data,
+                               ei.ctypes.data) == 0
+        return xyz, ni, v, ei
+
     def scaling_items(self, neq, k, A, lm, Diag, invDiag, leftright, F):
         A2, F2 = np.ascontiguousarray(A, np.float64).copy(), np.ascontiguousarray(F, np.float64).copy()
         lm = np.ascontiguousarray(lm, np.int32)

@@ -21,6 +21,7 @@
  */
 #include "oracle.h"
 #include "dense_solve.h"
+#include <stdio.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -2165,4 +2166,80 @@ void orc_scaling_csr(int neq, double *AA, const int *JA, const int *IA, const do
 void orc_left_unscaling(int neq, const double *invDiag, double *x)
 {
     for (int I = 0; I < neq; I++) x[I] *= sqrt(invDiag[I]);
+}
+
+/* ---- the drivers' file reads, as Preprocess.c does them with fscanf ------
+ * Parameter lines: 's' "%s\t:%[^\n]", 'd' "%lf\t:%[^\n]", 'i' "%d\t:%[^\n]"
+ * (01 2D Preprocess.c:44-61, 3D :45-83, 03 :45-74), 'p' "\n%[^\t]%[^\n]\n"
+ * (03 Preprocess.c:54).  Each value is written back as text ('d' with %.17g)
+ * into out, NUL-separated.  Returns the number of fields read. */
+int orc_read_params(const char *path, const char *kinds, char *out, int cap)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return -1;
+    char sval[4096], label[4096];
+    int n = 0, used = 0;
+    for (const char *k = kinds; *k; k++) {
+        char txt[4200];
+        int ok;
+        if (*k == 'd') {
+            double v;
+            ok = fscanf(f, "%lf\t:%[^\n]", &v, label) >= 1;
+            snprintf(txt, sizeof txt, "%.17g", v);
+        } else if (*k == 'i') {
+            int v;
+            ok = fscanf(f, "%d\t:%[^\n]", &v, label) >= 1;
+            snprintf(txt, sizeof txt, "%d", v);
+        } else if (*k == 'p') {
+            ok = fscanf(f, "\n%[^\t]%[^\n]\n", sval, label) >= 1;
+            snprintf(txt, sizeof txt, "%s", sval);
+        } else {
+            ok = fscanf(f, "%s\t:%[^\n]", sval, label) >= 1;
+            snprintf(txt, sizeof txt, "%s", sval);
+        }
+        if (!ok) break;
+        const int len = (int)strlen(txt);
+        if (used + len + 1 > cap) break;
+        memcpy(out + used, txt, (size_t)len + 1);
+        used += len + 1;
+        n++;
+    }
+    fclose(f);
+    return n;
+}
+
+/* mesh .dat: "%d" nnodes, per node ndim "%lf" and nint "%d", "%d" nel, per
+ * element nvert + ecols "%d" (01 2D Preprocess.c:71-99, 03 :80-100) */
+int orc_read_mesh(const char *path, int ndim, int nint, int nvert, int ecols, int *nnodes,
+                  int *nel, double *xyz, int *ni, int *vertex, int *ei)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return -1;
+    int nn = 0, ne = 0;
+    if (fscanf(f, "%d", &nn) != 1) { fclose(f); return -1; }
+    for (long i = 0; i < nn; i++) {
+        for (int d = 0; d < ndim; d++) {
+            double v = 0;
+            if (fscanf(f, "%lf", &v) != 1) { fclose(f); return -1; }
+            if (xyz) xyz[i * ndim + d] = v;
+        }
+        for (int d = 0; d < nint; d++) {
+            int v = 0;
+            if (fscanf(f, "%d", &v) != 1) { fclose(f); return -1; }
+            if (ni) ni[i * nint + d] = v;
+        }
+    }
+    if (fscanf(f, "%d", &ne) != 1) { fclose(f); return -1; }
+    for (long e = 0; e < ne; e++) {
+        for (int d = 0; d < nvert + ecols; d++) {
+            int v = 0;
+            if (fscanf(f, "%d", &v) != 1) { fclose(f); return -1; }
+            if (d < nvert) { if (vertex) vertex[e * nvert + d] = v; }
+            else if (ei) ei[e * ecols + d - nvert] = v;
+        }
+    }
+    fclose(f);
+    *nnodes = nn;
+    *nel = ne;
+    return 0;
 }

@@ -314,4 +314,9 @@ void orc_scaling_csr(int neq, double *AA, const int *JA, const int *IA, const do
                      const double *invDiag, int leftright, double *F);
 void orc_left_unscaling(int neq, const double *invDiag, double *x);
 
+/* the drivers' parameter and mesh file reads, with fscanf as Preprocess.c */
+int orc_read_params(const char *path, const char *kinds, char *out, int cap);
+int orc_read_mesh(const char *path, int ndim, int nint, int nvert, int ecols, int *nnodes,
+                  int *nel, double *xyz, int *ni, int *vertex, int *ei);
+
 #endif
