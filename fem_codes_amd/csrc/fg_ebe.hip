@@ -504,8 +504,8 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
     op->nhalo = (int)pl.hdofs.size();
     op->nslots = pl.nslots;
     op->nzero = (int)pl.zdofs.size();
-    op->smem64 = fg_ebe_smem_bytes<double>(k, pl.max_nloc, pl.max_ne, pl.max_ninc);
-    op->smem32 = fg_ebe_smem_bytes<float>(k, pl.max_nloc, pl.max_ne, pl.max_ninc);
+    op->smem64 = fg_ebe_smem_bytes<double>(k, ept, pl.max_nloc, pl.max_ne, pl.max_ninc);
+    op->smem32 = fg_ebe_smem_bytes<float>(k, ept, pl.max_nloc, pl.max_ne, pl.max_ninc);
     if (op->smem64 > 64 * 1024) {
         size_t shm = op->smem64;
         delete op;

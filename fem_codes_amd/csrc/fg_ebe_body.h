@@ -66,10 +66,16 @@ __host__ __device__ constexpr size_t fg_ebe_smem_idx_bytes(int nd, int ninc)
 {
     return (2 * ((size_t)ninc + nd + 1) + 15) / 16 * 16;
 }
+// k >= 8 with one element per thread: the row sums take the LDS of the P
+// values (each thread holds its element's P values in registers across a
+// barrier), so a block needs max(nd + 1, k ne) values instead of the sum
+__host__ __device__ constexpr bool fg_ebe_alias(int k, int ept) { return k >= 8 && ept == 1; }
 template <typename T>
-__host__ __device__ constexpr size_t fg_ebe_smem_bytes(int k, int nd, int ne, int ninc)
+__host__ __device__ constexpr size_t fg_ebe_smem_bytes(int k, int ept, int nd, int ne, int ninc)
 {
-    return fg_ebe_smem_idx_bytes(nd, ninc) + sizeof(T) * ((size_t)nd + 1 + (size_t)k * ne);
+    const size_t np = (size_t)nd + 1, ny = (size_t)k * ne;
+    return fg_ebe_smem_idx_bytes(nd, ninc) +
+           sizeof(T) * (fg_ebe_alias(k, ept) ? (np > ny ? np : ny) : np + ny);
 }
 
 // x[i] through a block-uniform base (scalar registers) and a 32-bit byte
@@ -160,7 +166,7 @@ __device__ __forceinline__ void fg_ebe_load(int b, const FgEbeLayout &L, const T
 }
 
 // LDS carve-up of a block: P values, element row sums, incidence lists
-template <int K, typename T>
+template <int K, int EPT, typename T>
 struct FgEbeSmem {
     T *sP, *sY;
     uint16_t *sInc, *sIoff;
@@ -169,7 +175,7 @@ struct FgEbeSmem {
         sInc = reinterpret_cast<uint16_t *>(smem);
         sIoff = sInc + ninc;
         sP = reinterpret_cast<T *>(smem + fg_ebe_smem_idx_bytes(nd, ninc));
-        sY = sP + nd + 1;
+        sY = fg_ebe_alias(K, EPT) ? sP : sP + nd + 1;
         (void)ne;
     }
 };
@@ -182,7 +188,7 @@ __device__ __forceinline__ void fg_ebe_gather(int b, const FgEbeLayout &L,
     const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
     const int e0 = L.eoff[b], ne = L.eoff[b + 1] - e0;
     const int i0 = L.ibase[b], ninc = L.ibase[b + 1] - i0;
-    FgEbeSmem<K, T> S(smem, nd, ne, ninc);
+    FgEbeSmem<K, EPT, T> S(smem, nd, ne, ninc);
     const uint16_t *io = L.ioff + d0 + b;
     const int *Db = L.dofs + d0, *Sb = L.slots + d0;
     // incidence lists: 16-byte aligned, padded to 8 entries (see the plan)
@@ -228,16 +234,21 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
     const unsigned ne = (unsigned)(L.eoff[b + 1] - e0);
     const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
     const int ninc = L.ibase[b + 1] - L.ibase[b];
-    FgEbeSmem<K, T> S(smem, nd, ne, ninc);
+    FgEbeSmem<K, EPT, T> S(smem, nd, ne, ninc);
     const T *Kb = Kv + (size_t)K * K * e0;
     const int *Db = L.dofs + d0, *Sb = L.slots + d0;
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
         const unsigned el = threadIdx.x + i * FG_TPB;
+        T p[K];
         if (el < ne) {
-            T p[K];
 #pragma unroll
             for (int j = 0; j < K; j++) p[j] = S.sP[r.li[i][j]];
+        }
+        // aliased LDS (EPT == 1, one pass of this loop): every thread has read
+        // its P values before any row sum overwrites them
+        if constexpr (fg_ebe_alias(K, EPT)) __syncthreads();
+        if (el < ne) {
 #pragma unroll
             for (int rr = 0; rr < K; rr++) {
                 const int c0 = fg_ebe_c0<BS>(rr);
