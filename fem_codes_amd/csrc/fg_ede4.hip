@@ -11,7 +11,7 @@
 // element order into zeroed edge matrices.
 //
 // Host: the numbering (an ordered map per node, the reference's sorted edge
-// lists).  Device: the assembly, one thread per (edge, row); an interior
+// lists).  Device: the assembly, one thread per (edge, row, node block); an interior
 // edge has exactly two elements, and the thread adds the first one's
 // contribution and then the second's, as the reference's element loop does,
 // so the result is bitwise the reference's without atomics.
@@ -33,17 +33,18 @@ struct fg_ede4 {
 namespace {
 
 // the 8x8 contribution of element e's edge `slot` (local vertex order o):
-// row r of the edge = (node a or b, dof i); Me is row-major 12x12
+// row r of the edge = (node a or b, dof r % 4); thread (J, r, hb) writes the
+// four entries of row r in column block hb (node a: hb = 0, node b: 1), from
+// four contiguous entries of a row of each element's row-major 12x12 matrix
 __global__ void k_ede4_assemble(int nedge, const int *__restrict__ order,
                                 const int *__restrict__ einc, const double *__restrict__ A12,
                                 double *__restrict__ Aede)
 {
     const long long t = (long long)blockIdx.x * FG_TPB + threadIdx.x;
-    if (t >= 8LL * nedge) return;
-    const int J = (int)(t >> 3), r = (int)(t & 7);
-    double m[8];
-#pragma unroll
-    for (int c = 0; c < 8; c++) m[c] = 0.0;
+    if (t >= 16LL * nedge) return;
+    const int J = (int)(t >> 4), r = (int)((t >> 1) & 7), hb = (int)(t & 1);
+    const bool diag = (r < 4) == (hb == 0);   // node-diagonal block: halved
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
     for (int h = 0; h < 2; h++) {
         const int q = einc[2 * J + h];
         if (q < 0) break;
@@ -52,25 +53,17 @@ __global__ void k_ede4_assemble(int nedge, const int *__restrict__ order,
         // edges: (I1,I2) = slot 0, (I1,I3) = slot 1, (I2,I3) = slot 2
         const int la = slot == 2 ? o[1] : o[0];
         const int lb = slot == 0 ? o[1] : o[2];
-        const double *Me = A12 + 144LL * e;
-        const int row = r < 4 ? 4 * la + r : 4 * lb + (r - 4);
-        const int own = r < 4 ? 4 * la : 4 * lb;      // node-diagonal block: halved
-        const int other = r < 4 ? 4 * lb : 4 * la;
+        const int row = 4 * (r < 4 ? la : lb) + (r & 3);
+        const double *src = A12 + 144LL * e + 12 * row + 4 * (hb ? lb : la);
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            const double d = 0.5 * Me[12 * row + own + c];
-            const double f = Me[12 * row + other + c];
-            if (r < 4) {
-                m[c] += d;
-                m[4 + c] += f;
-            } else {
-                m[c] += f;
-                m[4 + c] += d;
-            }
+            const double v = src[c];
+            m[c] += diag ? 0.5 * v : v;
         }
     }
+    double *dst = Aede + 64LL * J + 8 * r + 4 * hb;
 #pragma unroll
-    for (int c = 0; c < 8; c++) Aede[64LL * J + 8 * r + c] = m[c];
+    for (int c = 0; c < 4; c++) dst[c] = m[c];
 }
 
 }  // namespace
@@ -196,7 +189,7 @@ extern "C" int fg_ede4_assemble(fg_ede4 *g, const double *A12_dev, double *Aede_
     FG_ARG(g && A12_dev && Aede_dev, "ede4/A12/Aede");
     fg_ctx *c = g->ctx;
     if (g->nedge == 0) return FG_OK;
-    const long long n = 8LL * g->nedge;
+    const long long n = 16LL * g->nedge;
     const int grid = (int)((n + FG_TPB - 1) / FG_TPB);
     FG_LAUNCH(k_ede4_assemble, grid, FG_TPB, 0, c->stream, g->nedge, g->d_order, g->d_einc,
               A12_dev, Aede_dev);
