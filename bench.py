@@ -521,7 +521,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+              "(one GPU per process; the sharding over several GPUs is a design only)", file=sys.stderr)
         sys.exit(2)
 
     import fem_codes_amd as F

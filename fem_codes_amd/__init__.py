@@ -87,6 +87,7 @@ SIGNATURES = {
     "fg_ctx_create": (_i, [_i, C.POINTER(_vp)]),
     "fg_ctx_destroy": (_i, [_vp]),
     "fg_ctx_sync": (_i, [_vp]),
+    "fg_ctx_trim": (_i, [_vp]),
     "fg_malloc": (_i, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "fg_free": (_i, [_vp, _vp]),
     "fg_memcpy_h2d": (_i, [_vp, _vp, _vp, C.c_size_t]),
@@ -222,6 +223,10 @@ class Context:
 
     def sync(self):
         _check(lib.fg_ctx_sync(self.h), "fg_ctx_sync")
+
+    def trim(self):
+        """free the solver workspace kept between solves (fg_ctx_trim)"""
+        _check(lib.fg_ctx_trim(self.h), "fg_ctx_trim")
 
     def close(self):
         if self.h:

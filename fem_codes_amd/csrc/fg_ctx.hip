@@ -49,10 +49,37 @@ extern "C" int fg_ctx_destroy(fg_ctx *c)
     hipStreamSynchronize(c->stream);
     hipFree(c->red_part);
     hipFree(c->red_ticket);
+    hipFree(c->ws);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
     hipStreamDestroy(c->stream);
     delete c;
+    return FG_OK;
+}
+
+int fg_ctx_workspace(fg_ctx *c, size_t bytes, void **p)
+{
+    if (bytes == 0) bytes = 256;
+    if (bytes > c->ws_bytes) {
+        FG_HIP(hipStreamSynchronize(c->stream));
+        if (c->ws) FG_HIP(hipFree(c->ws));
+        c->ws = nullptr;
+        c->ws_bytes = 0;
+        FG_HIP(hipMalloc(&c->ws, bytes));
+        c->ws_bytes = bytes;
+    }
+    *p = c->ws;
+    return FG_OK;
+}
+
+extern "C" int fg_ctx_trim(fg_ctx *c)
+{
+    FG_ARG(c, "ctx");
+    FG_HIP(hipSetDevice(c->device));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    if (c->ws) FG_HIP(hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
     return FG_OK;
 }
 

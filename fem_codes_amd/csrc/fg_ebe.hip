@@ -480,16 +480,22 @@ static int default_ept(int k) { return fg_ebe_default_ept(k); }
 static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int per_block,
                       int kind, fg_op **out)
 {
-    FG_ARG(c && out && lm, "ctx/out/lm");
+    // sizes first (host-only checks): the kernels address P, Q and the halo
+    // partials through 32-bit byte offsets (fg_at), so every index times 8
+    // must stay below 2^32
     FG_ARG(neq >= 0 && nitems >= 0, "sizes");
+    FG_ARG(neq < FG_OFF32_MAX, "neq must be below 2^28 (32-bit byte offsets of P and Q)");
+    FG_ARG((long long)nitems * 16 < (1LL << 31), "nitems * k must fit in an int");
     FG_ARG(k == 2 || k == 3 || k == 4 || k == 8 || k == 12 || k == 16,
            "k must be 2, 3, 4, 8, 12 or 16");
+    FG_ARG(c && out && lm, "ctx/out/lm");
     int ept = per_block > 0 ? (per_block + FG_TPB - 1) / FG_TPB : default_ept(k);
     if (ept > 4) ept = 8;
     else if (ept > 2) ept = 4;
     FG_HIP(hipSetDevice(c->device));
     Plan pl;
     FG_CHECK(build_plan(neq, nitems, k, lm, FG_TPB * ept, pl));
+    FG_ARG(pl.nslots < FG_OFF32_MAX, "halo partial slots must stay below 2^28");
     fg_op *op = new fg_op();
     op->ctx = c;
     op->kind = kind;

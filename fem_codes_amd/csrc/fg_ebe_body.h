@@ -81,7 +81,8 @@ __host__ __device__ constexpr size_t fg_ebe_smem_bytes(int k, int ept, int nd, i
 // x[i] through a block-uniform base (scalar registers) and a 32-bit byte
 // offset, so each load or store needs one VGPR of address rather than a
 // 64-bit pair.  Requires i * sizeof(T) < 2^32: block-local arrays, and the
-// global vectors of neq + 1 < 2^28 entries (checked at operator creation).
+// global vectors of neq + 1 <= 2^28 entries and the halo partials (fewer
+// than 2^28 slots), both checked by ebe_create before any device work.
 template <typename T>
 __device__ __forceinline__ const T &fg_at(const T *x, unsigned i)
 {

@@ -640,27 +640,24 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
     // CGS needs the grid of cgs_grid(n) blocks to cover n in one sweep
     const bool cgs = ir && o->orth == FG_ORTH_CGS && !fused && cgs_grid(n) > 0;
 
-    // device workspace, released on every return path
-    struct Work {
-        GmState *st = nullptr;
-        void *U = nullptr, *W = nullptr;
-        double *W64 = nullptr, *nb2 = nullptr, *parts = nullptr, *cg = nullptr;
-        ~Work()
-        {
-            hipFree(st); hipFree(U); hipFree(W); hipFree(W64); hipFree(nb2); hipFree(parts);
-            hipFree(cg);
-        }
-    } wk;
+    // device workspace: carved from the context's solver workspace, which
+    // persists between calls (no allocation or clearing per solve: every
+    // entry of the basis, W and W64 is written before it is read, and the
+    // state is reset by k_state_init / k_cycle_start)
     constexpr int PMAX = VG_MAX > CG_GMAX ? VG_MAX : CG_GMAX;   // partials per buffer
-    FG_HIP(hipMalloc(&wk.st, sizeof(GmState)));
-    FG_HIP(hipMalloc(&wk.W64, sizeof(double) * ld));
-    FG_HIP(hipMalloc(&wk.nb2, sizeof(double)));
-    FG_HIP(hipMalloc(&wk.parts, sizeof(double) * 2 * PMAX));
-    if (cgs) FG_HIP(hipMalloc(&wk.cg, sizeof(double) * (FG_KMAX + 1) * CG_GMAX));
-    FG_HIP(hipMemsetAsync(wk.W64, 0, sizeof(double) * ld, c->stream));
-    GmState *st = wk.st;
-    double *W64 = wk.W64, *nb2 = wk.nb2;
-    Parts pp{{wk.parts, wk.parts + PMAX}, 0, vgrid(n)};
+    fg_carver cv;
+    const size_t o_st = cv.take(sizeof(GmState)), o_W64 = cv.take(sizeof(double) * ld);
+    const size_t o_nb2 = cv.take(sizeof(double)), o_parts = cv.take(sizeof(double) * 2 * PMAX);
+    const size_t o_cg = cgs ? cv.take(sizeof(double) * (FG_KMAX + 1) * CG_GMAX) : 0;
+    const size_t o_U = fused ? 0 : cv.take(sb * ld * (kmax + 1));
+    const size_t o_W = fused ? 0 : cv.take(sb * ld);
+    char *wsb = nullptr;
+    FG_CHECK(fg_ctx_workspace(c, cv.off, (void **)&wsb));
+    GmState *st = (GmState *)(wsb + o_st);
+    double *W64 = (double *)(wsb + o_W64), *nb2 = (double *)(wsb + o_nb2);
+    double *parts = (double *)(wsb + o_parts);
+    double *cgbuf = cgs ? (double *)(wsb + o_cg) : nullptr;
+    Parts pp{{parts, parts + PMAX}, 0, vgrid(n)};
 
     if (fused) {
         // eps = tol * ||B|| (pgmres.c:49-53), then the persistent solver
@@ -689,11 +686,7 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         return loc.status == FG_ERR_NOCONV ? FG_ERR_NOCONV : loc.status;
     }
     // the multi-kernel engine (FG_ENGINE_AUTO and FG_ENGINE_KERNELS)
-    FG_HIP(hipMalloc(&wk.U, sb * ld * (kmax + 1)));
-    FG_HIP(hipMalloc(&wk.W, sb * ld));
-    void *U = wk.U, *W = wk.W;
-    FG_HIP(hipMemsetAsync(U, 0, sb * ld * (kmax + 1), c->stream));
-    FG_HIP(hipMemsetAsync(W, 0, sb * ld, c->stream));
+    void *U = wsb + o_U, *W = wsb + o_W;
 
     FG_HIP(hipEventRecord(c->ev0, c->stream));
     // X = 0 (pgmres.c:46) ; eps = tol*||B|| (:49-53)
@@ -727,7 +720,7 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
             FG_LAUNCH(k_scale_to<float>, g, FG_TPB, 0, c->stream, n, W64, (const float *)nullptr,
                       (float *)U, st, 0);
             FG_CHECK(cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, kmax, pp,
-                                  cgs ? wk.cg : nullptr));
+                                  cgbuf));
             FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
             FG_LAUNCH(k_xupdate<float>, g, FG_TPB, 0, c->stream, n, X, (const float *)U, ld, st);
         } else {

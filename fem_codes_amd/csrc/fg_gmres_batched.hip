@@ -527,18 +527,24 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
         {
             if (ge) hipGraphExecDestroy(ge);
             if (g) hipGraphDestroy(g);
-            hipFree(st); hipFree(rec); hipFree(tab); hipFree(U); hipFree(W); hipFree(W64);
-            hipFree(parts);
         }
     } wk;
     const int ntab = 3 * (nch ? nch : 1) + 2 * np;
-    FG_HIP(hipMalloc(&wk.st, sizeof(BState) * np));
-    FG_HIP(hipMalloc(&wk.rec, sizeof(BRec) * np));
-    FG_HIP(hipMalloc(&wk.tab, sizeof(int) * ntab));
-    FG_HIP(hipMalloc(&wk.W64, sizeof(double) * ld));
-    FG_HIP(hipMalloc(&wk.parts, sizeof(double) * 2 * (nch ? nch : 1)));
-    FG_HIP(hipMalloc(&wk.U, sb * ld * (kmax + 1)));
-    FG_HIP(hipMalloc(&wk.W, sb * ld));
+    // buffers carved from the context's solver workspace (kept between calls)
+    fg_carver cv;
+    const size_t o_st = cv.take(sizeof(BState) * np), o_rec = cv.take(sizeof(BRec) * np);
+    const size_t o_tab = cv.take(sizeof(int) * ntab), o_W64 = cv.take(sizeof(double) * ld);
+    const size_t o_parts = cv.take(sizeof(double) * 2 * (nch ? nch : 1));
+    const size_t o_U = cv.take(sb * ld * (kmax + 1)), o_W = cv.take(sb * ld);
+    char *wsb = nullptr;
+    FG_CHECK(fg_ctx_workspace(c, cv.off, (void **)&wsb));
+    wk.st = (BState *)(wsb + o_st);
+    wk.rec = (BRec *)(wsb + o_rec);
+    wk.tab = (int *)(wsb + o_tab);
+    wk.W64 = (double *)(wsb + o_W64);
+    wk.parts = (double *)(wsb + o_parts);
+    wk.U = wsb + o_U;
+    wk.W = wsb + o_W;
     {
         std::vector<int> tab(ntab, 0);
         int *t = tab.data();

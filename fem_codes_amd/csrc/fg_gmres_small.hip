@@ -507,15 +507,24 @@ bool fg_gmres_small_applies(const fg_op *op, const fg_prec *pc, const fg_gmres_o
     return true;
 }
 
-int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
-                   const fg_gmres_opts *o, fg_gmres_info *info)
+namespace {
+
+// arguments and workspace layout of one solve; buffers are bound by bind()
+struct SPlan {
+    SArgs a;
+    int vpt;
+    long long ne0;
+    size_t off_R, off_H, off_E, off_S, off_K, off_A, off_J, bytes;
+};
+
+void plan_small(fg_op *op, fg_prec *pc, const double *B, double *X, const fg_gmres_opts *o,
+                SPlan &sp)
 {
     const int n = op->neq;
     int vpt = 1;
     while (vpt * STPB < n) vpt *= 2;
     const long long ld = ((long long)n + 1 + 63) / 64 * 64;
-
-    SArgs a;
+    SArgs &a = sp.a;
     memset(&a, 0, sizeof a);
     a.n = n; a.kmax = o->kmax; a.lmax = o->lmax; a.ld = ld; a.tol = o->tol;
     a.hist_cap = o->rho_hist ? o->hist_cap : 0;
@@ -538,58 +547,75 @@ int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X
     }
     a.B = B;
     a.X = X;
-
-    // workspace: basis, results, histories
-    const long long ne0 = o->e0_hist ? (long long)o->lmax + 1 : 0;
+    sp.vpt = vpt;
+    // workspace: basis, results, histories, per-solve transposed copies
+    sp.ne0 = o->e0_hist ? (long long)o->lmax + 1 : 0;
     const size_t szU = sizeof(double) * ld * (o->kmax + 1);
     const size_t szH = sizeof(double) * (a.hist_cap > 0 ? a.hist_cap : 0);
-    const size_t szE = sizeof(double) * ne0;
+    const size_t szE = sizeof(double) * sp.ne0;
     const bool csr = op->kind == FG_OP_CSR;
     const size_t nmi = (size_t)op->sm_mi * n;
     const size_t szS = csr ? 0 : sizeof(double) * op->nitems * op->k;             // rs
     const size_t szK = csr ? 0 : sizeof(double) * op->nitems * op->k * op->k;     // KeT
     const size_t szA = csr ? sizeof(double) * nmi : 0;                              // AAT
     const size_t szJ = a.pc == PC_JACOBI ? sizeof(double) * nmi : 0;                // invdeT
-    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-    const size_t off_R = up(szU), off_H = off_R + 256, off_E = off_H + up(szH);
-    const size_t off_S = off_E + up(szE), off_K = off_S + up(szS), off_A = off_K + up(szK);
-    const size_t off_J = off_A + up(szA);
-    void *buf = nullptr;
-    FG_HIP(hipMalloc(&buf, off_J + szJ + 256));
-    struct Free {
-        void *p;
-        ~Free() { hipFree(p); }
-    } fr{buf};
-    char *base = (char *)buf;
-    a.U = (double *)base;
-    a.res = (SRes *)(base + off_R);
-    a.rh = szH ? (double *)(base + off_H) : nullptr;
-    a.e0h = szE ? (double *)(base + off_E) : nullptr;
-    a.rs = szS ? (double *)(base + off_S) : nullptr;
-    a.KeT = szK ? (double *)(base + off_K) : nullptr;
-    a.AAT = szA ? (double *)(base + off_A) : nullptr;
-    a.invdeT = szJ ? (double *)(base + off_J) : nullptr;
+    fg_carver cv;
+    cv.take(szU);
+    sp.off_R = cv.take(sizeof(SRes));
+    sp.off_H = cv.take(szH);
+    sp.off_E = cv.take(szE);
+    sp.off_S = cv.take(szS);
+    sp.off_K = cv.take(szK);
+    sp.off_A = cv.take(szA);
+    sp.off_J = cv.take(szJ);
+    sp.bytes = cv.off;
+    (void)szH;
+}
 
+// point the solve's buffers into `base` (sp.bytes long); every entry the
+// kernel reads it writes first in the same solve, so nothing is cleared
+void bind(SPlan &sp, char *base, fg_op *op)
+{
+    SArgs &a = sp.a;
+    const bool csr = op->kind == FG_OP_CSR;
+    a.U = (double *)base;
+    a.res = (SRes *)(base + sp.off_R);
+    a.rh = a.hist_cap > 0 ? (double *)(base + sp.off_H) : nullptr;
+    a.e0h = sp.ne0 > 0 ? (double *)(base + sp.off_E) : nullptr;
+    a.rs = csr ? nullptr : (double *)(base + sp.off_S);
+    a.KeT = csr ? nullptr : (double *)(base + sp.off_K);
+    a.AAT = csr ? (double *)(base + sp.off_A) : nullptr;
+    a.invdeT = a.pc == PC_JACOBI ? (double *)(base + sp.off_J) : nullptr;
+}
+
+int run_small(fg_ctx *c, fg_op *op, const fg_gmres_opts *o, SPlan &sp, fg_gmres_info *info)
+{
+    const SArgs &a = sp.a;
     FG_HIP(hipEventRecord(c->ev0, c->stream));
-    if (op->kind == FG_OP_CSR) FG_CHECK(launch_k<0>(c, a, vpt));
-    else if (op->k == 2) FG_CHECK(launch_k<2>(c, a, vpt));
-    else if (op->k == 3) FG_CHECK(launch_k<3>(c, a, vpt));
-    else FG_CHECK(launch_k<4>(c, a, vpt));
+    if (op->kind == FG_OP_CSR) FG_CHECK(launch_k<0>(c, a, sp.vpt));
+    else if (op->k == 2) FG_CHECK(launch_k<2>(c, a, sp.vpt));
+    else if (op->k == 3) FG_CHECK(launch_k<3>(c, a, sp.vpt));
+    else FG_CHECK(launch_k<4>(c, a, sp.vpt));
     FG_HIP(hipEventRecord(c->ev1, c->stream));
     FG_HIP(hipEventSynchronize(c->ev1));
     float ms = 0.f;
     FG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     SRes r;
-    FG_HIP(hipMemcpy(&r, a.res, sizeof r, hipMemcpyDeviceToHost));
+    FG_HIP(hipMemcpyAsync(&r, a.res, sizeof r, hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
     if (a.rh) {
         const int nh = r.steps < a.hist_cap ? r.steps : a.hist_cap;
         if (nh > 0)
-            FG_HIP(hipMemcpy(o->rho_hist, a.rh, sizeof(double) * nh, hipMemcpyDeviceToHost));
+            FG_HIP(hipMemcpyAsync(o->rho_hist, a.rh, sizeof(double) * nh, hipMemcpyDeviceToHost,
+                                  c->stream));
     }
     if (a.e0h) {
-        const long long ne = r.ne0 < ne0 ? r.ne0 : ne0;
-        if (ne > 0) FG_HIP(hipMemcpy(o->e0_hist, a.e0h, sizeof(double) * ne, hipMemcpyDeviceToHost));
+        const long long ne = r.ne0 < sp.ne0 ? r.ne0 : sp.ne0;
+        if (ne > 0)
+            FG_HIP(hipMemcpyAsync(o->e0_hist, a.e0h, sizeof(double) * ne,
+                                  hipMemcpyDeviceToHost, c->stream));
     }
+    FG_HIP(hipStreamSynchronize(c->stream));
     if (info) {
         memset(info, 0, sizeof *info);
         info->steps = r.steps;
@@ -601,4 +627,18 @@ int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X
         info->status = r.status;
     }
     return r.status == FG_ERR_NOCONV ? FG_ERR_NOCONV : r.status;
+}
+
+}  // namespace
+
+int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
+                   const fg_gmres_opts *o, fg_gmres_info *info)
+{
+    SPlan sp;
+    plan_small(op, pc, B, X, o, sp);
+    // the context's solver workspace: kept between solves
+    void *buf = nullptr;
+    FG_CHECK(fg_ctx_workspace(c, sp.bytes, &buf));
+    bind(sp, (char *)buf, op);
+    return run_small(c, op, o, sp, info);
 }

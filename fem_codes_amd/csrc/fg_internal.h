@@ -50,6 +50,8 @@ constexpr int FG_TPB = 256;          // threads per block for every kernel
 constexpr int FG_RED_MAX = 4096;     // max blocks of a grid reduction
 constexpr int FG_KMAX = 64;          // max GMRES restart length
 constexpr int FG_SMALL_MAX = 4096;   // equations of a system the one-workgroup solver takes
+// vectors indexed through 32-bit byte offsets (8-byte entries) stay below this
+constexpr int FG_OFF32_MAX = 1 << 28;
 
 // ---------------------------------------------------------------------------
 // context
@@ -62,6 +64,26 @@ struct fg_ctx {
     double *red_part = nullptr;      // [FG_RED_MAX] partial sums
     unsigned *red_ticket = nullptr;  // [4] arrival tickets (self-resetting)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // solver workspace (Krylov basis, state, partial sums), grown on demand
+    // and kept until fg_ctx_trim / fg_ctx_destroy, so repeated solves (the
+    // corrections of a time loop) allocate nothing
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+// the context's solver workspace, at least `bytes` long (256-byte aligned);
+// growing it waits for the stream first.  One user at a time: the solvers
+// that take it (fg_gmres, fg_gmres_batched) do not nest.
+int fg_ctx_workspace(fg_ctx *c, size_t bytes, void **p);
+// carve consecutive 256-byte aligned pieces out of a workspace
+struct fg_carver {
+    size_t off = 0;
+    size_t take(size_t bytes)
+    {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    }
 };
 
 // ---------------------------------------------------------------------------

@@ -175,6 +175,15 @@ int predictor_loop(fg_ctx *c, int neq, size_t work_doubles, fg_op *A, fg_prec *p
             stop_time = fabs(p->final_time - t) < 1e-15;
         }
         if (stop_time) break;
+        // StopByTime ends only when t lands within 1e-15 of the final time;
+        // accumulated steps usually miss it by more (dt = 1e-4, T = 1 drifts
+        // ~2e-14), and the reference then never stops.  Once t has passed the
+        // final time by half a step the loop ends with FG_ERR_NOCONV; every
+        // run the reference itself ends, ends here in the same step.
+        if (!p->steady && t > p->final_time + 0.5 * dt) {
+            inf.status = FG_ERR_NOCONV;
+            break;
+        }
         if (p->max_steps > 0 && passo >= p->max_steps) {
             inf.status = FG_ERR_NOCONV;
             break;

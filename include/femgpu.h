@@ -54,6 +54,11 @@ const char *fg_version(void);
 int fg_ctx_create(int device, fg_ctx **out);
 int fg_ctx_destroy(fg_ctx *ctx);
 int fg_ctx_sync(fg_ctx *ctx);
+/* release the solver workspace the context keeps between solves (no
+ * reference counterpart: fg_gmres and fg_gmres_batched keep their Krylov
+ * basis and state in one per-context buffer, grown on demand, so the solves
+ * of a time loop allocate nothing; this frees it until the next solve) */
+int fg_ctx_trim(fg_ctx *ctx);
 int fg_malloc(fg_ctx *ctx, size_t bytes, void **dptr);
 int fg_free(fg_ctx *ctx, void *dptr);
 int fg_memcpy_h2d(fg_ctx *ctx, void *dst, const void *src, size_t bytes);
@@ -483,7 +488,9 @@ int fg_gmres_batched(fg_ctx *ctx, fg_op *op, fg_prec *pc, int nprob, const int *
  * u += alpha dt Da until StopByNorm (||Da|| < nl_tol ||a|| or
  * i >= nl_maxiter) or StopByIterations (i >= nl_maxiter); steps until
  * StopBySteadyState (time_tol ||u|| > ||u - u_old|| or |T - t| < dt) or
- * StopByTime (|T - t| < 1e-15).  The reference declares the 6-argument
+ * StopByTime (|T - t| < 1e-15; without the steady-state test the loop also
+ * ends, with FG_ERR_NOCONV, once t > T + dt/2, where the reference would run
+ * on without end).  The reference declares the 6-argument
  * solver(P, M, FS, FF, B, X) but has no implementation of it; here it is
  * fg_gmres with `gmres` options (a non-converged solve is reported in
  * `status` and the loop goes on, as a solver returning without convergence
@@ -507,7 +514,8 @@ typedef struct {
     int    unconverged;      /* solves that returned FG_ERR_NOCONV                 */
     double t;                /* final time (Parameters->CurrentTime)               */
     double ms;               /* wall time                                          */
-    int    status;           /* FG_OK, FG_ERR_NOCONV (max_steps) or an error       */
+    int    status;           /* FG_OK, FG_ERR_NOCONV (max_steps, t > T + dt/2) or  */
+                             /* an error                                           */
 } fg_predictor_info;
 /* The transient 2D transport driver (02_Transport_Equation): Build =
  * fg_supg2d_transient_build on `m` (Build_M_K_R_SUPG.c, nodal U, dU from

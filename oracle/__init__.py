@@ -104,7 +104,6 @@ class Oracle:
         L.orc_ddot.argtypes = [_i, _dp, _dp]
         L.orc_csrmv.argtypes = [_i, _dp, _ip, _ip, _dp, _dp]
         L.orc_ebemv.argtypes = [_i, _i, _i, _dp, _ip, _dp, _dp]
-        L.orc_ebemv_f32.argtypes = [_i, _i, _i, _fp, _ip, _fp, _fp]
         L.orc_edemv.argtypes = [_i, _i, _dp, _ip, _dp, _dp]
         L.orc_diag_apply.argtypes = [_i, _dp, _dp, _dp]
         L.orc_diag_setup_ebe.argtypes = [_i, _i, _i, _dp, _ip, _dp, _dp, _dp]
@@ -326,11 +325,6 @@ class Oracle:
     def ebemv(self, neq, nel, k, A, lm, P):
         Q = np.zeros(neq + 1)
         self.lib.orc_ebemv(neq, nel, k, A, lm, P, Q)
-        return Q
-
-    def ebemv_f32(self, neq, nel, k, A, lm, P):
-        Q = np.zeros(neq + 1, np.float32)
-        self.lib.orc_ebemv_f32(neq, nel, k, A, lm, P, Q)
         return Q
 
     def edemv(self, neq, nedge, A, lm2, P):

@@ -17,7 +17,12 @@
  *       schemes must agree, row sums of ke vanish for gamma=0 and u=0,
  *       sum(Fe)=Area*f for constant f, GMRES against a dense LU solve, and
  *       GMRES on diag(1..k) converges in k steps.
- *   nnz / nedge closed forms for the structured mesh (SURVEY formulas).
+ *   nnz / nedge closed forms for the structured mesh.
+ *   ILU(0) apply, FP64: bitwise against the reference's fixtures and the
+ *   identity (LU)_ij = A_ij on the pattern; FP32 (orc_ilu0_apply_f32): bit
+ *   for bit the pinned FP64 apply on inputs whose arithmetic is exact in
+ *   both precisions, and the componentwise backward-error bound of a
+ *   triangular solve on real factors (tests/test_oracle_fp32.py).
  */
 #include "oracle.h"
 #include "dense_solve.h"
@@ -116,29 +121,6 @@ void orc_ebemv(int neq, int nel, int k, const double *A, const int *lm,
             Q[le[r]] = Q[le[r]] + row;
         }
         Q[neq] = 0.0;
-    }
-}
-
-void orc_ebemv_f32(int neq, int nel, int k, const float *A, const int *lm,
-                   float *P, float *Q)
-{
-    float p[16];
-    for (int r = 0; r < neq; r++)
-        Q[r] = 0.0f;
-    P[neq] = 0.0f;
-    Q[neq] = 0.0f;
-    for (long e = 0; e < nel; e++) {
-        const int *le = lm + e * k;
-        const float *Ae = A + e * (long)k * k;
-        for (int c = 0; c < k; c++)
-            p[c] = P[le[c]];
-        for (int r = 0; r < k; r++) {
-            float row = Ae[r * k] * p[0];
-            for (int c = 1; c < k; c++)
-                row = row + Ae[r * k + c] * p[c];
-            Q[le[r]] = Q[le[r]] + row;
-        }
-        Q[neq] = 0.0f;
     }
 }
 

@@ -47,8 +47,6 @@ void orc_ebemv(int neq, int nel, int k, const double *A, const int *lm,
 void orc_edemv(int neq, int nedge, const double *A, const int *lm2,
                double *P, double *Q);
 /* single-precision twins used for the FP32 inner solve checks */
-void orc_ebemv_f32(int neq, int nel, int k, const float *A, const int *lm,
-                   float *P, float *Q);
 
 /* ---------------- preconditioners ----------------------------------------- */
 /* Diag_precond.c:3-14 : z = p*invDiag (p may alias z) */

@@ -78,3 +78,26 @@ def test_invalid_arguments_fail_loudly():
     rc = F.lib.fg_fill_lm(3, 1, 0, z, z, z, z)
     assert rc == F.FG_ERR_ARG
     assert "invalid argument" in F.lib.fg_last_error().decode()
+
+
+def test_operator_sizes_beyond_32bit_offsets_are_rejected():
+    """the blocked products address P and Q through 32-bit byte offsets:
+    fg_op_ebe_create / fg_op_ede_create refuse neq >= 2^28 (and nitems * k
+    beyond an int) before any device work, with or without a context"""
+    lm = np.zeros(3, np.int32)
+    out = F._vp()
+    for neq in (1 << 28, (1 << 29) + 5):
+        rc = F.lib.fg_op_ebe_create(None, neq, 1, 3, lm, 0, F.C.byref(out))
+        assert rc == F.FG_ERR_ARG
+        assert "2^28" in F.lib.fg_last_error().decode()
+        rc = F.lib.fg_op_ede_create(None, neq, 1, lm[:2].copy(), 0, F.C.byref(out))
+        assert rc == F.FG_ERR_ARG
+        assert "2^28" in F.lib.fg_last_error().decode()
+    rc = F.lib.fg_op_ebe_create(None, 100, 1 << 28, 12, lm, 0, F.C.byref(out))
+    assert rc == F.FG_ERR_ARG
+    assert "fit in an int" in F.lib.fg_last_error().decode()
+    # the largest admissible size passes the size checks and stops at the
+    # missing context
+    rc = F.lib.fg_op_ebe_create(None, (1 << 28) - 1, 1, 3, lm, 0, F.C.byref(out))
+    assert rc == F.FG_ERR_ARG
+    assert "ctx" in F.lib.fg_last_error().decode()
