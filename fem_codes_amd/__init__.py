@@ -117,6 +117,8 @@ SIGNATURES = {
     "fg_prec_set_invdiag": (_i, [_vp, _dp]),
     "fg_prec_destroy": (_i, [_vp]),
     "fg_gmres": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(GmresOpts), C.POINTER(GmresInfo)]),
+    "fg_gmres_small_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, C.POINTER(GmresOpts),
+                                    C.POINTER(GmresInfo)]),
     "fg_gmres_batched": (_i, [_vp, _vp, _vp, _i, _ip, _vp, _vp, C.POINTER(GmresOpts), _i,
                               _vp]),
     "fg_number_nodes": (_i, [_i, _ip, _ip]),

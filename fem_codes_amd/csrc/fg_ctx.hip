@@ -38,6 +38,20 @@ extern "C" int fg_ctx_create(int device, fg_ctx **out)
     FG_HIP(hipEventCreate(&c->ev0));
     FG_HIP(hipEventCreate(&c->ev1));
     FG_HIP(hipStreamSynchronize(c->stream));
+    // once per device: the device address of the allocation-free solver's
+    // scratch array, looked up with this device current
+    {
+        const int rc = fg_noalloc_init(device);
+        if (rc != FG_OK) {
+            hipFree(c->red_part);
+            hipFree(c->red_ticket);
+            hipEventDestroy(c->ev0);
+            hipEventDestroy(c->ev1);
+            hipStreamDestroy(c->stream);
+            delete c;
+            return rc;
+        }
+    }
     *out = c;
     return FG_OK;
 }

@@ -629,7 +629,31 @@ int run_small(fg_ctx *c, fg_op *op, const fg_gmres_opts *o, SPlan &sp, fg_gmres_
     return r.status == FG_ERR_NOCONV ? FG_ERR_NOCONV : r.status;
 }
 
+// ---- the allocation-free variant -----------------------------------------
+// Its workspace is this file-scope device array (one copy per GPU, held until
+// the process ends).  In host code `scratch` names a host-side stand-in, not
+// device memory: the device address of each GPU's copy is looked up once per
+// device (fg_noalloc_init, called by fg_ctx_create with that device current)
+// and only that address is passed to kernels.
+constexpr int NOALLOC_MAXDEV = 64;
+static __device__ __align__(256) char scratch[FG_NOALLOC_SCRATCH_BYTES];
+static void *g_scratch_dev[NOALLOC_MAXDEV];
+
 }  // namespace
+
+int fg_noalloc_init(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAXDEV) {
+        fg_set_error("device %d beyond the %d devices the scratch table holds", device,
+                     NOALLOC_MAXDEV);
+        return FG_ERR_ARG;
+    }
+    if (g_scratch_dev[device]) return FG_OK;
+    void *ptr = nullptr;
+    FG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = ptr;
+    return FG_OK;
+}
 
 int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X,
                    const fg_gmres_opts *o, fg_gmres_info *info)
@@ -640,5 +664,38 @@ int fg_gmres_small(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X
     void *buf = nullptr;
     FG_CHECK(fg_ctx_workspace(c, sp.bytes, &buf));
     bind(sp, (char *)buf, op);
+    return run_small(c, op, o, sp, info);
+}
+
+extern "C" int fg_gmres_small_noalloc(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B,
+                                      double *X, const fg_gmres_opts *o, fg_gmres_info *info)
+{
+    FG_ARG(c && op && B && X && o, "ctx/op/B/X/opts");
+    FG_ARG(o->kmax >= 1 && o->kmax <= FG_KMAX, "kmax in [1,64]");
+    FG_ARG(o->lmax >= 1, "lmax >= 1");
+    if (pc && pc->kind == FG_PREC_NONE) pc = nullptr;
+    FG_ARG(!pc || (pc->op->neq == op->neq && pc->op->ctx == c),
+           "preconditioner built on an operator of another size or context");
+    FG_ARG(op->ctx == c, "operator of another context");
+    if (!fg_gmres_small_applies(op, pc, o)) {
+        fg_set_error("invalid argument: the allocation-free GMRES takes FP64 solves of at most "
+                     "%d equations, CSR or EBE/EDE with k = 2..4, and no, Diag or element "
+                     "Jacobi preconditioning", FG_SMALL_MAX);
+        return FG_ERR_ARG;
+    }
+    SPlan sp;
+    plan_small(op, pc, B, X, o, sp);
+    if (sp.bytes > (size_t)FG_NOALLOC_SCRATCH_BYTES) {
+        fg_set_error("invalid argument: the solve needs %zu bytes of workspace, more than the "
+                     "%d bytes of the allocation-free variant's scratch (fewer elements, a "
+                     "shorter restart or shorter histories fit)", sp.bytes,
+                     FG_NOALLOC_SCRATCH_BYTES);
+        return FG_ERR_ARG;
+    }
+    FG_ARG(c->device >= 0 && c->device < NOALLOC_MAXDEV && g_scratch_dev[c->device],
+           "the scratch address of this device was not looked up (context not created by "
+           "fg_ctx_create)");
+    FG_HIP(hipSetDevice(c->device));
+    bind(sp, (char *)g_scratch_dev[c->device], op);
     return run_small(c, op, o, sp, info);
 }

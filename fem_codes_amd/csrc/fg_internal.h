@@ -52,6 +52,10 @@ constexpr int FG_KMAX = 64;          // max GMRES restart length
 constexpr int FG_SMALL_MAX = 4096;   // equations of a system the one-workgroup solver takes
 // vectors indexed through 32-bit byte offsets (8-byte entries) stay below this
 constexpr int FG_OFF32_MAX = 1 << 28;
+// per-GPU scratch of the allocation-free GMRES (fg_gmres_small_noalloc)
+constexpr int FG_NOALLOC_SCRATCH_BYTES = 8 << 20;
+// looks up the device address of that scratch for `device` (current)
+int fg_noalloc_init(int device);
 
 // ---------------------------------------------------------------------------
 // context

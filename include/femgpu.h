@@ -460,6 +460,22 @@ typedef struct {
 int fg_gmres(fg_ctx *ctx, fg_op *op, fg_prec *pc, const double *B, double *X,
              const fg_gmres_opts *opts, fg_gmres_info *info);
 
+/* fg_gmres_small_noalloc -- no counterpart in the reference: an entry point
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * solve as fg_gmres with engine FG_ENGINE_SMALL (FP64, at most 4096
+ * equations, CSR or EBE/EDE with k = 2..4, NONE / DIAG / element JACOBI;
+ * same steps, same bits), but its workspace (Krylov basis, results,
+ * histories, per-solve transposed copies) lives in a fixed device array of
+ * 8 MiB per GPU, held until the process ends, instead of allocated memory;
+ * it calls no allocating routine.  FG_ERR_ARG when the solve does not fit
+ * that array (kmax, the number of elements and the history lengths set its
+ * size) or the engine does not apply.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other host threads, other
+ * streams) race. */
+int fg_gmres_small_noalloc(fg_ctx *ctx, fg_op *op, fg_prec *pc, const double *B, double *X,
+                           const fg_gmres_opts *opts, fg_gmres_info *info);
+
 /* Batched restarted GMRES: nprob independent systems solved at once (no
  * reference counterpart as one call; each problem follows pgmres.c:4-211 as
  * fg_gmres does).  The systems are one block-diagonal operator `op` (and its
