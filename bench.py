@@ -1,31 +1,43 @@
 #!/usr/bin/env python3
 """Benchmark of the FEM_CODES solve path on one MI355X.
 
-Workload (the 2D SUPG steady-transport system at its single-GPU size):
-  * unit square, 2048 x 2048 cells split into 8,388,608 P1 triangles
-    (4,198,401 nodes, 4,190,209 equations), SUPG element matrices built on
-    the GPU for a seeded rotating velocity (-y+0.5, x-0.5)*U[0.5,2] and
-    diffusivity 10^U[-6,-3];
-  * one "step" = one FP64 EBE matrix-vector product Q = A P over the whole
-    mesh (ebemv, MatrixVector_Operations/EBE/ebemv.c:4-43), inputs resident
-    in HBM; `value` = algorithmic bytes / time in GB/s;
-  * the same product in FP32 and with the EDE and CSR storage schemes;
-  * time to relative residual 1e-8 of the element-Jacobi-preconditioned
-    GMRES(30) (--prec diag: the global diagonal), FP64 vs FP32-IR, on the
-    constant-velocity SUPG system (beta=(cos t, sin t), k=1e-4, f=1) at
-    2048^2 and at 64^2.  The rotating system above does not converge with
-    GMRES(30)+Jacobi in a practical number of steps (the element Jacobi
-    stagnates on it already at 16^2), so it is used for the products only.
-  * cpu_baseline: the reference ebemv compiled from its own sources
-    (oracle/_ref, -Ofast) or, where that library is absent, the oracle
-    restatement, timed on one host core over the full mesh.
+Every workload of the solve path runs by default (one leg each; --legs picks
+a subset):
 
-Prints one JSON line.  --gpus N>1 exits: this build runs on one GPU.
+  headline  the 2D SUPG steady-transport system at its single-GPU size: unit
+            square, 2048 x 2048 cells split into 8,388,608 P1 triangles
+            (4,190,209 equations), element matrices built on the GPU for the
+            seeded rotating velocity (-y+0.5, x-0.5)*U[0.5,2] and
+            diffusivity 10^U[-6,-3].  One "step" = one FP64 EBE product
+            Q = A P over the whole mesh (ebemv, MatrixVector_Operations/EBE/
+            ebemv.c:4-43), inputs resident in HBM; `value` = algorithmic
+            bytes / time in GB/s.  Also: the FP32 product and the EDE and CSR
+            schemes; GMRES(30) to 1e-8 on this rotating system (it stagnates:
+            the steps and the residual reached within a cap are recorded) and,
+            FP64 vs FP32-IR, on the constant-velocity system (beta = (cos t,
+            sin t), kappa 1e-4) with each storage scheme;
+  small     the reference's own case: 64 x 64, CSR/EBE, Diag, GMRES(30) to
+            1e-8, against the reference pgmres on one host core;
+  batched   256 independent SUPG problems, 32..512 cells per side, one
+            batched FP64 / FP32-IR GMRES(30) with graph-replayed cycles;
+  ss3d      3D SUPG (DIFCONV) on the 128^3 Kuhn cube (12.6M tetrahedra), a
+            seeded random numbering then RCM, EBE 4x4 products + CSR ILU(0),
+            GMRES(50) to 1e-10, FP64 vs FP32-IR;
+  euler     2D Euler, 1024^2 grid, 12x12 block element matrices, one
+            implicit step with the node block-diagonal preconditioner,
+            FP64 vs FP32-IR; and the edge scheme (8x8 edge matrices);
+  transient the 02 driver's CONE problem through Predictor_Old at 1024^2.
+
+Each leg reports `roofline` for its dominant kernel (algorithmic bytes of one
+launch over that kernel's mean duration, HIP events on the library stream)
+and `cpu_baseline`: the reference routine compiled from its own sources with
+its own speed flags (oracle/_ref, -Ofast) timed on one host core over a
+bounded sample, with the CPU model.  Prints one JSON line.  --gpus N>1 exits:
+this build runs on one GPU.
 """
 import argparse
 import json
 import os
-import platform
 import sys
 import time
 
@@ -38,12 +50,77 @@ METRIC = ("matvec HBM GB/s (% of peak) + time-to-1e-8 solve, FP32-IR vs FP64 GMR
 PEAK_GBS = 8000.0
 ENGINES = {"auto": 0, "kernels": 1, "fused": 2, "small": 3}
 ORTHS = {"mgs": 0, "cgs": 1}
+LEGS = ("headline", "small", "batched", "ss3d", "euler", "transient")
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def cpu_info():
+    """model name and core counts of the host (for the CPU baselines)"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu": model, "nproc": os.cpu_count(), "cores_available": avail}
+
+
+def roofline(ms, nbytes, kernel):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / PEAK_GBS, "traffic": None, "kernel": kernel,
+            "algorithmic_bytes": nbytes, "ms": ms}
+
+
+def ebe_kernel_name(k, fp32):
+    ept = 4 if k <= 2 else 2 if k <= 4 else 1
+    return f"k_ebe_apply<{k}, {ept}, {'float' if fp32 else 'double'}, 0>"
+
+
+def time_cpu(run, budget_s, max_reps):
+    """mean seconds of run() over a bounded sample (one warm-up call)"""
+    run()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        run()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or reps >= max_reps:
+            return el / reps, reps
+
+
+def cpu_product_baseline(name, run, nbytes, budget_s, max_reps, what, kind="reference"):
+    sec, reps = time_cpu(run, budget_s, max_reps)
+    out = dict(value=nbytes / sec / 1e9, unit="GB/s", cores=1, kind=kind,
+               sample=f"{reps} FP64 {name} products of the full {what} "
+                      f"({sec * 1e3:.1f} ms each), -Ofast, one thread",
+               ms_per_matvec=sec * 1e3)
+    out.update(cpu_info())
+    return out
+
+
+def matvec_table(op, P64, Q64, P32, Q32, reps):
+    r = {}
+    for fp32, (P, Q) in ((False, (P64, Q64)), (True, (P32, Q32))):
+        t = op.time_apply(P, Q, reps=reps)
+        b = op.algorithmic_bytes(fp32)
+        r["f32" if fp32 else "f64"] = dict(ms=t, GBs=b / (t * 1e-3) / 1e9,
+                                          pct_peak=100 * b / (t * 1e-3) / 1e9 / PEAK_GBS,
+                                          bytes=b)
+    return r
+
+
+# ---------------------------------------------------------------------------
 def setup_system(F, ctx, n, prob, schemes=("EBE", "EDE", "CSR")):
     x, y, t, v = F.structured_mesh2d(n)
     mesh = F.Mesh2D(ctx, x, y, t, v)
@@ -53,7 +130,7 @@ def setup_system(F, ctx, n, prob, schemes=("EBE", "EDE", "CSR")):
     Fv = ctx.empty(neq + 1)
     Fv.zero()
     mesh.build_supg(prob, ke, Fv)
-    ops = {}
+    ops, aux = {}, {}
     if "EBE" in schemes:
         ops["EBE"] = F.Operator.ebe(ctx, neq, lm, 3)
         ops["EBE"].set_values(ke)
@@ -61,21 +138,32 @@ def setup_system(F, ctx, n, prob, schemes=("EBE", "EDE", "CSR")):
         nnz, IA, JA, cbe = F.csr_pattern(neq, lm)
         ops["CSR"] = F.Operator.csr(ctx, neq, IA, JA)
         F.assemble_csr(ops["CSR"], cbe, ke)
-        del IA, JA, cbe
+        aux["CSR"] = cbe
     if "EDE" in schemes:
         nedge, order, ebe, lm2 = F.ede_pattern(neq, lm)
         ops["EDE"] = F.Operator.ede(ctx, neq, lm2)
         F.assemble_ede(ops["EDE"], order, ebe, ke)
-        del order, ebe, lm2
+        aux["EDE"] = (order, ebe)
     return dict(x=x, y=y, type=t, vertex=v, mesh=mesh, neq=neq, nel=nel, lm=lm, ke=ke, F=Fv,
-                ops=ops)
+                ops=ops, aux=aux)
 
 
-def time_solve(F, ctx, sysd, precision, engine, kmax=30, tol=1e-8, lmax=100000, prec="diag",
-               orth=0, scheme="EBE"):
-    op = sysd["ops"].get("EBE_default", sysd["ops"]["EBE"]) if scheme == "EBE" else \
-        sysd["ops"][scheme]
-    pc = F.Preconditioner(op, F.PREC_JACOBI if prec == "jacobi" else F.PREC_DIAG)
+def reassemble(F, sysd, prob):
+    """rebuild the element matrices for another problem, into every scheme"""
+    sysd["F"].zero()
+    sysd["mesh"].build_supg(prob, sysd["ke"], sysd["F"])
+    for name, op in sysd["ops"].items():
+        if name == "CSR":
+            F.assemble_csr(op, sysd["aux"]["CSR"], sysd["ke"])
+        elif name == "EDE":
+            F.assemble_ede(op, *sysd["aux"]["EDE"], sysd["ke"])
+        else:
+            op.set_values(sysd["ke"])
+
+
+def time_solve(F, ctx, sysd, op, prec_kind, precision, engine, kmax=30, tol=1e-8,
+               lmax=100000, orth=0):
+    pc = F.Preconditioner(op, prec_kind)
     B = ctx.empty(sysd["neq"] + 1)
     F.lib.fg_memcpy_d2d(ctx.h, B.ptr, sysd["F"].ptr, B.nbytes)
     pc.setup(B)
@@ -89,74 +177,234 @@ def time_solve(F, ctx, sysd, precision, engine, kmax=30, tol=1e-8, lmax=100000, 
     return info, wall, X
 
 
-def cpu_baseline(sysd, ke_host, budget_s=12.0):
-    """reference ebemv (or the oracle port) on one core over the full mesh"""
+def solve_record(info, wall, tol):
+    return dict(seconds=wall, gpu_ms=info.ms, steps=info.steps, cycles=info.cycles,
+                true_rel_res=info.true_res / (info.eps / tol) if info.eps > 0 else None,
+                status=info.status)
+
+
+def headline_leg(F, ctx, args, rng):
+    """config 2: products of the three storage schemes, roofline, the
+    reference CPU product, and the time-to-solve runs"""
+    scale = float(rng.uniform(0.5, 2.0))
+    kappa = float(10.0 ** rng.uniform(-6.0, -3.0))
+    theta = float(rng.uniform(0.0, 2 * np.pi))
+    n = args.n
+    t0 = time.perf_counter()
+    rot = F.Supg2dProblem(F.P_ROTATING, kappa, 0.0, 1.0, 0.0, 0.0, scale, F.H_2SQRTAREA)
+    schemes = ("EBE",) if args.matvec_only else ("EBE", "EDE", "CSR")
+    sysd = setup_system(F, ctx, n, rot, schemes)
+    ctx.sync()
+    neq = sysd["neq"]
+    log(f"setup n={n} neq={neq} nel={sysd['nel']} in {time.perf_counter() - t0:.1f}s")
+    P64 = ctx.to_device(rng.standard_normal(neq + 1))
+    Q64 = ctx.empty(neq + 1)
+    P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
+    Q32 = ctx.empty(neq + 1, np.float32)
+    from fem_codes_amd import autotune
+    (best_epb, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(ctx, neq, sysd["lm"], 3,
+                                                                  sysd["ke"])
+    log("autotune EBE f64 ms by (elements/block, workgroups/CU):",
+        {k: round(v, 4) for k, v in tune_ms.items()}, "->", (best_epb, best_pipe))
+    default_op = sysd["ops"]["EBE"]
+    sysd["ops"]["EBE"] = ebe_tuned
+    ebe = ebe_tuned
+    for _ in range(args.warmup):
+        ebe.apply(P64, Q64)
+    ctx.sync()
+    w0 = time.perf_counter()
+    ms = ebe.time_apply(P64, Q64, reps=args.steps)
+    ctx.sync()
+    wall = time.perf_counter() - w0
+    ms_main = ebe.time_apply(P64, Q64, reps=args.steps, main_only=True)
+    bytes64 = ebe.algorithmic_bytes(False)
+    value = bytes64 / (ms * 1e-3) / 1e9
+    log(f"EBE f64: {ms:.4f} ms/matvec (block kernel {ms_main:.4f} ms) -> {value:.1f} GB/s")
+    kernel = (f"k_ebe_apply_pipe<3, {best_epb // 256}, double, 0>" if best_pipe
+              else f"k_ebe_apply<3, {best_epb // 256}, double, 0>")
+    roof = roofline(ms_main, bytes64, kernel)
+    roof["schedule"] = {"elems_per_block": best_epb, "wg_per_cu": best_pipe}
+    tpath = os.path.join(ROOT, "profiles", "ebe_f64_2048_traffic.json")
+    if os.path.exists(tpath) and n == 2048:
+        with open(tpath) as f:
+            tj = json.load(f)
+        if tj.get("kernel") == kernel:
+            roof["traffic"] = tj.get("hbm_bytes_per_launch")
+
+    matvec = {}
+    for name, op in list(sysd["ops"].items()) + [("EBE_default", default_op)]:
+        matvec[name] = matvec_table(op, P64, Q64, P32, Q32, args.steps)
+        log(name, {k: round(v["GBs"], 1) for k, v in matvec[name].items()})
+    default_op.close()
+    res = {"value": value, "ms": ms, "wall_ms_per_step": wall * 1e3 / args.steps,
+           "scale": scale, "kappa": kappa, "theta": theta, "roofline": roof,
+           "matvec": matvec, "ebe_autotune_ms": {f"{k[0]}/{k[1]}": v for k, v in tune_ms.items()},
+           "neq": neq, "nel": sysd["nel"]}
+
+    if args.calibrate_copy:
+        m = 64 << 20
+        X = ctx.to_device(np.ones(m))
+        Y = ctx.empty(m, np.float32)
+        ctx.sync()
+        c0 = time.perf_counter()
+        for _ in range(3):
+            F.lib.fg_cast_f64_to_f32(ctx.h, m, X.ptr, Y.ptr)
+        ctx.sync()
+        cs = (time.perf_counter() - c0) / 3
+        res["copy_calibration"] = dict(bytes=12 * m, ms=cs * 1e3, GBs=12 * m / cs / 1e9)
+        del X, Y
+
+    if not args.no_cpu and not args.matvec_only:
+        res["cpu_baseline"] = headline_cpu(sysd)
+        log("cpu", res["cpu_baseline"])
+
+    if args.no_solve or args.matvec_only:
+        return res, sysd
+    solve = {}
+    eng = ENGINES[args.engine]
+    # (a) the workload's own system: rotating flow, kappa 10^U[-6,-3];
+    # closed streamlines, GMRES(30) stagnates: capped at args.rot_cycles
+    rot_res = {"precond": "Diag", "kmax": 30, "cycle_cap": args.rot_cycles}
+    for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+        info, w, X = time_solve(F, ctx, sysd, ebe, F.PREC_DIAG, prec, eng,
+                                lmax=args.rot_cycles, orth=ORTHS[args.orth])
+        rot_res[tag] = solve_record(info, w, 1e-8)
+        log("rotating", tag, rot_res[tag])
+    solve[f"rotating_{n}"] = rot_res
+    # (b) the constant-velocity system: FP64 vs FP32-IR, each scheme (Diag),
+    # and the element Jacobi of Jacobi_precond_EBE.c on the EBE scheme
+    const = F.Supg2dProblem(F.P_CONST, 1e-4, 0.0, 1.0, float(np.cos(theta)),
+                            float(np.sin(theta)), 1.0, F.H_2SQRTAREA)
+    reassemble(F, sysd, const)
+    cres = {"kmax": 30, "tol": 1e-8, "engine": args.engine, "fp32_ir_orth": args.orth}
+    runs = [("EBE", F.PREC_JACOBI, "EBE_jacobi")] + \
+           [(s, F.PREC_DIAG, f"{s}_diag") for s in sysd["ops"]]
+    for scheme, pk, tag in runs:
+        r, Xs = {}, {}
+        for prec, pt in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+            info, w, X = time_solve(F, ctx, sysd, sysd["ops"][scheme], pk, prec, eng,
+                                    orth=ORTHS[args.orth])
+            r[pt] = solve_record(info, w, 1e-8)
+            Xs[pt] = X.get()
+        r["speedup_fp32_ir"] = r["fp64"]["seconds"] / r["fp32_ir"]["seconds"]
+        r["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(Xs["fp32_ir"][:neq] - Xs["fp64"][:neq])
+                                             / np.linalg.norm(Xs["fp64"][:neq]))
+        if tag == "EBE_jacobi":
+            Xref = Xs["fp64"]
+        else:
+            r["rel_diff_vs_EBE_jacobi_fp64"] = float(np.linalg.norm(Xs["fp64"][:neq] -
+                                                                    Xref[:neq]) /
+                                                     np.linalg.norm(Xref[:neq]))
+        cres[tag] = r
+        log("const", tag, {k: (v if not isinstance(v, dict) else
+                               {kk: vv for kk, vv in v.items() if kk in ("seconds", "steps")})
+                           for k, v in r.items()})
+    solve[f"const_{n}"] = cres
+    res["solve"] = solve
+    return res, sysd
+
+
+def headline_cpu(sysd):
+    """the reference ebemv on one core over the full mesh"""
     import oracle as O
     neq, nel = sysd["neq"], sysd["nel"]
     P = np.random.default_rng(7).standard_normal(neq + 1)
-    kind = "port"
+    ke_host = sysd["ke"].get()
+    bytes_ = nel * (9 * 8 + 12) + 2 * neq * 8
     if O.Reference.available():
         r = O.Reference(fast=True)
         r.setup(sysd["x"], sysd["y"], sysd["type"], sysd["vertex"], "EBE")
         r.set_matrix(ke_host)
-        run = lambda: r.matvec(P)  # noqa: E731
-        kind = "reference"
+        return cpu_product_baseline("ebemv", lambda: r.matvec(P), bytes_, 12.0, 200,
+                                    f"{nel}-element mesh")
+    o = O.Oracle(fast=True)
+    lm = sysd["lm"]
+    return cpu_product_baseline("ebemv", lambda: o.ebemv(neq, nel, 3, ke_host, lm, P.copy()),
+                                bytes_, 12.0, 200, f"{nel}-element mesh", kind="port")
+
+
+def small_leg(F, ctx, args, theta):
+    """config 1: 64^2 (3,969 equations), Diag, GMRES(30) to 1e-8; each engine,
+    CSR and EBE; the reference pgmres on one host core"""
+    const = F.Supg2dProblem(F.P_CONST, 1e-4, 0.0, 1.0, float(np.cos(theta)),
+                            float(np.sin(theta)), 1.0, F.H_2SQRTAREA)
+    small = setup_system(F, ctx, 64, const, ("EBE", "CSR"))
+    sres = {"precond": "diag", "kmax": 30, "tol": 1e-8}
+    runs = [(sch, F.GMRES_FP64, eng) for sch in ("CSR", "EBE") for eng in ("auto", "kernels")]
+    runs.append(("EBE", F.GMRES_FP32_IR, "kernels"))
+    for sch, prec, eng in runs:
+        tag = f"{sch}_{'fp64' if prec == F.GMRES_FP64 else 'fp32_ir'}_{eng}"
+        best = None
+        for _ in range(3):   # the first call of an engine includes its code-object load
+            info, wall_s, X = time_solve(F, ctx, small, small["ops"][sch], F.PREC_DIAG, prec,
+                                         ENGINES[eng], orth=ORTHS[args.orth])
+            best = wall_s if best is None else min(best, wall_s)
+        sres[tag] = dict(seconds=best, steps=info.steps, cycles=info.cycles,
+                         status=info.status, X=X.get())
+    if not args.no_cpu:
+        cs, Xc = cpu_solve_small(64, (0, 1e-4, 0.0, 1.0, float(np.cos(theta)),
+                                      float(np.sin(theta)), 1.0, 0))
+        sres["cpu_baseline"] = cs
+        for k, r in sres.items():
+            if isinstance(r, dict) and "X" in r:
+                Xg = r.pop("X")
+                r["rel_err_vs_cpu"] = float(np.linalg.norm(Xg[:-1] - Xc[:-1]) /
+                                            np.linalg.norm(Xc[:-1]))
+                r["speedup_vs_cpu"] = cs["seconds"] / r["seconds"]
     else:
-        o = O.Oracle(fast=True)
-        lm = sysd["lm"]
-        run = lambda: o.ebemv(neq, nel, 3, ke_host, lm, P.copy())  # noqa: E731
-    run()
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        run()
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or reps >= 200:
-            break
-    sec = el / reps
-    bytes_ = nel * (9 * 8 + 12) + 2 * neq * 8
-    return dict(value=bytes_ / sec / 1e9, unit="GB/s", cores=1, kind=kind,
-                sample=f"{reps} EBE FP64 matvecs of the full {nel}-element mesh "
-                       f"({sec * 1e3:.1f} ms each), -Ofast single thread",
-                ms_per_matvec=sec * 1e3, cpu=platform.processor() or platform.machine(),
-                nproc=os.cpu_count())
+        for r in sres.values():
+            if isinstance(r, dict):
+                r.pop("X", None)
+    log("64^2", {k: (v["seconds"] if isinstance(v, dict) else v) for k, v in sres.items()})
+    return sres
 
 
 def cpu_solve_small(n, prob_args):
-    """reference pgmres at the 64^2 constant-velocity system (Diag, GMRES(30))"""
+    """the reference pgmres at the 64^2 constant-velocity system (Diag,
+    GMRES(30)), CSR storage as the workload states"""
     import oracle as O
     o = O.Oracle()
     x, y, t, v = o.mesh2d(n)
     neq, idv = o.number_nodes(t)
     nel = len(v) // 3
     lm = o.fill_lm(neq, nel, v, t, idv)
-    ke, Fv = o.build_supg2d(x, y, t, idv, v, neq, np.zeros(neq + 1), O.prob2d(*prob_args))
     if O.Reference.available():
         r = O.Reference(fast=True)
-        r.setup(x, y, t, v, "EBE")
+        r.setup(x, y, t, v, "CSR")
         r.set_problem(O.prob2d(*prob_args))
-        r.set_matrix(ke)
+        A, Fv = r.build(np.zeros(r.neq + 1))
         Fs = r.precond_setup("Diag", Fv)
-        t0 = time.perf_counter()
-        X, steps = r.pgmres(30, 100000, 1e-8, Fs)
-        sec = time.perf_counter() - t0
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            X, steps = r.pgmres(30, 100000, 1e-8, Fs)
+            sec = time.perf_counter() - t0
+            best = sec if best is None else min(best, sec)
         kind = "reference"
     else:
+        ke, Fv = o.build_supg2d(x, y, t, idv, v, neq, np.zeros(neq + 1), O.prob2d(*prob_args))
         fo = O.Oracle(fast=True)
         D, iD, Fs = fo.diag_setup_ebe(neq, nel, 3, ke, lm, Fv)
         t0 = time.perf_counter()
-        X, steps, _, _, _ = fo.pgmres(neq, 30, 100000, 1e-8, Fs, "EBE", (ke, lm, 3, nel), invDiag=iD)
-        sec = time.perf_counter() - t0
+        X, steps, _, _, _ = fo.pgmres(neq, 30, 100000, 1e-8, Fs, "EBE", (ke, lm, 3, nel),
+                                      invDiag=iD)
+        best = time.perf_counter() - t0
         kind = "port"
-    return dict(seconds=sec, steps=int(steps), kind=kind), X, ke, Fv
+    out = dict(value=best, unit="s", seconds=best, steps=int(steps), cores=1, kind=kind,
+               sample="one pgmres solve of the 64^2 system (CSR, Diag, GMRES(30), 1e-8), "
+                      "best of 3, -Ofast, one thread")
+    out.update(cpu_info())
+    return out, X
 
 
-def batched_leg(F, ctx, nprob, seed, prec="jacobi", kmax=30, tol=1e-8, sequential=True):
+# ---------------------------------------------------------------------------
+def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
     """nprob independent constant-velocity SUPG problems, n ~ U{32..512} cells
     per side, angle t ~ U[0, 2 pi), diffusivity 10^U[-5,-2], assembled on the
     GPU, solved (a) together by the batched GMRES (FP64 and FP32-IR, graph
-    replay of each cycle) and (b) one after another by the single-system
-    GMRES, from the same left-preconditioned right-hand sides."""
+    replay of each cycle), (b) one after another by the single-system GMRES,
+    from the same Diag-preconditioned right-hand sides; the reference pgmres
+    on one host core solves a bounded sample of the same problems."""
     rng = np.random.default_rng(seed)
     sizes = rng.integers(32, 513, nprob)
     t0 = time.perf_counter()
@@ -190,8 +438,7 @@ def batched_leg(F, ctx, nprob, seed, prec="jacobi", kmax=30, tol=1e-8, sequentia
         off += ke.n
     op = F.Operator.ebe(ctx, N, lm, 3)
     op.set_values(ke_all)
-    kind = F.PREC_JACOBI if prec == "jacobi" else F.PREC_DIAG
-    pc = F.Preconditioner(op, kind)
+    pc = F.Preconditioner(op, F.PREC_DIAG)
     pc.setup(B)
     ctx.sync()
     log(f"batched: {nprob} problems, {nel} elements, {N} equations, setup "
@@ -199,7 +446,15 @@ def batched_leg(F, ctx, nprob, seed, prec="jacobi", kmax=30, tol=1e-8, sequentia
     res = {"problems": int(nprob), "elements": int(nel), "equations": N,
            "sizes": {"min": int(sizes.min()), "max": int(sizes.max()),
                      "mean_cells_per_side": float(sizes.mean())},
-           "precond": prec, "kmax": kmax, "tol": tol}
+           "precond": "diag", "kmax": kmax, "tol": tol}
+    # the batched product over every problem at once: the dominant kernel
+    P32 = ctx.to_device(rng.standard_normal(N + 1).astype(np.float32))
+    Q32 = ctx.empty(N + 1, np.float32)
+    P64 = ctx.to_device(rng.standard_normal(N + 1))
+    Q64 = ctx.empty(N + 1)
+    res["matvec"] = matvec_table(op, P64, Q64, P32, Q32, 20)
+    ms_main = op.time_apply(P32, Q32, reps=20, main_only=True)
+    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True), ebe_kernel_name(3, True))
     Xs = {}
     for precision, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         X = ctx.empty(N + 1)
@@ -221,15 +476,15 @@ def batched_leg(F, ctx, nprob, seed, prec="jacobi", kmax=30, tol=1e-8, sequentia
             for p in range(nprob) if seg[p + 1] > seg[p]]
     res["fp32_ir_max_rel_err_vs_fp64"] = max(errs)
     res["speedup_fp32_ir"] = res["batched_fp64"]["seconds"] / res["batched_fp32_ir"]["seconds"]
+    Bh = B.get()
     del ke_all, op, pc
-    if sequential:
+    if not args.no_sequential:
         # the same problems, one fg_gmres call each (operators built first)
-        Bh = B.get()
         tot = {"fp64": 0.0, "fp32_ir": 0.0}
         for p, (neq_p, lm_p) in enumerate(parts):
             opp = F.Operator.ebe(ctx, neq_p, lm_p, 3)
             opp.set_values(kes[p])
-            pcp = F.Preconditioner(opp, kind)
+            pcp = F.Preconditioner(opp, F.PREC_DIAG)
             Bp = ctx.to_device(np.append(Bh[seg[p]:seg[p + 1]], 0.0))
             pcp.setup(None)
             for precision, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
@@ -245,15 +500,58 @@ def batched_leg(F, ctx, nprob, seed, prec="jacobi", kmax=30, tol=1e-8, sequentia
         res["batched_speedup_vs_sequential"] = {
             k: tot[k] / res[f"batched_{k}"]["seconds"] for k in tot}
         log("sequential", tot)
+    if not args.no_cpu:
+        res["cpu_baseline"] = batched_cpu(probs, Xs["fp64"], seg, res["batched_fp32_ir"]["seconds"],
+                                          nprob)
+        log("batched cpu", res["cpu_baseline"])
     return res
 
 
-def ss3d_leg(F, ctx, n=128, sweeps=4, seed=20260303):
+def batched_cpu(probs, X_gpu, seg, gpu_seconds, nprob, budget_s=15.0):
+    """the reference pgmres (Diag, GMRES(30), 1e-8) on one core, problem by
+    problem in order, until the budget is spent; problems per second over
+    that sample"""
+    import oracle as O
+    if not O.Reference.available():
+        return None
+    solved, sec, steps, err = 0, 0.0, 0, 0.0
+    for p, (n, th, kappa) in enumerate(probs):
+        if sec > budget_s:
+            break
+        o = O.Oracle()
+        x, y, t, v = o.mesh2d(n)
+        r = O.Reference(fast=True)
+        r.setup(x, y, t, v, "EBE")
+        r.set_problem(O.prob2d(0, kappa, 0.0, 1.0, float(np.cos(th)), float(np.sin(th)), 1.0, 0))
+        A, Fv = r.build(np.zeros(r.neq + 1))
+        Fsp = r.precond_setup("Diag", Fv)
+        t0 = time.perf_counter()
+        X, it = r.pgmres(30, 100000, 1e-8, Fsp)
+        sec += time.perf_counter() - t0
+        steps += it
+        Xg = X_gpu[seg[p]:seg[p + 1]]
+        err = max(err, float(np.linalg.norm(Xg - X[:-1]) / max(np.linalg.norm(X[:-1]), 1e-300)))
+        solved += 1
+    cells = sum(probs[p][0] ** 2 for p in range(solved))
+    out = dict(value=solved / sec, unit="problems/s", cores=1, kind="reference",
+               sample=f"reference pgmres, one problem after another, the first {solved} of "
+                      f"{nprob} problems ({cells} cells, {steps} Arnoldi steps) in "
+                      f"{sec:.1f} s, -Ofast, one thread",
+               seconds=sec, problems=solved,
+               gpu_fp32_ir_problems_per_s=nprob / gpu_seconds,
+               gpu_fp64_max_rel_diff_vs_cpu_on_sample=err)
+    out.update(cpu_info())
+    return out
+
+
+# ---------------------------------------------------------------------------
+def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
     """3D SUPG steady transport (DIFCONV) on the n^3 cube cut into 6 n^3
     tetrahedra, a seeded random node numbering then reverse Cuthill-McKee;
     FP32 EBE 4x4 products + ILU(0) of the CSR matrix (its triangular solves
     as `sweeps` Jacobi sweeps) inside FP64 refinement (FP32-IR), GMRES(50)
-    to 1e-10, against FP64 GMRES with the same operator and preconditioner."""
+    to 1e-10, against FP64 GMRES with the same operator and preconditioner,
+    and FP64 GMRES with the exact (level-scheduled) ILU(0) solves."""
     t0 = time.perf_counter()
     x, y, z, t, v = F.structured_mesh3d(n)
     nn = len(x)
@@ -270,6 +568,7 @@ def ss3d_leg(F, ctx, n=128, sweeps=4, seed=20260303):
     lm = F.renumber_lm(neq, lm, P)
     nnz, IA, JA, cbe = F.csr_pattern_k(neq, lm, 4)
     bw1 = F.csr_bandwidth(neq, IA, JA)
+    t_struct = time.perf_counter() - t0
     ke = ctx.empty(16 * nel)
     F0 = ctx.empty(neq + 1)
     F0.zero()
@@ -289,20 +588,46 @@ def ss3d_leg(F, ctx, n=128, sweeps=4, seed=20260303):
     pc.setup(B)
     ctx.sync()
     log(f"ss3d: n={n} nel={nel} neq={neq} nnz={nnz} bandwidth {bw0} -> {bw1} (RCM), ILU levels "
-        f"{levels[0]}, setup {time.perf_counter() - t0:.1f}s")
+        f"{levels[0]}, setup {time.perf_counter() - t0:.1f}s (structures {t_struct:.1f}s)")
     rng = np.random.default_rng(1)
     res = {"n": n, "elements": nel, "equations": neq, "nnz": int(nnz),
            "bandwidth_random": int(bw0), "bandwidth_rcm": int(bw1), "ilu_levels": levels[0],
-           "ilu_sweeps": sweeps, "kmax": 50, "tol": 1e-10}
+           "ilu_sweeps": sweeps, "kmax": 50, "tol": 1e-10, "structure_seconds": t_struct}
     mv = {}
+    P64 = ctx.to_device(rng.standard_normal(neq + 1))
+    Q64 = ctx.empty(neq + 1)
+    P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
+    Q32 = ctx.empty(neq + 1, np.float32)
     for nm, op in (("EBE", ebe), ("CSR", csr)):
-        for dt in (np.float64, np.float32):
-            Pd = ctx.to_device(rng.standard_normal(neq + 1).astype(dt))
-            Qd = ctx.empty(neq + 1, dt)
-            ms = op.time_apply(Pd, Qd, reps=10)
-            b = op.algorithmic_bytes(dt == np.float32)
-            mv[f"{nm}_{'f32' if dt == np.float32 else 'f64'}"] = dict(ms=ms, GBs=b / ms / 1e6)
+        mv[nm] = matvec_table(op, P64, Q64, P32, Q32, 10)
     res["matvec"] = mv
+    ms_main = ebe.time_apply(P32, Q32, reps=10, main_only=True)
+    res["roofline"] = roofline(ms_main, ebe.algorithmic_bytes(True), ebe_kernel_name(4, True))
+    # ILU(0) apply: 4 sweeps (the solve's) and the exact level-scheduled solves
+    Z64 = ctx.empty(neq + 1)
+    Z32 = ctx.empty(neq + 1, np.float32)
+    ilu = {}
+    for tag, (Pv, Zv) in (("f64", (P64, Z64)), ("f32", (P32, Z32))):
+        pc.apply(Pv, Zv)
+        ctx.sync()
+        w0 = time.perf_counter()
+        for _ in range(5):
+            pc.apply(Pv, Zv)
+        ctx.sync()
+        ilu[f"sweeps{sweeps}_{tag}_ms"] = (time.perf_counter() - w0) / 5 * 1e3
+    pcx = F.Preconditioner(csr, F.PREC_ILU0)
+    Bx = ctx.to_device(Fn)
+    pcx.setup(Bx)
+    pcx.apply(P64, Z64)
+    ctx.sync()
+    w0 = time.perf_counter()
+    for _ in range(3):
+        pcx.apply(P64, Z64)
+    ctx.sync()
+    exact_ms = (time.perf_counter() - w0) / 3 * 1e3
+    ilu["exact_f64_ms"] = exact_ms
+    res["ilu_apply"] = ilu
+    log("ss3d ILU(0) apply", ilu)
     Xs = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         X = ctx.empty(neq + 1)
@@ -311,16 +636,47 @@ def ss3d_leg(F, ctx, n=128, sweeps=4, seed=20260303):
         info, _, _ = F.gmres(ctx, ebe, pc, B, X, kmax=50, lmax=400, tol=1e-10, precision=prec)
         ctx.sync()
         Xs[tag] = X.get()
-        res[tag] = dict(seconds=time.perf_counter() - w0, steps=info.steps, cycles=info.cycles,
-                        status=info.status, true_rel_res=info.true_res / (info.eps / 1e-10))
+        res[tag] = solve_record(info, time.perf_counter() - w0, 1e-10)
         log("ss3d", tag, res[tag])
     res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
     res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(Xs["fp32_ir"][:neq] - Xs["fp64"][:neq]) /
                                            np.linalg.norm(Xs["fp64"][:neq]))
+    # FP64 with the exact ILU(0) solves (the drop-in's preconditioner): a full
+    # solve when its estimated time fits the cap, else one restart cycle and
+    # the time per step
+    est = 2.2 * exact_ms * 1e-3 * res["fp64"]["steps"]
+    cap_cycles = 400 if est < args.exact_ilu_cap_s else 1
+    X = ctx.empty(neq + 1)
+    ctx.sync()
+    w0 = time.perf_counter()
+    info, _, _ = F.gmres(ctx, ebe, pcx, Bx, X, kmax=50, lmax=cap_cycles, tol=1e-10)
+    ctx.sync()
+    ex = solve_record(info, time.perf_counter() - w0, 1e-10)
+    ex["cycle_cap"] = cap_cycles
+    ex["ms_per_step"] = ex["seconds"] * 1e3 / max(1, info.steps)
+    if cap_cycles == 1:
+        ex["seconds_extrapolated_to_sweep_steps"] = ex["ms_per_step"] * res["fp64"]["steps"] * 1e-3
+    else:
+        ex["rel_diff_vs_sweeps_fp64"] = float(np.linalg.norm(X.get()[:neq] - Xs["fp64"][:neq]) /
+                                              np.linalg.norm(Xs["fp64"][:neq]))
+    res["fp64_exact_ilu"] = ex
+    log("ss3d exact ILU fp64", ex)
+    if not args.no_cpu:
+        import oracle as O
+        if O.Reference3D.available():
+            r = O.Reference3D(fast=True)
+            r.ebemv_setup(neq, lm, ke.get())
+            Ph = rng.standard_normal(neq + 1)
+            b = nel * (16 * 8 + 16) + 2 * neq * 8
+            res["cpu_baseline"] = cpu_product_baseline("ebemv3Dndof1", lambda: r.ebemv(Ph), b,
+                                                       10.0, 40, f"{nel}-tetrahedron mesh")
+            del r
+            log("ss3d cpu", res["cpu_baseline"])
     return res
 
 
-def transient_leg(F, ctx, n=1024, steps=10):
+# ---------------------------------------------------------------------------
+def transient_leg(F, ctx, args, n=1024, steps=10):
     """2D transient transport, the reference's CONE problem (02_Transport_
     Equation/2D/CONE: kappa 1e-8, no source or reaction, boundary value 0,
     velocity the rotation about (5,5), initial state exp(-r^2/2) about
@@ -334,8 +690,9 @@ def transient_leg(F, ctx, n=1024, steps=10):
     x, y, t, v = F.structured_mesh2d(n)
     x, y = 10.0 * x, 10.0 * y
     mesh = F.Mesh2D(ctx, x, y, t, v)
-    neq = mesh.neq
-    op = F.Operator.ebe(ctx, neq, mesh.lm(), 3)
+    neq, nel = mesh.neq, mesh.nel
+    lm = mesh.lm()
+    op = F.Operator.ebe(ctx, neq, lm, 3)
     pc = F.Preconditioner(op, F.PREC_DIAG)
     pb = F.Supg2dProblem(F.P_ROTATE_ABOUT, 1e-8, 0.0, 0.0, 5.0, 5.0, 1.0, F.H_2SQRTAREA)
     up = ctx.to_device(np.zeros(len(x)))
@@ -344,7 +701,7 @@ def transient_leg(F, ctx, n=1024, steps=10):
     free = t > 0
     u0[idv[free]] = np.exp(-0.5 * ((x[free] - 5.0) ** 2 + (y[free] - 7.5) ** 2))
     dt = 1e-3
-    res = {"n": n, "equations": neq, "elements": mesh.nel, "steps": steps, "dt": dt,
+    res = {"n": n, "equations": neq, "elements": nel, "steps": steps, "dt": dt,
            "problem": "CONE", "precond": "Diag", "kmax": 30, "tol": 1e-8}
     U = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
@@ -366,10 +723,32 @@ def transient_leg(F, ctx, n=1024, steps=10):
     res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
     res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(U["fp32_ir"][:neq] - U["fp64"][:neq]) /
                                            np.linalg.norm(U["fp64"][:neq]))
+    # the product inside every Arnoldi step (the element matrices of the last
+    # correction)
+    rng = np.random.default_rng(3)
+    P64 = ctx.to_device(rng.standard_normal(neq + 1))
+    Q64 = ctx.empty(neq + 1)
+    P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
+    Q32 = ctx.empty(neq + 1, np.float32)
+    res["matvec"] = matvec_table(op, P64, Q64, P32, Q32, 20)
+    ms_main = op.time_apply(P64, Q64, reps=20, main_only=True)
+    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(False), ebe_kernel_name(3, False))
+    if not args.no_cpu:
+        import oracle as O
+        if O.Reference.available():
+            r = O.Reference(fast=True)
+            r.setup(x, y, t, v, "EBE")
+            r.set_matrix(op.get_values())
+            Ph = rng.standard_normal(neq + 1)
+            b = nel * (9 * 8 + 12) + 2 * neq * 8
+            res["cpu_baseline"] = cpu_product_baseline("ebemv", lambda: r.matvec(Ph), b, 6.0, 200,
+                                                       f"{nel}-element mesh")
+            log("transient cpu", res["cpu_baseline"])
     return res
 
 
-def euler_leg(F, ctx, n=1024, seed=20260305):
+# ---------------------------------------------------------------------------
+def euler_leg(F, ctx, args, n=1024, seed=20260305):
     """2D Euler: the n x n grid of P1 triangles, 4 conserved variables per
     node, a uniform Mach-3 inflow (rho 1 with a seeded 1% perturbation, u 1,
     v 0, c 1/3), every variable prescribed on the inflow side x = 0; one
@@ -417,16 +796,13 @@ def euler_leg(F, ctx, n=1024, seed=20260305):
     res = {"n": n, "elements": nel, "equations": neq, "element_build_ms": min(bt) * 1e3,
            "element_matrix_bytes": 144 * 8 * nel, "precond": "BlockDiag", "kmax": 30,
            "tol": 1e-8}
-    mv = {}
-    for dt in (np.float64, np.float32):
-        Pd = ctx.to_device(rng.standard_normal(neq + 1).astype(dt))
-        Qd = ctx.empty(neq + 1, dt)
-        ms = op.time_apply(Pd, Qd, reps=20)
-        b = op.algorithmic_bytes(dt == np.float32)
-        mv["f32" if dt == np.float32 else "f64"] = dict(ms=ms, GBs=b / ms / 1e6,
-                                                        pct_peak=100 * b / ms / 1e6 / PEAK_GBS,
-                                                        bytes=b)
-    res["matvec_ebe12"] = mv
+    P64 = ctx.to_device(rng.standard_normal(neq + 1))
+    Q64 = ctx.empty(neq + 1)
+    P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
+    Q32 = ctx.empty(neq + 1, np.float32)
+    res["matvec_ebe12"] = matvec_table(op, P64, Q64, P32, Q32, 20)
+    ms_main = op.time_apply(P32, Q32, reps=20, main_only=True)
+    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True), ebe_kernel_name(12, True))
     Xs = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         B = ctx.to_device(Rh)
@@ -437,12 +813,22 @@ def euler_leg(F, ctx, n=1024, seed=20260305):
         info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=30, lmax=2000, tol=1e-8, precision=prec)
         ctx.sync()
         Xs[tag] = X.get()
-        res[tag] = dict(seconds=time.perf_counter() - w0, steps=info.steps, cycles=info.cycles,
-                        status=info.status, true_rel_res=info.true_res / (info.eps / 1e-8))
+        res[tag] = solve_record(info, time.perf_counter() - w0, 1e-8)
         log("euler", tag, res[tag])
     res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
     res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(Xs["fp32_ir"][:neq] - Xs["fp64"][:neq]) /
                                            max(np.linalg.norm(Xs["fp64"][:neq]), 1e-300))
+    if not args.no_cpu:
+        import oracle as O
+        if O.ReferenceEuler.available():
+            r = O.ReferenceEuler(fast=True)
+            r.setup(nn, nel, neq, v, ids32, lm, A.get())
+            Ph = rng.standard_normal(neq + 1)
+            b = nel * (144 * 8 + 48) + 2 * neq * 8
+            res["cpu_baseline"] = cpu_product_baseline("ebemvNDOF4", lambda: r.ebemv(Ph), b, 8.0,
+                                                       40, f"{nel}-element mesh")
+            del r
+            log("euler cpu", res["cpu_baseline"])
     # the edge-by-edge scheme (EDE): the same system as 8x8 edge matrices
     t1 = time.perf_counter()
     g = F.Ede4(ctx, nn, v, ids32, neq)
@@ -462,16 +848,7 @@ def euler_leg(F, ctx, n=1024, seed=20260305):
         at.append(time.perf_counter() - w0)
     ede["edge_assembly_ms"] = min(at) * 1e3
     op8.set_values(Aede)
-    mv8 = {}
-    for dt in (np.float64, np.float32):
-        Pd = ctx.to_device(rng.standard_normal(neq + 1).astype(dt))
-        Qd = ctx.empty(neq + 1, dt)
-        ms = op8.time_apply(Pd, Qd, reps=20)
-        b = op8.algorithmic_bytes(dt == np.float32)
-        mv8["f32" if dt == np.float32 else "f64"] = dict(ms=ms, GBs=b / ms / 1e6,
-                                                         pct_peak=100 * b / ms / 1e6 / PEAK_GBS,
-                                                         bytes=b)
-    ede["matvec_ede8"] = mv8
+    ede["matvec_ede8"] = matvec_table(op8, P64, Q64, P32, Q32, 20)
     B = ctx.to_device(Rh)
     pc8.setup(B)
     X = ctx.empty(neq + 1)
@@ -482,7 +859,7 @@ def euler_leg(F, ctx, n=1024, seed=20260305):
     ede["fp64"] = dict(seconds=time.perf_counter() - w0, steps=info.steps, status=info.status)
     ede["rel_diff_vs_ebe_fp64"] = float(np.linalg.norm(X.get()[:neq] - Xs["fp64"][:neq]) /
                                         max(np.linalg.norm(Xs["fp64"][:neq]), 1e-300))
-    log("euler ede", ede)
+    log("euler ede", {k: v for k, v in ede.items() if k != "matvec_ede8"})
     res["ede"] = ede
     g.close()
     return res
@@ -495,207 +872,91 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=20260202)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--legs", default="all",
+                    help="comma-separated subset of " + ",".join(LEGS) + " (default all)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-solve", action="store_true", help="headline leg: products only")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched leg: skip the one-by-one comparison")
     ap.add_argument("--engine", default="kernels", choices=sorted(ENGINES),
-                    help="GMRES engine for the time-to-solve runs")
-    ap.add_argument("--prec", default="jacobi", choices=("diag", "jacobi"),
-                    help="preconditioner of the time-to-solve runs")
+                    help="GMRES engine for the 2048^2 time-to-solve runs")
+    ap.add_argument("--orth", default="cgs", choices=sorted(ORTHS),
+                    help="orthogonalisation of the FP32-IR inner cycle (FP64 is always MGS)")
+    ap.add_argument("--rot-cycles", type=int, default=40,
+                    help="restart cycles allowed to the stagnating rotating-flow solve")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the products, run a cast kernel of known traffic (64Mi "
                          "doubles read, 64Mi floats written) three times, the scale of "
                          "the FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--orth", default="cgs", choices=sorted(ORTHS),
-                    help="orthogonalisation of the FP32-IR inner cycle (FP64 is always MGS)")
-    ap.add_argument("--batched", type=int, default=0, metavar="NPROB",
-                    help="also run the batched leg: NPROB independent problems of "
-                         "32..512 cells per side (256 = the batched workload)")
-    ap.add_argument("--ss3d", type=int, default=0, metavar="N",
-                    help="also run the 3D leg on the N^3 Kuhn cube (128 = the 3D workload)")
-    ap.add_argument("--transient", type=int, default=0, metavar="N",
-                    help="also run the transient transport leg (Predictor_Old) on an N^2 grid")
-    ap.add_argument("--euler", type=int, default=0, metavar="N",
-                    help="also run the Euler leg on the N x N grid (1024 = the Euler workload)")
+    ap.add_argument("--batched", type=int, default=256, metavar="NPROB")
+    ap.add_argument("--ss3d", type=int, default=128, metavar="N")
+    ap.add_argument("--exact-ilu-cap-s", type=float, default=40.0,
+                    help="3D leg: the exact-ILU(0) FP64 solve runs in full when its estimate "
+                         "is below this, else one restart cycle")
+    ap.add_argument("--transient", type=int, default=1024, metavar="N")
+    ap.add_argument("--euler", type=int, default=1024, metavar="N")
     ap.add_argument("--matvec-only", action="store_true",
-                    help="only the EBE FP64 product (for profiling runs)")
+                    help="only the headline EBE products (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
-              "(one GPU per process; the sharding over several GPUs is a design only)", file=sys.stderr)
+              "(one GPU per process; the sharding over several GPUs is a design only)",
+              file=sys.stderr)
         sys.exit(2)
+    legs = set(LEGS) if args.legs == "all" else set(args.legs.split(","))
+    unknown = legs - set(LEGS)
+    if unknown:
+        ap.error(f"unknown legs {sorted(unknown)}")
+    if args.matvec_only:
+        legs = {"headline"}
+    legs.add("headline")   # `value` is the headline product
 
     import fem_codes_amd as F
     ctx = F.Context(0)
     rng = np.random.default_rng(args.seed)
-    scale = float(rng.uniform(0.5, 2.0))
-    kappa = float(10.0 ** rng.uniform(-6.0, -3.0))
-    theta = float(rng.uniform(0.0, 2 * np.pi))
+    t_start = time.perf_counter()
+    head, sysd = headline_leg(F, ctx, args, rng)
+    theta = head["theta"]
+    # free the 2048^2 system before the other legs
+    for op in sysd["ops"].values():
+        op.close()
+    sysd["mesh"].close()
+    del sysd
+    ctx.trim()
+    small = small_leg(F, ctx, args, theta) if "small" in legs else None
+    batched = batched_leg(F, ctx, args, args.batched, args.seed + 4) if "batched" in legs else None
+    ctx.trim()
+    ss3d = ss3d_leg(F, ctx, args, args.ss3d) if "ss3d" in legs else None
+    ctx.trim()
+    euler = euler_leg(F, ctx, args, args.euler) if "euler" in legs else None
+    ctx.trim()
+    transient = transient_leg(F, ctx, args, args.transient) if "transient" in legs else None
+    log(f"total {time.perf_counter() - t_start:.1f}s")
+
     n = args.n
-    t0 = time.perf_counter()
-    rot = F.Supg2dProblem(F.P_ROTATING, kappa, 0.0, 1.0, 0.0, 0.0, scale, F.H_2SQRTAREA)
-    schemes = ("EBE",) if args.matvec_only else ("EBE", "EDE", "CSR")
-    sysd = setup_system(F, ctx, n, rot, schemes)
-    ctx.sync()
-    log(f"setup n={n} neq={sysd['neq']} nel={sysd['nel']} in {time.perf_counter() - t0:.1f}s")
-
-    neq = sysd["neq"]
-    P64 = ctx.to_device(rng.standard_normal(neq + 1))
-    Q64 = ctx.empty(neq + 1)
-    P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
-    Q32 = ctx.empty(neq + 1, np.float32)
-    # elements per block of the headline product: the autotuner's choice
-    from fem_codes_amd import autotune
-    (best_epb, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(ctx, neq, sysd["lm"], 3,
-                                                                  sysd["ke"])
-    log("autotune EBE f64 ms by (elements/block, workgroups/CU):",
-        {k: round(v, 4) for k, v in tune_ms.items()}, "->", (best_epb, best_pipe))
-    sysd["ops"]["EBE_default"] = sysd["ops"]["EBE"]
-    sysd["ops"]["EBE"] = ebe_tuned
-    ebe = ebe_tuned
-    for _ in range(args.warmup):
-        ebe.apply(P64, Q64)
-    ctx.sync()
-    w0 = time.perf_counter()
-    ms = ebe.time_apply(P64, Q64, reps=args.steps)
-    ctx.sync()
-    wall = time.perf_counter() - w0
-    ms_main = ebe.time_apply(P64, Q64, reps=args.steps, main_only=True)
-    bytes64 = ebe.algorithmic_bytes(False)
-    value = bytes64 / (ms * 1e-3) / 1e9
-    achieved = bytes64 / (ms_main * 1e-3) / 1e9
-    log(f"EBE f64: {ms:.4f} ms/matvec (block kernel {ms_main:.4f} ms) -> {value:.1f} GB/s")
-
-    matvec = {}
-    for name, op in sysd["ops"].items():
-        r = {}
-        for fp32, (P, Q) in ((False, (P64, Q64)), (True, (P32, Q32))):
-            t = op.time_apply(P, Q, reps=args.steps)
-            b = op.algorithmic_bytes(fp32)
-            r["f32" if fp32 else "f64"] = dict(ms=t, GBs=b / (t * 1e-3) / 1e9,
-                                              pct_peak=100 * b / (t * 1e-3) / 1e9 / PEAK_GBS,
-                                              bytes=b)
-        matvec[name] = r
-        log(name, {k: round(v["GBs"], 1) for k, v in r.items()})
-
-    copy = None
-    if args.calibrate_copy:
-        m = 64 << 20
-        X = ctx.to_device(np.ones(m))
-        Y = ctx.empty(m, np.float32)
-        ctx.sync()
-        c0 = time.perf_counter()
-        for _ in range(3):
-            F.lib.fg_cast_f64_to_f32(ctx.h, m, X.ptr, Y.ptr)
-        ctx.sync()
-        cs = (time.perf_counter() - c0) / 3
-        copy = dict(bytes=12 * m, ms=cs * 1e3, GBs=12 * m / cs / 1e9)
-        log("copy calibration", copy)
-        del X, Y
-
-    # HBM bytes per launch from the committed counter runs, when they measured
-    # the kernel this run chose
-    kernel = (f"k_ebe_apply_pipe<3, {best_epb // 256}, double, 0>" if best_pipe
-              else f"k_ebe_apply<3, {best_epb // 256}, double, 0>")
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "ebe_f64_2048_traffic.json")
-    if os.path.exists(tpath) and n == 2048:
-        with open(tpath) as f:
-            tj = json.load(f)
-        if tj.get("kernel") == kernel:
-            traffic = tj.get("hbm_bytes_per_launch")
-
-    solve = {}
-    if not args.no_solve and not args.matvec_only:
-        const = F.Supg2dProblem(F.P_CONST, 1e-4, 0.0, 1.0, float(np.cos(theta)),
-                                float(np.sin(theta)), 1.0, F.H_2SQRTAREA)
-        del sysd["ops"]["EDE"], sysd["ops"]["CSR"]
-        sysd["mesh"].build_supg(const, sysd["ke"], sysd["F"])
-        sysd["ops"]["EBE"].set_values(sysd["ke"])
-        del sysd["ops"]["EBE_default"]
-        res = {"engine": args.engine, "precond": args.prec, "fp32_ir_orth": args.orth}
-        X64 = X32 = None
-        for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
-            info, wall_s, X = time_solve(F, ctx, sysd, prec, ENGINES[args.engine],
-                                         prec=args.prec, orth=ORTHS[args.orth])
-            res[tag] = dict(seconds=wall_s, gpu_ms=info.ms, steps=info.steps, cycles=info.cycles,
-                            true_rel_res=info.true_res / (info.eps / 1e-8), status=info.status)
-            if tag == "fp64":
-                X64 = X.get()
-            else:
-                X32 = X.get()
-            log(tag, res[tag])
-        res["speedup_fp32_ir"] = res["fp64"]["seconds"] / res["fp32_ir"]["seconds"]
-        res["fp32_ir_rel_err_vs_fp64"] = float(np.linalg.norm(X32[:neq] - X64[:neq]) /
-                                               np.linalg.norm(X64[:neq]))
-        solve[f"const_{n}"] = res
-        # 64^2, the reference's own case (3,969 equations): Diag, GMRES(30)
-        # to 1e-8, FP64 on each engine (auto = the one-workgroup engine at
-        # this size), EBE and CSR; the reference pgmres on one host core
-        small = setup_system(F, ctx, 64, const, ("EBE", "CSR"))
-        sres = {"precond": "diag"}
-        runs = [(sch, F.GMRES_FP64, eng) for sch in ("EBE", "CSR") for eng in ("auto", "kernels")]
-        runs.append(("EBE", F.GMRES_FP32_IR, "kernels"))
-        for sch, prec, eng in runs:
-            tag = f"{sch}_{'fp64' if prec == F.GMRES_FP64 else 'fp32_ir'}_{eng}"
-            best = None
-            for _ in range(3):   # first call of an engine includes its code-object load
-                info, wall_s, X = time_solve(F, ctx, small, prec, ENGINES[eng], prec="diag",
-                                             orth=ORTHS[args.orth], scheme=sch)
-                best = wall_s if best is None else min(best, wall_s)
-            sres[tag] = dict(seconds=best, steps=info.steps, cycles=info.cycles,
-                             status=info.status, X=X.get())
-        if not args.no_cpu:
-            cs, Xc, _, _ = cpu_solve_small(64, (0, 1e-4, 0.0, 1.0, float(np.cos(theta)),
-                                                float(np.sin(theta)), 1.0, 0))
-            sres["cpu"] = cs
-            for k, r in sres.items():
-                if isinstance(r, dict) and "X" in r:
-                    Xg = r.pop("X")
-                    r["rel_err_vs_cpu"] = float(np.linalg.norm(Xg[:-1] - Xc[:-1]) /
-                                                np.linalg.norm(Xc[:-1]))
-                    r["speedup_vs_cpu"] = cs["seconds"] / r["seconds"]
-        else:
-            for r in sres.values():
-                if isinstance(r, dict):
-                    r.pop("X", None)
-        solve["const_64"] = sres
-        log("64^2", sres)
-
-    batched = None
-    if args.batched > 0:
-        batched = batched_leg(F, ctx, args.batched, args.seed + 4, prec=args.prec)
-    ss3d = ss3d_leg(F, ctx, args.ss3d) if args.ss3d > 0 else None
-    euler = euler_leg(F, ctx, args.euler) if args.euler > 0 else None
-    transient = transient_leg(F, ctx, args.transient) if args.transient > 0 else None
-
-    cpu = None
-    if not args.no_cpu and not args.matvec_only:
-        if "ke" in sysd:
-            sysd["mesh"].build_supg(rot, sysd["ke"], None)
-        cpu = cpu_baseline(sysd, sysd["ke"].get())
-        log("cpu", cpu)
-
     out = {
-        "metric": METRIC, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "metric": METRIC, "value": head["value"], "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
         "config": {"workload": f"2D SUPG transport, {n}x{n} grid of P1 triangles "
-                               f"({sysd['nel']} elements, {neq} equations), EBE 3x3 FP64 matvec",
-                   "velocity_scale": scale, "diffusivity": kappa, "pct_of_peak": 100 * value / PEAK_GBS,
-                   "wall_ms_per_step": wall * 1e3 / args.steps},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_GBS, "traffic": traffic,
-                     "kernel": kernel,
-                     "schedule": {"elems_per_block": best_epb, "wg_per_cu": best_pipe},
-                     "algorithmic_bytes": bytes64, "ms": ms_main},
-        "cpu_baseline": cpu,
-        "matvec": matvec,
-        "copy_calibration": copy,
-        "ebe_autotune_ms": {f"{k[0]}/{k[1]}": v for k, v in tune_ms.items()},
-        "solve": solve,
+                               f"({head['nel']} elements, {head['neq']} equations), EBE 3x3 "
+                               f"FP64 matvec",
+                   "velocity_scale": head["scale"], "diffusivity": head["kappa"],
+                   "pct_of_peak": 100 * head["value"] / PEAK_GBS,
+                   "wall_ms_per_step": head["wall_ms_per_step"]},
+        "roofline": head["roofline"],
+        "cpu_baseline": head.get("cpu_baseline"),
+        "matvec": head["matvec"],
+        "copy_calibration": head.get("copy_calibration"),
+        "ebe_autotune_ms": head["ebe_autotune_ms"],
+        "solve": head.get("solve"),
+        "small": small,
         "batched": batched,
         "ss3d": ss3d,
-        "euler": euler, "transient": transient,
+        "euler": euler,
+        "transient": transient,
+        "host": cpu_info(),
     }
     print(json.dumps(out))
 

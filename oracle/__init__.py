@@ -680,14 +680,17 @@ class Reference3D:
     def available(cls) -> bool:
         return os.path.exists(cls.PATH)
 
-    def __init__(self):
-        L = self.lib = C.CDLL(self.PATH)
+    def __init__(self, fast: bool = False):
+        path = self.PATH.replace(".so", "_fast.so") if fast else self.PATH
+        L = self.lib = C.CDLL(path)
         L.ref3d_setup.restype = _i
         L.ref3d_setup.argtypes = [_i, _i, _dp, _dp, _dp, _ip, _ip, _d]
         L.ref3d_lm.argtypes = [_ip]
         L.ref3d_build.argtypes = [_dp, _dp]
         L.ref3d_ilu0_setup.argtypes = [_i, _i, _dp, _ip, _ip, _dp, _dp, _dp]
         L.ref3d_ilu0_apply.argtypes = [_dp, _dp]
+        L.ref3d_ebemv_setup.argtypes = [_i, _i, _ip, _dp]
+        L.ref3d_ebemv.argtypes = [_dp, _dp]
 
     def setup(self, x, y, z, utype, vertex, cst=1.0):
         self.nel = len(vertex) // 4
@@ -721,6 +724,17 @@ class Reference3D:
         self.lib.ref3d_ilu0_apply(np.ascontiguousarray(p, np.float64).copy(), z)
         return z
 
+    def ebemv_setup(self, neq, lm, A):
+        """the reference ebemv3Dndof1 on these element matrices (nel*16)"""
+        self.neq_ebe = neq
+        self.lib.ref3d_ebemv_setup(neq, len(lm) // 4, np.ascontiguousarray(lm, np.int32),
+                                   np.ascontiguousarray(A, np.float64))
+
+    def ebemv(self, P):
+        Q = np.zeros(self.neq_ebe + 1)
+        self.lib.ref3d_ebemv(np.ascontiguousarray(P, np.float64).copy(), Q)
+        return Q
+
 
 class ReferenceEuler:
     """The reference's 4-dof (Euler 2D) element products and node block-
@@ -732,8 +746,9 @@ class ReferenceEuler:
     def available(cls) -> bool:
         return os.path.exists(cls.PATH)
 
-    def __init__(self):
-        L = self.lib = C.CDLL(self.PATH)
+    def __init__(self, fast: bool = False):
+        path = self.PATH.replace(".so", "_fast.so") if fast else self.PATH
+        L = self.lib = C.CDLL(path)
         L.refeu_setup.argtypes = [_i, _i, _i, _ip, _ip, _ip, _dp, C.c_char_p]
         L.refeu_ebemv.argtypes = [_dp, _dp]
         L.refeu_ebe2mv.argtypes = [_dp, _dp]

@@ -125,3 +125,31 @@ void ref3d_ilu0_apply(double *p, double *z)
 {
     ILUp_precond(&PI_, &MI, NULL, p, z);
 }
+
+/* EBE/ebemv3Dndof1.c:3-45 on caller arrays: lm[nel*4] and the element
+ * matrices A[nel*16] laid out as Preprocess.c:240-262 (row pointers into one
+ * block); Q = A P, P and Q of neq+1 entries */
+int ebemv3Dndof1(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
+static ParametersType PE3;
+static MatrixDataType ME3;
+static FemStructsType FE3;
+void ref3d_ebemv_setup(int neq, int nel, const int *lm, const double *A)
+{
+    memset(&PE3, 0, sizeof PE3); memset(&ME3, 0, sizeof ME3); memset(&FE3, 0, sizeof FE3);
+    PE3.neq = neq;
+    PE3.nel = nel;
+    int **L = (int **)calloc(nel, sizeof(int *));
+    int *Laux = (int *)calloc((size_t)nel * 4, sizeof(int));
+    memcpy(Laux, lm, sizeof(int) * 4 * (size_t)nel);
+    for (int e = 0; e < nel; e++) L[e] = &Laux[4 * e];
+    double **Ar = (double **)calloc(nel, sizeof(double *));
+    double *Aaux = (double *)calloc((size_t)nel * 16, sizeof(double));
+    memcpy(Aaux, A, sizeof(double) * 16 * (size_t)nel);
+    for (int e = 0; e < nel; e++) Ar[e] = &Aaux[16 * e];
+    FE3.lm = L;
+    FE3.lmaux = Laux;
+    ME3.A = Ar;
+    ME3.Aaux = Aaux;
+}
+
+void ref3d_ebemv(double *P, double *Q) { ebemv3Dndof1(&PE3, &ME3, &FE3, P, Q); }
