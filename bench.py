@@ -201,11 +201,16 @@ def headline_leg(F, ctx, args, rng):
     Q64 = ctx.empty(neq + 1)
     P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
     Q32 = ctx.empty(neq + 1, np.float32)
-    from fem_codes_amd import autotune
-    (best_epb, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(ctx, neq, sysd["lm"], 3,
-                                                                  sysd["ke"])
-    log("autotune EBE f64 ms by (elements/block, workgroups/CU):",
-        {k: round(v, 4) for k, v in tune_ms.items()}, "->", (best_epb, best_pipe))
+    if args.no_autotune:
+        best_epb, best_pipe, tune_ms = 512, 0, {}
+        ebe_tuned = F.Operator.ebe(ctx, neq, sysd["lm"], 3, best_epb)
+        ebe_tuned.set_values(sysd["ke"])
+    else:
+        from fem_codes_amd import autotune
+        (best_epb, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(ctx, neq, sysd["lm"], 3,
+                                                                      sysd["ke"])
+        log("autotune EBE f64 ms by (elements/block, workgroups/CU):",
+            {k: round(v, 4) for k, v in tune_ms.items()}, "->", (best_epb, best_pipe))
     default_op = sysd["ops"]["EBE"]
     sysd["ops"]["EBE"] = ebe_tuned
     ebe = ebe_tuned
@@ -446,7 +451,7 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
     res = {"problems": int(nprob), "elements": int(nel), "equations": N,
            "sizes": {"min": int(sizes.min()), "max": int(sizes.max()),
                      "mean_cells_per_side": float(sizes.mean())},
-           "precond": "diag", "kmax": kmax, "tol": tol}
+           "precond": "diag", "kmax": kmax, "tol": tol, "graph_replay": not args.no_graph}
     # the batched product over every problem at once: the dominant kernel
     P32 = ctx.to_device(rng.standard_normal(N + 1).astype(np.float32))
     Q32 = ctx.empty(N + 1, np.float32)
@@ -461,7 +466,7 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
         ctx.sync()
         w0 = time.perf_counter()
         rc, infos = F.gmres_batched(ctx, op, pc, seg, B, X, kmax=kmax, lmax=100000, tol=tol,
-                                    precision=precision, graph=True)
+                                    precision=precision, graph=not args.no_graph)
         ctx.sync()
         wall = time.perf_counter() - w0
         Xs[tag] = X.get()
@@ -878,6 +883,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="headline leg: products only")
     ap.add_argument("--no-sequential", action="store_true",
                     help="batched leg: skip the one-by-one comparison")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched leg: launch every cycle's kernels directly instead of "
+                         "replaying a captured hipGraph (kernel-trace profiling runs)")
     ap.add_argument("--engine", default="kernels", choices=sorted(ENGINES),
                     help="GMRES engine for the 2048^2 time-to-solve runs")
     ap.add_argument("--orth", default="cgs", choices=sorted(ORTHS),
@@ -897,6 +905,11 @@ def main():
     ap.add_argument("--euler", type=int, default=1024, metavar="N")
     ap.add_argument("--matvec-only", action="store_true",
                     help="only the headline EBE products (for profiling runs)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="headline leg: the default 512-element blocks, no tuning sweep")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="profiling runs of the other legs: skip the headline leg (the JSON "
+                         "line then has no value)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
@@ -909,20 +922,27 @@ def main():
         ap.error(f"unknown legs {sorted(unknown)}")
     if args.matvec_only:
         legs = {"headline"}
-    legs.add("headline")   # `value` is the headline product
+    if not args.no_headline:
+        legs.add("headline")   # `value` is the headline product
 
     import fem_codes_amd as F
     ctx = F.Context(0)
     rng = np.random.default_rng(args.seed)
     t_start = time.perf_counter()
-    head, sysd = headline_leg(F, ctx, args, rng)
+    if "headline" in legs:
+        head, sysd = headline_leg(F, ctx, args, rng)
+        # free the 2048^2 system before the other legs
+        for op in sysd["ops"].values():
+            op.close()
+        sysd["mesh"].close()
+        del sysd
+        ctx.trim()
+    else:
+        rng.uniform(0.5, 2.0), rng.uniform(-6.0, -3.0)
+        head = {"theta": float(rng.uniform(0.0, 2 * np.pi)), "value": None, "ms": None,
+                "nel": None, "neq": None, "scale": None, "kappa": None, "roofline": None,
+                "matvec": None, "ebe_autotune_ms": None, "wall_ms_per_step": None}
     theta = head["theta"]
-    # free the 2048^2 system before the other legs
-    for op in sysd["ops"].values():
-        op.close()
-    sysd["mesh"].close()
-    del sysd
-    ctx.trim()
     small = small_leg(F, ctx, args, theta) if "small" in legs else None
     batched = batched_leg(F, ctx, args, args.batched, args.seed + 4) if "batched" in legs else None
     ctx.trim()
@@ -943,7 +963,7 @@ def main():
                                f"({head['nel']} elements, {head['neq']} equations), EBE 3x3 "
                                f"FP64 matvec",
                    "velocity_scale": head["scale"], "diffusivity": head["kappa"],
-                   "pct_of_peak": 100 * head["value"] / PEAK_GBS,
+                   "pct_of_peak": 100 * head["value"] / PEAK_GBS if head["value"] else None,
                    "wall_ms_per_step": head["wall_ms_per_step"]},
         "roofline": head["roofline"],
         "cpu_baseline": head.get("cpu_baseline"),

@@ -1,0 +1,356 @@
+"""The benchmark workloads at their full sizes on the GPU, against the oracle
+where the oracle finishes in seconds and through size-independent properties
+elsewhere.
+
+* 64 x 64 (3,969 equations), CSR, Diag, GMRES(30) to 1e-8: the oracle's
+  pgmres restatement gives the same step count and X within 1e-12.
+* 2048 x 2048 (8,388,608 triangles, 4,190,209 equations), rotating-flow
+  SUPG system: the FP64 EBE product within 1e-14 of the oracle's ebemv
+  (block-ordered halo sums), EBE = EDE = CSR within 1e-13; on the
+  constant-velocity system FP64 and FP32-IR GMRES(30) reach the true
+  residual 1e-8 (1.05e-8 allowed for the FP32 copy's last cycle) and agree
+  within 1e-6 relative.
+* 128^3 Kuhn cube (12,582,912 tetrahedra), seeded random numbering then RCM:
+  the device ILU(0) factors and exact solves bit for bit the oracle's, the
+  4x4 EBE product within 1e-13 of the CSR product, FP64 and FP32-IR
+  GMRES(50) with the sweep ILU(0) reach 1e-10 and agree within 1e-6.
+* 256 batched problems (32..512 cells per side): every problem converges
+  in FP64 and FP32-IR, and each FP32-IR solution lies within 1e-6 of its own
+  FP64 solution.
+* Euler 1024^2 (2,097,152 triangles, 4 dofs per node): the 12x12 element
+  product equals the 8x8 edge product within 1e-13 (same matrix, other
+  storage); the block-diagonal GMRES converges with both schemes, the two
+  solutions agree within 1e-12, and FP32-IR lies within 1e-6 of FP64.
+"""
+import numpy as np
+import pytest
+
+import fem_codes_amd as F
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"\n[fullsize] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = F.Context(0)
+    yield c
+    c.trim()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _nrel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _solve(ctx, op, kind, Fv, neq, precision, kmax=30, tol=1e-8, lmax=100000, orth=F.ORTH_MGS,
+           pc=None):
+    if pc is None:
+        pc = F.Preconditioner(op, kind)
+        B = ctx.to_device(Fv)
+        pc.setup(B)
+    else:
+        B = ctx.to_device(Fv)
+    X = ctx.empty(neq + 1)
+    info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=kmax, lmax=lmax, tol=tol, precision=precision,
+                         orth=orth if precision == F.GMRES_FP32_IR else F.ORTH_MGS)
+    return info, X.get()
+
+
+def _true_rel(info, tol):
+    return info.true_res / (info.eps / tol)
+
+
+# ---------------------------------------------------------------------------
+def test_64x64_csr_diag_against_oracle(ctx):
+    say("64^2 CSR Diag GMRES(30) vs the oracle's pgmres")
+    o = O.Oracle()
+    th = 0.9
+    x, y, t, v = F.structured_mesh2d(64)
+    mesh = F.Mesh2D(ctx, x, y, t, v)
+    neq, nel = mesh.neq, mesh.nel
+    pb = F.Supg2dProblem(F.P_CONST, 1e-4, 0.0, 1.0, float(np.cos(th)), float(np.sin(th)), 1.0,
+                         F.H_2SQRTAREA)
+    ke, Fd = ctx.empty(9 * nel), ctx.empty(neq + 1)
+    Fd.zero()
+    mesh.build_supg(pb, ke, Fd)
+    lm = mesh.lm()
+    nnz, IA, JA, cbe = F.csr_pattern(neq, lm)
+    csr = F.Operator.csr(ctx, neq, IA, JA)
+    F.assemble_csr(csr, cbe, ke)
+    _, idv = o.number_nodes(t)
+    keo, Fo = o.build_supg2d(x, y, t, idv, v, neq, np.zeros(neq + 1),
+                             O.prob2d(0, 1e-4, 0.0, 1.0, float(np.cos(th)), float(np.sin(th)),
+                                      1.0, 0))
+    AA = o.csr_assembly(nel, nnz, keo, cbe)
+    D, iD, Fs = o.diag_setup_csr(neq, AA, JA, IA, Fo)
+    Xo, steps, _, rho, _ = o.pgmres(neq, 30, 100000, 1e-8, Fs, "CSR", (AA, JA, IA), invDiag=iD)
+    for engine in (F.ENGINE_AUTO, F.ENGINE_KERNELS):
+        pc = F.Preconditioner(csr, F.PREC_DIAG)
+        B = ctx.to_device(Fo)
+        pc.setup(B)
+        assert np.array_equal(B.get()[:neq], Fs[:neq])        # Diag_precond_CSR_setup bits
+        X = ctx.empty(neq + 1)
+        info, hist, _ = F.gmres(ctx, csr, pc, B, X, kmax=30, lmax=100000, tol=1e-8,
+                                engine=engine, hist_cap=steps + 1)
+        assert info.status == F.FG_OK and info.steps == steps, (engine, info.steps, steps)
+        assert _rel(X.get()[:neq], Xo[:neq]) <= 1e-12
+        k = min(len(hist), 30)
+        assert np.all(np.abs(hist[:k] - rho[:k]) <= 1e-12 * rho[:k] + 1e-14 * rho[0])
+
+
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def sys2048(ctx):
+    say("2048^2 setup (rotating flow, EBE/EDE/CSR)")
+    n = 2048
+    rng = np.random.default_rng(20260202)
+    scale = float(rng.uniform(0.5, 2.0))
+    kappa = float(10.0 ** rng.uniform(-6.0, -3.0))
+    x, y, t, v = F.structured_mesh2d(n)
+    mesh = F.Mesh2D(ctx, x, y, t, v)
+    neq, nel = mesh.neq, mesh.nel
+    assert (neq, nel) == (4190209, 8388608)
+    lm = mesh.lm()
+    ke, Fd = ctx.empty(9 * nel), ctx.empty(neq + 1)
+    Fd.zero()
+    mesh.build_supg(F.Supg2dProblem(F.P_ROTATING, kappa, 0.0, 1.0, 0.0, 0.0, scale,
+                                    F.H_2SQRTAREA), ke, Fd)
+    ebe = F.Operator.ebe(ctx, neq, lm, 3)
+    ebe.set_values(ke)
+    nnz, IA, JA, cbe = F.csr_pattern(neq, lm)
+    assert nnz == 29315089
+    csr = F.Operator.csr(ctx, neq, IA, JA)
+    F.assemble_csr(csr, cbe, ke)
+    nedge, order, e2e, lm2 = F.ede_pattern(neq, lm)
+    assert nedge == 12570624
+    ede = F.Operator.ede(ctx, neq, lm2)
+    F.assemble_ede(ede, order, e2e, ke)
+    d = dict(mesh=mesh, neq=neq, nel=nel, lm=lm, ke=ke, F=Fd, ebe=ebe, csr=csr, ede=ede,
+             cbe=cbe, order=order, e2e=e2e, theta=float(rng.uniform(0.0, 2 * np.pi)))
+    yield d
+    for k in ("ebe", "csr", "ede", "mesh"):
+        d[k].close()
+
+
+def test_2048_products_against_oracle(ctx, sys2048):
+    s = sys2048
+    neq, nel = s["neq"], s["nel"]
+    say("2048^2 FP64 products: EBE vs the oracle's ebemv, EBE = EDE = CSR")
+    P = np.random.default_rng(11).standard_normal(neq + 1)
+    Q = {k: s[k].matvec(P) for k in ("ebe", "ede", "csr")}
+    ke = s["ke"].get()
+    Qo = O.Oracle().ebemv(neq, nel, 3, ke, s["lm"], P.copy())
+    assert _rel(Q["ebe"][:neq], Qo[:neq]) <= 1e-14
+    assert Q["ebe"][neq] == 0.0
+    assert _rel(Q["ede"][:neq], Q["ebe"][:neq]) <= 1e-13
+    assert _rel(Q["csr"][:neq], Q["ebe"][:neq]) <= 1e-13
+    # FP32 copies: within FP32 rounding of the FP64 product
+    for k in ("ebe", "ede", "csr"):
+        q32 = s[k].matvec(P.astype(np.float32), np.float32).astype(np.float64)
+        assert _rel(q32[:neq], Qo[:neq]) <= 1e-5, k
+
+
+def test_2048_fp64_vs_fp32ir_solves(ctx, sys2048):
+    s = sys2048
+    neq = s["neq"]
+    th = s["theta"]
+    say("2048^2 constant-velocity system: FP64 and FP32-IR GMRES(30), Diag, EBE")
+    s["F"].zero()
+    s["mesh"].build_supg(F.Supg2dProblem(F.P_CONST, 1e-4, 0.0, 1.0, float(np.cos(th)),
+                                         float(np.sin(th)), 1.0, F.H_2SQRTAREA), s["ke"], s["F"])
+    s["ebe"].set_values(s["ke"])
+    Fh = s["F"].get()
+    i64, X64 = _solve(ctx, s["ebe"], F.PREC_DIAG, Fh, neq, F.GMRES_FP64)
+    assert i64.status == F.FG_OK and _true_rel(i64, 1e-8) <= 1e-8
+    say(f"  FP64: {i64.steps} steps; now FP32-IR")
+    i32, X32 = _solve(ctx, s["ebe"], F.PREC_DIAG, Fh, neq, F.GMRES_FP32_IR, orth=F.ORTH_CGS)
+    assert i32.status == F.FG_OK and _true_rel(i32, 1e-8) <= 1.05e-8
+    assert _nrel(X32[:neq], X64[:neq]) <= 1e-6
+    # the same system through the CSR and EDE storage: the same answer
+    F.assemble_csr(s["csr"], s["cbe"], s["ke"])
+    ic, Xc = _solve(ctx, s["csr"], F.PREC_DIAG, Fh, neq, F.GMRES_FP64)
+    assert ic.status == F.FG_OK and _nrel(Xc[:neq], X64[:neq]) <= 1e-9
+    F.assemble_ede(s["ede"], s["order"], s["e2e"], s["ke"])
+    ie, Xe = _solve(ctx, s["ede"], F.PREC_DIAG, Fh, neq, F.GMRES_FP64)
+    assert ie.status == F.FG_OK and _nrel(Xe[:neq], X64[:neq]) <= 1e-9
+
+
+# ---------------------------------------------------------------------------
+def test_128cube_rcm_ilu0_and_solves(ctx):
+    say("128^3 setup: Kuhn mesh, random numbering, RCM")
+    n = 128
+    x, y, z, t, v = F.structured_mesh3d(n)
+    nn = len(x)
+    perm = np.random.default_rng(20260303).permutation(nn)
+    inv = np.empty(nn, np.int64)
+    inv[perm] = np.arange(nn)
+    x, y, z, t, v = x[perm], y[perm], z[perm], t[perm], inv[v].astype(np.int32)
+    m = F.Mesh3D(ctx, x, y, z, t, v)
+    neq, nel = m.neq, m.nel
+    assert (neq, nel) == (2048383, 12582912)
+    lm = m.lm()
+    nnz, IA, JA, cbe = F.csr_pattern_k(neq, lm, 4)
+    bw0 = F.csr_bandwidth(neq, IA, JA)
+    P = F.rcm(neq, IA, JA)
+    assert np.array_equal(np.sort(P), np.arange(neq))
+    lm = F.renumber_lm(neq, lm, P)
+    nnz, IA, JA, cbe = F.csr_pattern_k(neq, lm, 4)
+    bw1 = F.csr_bandwidth(neq, IA, JA)
+    assert bw1 < bw0 // 100
+    ke, F0 = ctx.empty(16 * nel), ctx.empty(neq + 1)
+    F0.zero()
+    m.build_supg(F.Supg3dProblem(F.P3_DIFCONV, 1.0, 0, 0, 0, 0, 0, 0), ke, F0)
+    Fn = np.zeros(neq + 1)
+    Fn[:neq] = F0.get()[P]
+    csr = F.Operator.csr(ctx, neq, IA, JA)
+    F.assemble_csr(csr, cbe, ke, k=4)
+    ebe = F.Operator.ebe(ctx, neq, lm, 4)
+    ebe.set_values(ke)
+    say("128^3 EBE 4x4 product vs CSR product")
+    Pv = np.random.default_rng(5).standard_normal(neq + 1)
+    Qe, Qc = ebe.matvec(Pv), csr.matvec(Pv)
+    assert _rel(Qe[:neq], Qc[:neq]) <= 1e-13
+    say("128^3 exact ILU(0) on the device vs the oracle (factors, solve)")
+    pcx = F.Preconditioner(csr, F.PREC_ILU0)
+    pcx.setup(None)
+    LU, D = pcx.ilu_factors()
+    o = O.Oracle()
+    AAo = o.csr_assembly_k(nel, 4, nnz, ke.get(), cbe)
+    rc, LUo, Do = o.ilu0_factor(neq, AAo, JA, IA)
+    assert rc == 0
+    off = np.ones(nnz, bool)
+    rows = np.repeat(np.arange(neq), np.diff(IA))
+    off &= JA != rows
+    assert np.array_equal(LU[off], LUo[off]) and np.array_equal(D[:neq], Do[:neq])
+    p = ctx.to_device(Pv)
+    z = ctx.empty(neq + 1)
+    pcx.apply(p, z)
+    zo = o.ilu0_apply(neq, LUo, JA, IA, Do, Pv)
+    assert np.array_equal(z.get()[:neq], zo[:neq])
+    say("128^3 GMRES(50) to 1e-10 with 4-sweep ILU(0): FP64 and FP32-IR")
+    pc = F.Preconditioner(csr, F.PREC_ILU0)
+    pc.ilu_sweeps(4)
+    B = ctx.to_device(Fn)
+    pc.setup(B)
+    Bh = B.get()
+    i64, X64 = _solve(ctx, ebe, None, Bh, neq, F.GMRES_FP64, kmax=50, tol=1e-10, lmax=400, pc=pc)
+    assert i64.status == F.FG_OK and _true_rel(i64, 1e-10) <= 1e-10
+    i32, X32 = _solve(ctx, ebe, None, Bh, neq, F.GMRES_FP32_IR, kmax=50, tol=1e-10, lmax=400,
+                      pc=pc)
+    assert i32.status == F.FG_OK and _true_rel(i32, 1e-10) <= 1.05e-10
+    assert _nrel(X32[:neq], X64[:neq]) <= 1e-6
+    for obj in (pc, pcx, csr, ebe, m):
+        obj.close()
+
+
+# ---------------------------------------------------------------------------
+def test_256_batched_problems(ctx):
+    say("256 batched problems, 32..512 cells per side: setup")
+    rng = np.random.default_rng(20260206)
+    nprob = 256
+    sizes = rng.integers(32, 513, nprob)
+    parts, kes, Fs = [], [], []
+    for nc in sizes:
+        th = float(rng.uniform(0.0, 2 * np.pi))
+        kappa = float(10.0 ** rng.uniform(-5.0, -2.0))
+        x, y, t, v = F.structured_mesh2d(int(nc))
+        mesh = F.Mesh2D(ctx, x, y, t, v)
+        ke, Fv = ctx.empty(9 * mesh.nel), ctx.empty(mesh.neq + 1)
+        Fv.zero()
+        mesh.build_supg(F.Supg2dProblem(F.P_CONST, kappa, 0.0, 1.0, float(np.cos(th)),
+                                        float(np.sin(th)), 1.0, F.H_2SQRTAREA), ke, Fv)
+        parts.append((mesh.neq, mesh.lm()))
+        kes.append(ke)
+        Fs.append(Fv)
+        mesh.close()
+    lm, seg = F.concat_lm(parts)
+    N = int(seg[-1])
+    nel = len(lm) // 3
+    ke_all, B = ctx.empty(9 * nel), ctx.empty(N + 1)
+    B.zero()
+    off = 0
+    for p, (ke, Fv) in enumerate(zip(kes, Fs)):
+        F.lib.fg_memcpy_d2d(ctx.h, ke_all.ptr + 8 * off, ke.ptr, ke.nbytes)
+        F.lib.fg_memcpy_d2d(ctx.h, B.ptr + 8 * int(seg[p]), Fv.ptr, 8 * parts[p][0])
+        off += ke.n
+    op = F.Operator.ebe(ctx, N, lm, 3)
+    op.set_values(ke_all)
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    pc.setup(B)
+    Xs = {}
+    for precision, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
+        say(f"  batched {tag}")
+        X = ctx.empty(N + 1)
+        rc, infos = F.gmres_batched(ctx, op, pc, seg, B, X, kmax=30, lmax=100000, tol=1e-8,
+                                    precision=precision, graph=True)
+        assert rc == F.FG_OK
+        assert all(f.status == F.FG_OK for f in infos)
+        tr = [f.true_res / (f.eps / 1e-8) for f in infos]
+        assert max(tr) <= (1e-8 if precision == F.GMRES_FP64 else 1.05e-8), max(tr)
+        Xs[tag] = X.get()
+    for p in range(nprob):
+        a, b = seg[p], seg[p + 1]
+        assert _nrel(Xs["fp32_ir"][a:b], Xs["fp64"][a:b]) <= 1e-6, p
+    op.close()
+
+
+# ---------------------------------------------------------------------------
+def test_euler_1024_element_and_edge_schemes(ctx):
+    say("Euler 1024^2: build, 12x12 element and 8x8 edge operators")
+    n = 1024
+    x, y, t, v = F.structured_mesh2d(n)
+    nn, nel = len(x), len(v) // 3
+    inflow = np.abs(x) <= 1e-15
+    ids = np.full(4 * nn, -1, np.int64)
+    free = np.repeat(~inflow, 4)
+    neq = int(free.sum())
+    ids[free] = np.arange(neq)
+    lm = ids[(4 * v[:, None].astype(np.int64) + np.arange(4)).reshape(-1)]
+    lm = np.where(lm < 0, neq, lm).astype(np.int32)
+    rng = np.random.default_rng(20260305)
+    gam = 1.4
+    rho = 1.0 + 0.01 * rng.standard_normal(nn)
+    E = (1.0 / (9.0 * gam)) / ((gam - 1.0) * rho) + 0.5
+    U = np.stack([rho, rho, 0.0 * rho, rho * E], 1).reshape(-1)
+    m = F.Euler2D(ctx, x, y, v, lm, neq)
+    A, R = ctx.empty(144 * nel), ctx.empty(neq + 1)
+    m.build(F.EulerParams(gam, 3.0, 0, 1.0, 1e-3, 1e-10), ctx.to_device(U),
+            ctx.to_device(np.zeros(4 * nn)), A, R)
+    op = F.Operator.ebe(ctx, neq, lm, 12)
+    op.set_values(A)
+    ids32 = np.where(ids < 0, -1, ids).astype(np.int32)
+    g = F.Ede4(ctx, nn, v, ids32, neq)
+    _, _, lm2, en = g.arrays()
+    Aede = ctx.empty(64 * g.nedge)
+    g.assemble(A, Aede)
+    op8 = F.Operator.ebe(ctx, neq, lm2, 8)
+    op8.set_values(Aede)
+    Pv = rng.standard_normal(neq + 1)
+    assert _rel(op8.matvec(Pv)[:neq], op.matvec(Pv)[:neq]) <= 1e-13
+    Rh = R.get()
+    say("Euler 1024^2: block-diagonal GMRES(30) on both schemes, FP64 and FP32-IR")
+    pc = F.Preconditioner.blockdiag(op, nn, v, ids32)
+    B = ctx.to_device(Rh)
+    pc.setup(B)
+    Bh = B.get()
+    i64, X64 = _solve(ctx, op, None, Bh, neq, F.GMRES_FP64, pc=pc, lmax=2000)
+    assert i64.status == F.FG_OK and _true_rel(i64, 1e-8) <= 1e-8
+    i32, X32 = _solve(ctx, op, None, Bh, neq, F.GMRES_FP32_IR, pc=pc, lmax=2000)
+    assert i32.status == F.FG_OK and _true_rel(i32, 1e-8) <= 1.05e-8
+    assert _nrel(X32[:neq], X64[:neq]) <= 1e-6
+    pc8 = F.Preconditioner.blockdiag(op8, nn, en, ids32)
+    B8 = ctx.to_device(Rh)
+    pc8.setup(B8)
+    assert _rel(B8.get()[:neq], Bh[:neq]) <= 1e-13     # same node blocks, same scaling
+    ie, Xe = _solve(ctx, op8, None, B8.get(), neq, F.GMRES_FP64, pc=pc8, lmax=2000)
+    assert ie.status == F.FG_OK and ie.steps == i64.steps
+    assert _nrel(Xe[:neq], X64[:neq]) <= 1e-12
+    g.close()
