@@ -644,7 +644,8 @@ int fg_ebe_diag(fg_op *op, double *D)
     return FG_OK;
 }
 
-// ---- element-local Jacobi (EBE/Jacobi_precond_EBE.c:8-47) ------------------
+// ---- element-local Jacobi (EBE/Jacobi_precond_EBE.c:8-47, k = 3;
+// EBE/JacobiDOF4_precond_EBE.c:7-83, k = 12; any k the operator has) --------
 // z_g = sum over elements e containing g, in element order, of p_g / A^e_jj.
 // invde holds 1/A^e_jj per incidence (same order as the diagonal incidence).
 __global__ void k_ebe_jacobi_setup(int ninc, const long long *__restrict__ idiag,

@@ -121,7 +121,14 @@ int fg_op_set_pipeline(fg_op *op, int wg_per_cu);
 /* ---- preconditioners ------------------------------------------------------ */
 #define FG_PREC_NONE    0   /* NO_precond.c / NO_precond_setup.c            */
 #define FG_PREC_DIAG    1   /* Diag_precond.c + Diag_precond_{EBE,EDE,CSR}_setup.c */
-#define FG_PREC_JACOBI  2   /* EBE/Jacobi_precond_EBE{,_setup}.c (k=3 EBE only) */
+#define FG_PREC_JACOBI  2   /* element Jacobi of an EBE operator of any k:      */
+                            /* k = 3: EBE/Jacobi_precond_EBE{,_setup}.c:8-47    */
+                            /* k = 12: EBE/JacobiDOF4_precond_EBE{,_setup}.c    */
+                            /* :7-83 (the Euler driver's "Jacobi",              */
+                            /* 03 setPreconditioner.c:89-92); z_g = sum over the */
+                            /* elements of g, in element order, of p_g / A^e_jj; */
+                            /* applied correctly in place (the reference zeroes */
+                            /* z first, so its in-place call returns 0)         */
 int fg_prec_create(fg_op *op, int kind, fg_prec **out);
 /* setup: build from the operator's current values and left-precondition F
  * in place (device, neq+1 entries), as every reference *_setup does.

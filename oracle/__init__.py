@@ -111,6 +111,8 @@ class Oracle:
         L.orc_diag_setup_csr.argtypes = [_i, _dp, _ip, _ip, _dp, _dp, _dp]
         L.orc_jacobi_ebe_setup.argtypes = [_i, _i, _dp, _ip, _dp, _dp]
         L.orc_jacobi_ebe_apply.argtypes = [_i, _i, _dp, _ip, _dp, _dp]
+        L.orc_jacobi_ebe_setup_k.argtypes = [_i, _i, _i, _dp, _ip, _dp, _dp]
+        L.orc_jacobi_ebe_apply_k.argtypes = [_i, _i, _i, _dp, _ip, _dp, _dp]
         L.orc_pgmres.restype = _i
         L.orc_pgmres.argtypes = [_i, _i, _i, _d, _dp, _dp, C.POINTER(OrcOps),
                                  _vp, _i, _vp, C.POINTER(C.c_int)]
@@ -356,6 +358,19 @@ class Oracle:
     def jacobi_ebe_apply(self, neq, nel, invDe, lm, p):
         z = np.zeros(neq + 1)
         self.lib.orc_jacobi_ebe_apply(neq, nel, invDe, lm, p, z)
+        return z
+
+    def jacobi_ebe_setup_k(self, neq, nel, k, A, lm, F):
+        """JacobiDOF4_precond_EBE_setup (k = 12) or any k: (invDe, F preconditioned)"""
+        invDe, F2 = np.zeros(k * nel), np.ascontiguousarray(F, np.float64).copy()
+        self.lib.orc_jacobi_ebe_setup_k(neq, nel, k, np.ascontiguousarray(A, np.float64),
+                                        np.ascontiguousarray(lm, np.int32), invDe, F2)
+        return invDe, F2
+
+    def jacobi_ebe_apply_k(self, neq, nel, k, invDe, lm, p):
+        z = np.zeros(neq + 1)
+        self.lib.orc_jacobi_ebe_apply_k(neq, nel, k, invDe, np.ascontiguousarray(lm, np.int32),
+                                        np.ascontiguousarray(p, np.float64), z)
         return z
 
     # ---- GMRES over a stored operator ----
@@ -755,6 +770,8 @@ class ReferenceEuler:
         L.refeu_blockdiag_setup.argtypes = [_dp, _dp, _dp, _ip]
         L.refeu_blockdiag_apply.argtypes = [_dp, _dp]
         L.refeu_block_scaling.argtypes = [_dp, _dp]
+        L.refeu_jacobi4_setup.argtypes = [_dp, _dp]
+        L.refeu_jacobi4_apply.argtypes = [_dp, _dp]
 
     def setup(self, nnodes, nel, neq, vertex, ids, lm, A, precond="BlockDiag"):
         self.nnodes, self.nel, self.neq = nnodes, nel, neq
@@ -770,6 +787,19 @@ class ReferenceEuler:
         Q = np.zeros(self.neq + 1)
         self.lib.refeu_ebe2mv(np.ascontiguousarray(P, np.float64).copy(), Q)
         return Q
+
+    def jacobi4_setup(self, F):
+        """JacobiDOF4_precond_EBE_setup: (invDe [nel*12], F preconditioned)"""
+        Fp = np.ascontiguousarray(F, np.float64).copy()
+        invDe = np.zeros(12 * self.nel)
+        self.lib.refeu_jacobi4_setup(Fp, invDe)
+        return invDe, Fp
+
+    def jacobi4_apply(self, p):
+        """JacobiDOF4_precond_EBE out of place, after jacobi4_setup"""
+        z = np.zeros(self.neq + 1)
+        self.lib.refeu_jacobi4_apply(np.ascontiguousarray(p, np.float64).copy(), z)
+        return z
 
     def blockdiag_setup(self, F):
         Fp = np.ascontiguousarray(F, np.float64).copy()

@@ -242,6 +242,43 @@ void orc_jacobi_ebe_setup(int neq, int nel, const double *A, const int *lm,
     free(faux);
 }
 
+/* EBE/JacobiDOF4_precond_EBE.c:7-83 for k = 12 (and any k dofs per
+ * element): z = 0, then per element every z_j = p[lm_j] invDe[e][j] first and
+ * the k sums z[lm_j] += z_j in j order (:30-78); z[neq] = 0 */
+void orc_jacobi_ebe_apply_k(int neq, int nel, int k, const double *invDe, const int *lm,
+                            const double *p, double *z)
+{
+    double zj[16];
+    for (int r = 0; r < neq; r++)
+        z[r] = 0.0;
+    z[neq] = 0.0;
+    for (long e = 0; e < nel; e++) {
+        const int *le = lm + (long)k * e;
+        for (int j = 0; j < k; j++)
+            zj[j] = p[le[j]] * invDe[(long)k * e + j];
+        for (int j = 0; j < k; j++)
+            z[le[j]] = z[le[j]] + zj[j];
+    }
+    z[neq] = 0.0;
+}
+
+/* EBE/JacobiDOF4_precond_EBE_setup.c:4-64: invDe[e][j] = 1 / A_e[j][j]
+ * (A[I][13 j] for k = 12, :19-31), then F <- Jacobi(copy of F) (:52-61) */
+void orc_jacobi_ebe_setup_k(int neq, int nel, int k, const double *A, const int *lm,
+                            double *invDe, double *F)
+{
+    for (long e = 0; e < nel; e++)
+        for (int j = 0; j < k; j++)
+            invDe[(long)k * e + j] = 1.0 / A[(long)k * k * e + (long)j * (k + 1)];
+    double *faux = (double *)calloc((size_t)neq + 1, sizeof(double));
+    for (int r = 0; r < neq; r++)
+        faux[r] = F[r];
+    faux[neq] = 0.0;
+    F[neq] = 0.0;
+    orc_jacobi_ebe_apply_k(neq, nel, k, invDe, lm, faux, F);
+    free(faux);
+}
+
 /* ======================================================================== */
 /* operator callbacks                                                        */
 /* ======================================================================== */

@@ -68,6 +68,11 @@ void orc_jacobi_ebe_setup(int neq, int nel, const double *A, const int *lm,
  * zeroes z first, so an aliased call returns zero) */
 void orc_jacobi_ebe_apply(int neq, int nel, const double *invDe, const int *lm,
                           const double *p, double *z);
+/* JacobiDOF4_precond_EBE(_setup).c: the element Jacobi of k-dof elements */
+void orc_jacobi_ebe_apply_k(int neq, int nel, int k, const double *invDe, const int *lm,
+                            const double *p, double *z);
+void orc_jacobi_ebe_setup_k(int neq, int nel, int k, const double *A, const int *lm,
+                            double *invDe, double *F);
 
 /* ---------------- restarted GMRES ----------------------------------------- */
 /* operator callback: y = Op(x); x and y have neq+1 slots and may alias */

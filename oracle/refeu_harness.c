@@ -349,3 +349,18 @@ int refeu_pred_run(const double *u0, double alpha, double dt, double final_time,
     p_scheme = 0;
     return p_rc;
 }
+
+/* EBE/JacobiDOF4_precond_EBE_setup.c (tag 1) on the element matrices of
+ * refeu_setup: invDe [nel*12] out, F (neq+1) preconditioned in place; then
+ * JacobiDOF4_precond_EBE out of place (the setup's own call pattern) */
+int JacobiDOF4_precond_EBE(ParametersType *, MatrixDataType *, FemStructsType *, double *,
+                           double *);
+int JacobiDOF4_precond_EBE_setup(ParametersType *, MatrixDataType *, FemStructsType *, int,
+                                 double *);
+void refeu_jacobi4_setup(double *F, double *invDe)
+{
+    JacobiDOF4_precond_EBE_setup(&PE, &ME, &FE, 1, F);
+    memcpy(invDe, ME.invDeaux, sizeof(double) * 12 * (size_t)PE.nel);
+}
+
+void refeu_jacobi4_apply(double *p, double *z) { JacobiDOF4_precond_EBE(&PE, &ME, &FE, p, z); }

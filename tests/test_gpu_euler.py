@@ -199,3 +199,48 @@ def test_gmres_blockdiag_precond(ctx, path):
         K[np.ix_(g, g)] += A[e]
     x_ref = np.linalg.solve(K[:neq, :neq], d["F"][:neq])
     assert _rel(X.get()[:neq], x_ref) <= 1e-8
+
+
+# ---- JacobiDOF4_precond_EBE(_setup) on the 12-dof element operator ---------
+JAC4 = golden_files("jac4_*.npz")
+
+
+@pytest.mark.parametrize("path", JAC4, ids=os.path.basename)
+def test_jacobi_dof4_bitwise(ctx, path):
+    """FG_PREC_JACOBI on the k = 12 operator is JacobiDOF4_precond_EBE: the
+    setup's preconditioned F and the apply (out of place and in place) equal
+    the reference's bits (per incidence 1/A_jj, sums in element order)"""
+    j = load_golden(path)
+    d = load_golden(os.path.join(os.path.dirname(path), str(j["inputs"]) + ".npz"))
+    neq = int(d["neq"])
+    op = F.Operator.ebe(ctx, neq, d["lm"], 12)
+    op.set_values(d["A"])
+    pc = F.Preconditioner(op, F.PREC_JACOBI)
+    Fd = ctx.to_device(d["F"])
+    pc.setup(Fd)
+    Fh = Fd.get()
+    assert np.array_equal(Fh[:neq], j["F_jac"][:neq]) and Fh[neq] == 0.0
+    p, z = ctx.to_device(d["P"]), ctx.empty(neq + 1)
+    pc.apply(p, z)
+    assert np.array_equal(z.get()[:neq], j["z_jac"][:neq])
+    pc.apply(p, p)                       # in place, as pgmres.c:104 calls it
+    assert np.array_equal(p.get()[:neq], j["z_jac"][:neq])
+    # FP32 apply: within FP32 rounding
+    p32, z32 = ctx.to_device(d["P"].astype(np.float32)), ctx.empty(neq + 1, np.float32)
+    pc.apply(p32, z32)
+    assert _rel(z32.get()[:neq].astype(np.float64), j["z_jac"][:neq]) <= 2e-6
+    # and a Jacobi-preconditioned GMRES against a dense solve of the same system
+    B = ctx.to_device(d["F"])
+    pc.setup(B)
+    X = ctx.empty(neq + 1)
+    info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=30, lmax=200, tol=1e-12)
+    assert info.status == F.FG_OK
+    K = np.zeros((neq + 1, neq + 1))
+    A = d["A"].reshape(-1, 12, 12)
+    for e in range(A.shape[0]):
+        g = d["lm"][12 * e:12 * e + 12]
+        K[np.ix_(g, g)] += A[e]
+    xs = np.linalg.solve(K[:neq, :neq], d["F"][:neq])
+    assert _rel(X.get()[:neq], xs) <= 1e-8
+    pc.close()
+    op.close()

@@ -182,3 +182,58 @@ def test_oracle_vs_reference_random(o):
         assert np.array_equal(R.ebe2mv(c["P"]),
                               o.ebe2mv_ndof4(neq, nel, c["A"], c["lm"], nn, Idr, c["P"].copy(),
                                              invBD=iBDr))
+
+
+# ---- JacobiDOF4_precond_EBE(_setup): the element Jacobi of 12-dof elements --
+JAC4 = golden_files("jac4_*.npz")
+
+
+def _jac4_case(path):
+    j = load_golden(path)
+    d = load_golden(os.path.join(os.path.dirname(path), str(j["inputs"]) + ".npz"))
+    return j, d
+
+
+@pytest.mark.parametrize("path", JAC4, ids=os.path.basename)
+def test_jacobi_dof4_golden(o, path):
+    """setup (invDe, preconditioned F) and the apply, bitwise against the
+    reference's JacobiDOF4 routines"""
+    j, d = _jac4_case(path)
+    neq, nel = int(d["neq"]), int(d["nel"])
+    invDe, Fj = o.jacobi_ebe_setup_k(neq, nel, 12, d["A"], d["lm"], d["F"])
+    assert np.array_equal(invDe, j["invDe"])
+    assert np.array_equal(Fj[:neq], j["F_jac"][:neq]) and Fj[neq] == 0.0
+    z = o.jacobi_ebe_apply_k(neq, nel, 12, invDe, d["lm"], d["P"])
+    assert np.array_equal(z[:neq], j["z_jac"][:neq])
+
+
+@pytest.mark.parametrize("path", JAC4, ids=os.path.basename)
+def test_jacobi_dof4_is_a_diagonal(o, path):
+    """independently of the reference: the operator is diagonal, entry i the
+    sum over the elements of i of 1/A^e_ii (from a dense walk over lm)"""
+    j, d = _jac4_case(path)
+    neq, nel = int(d["neq"]), int(d["nel"])
+    A = d["A"].reshape(nel, 12, 12)
+    diag = np.zeros(neq + 1)
+    for e in range(nel):
+        for a in range(12):
+            diag[d["lm"][12 * e + a]] += 1.0 / A[e, a, a]
+    invDe, _ = o.jacobi_ebe_setup_k(neq, nel, 12, d["A"], d["lm"], d["F"])
+    z = o.jacobi_ebe_apply_k(neq, nel, 12, invDe, d["lm"], d["P"])
+    ref = d["P"][:neq] * diag[:neq]
+    assert np.max(np.abs(z[:neq] - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("path", golden_files("ss2d_n*_ebe.npz")[:2], ids=os.path.basename)
+def test_jacobi_k_generic_equals_the_pinned_3dof_form(o, path):
+    """the k-generic restatement at k = 3 gives the bits of the 3-dof form
+    pinned against Jacobi_precond_EBE(_setup).c"""
+    d = load_golden(path)
+    neq, nel = int(d["neq"]), len(d["lm"]) // 3
+    i3, F3 = o.jacobi_ebe_setup(neq, nel, d["A_u0"], d["lm"], d["F_u0"])
+    assert np.array_equal(i3, d["invDe"]) and np.array_equal(F3[:neq], d["F_jac"][:neq])
+    ik, Fk = o.jacobi_ebe_setup_k(neq, nel, 3, d["A_u0"], d["lm"], d["F_u0"])
+    assert np.array_equal(i3, ik) and np.array_equal(F3, Fk)
+    P = np.random.default_rng(4).standard_normal(neq + 1)
+    assert np.array_equal(o.jacobi_ebe_apply(neq, nel, i3, d["lm"], P),
+                          o.jacobi_ebe_apply_k(neq, nel, 3, ik, d["lm"], P))
