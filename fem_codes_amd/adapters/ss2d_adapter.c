@@ -19,97 +19,92 @@
  * Semantics kept: Q is overwritten and P[neq] = Q[neq] = 0; pgmres reads
  * KrylovBasisVectorsQuantity, SolverMaxIter and SolverTolerance, solves
  * A u = F with FemStructs->u zeroed first, and adds its Arnoldi step count
- * to Parameters->SolverIterations.  The element/edge/CSR values are
- * re-uploaded on every call because the driver rebuilds them in every
- * Picard iteration (Process.c:296-322); the structure (lm, lm2, IA/JA) is
+ * to Parameters->SolverIterations.  The driver rebuilds the element / edge /
+ * CSR values in every Picard iteration (Process.c:296-322) and then calls
+ * the solver, so the values are uploaded by the first pgmres call after the
+ * previous one and by a product called before any upload, not by every
+ * product (the rule is in fga_core.h); the structure (lm, lm2, IA/JA) is
  * uploaded once and cached, keyed by its host pointers.
- * Supported left preconditioners inside pgmres: NO_precond and Diag_precond
- * (MatrixData->invDiag as computed by the driver's setup); precondR must be
- * NO_precond.  Anything else is reported and returns 1: there is no CPU
- * fallback.  Errors print the library message and return 1 (the reference
- * exits; its callers ignore the return value).
+ * Supported left preconditioners inside pgmres: NO_precond, Diag_precond
+ * (MatrixData->invDiag as computed by the driver's setup) and
+ * Jacobi_precond_EBE (EBE scheme: the device element Jacobi of the same
+ * values, applied correctly in place where the reference's in-place call
+ * returns 0); precondR must be NO_precond.  Anything else is reported and
+ * returns 1: there is no CPU fallback.  Errors print the library message and
+ * return 1 (the reference exits; its callers ignore the return value).
  */
 #include <stdio.h>
 #include <string.h>
 
 #include "SSTranspEquation.h"
-#include "femgpu.h"
+#include "fga_core.h"
 
 int NO_precond(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
 int Diag_precond(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
+/* weak: the SS-2D driver's own table never selects it (setPreconditioner.c
+ * of that driver), so it is linked only where a caller provides it */
+int Jacobi_precond_EBE(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *)
+    __attribute__((weak));
 int ebemv(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
 int edemv(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
 int csrmv(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
 
 enum { SCH_EBE = 0, SCH_EDE = 1, SCH_CSR = 2 };
 
-static fg_ctx *g_ctx;
 static struct {
     int scheme, neq, nitems;
     const void *key;          /* lm / lm2 / JA host pointer of the cached structure */
     fg_op *op;
-    fg_prec *pc;
-    double *dP, *dQ;          /* neq+1 device vectors */
+    fg_prec *pc, *jac;
 } g;
-
-static int fail(const char *where)
-{
-    printf("femgpu %s: %s\n", where, fg_last_error());
-    return 1;
-}
-
-static int ensure_ctx(void)
-{
-    if (g_ctx) return 0;
-    if (fg_ctx_create(0, &g_ctx) != FG_OK) return fail("context");
-    return 0;
-}
 
 static void drop_op(void)
 {
     if (g.pc) fg_prec_destroy(g.pc);
+    if (g.jac) fg_prec_destroy(g.jac);
     if (g.op) fg_op_destroy(g.op);
-    if (g.dP) fg_free(g_ctx, g.dP);
-    if (g.dQ) fg_free(g_ctx, g.dQ);
     memset(&g, 0, sizeof g);
+    fga_new_operator();
 }
 
-/* the operator of the requested scheme, structure cached, values refreshed */
-static int ensure_op(int scheme, ParametersType *Pm, MatrixDataType *M, FemStructsType *FS)
+/* the operator of the requested scheme, structure cached by its host pointers */
+static int ensure_structure(int scheme, ParametersType *Pm, MatrixDataType *M,
+                            FemStructsType *FS)
 {
-    if (ensure_ctx()) return 1;
+    if (fga_ensure_ctx()) return 1;
     int neq = Pm->neq;
     const void *key = scheme == SCH_EBE ? (const void *)FS->lmaux
                     : scheme == SCH_EDE ? (const void *)FS->lm2aux : (const void *)M->JA;
     int nitems = scheme == SCH_EBE ? Pm->nel : scheme == SCH_EDE ? Pm->nedge : Pm->nnzero;
-    if (!g.op || g.scheme != scheme || g.neq != neq || g.key != key || g.nitems != nitems) {
-        drop_op();
-        int rc = FG_OK;
-        if (scheme == SCH_EBE)
-            rc = fg_op_ebe_create(g_ctx, neq, Pm->nel, NNOEL * NDOF, FS->lmaux, 0, &g.op);
-        else if (scheme == SCH_EDE)
-            rc = fg_op_ede_create(g_ctx, neq, Pm->nedge, FS->lm2aux, 0, &g.op);
-        else
-            rc = fg_op_csr_create(g_ctx, neq, Pm->nnzero, M->IA, M->JA, &g.op);
-        if (rc != FG_OK) return fail("operator");
-        if (fg_malloc(g_ctx, sizeof(double) * (neq + 1), (void **)&g.dP) != FG_OK ||
-            fg_malloc(g_ctx, sizeof(double) * (neq + 1), (void **)&g.dQ) != FG_OK)
-            return fail("vectors");
-        g.scheme = scheme; g.neq = neq; g.key = key; g.nitems = nitems;
-    }
-    const double *vals = scheme == SCH_CSR ? M->AA : M->Aaux;
-    if (fg_op_set_values_host(g.op, vals) != FG_OK) return fail("values");
+    if (g.op && g.scheme == scheme && g.neq == neq && g.key == key && g.nitems == nitems)
+        return 0;
+    drop_op();
+    int rc = FG_OK;
+    if (scheme == SCH_EBE)
+        rc = fg_op_ebe_create(fga_ctx, neq, Pm->nel, NNOEL * NDOF, FS->lmaux, 0, &g.op);
+    else if (scheme == SCH_EDE)
+        rc = fg_op_ede_create(fga_ctx, neq, Pm->nedge, FS->lm2aux, 0, &g.op);
+    else
+        rc = fg_op_csr_create(fga_ctx, neq, Pm->nnzero, M->IA, M->JA, &g.op);
+    if (rc != FG_OK) return fga_fail("operator");
+    g.scheme = scheme; g.neq = neq; g.key = key; g.nitems = nitems;
+    return 0;
+}
+
+static int upload_values(MatrixDataType *M)
+{
+    const double *vals = g.scheme == SCH_CSR ? M->AA : M->Aaux;
+    if (fg_op_set_values_host(g.op, vals) != FG_OK) return fga_fail("values");
+    fga_uploaded();
     return 0;
 }
 
 static int product(int scheme, ParametersType *Pm, MatrixDataType *M, FemStructsType *FS,
                    double *P, double *Q)
 {
-    if (ensure_op(scheme, Pm, M, FS)) return 1;
-    size_t bytes = sizeof(double) * (Pm->neq + 1);
-    if (fg_memcpy_h2d(g_ctx, g.dP, P, bytes) || fg_op_apply_f64(g.op, g.dP, g.dQ) ||
-        fg_memcpy_d2h(g_ctx, Q, g.dQ, bytes))
-        return fail("matvec");
+    if (ensure_structure(scheme, Pm, M, FS)) return 1;
+    if (fga_needs_product_upload() && upload_values(M)) return 1;
+    if (fga_product(g.op, Pm->neq, P, Q)) return 1;
     if (scheme != SCH_CSR) P[Pm->neq] = 0.0;   /* ebemv.c:21 mutates its input */
     return 0;
 }
@@ -140,30 +135,37 @@ int pgmres(ParametersType *Pm, MatrixDataType *M, FemStructsType *FS, FemFunctio
         printf("femgpu pgmres: only NO_precond is supported as right preconditioner\n");
         return 1;
     }
-    if (FF->precond != NO_precond && FF->precond != Diag_precond) {
+    const int jac = Jacobi_precond_EBE && FF->precond == Jacobi_precond_EBE;
+    if (FF->precond != NO_precond && FF->precond != Diag_precond && !jac) {
         printf("femgpu pgmres: unsupported left preconditioner\n");
         return 1;
     }
-    if (ensure_op(scheme, Pm, M, FS)) return 1;
+    if (jac && scheme != SCH_EBE) {
+        printf("femgpu pgmres: Jacobi_precond_EBE needs the EBE scheme\n");
+        return 1;
+    }
+    if (ensure_structure(scheme, Pm, M, FS)) return 1;
+    /* the driver rebuilt the values since the last solve (Process.c:296-322) */
+    if (fga_needs_build_upload() && upload_values(M)) return 1;
     fg_prec *pc = NULL;
     if (FF->precond == Diag_precond) {
-        if (!g.pc && fg_prec_create(g.op, FG_PREC_DIAG, &g.pc) != FG_OK) return fail("prec");
-        if (fg_prec_set_invdiag(g.pc, M->invDiag) != FG_OK) return fail("invDiag");
+        if (!g.pc && fg_prec_create(g.op, FG_PREC_DIAG, &g.pc) != FG_OK) return fga_fail("prec");
+        if (fg_prec_set_invdiag(g.pc, M->invDiag) != FG_OK) return fga_fail("invDiag");
         pc = g.pc;
+    } else if (jac) {
+        /* the element Jacobi of the uploaded values (the bits of the driver's
+         * Jacobi_precond_EBE_setup, which has already preconditioned F) */
+        if (!g.jac && fg_prec_create(g.op, FG_PREC_JACOBI, &g.jac) != FG_OK)
+            return fga_fail("Jacobi");
+        if (fg_prec_setup(g.jac, NULL) != FG_OK) return fga_fail("Jacobi setup");
+        pc = g.jac;
     }
-    int neq = Pm->neq;
-    size_t bytes = sizeof(double) * (neq + 1);
     fg_gmres_opts o;
-    memset(&o, 0, sizeof o);
-    o.kmax = Pm->KrylovBasisVectorsQuantity;
-    o.lmax = Pm->SolverMaxIter;
-    o.tol = Pm->SolverTolerance;
-    o.precision = FG_GMRES_FP64;
+    fga_opts(&o, Pm->KrylovBasisVectorsQuantity, Pm->SolverMaxIter, Pm->SolverTolerance);
     fg_gmres_info info;
-    if (fg_memcpy_h2d(g_ctx, g.dP, FS->F, bytes) != FG_OK) return fail("F");
-    int rc = fg_gmres(g_ctx, g.op, pc, g.dP, g.dQ, &o, &info);
-    if (rc != FG_OK && rc != FG_ERR_NOCONV) return fail("gmres");
-    if (fg_memcpy_d2h(g_ctx, FS->u, g.dQ, sizeof(double) * neq) != FG_OK) return fail("u");
+    memset(FS->u, 0, sizeof(double) * (size_t)(Pm->neq + 1));
+    if (fga_solve(g.op, pc, Pm->neq, &o, FS->F, FS->u, &info)) return 1;
+    FS->u[Pm->neq] = 0.0;
     Pm->SolverIterations += info.steps;
     return 0;
 }

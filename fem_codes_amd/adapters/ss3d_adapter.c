@@ -28,8 +28,10 @@
  * host; the device factorises the same CSR values (bit for bit the factors
  * of ILUp_precond_setup.c) and applies them inside the solve.  Anything else
  * is reported and returns 1: there is no CPU fallback.  The element and CSR
- * values are re-uploaded on every call (the driver rebuilds them per
- * nonlinear iteration); the structure is uploaded once, keyed by its host
+ * values are uploaded by the first solver call after the previous one (the
+ * driver rebuilds them per nonlinear iteration) and by a product called
+ * before any upload (Compute_Residual.c:18 after a solve reuses them; the
+ * rule is in fga_core.h); the structure is uploaded once, keyed by its host
  * pointers.
  */
 #include <stdio.h>
@@ -37,7 +39,7 @@
 #include <strings.h>
 
 #include "SSTransportEquation3D.h"
-#include "femgpu.h"
+#include "fga_core.h"
 
 int NO_precond(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
 int ILUp_precond(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
@@ -46,67 +48,61 @@ int csrmv(ParametersType *, MatrixDataType *, FemStructsType *, double *, double
 
 enum { SCH_EBE = 0, SCH_CSR = 2 };
 
-static fg_ctx *g_ctx;
 static struct {
     int scheme, neq, nitems;
     const void *key;
     fg_op *op;
     fg_prec *ilu;
-    double *dP, *dQ;
 } g;
-
-static int fail(const char *where)
-{
-    printf("femgpu %s: %s\n", where, fg_last_error());
-    return 1;
-}
-
-static int ensure_ctx(void)
-{
-    if (g_ctx) return 0;
-    if (fg_ctx_create(0, &g_ctx) != FG_OK) return fail("context");
-    return 0;
-}
 
 static void drop_op(void)
 {
     if (g.ilu) fg_prec_destroy(g.ilu);
     if (g.op) fg_op_destroy(g.op);
-    if (g.dP) fg_free(g_ctx, g.dP);
-    if (g.dQ) fg_free(g_ctx, g.dQ);
     memset(&g, 0, sizeof g);
+    fga_new_operator();
 }
 
-static int ensure_op(int scheme, ParametersType *Pm, MatrixDataType *M, FemStructsType *FS)
+static int ensure_structure(int scheme, ParametersType *Pm, MatrixDataType *M,
+                            FemStructsType *FS)
 {
-    if (ensure_ctx()) return 1;
+    if (fga_ensure_ctx()) return 1;
     int neq = Pm->neq;
     const void *key = scheme == SCH_EBE ? (const void *)FS->lmaux : (const void *)M->JA;
     int nitems = scheme == SCH_EBE ? Pm->nel : Pm->nnzero;
-    if (!g.op || g.scheme != scheme || g.neq != neq || g.key != key || g.nitems != nitems) {
-        drop_op();
-        int rc = scheme == SCH_EBE
-                     ? fg_op_ebe_create(g_ctx, neq, Pm->nel, NNOEL * NDOF, FS->lmaux, 0, &g.op)
-                     : fg_op_csr_create(g_ctx, neq, Pm->nnzero, M->IA, M->JA, &g.op);
-        if (rc != FG_OK) return fail("operator");
-        if (fg_malloc(g_ctx, sizeof(double) * (neq + 1), (void **)&g.dP) != FG_OK ||
-            fg_malloc(g_ctx, sizeof(double) * (neq + 1), (void **)&g.dQ) != FG_OK)
-            return fail("vectors");
-        g.scheme = scheme; g.neq = neq; g.key = key; g.nitems = nitems;
-    }
-    const double *vals = scheme == SCH_CSR ? M->AA : M->Aaux;
-    if (fg_op_set_values_host(g.op, vals) != FG_OK) return fail("values");
+    if (g.op && g.scheme == scheme && g.neq == neq && g.key == key && g.nitems == nitems)
+        return 0;
+    drop_op();
+    int rc = scheme == SCH_EBE
+                 ? fg_op_ebe_create(fga_ctx, neq, Pm->nel, NNOEL * NDOF, FS->lmaux, 0, &g.op)
+                 : fg_op_csr_create(fga_ctx, neq, Pm->nnzero, M->IA, M->JA, &g.op);
+    if (rc != FG_OK) return fga_fail("operator");
+    g.scheme = scheme; g.neq = neq; g.key = key; g.nitems = nitems;
+    return 0;
+}
+
+static int upload_values(MatrixDataType *M)
+{
+    const double *vals = g.scheme == SCH_CSR ? M->AA : M->Aaux;
+    if (fg_op_set_values_host(g.op, vals) != FG_OK) return fga_fail("values");
+    fga_uploaded();
+    return 0;
+}
+
+/* a solver entry: operator and the values of the current build */
+static int ensure_op(int scheme, ParametersType *Pm, MatrixDataType *M, FemStructsType *FS)
+{
+    if (ensure_structure(scheme, Pm, M, FS)) return 1;
+    if (fga_needs_build_upload()) return upload_values(M);
     return 0;
 }
 
 static int product(int scheme, ParametersType *Pm, MatrixDataType *M, FemStructsType *FS,
                    double *P, double *Q)
 {
-    if (ensure_op(scheme, Pm, M, FS)) return 1;
-    size_t bytes = sizeof(double) * (Pm->neq + 1);
-    if (fg_memcpy_h2d(g_ctx, g.dP, P, bytes) || fg_op_apply_f64(g.op, g.dP, g.dQ) ||
-        fg_memcpy_d2h(g_ctx, Q, g.dQ, bytes))
-        return fail("matvec");
+    if (ensure_structure(scheme, Pm, M, FS)) return 1;
+    if (fga_needs_product_upload() && upload_values(M)) return 1;
+    if (fga_product(g.op, Pm->neq, P, Q)) return 1;
     if (scheme == SCH_EBE) P[Pm->neq] = 0.0;   /* ebemv3Dndof1.c mutates its input */
     return 0;
 }
@@ -132,18 +128,9 @@ static int scheme_of(FemFunctionsType *FF)
 /* the device solve of A u = F; pc NULL or the ILU(0) of the CSR operator */
 static int solve(ParametersType *Pm, FemStructsType *FS, fg_prec *pc, fg_gmres_info *info)
 {
-    int neq = Pm->neq;
     fg_gmres_opts o;
-    memset(&o, 0, sizeof o);
-    o.kmax = Pm->KrylovBasisVectorsQuantity;
-    o.lmax = Pm->SolverMaxIter;
-    o.tol = Pm->SolverTolerance;
-    o.precision = FG_GMRES_FP64;
-    if (fg_memcpy_h2d(g_ctx, g.dP, FS->F, sizeof(double) * (neq + 1)) != FG_OK) return fail("F");
-    int rc = fg_gmres(g_ctx, g.op, pc, g.dP, g.dQ, &o, info);
-    if (rc != FG_OK && rc != FG_ERR_NOCONV) return fail("gmres");
-    if (fg_memcpy_d2h(g_ctx, FS->u, g.dQ, sizeof(double) * (neq + 1)) != FG_OK) return fail("u");
-    return 0;
+    fga_opts(&o, Pm->KrylovBasisVectorsQuantity, Pm->SolverMaxIter, Pm->SolverTolerance);
+    return fga_solve(g.op, pc, Pm->neq, &o, FS->F, FS->u, info);
 }
 
 int gmres(ParametersType *Pm, MatrixDataType *M, FemStructsType *FS, FemFunctionsType *FF)
@@ -178,8 +165,8 @@ int pgmres(ParametersType *Pm, MatrixDataType *M, FemStructsType *FS, FemFunctio
     if (ensure_op(scheme, Pm, M, FS)) return 1;
     fg_prec *pc = NULL;
     if (ilu) {
-        if (!g.ilu && fg_prec_create(g.op, FG_PREC_ILU0, &g.ilu) != FG_OK) return fail("ILU(0)");
-        if (fg_prec_setup(g.ilu, NULL) != FG_OK) return fail("ILU(0) factor");
+        if (!g.ilu && fg_prec_create(g.op, FG_PREC_ILU0, &g.ilu) != FG_OK) return fga_fail("ILU(0)");
+        if (fg_prec_setup(g.ilu, NULL) != FG_OK) return fga_fail("ILU(0) factor");
         pc = g.ilu;
     }
     fg_gmres_info info;

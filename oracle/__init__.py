@@ -753,7 +753,10 @@ class Reference3D:
 
 class ReferenceEuler:
     """The reference's 4-dof (Euler 2D) element products and node block-
-    diagonal routines compiled from their sources (oracle/_ref)."""
+    diagonal routines compiled from their sources (oracle/_ref).  `path`
+    selects another build of the same harness (the drop-in build links the
+    GPU adapter instead of the reference's products and setups and leaves
+    out the entry points that read host block arrays)."""
 
     PATH = os.path.join(HERE, "_ref", "libref_eu2d.so")
 
@@ -761,17 +764,19 @@ class ReferenceEuler:
     def available(cls) -> bool:
         return os.path.exists(cls.PATH)
 
-    def __init__(self, fast: bool = False):
-        path = self.PATH.replace(".so", "_fast.so") if fast else self.PATH
+    def __init__(self, fast: bool = False, path: str = None):
+        if path is None:
+            path = self.PATH.replace(".so", "_fast.so") if fast else self.PATH
         L = self.lib = C.CDLL(path)
-        L.refeu_setup.argtypes = [_i, _i, _i, _ip, _ip, _ip, _dp, C.c_char_p]
-        L.refeu_ebemv.argtypes = [_dp, _dp]
-        L.refeu_ebe2mv.argtypes = [_dp, _dp]
-        L.refeu_blockdiag_setup.argtypes = [_dp, _dp, _dp, _ip]
-        L.refeu_blockdiag_apply.argtypes = [_dp, _dp]
-        L.refeu_block_scaling.argtypes = [_dp, _dp]
-        L.refeu_jacobi4_setup.argtypes = [_dp, _dp]
-        L.refeu_jacobi4_apply.argtypes = [_dp, _dp]
+        for name, args in (("refeu_setup", [_i, _i, _i, _ip, _ip, _ip, _dp, C.c_char_p]),
+                           ("refeu_ebemv", [_dp, _dp]), ("refeu_ebe2mv", [_dp, _dp]),
+                           ("refeu_blockdiag_setup", [_dp, _dp, _dp, _ip]),
+                           ("refeu_blockdiag_apply", [_dp, _dp]),
+                           ("refeu_block_scaling", [_dp, _dp]),
+                           ("refeu_jacobi4_setup", [_dp, _dp]),
+                           ("refeu_jacobi4_apply", [_dp, _dp])):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = args
 
     def setup(self, nnodes, nel, neq, vertex, ids, lm, A, precond="BlockDiag"):
         self.nnodes, self.nel, self.neq = nnodes, nel, neq
@@ -941,8 +946,8 @@ class ReferencePredictor:
     def available(cls) -> bool:
         return os.path.exists(cls.PATH)
 
-    def __init__(self):
-        L = self.lib = C.CDLL(self.PATH)
+    def __init__(self, path: str = None):
+        L = self.lib = C.CDLL(path or self.PATH)
         L.refpred_setup.argtypes = [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _ip, _i, _d, _d, _d,
                                     _d, _d, _d, _d]
         L.refpred_run.argtypes = [_dp, _d, _d, _d, _d, _i, _i, _i, _d, _d, _i, _i, _dp, _ip,

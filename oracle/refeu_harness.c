@@ -9,9 +9,22 @@
  * of every dof of every node (Node[].id[4], -1 = prescribed), lm and the
  * 12x12 element matrices; the harness lays them out as the Euler driver does
  * (MatrixData->A row pointers into one Aaux block).
+ *
+ * Built a second time with -DFEMGPU_DROPIN and linked with the drop-in
+ * adapter fem_codes_amd/adapters/eu2d_adapter.c instead of the reference's
+ * products, block setups, Block_scaling.c and JacobiDOF4 routines
+ * (oracle/_ref/libeu2d_dropin.so): the time loop is then the reference's
+ * Predictor_Old over the reference's build with the adapter's 6-argument
+ * pgmres on the GPU as the solver.  The entry points that read the block
+ * arrays of MatrixData (which stay on the device there) are left out of that
+ * build.
  */
 #include "EulerEquations.h"
 #include "../../../00_CommonFiles/Solvers_and_Preconditioners/preconditioners.h"
+#ifdef FEMGPU_DROPIN
+int pgmres(ParametersType *, MatrixDataType *, FemStructsType *, FemFunctionsType *, double *,
+           double *);
+#endif
 
 int ebemvNDOF4(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
 int ebe2mvNDOF4(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
@@ -55,6 +68,7 @@ void refeu_setup(int nnodes, int nel, int neq, const int *vertex, const int *ids
 void refeu_ebemv(double *P, double *Q) { ebemvNDOF4(&PE, &ME, &FE, P, Q); }
 void refeu_ebe2mv(double *P, double *Q) { ebe2mvNDOF4(&PE, &ME, &FE, P, Q); }
 
+#ifndef FEMGPU_DROPIN
 /* BlockDiag_precond_EBE_setup.c (tag 1): BD, invBD [nnodes*16], Id
  * [nnodes*4], F (neq+1) preconditioned in place */
 void refeu_blockdiag_setup(double *F, double *BD, double *invBD, int *Id)
@@ -68,8 +82,11 @@ void refeu_blockdiag_setup(double *F, double *BD, double *invBD, int *Id)
         for (int d = 0; d < 4; d++) Id[4 * i + d] = ME.Id[i][d];
 }
 
+#endif
+
 void refeu_blockdiag_apply(double *p, double *z) { BlockDiag_precond(&PE, &ME, &FE, p, z); }
 
+#ifndef FEMGPU_DROPIN
 /* Block_scaling.c:10-59 Block_scaling_EBE (iterations = 0: the setup runs
  * with tag 1); F (neq+1) in/out, A (nel*144) out */
 void refeu_block_scaling(double *F, double *A)
@@ -80,6 +97,7 @@ void refeu_block_scaling(double *F, double *A)
     memcpy(F, FE.F, sizeof(double) * (PE.neq + 1));
     memcpy(A, ME.Aaux, sizeof(double) * 144 * (size_t)PE.nel);
 }
+#endif
 
 /* ---- the edge-by-edge scheme of the Euler driver: ede_Initialization.c,
  * ede_assembly.c, EDE/edemvNDOF4.c, EDE/ede2mvNDOF4.c and
@@ -141,6 +159,7 @@ void refeu_ede_set_matrix(const double *Aede)
 
 void refeu_edemv(double *P, double *Q) { edemvNDOF4(&PD, &MD, &FD, P, Q); }
 
+#ifndef FEMGPU_DROPIN
 /* BlockDiag_precond_EDE_setup (tag 1): BD, invBD [nnodes*16], Id, F in place */
 void refeu_ede_blockdiag_setup(double *F, double *BD, double *invBD, int *Id)
 {
@@ -151,6 +170,7 @@ void refeu_ede_blockdiag_setup(double *F, double *BD, double *invBD, int *Id)
         for (int d = 0; d < 4; d++) Id[4 * i + d] = MD.Id[i][d];
     }
 }
+#endif
 
 /* ede2mvNDOF4 (after refeu_ede_blockdiag_setup) */
 void refeu_ede2mv(double *P, double *Q) { ede2mvNDOF4(&PD, &MD, &FD, P, Q); }
@@ -277,6 +297,25 @@ static int cbp_solver(ParametersType *P, MatrixDataType *M, FemStructsType *S, F
     return rc;
 }
 
+static int p_precond;   /* drop-in build: 0 as the scheme's table, 1 BlockDiag, 2 Jacobi */
+#ifdef FEMGPU_DROPIN
+int NO_precond(ParametersType *, MatrixDataType *, FemStructsType *, double *, double *);
+int JacobiDOF4_precond_EBE(ParametersType *, MatrixDataType *, FemStructsType *, double *,
+                           double *);
+int JacobiDOF4_precond_EBE_setup(ParametersType *, MatrixDataType *, FemStructsType *, int,
+                                 double *);
+/* the GPU solver's parameters and the preconditioner inside the next run */
+void refeu_dropin_params(int kmax, int lmax, double tol, int precond)
+{
+    PE.KrylovBasisVectorsQuantity = kmax;
+    PE.LinearMaxIter = lmax;
+    PE.SolverTolerance = tol;
+    p_precond = precond;
+}
+static int p_iterations;
+int refeu_dropin_iterations(void) { return p_iterations; }
+#endif
+
 /* the scheme of the next refeu_pred_run (setMatrixVectorProductType.c /
  * setPreconditioner.c of the Euler driver with Preconditioner BlockDiag for
  * EBE2 and EDE2, NOT for EDE); EDE schemes need refeu_ede_setup first */
@@ -339,7 +378,26 @@ int refeu_pred_run(const double *u0, double alpha, double dt, double final_time,
     memset(&FO, 0, sizeof FO);
     FO.Build = Build_M_K_F_SUPG;
     FO.solver = cbp_solver;
+#ifdef FEMGPU_DROPIN
+    FO.solver = pgmres;
+    if (p_scheme == 0) FB.mv = ebemvNDOF4;   /* setMatrixVectorProductType.c:12 */
+    FB.precond = NO_precond;
+    FB.precondR = NO_precond;
+    if (p_precond == 1) {          /* BlockDiag inside the solve (EBE / EDE) */
+        FB.precond_setup = p_scheme == 2 ? BlockDiag_precond_EDE_setup
+                                         : BlockDiag_precond_EBE_setup;
+        FB.precond = BlockDiag_precond;
+    } else if (p_precond == 2) {   /* JacobiDOF4 inside the solve (EBE) */
+        FB.precond_setup = JacobiDOF4_precond_EBE_setup;
+        FB.precond = JacobiDOF4_precond_EBE;
+    }
+    Pp->iterations = 0;
+#endif
     Predictor_Old(Pp, Mp, Sp, &FB, &FO);
+#ifdef FEMGPU_DROPIN
+    p_iterations = Pp->iterations;
+    p_precond = 0;
+#endif
     memcpy(u, Sp->u, sizeof(double) * (PE.neq + 1));
     *nsteps = p_steps;
     *t_final = Pp->CurrentTime;
@@ -350,6 +408,7 @@ int refeu_pred_run(const double *u0, double alpha, double dt, double final_time,
     return p_rc;
 }
 
+#ifndef FEMGPU_DROPIN
 /* EBE/JacobiDOF4_precond_EBE_setup.c (tag 1) on the element matrices of
  * refeu_setup: invDe [nel*12] out, F (neq+1) preconditioned in place; then
  * JacobiDOF4_precond_EBE out of place (the setup's own call pattern) */
@@ -362,5 +421,6 @@ void refeu_jacobi4_setup(double *F, double *invDe)
     JacobiDOF4_precond_EBE_setup(&PE, &ME, &FE, 1, F);
     memcpy(invDe, ME.invDeaux, sizeof(double) * 12 * (size_t)PE.nel);
 }
+#endif
 
 void refeu_jacobi4_apply(double *p, double *z) { JacobiDOF4_precond_EBE(&PE, &ME, &FE, p, z); }
