@@ -144,6 +144,7 @@ def setup_system(F, ctx, n, prob, schemes=("EBE", "EDE", "CSR")):
         ops["EDE"] = F.Operator.ede(ctx, neq, lm2)
         F.assemble_ede(ops["EDE"], order, ebe, ke)
         aux["EDE"] = (order, ebe)
+        aux["lm2"] = lm2
     return dict(x=x, y=y, type=t, vertex=v, mesh=mesh, neq=neq, nel=nel, lm=lm, ke=ke, F=Fv,
                 ops=ops, aux=aux)
 
@@ -236,6 +237,20 @@ def headline_leg(F, ctx, args, rng):
         if tj.get("kernel") == kernel:
             roof["traffic"] = tj.get("hbm_bytes_per_launch")
 
+    # the edge scheme: edges per block x schedule tuned the same way
+    ede_tune = None
+    if "EDE" in sysd["ops"] and not args.no_autotune:
+        from fem_codes_amd import autotune
+        vals = sysd["ops"]["EDE"].get_values()
+        (epb_e, pipe_e), tms_e, ede_tuned = autotune.tune_ebe(ctx, neq, sysd["aux"]["lm2"], 2,
+                                                              vals, edges=True)
+        del vals
+        log("autotune EDE f64 ms by (edges/block, workgroups/CU):",
+            {k: round(v, 4) for k, v in tms_e.items()}, "->", (epb_e, pipe_e))
+        sysd["ops"]["EDE"].close()
+        sysd["ops"]["EDE"] = ede_tuned
+        ede_tune = {"edges_per_block": epb_e, "wg_per_cu": pipe_e,
+                    "ms": {f"{k[0]}/{k[1]}": v for k, v in tms_e.items()}}
     matvec = {}
     for name, op in list(sysd["ops"].items()) + [("EBE_default", default_op)]:
         matvec[name] = matvec_table(op, P64, Q64, P32, Q32, args.steps)
@@ -244,7 +259,7 @@ def headline_leg(F, ctx, args, rng):
     res = {"value": value, "ms": ms, "wall_ms_per_step": wall * 1e3 / args.steps,
            "scale": scale, "kappa": kappa, "theta": theta, "roofline": roof,
            "matvec": matvec, "ebe_autotune_ms": {f"{k[0]}/{k[1]}": v for k, v in tune_ms.items()},
-           "neq": neq, "nel": sysd["nel"]}
+           "ede_autotune": ede_tune, "neq": neq, "nel": sysd["nel"]}
 
     if args.calibrate_copy:
         m = 64 << 20
@@ -970,6 +985,7 @@ def main():
         "matvec": head["matvec"],
         "copy_calibration": head.get("copy_calibration"),
         "ebe_autotune_ms": head["ebe_autotune_ms"],
+        "ede_autotune": head.get("ede_autotune"),
         "solve": head.get("solve"),
         "small": small,
         "batched": batched,

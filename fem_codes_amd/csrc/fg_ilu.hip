@@ -472,17 +472,46 @@ __global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nrb, const int *__rest
     const int b = fg_xcd_remap(blockIdx.x, nrb);
     const int r0 = rb[b], r1 = rb[b + 1];
     const int j0 = IA[r0], j1 = IA[r1];
+    // blocks hold at most FG_TPB rows: one row per thread in the sum phase,
+    // its bounds, base and pivot loaded before the barrier
+    const int r = r0 + (int)threadIdx.x;
+    int ra = 0, re = 0;
+    T br = T(0), dr = T(0);
+    if (r < r1) {
+        ra = IA[r];
+        re = IA[r + 1];
+        br = base[r];
+        if (UPPER) dr = D[r];
+    }
     if (j1 - j0 <= TRI_NNZB) {
-        // the factor's values and columns are read once per sweep: non-temporal
-        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB)
-            s[j - j0] = __builtin_nontemporal_load(&V[j]) * x[__builtin_nontemporal_load(&JA[j])];
+        // the factor's values and columns are read once per sweep
+        // (non-temporal); every thread issues all of its loads of the block
+        // before the gathers of x, so TRI_NPT load chains are in flight
+        constexpr int TRI_NPT = TRI_NNZB / FG_TPB;
+        int col[TRI_NPT];
+        T v[TRI_NPT], xv[TRI_NPT];
+#pragma unroll
+        for (int q = 0; q < TRI_NPT; q++) {
+            const int j = j0 + (int)threadIdx.x + q * FG_TPB;
+            col[q] = j < j1 ? __builtin_nontemporal_load(&JA[j]) : 0;
+            v[q] = j < j1 ? __builtin_nontemporal_load(&V[j]) : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < TRI_NPT; q++) {
+            const int j = j0 + (int)threadIdx.x + q * FG_TPB;
+            xv[q] = j < j1 ? x[col[q]] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < TRI_NPT; q++) {
+            const int j = (int)threadIdx.x + q * FG_TPB;
+            if (j < j1 - j0) s[j] = v[q] * xv[q];
+        }
         __syncthreads();
-        for (int r = r0 + threadIdx.x; r < r1; r += FG_TPB) {
+        if (r < r1) {
             T acc = T(0);
-            const int a = IA[r] - j0, e = IA[r + 1] - j0;
-            for (int j = a; j < e; j++) acc = acc + s[j];
-            const T y = base[r] - acc;
-            out[r] = UPPER ? D[r] * y : y;
+            for (int j = ra - j0; j < re - j0; j++) acc = acc + s[j];
+            const T y = br - acc;
+            out[r] = UPPER ? dr * y : y;
         }
     } else {   // one row longer than a block
         T acc = T(0);
