@@ -202,16 +202,17 @@ def headline_leg(F, ctx, args, rng):
     Q64 = ctx.empty(neq + 1)
     P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
     Q32 = ctx.empty(neq + 1, np.float32)
+    from fem_codes_amd import autotune
     if args.no_autotune:
-        best_epb, best_pipe, tune_ms = 512, 0, {}
+        best_epb, best_ord, best_pipe, tune_ms = 512, F.ORDER_BFS, 0, {}
         ebe_tuned = F.Operator.ebe(ctx, neq, sysd["lm"], 3, best_epb)
         ebe_tuned.set_values(sysd["ke"])
     else:
-        from fem_codes_amd import autotune
-        (best_epb, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(ctx, neq, sysd["lm"], 3,
-                                                                      sysd["ke"])
-        log("autotune EBE f64 ms by (elements/block, workgroups/CU):",
-            {k: round(v, 4) for k, v in tune_ms.items()}, "->", (best_epb, best_pipe))
+        (best_epb, best_ord, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(
+            ctx, neq, sysd["lm"], 3, sysd["ke"], orderings=autotune.ORDERINGS)
+        log("autotune EBE f64 ms by elements/block / ordering / workgroups per CU:",
+            {autotune.key_str(k): round(v, 4) for k, v in tune_ms.items()}, "->",
+            autotune.key_str((best_epb, best_ord, best_pipe)))
     default_op = sysd["ops"]["EBE"]
     sysd["ops"]["EBE"] = ebe_tuned
     ebe = ebe_tuned
@@ -229,7 +230,8 @@ def headline_leg(F, ctx, args, rng):
     kernel = (f"k_ebe_apply_pipe<3, {best_epb // 256}, double, 0>" if best_pipe
               else f"k_ebe_apply<3, {best_epb // 256}, double, 0>")
     roof = roofline(ms_main, bytes64, kernel)
-    roof["schedule"] = {"elems_per_block": best_epb, "wg_per_cu": best_pipe}
+    roof["schedule"] = {"elems_per_block": best_epb, "wg_per_cu": best_pipe,
+                        "ordering": autotune.ORDER_NAMES[best_ord]}
     tpath = os.path.join(ROOT, "profiles", "ebe_f64_2048_traffic.json")
     if os.path.exists(tpath) and n == 2048:
         with open(tpath) as f:
@@ -242,15 +244,17 @@ def headline_leg(F, ctx, args, rng):
     if "EDE" in sysd["ops"] and not args.no_autotune:
         from fem_codes_amd import autotune
         vals = sysd["ops"]["EDE"].get_values()
-        (epb_e, pipe_e), tms_e, ede_tuned = autotune.tune_ebe(ctx, neq, sysd["aux"]["lm2"], 2,
-                                                              vals, edges=True)
+        (epb_e, ord_e, pipe_e), tms_e, ede_tuned = autotune.tune_ebe(
+            ctx, neq, sysd["aux"]["lm2"], 2, vals, edges=True, orderings=autotune.ORDERINGS)
         del vals
-        log("autotune EDE f64 ms by (edges/block, workgroups/CU):",
-            {k: round(v, 4) for k, v in tms_e.items()}, "->", (epb_e, pipe_e))
+        log("autotune EDE f64 ms by edges/block / ordering / workgroups per CU:",
+            {autotune.key_str(k): round(v, 4) for k, v in tms_e.items()}, "->",
+            autotune.key_str((epb_e, ord_e, pipe_e)))
         sysd["ops"]["EDE"].close()
         sysd["ops"]["EDE"] = ede_tuned
         ede_tune = {"edges_per_block": epb_e, "wg_per_cu": pipe_e,
-                    "ms": {f"{k[0]}/{k[1]}": v for k, v in tms_e.items()}}
+                    "ordering": autotune.ORDER_NAMES[ord_e],
+                    "ms": {autotune.key_str(k): v for k, v in tms_e.items()}}
     matvec = {}
     for name, op in list(sysd["ops"].items()) + [("EBE_default", default_op)]:
         matvec[name] = matvec_table(op, P64, Q64, P32, Q32, args.steps)
@@ -258,7 +262,7 @@ def headline_leg(F, ctx, args, rng):
     default_op.close()
     res = {"value": value, "ms": ms, "wall_ms_per_step": wall * 1e3 / args.steps,
            "scale": scale, "kappa": kappa, "theta": theta, "roofline": roof,
-           "matvec": matvec, "ebe_autotune_ms": {f"{k[0]}/{k[1]}": v for k, v in tune_ms.items()},
+           "matvec": matvec, "ebe_autotune_ms": {autotune.key_str(k): v for k, v in tune_ms.items()},
            "ede_autotune": ede_tune, "neq": neq, "nel": sysd["nel"]}
 
     if args.calibrate_copy:
@@ -496,6 +500,9 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
             for p in range(nprob) if seg[p + 1] > seg[p]]
     res["fp32_ir_max_rel_err_vs_fp64"] = max(errs)
     res["speedup_fp32_ir"] = res["batched_fp64"]["seconds"] / res["batched_fp32_ir"]["seconds"]
+    if not args.no_autotune:
+        res["tuned"] = batched_tuned(F, ctx, args, N, lm, ke_all, seg, B, Xs["fp64"], tol,
+                                     res["batched_fp32_ir"]["seconds"])
     Bh = B.get()
     del ke_all, op, pc
     if not args.no_sequential:
@@ -525,6 +532,49 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
                                           nprob)
         log("batched cpu", res["cpu_baseline"])
     return res
+
+
+def batched_tuned(F, ctx, args, N, lm, ke_all, seg, B, X64, tol, default_s):
+    """the FP32-IR batched solve with the autotuner's choices: elements per
+    block x ordering x schedule of the FP32 product, then the restart length
+    over full solves (B already carries the Diag left-preconditioning, and
+    the tuned operator's Diag is the same sum of element diagonals)"""
+    from fem_codes_amd import autotune
+    key, tms, op_t = autotune.tune_ebe(ctx, N, lm, 3, ke_all, fp32=True, reps=20,
+                                       orderings=autotune.ORDERINGS)
+    log("batched autotune FP32 product ms:", {autotune.key_str(k): round(v, 4)
+                                             for k, v in tms.items()}, "->", autotune.key_str(key))
+    pc_t = F.Preconditioner(op_t, F.PREC_DIAG)
+    pc_t.setup(None)
+
+    def run(km, X=None):
+        X = X if X is not None else ctx.empty(N + 1)
+        ctx.sync()
+        w0 = time.perf_counter()
+        rc, infos = F.gmres_batched(ctx, op_t, pc_t, seg, B, X, kmax=km, lmax=100000, tol=tol,
+                                    precision=F.GMRES_FP32_IR, graph=not args.no_graph)
+        ctx.sync()
+        return time.perf_counter() - w0, rc == F.FG_OK, infos
+
+    best_km, km_s = autotune.tune_restart(lambda km: run(km)[:2])
+    log("batched autotune FP32-IR seconds by restart length:", km_s, "->", best_km)
+    X = ctx.empty(N + 1)
+    sec, ok, infos = run(best_km, X)
+    d = X.get()[:N] - X64[:N]
+    err = max(float(np.linalg.norm(d[seg[p]:seg[p + 1]]) /
+                    max(np.linalg.norm(X64[seg[p]:seg[p + 1]]), 1e-300))
+              for p in range(len(seg) - 1) if seg[p + 1] > seg[p])
+    out = {"product": autotune.key_str(key),
+           "product_ms": {autotune.key_str(k): v for k, v in tms.items()},
+           "restart_seconds": {str(k): v for k, v in km_s.items()}, "kmax": best_km,
+           "fp32_ir": dict(seconds=sec, converged=int(sum(f.status == F.FG_OK for f in infos)),
+                           total_steps=int(sum(f.steps for f in infos)),
+                           max_steps=int(max(f.steps for f in infos))),
+           "max_rel_err_vs_fp64_kmax30": err, "speedup_vs_default_fp32_ir": default_s / sec}
+    log("batched tuned", {k: v for k, v in out.items() if k not in ("product_ms",)})
+    pc_t.close()
+    op_t.close()
+    return out
 
 
 def batched_cpu(probs, X_gpu, seg, gpu_seconds, nprob, budget_s=15.0):

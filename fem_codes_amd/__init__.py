@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(HERE, "libfemgpu.so")
 
 FG_OK, FG_ERR_HIP, FG_ERR_ARG, FG_ERR_BREAKDOWN, FG_ERR_NOCONV, FG_ERR_NOMEM = range(6)
 PREC_NONE, PREC_DIAG, PREC_JACOBI, PREC_ILU0, PREC_BLOCKDIAG = 0, 1, 2, 3, 4
+ORDER_BFS, ORDER_MESH = 0, 1
 EBE2_BLOCKDIAG, EBE2_NOBLOCKDIAG = 0, 1
 P3_DIFCONV, P3_CONST = 0, 1
 GMRES_FP64, GMRES_FP32_IR = 0, 1
@@ -98,6 +99,7 @@ SIGNATURES = {
     "fg_cast_f32_to_f64": (_i, [_vp, _i, _vp, _vp]),
     "fg_op_ebe_create": (_i, [_vp, _i, _i, _i, _ip, _i, C.POINTER(_vp)]),
     "fg_op_ede_create": (_i, [_vp, _i, _i, _ip, _i, C.POINTER(_vp)]),
+    "fg_op_ebe_create_ex": (_i, [_vp, _i, _i, _i, _ip, _i, _i, _i, C.POINTER(_vp)]),
     "fg_op_csr_create": (_i, [_vp, _i, _i, _ip, _ip, C.POINTER(_vp)]),
     "fg_op_destroy": (_i, [_vp]),
     "fg_op_set_values_host": (_i, [_vp, _dp]),
@@ -303,21 +305,31 @@ class Operator:
         self.neq, self.nitems, self.k, self.nnz = neq.value, nit.value, k.value, nnz.value
 
     @classmethod
-    def ebe(cls, ctx, neq, lm, k, elems_per_block=0):
-        """ebemv family: lm (nel*k) with sink = neq."""
+    def ebe(cls, ctx, neq, lm, k, elems_per_block=0, ordering=None):
+        """ebemv family: lm (nel*k) with sink = neq.  ordering None or
+        ORDER_BFS (blocks grown breadth-first) or ORDER_MESH (runs of
+        consecutive elements in lm's order)."""
         lm = np.ascontiguousarray(lm, np.int32)
         h = _vp()
-        _check(lib.fg_op_ebe_create(ctx.h, neq, lm.size // k, k, lm, elems_per_block,
-                                    C.byref(h)), "fg_op_ebe_create")
+        if ordering is None:
+            _check(lib.fg_op_ebe_create(ctx.h, neq, lm.size // k, k, lm, elems_per_block,
+                                        C.byref(h)), "fg_op_ebe_create")
+        else:
+            _check(lib.fg_op_ebe_create_ex(ctx.h, neq, lm.size // k, k, lm, elems_per_block,
+                                           int(ordering), 0, C.byref(h)), "fg_op_ebe_create_ex")
         return cls(ctx, h, "EBE")
 
     @classmethod
-    def ede(cls, ctx, neq, lm2, edges_per_block=0):
+    def ede(cls, ctx, neq, lm2, edges_per_block=0, ordering=None):
         """edemv: lm2 (nedge*2) with sink = neq."""
         lm2 = np.ascontiguousarray(lm2, np.int32)
         h = _vp()
-        _check(lib.fg_op_ede_create(ctx.h, neq, lm2.size // 2, lm2, edges_per_block,
-                                    C.byref(h)), "fg_op_ede_create")
+        if ordering is None:
+            _check(lib.fg_op_ede_create(ctx.h, neq, lm2.size // 2, lm2, edges_per_block,
+                                        C.byref(h)), "fg_op_ede_create")
+        else:
+            _check(lib.fg_op_ebe_create_ex(ctx.h, neq, lm2.size // 2, 2, lm2, edges_per_block,
+                                           int(ordering), 1, C.byref(h)), "fg_op_ebe_create_ex")
         return cls(ctx, h, "EDE")
 
     @classmethod

@@ -12,44 +12,90 @@ k, the precision and the mesh, so it is measured: every candidate operator is
 built, given the same values, timed with HIP events over `reps` products, and
 the fastest is returned.  Candidates whose LDS footprint does not fit are
 skipped.  Every schedule computes identical results.
+
+Two more knobs:
+- the element ordering of the block plan (breadth-first growth, or runs of
+  consecutive elements in the caller's order, which keeps an ordering the
+  caller already chose); it is tried at the best block size, since it
+  changes the halo fraction and the gather locality but not the register
+  budget.  Both orderings give the same products to rounding (1e-14).
+- the restart length of the solve (tune_restart): longer cycles take fewer
+  steps on stagnating systems but cost more orthogonalisation per step, so
+  the fastest converged solve over a candidate list is kept.  The result
+  differs from the reference's own restart length only within the solve
+  tolerance.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from . import DeviceArray, FemGpuError, Operator
+from . import ORDER_BFS, ORDER_MESH, DeviceArray, FemGpuError, Operator
 
 CANDIDATES = (256, 512, 1024, 2048)
 PIPELINES = (0, 1, 2, 3, 4, 6)
+ORDERINGS = (ORDER_BFS, ORDER_MESH)
+ORDER_NAMES = {ORDER_BFS: "bfs", ORDER_MESH: "mesh"}
+RESTARTS = (20, 30, 45, 64)
 
 
 def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
              reps: int = 10, candidates=CANDIDATES, edges: bool = False,
-             pipelines=PIPELINES):
-    """Return ((best_elems_per_block, best_pipeline), {(epb, pipe): ms}, best_operator).
+             pipelines=PIPELINES, orderings=(ORDER_BFS,)):
+    """Return ((epb, ordering, pipeline), {(epb, ordering, pipe): ms}, best_operator).
 
     values: AoS element (or edge) matrices, host array or DeviceArray.
-    edges: build edge-by-edge operators (k must be 2)."""
+    edges: build edge-by-edge operators (k must be 2).
+    orderings: the first is tuned over every block size; the others are
+    timed at the block size that won."""
     dt = np.float32 if fp32 else np.float64
     P = ctx.to_device(np.random.default_rng(0).standard_normal(neq + 1).astype(dt))
     Q = DeviceArray(ctx, neq + 1, dt)
     times, ops = {}, {}
-    for epb in candidates:
+
+    def trial(epb, order):
         try:
-            op = Operator.ede(ctx, neq, lm, epb) if edges else Operator.ebe(ctx, neq, lm, k, epb)
+            if edges:
+                op = Operator.ede(ctx, neq, lm, epb, ordering=order)
+            else:
+                op = Operator.ebe(ctx, neq, lm, k, epb, ordering=order)
         except FemGpuError:
-            continue
+            return
         op.set_values(values)
         for pipe in pipelines:
             op.set_pipeline(pipe)
-            times[(epb, pipe)] = op.time_apply(P, Q, reps=reps)
-        ops[epb] = op
+            times[(epb, order, pipe)] = op.time_apply(P, Q, reps=reps)
+        ops[(epb, order)] = op
+
+    for epb in candidates:
+        trial(epb, orderings[0])
     if not times:
         raise FemGpuError("no block size fits this mesh")
+    epb0 = min(times, key=times.get)[0]
+    for order in orderings[1:]:
+        trial(epb0, order)
     best = min(times, key=times.get)
-    for epb, op in ops.items():
-        if epb != best[0]:
+    for key, op in ops.items():
+        if key != best[:2]:
             op.close()
-    op = ops[best[0]]
-    op.set_pipeline(best[1])
+    op = ops[best[:2]]
+    op.set_pipeline(best[2])
     return best, times, op
+
+
+def tune_restart(solve, restarts=RESTARTS):
+    """solve(kmax) -> (seconds, converged) runs one solve with restart
+    length kmax.  Returns (best_kmax, {kmax: seconds or None}); a solve that
+    did not converge is recorded as None and never chosen."""
+    times = {}
+    for km in restarts:
+        sec, ok = solve(int(km))
+        times[int(km)] = float(sec) if ok else None
+    done = {k: v for k, v in times.items() if v is not None}
+    if not done:
+        raise FemGpuError("no restart length converged")
+    return min(done, key=done.get), times
+
+
+def key_str(key):
+    """(epb, ordering, pipe) -> 'epb/ordering/pipe' for JSON records"""
+    return f"{key[0]}/{ORDER_NAMES.get(key[1], key[1])}/{key[2]}"

@@ -49,7 +49,9 @@ struct Plan {
     int nslots = 0;
 };
 
-int build_plan(int neq, int nel, int k, const int *lm, int eb, Plan &pl)
+// ordering FG_ORDER_BFS: blocks grown breadth-first over shared dofs;
+// FG_ORDER_MESH: runs of eb consecutive (kept) elements in the caller's order
+int build_plan(int neq, int nel, int k, const int *lm, int eb, int ordering, Plan &pl)
 {
     // elements whose dofs are all Dirichlet add nothing to Q (their rows go
     // to the sink slot) and are dropped from the operator
@@ -91,6 +93,21 @@ int build_plan(int neq, int nel, int k, const int *lm, int eb, Plan &pl)
     std::vector<int> mark(nel, -1), queue;
     queue.reserve(8 * (size_t)eb);
     int seed = 0;
+    if (ordering == FG_ORDER_MESH) {
+        int cnt = 0;
+        for (int e = 0; e < nel; e++) {
+            if (state[e] != -1) continue;
+            if (cnt == 0) pl.nblk++;
+            state[e] = pl.nblk - 1;
+            order.push_back(e);
+            if (++cnt == eb) {
+                pl.eoff.push_back((int)order.size());
+                cnt = 0;
+            }
+        }
+        if (cnt > 0) pl.eoff.push_back((int)order.size());
+        seed = nel;
+    }
     for (;;) {
         while (seed < nel && state[seed] != -1) seed++;
         if (seed >= nel) break;
@@ -478,8 +495,9 @@ int fg_ebe_get_values_dev(fg_op *op, double *A)
 static int default_ept(int k) { return fg_ebe_default_ept(k); }
 
 static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int per_block,
-                      int kind, fg_op **out)
+                      int ordering, int kind, fg_op **out)
 {
+    FG_ARG(ordering == FG_ORDER_BFS || ordering == FG_ORDER_MESH, "ordering");
     // sizes first (host-only checks): the kernels address P, Q and the halo
     // partials through 32-bit byte offsets (fg_at), so every index times 8
     // must stay below 2^32
@@ -494,7 +512,7 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
     else if (ept > 2) ept = 4;
     FG_HIP(hipSetDevice(c->device));
     Plan pl;
-    FG_CHECK(build_plan(neq, nitems, k, lm, FG_TPB * ept, pl));
+    FG_CHECK(build_plan(neq, nitems, k, lm, FG_TPB * ept, ordering, pl));
     FG_ARG(pl.nslots < FG_OFF32_MAX, "halo partial slots must stay below 2^28");
     fg_op *op = new fg_op();
     op->ctx = c;
@@ -591,13 +609,21 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
 extern "C" int fg_op_ebe_create(fg_ctx *c, int neq, int nel, int k, const int *lm,
                                 int elems_per_block, fg_op **out)
 {
-    return ebe_create(c, neq, nel, k, lm, elems_per_block, FG_OP_EBE, out);
+    return ebe_create(c, neq, nel, k, lm, elems_per_block, FG_ORDER_BFS, FG_OP_EBE, out);
 }
 
 extern "C" int fg_op_ede_create(fg_ctx *c, int neq, int nedge, const int *lm2,
                                 int edges_per_block, fg_op **out)
 {
-    return ebe_create(c, neq, nedge, 2, lm2, edges_per_block, FG_OP_EDE, out);
+    return ebe_create(c, neq, nedge, 2, lm2, edges_per_block, FG_ORDER_BFS, FG_OP_EDE, out);
+}
+
+extern "C" int fg_op_ebe_create_ex(fg_ctx *c, int neq, int nitems, int k, const int *lm,
+                                   int items_per_block, int ordering, int edges, fg_op **out)
+{
+    FG_ARG(!edges || k == 2, "an edge operator has k = 2");
+    return ebe_create(c, neq, nitems, k, lm, items_per_block, ordering,
+                      edges ? FG_OP_EDE : FG_OP_EBE, out);
 }
 
 int fg_ebe_set_values_dev(fg_op *op, const double *A)

@@ -81,6 +81,21 @@ int fg_op_ebe_create(fg_ctx *ctx, int neq, int nel, int k, const int *lm,
  * 2x2 matrices; identical machinery to EBE with k=2. */
 int fg_op_ede_create(fg_ctx *ctx, int neq, int nedge, const int *lm2,
                      int edges_per_block, fg_op **out);
+/* The element / edge operator with the block plan's element ordering as a
+ * tuning knob (no reference counterpart):
+ *   FG_ORDER_BFS   blocks grown breadth-first over shared dofs from the
+ *                  lowest-numbered free element (the default of the two
+ *                  calls above);
+ *   FG_ORDER_MESH  blocks of items_per_block consecutive items in the
+ *                  caller's order (an ordering the caller chose, such as a
+ *                  space-filling or RCM order of the elements, is kept).
+ * Results agree to rounding (1e-14): a dof's contributions are summed in
+ * element order inside a block and in block order across blocks.  edges = 1
+ * builds the edge operator (k = 2, fg_op_ede_create). */
+#define FG_ORDER_BFS  0
+#define FG_ORDER_MESH 1
+int fg_op_ebe_create_ex(fg_ctx *ctx, int neq, int nitems, int k, const int *lm,
+                        int items_per_block, int ordering, int edges, fg_op **out);
 /* CSR operator (CSR/csrmv.c:3-24); IA[neq+1], JA[nnz] host. */
 int fg_op_csr_create(fg_ctx *ctx, int neq, int nnz, const int *IA, const int *JA, fg_op **out);
 int fg_op_destroy(fg_op *op);

@@ -101,3 +101,16 @@ def test_operator_sizes_beyond_32bit_offsets_are_rejected():
     rc = F.lib.fg_op_ebe_create(None, (1 << 28) - 1, 1, 3, lm, 0, F.C.byref(out))
     assert rc == F.FG_ERR_ARG
     assert "ctx" in F.lib.fg_last_error().decode()
+
+
+def test_ordering_knob_arguments():
+    """fg_op_ebe_create_ex: an unknown ordering, or an edge operator with
+    k != 2, is refused before any device work"""
+    lm = np.zeros(3, np.int32)
+    out = F._vp()
+    rc = F.lib.fg_op_ebe_create_ex(None, 10, 1, 3, lm, 0, 7, 0, F.C.byref(out))
+    assert rc == F.FG_ERR_ARG and "ordering" in F.lib.fg_last_error().decode()
+    rc = F.lib.fg_op_ebe_create_ex(None, 10, 1, 3, lm, 0, F.ORDER_MESH, 1, F.C.byref(out))
+    assert rc == F.FG_ERR_ARG and "k = 2" in F.lib.fg_last_error().decode()
+    rc = F.lib.fg_op_ebe_create_ex(None, 10, 1, 3, lm, 0, F.ORDER_MESH, 0, F.C.byref(out))
+    assert rc == F.FG_ERR_ARG and "ctx" in F.lib.fg_last_error().decode()

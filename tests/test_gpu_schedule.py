@@ -55,3 +55,93 @@ def test_ede_pipeline_identical(system):
     op.set_pipeline(3)
     assert np.array_equal(op.matvec(P), ref)
     op.close()
+
+
+@pytest.mark.parametrize("epb", [256, 1024])
+def test_ebe_mesh_ordering(system, epb):
+    """blocks of consecutive elements (FG_ORDER_MESH) against breadth-first
+    blocks and the oracle, in the mesh's own element order and in a shuffled
+    one (the caller's ordering is kept, so blocks are scattered)"""
+    ctx, neq, nel, lm, ke, P = system
+    keh = ke.get().reshape(nel, 9)
+    Qo = O.Oracle().ebemv(neq, nel, 3, keh.ravel(), lm, P.copy())
+    perm = np.random.default_rng(5).permutation(nel)
+    for lmx, kex in ((lm, keh), (lm.reshape(nel, 3)[perm].ravel(), keh[perm])):
+        for order in (F.ORDER_BFS, F.ORDER_MESH):
+            op = F.Operator.ebe(ctx, neq, lmx, 3, epb, ordering=order)
+            op.set_values(np.ascontiguousarray(kex).ravel())
+            for wpc in (0, 2):
+                op.set_pipeline(wpc)
+                Q = op.matvec(P)
+                assert np.max(np.abs(Q[:neq] - Qo[:neq])) <= 1e-14 * np.max(np.abs(Qo[:neq]))
+                assert Q[neq] == 0.0
+                Q32 = op.matvec(P.astype(np.float32), np.float32)
+                assert np.max(np.abs(Q32[:neq] - Qo[:neq])) <= 1e-5 * np.max(np.abs(Qo[:neq]))
+            op.close()
+
+
+def test_ede_mesh_ordering(system):
+    ctx, neq, nel, lm, ke, P = system
+    nedge, order, ebe, lm2 = F.ede_pattern(neq, lm)
+    ref_op = F.Operator.ede(ctx, neq, lm2)
+    F.assemble_ede(ref_op, order, ebe, ke)
+    ref = ref_op.matvec(P)
+    vals = ref_op.get_values()
+    op = F.Operator.ede(ctx, neq, lm2, 512, ordering=F.ORDER_MESH)
+    op.set_values(vals)
+    Q = op.matvec(P)
+    assert np.max(np.abs(Q[:neq] - ref[:neq])) <= 1e-14 * np.max(np.abs(ref[:neq]))
+    op.close()
+    ref_op.close()
+
+
+def test_autotuner_choice_is_correct(system):
+    """tune_ebe over block sizes, both orderings and two schedules returns
+    the fastest timed candidate, and that operator's product is the oracle's"""
+    from fem_codes_amd import autotune
+    ctx, neq, nel, lm, ke, P = system
+    best, times, op = autotune.tune_ebe(ctx, neq, lm, 3, ke, reps=3, candidates=(256, 512),
+                                        pipelines=(0, 2), orderings=autotune.ORDERINGS)
+    assert best == min(times, key=times.get)
+    assert {k[1] for k in times} == set(autotune.ORDERINGS)
+    Qo = O.Oracle().ebemv(neq, nel, 3, ke.get(), lm, P.copy())
+    Q = op.matvec(P)
+    assert np.max(np.abs(Q[:neq] - Qo[:neq])) <= 1e-14 * np.max(np.abs(Qo[:neq]))
+    op.close()
+
+
+def test_restart_tuner_on_device(system):
+    """tune_restart over real FP32-IR solves of a constant-velocity system
+    (256^2 cells) with the Diag preconditioner: every candidate that
+    converges reaches the tolerance, and the chosen one is the fastest"""
+    from fem_codes_amd import autotune
+    ctx = system[0]
+    x, y, t, v = F.structured_mesh2d(256)
+    mesh = F.Mesh2D(ctx, x, y, t, v)
+    neq, nel = mesh.neq, mesh.nel
+    ke = ctx.empty(9 * nel)
+    B = ctx.empty(neq + 1)
+    B.zero()
+    pb = F.Supg2dProblem(F.P_CONST, 1e-3, 0.0, 1.0, 0.6, 0.8, 1.0, F.H_2SQRTAREA)
+    mesh.build_supg(pb, ke, B)
+    op = F.Operator.ebe(ctx, neq, mesh.lm(), 3)
+    op.set_values(ke)
+    mesh.close()
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    pc.setup(B)
+    res = {}
+
+    def solve(km):
+        X = ctx.empty(neq + 1)
+        info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=km, lmax=200, tol=1e-8,
+                             precision=F.GMRES_FP32_IR)
+        res[km] = info
+        return info.ms * 1e-3, info.status == F.FG_OK
+
+    km, times = autotune.tune_restart(solve, restarts=(20, 40))
+    assert times[km] == min(v for v in times.values() if v is not None)
+    for k, info in res.items():
+        if info.status == F.FG_OK:
+            assert info.true_res <= 1.01 * info.eps
+    pc.close()
+    op.close()
