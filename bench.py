@@ -502,7 +502,8 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
     res["speedup_fp32_ir"] = res["batched_fp64"]["seconds"] / res["batched_fp32_ir"]["seconds"]
     if not args.no_autotune:
         res["tuned"] = batched_tuned(F, ctx, args, N, lm, ke_all, seg, B, Xs["fp64"], tol,
-                                     res["batched_fp32_ir"]["seconds"])
+                                     res["batched_fp32_ir"]["seconds"],
+                                     res["batched_fp32_ir"]["max_steps"])
     Bh = B.get()
     del ke_all, op, pc
     if not args.no_sequential:
@@ -534,11 +535,13 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
     return res
 
 
-def batched_tuned(F, ctx, args, N, lm, ke_all, seg, B, X64, tol, default_s):
+def batched_tuned(F, ctx, args, N, lm, ke_all, seg, B, X64, tol, default_s, default_steps):
     """the FP32-IR batched solve with the autotuner's choices: elements per
     block x ordering x schedule of the FP32 product, then the restart length
     over full solves (B already carries the Diag left-preconditioning, and
-    the tuned operator's Diag is the same sum of element diagonals)"""
+    the tuned operator's Diag is the same sum of element diagonals).  Each
+    trial solve is capped at three times the Arnoldi steps of the default
+    solve, so a restart length that stagnates ends as 'not converged'."""
     from fem_codes_amd import autotune
     key, tms, op_t = autotune.tune_ebe(ctx, N, lm, 3, ke_all, fp32=True, reps=20,
                                        orderings=autotune.ORDERINGS)
@@ -551,7 +554,8 @@ def batched_tuned(F, ctx, args, N, lm, ke_all, seg, B, X64, tol, default_s):
         X = X if X is not None else ctx.empty(N + 1)
         ctx.sync()
         w0 = time.perf_counter()
-        rc, infos = F.gmres_batched(ctx, op_t, pc_t, seg, B, X, kmax=km, lmax=100000, tol=tol,
+        lmax = -(-3 * default_steps // km)
+        rc, infos = F.gmres_batched(ctx, op_t, pc_t, seg, B, X, kmax=km, lmax=lmax, tol=tol,
                                     precision=F.GMRES_FP32_IR, graph=not args.no_graph)
         ctx.sync()
         return time.perf_counter() - w0, rc == F.FG_OK, infos
@@ -871,8 +875,26 @@ def euler_leg(F, ctx, args, n=1024, seed=20260305):
     P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
     Q32 = ctx.empty(neq + 1, np.float32)
     res["matvec_ebe12"] = matvec_table(op, P64, Q64, P32, Q32, 20)
+    # the row sums on the matrix cores against the vector row sums: the
+    # block kernel alone, both precisions; the faster form (FP32, the inner
+    # cycle's product) is kept for the solves
+    trial, Qr = {}, {}
+    for kind, tag in ((F.ROWS_VALU, "valu"), (F.ROWS_MFMA, "mfma")):
+        op.set_row_kernel(kind)
+        trial[tag] = {"fp64_ms": op.time_apply(P64, Q64, reps=20, main_only=True),
+                      "fp32_ms": op.time_apply(P32, Q32, reps=20, main_only=True)}
+        op.apply(P64, Q64)
+        Qr[tag] = Q64.get()
+    trial["rel_diff_fp64"] = float(np.max(np.abs(Qr["mfma"][:neq] - Qr["valu"][:neq])) /
+                                   np.max(np.abs(Qr["valu"][:neq])))
+    trial["kept"] = "mfma" if trial["mfma"]["fp32_ms"] < trial["valu"]["fp32_ms"] else "valu"
+    op.set_row_kernel(F.ROWS_MFMA if trial["kept"] == "mfma" else F.ROWS_VALU)
+    res["rows_trial"] = trial
+    log("euler rows VALU vs MFMA", trial)
     ms_main = op.time_apply(P32, Q32, reps=20, main_only=True)
-    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True), ebe_kernel_name(12, True))
+    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True),
+                               "k_ebe_apply_mfma12<float, 0>" if trial["kept"] == "mfma"
+                               else ebe_kernel_name(12, True))
     Xs = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         B = ctx.to_device(Rh)

@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(HERE, "libfemgpu.so")
 FG_OK, FG_ERR_HIP, FG_ERR_ARG, FG_ERR_BREAKDOWN, FG_ERR_NOCONV, FG_ERR_NOMEM = range(6)
 PREC_NONE, PREC_DIAG, PREC_JACOBI, PREC_ILU0, PREC_BLOCKDIAG = 0, 1, 2, 3, 4
 ORDER_BFS, ORDER_MESH = 0, 1
+ROWS_VALU, ROWS_MFMA = 0, 1
 EBE2_BLOCKDIAG, EBE2_NOBLOCKDIAG = 0, 1
 P3_DIFCONV, P3_CONST = 0, 1
 GMRES_FP64, GMRES_FP32_IR = 0, 1
@@ -111,6 +112,7 @@ SIGNATURES = {
     "fg_op_algorithmic_bytes": (_d, [_vp, _i]),
     "fg_op_time_apply": (_i, [_vp, _i, _vp, _vp, _i, _i, C.POINTER(_d)]),
     "fg_op_set_pipeline": (_i, [_vp, _i]),
+    "fg_op_set_row_kernel": (_i, [_vp, _i]),
     "fg_prec_create": (_i, [_vp, _i, C.POINTER(_vp)]),
     "fg_prec_setup": (_i, [_vp, _vp]),
     "fg_prec_apply_f64": (_i, [_vp, _vp, _vp]),
@@ -393,6 +395,10 @@ class Operator:
         wg_per_cu workgroups per CU with next-block prefetch"""
         _check(lib.fg_op_set_pipeline(self.h, int(wg_per_cu)), "fg_op_set_pipeline")
         self.pipeline = int(wg_per_cu)
+
+    def set_row_kernel(self, kind):
+        """ROWS_VALU or ROWS_MFMA (k = 12 element operators)"""
+        _check(lib.fg_op_set_row_kernel(self.h, int(kind)), "fg_op_set_row_kernel")
 
     def close(self):
         if self.h and self.ctx.h:

@@ -35,7 +35,7 @@ CANDIDATES = (256, 512, 1024, 2048)
 PIPELINES = (0, 1, 2, 3, 4, 6)
 ORDERINGS = (ORDER_BFS, ORDER_MESH)
 ORDER_NAMES = {ORDER_BFS: "bfs", ORDER_MESH: "mesh"}
-RESTARTS = (20, 30, 45, 64)
+RESTARTS = (10, 15, 20, 30, 45, 64)
 
 
 def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,

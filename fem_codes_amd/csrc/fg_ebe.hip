@@ -293,6 +293,24 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_apply_pipe(FgEbeLayout L,
     }
 }
 
+// k = 12 with the row sums on the matrix cores (fg_op_set_row_kernel)
+template <typename T, int BS>
+__global__ __launch_bounds__(FG_TPB) void k_ebe_apply_mfma12(FgEbeLayout L,
+                                                             const T *__restrict__ Kv,
+                                                             const T *__restrict__ P,
+                                                             T *__restrict__ Q,
+                                                             T *__restrict__ hpart, const int *skip)
+{
+    if (skip && *skip) return;
+    extern __shared__ unsigned char smem_raw[];
+    const int b = fg_xcd_remap(blockIdx.x, L.nblk);
+    FgEbeRegs<12, 1, T> r;
+    fg_ebe_gather<12, 1, T>(b, L, P, smem_raw, r);
+    fg_ebe_rows_mfma12<T, BS>(b, L, Kv, smem_raw);
+    __syncthreads();
+    fg_ebe_reduce<12, 1, T>(b, L, Q, hpart, smem_raw, r);
+}
+
 // shared dofs: partials in block order; dofs no element touches: 0; sink: 0
 template <typename T>
 __global__ __launch_bounds__(FG_TPB) void k_ebe_halo(
@@ -341,6 +359,13 @@ static int launch_apply_k(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *
     fg_ctx *c = op->ctx;
     const size_t shm = sizeof(T) == 8 ? op->smem64 : op->smem32;
     const FgEbeLayout L = fg_ebe_layout(op);
+    if constexpr (K == 12 && EPT == 1) {
+        if (op->rows_mfma) {
+            FG_LAUNCH((k_ebe_apply_mfma12<T, BS>), op->nblk, FG_TPB, shm, c->stream, L, Kv, P, Q,
+                      hp, skip);
+            return FG_OK;
+        }
+    }
     if (op->pipe_wpc > 0) {
         long long g = (long long)c->num_cu * op->pipe_wpc;
         g = g / 8 * 8;

@@ -225,6 +225,11 @@ __device__ __forceinline__ void fg_ebe_gather(int b, const FgEbeLayout &L,
 template <int BS>
 __device__ __forceinline__ constexpr int fg_ebe_c0(int rr) { return BS > 0 && rr < BS ? BS : 0; }
 
+template <int K, int EPT, typename T>
+__device__ __forceinline__ void fg_ebe_reduce(int b, const FgEbeLayout &L, T *__restrict__ Q,
+                                              T *__restrict__ hpart, unsigned char *smem,
+                                              const FgEbeRegs<K, EPT, T> &r);
+
 template <int K, int EPT, typename T, int BS = 0>
 __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
                                               const T *__restrict__ Kv, T *__restrict__ Q,
@@ -237,7 +242,6 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
     const int ninc = L.ibase[b + 1] - L.ibase[b];
     FgEbeSmem<K, EPT, T> S(smem, nd, ne, ninc);
     const T *Kb = Kv + (size_t)K * K * e0;
-    const int *Db = L.dofs + d0, *Sb = L.slots + d0;
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
         const unsigned el = threadIdx.x + i * FG_TPB;
@@ -287,6 +291,22 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
         }
     }
     __syncthreads();
+    fg_ebe_reduce<K, EPT, T>(b, L, Q, hpart, smem, r);
+}
+
+// the incidence sums of a block's row sums (in LDS): every local dof adds its
+// incidences in their stored order; dofs complete in the block -> Q, shared
+// dofs -> their halo slot
+template <int K, int EPT, typename T>
+__device__ __forceinline__ void fg_ebe_reduce(int b, const FgEbeLayout &L, T *__restrict__ Q,
+                                              T *__restrict__ hpart, unsigned char *smem,
+                                              const FgEbeRegs<K, EPT, T> &r)
+{
+    const unsigned ne = (unsigned)(L.eoff[b + 1] - L.eoff[b]);
+    const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
+    const int ninc = L.ibase[b + 1] - L.ibase[b];
+    FgEbeSmem<K, EPT, T> S(smem, nd, ne, ninc);
+    const int *Db = L.dofs + d0, *Sb = L.slots + d0;
     constexpr int ND = FgEbeRegs<K, EPT, T>::ND;
 #pragma unroll
     for (int u = 0; u < ND; u++) {
@@ -310,6 +330,98 @@ __device__ __forceinline__ void fg_ebe_finish(int b, const FgEbeLayout &L,
             fg_at(Q, (unsigned)fg_at(Db, l)) = acc;
         else
             fg_at(hpart, (unsigned)s) = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 12 x 12 element rows on the matrix cores (an alternative to the VALU row
+// sums above for k = 12, one element per thread).  A wave takes tiles of 16
+// elements; for row rr of the tile's elements, three 16x16x4 MFMAs (one per
+// group of four columns) multiply A[i][c] = K_{e_i}[rr][4g+c] by
+// B[c][j] = p_{e_j}[4g+c], and the wanted row sum of element i is the
+// diagonal entry D[i][i].  Only 16 of the 256 outputs are used: the 12x12
+// products are 0.25 flop per byte, and this form exists to measure whether
+// moving the multiply-adds off the VALU frees issue slots for loads.
+// Operand layouts (16x16x4, 64 lanes, lane = i + 16 c): A[i][c] and B[c][j]
+// one value per lane.  Output, register v of lane j + 16 q (measured on
+// gfx950 with a one-wave probe of both builtins): FP32 D[4 q + v][j], FP64
+// D[q + 4 v][j].  So D[i][i] is in lane i + 16 (i / 4), register i % 4
+// (FP32), or lane i + 16 (i % 4), register i / 4 (FP64).
+// BS = 4: the row node's own 4x4 block (column group rr / 4) is skipped.
+template <typename T> struct FgMfma16;
+template <> struct FgMfma16<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc mma(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // lane i + 16 c holds D[i][i] in register *v
+    static __device__ __forceinline__ bool diag(unsigned i, unsigned c, unsigned *v)
+    {
+        *v = i >> 2;
+        return (i & 3) == c;
+    }
+};
+template <> struct FgMfma16<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc mma(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ bool diag(unsigned i, unsigned c, unsigned *v)
+    {
+        *v = i & 3;
+        return (i >> 2) == c;
+    }
+};
+
+template <typename T, int BS>
+__device__ __forceinline__ void fg_ebe_rows_mfma12(int b, const FgEbeLayout &L,
+                                                   const T *__restrict__ Kv, unsigned char *smem)
+{
+    constexpr int K = 12, G = K / 4;
+    constexpr int WAVES = FG_TPB / 64, NT = FG_TPB / 16 / WAVES;   // tiles per wave
+    static_assert(fg_ebe_vec(K) == 4 && fg_ebe_alias(K, 1), "grouped, aliased layout");
+    using M = FgMfma16<T>;
+    const int e0 = L.eoff[b];
+    const unsigned ne = (unsigned)(L.eoff[b + 1] - e0);   // <= FG_TPB (one element per thread)
+    const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
+    const int ninc = L.ibase[b + 1] - L.ibase[b];
+    FgEbeSmem<K, 1, T> S(smem, nd, ne, ninc);
+    const T *Kb = Kv + (size_t)K * K * e0;
+    const uint16_t *Lb = L.lml + (size_t)K * e0;
+    const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned i = lane & 15, c = lane >> 4;
+    // B operands of every tile first: the row sums overwrite the P values
+    T bp[NT][G];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const unsigned el = (wave + t * WAVES) * 16 + i;
+#pragma unroll
+        for (int g = 0; g < G; g++)
+            bp[t][g] = el < ne ? S.sP[fg_ld_nt(Lb, (4 * g + c) * ne + el)] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const unsigned base = (wave + t * WAVES) * 16;
+        if (base >= ne) break;   // wave-uniform: MFMAs run with the whole wave
+        const unsigned el = base + i;
+#pragma unroll
+        for (int rr = 0; rr < K; rr++) {
+            typename M::acc d = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                if (BS != 0 && g == rr / 4) continue;
+                const unsigned q = ((rr * G + g) * ne + el) * 4 + c;
+                T a = T(0);
+                if (el < ne) a = sizeof(T) == 4 ? fg_ld_nt(Kb, q) : fg_at(Kb, q);
+                d = M::mma(a, bp[t][g], d);
+            }
+            unsigned v;
+            if (M::diag(i, c, &v) && el < ne)
+                S.sY[rr * ne + el] = v == 0 ? d[0] : v == 1 ? d[1] : v == 2 ? d[2] : d[3];
+        }
     }
 }
 

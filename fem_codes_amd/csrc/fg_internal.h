@@ -108,6 +108,7 @@ struct fg_op {
     int nblk = 0, ept = 0, max_nloc = 0, nkept = 0;
     size_t smem64 = 0, smem32 = 0;  // LDS bytes per block (FP64 / FP32 values)
     int pipe_wpc = 0;            // 0: a block per workgroup; > 0: persistent, WGs per CU
+    int rows_mfma = 0;           // k = 12: element row sums on the matrix cores
     int *d_eoff = nullptr;       // [nblk+1] first element (blocked order) of each block
     int *d_doff = nullptr;       // [nblk+1] first local dof of each block
     int *d_ibase = nullptr;      // [nblk+1] first incidence of each block

@@ -123,6 +123,17 @@ extern "C" int fg_op_set_pipeline(fg_op *op, int wg_per_cu)
     return FG_OK;
 }
 
+extern "C" int fg_op_set_row_kernel(fg_op *op, int kind)
+{
+    FG_ARG(op, "op");
+    FG_ARG(kind == FG_ROWS_VALU || kind == FG_ROWS_MFMA, "kind: FG_ROWS_VALU or FG_ROWS_MFMA");
+    if (kind == FG_ROWS_MFMA)
+        FG_ARG(op->kind == FG_OP_EBE && op->k == 12 && op->ept == 1,
+               "matrix-core rows need a 12-dof element operator with 256 elements per block");
+    op->rows_mfma = kind == FG_ROWS_MFMA;
+    return FG_OK;
+}
+
 extern "C" double fg_op_algorithmic_bytes(const fg_op *op, int fp32)
 {
     if (!op) return 0.0;

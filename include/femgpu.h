@@ -132,6 +132,17 @@ int fg_op_time_apply(fg_op *op, int fp32, void *P, void *Q, int reps, int main_o
  * walking its blocks and loading the next block's element matrices while it
  * reduces the current one.  Results are identical either way. */
 int fg_op_set_pipeline(fg_op *op, int wg_per_cu);
+/* row sums of the 12 x 12 element products (no reference counterpart: a
+ * kernel choice).  FG_ROWS_VALU (default): each thread forms its element's
+ * row sums with vector multiply-adds.  FG_ROWS_MFMA: a wave takes 16
+ * elements at a time and forms their row sums with 16x16x4 matrix-core
+ * instructions (the diagonal of A_rows x [p_1 .. p_16]); one element block
+ * per workgroup whatever fg_op_set_pipeline says.  Needs k = 12 and 256
+ * elements per block; results agree to rounding (1e-14).  Set on the base
+ * operator of an ebe2 operator, it applies to the off-block product too. */
+#define FG_ROWS_VALU 0
+#define FG_ROWS_MFMA 1
+int fg_op_set_row_kernel(fg_op *op, int kind);
 
 /* ---- preconditioners ------------------------------------------------------ */
 #define FG_PREC_NONE    0   /* NO_precond.c / NO_precond_setup.c            */

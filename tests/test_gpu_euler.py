@@ -244,3 +244,75 @@ def test_jacobi_dof4_bitwise(ctx, path):
     assert _rel(X.get()[:neq], xs) <= 1e-8
     pc.close()
     op.close()
+
+
+@pytest.mark.parametrize("path", CASES, ids=os.path.basename)
+def test_mfma_rows_fixtures(ctx, path):
+    """the 12x12 row sums on the matrix cores (fg_op_set_row_kernel): the
+    reference's ebemvNDOF4 / ebe2mvNDOF4 outputs to the same tolerances"""
+    d = load_golden(path)
+    neq = int(d["neq"])
+    op, pc = _setup(ctx, d)
+    op.set_row_kernel(F.ROWS_MFMA)
+    Q = op.matvec(d["P"])
+    assert _rel(Q[:neq], d["Q"][:neq]) <= 1e-14 and Q[neq] == 0.0
+    Q32 = op.matvec(d["P"], np.float32)
+    assert _rel(Q32[:neq].astype(np.float64), d["Q"][:neq]) <= 2e-6
+    pc.setup(None)
+    for form, key in ((F.EBE2_BLOCKDIAG, "Q2_bd"), (F.EBE2_NOBLOCKDIAG, "Q2_not")):
+        e2 = F.Operator.ebe2(op, pc, form)
+        Q = e2.matvec(d["P"])
+        assert _rel(Q[:neq], d[key][:neq]) <= 1e-14 and Q[neq] == 0.0
+        Q32 = e2.matvec(d["P"], np.float32)
+        assert _rel(Q32[:neq].astype(np.float64), d[key][:neq]) <= 2e-6
+        e2.close()
+
+
+def test_mfma_rows_large_mesh(ctx):
+    """96^2 cells (18,432 elements of 12 dofs, 73 blocks, ragged last tiles),
+    a quarter of the nodes Dirichlet in one or more components, random
+    values: matrix-core rows against the vector rows and the oracle"""
+    rng = np.random.default_rng(44)
+    n = 96
+    x, y, t, v = F.structured_mesh2d(n)
+    nn = (n + 1) ** 2
+    ids = np.arange(4 * nn).reshape(nn, 4)
+    ids[rng.random((nn, 4)) < 0.25] = -1
+    free = ids >= 0
+    ids[free] = np.arange(int(free.sum()))
+    neq = int(free.sum())
+    ids[~free] = neq
+    tri = np.asarray(v, np.int64).reshape(-1, 3)
+    lm = ids[tri].reshape(-1).astype(np.int32)
+    nel = tri.shape[0]
+    A = rng.standard_normal(144 * nel)
+    P = rng.standard_normal(neq + 1)
+    P[neq] = 0.0
+    Qo = O.Oracle().ebemv(neq, nel, 12, A, lm, P.copy())
+    op = F.Operator.ebe(ctx, neq, lm, 12)
+    op.set_values(A)
+    Qv = op.matvec(P)
+    op.set_row_kernel(F.ROWS_MFMA)
+    Qm = op.matvec(P)
+    assert _rel(Qm[:neq], Qo[:neq]) <= 1e-14 and _rel(Qv[:neq], Qo[:neq]) <= 1e-14
+    assert Qm[neq] == 0.0
+    Qm32 = op.matvec(P, np.float32)
+    assert _rel(Qm32[:neq].astype(np.float64), Qo[:neq]) <= 2e-6
+    op.set_row_kernel(F.ROWS_VALU)
+    assert np.array_equal(op.matvec(P), Qv)
+    op.close()
+
+
+def test_mfma_rows_rejected_where_not_built(ctx):
+    d = load_golden(CASES[0])
+    neq = int(d["neq"])
+    op = F.Operator.ebe(ctx, neq, d["lm"], 12, 512)
+    with pytest.raises(F.FemGpuError):
+        op.set_row_kernel(F.ROWS_MFMA)
+    op.set_row_kernel(F.ROWS_VALU)
+    op.close()
+    x, y, t, v = F.structured_mesh2d(4)
+    tri = F.Operator.ebe(ctx, 25, np.asarray(v, np.int32), 3)
+    with pytest.raises(F.FemGpuError):
+        tri.set_row_kernel(F.ROWS_MFMA)
+    tri.close()
