@@ -6,7 +6,8 @@
 //   CSR pattern     csr_Initialization.c:5-218 (reordering NOT)
 //   EDE pattern     ede_Initialization.c:4-171
 // with counting sorts instead of the reference's sorted linked lists (same
-// sets, same numbering; O(nnz) instead of O(nnz * row length)).
+// sets, same numbering; O(nnz) instead of O(nnz * row length)), each pass
+// split over the host cores (fg_host.h).
 //
 // Device assembly:
 //   k_supg2d     one thread per triangle: Build_K_F_SUPG.c:35-149 (ke, Fe,
@@ -21,6 +22,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "fg_host.h"
 #include "fg_internal.h"
 
 
@@ -47,36 +49,62 @@ extern "C" int fg_fill_lm(int neq, int nel, int nnoel, const int *vertex, const 
 }
 
 namespace {
-// rows (0..nrows-1) of (row, col) pairs -> sorted unique columns per row
-// returns offsets (nrows+1) and columns; counting sort by row then sort each row
-void sorted_pattern(int nrows, const std::vector<int> &pr, const std::vector<int> &pc,
-                    std::vector<long long> &off, std::vector<int> &cols)
+// (row, col) pairs -> sorted unique columns of every row (rows 0..nrows-1).
+// pair(q, &r, &c) gives the pair at position q in [0, nq), or false when
+// position q contributes none.  Parallel: counts per row (atomic), columns
+// placed into their row's segment (atomic cursors: any order), each segment
+// sorted and made unique, the segments compacted.  The result is the same
+// whatever the thread count or timing.
+template <class Pair>
+int build_pattern(FgPool &pool, int nrows, long long nq, Pair &&pair, std::vector<long long> &off,
+                  std::vector<int> &cols, bool need_cols)
 {
-    off.assign((size_t)nrows + 1, 0);
-    for (size_t q = 0; q < pr.size(); q++) off[pr[q] + 1]++;
-    for (int r = 0; r < nrows; r++) off[r + 1] += off[r];
-    std::vector<int> tmp(pr.size());
+    std::vector<long long> seg((size_t)nrows + 1, 0);
+    pool.range(nq, [&](long long b, long long e, int) {
+        for (long long q = b; q < e; q++) {
+            int r, c;
+            if (pair(q, &r, &c)) __atomic_fetch_add(&seg[r], 1LL, __ATOMIC_RELAXED);
+        }
+    });
+    seg[nrows] = 0;
+    const long long total = fg_exclusive_scan(seg);
+    seg[nrows] = total;
+    std::vector<int> tmp(total > 0 ? total : 1);
     {
-        std::vector<long long> fill(off.begin(), off.end() - 1);
-        for (size_t q = 0; q < pr.size(); q++) tmp[fill[pr[q]]++] = pc[q];
+        std::vector<long long> cur(seg.begin(), seg.end() - 1);
+        pool.range(nq, [&](long long b, long long e, int) {
+            for (long long q = b; q < e; q++) {
+                int r, c;
+                if (pair(q, &r, &c)) tmp[__atomic_fetch_add(&cur[r], 1LL, __ATOMIC_RELAXED)] = c;
+            }
+        });
     }
+    off.assign((size_t)nrows + 1, 0);
+    pool.range(nrows, [&](long long b, long long e, int) {
+        for (long long r = b; r < e; r++) {
+            int *p0 = tmp.data() + seg[r], *p1 = tmp.data() + seg[r + 1];
+            std::sort(p0, p1);
+            off[r] = std::unique(p0, p1) - p0;
+        }
+    });
+    off[nrows] = 0;
+    const long long nnz = fg_exclusive_scan(off);
+    off[nrows] = nnz;
     cols.clear();
-    cols.reserve(pr.size());
-    std::vector<long long> noff((size_t)nrows + 1, 0);
-    for (int r = 0; r < nrows; r++) {
-        auto b = tmp.begin() + off[r], e = tmp.begin() + off[r + 1];
-        std::sort(b, e);
-        auto u = std::unique(b, e);
-        cols.insert(cols.end(), b, u);
-        noff[r + 1] = (long long)cols.size();
-    }
-    off.swap(noff);
+    if (!need_cols) return FG_OK;
+    cols.resize(nnz > 0 ? nnz : 1);
+    pool.range(nrows, [&](long long b, long long e, int) {
+        for (long long r = b; r < e; r++)
+            std::copy(tmp.data() + seg[r], tmp.data() + seg[r] + (off[r + 1] - off[r]),
+                      cols.data() + off[r]);
+    });
+    cols.resize(nnz);
+    return FG_OK;
 }
 
-int find_col(const std::vector<int> &cols, long long lo, long long hi, int c)
+int find_col(const int *cols, long long lo, long long hi, int c)
 {
-    auto it = std::lower_bound(cols.begin() + lo, cols.begin() + hi, c);
-    return (int)(it - cols.begin());
+    return (int)(std::lower_bound(cols + lo, cols + hi, c) - cols);
 }
 
 void ede_order(const int *l, int *o)
@@ -104,76 +132,147 @@ extern "C" int fg_csr_pattern(int neq, int nel, const int *lm, int *IA, int *JA,
 extern "C" int fg_csr_pattern_k(int neq, int nel, int k, const int *lm, int *IA, int *JA,
                                 int *csr_by_elem, long long *nnz_out)
 {
-    FG_ARG(lm && nnz_out && k >= 1 && k <= 16, "lm/nnz/k");
-    const int kk = k * k;
-    std::vector<int> pr, pc;
-    pr.reserve((size_t)nel * kk);
-    pc.reserve((size_t)nel * kk);
-    for (long e = 0; e < nel; e++)
-        for (int a = 0; a < k; a++) {
-            int I = lm[k * e + a];
-            if (I == neq) continue;
-            for (int b = 0; b < k; b++) {
-                int J = lm[k * e + b];
-                if (J != neq) { pr.push_back(I); pc.push_back(J); }
-            }
+    FG_ARG(lm && nnz_out && k >= 1 && k <= 16 && neq >= 0 && nel >= 0, "lm/nnz/k");
+    FG_ARG((long long)nel * k * k < (1LL << 31), "nel * k * k must fit in an int");
+    const long long nq = (long long)nel * k;
+    for (long long q = 0; q < nq; q++)
+        FG_ARG(lm[q] >= 0 && lm[q] <= neq, "lm entries must lie in [0, neq]");
+    FgPool pool;
+    // equation -> incident elements, ascending.  Worker t owns the
+    // equations [neq t / T, neq (t+1) / T): it streams lm and counts, then
+    // places, only its own equations (no atomics)
+    std::vector<long long> ioff((size_t)neq + 1, 0);
+    const int T = pool.size();
+    auto owned = [&](int t, int *r0, int *r1) {
+        *r0 = (int)((long long)neq * t / T);
+        *r1 = (int)((long long)neq * (t + 1) / T);
+    };
+    pool.run([&](int t) {
+        int r0, r1;
+        owned(t, &r0, &r1);
+        for (long long q = 0; q < nq; q++)
+            if (lm[q] >= r0 && lm[q] < r1) ioff[lm[q]]++;
+    });
+    ioff[neq] = 0;
+    const long long ninc = fg_exclusive_scan(ioff);
+    ioff[neq] = ninc;
+    // incidences as lm positions q = e k + a (element e, local row a)
+    std::vector<int> inc(ninc > 0 ? ninc : 1);
+    {
+        std::vector<long long> cur(ioff.begin(), ioff.end() - 1);
+        pool.run([&](int t) {
+            int r0, r1;
+            owned(t, &r0, &r1);
+            for (long long q = 0; q < nq; q++)
+                if (lm[q] >= r0 && lm[q] < r1) inc[cur[lm[q]]++] = (int)q;
+        });
+    }
+    // row r: the sorted, distinct non-sink dofs of its elements.  One pass:
+    // every worker keeps the columns of its contiguous rows, which are then
+    // copied to their offsets.
+    auto row_cols = [&](long long r, std::vector<int> &buf) {
+        buf.clear();
+        for (long long q = ioff[r]; q < ioff[r + 1]; q++) {
+            const int *le = lm + (long long)k * (inc[q] / k);
+            for (int b = 0; b < k; b++)
+                if (le[b] != neq) buf.push_back(le[b]);
         }
-    std::vector<long long> off;
-    std::vector<int> cols;
-    sorted_pattern(neq, pr, pc, off, cols);
-    long long nnz = (long long)cols.size();
+        std::sort(buf.begin(), buf.end());
+        buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+    };
+    std::vector<long long> off((size_t)neq + 1, 0);
+    std::vector<std::vector<int>> tcols(T);
+    std::vector<long long> tfirst(T, -1);
+    pool.range(neq, [&](long long b, long long e, int t) {
+        std::vector<int> buf;
+        tfirst[t] = b;
+        for (long long r = b; r < e; r++) {
+            row_cols(r, buf);
+            off[r] = (long long)buf.size();
+            if (JA) tcols[t].insert(tcols[t].end(), buf.begin(), buf.end());
+        }
+    });
+    off[neq] = 0;
+    const long long nnz = fg_exclusive_scan(off);
+    off[neq] = nnz;
     *nnz_out = nnz;
     if (!JA) return FG_OK;
     FG_ARG(nnz < 2147483647LL, "nnz fits int32");
+    FG_ARG(IA, "IA");
     for (int r = 0; r <= neq; r++) IA[r] = (int)off[r];
-    memcpy(JA, cols.data(), sizeof(int) * cols.size());
-    if (csr_by_elem)
-        for (long e = 0; e < nel; e++)
-            for (int a = 0; a < k; a++)
-                for (int b = 0; b < k; b++) {
-                    int I = lm[k * e + a], J = lm[k * e + b];
-                    csr_by_elem[kk * e + k * a + b] =
-                        (I != neq && J != neq) ? find_col(cols, off[I], off[I + 1], J) : (int)nnz;
+    pool.run([&](int t) {
+        if (tfirst[t] >= 0) std::copy(tcols[t].begin(), tcols[t].end(), JA + off[tfirst[t]]);
+    });
+    std::vector<std::vector<int>>().swap(tcols);
+    if (csr_by_elem) {
+        // rows of the sink: the sink slot nnz
+        pool.range(nq, [&](long long b, long long e, int) {
+            for (long long q = b; q < e; q++)
+                if (lm[q] == neq)
+                    for (int bb = 0; bb < k; bb++) csr_by_elem[(long long)k * q + bb] = (int)nnz;
+        });
+        // the other rows from their equation's incidences: the slot of each
+        // column in the (cache-resident) row
+        pool.range(neq, [&](long long b, long long e, int) {
+            for (long long r = b; r < e; r++)
+                for (long long q = ioff[r]; q < ioff[r + 1]; q++) {
+                    const long long qq = inc[q];
+                    const int *le = lm + (long long)k * (qq / k);
+                    for (int bb = 0; bb < k; bb++)
+                        csr_by_elem[(long long)k * qq + bb] =
+                            le[bb] != neq ? find_col(JA, off[r], off[r + 1], le[bb]) : (int)nnz;
                 }
+        });
+    }
     return FG_OK;
 }
 
 extern "C" int fg_ede_pattern(int neq, int nel, const int *lm, int *order, int *edge_by_elem,
                               int *lm2, int *nedge_out)
 {
-    FG_ARG(lm && nedge_out, "lm/nedge");
-    std::vector<int> pr, pc;
-    pr.reserve((size_t)nel * 3);
-    pc.reserve((size_t)nel * 3);
+    FG_ARG(lm && nedge_out && neq >= 0 && nel >= 0, "lm/nedge");
+    FG_ARG((long long)nel * 3 < (1LL << 31), "nel * 3 must fit in an int");
+    for (long long q = 0; q < 3LL * nel; q++)
+        FG_ARG(lm[q] >= 0 && lm[q] <= neq, "lm entries must lie in [0, neq]");
+    FgPool pool;
+    // each element's vertices in the comparison order, then its three edges
+    // (I1,I2), (I1,I3), (I2,I3) as (row, column) pairs
     std::vector<int> ord((size_t)nel * 3);
-    for (long e = 0; e < nel; e++) {
-        int *o = &ord[3 * e];
-        ede_order(lm + 3 * e, o);
-        int I1 = lm[3 * e + o[0]], I2 = lm[3 * e + o[1]], I3 = lm[3 * e + o[2]];
-        pr.push_back(I1); pc.push_back(I2);
-        pr.push_back(I1); pc.push_back(I3);
-        pr.push_back(I2); pc.push_back(I3);
-    }
+    pool.range(nel, [&](long long b, long long e1, int) {
+        for (long long e = b; e < e1; e++) ede_order(lm + 3 * e, &ord[3 * e]);
+    });
+    auto vtx = [&](long long e, int j) { return lm[3 * e + ord[3 * e + j]]; };
+    auto pair = [&](long long q, int *r, int *c) {
+        const long long e = q / 3;
+        const int j = (int)(q - 3 * e);
+        *r = vtx(e, j == 2 ? 1 : 0);
+        *c = vtx(e, j == 0 ? 1 : 2);
+        return true;
+    };
     std::vector<long long> off;
     std::vector<int> cols;
-    sorted_pattern(neq + 1, pr, pc, off, cols);   // rows 0..neq (row neq: the sink edge)
-    int nedge = (int)cols.size() - 1;            // ede_Initialization.c:103
+    // rows 0..neq (row neq: the sink edge)
+    FG_CHECK(build_pattern(pool, neq + 1, 3LL * nel, pair, off, cols, lm2 != nullptr));
+    const int nedge = (int)off[neq + 1] - 1;   // ede_Initialization.c:103
     *nedge_out = nedge;
     if (!lm2) return FG_OK;
     if (order) memcpy(order, ord.data(), sizeof(int) * ord.size());
     if (edge_by_elem)
-        for (long e = 0; e < nel; e++) {
-            const int *o = &ord[3 * e];
-            int I1 = lm[3 * e + o[0]], I2 = lm[3 * e + o[1]], I3 = lm[3 * e + o[2]];
-            edge_by_elem[3 * e + 0] = find_col(cols, off[I1], off[I1 + 1], I2);
-            edge_by_elem[3 * e + 1] = find_col(cols, off[I1], off[I1 + 1], I3);
-            edge_by_elem[3 * e + 2] = find_col(cols, off[I2], off[I2 + 1], I3);
-        }
-    for (int r = 0; r < neq; r++)
-        for (long long q = off[r]; q < off[r + 1]; q++) {
-            lm2[2 * q] = r;
-            lm2[2 * q + 1] = cols[q];
-        }
+        pool.range(nel, [&](long long b, long long e1, int) {
+            for (long long e = b; e < e1; e++) {
+                const int I1 = vtx(e, 0), I2 = vtx(e, 1), I3 = vtx(e, 2);
+                edge_by_elem[3 * e + 0] = find_col(cols.data(), off[I1], off[I1 + 1], I2);
+                edge_by_elem[3 * e + 1] = find_col(cols.data(), off[I1], off[I1 + 1], I3);
+                edge_by_elem[3 * e + 2] = find_col(cols.data(), off[I2], off[I2 + 1], I3);
+            }
+        });
+    pool.range(neq, [&](long long b, long long e1, int) {
+        for (long long r = b; r < e1; r++)
+            for (long long q = off[r]; q < off[r + 1]; q++) {
+                lm2[2 * q] = (int)r;
+                lm2[2 * q + 1] = cols[q];
+            }
+    });
     return FG_OK;
 }
 
