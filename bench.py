@@ -59,15 +59,23 @@ def log(*a):
 
 def cpu_info():
     """model name and core counts of the host (for the CPU baselines)"""
-    model = None
+    model, fields = None, {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                if not line.strip():
+                    break   # first processor only
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k in ("cpu family", "model", "cpu MHz", "cache size"):
+                    fields[k.replace(" ", "_")] = v
     except OSError:
         pass
+    if fields:
+        model = f"{model} (family {fields.get('cpu_family')}, model {fields.get('model')}, " \
+                f"{fields.get('cpu_MHz')} MHz, cache {fields.get('cache_size')})"
     try:
         avail = len(os.sched_getaffinity(0))
     except AttributeError:
