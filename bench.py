@@ -478,7 +478,8 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
     res = {"problems": int(nprob), "elements": int(nel), "equations": N,
            "sizes": {"min": int(sizes.min()), "max": int(sizes.max()),
                      "mean_cells_per_side": float(sizes.mean())},
-           "precond": "diag", "kmax": kmax, "tol": tol, "graph_replay": not args.no_graph}
+           "precond": "diag", "kmax": kmax, "tol": tol, "graph_replay": not args.no_graph,
+           "fp32_ir_orth": args.orth}
     # the batched product over every problem at once: the dominant kernel
     P32 = ctx.to_device(rng.standard_normal(N + 1).astype(np.float32))
     Q32 = ctx.empty(N + 1, np.float32)
@@ -493,7 +494,9 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
         ctx.sync()
         w0 = time.perf_counter()
         rc, infos = F.gmres_batched(ctx, op, pc, seg, B, X, kmax=kmax, lmax=100000, tol=tol,
-                                    precision=precision, graph=not args.no_graph)
+                                    precision=precision, graph=not args.no_graph,
+                                    orth=ORTHS[args.orth] if precision == F.GMRES_FP32_IR
+                                    else F.ORTH_MGS)
         ctx.sync()
         wall = time.perf_counter() - w0
         Xs[tag] = X.get()
@@ -564,7 +567,8 @@ def batched_tuned(F, ctx, args, N, lm, ke_all, seg, B, X64, tol, default_s, defa
         w0 = time.perf_counter()
         lmax = -(-3 * default_steps // km)
         rc, infos = F.gmres_batched(ctx, op_t, pc_t, seg, B, X, kmax=km, lmax=lmax, tol=tol,
-                                    precision=F.GMRES_FP32_IR, graph=not args.no_graph)
+                                    precision=F.GMRES_FP32_IR, graph=not args.no_graph,
+                                    orth=ORTHS[args.orth])
         ctx.sync()
         return time.perf_counter() - w0, rc == F.FG_OK, infos
 

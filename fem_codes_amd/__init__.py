@@ -523,14 +523,15 @@ def concat_lm(parts):
 
 def gmres_batched(ctx: Context, op: Operator, prec: Preconditioner | None, seg,
                   B: DeviceArray, X: DeviceArray, kmax=30, lmax=1000, tol=1e-8,
-                  precision=GMRES_FP64, inner_tol=0.0, graph=False):
+                  precision=GMRES_FP64, inner_tol=0.0, graph=False, orth=ORTH_MGS):
     """fg_gmres_batched: independent problems of a block-diagonal operator
-    (problem p = equations [seg[p], seg[p+1])) solved together.
+    (problem p = equations [seg[p], seg[p+1])) solved together.  orth:
+    ORTH_MGS, or ORTH_CGS for the FP32-IR inner cycle.
     Returns (status, [GmresInfo per problem])."""
     seg = np.ascontiguousarray(seg, np.int32)
     np_ = len(seg) - 1
     infos = (GmresInfo * np_)()
-    o = GmresOpts(kmax, lmax, tol, precision, inner_tol, 0, None, None, 0, ORTH_MGS)
+    o = GmresOpts(kmax, lmax, tol, precision, inner_tol, 0, None, None, 0, int(orth))
     rc = lib.fg_gmres_batched(ctx.h, op.h, prec.h if prec is not None else None, np_, seg,
                               B.ptr, X.ptr, C.byref(o), int(bool(graph)), C.cast(infos, _vp))
     _check(rc, "fg_gmres_batched", ok=(FG_OK, FG_ERR_NOCONV, FG_ERR_BREAKDOWN))

@@ -26,7 +26,9 @@
 //
 // Precision: FG_GMRES_FP64 (modified Gram-Schmidt, as pgmres.c) or
 // FG_GMRES_FP32_IR (FP32 basis / matvec / preconditioner inside, FP64
-// residual, Hessenberg least squares and X), as fg_gmres.
+// residual, Hessenberg least squares and X), as fg_gmres; its inner cycle
+// orthogonalises with MGS or, FG_ORTH_CGS, classical Gram-Schmidt (one pass
+// for all coefficients of a step, one update pass).
 #include <math.h>
 #include <string.h>
 
@@ -305,6 +307,120 @@ __global__ __launch_bounds__(FG_TPB) void k_b_axpy(Chunks C, BState *st, int i, 
     b_put(acc, pout);
 }
 
+// Classical Gram-Schmidt for the FP32 inner cycle (FG_ORTH_CGS): every
+// chunk keeps its BCH entries of w in registers (8 per thread).
+constexpr int BNV = BCH / FG_TPB;
+
+// [w = D w] ; partials of (u_j, w) for j = 0..i -> pout[j * nch + chunk]
+template <typename TB>
+__global__ __launch_bounds__(FG_TPB) void k_b_cgs_dots(Chunks C, const BState *st, int i, int nch,
+                                                       const TB *__restrict__ invD,
+                                                       TB *__restrict__ W,
+                                                       const TB *__restrict__ U, long long ld,
+                                                       double *__restrict__ pout)
+{
+    if (st[C.cp[blockIdx.x]].done) return;
+    __shared__ double red[FG_KMAX + 1][FG_TPB / 64];
+    const int cs = C.cs[blockIdx.x], ce = C.ce[blockIdx.x];
+    const int base = cs + (int)threadIdx.x;
+    TB w[BNV];
+#pragma unroll
+    for (int q = 0; q < BNV; q++) {
+        const int k = base + q * FG_TPB;
+        w[q] = k < ce ? W[k] : TB(0);
+    }
+    if (invD) {
+#pragma unroll
+        for (int q = 0; q < BNV; q++) {
+            const int k = base + q * FG_TPB;
+            if (k < ce) {
+                w[q] = w[q] * invD[k];
+                W[k] = w[q];
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int j = 0; j <= i; j++) {
+        const TB *Uj = U + (size_t)j * ld;
+        TB u[BNV];
+#pragma unroll
+        for (int q = 0; q < BNV; q++) {
+            const int k = base + q * FG_TPB;
+            u[q] = k < ce ? Uj[k] : TB(0);
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < BNV; q++) acc = acc + (double)u[q] * (double)w[q];
+        acc = fg_wave_sum(acc);
+        if (lane == 0) red[j][wid] = acc;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j <= i; j += FG_TPB) {
+        double v = red[j][0];
+#pragma unroll
+        for (int q = 1; q < FG_TPB / 64; q++) v = v + red[j][q];
+        pout[(size_t)j * nch + blockIdx.x] = v;
+    }
+}
+
+// h_{j,i} = problem p's chunk partials of coefficient j, in chunk order (one
+// block per problem)
+__global__ __launch_bounds__(FG_TPB) void k_b_cgs_reduce(Chunks C, BState *st, int i, int nch,
+                                                         const double *__restrict__ pin)
+{
+    const int p = blockIdx.x;
+    BState *s = st + p;
+    if (s->done) return;
+    for (int j = 0; j <= i; j++) {
+        const double h = b_sum(pin + (size_t)j * nch, C.pc0[p], C.pc1[p]);
+        if (threadIdx.x == 0) Hij(s, j, i) = h;
+    }
+}
+
+// w -= sum_j h_{j,i} u_j (j ascending) ; partials of ||w||^2
+template <typename TB>
+__global__ __launch_bounds__(FG_TPB) void k_b_cgs_update(Chunks C, const BState *st, int i,
+                                                         TB *__restrict__ W,
+                                                         const TB *__restrict__ U, long long ld,
+                                                         double *__restrict__ pout)
+{
+    const BState *s = st + C.cp[blockIdx.x];
+    if (s->done) return;
+    __shared__ TB a[FG_KMAX + 1];
+    for (int j = threadIdx.x; j <= i; j += FG_TPB) a[j] = (TB)(-s->H[j * LDH + i]);
+    __syncthreads();
+    const int cs = C.cs[blockIdx.x], ce = C.ce[blockIdx.x];
+    const int base = cs + (int)threadIdx.x;
+    TB w[BNV];
+#pragma unroll
+    for (int q = 0; q < BNV; q++) {
+        const int k = base + q * FG_TPB;
+        w[q] = k < ce ? W[k] : TB(0);
+    }
+    for (int j = 0; j <= i; j++) {
+        const TB *Uj = U + (size_t)j * ld;
+        const TB aj = a[j];
+        TB u[BNV];
+#pragma unroll
+        for (int q = 0; q < BNV; q++) {
+            const int k = base + q * FG_TPB;
+            u[q] = k < ce ? Uj[k] : TB(0);
+        }
+#pragma unroll
+        for (int q = 0; q < BNV; q++) w[q] = w[q] + aj * u[q];
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < BNV; q++) {
+        const int k = base + q * FG_TPB;
+        if (k < ce) {
+            W[k] = w[q];
+            acc = acc + (double)w[q] * (double)w[q];
+        }
+    }
+    b_put(acc, pout);
+}
+
 // h_{i+1,i} = ||w|| ; u_{i+1} = w / h_{i+1,i} ; the first chunk of each
 // problem applies the Givens rotations of its column (pgmres.c:118-163)
 template <typename TB>
@@ -420,7 +536,7 @@ struct Parts {
 template <typename TB>
 int b_cycle(fg_ctx *c, fg_op *op, fg_prec *pc, const Chunks &C, int nch, int np, BState *st,
             TB *U, long long ld, TB *W, const double *W64, double *X, int kmax, int lmax, int ir,
-            Parts &pp)
+            Parts &pp, double *cgparts)
 {
     const int n = op->neq, fp32 = sizeof(TB) == 4;
     const TB *invD = nullptr;
@@ -437,6 +553,19 @@ int b_cycle(fg_ctx *c, fg_op *op, fg_prec *pc, const Chunks &C, int nch, int np,
         TB *Ui = U + (size_t)i * ld;
         FG_CHECK(fg_op_apply_any(op, fp32, Ui, W, nullptr));
         if (pc && !diag) FG_CHECK(fg_prec_apply_any(pc, fp32, W, W, nullptr));
+        if (cgparts) {
+            // CGS: all coefficients from one pass, one update pass
+            FG_LAUNCH(k_b_cgs_dots<TB>, nch, FG_TPB, 0, c->stream, C, (const BState *)st, i, nch,
+                      invD, W, (const TB *)U, ld, cgparts);
+            FG_LAUNCH(k_b_cgs_reduce, np, FG_TPB, 0, c->stream, C, st, i, nch,
+                      (const double *)cgparts);
+            FG_LAUNCH(k_b_cgs_update<TB>, nch, FG_TPB, 0, c->stream, C, (const BState *)st, i, W,
+                      (const TB *)U, ld, pp.out());
+            pp.flip();
+            FG_LAUNCH(k_b_givens<TB>, nch, FG_TPB, 0, c->stream, C, st, i, (const TB *)W,
+                      U + (size_t)(i + 1) * ld, (const double *)pp.in(), n);
+            continue;
+        }
         FG_LAUNCH(k_b_dot0<TB>, nch, FG_TPB, 0, c->stream, C, (const BState *)st, invD, W,
                   (const TB *)U, pp.out());
         pp.flip();
@@ -487,7 +616,9 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
     FG_ARG(o->kmax >= 1 && o->kmax <= FG_KMAX, "kmax in [1,64]");
     FG_ARG(o->lmax >= 1, "lmax >= 1");
     FG_ARG(o->precision == FG_GMRES_FP64 || o->precision == FG_GMRES_FP32_IR, "precision");
-    FG_ARG(o->orth == FG_ORTH_MGS, "the batched solver orthogonalises with MGS");
+    FG_ARG(o->orth == FG_ORTH_MGS || (o->orth == FG_ORTH_CGS &&
+                                       o->precision == FG_GMRES_FP32_IR),
+           "orth: MGS, or CGS for the FP32 inner cycle");
     if (pc && pc->kind == FG_PREC_NONE) pc = nullptr;
     FG_ARG(!pc || (pc->op->neq == op->neq && pc->op->ctx == c),
            "preconditioner built on an operator of another size or context");
@@ -536,6 +667,8 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
     const size_t o_tab = cv.take(sizeof(int) * ntab), o_W64 = cv.take(sizeof(double) * ld);
     const size_t o_parts = cv.take(sizeof(double) * 2 * (nch ? nch : 1));
     const size_t o_U = cv.take(sb * ld * (kmax + 1)), o_W = cv.take(sb * ld);
+    const bool cgs = o->orth == FG_ORTH_CGS;
+    const size_t o_cg = cgs ? cv.take(sizeof(double) * (kmax + 1) * (nch ? nch : 1)) : 0;
     char *wsb = nullptr;
     FG_CHECK(fg_ctx_workspace(c, cv.off, (void **)&wsb));
     wk.st = (BState *)(wsb + o_st);
@@ -545,6 +678,7 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
     wk.parts = (double *)(wsb + o_parts);
     wk.U = wsb + o_U;
     wk.W = wsb + o_W;
+    double *cgparts = cgs ? (double *)(wsb + o_cg) : nullptr;
     {
         std::vector<int> tab(ntab, 0);
         int *t = tab.data();
@@ -589,9 +723,11 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
             if (!wk.ge) {
                 FG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int rc = ir ? b_cycle<float>(c, op, pc, C, nch, np, wk.st, (float *)wk.U, ld,
-                                             (float *)wk.W, wk.W64, X, kmax, o->lmax, 1, pp)
+                                             (float *)wk.W, wk.W64, X, kmax, o->lmax, 1, pp,
+                                             cgparts)
                             : b_cycle<double>(c, op, pc, C, nch, np, wk.st, (double *)wk.U, ld,
-                                              (double *)wk.W, wk.W64, X, kmax, o->lmax, 0, pp);
+                                              (double *)wk.W, wk.W64, X, kmax, o->lmax, 0, pp,
+                                              nullptr);
                 hipError_t e = hipStreamEndCapture(c->stream, &wk.g);
                 FG_CHECK(rc);
                 if (e != hipSuccess) {
@@ -603,10 +739,10 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
             FG_HIP(hipGraphLaunch(wk.ge, c->stream));
         } else if (ir) {
             FG_CHECK(b_cycle<float>(c, op, pc, C, nch, np, wk.st, (float *)wk.U, ld,
-                                    (float *)wk.W, wk.W64, X, kmax, o->lmax, 1, pp));
+                                    (float *)wk.W, wk.W64, X, kmax, o->lmax, 1, pp, cgparts));
         } else {
             FG_CHECK(b_cycle<double>(c, op, pc, C, nch, np, wk.st, (double *)wk.U, ld,
-                                     (double *)wk.W, wk.W64, X, kmax, o->lmax, 0, pp));
+                                     (double *)wk.W, wk.W64, X, kmax, o->lmax, 0, pp, nullptr));
         }
         iter++;
     }

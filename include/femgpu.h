@@ -519,8 +519,9 @@ int fg_gmres_small_noalloc(fg_ctx *ctx, fg_op *op, fg_prec *pc, const double *B,
  * left-preconditioned.  Every problem has its own Krylov basis, Hessenberg
  * matrix, rotations and stopping test tol * ||B_p|| and its own cycle count
  * (opts->lmax each); the problems advance in lockstep and a finished
- * problem drops out of the kernels.  Precision FP64 or FP32-IR; MGS only;
- * opts->rho_hist / e0_hist and engine are ignored.  graph = 1 captures one
+ * problem drops out of the kernels.  Precision FP64 (MGS) or FP32-IR (inner
+ * cycle MGS, or CGS with opts->orth = FG_ORTH_CGS); opts->rho_hist /
+ * e0_hist and engine are ignored.  graph = 1 captures one
  * restart cycle into a hipGraph and replays it every cycle (preconditioner
  * NONE, Diag or element Jacobi).  info[nprob] (may be NULL).  Returns FG_OK
  * when every problem converged, else FG_ERR_NOCONV or FG_ERR_BREAKDOWN. */

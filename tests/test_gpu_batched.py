@@ -89,9 +89,11 @@ def test_batched_fp64_diag(ctx):
                      int(d["neq"]), _scheme_spread(path, "X_diag"))
 
 
-@pytest.mark.parametrize("precision", [F.GMRES_FP64, F.GMRES_FP32_IR])
+@pytest.mark.parametrize("precision,orth", [(F.GMRES_FP64, F.ORTH_MGS),
+                                            (F.GMRES_FP32_IR, F.ORTH_MGS),
+                                            (F.GMRES_FP32_IR, F.ORTH_CGS)])
 @pytest.mark.parametrize("prec", [F.PREC_DIAG, F.PREC_JACOBI])
-def test_batched_graph_bitwise(ctx, precision, prec):
+def test_batched_graph_bitwise(ctx, precision, orth, prec):
     """graph replay of the cycle = direct launches, bit for bit, on two solves
     in a row (the graph is rebuilt per call; each call replays it every cycle)"""
     ds, op, seg, B = _batch(ctx, [p for p in EBE if "rot" not in p])
@@ -102,23 +104,25 @@ def test_batched_graph_bitwise(ctx, precision, prec):
     for graph in (False, True, True):
         X = ctx.empty(len(B))
         rc, infos = F.gmres_batched(ctx, op, pc, seg, Bd, X, kmax=20, lmax=200, tol=1e-8,
-                                    precision=precision, graph=graph)
+                                    precision=precision, graph=graph, orth=orth)
         assert rc == F.FG_OK
         out.append((X.get(), [(f.steps, f.cycles) for f in infos]))
     for X, st in out[1:]:
         assert np.array_equal(X, out[0][0]) and st == out[0][1]
 
 
-def test_batched_fp32_ir(ctx):
-    """FP32-IR: every problem reaches tol with its FP64 residual, and X is
-    within 1e-6 of the reference's Diag-preconditioned solution"""
+@pytest.mark.parametrize("orth", [F.ORTH_MGS, F.ORTH_CGS])
+def test_batched_fp32_ir(ctx, orth):
+    """FP32-IR (inner cycle MGS or CGS): every problem reaches tol with its
+    FP64 residual, and X is within 1e-6 of the reference's Diag-preconditioned
+    solution"""
     paths = [p for p in EBE if "rot" not in p]
     ds, op, seg, B = _batch(ctx, paths)
     pc = F.Preconditioner(op, F.PREC_DIAG)
     Bd, X = ctx.to_device(B), ctx.empty(len(B))
     pc.setup(Bd)
     rc, infos = F.gmres_batched(ctx, op, pc, seg, Bd, X, kmax=30, lmax=200, tol=1e-8,
-                                precision=F.GMRES_FP32_IR, graph=True)
+                                precision=F.GMRES_FP32_IR, graph=True, orth=orth)
     assert rc == F.FG_OK
     Xh = X.get()
     for i, d in enumerate(ds):
@@ -149,3 +153,12 @@ def test_batched_uneven_limits(ctx):
             d = ds[k]
             assert _rel(_split(X.get(), seg, i), d["X_not"][:int(d["neq"])]) <= 1e-10
         k += 1
+
+
+def test_batched_cgs_needs_fp32_ir(ctx):
+    """CGS is offered for the FP32 inner cycle only (the FP64 drop-in keeps
+    the reference's MGS)"""
+    ds, op, seg, B = _batch(ctx, EBE[:2])
+    Bd, X = ctx.to_device(B), ctx.empty(len(B))
+    with pytest.raises(F.FemGpuError):
+        F.gmres_batched(ctx, op, None, seg, Bd, X, precision=F.GMRES_FP64, orth=F.ORTH_CGS)
