@@ -45,24 +45,28 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_stream(
     }
     if (j1 - j0 <= FG_CSR_NNZB) {
         // AA and JA are read once per product: non-temporal, so that they do
-        // not evict the P entries other row blocks gather again
-        int col[FG_CSR_NPT];
-        T a[FG_CSR_NPT], p[FG_CSR_NPT];
+        // not evict the P entries other row blocks gather again.  Slots past
+        // the block's last nonzero load that nonzero again and store to a
+        // dummy LDS word: every load is issued on every path (no branch per
+        // load), so the memory counter waits only for what each step needs.
+        __shared__ T dummy;
+        if (j1 > j0) {
+            int col[FG_CSR_NPT];
+            T a[FG_CSR_NPT], p[FG_CSR_NPT];
 #pragma unroll
-        for (int q = 0; q < FG_CSR_NPT; q++) {
-            const int j = j0 + (int)threadIdx.x + q * FG_TPB;
-            col[q] = j < j1 ? __builtin_nontemporal_load(&JA[j]) : 0;
-            a[q] = j < j1 ? __builtin_nontemporal_load(&AA[j]) : T(0);
-        }
+            for (int q = 0; q < FG_CSR_NPT; q++) {
+                const int j = j0 + (int)threadIdx.x + q * FG_TPB;
+                const int jc = j < j1 ? j : j1 - 1;
+                col[q] = __builtin_nontemporal_load(&JA[jc]);
+                a[q] = __builtin_nontemporal_load(&AA[jc]);
+            }
 #pragma unroll
-        for (int q = 0; q < FG_CSR_NPT; q++) {
-            const int j = j0 + (int)threadIdx.x + q * FG_TPB;
-            p[q] = j < j1 ? P[col[q]] : T(0);
-        }
+            for (int q = 0; q < FG_CSR_NPT; q++) p[q] = P[col[q]];
 #pragma unroll
-        for (int q = 0; q < FG_CSR_NPT; q++) {
-            const int j = (int)threadIdx.x + q * FG_TPB;
-            if (j < j1 - j0) s[j] = a[q] * p[q];
+            for (int q = 0; q < FG_CSR_NPT; q++) {
+                const int j = (int)threadIdx.x + q * FG_TPB;
+                *(j < j1 - j0 ? &s[j] : &dummy) = a[q] * p[q];
+            }
         }
         __syncthreads();
         if (r < r1) {

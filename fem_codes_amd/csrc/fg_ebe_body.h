@@ -147,20 +147,22 @@ __device__ __forceinline__ void fg_ebe_load(int b, const FgEbeLayout &L, const T
     const unsigned ne = (unsigned)(L.eoff[b + 1] - e0);
     const T *Kb = Kv + (size_t)K * K * e0;
     const uint16_t *Lb = L.lml + (size_t)K * e0;
+    // threads past the block's last element load its last element again
+    // (ne >= 1), so every load is issued on every path and the memory counter
+    // waits stay exact; their results are never stored
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
-        const unsigned el = threadIdx.x + i * FG_TPB;
-        if (el < ne) {
+        const unsigned el0 = threadIdx.x + i * FG_TPB;
+        const unsigned el = el0 < ne ? el0 : ne - 1;
 #pragma unroll
-            for (int j = 0; j < K; j++) r.li[i][j] = fg_ld_nt(Lb, j * ne + el);
-            if constexpr (FgEbeRegs<K, EPT, T>::PRE) {
-                if constexpr (fg_ebe_vec(K) == 4) {
+        for (int j = 0; j < K; j++) r.li[i][j] = fg_ld_nt(Lb, j * ne + el);
+        if constexpr (FgEbeRegs<K, EPT, T>::PRE) {
+            if constexpr (fg_ebe_vec(K) == 4) {
 #pragma unroll
-                    for (int g = 0; g < K * K / 4; g++) fg_ld4(Kb, g * ne + el, &r.kv[i][4 * g]);
-                } else {
+                for (int g = 0; g < K * K / 4; g++) fg_ld4(Kb, g * ne + el, &r.kv[i][4 * g]);
+            } else {
 #pragma unroll
-                    for (int q = 0; q < K * K; q++) r.kv[i][q] = fg_ld_nt(Kb, q * ne + el);
-                }
+                for (int q = 0; q < K * K; q++) r.kv[i][q] = fg_ld_nt(Kb, q * ne + el);
             }
         }
     }
@@ -425,6 +427,85 @@ __device__ __forceinline__ void fg_ebe_rows_mfma12(int b, const FgEbeLayout &L,
     }
 }
 
+// load and gather of one block in the order the memory counter needs.  A
+// wave waits for its loads in issue order (a wait for one load is a wait for
+// every load issued before it), so the loads are issued as: the block's
+// incidence lists, dof ids, halo slots and incidence offsets; then the P
+// gathers (which wait for the dof ids only); then the element values and
+// local indices -- and the LDS writes, which wait for the first two groups,
+// come after all of them.  The value stream is thus in flight during the
+// whole gather instead of being waited for by its first LDS write.
+template <int K, int EPT, typename T>
+__device__ __forceinline__ void fg_ebe_prologue(int b, const FgEbeLayout &L,
+                                                const T *__restrict__ Kv,
+                                                const T *__restrict__ P, unsigned char *smem,
+                                                FgEbeRegs<K, EPT, T> &r)
+{
+    const int d0 = L.doff[b], nd = L.doff[b + 1] - d0;
+    const int e0 = L.eoff[b], ne = L.eoff[b + 1] - e0;
+    const int i0 = L.ibase[b], ninc = L.ibase[b + 1] - i0;
+    FgEbeSmem<K, EPT, T> S(smem, nd, ne, ninc);
+    const uint16_t *io = L.ioff + d0 + b;
+    const int *Db = L.dofs + d0, *Sb = L.slots + d0;
+    const uint4 *Ib = reinterpret_cast<const uint4 *>(L.inc + i0);
+    uint4 *sInc4 = reinterpret_cast<uint4 *>(S.sInc);
+    constexpr int ND = FgEbeRegs<K, EPT, T>::ND;
+    // incidence lists: at most K * 256 * EPT slots, in 16-byte groups
+    constexpr int NI = (K * EPT + 7) / 8;
+    const int ninc8 = ninc >> 3;
+    // indices past a list's end are clamped to its last entry (every block
+    // has at least one incidence group and one dof): all loads are issued on
+    // every path, the surplus results are not stored
+    uint4 ic[NI];
+#pragma unroll
+    for (int v = 0; v < NI; v++) {
+        const int q = threadIdx.x + v * FG_TPB;
+        ic[v] = fg_at(Ib, (unsigned)(q < ninc8 ? q : ninc8 - 1));
+    }
+    uint16_t iv[ND];
+#pragma unroll
+    for (int u = 0; u < ND; u++) {
+        const unsigned l0 = threadIdx.x + u * FG_TPB;
+        const unsigned l = (int)l0 < nd ? l0 : nd - 1;
+        r.gd[u] = fg_at(Db, l);
+        r.sd[u] = fg_at(Sb, l);
+        iv[u] = fg_at(io, l);
+    }
+    const uint16_t iend = io[nd];
+    T pv[ND];
+#pragma unroll
+    for (int u = 0; u < ND; u++) pv[u] = fg_at(P, (unsigned)r.gd[u]);
+    fg_ebe_load<K, EPT, T>(b, L, Kv, r);
+    // the LDS writes are unconditional (surplus lanes write a dummy slot), so
+    // that the compiler keeps the loads above where they are instead of
+    // sinking each into a branch next to its only store
+    __shared__ uint4 dummy4;
+    __shared__ T dummyT;
+    __shared__ uint16_t dummy16;
+#pragma unroll
+    for (int v = 0; v < NI; v++) {
+        const int q = threadIdx.x + v * FG_TPB;
+        *(q < ninc8 ? &sInc4[q] : &dummy4) = ic[v];
+    }
+    for (int q = threadIdx.x + NI * FG_TPB; q < ninc8; q += FG_TPB) sInc4[q] = fg_at(Ib, (unsigned)q);
+#pragma unroll
+    for (int u = 0; u < ND; u++) {
+        const unsigned l = threadIdx.x + u * FG_TPB;
+        const bool in = (int)l < nd;
+        *(in ? &S.sP[l] : &dummyT) = pv[u];
+        *(in ? &S.sIoff[l] : &dummy16) = iv[u];
+    }
+    for (unsigned l = threadIdx.x + ND * FG_TPB; (int)l < nd; l += FG_TPB) {
+        S.sP[l] = fg_at(P, (unsigned)fg_at(Db, l));
+        S.sIoff[l] = fg_at(io, l);
+    }
+    if (threadIdx.x == 0) {
+        S.sP[nd] = T(0);
+        S.sIoff[nd] = iend;
+    }
+    __syncthreads();
+}
+
 // the three phases back to back (one block per workgroup)
 template <int K, int EPT, typename T, int BS = 0>
 __device__ __forceinline__ void fg_ebe_block(int b, const FgEbeLayout &L, const T *__restrict__ Kv,
@@ -432,7 +513,6 @@ __device__ __forceinline__ void fg_ebe_block(int b, const FgEbeLayout &L, const 
                                              T *__restrict__ hpart, unsigned char *smem)
 {
     FgEbeRegs<K, EPT, T> r;
-    fg_ebe_load<K, EPT, T>(b, L, Kv, r);
-    fg_ebe_gather<K, EPT, T>(b, L, P, smem, r);
+    fg_ebe_prologue<K, EPT, T>(b, L, Kv, P, smem, r);
     fg_ebe_finish<K, EPT, T, BS>(b, L, Kv, Q, hpart, smem, r);
 }

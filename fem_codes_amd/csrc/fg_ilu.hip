@@ -487,24 +487,27 @@ __global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nrb, const int *__rest
         // the factor's values and columns are read once per sweep
         // (non-temporal); every thread issues all of its loads of the block
         // before the gathers of x, so TRI_NPT load chains are in flight
+        // (slots past the block's last entry load that entry again and store
+        // to a dummy LDS word: no branch per load, exact counter waits)
         constexpr int TRI_NPT = TRI_NNZB / FG_TPB;
-        int col[TRI_NPT];
-        T v[TRI_NPT], xv[TRI_NPT];
+        __shared__ T dummy;
+        if (j1 > j0) {
+            int col[TRI_NPT];
+            T v[TRI_NPT], xv[TRI_NPT];
 #pragma unroll
-        for (int q = 0; q < TRI_NPT; q++) {
-            const int j = j0 + (int)threadIdx.x + q * FG_TPB;
-            col[q] = j < j1 ? __builtin_nontemporal_load(&JA[j]) : 0;
-            v[q] = j < j1 ? __builtin_nontemporal_load(&V[j]) : T(0);
-        }
+            for (int q = 0; q < TRI_NPT; q++) {
+                const int j = j0 + (int)threadIdx.x + q * FG_TPB;
+                const int jc = j < j1 ? j : j1 - 1;
+                col[q] = __builtin_nontemporal_load(&JA[jc]);
+                v[q] = __builtin_nontemporal_load(&V[jc]);
+            }
 #pragma unroll
-        for (int q = 0; q < TRI_NPT; q++) {
-            const int j = j0 + (int)threadIdx.x + q * FG_TPB;
-            xv[q] = j < j1 ? x[col[q]] : T(0);
-        }
+            for (int q = 0; q < TRI_NPT; q++) xv[q] = x[col[q]];
 #pragma unroll
-        for (int q = 0; q < TRI_NPT; q++) {
-            const int j = (int)threadIdx.x + q * FG_TPB;
-            if (j < j1 - j0) s[j] = v[q] * xv[q];
+            for (int q = 0; q < TRI_NPT; q++) {
+                const int j = (int)threadIdx.x + q * FG_TPB;
+                *(j < j1 - j0 ? &s[j] : &dummy) = v[q] * xv[q];
+            }
         }
         __syncthreads();
         if (r < r1) {
