@@ -105,8 +105,9 @@ __global__ __launch_bounds__(FG_TPB) void k_res_part(int n, const double *__rest
 #pragma unroll
         for (int q = 0; q < VU; q++) {
             const int k = base + q * FG_TPB;
-            b[q] = k < n ? B[k] : 0.0;
-            w[q] = k < n ? W[k] : 0.0;
+            const int kc = k < n ? k : n - 1;
+            b[q] = B[kc];
+            w[q] = W[kc];
         }
 #pragma unroll
         for (int q = 0; q < VU; q++) {
@@ -173,9 +174,10 @@ __global__ __launch_bounds__(FG_TPB) void k_prec_dot0(int n, const TB *__restric
 #pragma unroll
         for (int q = 0; q < VU; q++) {
             const int k = base + q * FG_TPB;
-            w[q] = k < n ? W[k] : TB(0);
-            u[q] = k < n ? U0[k] : TB(0);
-            d[q] = (invD && k < n) ? invD[k] : TB(1);
+            const int kc = k < n ? k : n - 1;
+            w[q] = W[kc];
+            u[q] = U0[kc];
+            d[q] = invD ? invD[kc] : TB(1);
         }
 #pragma unroll
         for (int q = 0; q < VU; q++) {
@@ -212,9 +214,10 @@ __global__ __launch_bounds__(FG_TPB) void k_axpy_dot(int n, int i, int j, TB *__
 #pragma unroll
         for (int q = 0; q < VU; q++) {
             const int k = base + q * FG_TPB;
-            w[q] = k < n ? W[k] : TB(0);
-            u[q] = k < n ? Uj[k] : TB(0);
-            v[q] = k < n ? Uj1[k] : TB(0);
+            const int kc = k < n ? k : n - 1;
+            w[q] = W[kc];
+            u[q] = Uj[kc];
+            v[q] = Uj1[kc];
         }
 #pragma unroll
         for (int q = 0; q < VU; q++) {
@@ -246,8 +249,9 @@ __global__ __launch_bounds__(FG_TPB) void k_axpy_norm(int n, int i, TB *__restri
 #pragma unroll
         for (int q = 0; q < VU; q++) {
             const int k = base + q * FG_TPB;
-            w[q] = k < n ? W[k] : TB(0);
-            u[q] = k < n ? Ui[k] : TB(0);
+            const int kc = k < n ? k : n - 1;
+            w[q] = W[kc];
+            u[q] = Ui[kc];
         }
 #pragma unroll
         for (int q = 0; q < VU; q++) {
@@ -284,7 +288,7 @@ __global__ __launch_bounds__(FG_TPB) void k_scale_givens(int n, int i, const TB 
 #pragma unroll
         for (int q = 0; q < VU; q++) {
             const int k = base + q * FG_TPB;
-            w[q] = k < n ? W[k] : TB(0);
+            w[q] = W[k < n ? k : n - 1];
         }
 #pragma unroll
         for (int q = 0; q < VU; q++) {
@@ -349,22 +353,31 @@ __global__ __launch_bounds__(FG_TPB) void k_cgs_dots(int n, int i, const TB *__r
                                                      const TB *__restrict__ U, long long ld,
                                                      const GmState *st, double *__restrict__ pout)
 {
-    if (st->done) return;
+    if (st->done || n <= 0) return;
     __shared__ double red[FG_KMAX + 1][FG_TPB / 64];
     const int G = gridDim.x;
     const int base = blockIdx.x * FG_TPB * CG_NV + threadIdx.x;
+    // loads past n read entry n - 1 (issued on every path: exact counter
+    // waits) and are zeroed by a select
     TB w[CG_NV];
 #pragma unroll
     for (int q = 0; q < CG_NV; q++) {
         const int k = base + q * FG_TPB;
-        w[q] = k < n ? W[k] : TB(0);
+        const TB t = W[k < n ? k : n - 1];
+        w[q] = k < n ? t : TB(0);
     }
     if (invD) {
+        TB d[CG_NV];
+#pragma unroll
+        for (int q = 0; q < CG_NV; q++) {
+            const int k = base + q * FG_TPB;
+            d[q] = invD[k < n ? k : n - 1];
+        }
 #pragma unroll
         for (int q = 0; q < CG_NV; q++) {
             const int k = base + q * FG_TPB;
             if (k < n) {
-                w[q] = w[q] * invD[k];
+                w[q] = w[q] * d[q];
                 W[k] = w[q];
             }
         }
@@ -376,7 +389,7 @@ __global__ __launch_bounds__(FG_TPB) void k_cgs_dots(int n, int i, const TB *__r
 #pragma unroll
         for (int q = 0; q < CG_NV; q++) {
             const int k = base + q * FG_TPB;
-            u[q] = k < n ? Uj[k] : TB(0);
+            u[q] = Uj[k < n ? k : n - 1];
         }
         double acc = 0.0;
 #pragma unroll
@@ -416,7 +429,7 @@ __global__ __launch_bounds__(FG_TPB) void k_cgs_update(int n, int i, TB *__restr
                                                        const GmState *st,
                                                        double *__restrict__ pout)
 {
-    if (st->done) return;
+    if (st->done || n <= 0) return;
     __shared__ TB a[FG_KMAX + 1];
     for (int j = threadIdx.x; j <= i; j += FG_TPB) a[j] = (TB)(-st->hc[j]);
     __syncthreads();
@@ -425,7 +438,7 @@ __global__ __launch_bounds__(FG_TPB) void k_cgs_update(int n, int i, TB *__restr
 #pragma unroll
     for (int q = 0; q < CG_NV; q++) {
         const int k = base + q * FG_TPB;
-        w[q] = k < n ? W[k] : TB(0);
+        w[q] = W[k < n ? k : n - 1];
     }
     for (int j = 0; j <= i; j++) {
         const TB *Uj = U + (size_t)j * ld;
@@ -434,7 +447,7 @@ __global__ __launch_bounds__(FG_TPB) void k_cgs_update(int n, int i, TB *__restr
 #pragma unroll
         for (int q = 0; q < CG_NV; q++) {
             const int k = base + q * FG_TPB;
-            u[q] = k < n ? Uj[k] : TB(0);
+            u[q] = Uj[k < n ? k : n - 1];
         }
 #pragma unroll
         for (int q = 0; q < CG_NV; q++) w[q] = w[q] + aj * u[q];

@@ -140,8 +140,9 @@ __global__ __launch_bounds__(FG_TPB) void k_b_res_part(Chunks C, const BState *s
 #pragma unroll
         for (int q = 0; q < BU; q++) {
             const int k = k0 + q * FG_TPB;
-            b[q] = k < ce_ ? B[k] : 0.0;
-            w[q] = k < ce_ ? W[k] : 0.0;
+            const int kc = k < ce_ ? k : ce_ - 1;
+            b[q] = B[kc];
+            w[q] = W[kc];
         }
 #pragma unroll
         for (int q = 0; q < BU; q++) {
@@ -249,9 +250,10 @@ __global__ __launch_bounds__(FG_TPB) void k_b_dot0(Chunks C, const BState *st,
 #pragma unroll
         for (int q = 0; q < BU; q++) {
             const int k = k0 + q * FG_TPB;
-            w[q] = k < ce_ ? W[k] : TB(0);
-            u[q] = k < ce_ ? U0[k] : TB(0);
-            d[q] = (invD && k < ce_) ? invD[k] : TB(1);
+            const int kc = k < ce_ ? k : ce_ - 1;
+            w[q] = W[kc];
+            u[q] = U0[kc];
+            d[q] = invD ? invD[kc] : TB(1);
         }
 #pragma unroll
         for (int q = 0; q < BU; q++) {
@@ -290,9 +292,10 @@ __global__ __launch_bounds__(FG_TPB) void k_b_axpy(Chunks C, BState *st, int i, 
 #pragma unroll
         for (int q = 0; q < BU; q++) {
             const int k = k0 + q * FG_TPB;
-            w[q] = k < ce_ ? W[k] : TB(0);
-            u[q] = k < ce_ ? Uj[k] : TB(0);
-            v[q] = (!last && k < ce_) ? Uj1[k] : TB(0);
+            const int kc = k < ce_ ? k : ce_ - 1;
+            w[q] = W[kc];
+            u[q] = Uj[kc];
+            v[q] = last ? TB(0) : Uj1[kc];
         }
 #pragma unroll
         for (int q = 0; q < BU; q++) {
@@ -323,18 +326,27 @@ __global__ __launch_bounds__(FG_TPB) void k_b_cgs_dots(Chunks C, const BState *s
     __shared__ double red[FG_KMAX + 1][FG_TPB / 64];
     const int cs = C.cs[blockIdx.x], ce = C.ce[blockIdx.x];
     const int base = cs + (int)threadIdx.x;
+    // loads past the chunk's end read its last entry (issued on every path:
+    // exact counter waits) and are zeroed by a select
     TB w[BNV];
 #pragma unroll
     for (int q = 0; q < BNV; q++) {
         const int k = base + q * FG_TPB;
-        w[q] = k < ce ? W[k] : TB(0);
+        const TB t = W[k < ce ? k : ce - 1];
+        w[q] = k < ce ? t : TB(0);
     }
     if (invD) {
+        TB d[BNV];
+#pragma unroll
+        for (int q = 0; q < BNV; q++) {
+            const int k = base + q * FG_TPB;
+            d[q] = invD[k < ce ? k : ce - 1];
+        }
 #pragma unroll
         for (int q = 0; q < BNV; q++) {
             const int k = base + q * FG_TPB;
             if (k < ce) {
-                w[q] = w[q] * invD[k];
+                w[q] = w[q] * d[q];
                 W[k] = w[q];
             }
         }
@@ -346,7 +358,7 @@ __global__ __launch_bounds__(FG_TPB) void k_b_cgs_dots(Chunks C, const BState *s
 #pragma unroll
         for (int q = 0; q < BNV; q++) {
             const int k = base + q * FG_TPB;
-            u[q] = k < ce ? Uj[k] : TB(0);
+            u[q] = Uj[k < ce ? k : ce - 1];
         }
         double acc = 0.0;
 #pragma unroll
@@ -395,7 +407,7 @@ __global__ __launch_bounds__(FG_TPB) void k_b_cgs_update(Chunks C, const BState 
 #pragma unroll
     for (int q = 0; q < BNV; q++) {
         const int k = base + q * FG_TPB;
-        w[q] = k < ce ? W[k] : TB(0);
+        w[q] = W[k < ce ? k : ce - 1];
     }
     for (int j = 0; j <= i; j++) {
         const TB *Uj = U + (size_t)j * ld;
@@ -404,7 +416,7 @@ __global__ __launch_bounds__(FG_TPB) void k_b_cgs_update(Chunks C, const BState 
 #pragma unroll
         for (int q = 0; q < BNV; q++) {
             const int k = base + q * FG_TPB;
-            u[q] = k < ce ? Uj[k] : TB(0);
+            u[q] = Uj[k < ce ? k : ce - 1];
         }
 #pragma unroll
         for (int q = 0; q < BNV; q++) w[q] = w[q] + aj * u[q];
