@@ -260,7 +260,8 @@ def headline_leg(F, ctx, args, rng):
             autotune.key_str((epb_e, ord_e, pipe_e)))
         sysd["ops"]["EDE"].close()
         sysd["ops"]["EDE"] = ede_tuned
-        ede_tune = {"edges_per_block": epb_e, "wg_per_cu": pipe_e,
+        ede_tune = {"kernel": "rows" if pipe_e == autotune.ROWS else "blocked",
+                    "edges_per_block": epb_e, "wg_per_cu": pipe_e,
                     "ordering": autotune.ORDER_NAMES[ord_e],
                     "ms": {autotune.key_str(k): v for k, v in tms_e.items()}}
     matvec = {}

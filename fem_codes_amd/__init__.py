@@ -28,6 +28,7 @@ FG_OK, FG_ERR_HIP, FG_ERR_ARG, FG_ERR_BREAKDOWN, FG_ERR_NOCONV, FG_ERR_NOMEM = r
 PREC_NONE, PREC_DIAG, PREC_JACOBI, PREC_ILU0, PREC_BLOCKDIAG = 0, 1, 2, 3, 4
 ORDER_BFS, ORDER_MESH = 0, 1
 ROWS_VALU, ROWS_MFMA = 0, 1
+EDGE_ROWS, EDGE_BLOCKED, EDGE_AUTO = 0, 1, 2
 EBE2_BLOCKDIAG, EBE2_NOBLOCKDIAG = 0, 1
 P3_DIFCONV, P3_CONST = 0, 1
 GMRES_FP64, GMRES_FP32_IR = 0, 1
@@ -113,6 +114,7 @@ SIGNATURES = {
     "fg_op_time_apply": (_i, [_vp, _i, _vp, _vp, _i, _i, C.POINTER(_d)]),
     "fg_op_set_pipeline": (_i, [_vp, _i]),
     "fg_op_set_row_kernel": (_i, [_vp, _i]),
+    "fg_op_set_edge_kernel": (_i, [_vp, _i]),
     "fg_prec_create": (_i, [_vp, _i, C.POINTER(_vp)]),
     "fg_prec_setup": (_i, [_vp, _vp]),
     "fg_prec_apply_f64": (_i, [_vp, _vp, _vp]),
@@ -399,6 +401,11 @@ class Operator:
     def set_row_kernel(self, kind):
         """ROWS_VALU or ROWS_MFMA (k = 12 element operators)"""
         _check(lib.fg_op_set_row_kernel(self.h, int(kind)), "fg_op_set_row_kernel")
+
+    def set_edge_kernel(self, kind):
+        """EDGE_AUTO (default: rows for FP64, blocked for FP32), EDGE_ROWS or
+        EDGE_BLOCKED (EDE operators)"""
+        _check(lib.fg_op_set_edge_kernel(self.h, int(kind)), "fg_op_set_edge_kernel")
 
     def close(self):
         if self.h and self.ctx.h:

@@ -29,13 +29,14 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import ORDER_BFS, ORDER_MESH, DeviceArray, FemGpuError, Operator
+from . import EDGE_AUTO, EDGE_BLOCKED, EDGE_ROWS, ORDER_BFS, ORDER_MESH, DeviceArray, FemGpuError, Operator
 
 CANDIDATES = (256, 512, 1024, 2048)
 PIPELINES = (0, 1, 2, 3, 4, 6)
 ORDERINGS = (ORDER_BFS, ORDER_MESH)
 ORDER_NAMES = {ORDER_BFS: "bfs", ORDER_MESH: "mesh"}
 RESTARTS = (10, 15, 20, 30, 45, 64)
+ROWS = "rows"   # the schedule entry of the half-edge row kernel (edges)
 
 
 def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
@@ -44,7 +45,8 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
     """Return ((epb, ordering, pipeline), {(epb, ordering, pipe): ms}, best_operator).
 
     values: AoS element (or edge) matrices, host array or DeviceArray.
-    edges: build edge-by-edge operators (k must be 2).
+    edges: build edge-by-edge operators (k must be 2); the half-edge row
+    kernel is timed too, as the candidate (0, ORDER_BFS, ROWS).
     orderings: the first is tuned over every block size; the others are
     timed at the block size that won."""
     dt = np.float32 if fp32 else np.float64
@@ -61,6 +63,11 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
         except FemGpuError:
             return
         op.set_values(values)
+        if edges:
+            if not ops:
+                op.set_edge_kernel(EDGE_ROWS)
+                times[(0, ORDER_BFS, ROWS)] = op.time_apply(P, Q, reps=reps)
+            op.set_edge_kernel(EDGE_BLOCKED)
         for pipe in pipelines:
             op.set_pipeline(pipe)
             times[(epb, order, pipe)] = op.time_apply(P, Q, reps=reps)
@@ -70,15 +77,23 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
         trial(epb, orderings[0])
     if not times:
         raise FemGpuError("no block size fits this mesh")
-    epb0 = min(times, key=times.get)[0]
+    blocked = {k: v for k, v in times.items() if k[2] != ROWS}
+    epb0 = min(blocked, key=blocked.get)[0]
     for order in orderings[1:]:
         trial(epb0, order)
     best = min(times, key=times.get)
+    blocked = {k: v for k, v in times.items() if k[2] != ROWS}
+    bbest = min(blocked, key=blocked.get)
     for key, op in ops.items():
-        if key != best[:2]:
+        if key != bbest[:2]:
             op.close()
-    op = ops[best[:2]]
-    op.set_pipeline(best[2])
+    op = ops[bbest[:2]]
+    op.set_pipeline(bbest[2])
+    if edges:
+        # rows won: the tuned precision uses them (EDGE_AUTO keeps the blocked
+        # schedule for FP32 products when FP64 was tuned)
+        op.set_edge_kernel((EDGE_ROWS if fp32 else EDGE_AUTO) if best[2] == ROWS
+                           else EDGE_BLOCKED)
     return best, times, op
 
 

@@ -419,6 +419,8 @@ static int ebe_apply(fg_op *op, const T *Kv, T *P, T *Q, T *hp, const int *skip)
 // the block kernel alone (timing of the dominant kernel)
 int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
 {
+    if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, fp32))
+        return fg_ede_rows_apply(op, fp32, P, Q, nullptr);
     if (op->nblk == 0) return FG_OK;
     if (fp32) {
         switch (op->k) {
@@ -445,10 +447,12 @@ int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
 
 int fg_ebe_apply_f64(fg_op *op, double *P, double *Q, const int *skip)
 {
+    if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, 0)) return fg_ede_rows_apply(op, 0, P, Q, skip);
     return ebe_apply<double>(op, op->d_K64, P, Q, op->d_hpart64, skip);
 }
 int fg_ebe_apply_f32(fg_op *op, float *P, float *Q, const int *skip)
 {
+    if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, 1)) return fg_ede_rows_apply(op, 1, P, Q, skip);
     return ebe_apply<float>(op, op->d_K32, P, Q, op->d_hpart32, skip);
 }
 
@@ -619,6 +623,7 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
     }
     FG_CHECK(upload(ioff, &op->d_inc_off));
     FG_CHECK(upload(idiag, &op->d_inc_diag));
+    if (kind == FG_OP_EDE) FG_CHECK(fg_ede_rows_build(op, lm, pl.kpos.data()));
     if (small) {
         std::vector<int> lmT((size_t)k * nitems);
         for (int e = 0; e < nitems; e++)
@@ -659,11 +664,13 @@ int fg_ebe_set_values_dev(fg_op *op, const double *A)
     int g = fg_vec_grid(c, n);
     FG_LAUNCH(k_ebe_scatter_values, g, FG_TPB, 0, c->stream, n, op->k * op->k, A, op->d_kpos,
               op->d_kstride, op->d_K64, op->d_K32);
+    op->rows_stale = 1;
     return FG_OK;
 }
 
 void fg_ebe_free(fg_op *op)
 {
+    fg_ede_rows_free(op);
     hipFree(op->d_eoff); hipFree(op->d_doff); hipFree(op->d_dofs);
     hipFree(op->d_ibase); hipFree(op->d_ioff); hipFree(op->d_inc);
     hipFree(op->d_slots); hipFree(op->d_lml); hipFree(op->d_K64); hipFree(op->d_K32);

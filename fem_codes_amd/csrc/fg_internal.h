@@ -145,6 +145,16 @@ struct fg_op {
     int *d_sm_iT = nullptr;      // EBE/EDE: r*nitems + e of the c-th element of row g
                                  // (element order); CSR: JA of the c-th entry of row g
     int *d_sm_lmT = nullptr;     // EBE/EDE: lm transposed, [k][nitems]
+    // EDE (k = 2): half-edge rows (fg_ede_rows.hip), a second copy of the
+    // edge values ordered by equation, kept in step with the blocked copy
+    int edge_rows = 0;           // FG_EDGE_ROWS / _BLOCKED / _AUTO (rows for FP64 only)
+    int rows_stale = 1;          // the rows' values lag the blocked values
+    int nrrb = 0;                // row blocks
+    long long nhalf = 0;         // half-edges (rows' entries)
+    int *d_rrb = nullptr, *d_rIA = nullptr, *d_hcol = nullptr, *d_hsrc = nullptr;
+    uint8_t *d_hrow = nullptr;
+    double *d_hv64 = nullptr;
+    float *d_hv32 = nullptr;
 
     // ---- EBE2 (shares the storage of `base` and the blocks of `bd`) ----
     fg_op *base = nullptr;
@@ -269,5 +279,13 @@ __device__ __forceinline__ bool fg_grid_sum(double v, double *part, unsigned *ti
 // ---------------------------------------------------------------------------
 // skip: optional device flag; when *skip != 0 the launched kernels return at once
 int fg_op_apply_any(fg_op *op, int fp32, void *P, void *Q, const int *skip = nullptr);
+// EDE half-edge rows (fg_ede_rows.hip)
+int fg_ede_rows_build(fg_op *op, const int *lm2, const long long *kpos);
+int fg_ede_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
+void fg_ede_rows_free(fg_op *op);
+inline bool fg_ede_rows_used(const fg_op *op, int fp32)
+{
+    return op->d_rrb && (op->edge_rows == FG_EDGE_ROWS || (op->edge_rows == FG_EDGE_AUTO && !fp32));
+}
 int fg_prec_apply_any(fg_prec *pc, int fp32, const void *p, void *z, const int *skip = nullptr);
 int fg_vec_grid(fg_ctx *ctx, long long n);

@@ -134,6 +134,16 @@ extern "C" int fg_op_set_row_kernel(fg_op *op, int kind)
     return FG_OK;
 }
 
+extern "C" int fg_op_set_edge_kernel(fg_op *op, int kind)
+{
+    FG_ARG(op, "op");
+    FG_ARG(op->kind == FG_OP_EDE && op->k == 2, "edge kernels apply to EDE operators");
+    FG_ARG(kind == FG_EDGE_ROWS || kind == FG_EDGE_BLOCKED || kind == FG_EDGE_AUTO,
+           "kind: FG_EDGE_ROWS, FG_EDGE_BLOCKED or FG_EDGE_AUTO");
+    op->edge_rows = kind;
+    return FG_OK;
+}
+
 extern "C" double fg_op_algorithmic_bytes(const fg_op *op, int fp32)
 {
     if (!op) return 0.0;

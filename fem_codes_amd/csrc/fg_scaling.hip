@@ -91,6 +91,7 @@ extern "C" int fg_scaling_diag(fg_prec *pc, int mode, double *F)
             FG_LAUNCH(k_scale_items, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, op->k, neq,
                       mode, op->d_lm, op->d_kpos, op->d_kstride, pc->d_invdiag, op->d_K64,
                       op->d_K32);
+        op->rows_stale = 1;
     }
     if (mode == FG_SCALING_LEFTRIGHT)
         FG_LAUNCH(k_mul_sqrt, fg_vec_grid(c, neq), FG_TPB, 0, c->stream, neq, pc->d_diag, F);

@@ -145,3 +145,82 @@ def test_restart_tuner_on_device(system):
             assert info.true_res <= 1.01 * info.eps
     pc.close()
     op.close()
+
+
+def test_ede_rows_bitwise(system):
+    """the half-edge row product (default) is the oracle's edemv bit for bit
+    in FP64 (edemv.c:23-36 term grouping, edge order per equation); the
+    blocked kernel agrees to 1e-14; FP32 to 1e-6"""
+    ctx, neq, nel, lm, ke, P = system
+    nedge, order, ebe, lm2 = F.ede_pattern(neq, lm)
+    op = F.Operator.ede(ctx, neq, lm2)
+    F.assemble_ede(op, order, ebe, ke)
+    vals = op.get_values()
+    Qo = O.Oracle().edemv(neq, nedge, vals, lm2, P.copy())
+    Q = op.matvec(P)
+    assert np.array_equal(Q, Qo)
+    Q32 = op.matvec(P.astype(np.float32), np.float32)
+    assert np.max(np.abs(Q32[:neq] - Qo[:neq])) <= 1e-6 * np.max(np.abs(Qo[:neq]))
+    op.set_edge_kernel(F.EDGE_BLOCKED)
+    Qb = op.matvec(P)
+    assert np.max(np.abs(Qb[:neq] - Qo[:neq])) <= 1e-14 * np.max(np.abs(Qo[:neq]))
+    Qb32 = op.matvec(P.astype(np.float32), np.float32)
+    op.set_edge_kernel(F.EDGE_ROWS)
+    assert np.array_equal(op.matvec(P), Qo)
+    Qr32 = op.matvec(P.astype(np.float32), np.float32)
+    assert np.max(np.abs(Qr32[:neq] - Qo[:neq])) <= 1e-6 * np.max(np.abs(Qo[:neq]))
+    # the default (AUTO): rows in FP64, blocked in FP32
+    op.set_edge_kernel(F.EDGE_AUTO)
+    assert np.array_equal(op.matvec(P), Qo)
+    assert np.array_equal(op.matvec(P.astype(np.float32), np.float32), Qb32)
+    op.close()
+
+
+def test_ede_rows_follow_value_changes(system):
+    """new values (set_values) and a Left/LeftRight scaling of the stored
+    values reach the row copy before the next product"""
+    ctx, neq, nel, lm, ke, P = system
+    nedge, order, ebe, lm2 = F.ede_pattern(neq, lm)
+    op = F.Operator.ede(ctx, neq, lm2)
+    F.assemble_ede(op, order, ebe, ke)
+    op.matvec(P)                                   # rows in use
+    rng = np.random.default_rng(9)
+    vals = rng.uniform(0.5, 1.5, 4 * nedge)   # positive diagonals: LeftRight takes sqrt
+    op.set_values(vals)
+    assert np.array_equal(op.matvec(P), O.Oracle().edemv(neq, nedge, vals, lm2, P.copy()))
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    Fd = ctx.to_device(np.ones(neq + 1))
+    pc.scaling(F.SCALING_LEFTRIGHT, Fd)
+    scaled = op.get_values()
+    assert not np.array_equal(scaled, vals)
+    assert np.array_equal(op.matvec(P), O.Oracle().edemv(neq, nedge, scaled, lm2, P.copy()))
+    pc.close()
+    op.close()
+
+
+def test_ede_edge_kernel_rejected_elsewhere(system):
+    ctx, neq, nel, lm, ke, P = system
+    op = F.Operator.ebe(ctx, neq, lm, 3)
+    with pytest.raises(F.FemGpuError):
+        op.set_edge_kernel(F.EDGE_BLOCKED)
+    op.close()
+
+
+def test_autotuner_edges_includes_rows(system):
+    """tune_ebe over edges times the half-edge row kernel beside the blocked
+    schedules, and the operator it returns computes edemv"""
+    from fem_codes_amd import autotune
+    ctx, neq, nel, lm, ke, P = system
+    nedge, order, ebe, lm2 = F.ede_pattern(neq, lm)
+    src = F.Operator.ede(ctx, neq, lm2)
+    F.assemble_ede(src, order, ebe, ke)
+    vals = src.get_values()
+    src.close()
+    best, times, op = autotune.tune_ebe(ctx, neq, lm2, 2, vals, reps=3, edges=True,
+                                        candidates=(512, 1024), pipelines=(0,))
+    assert (0, F.ORDER_BFS, autotune.ROWS) in times
+    assert best == min(times, key=times.get)
+    Qo = O.Oracle().edemv(neq, nedge, vals, lm2, P.copy())
+    Q = op.matvec(P)
+    assert np.max(np.abs(Q[:neq] - Qo[:neq])) <= 1e-14 * np.max(np.abs(Qo[:neq]))
+    op.close()
