@@ -90,8 +90,20 @@ def roofline(ms, nbytes, kernel):
             "algorithmic_bytes": nbytes, "ms": ms}
 
 
+def leg_traffic(leg, kernel):
+    """measured HBM bytes per launch of a leg's roofline kernel, from the
+    FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/traffic_<leg>.json
+    (null when the file is absent or names another kernel)"""
+    path = os.path.join(ROOT, "profiles", f"traffic_{leg}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tj = json.load(f)
+    return tj.get("hbm_bytes_per_launch") if tj.get("kernel") == kernel else None
+
+
 def ebe_kernel_name(k, fp32):
-    ept = 4 if k <= 2 else 2 if k <= 4 else 1
+    ept = 4 if k <= 2 else 2 if k == 3 else 4 if k == 4 else 1   # the library's defaults
     return f"k_ebe_apply<{k}, {ept}, {'float' if fp32 else 'double'}, 0>"
 
 
@@ -240,12 +252,8 @@ def headline_leg(F, ctx, args, rng):
     roof = roofline(ms_main, bytes64, kernel)
     roof["schedule"] = {"elems_per_block": best_epb, "wg_per_cu": best_pipe,
                         "ordering": autotune.ORDER_NAMES[best_ord]}
-    tpath = os.path.join(ROOT, "profiles", "ebe_f64_2048_traffic.json")
-    if os.path.exists(tpath) and n == 2048:
-        with open(tpath) as f:
-            tj = json.load(f)
-        if tj.get("kernel") == kernel:
-            roof["traffic"] = tj.get("hbm_bytes_per_launch")
+    if n == 2048:
+        roof["traffic"] = leg_traffic("headline", kernel)
 
     # the edge scheme: edges per block x schedule tuned the same way
     ede_tune = None
@@ -489,6 +497,7 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
     res["matvec"] = matvec_table(op, P64, Q64, P32, Q32, 20)
     ms_main = op.time_apply(P32, Q32, reps=20, main_only=True)
     res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True), ebe_kernel_name(3, True))
+    res["roofline"]["traffic"] = leg_traffic("batched", res["roofline"]["kernel"])
     Xs = {}
     for precision, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         X = ctx.empty(N + 1)
@@ -665,8 +674,21 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
     Fn[:neq] = Fh[P]
     csr = F.Operator.csr(ctx, neq, IA, JA)
     F.assemble_csr(csr, cbe, ke, k=4)
-    ebe = F.Operator.ebe(ctx, neq, lm, 4)
-    ebe.set_values(ke)
+    ss3d_tune = None
+    if args.no_autotune:
+        ebe = F.Operator.ebe(ctx, neq, lm, 4)
+        ebe.set_values(ke)
+        ebe_kernel = ebe_kernel_name(4, True)
+    else:
+        # the FP32 product of the inner cycle: elements per block x ordering
+        from fem_codes_amd import autotune
+        key, tms, ebe = autotune.tune_ebe(ctx, neq, lm, 4, ke, fp32=True, reps=10,
+                                          pipelines=(0,), orderings=autotune.ORDERINGS)
+        ss3d_tune = {"product": autotune.key_str(key),
+                     "ms": {autotune.key_str(k): v for k, v in tms.items()}}
+        log("ss3d autotune FP32 product ms:", {autotune.key_str(k): round(v, 4)
+                                               for k, v in tms.items()}, "->", autotune.key_str(key))
+        ebe_kernel = f"k_ebe_apply<4, {key[0] // 256}, float, 0>"
     del cbe, IA, JA
     pc = F.Preconditioner(csr, F.PREC_ILU0)
     levels = pc.ilu_levels()
@@ -689,7 +711,9 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
         mv[nm] = matvec_table(op, P64, Q64, P32, Q32, 10)
     res["matvec"] = mv
     ms_main = ebe.time_apply(P32, Q32, reps=10, main_only=True)
-    res["roofline"] = roofline(ms_main, ebe.algorithmic_bytes(True), ebe_kernel_name(4, True))
+    res["roofline"] = roofline(ms_main, ebe.algorithmic_bytes(True), ebe_kernel)
+    res["autotune"] = ss3d_tune
+    res["roofline"]["traffic"] = leg_traffic("ss3d", res["roofline"]["kernel"])
     # ILU(0) apply: 4 sweeps (the solve's) and the exact level-scheduled solves
     Z64 = ctx.empty(neq + 1)
     Z32 = ctx.empty(neq + 1, np.float32)
@@ -820,6 +844,7 @@ def transient_leg(F, ctx, args, n=1024, steps=10):
     res["matvec"] = matvec_table(op, P64, Q64, P32, Q32, 20)
     ms_main = op.time_apply(P64, Q64, reps=20, main_only=True)
     res["roofline"] = roofline(ms_main, op.algorithmic_bytes(False), ebe_kernel_name(3, False))
+    res["roofline"]["traffic"] = leg_traffic("transient", res["roofline"]["kernel"])
     if not args.no_cpu:
         import oracle as O
         if O.Reference.available():
@@ -908,6 +933,7 @@ def euler_leg(F, ctx, args, n=1024, seed=20260305):
     res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True),
                                "k_ebe_apply_mfma12<float, 0>" if trial["kept"] == "mfma"
                                else ebe_kernel_name(12, True))
+    res["roofline"]["traffic"] = leg_traffic("euler", res["roofline"]["kernel"])
     Xs = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         B = ctx.to_device(Rh)

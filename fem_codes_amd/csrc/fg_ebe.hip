@@ -262,10 +262,14 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_apply(FgEbeLayout L, const T *__
     fg_ebe_block<K, EPT, T, BS>(b, L, Kv, P, Q, hpart, smem_raw);
 }
 
-// persistent form: workgroup w walks logical blocks w, w+G, ... and issues the
-// loads of its next block's element matrices before it reduces the current
-// one, so they are in flight during the LDS phase.  G is a multiple of 8 so
-// the XCD-contiguous remap stays aligned with the hardware's round-robin.
+// persistent form: workgroup w walks logical blocks w, w+G, ...  While it
+// finishes block b (row sums, incidence sums, stores) the index lists and
+// element values of its next block are already in flight: they are issued
+// before the finish, after b's own loads, so every wait of the finish (for
+// b's values) leaves them outstanding.  The next block's P gathers follow
+// the finish.  Past the last block the loads read the last block again
+// (unconditional: exact counter waits).  G is a multiple of 8 so the
+// XCD-contiguous remap stays aligned with the hardware's round-robin.
 template <int K, int EPT, typename T, int BS>
 __global__ __launch_bounds__(FG_TPB) void k_ebe_apply_pipe(FgEbeLayout L,
                                                            const T *__restrict__ Kv,
@@ -278,18 +282,35 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_apply_pipe(FgEbeLayout L,
     const int nblk = L.nblk;
     int s = blockIdx.x;
     if (s >= nblk) return;
-    FgEbeRegs<K, EPT, T> cur, nxt;
-    fg_ebe_load<K, EPT, T>(fg_xcd_remap(s, nblk), L, Kv, cur);
+    FgEbeRegs<K, EPT, T> cur;
+    // block numbers are wavefront-uniform: readfirstlane keeps the layout
+    // reads (eoff, doff, ibase) scalar loads, which the vector memory
+    // counter does not track
+    {
+        FgEbeIdx<K, EPT, T> x;
+        const int b = __builtin_amdgcn_readfirstlane(fg_xcd_remap(s, nblk));
+        fg_ebe_idx_load<K, EPT, T>(b, L, cur, x);
+        fg_ebe_load<K, EPT, T>(b, L, Kv, cur);
+        fg_ebe_fill<K, EPT, T>(b, L, P, smem_raw, cur, x);
+    }
     for (;;) {
-        const int b = fg_xcd_remap(s, nblk);
+        const int b = __builtin_amdgcn_readfirstlane(fg_xcd_remap(s, nblk));
         const int sn = s + (int)gridDim.x;
-        fg_ebe_gather<K, EPT, T>(b, L, P, smem_raw, cur);
-        if (sn < nblk) fg_ebe_load<K, EPT, T>(fg_xcd_remap(sn, nblk), L, Kv, nxt);
-        fg_ebe_finish<K, EPT, T, BS>(b, L, Kv, Q, hpart, smem_raw, cur);
+        const int bn = __builtin_amdgcn_readfirstlane(fg_xcd_remap(sn < nblk ? sn : nblk - 1, nblk));
+        // this block's extents, read before the next block's loads are issued
+        // (later reads would wait for those loads: the kernel's stores may
+        // alias the layout arrays as far as the compiler knows)
+        const FgEbeBlk B = fg_ebe_blk(L, b);
+        FgEbeRegs<K, EPT, T> nxt;
+        FgEbeIdx<K, EPT, T> xn;
+        fg_ebe_idx_load<K, EPT, T>(bn, L, nxt, xn);
+        fg_ebe_load<K, EPT, T>(bn, L, Kv, nxt);
+        fg_ebe_finish<K, EPT, T, BS>(B, L, Kv, Q, hpart, smem_raw, cur);
         if (sn >= nblk) break;
+        __syncthreads();   // LDS is reused by the next block
+        fg_ebe_fill<K, EPT, T>(bn, L, P, smem_raw, nxt, xn);
         cur = nxt;
         s = sn;
-        __syncthreads();   // LDS is reused by the next gather
     }
 }
 
@@ -308,7 +329,7 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_apply_mfma12(FgEbeLayout L,
     fg_ebe_gather<12, 1, T>(b, L, P, smem_raw, r);
     fg_ebe_rows_mfma12<T, BS>(b, L, Kv, smem_raw);
     __syncthreads();
-    fg_ebe_reduce<12, 1, T>(b, L, Q, hpart, smem_raw, r);
+    fg_ebe_reduce<12, 1, T>(fg_ebe_blk(L, b), L, Q, hpart, smem_raw, r);
 }
 
 // shared dofs: partials in block order; dofs no element touches: 0; sink: 0

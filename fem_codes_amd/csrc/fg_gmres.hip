@@ -768,11 +768,16 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         info->eps = eps;
         info->ms = ms;
         info->status = rc;
-        // true preconditioned residual of the returned X (FP64)
-        HostState h2{};
-        FG_CHECK(residual(c, op, pc, B, X, W64, st, 0.0, pp));
-        FG_CHECK(read_state(c, st, h2));
-        info->true_res = h2.e0;
+        // true preconditioned residual of the returned X (FP64); a converged
+        // FP32-IR solve has just computed it for this X (its stopping test)
+        if (ir && hs.conv) {
+            info->true_res = hs.e0;
+        } else {
+            HostState h2{};
+            FG_CHECK(residual(c, op, pc, B, X, W64, st, 0.0, pp));
+            FG_CHECK(read_state(c, st, h2));
+            info->true_res = h2.e0;
+        }
     }
     return rc == FG_ERR_NOCONV ? FG_ERR_NOCONV : rc;
 }
