@@ -223,17 +223,21 @@ def headline_leg(F, ctx, args, rng):
     P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
     Q32 = ctx.empty(neq + 1, np.float32)
     from fem_codes_amd import autotune
+    default_op = sysd["ops"]["EBE"]          # the library default: 512 per block, BFS
     if args.no_autotune:
         best_epb, best_ord, best_pipe, tune_ms = 512, F.ORDER_BFS, 0, {}
-        ebe_tuned = F.Operator.ebe(ctx, neq, sysd["lm"], 3, best_epb)
-        ebe_tuned.set_values(sysd["ke"])
+        ebe_tuned = default_op
     else:
         (best_epb, best_ord, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(
             ctx, neq, sysd["lm"], 3, sysd["ke"], orderings=autotune.ORDERINGS)
         log("autotune EBE f64 ms by elements/block / ordering / workgroups per CU:",
             {autotune.key_str(k): round(v, 4) for k, v in tune_ms.items()}, "->",
             autotune.key_str((best_epb, best_ord, best_pipe)))
-    default_op = sysd["ops"]["EBE"]
+        if (best_epb, best_ord, best_pipe) == (512, F.ORDER_BFS, 0):
+            # the tuner chose the default configuration: keep the operator
+            # built at setup (allocated before the tuning candidates)
+            ebe_tuned.close()
+            ebe_tuned = default_op
     sysd["ops"]["EBE"] = ebe_tuned
     ebe = ebe_tuned
     for _ in range(args.warmup):
@@ -273,10 +277,12 @@ def headline_leg(F, ctx, args, rng):
                     "ordering": autotune.ORDER_NAMES[ord_e],
                     "ms": {autotune.key_str(k): v for k, v in tms_e.items()}}
     matvec = {}
-    for name, op in list(sysd["ops"].items()) + [("EBE_default", default_op)]:
+    extra = [] if ebe_tuned is default_op else [("EBE_default", default_op)]
+    for name, op in list(sysd["ops"].items()) + extra:
         matvec[name] = matvec_table(op, P64, Q64, P32, Q32, args.steps)
         log(name, {k: round(v["GBs"], 1) for k, v in matvec[name].items()})
-    default_op.close()
+    if ebe_tuned is not default_op:
+        default_op.close()
     res = {"value": value, "ms": ms, "wall_ms_per_step": wall * 1e3 / args.steps,
            "scale": scale, "kappa": kappa, "theta": theta, "roofline": roof,
            "matvec": matvec, "ebe_autotune_ms": {autotune.key_str(k): v for k, v in tune_ms.items()},
