@@ -403,7 +403,7 @@ class Operator:
         _check(lib.fg_op_set_row_kernel(self.h, int(kind)), "fg_op_set_row_kernel")
 
     def set_edge_kernel(self, kind):
-        """EDGE_AUTO (default: rows for FP64, blocked for FP32), EDGE_ROWS or
+        """EDGE_AUTO (default: rows in both precisions), EDGE_ROWS or
         EDGE_BLOCKED (EDE operators)"""
         _check(lib.fg_op_set_edge_kernel(self.h, int(kind)), "fg_op_set_edge_kernel")
 

@@ -90,10 +90,8 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
     op = ops[bbest[:2]]
     op.set_pipeline(bbest[2])
     if edges:
-        # rows won: the tuned precision uses them (EDGE_AUTO keeps the blocked
-        # schedule for FP32 products when FP64 was tuned)
-        op.set_edge_kernel((EDGE_ROWS if fp32 else EDGE_AUTO) if best[2] == ROWS
-                           else EDGE_BLOCKED)
+        # rows won: both precisions use them (EDGE_AUTO)
+        op.set_edge_kernel(EDGE_AUTO if best[2] == ROWS else EDGE_BLOCKED)
     return best, times, op
 
 

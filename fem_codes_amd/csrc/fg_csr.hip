@@ -1,17 +1,25 @@
 // fg_csr.hip -- CSR operator (MatrixVector_Operations/CSR/csrmv.c:3-24).
 //
-// Row-block ("CSR-stream") SpMV: the host cuts the rows into blocks of at
-// most 256 rows and FG_CSR_NNZB nonzeros.  A workgroup streams its block's
-// AA/JA range with coalesced loads, stores the products AA[j]*P[JA[j]] in
-// LDS, and then each thread sums one row's products from LDS in slot order.
-// The per-row summation order is the reference's (csrmv.c:15-16: Q[i]=0,
-// then Q[i] += AA[j]*P[JA[j]] for ascending j), so with FMA contraction
-// disabled the FP64 result is bitwise the reference's.  A row longer than a
-// block (never produced by P1 meshes) gets a block of its own and a tree sum.
+// The product runs on a sliced copy of the rows (fg_sell.h): 64 rows per
+// wavefront slice, lane = row, the k-th entries of a slice's rows stored
+// side by side, so every JA/AA load is one coalesced wavefront access and
+// each lane sums its row in registers.  The per-row summation order is the
+// reference's (csrmv.c:15-16: Q[i]=0, then Q[i] += AA[j]*P[JA[j]] for
+// ascending j), so with FMA contraction disabled the FP64 result is bitwise
+// the reference's.  The reference-order arrays IA/JA/AA stay the primary
+// copy (assembly, scaling, ILU(0), the diagonal setup and the one-workgroup
+// solver read them); every write of AA refreshes the sliced values.
+//
+// A pattern whose slices would carry more than a quarter of padding (rows of
+// very different lengths side by side) keeps the row-block ("CSR-stream")
+// product instead: blocks of at most 256 rows and FG_CSR_NNZB nonzeros, the
+// products staged in LDS, each thread summing one row in slot order; a row
+// longer than a block gets a block of its own and a tree sum.
 #include <algorithm>
 #include <vector>
 
 #include "fg_internal.h"
+#include "fg_sell.h"
 
 constexpr int FG_CSR_NNZB = 2048;
 
@@ -83,12 +91,68 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_stream(
     if (blockIdx.x == 0 && threadIdx.x == 0) Q[neq] = T(0);
 }
 
+// four slices (one per wavefront) per workgroup
+template <typename T>
+__global__ __launch_bounds__(FG_TPB) void k_csr_sliced(int nslice, const int *__restrict__ soff,
+                                                       const int *__restrict__ scol,
+                                                       const T *__restrict__ sval,
+                                                       const T *__restrict__ P,
+                                                       T *__restrict__ Q, int neq, const int *skip)
+{
+    if (skip && *skip) return;
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
+                  (int)(threadIdx.x / FG_SELL_W);
+    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    if (s < nslice) {
+        const int o0 = soff[s];
+        const T acc = fg_sell_row(fg_sell_prod<T>{scol, sval, P}, o0, soff[s + 1] - o0, lane);
+        const int r = s * FG_SELL_W + lane;
+        if (r < neq) Q[r] = acc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) Q[neq] = T(0);
+}
+
+// AA / AA32 (reference order) -> the sliced copies
+__global__ void k_csr_mirror(int nnz, const long long *__restrict__ pos,
+                             const double *__restrict__ AA, const float *__restrict__ AA32,
+                             double *__restrict__ s64, float *__restrict__ s32)
+{
+    for (int j = blockIdx.x * FG_TPB + threadIdx.x; j < nnz; j += gridDim.x * FG_TPB) {
+        const long long q = pos[j];
+        s64[q] = AA[j];
+        s32[q] = AA32[j];
+    }
+}
+
+int fg_csr_values_changed(fg_op *op)
+{
+    if (!op->d_spos || op->nnz == 0) return FG_OK;
+    fg_ctx *c = op->ctx;
+    FG_LAUNCH(k_csr_mirror, fg_vec_grid(c, op->nnz), FG_TPB, 0, c->stream, (int)op->nnz,
+              op->d_spos, op->d_AA, op->d_AA32, op->d_sv64, op->d_sv32);
+    return FG_OK;
+}
+
 int fg_csr_apply(fg_op *op, int fp32, const void *P, void *Q, const int *skip)
 {
     fg_ctx *c = op->ctx;
     if (op->nrb == 0) {
         if (fp32) FG_HIP(hipMemsetAsync((float *)Q + op->neq, 0, sizeof(float), c->stream));
         else FG_HIP(hipMemsetAsync((double *)Q + op->neq, 0, sizeof(double), c->stream));
+        return FG_OK;
+    }
+    if (op->d_spos) {
+        constexpr int SPW = FG_TPB / FG_SELL_W;
+        const int g = (op->nslice + SPW - 1) / SPW;
+        if (fp32)
+            FG_LAUNCH(k_csr_sliced<float>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                      op->d_scol, (const float *)op->d_sv32, (const float *)P, (float *)Q,
+                      op->neq, skip);
+        else
+            FG_LAUNCH(k_csr_sliced<double>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                      op->d_scol, (const double *)op->d_sv64, (const double *)P, (double *)Q,
+                      op->neq, skip);
         return FG_OK;
     }
     if (fp32)
@@ -144,6 +208,35 @@ extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, cons
     FG_HIP(hipMalloc(&op->d_AA32, sizeof(float) * (nnz ? nnz : 1)));
     FG_HIP(hipMemset(op->d_AA, 0, sizeof(double) * (nnz ? nnz : 1)));
     FG_HIP(hipMemset(op->d_AA32, 0, sizeof(float) * (nnz ? nnz : 1)));
+    {
+        // the sliced copy, unless its padding exceeds a quarter of the entries
+        const fg_sell_layout L = fg_sell_plan(neq, IA);
+        if (nnz > 0 && L.nslot <= (long long)nnz + nnz / 4 + 4096) {
+            std::vector<int> scol(L.nslot, -1);
+            std::vector<long long> spos(nnz);
+            for (int r = 0; r < neq; r++)
+                for (int j = IA[r]; j < IA[r + 1]; j++) {
+                    const long long q = L.slot(r, j - IA[r]);
+                    scol[q] = JA[j];
+                    spos[j] = q;
+                }
+            op->nslice = L.nslice;
+            op->nslot = L.nslot;
+            FG_HIP(hipMalloc(&op->d_soff, sizeof(int) * L.soff.size()));
+            FG_HIP(hipMemcpy(op->d_soff, L.soff.data(), sizeof(int) * L.soff.size(),
+                             hipMemcpyHostToDevice));
+            FG_HIP(hipMalloc(&op->d_scol, sizeof(int) * L.nslot));
+            FG_HIP(hipMemcpy(op->d_scol, scol.data(), sizeof(int) * L.nslot,
+                             hipMemcpyHostToDevice));
+            FG_HIP(hipMalloc(&op->d_spos, sizeof(long long) * nnz));
+            FG_HIP(hipMemcpy(op->d_spos, spos.data(), sizeof(long long) * nnz,
+                             hipMemcpyHostToDevice));
+            FG_HIP(hipMalloc(&op->d_sv64, sizeof(double) * L.nslot));
+            FG_HIP(hipMalloc(&op->d_sv32, sizeof(float) * L.nslot));
+            FG_HIP(hipMemset(op->d_sv64, 0, sizeof(double) * L.nslot));
+            FG_HIP(hipMemset(op->d_sv32, 0, sizeof(float) * L.nslot));
+        }
+    }
     if (neq <= FG_SMALL_MAX) {
         // the one-workgroup solver's transposed column lists
         int mi = 0;
@@ -165,6 +258,8 @@ void fg_csr_free(fg_op *op)
 {
     hipFree(op->d_rb); hipFree(op->d_IA); hipFree(op->d_JA); hipFree(op->d_AA); hipFree(op->d_AA32);
     hipFree(op->d_sm_iT);
+    hipFree(op->d_soff); hipFree(op->d_scol); hipFree(op->d_spos); hipFree(op->d_sv64);
+    hipFree(op->d_sv32);
 }
 
 // CSR/Diag_precond_CSR_setup.c:10-14: the last slot of row i with column i

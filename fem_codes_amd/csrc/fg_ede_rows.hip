@@ -9,17 +9,17 @@
 // half-edge (edge, end) at r, in edge order.
 //
 // MI355X design: the half-edges are stored grouped by equation, in edge
-// order, each with its two coefficients ordered (own dof, other dof), the
-// other dof's index, and the equation's index inside its row block (one
-// byte) -- 2s + 5 bytes per half-edge, the reference's 4s + 8 per edge.  A
-// row block (at most 256 equations, 2048 half-edges) is one workgroup: the
-// block's own P entries are staged in LDS, every thread streams eight
-// half-edges (coefficient pairs as one vector load) and gathers P of their
-// other ends, forms each term t = c_own*p_own + c_other*p_other -- the two
-// rounded products of the reference added once, so t is the reference's
-// term bit for bit -- into LDS, and each equation then adds its terms in
-// edge order starting from 0: Q equals the reference's edemv bit for bit
-// (with FMA contraction disabled).  No atomics, no halo pass.
+// order, each with its two coefficients ordered (own dof, other dof) and the
+// other dof's index -- 2s + 4 bytes per half-edge, the reference's 4s + 8
+// per edge -- in wavefront slices (fg_sell.h): 64 equations per slice, one
+// per lane, the k-th half-edges of a slice's equations side by side.  Each
+// lane loads its equation's P once, streams its half-edges (coefficient
+// pairs as one vector load) with every load of the row issued at once,
+// gathers P of their other ends, forms each term t = c_own*p_own +
+// c_other*p_other -- the two rounded products of the reference added once,
+// so t is the reference's term bit for bit -- and adds the terms in edge
+// order starting from 0: Q equals the reference's edemv bit for bit (with
+// FMA contraction disabled).  No LDS, no barrier, no atomics, no halo pass.
 //
 // The blocked layout of the same operator (fg_ebe.hip) stays the primary
 // copy of the values (assembly, scaling, preconditioner setups and the
@@ -29,25 +29,25 @@
 #include <vector>
 
 #include "fg_internal.h"
-
-constexpr int FG_EDE_NNZB = 2048;
-constexpr int FG_EDE_NPT = FG_EDE_NNZB / FG_TPB;
+#include "fg_sell.h"
 
 template <typename T> struct EdePair;
 template <> struct EdePair<double> { typedef double type __attribute__((ext_vector_type(2))); };
 template <> struct EdePair<float> { typedef float type __attribute__((ext_vector_type(2))); };
 
 // rows' coefficient pairs from the blocked values (edge values at p .. p+3:
-// end 0 owns (A0, A1), end 1 owns (A3, A2))
-__global__ void k_ede_rows_mirror(long long nh, const int *__restrict__ hsrc,
+// end 0 owns (A0, A1), end 1 owns (A3, A2)); src = (p << 1) | end, -1 for a
+// padding slot
+__global__ void k_ede_rows_mirror(long long nslot, const long long *__restrict__ hsrc,
                                   const double *__restrict__ K64, const float *__restrict__ K32,
                                   double *__restrict__ hv64, float *__restrict__ hv32)
 {
-    for (long long h = (long long)blockIdx.x * FG_TPB + threadIdx.x; h < nh;
+    for (long long h = (long long)blockIdx.x * FG_TPB + threadIdx.x; h < nslot;
          h += (long long)gridDim.x * FG_TPB) {
-        const int sv = hsrc[h];
+        const long long sv = hsrc[h];
+        if (sv < 0) continue;
         const long long p = sv >> 1;
-        const int a = sv & 1;
+        const int a = (int)(sv & 1);
         const long long own = p + (a ? 3 : 0), oth = p + (a ? 2 : 1);
         hv64[2 * h] = K64[own];
         hv64[2 * h + 1] = K64[oth];
@@ -56,75 +56,66 @@ __global__ void k_ede_rows_mirror(long long nh, const int *__restrict__ hsrc,
     }
 }
 
+// one half-edge of a sliced equation row: the other end's dof (-1: padding,
+// neq: the Dirichlet sink, whose P reads as 0 -- edemv.c:20 sets P[neq] = 0)
+template <typename TV>
+struct HalfEdgeSlot {
+    using T = TV;
+    using V2 = typename EdePair<T>::type;
+    struct Rec {
+        int c;
+        V2 v;
+        T p;
+    };
+    const int *__restrict__ col;
+    const V2 *__restrict__ val;
+    const T *__restrict__ P;
+    T pown;
+    int neq;
+    __device__ __forceinline__ Rec load(long long q) const
+    {
+        Rec r;
+        r.c = __builtin_nontemporal_load(&col[q]);
+        r.v = __builtin_nontemporal_load(&val[q]);
+        r.p = T(0);
+        return r;
+    }
+    __device__ __forceinline__ void gather(Rec &r) const
+    {
+        const T t = P[r.c < 0 ? 0 : r.c];
+        r.p = r.c < neq ? t : T(0);
+    }
+    __device__ __forceinline__ T term(const Rec &r) const
+    {
+        const T own = r.v.x * pown;
+        const T oth = r.v.y * r.p;
+        return r.c < 0 ? T(0) : own + oth;
+    }
+};
+
+// four slices (one per wavefront) per workgroup
 template <typename T>
-__global__ __launch_bounds__(FG_TPB) void k_ede_rows(int nrb, const int *__restrict__ rb,
-                                                     const int *__restrict__ rIA,
-                                                     const int *__restrict__ hcol,
-                                                     const uint8_t *__restrict__ hrow,
-                                                     const T *__restrict__ hv,
+__global__ __launch_bounds__(FG_TPB) void k_ede_rows(int nslice, const int *__restrict__ soff,
+                                                     const int *__restrict__ scol,
+                                                     const T *__restrict__ sval,
                                                      T *__restrict__ P, T *__restrict__ Q,
                                                      int neq, const int *skip)
 {
     if (skip && *skip) return;
     using V2 = typename EdePair<T>::type;
-    __shared__ T s[FG_EDE_NNZB];
-    __shared__ T sp[FG_TPB];
-    __shared__ T ws[FG_TPB / 64];
-    __shared__ T dummy;
-    const int b = fg_xcd_remap(blockIdx.x, nrb);
-    const int r0 = rb[b], r1 = rb[b + 1];   // r1 > r0
-    const int j0 = rIA[r0], j1 = rIA[r1];
-    const V2 *hv2 = reinterpret_cast<const V2 *>(hv);
-    // this thread's equation (clamped: every load issued on every path)
-    const int r = r0 + (int)threadIdx.x;
-    const int rc = r < r1 ? r : r1 - 1;
-    const int ra = rIA[rc], re = rIA[rc + 1];
-    const T pr = P[rc];
-    if (j1 - j0 <= FG_EDE_NNZB) {
-        if (j1 > j0) {
-            int col[FG_EDE_NPT];
-            V2 c[FG_EDE_NPT];
-            uint8_t lr[FG_EDE_NPT];
-            T po[FG_EDE_NPT];
-#pragma unroll
-            for (int q = 0; q < FG_EDE_NPT; q++) {
-                const int j = j0 + (int)threadIdx.x + q * FG_TPB;
-                const int jc = j < j1 ? j : j1 - 1;
-                col[q] = __builtin_nontemporal_load(&hcol[jc]);
-                c[q] = __builtin_nontemporal_load(&hv2[jc]);
-                lr[q] = __builtin_nontemporal_load(&hrow[jc]);
-            }
-            // the sink's P reads as 0 (edemv.c:20 sets P[neq] = 0)
-#pragma unroll
-            for (int q = 0; q < FG_EDE_NPT; q++) {
-                const T t = P[col[q]];
-                po[q] = col[q] < neq ? t : T(0);
-            }
-            sp[threadIdx.x] = pr;
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < FG_EDE_NPT; q++) {
-                const int j = (int)threadIdx.x + q * FG_TPB;
-                const T own = c[q].x * sp[lr[q]];
-                const T oth = c[q].y * po[q];
-                *(j < j1 - j0 ? &s[j] : &dummy) = own + oth;
-            }
-        }
-        __syncthreads();
-        if (r < r1) {
-            T acc = T(0);
-            for (int j = ra - j0; j < re - j0; j++) acc = acc + s[j];
-            Q[r] = acc;
-        }
-    } else {   // one equation with more half-edges than a block holds
-        T acc = T(0);
-        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB) {
-            const int o = hcol[j];
-            const T p2 = o < neq ? P[o] : T(0);
-            acc = acc + (hv[2 * j] * pr + hv[2 * j + 1] * p2);
-        }
-        acc = fg_block_sum(acc, ws);
-        if (threadIdx.x == 0) Q[r0] = acc;
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
+                  (int)(threadIdx.x / FG_SELL_W);
+    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    if (s < nslice) {
+        const int r = s * FG_SELL_W + lane;
+        // lanes past the last equation read the sink's P; all their slots
+        // are padding
+        const T pown = P[r < neq ? r : neq];
+        const int o0 = soff[s];
+        const HalfEdgeSlot<T> row{scol, reinterpret_cast<const V2 *>(sval), P, pown, neq};
+        const T acc = fg_sell_row(row, o0, soff[s + 1] - o0, lane);
+        if (r < neq) Q[r] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Q[neq] = T(0);
@@ -153,47 +144,34 @@ int fg_ede_rows_build(fg_op *op, const int *lm2, const long long *kpos)
         }
     }
     for (int r = 0; r < neq; r++) rIA[r + 1] += rIA[r];
-    const long long nh = rIA[neq];
-    std::vector<int> hcol(nh), hsrc(nh);
+    const fg_sell_layout L = fg_sell_plan(neq, rIA.data());
+    std::vector<int> scol(L.nslot, -1);
+    std::vector<long long> ssrc(L.nslot, -1);
     {
-        std::vector<int> fill(rIA.begin(), rIA.end() - 1);
+        // each equation's half-edges in edge order
+        std::vector<int> fill((size_t)neq, 0);
         for (int e = 0; e < nedge; e++) {
             if (kpos[e] < 0) continue;
             for (int a = 0; a < 2; a++) {
                 const int r = lm2[2 * (size_t)e + a];
                 if (r == neq) continue;
-                const int h = fill[r]++;
-                hcol[h] = lm2[2 * (size_t)e + 1 - a];
-                hsrc[h] = (int)((kpos[e] << 1) | a);
+                const long long q = L.slot(r, fill[r]++);
+                scol[q] = lm2[2 * (size_t)e + 1 - a];
+                ssrc[q] = (kpos[e] << 1) | a;
             }
         }
     }
-    // row blocks: at most 256 equations and FG_EDE_NNZB half-edges (a longer
-    // row gets a block of its own)
-    std::vector<int> rb{0};
-    std::vector<uint8_t> hrow(nh);
-    int rows = 0, nz = 0;
-    for (int r = 0; r < neq; r++) {
-        const int rn = rIA[r + 1] - rIA[r];
-        if (rows > 0 && (rows == FG_TPB || nz + rn > FG_EDE_NNZB)) {
-            rb.push_back(r);
-            rows = 0;
-            nz = 0;
-        }
-        for (int h = rIA[r]; h < rIA[r + 1]; h++) hrow[h] = (uint8_t)rows;
-        rows++;
-        nz += rn;
-    }
-    if (rows > 0) rb.push_back(neq);
-    op->nrrb = (int)rb.size() - 1;
-    op->nhalf = nh;
-    FG_CHECK(upload(rb, &op->d_rrb));
-    FG_CHECK(upload(rIA, &op->d_rIA));
-    FG_CHECK(upload(hcol, &op->d_hcol));
-    FG_CHECK(upload(hsrc, &op->d_hsrc));
-    FG_CHECK(upload(hrow, &op->d_hrow));
-    FG_HIP(hipMalloc(&op->d_hv64, sizeof(double) * 2 * (nh ? nh : 1)));
-    FG_HIP(hipMalloc(&op->d_hv32, sizeof(float) * 2 * (nh ? nh : 1)));
+    op->nslice = L.nslice;
+    op->nslot = L.nslot;
+    op->nhalf = rIA[neq];
+    FG_CHECK(upload(L.soff, &op->d_soff));
+    FG_CHECK(upload(scol, &op->d_scol));
+    FG_CHECK(upload(ssrc, &op->d_hsrc));
+    const size_t nv = 2 * (size_t)(L.nslot ? L.nslot : 1);
+    FG_HIP(hipMalloc(&op->d_sv64, sizeof(double) * nv));
+    FG_HIP(hipMalloc(&op->d_sv32, sizeof(float) * nv));
+    FG_HIP(hipMemset(op->d_sv64, 0, sizeof(double) * nv));
+    FG_HIP(hipMemset(op->d_sv32, 0, sizeof(float) * nv));
     op->edge_rows = FG_EDGE_AUTO;
     op->rows_stale = 1;
     return FG_OK;
@@ -203,12 +181,12 @@ int fg_ede_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 {
     fg_ctx *c = op->ctx;
     if (op->rows_stale) {
-        if (op->nhalf > 0)
-            FG_LAUNCH(k_ede_rows_mirror, fg_vec_grid(c, op->nhalf), FG_TPB, 0, c->stream,
-                      op->nhalf, op->d_hsrc, op->d_K64, op->d_K32, op->d_hv64, op->d_hv32);
+        if (op->nslot > 0)
+            FG_LAUNCH(k_ede_rows_mirror, fg_vec_grid(c, op->nslot), FG_TPB, 0, c->stream,
+                      op->nslot, op->d_hsrc, op->d_K64, op->d_K32, op->d_sv64, op->d_sv32);
         op->rows_stale = 0;
     }
-    if (op->nrrb == 0) {
+    if (op->nslice == 0) {
         // no equation: Q[neq] = P[neq] = 0
         if (fp32) {
             FG_HIP(hipMemsetAsync((float *)Q + op->neq, 0, sizeof(float), c->stream));
@@ -219,19 +197,20 @@ int fg_ede_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
         }
         return FG_OK;
     }
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int g = (op->nslice + SPW - 1) / SPW;
     if (fp32)
-        FG_LAUNCH(k_ede_rows<float>, op->nrrb, FG_TPB, 0, c->stream, op->nrrb, op->d_rrb,
-                  op->d_rIA, op->d_hcol, op->d_hrow, (const float *)op->d_hv32, (float *)P,
-                  (float *)Q, op->neq, skip);
+        FG_LAUNCH(k_ede_rows<float>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                  op->d_scol, (const float *)op->d_sv32, (float *)P, (float *)Q, op->neq, skip);
     else
-        FG_LAUNCH(k_ede_rows<double>, op->nrrb, FG_TPB, 0, c->stream, op->nrrb, op->d_rrb,
-                  op->d_rIA, op->d_hcol, op->d_hrow, (const double *)op->d_hv64, (double *)P,
-                  (double *)Q, op->neq, skip);
+        FG_LAUNCH(k_ede_rows<double>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                  op->d_scol, (const double *)op->d_sv64, (double *)P, (double *)Q, op->neq,
+                  skip);
     return FG_OK;
 }
 
 void fg_ede_rows_free(fg_op *op)
 {
-    hipFree(op->d_rrb); hipFree(op->d_rIA); hipFree(op->d_hcol); hipFree(op->d_hsrc);
-    hipFree(op->d_hrow); hipFree(op->d_hv64); hipFree(op->d_hv32);
+    hipFree(op->d_soff); hipFree(op->d_scol); hipFree(op->d_hsrc);
+    hipFree(op->d_sv64); hipFree(op->d_sv32);
 }

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "fg_internal.h"
+#include "fg_sell.h"
 
 namespace {
 
@@ -79,10 +80,12 @@ struct fg_ilu {
     int sweeps = 0;            // 0: exact level-scheduled solves; s > 0: s Jacobi sweeps
     double *d_s64[2] = {nullptr, nullptr};   // sweep scratch
     float *d_s32[2] = {nullptr, nullptr};
-    // strictly lower / upper parts as compact CSR matrices for the sweeps
+    // strictly lower / upper parts as sliced rows (fg_sell.h) for the sweeps
     struct Tri {
-        int nrb = 0, nnz = 0;
-        int *d_rb = nullptr, *d_IA = nullptr, *d_JA = nullptr, *d_map = nullptr;
+        int nslice = 0;
+        long long nslot = 0;
+        int *d_soff = nullptr, *d_JA = nullptr;
+        long long *d_map = nullptr;   // [nslot] factor slot of each entry, -1: padding
         double *d_v64 = nullptr;
         float *d_v32 = nullptr;
     } lt, ut;
@@ -251,63 +254,57 @@ int blocks_for(int rows) { return (rows + FG_TPB - 1) / FG_TPB; }
 
 }  // namespace
 
-// compact CSR of the strictly lower (lower = true) or upper part of A's
-// pattern, with row blocks of at most 256 rows / TRI_NNZB entries and the A
-// slot of each entry
-constexpr int TRI_NNZB = 2048;
-
+// the strictly lower (lower = true) or upper part of A's pattern as sliced
+// rows, with the factor slot of each entry
 static int build_tri(int neq, const std::vector<int> &IA, const std::vector<int> &JA,
                      const std::vector<int> &lend, const std::vector<int> &ubeg, bool lower,
                      fg_ilu::Tri &t)
 {
-    std::vector<int> tIA(neq + 1, 0), tJA, map, rb{0};
-    int rows = 0, nz = 0;
+    std::vector<int> tIA(neq + 1, 0);
+    for (int i = 0; i < neq; i++)
+        tIA[i + 1] = tIA[i] + (lower ? lend[i] - IA[i] : IA[i + 1] - ubeg[i]);
+    const fg_sell_layout L = fg_sell_plan(neq, tIA.data());
+    std::vector<int> col(L.nslot, -1);
+    std::vector<long long> map(L.nslot, -1);
     for (int i = 0; i < neq; i++) {
         const int q0 = lower ? IA[i] : ubeg[i], q1 = lower ? lend[i] : IA[i + 1];
         for (int q = q0; q < q1; q++) {
-            tJA.push_back(JA[q]);
-            map.push_back(q);
+            const long long at = L.slot(i, q - q0);
+            col[at] = JA[q];
+            map[at] = q;
         }
-        tIA[i + 1] = (int)tJA.size();
-        const int rn = q1 - q0;
-        if (rows > 0 && (rows == FG_TPB || nz + rn > TRI_NNZB)) {
-            rb.push_back(i);
-            rows = 0;
-            nz = 0;
-        }
-        rows++;
-        nz += rn;
     }
-    if (rows > 0) rb.push_back(neq);
-    t.nrb = (int)rb.size() - 1;
-    t.nnz = (int)tJA.size();
-    const size_t nzz = t.nnz ? t.nnz : 1;
-    FG_HIP(hipMalloc(&t.d_rb, sizeof(int) * rb.size()));
-    FG_HIP(hipMalloc(&t.d_IA, sizeof(int) * (neq + 1)));
-    FG_HIP(hipMalloc(&t.d_JA, sizeof(int) * nzz));
-    FG_HIP(hipMalloc(&t.d_map, sizeof(int) * nzz));
-    FG_HIP(hipMalloc(&t.d_v64, sizeof(double) * nzz));
-    FG_HIP(hipMalloc(&t.d_v32, sizeof(float) * nzz));
-    FG_HIP(hipMemcpy(t.d_rb, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice));
-    FG_HIP(hipMemcpy(t.d_IA, tIA.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice));
-    if (t.nnz) {
-        FG_HIP(hipMemcpy(t.d_JA, tJA.data(), sizeof(int) * t.nnz, hipMemcpyHostToDevice));
-        FG_HIP(hipMemcpy(t.d_map, map.data(), sizeof(int) * t.nnz, hipMemcpyHostToDevice));
+    t.nslice = L.nslice;
+    t.nslot = L.nslot;
+    const size_t ns = t.nslot ? t.nslot : 1;
+    FG_HIP(hipMalloc(&t.d_soff, sizeof(int) * L.soff.size()));
+    FG_HIP(hipMalloc(&t.d_JA, sizeof(int) * ns));
+    FG_HIP(hipMalloc(&t.d_map, sizeof(long long) * ns));
+    FG_HIP(hipMalloc(&t.d_v64, sizeof(double) * ns));
+    FG_HIP(hipMalloc(&t.d_v32, sizeof(float) * ns));
+    FG_HIP(hipMemcpy(t.d_soff, L.soff.data(), sizeof(int) * L.soff.size(),
+                     hipMemcpyHostToDevice));
+    if (t.nslot) {
+        FG_HIP(hipMemcpy(t.d_JA, col.data(), sizeof(int) * t.nslot, hipMemcpyHostToDevice));
+        FG_HIP(hipMemcpy(t.d_map, map.data(), sizeof(long long) * t.nslot,
+                         hipMemcpyHostToDevice));
     }
     return FG_OK;
 }
 
 static void free_tri(fg_ilu::Tri &t)
 {
-    hipFree(t.d_rb); hipFree(t.d_IA); hipFree(t.d_JA); hipFree(t.d_map);
-    hipFree(t.d_v64); hipFree(t.d_v32);
+    hipFree(t.d_soff); hipFree(t.d_JA); hipFree(t.d_map); hipFree(t.d_v64); hipFree(t.d_v32);
 }
 
-__global__ void k_tri_gather(int n, const int *__restrict__ map, const double *__restrict__ LU,
-                             double *__restrict__ v64, float *__restrict__ v32)
+__global__ void k_tri_gather(long long n, const long long *__restrict__ map,
+                             const double *__restrict__ LU, double *__restrict__ v64,
+                             float *__restrict__ v32)
 {
-    for (int t = blockIdx.x * FG_TPB + threadIdx.x; t < n; t += gridDim.x * FG_TPB) {
-        const double a = LU[map[t]];
+    for (long long t = (long long)blockIdx.x * FG_TPB + threadIdx.x; t < n;
+         t += (long long)gridDim.x * FG_TPB) {
+        const long long q = map[t];
+        const double a = q < 0 ? 0.0 : LU[q];
         v64[t] = a;
         v32[t] = (float)a;
     }
@@ -414,9 +411,10 @@ int fg_ilu_factor(fg_ilu *u, fg_op *op)
         FG_LAUNCH(k_ilu_cast, fg_vec_grid(c, u->neq), FG_TPB, 0, c->stream, (long long)u->neq,
                   u->d_D, u->d_D32);
     for (fg_ilu::Tri *t : {&u->lt, &u->ut})
-        if (t->nnz)
-            FG_LAUNCH(k_tri_gather, fg_vec_grid(c, t->nnz), FG_TPB, 0, c->stream, t->nnz,
-                      (const int *)t->d_map, (const double *)u->d_LU, t->d_v64, t->d_v32);
+        if (t->nslot)
+            FG_LAUNCH(k_tri_gather, fg_vec_grid(c, t->nslot), FG_TPB, 0, c->stream, t->nslot,
+                      (const long long *)t->d_map, (const double *)u->d_LU, t->d_v64,
+                      t->d_v32);
     return FG_OK;
 }
 
@@ -454,76 +452,32 @@ static int ilu_apply_t(fg_ilu *u, fg_ctx *c, const T *LU, const T *D, const T *p
 // can use it as a preconditioner; every sweep is one fully parallel pass.
 // out[r] = base[r] - (T x)[r]            (UPPER = false: a forward sweep)
 // out[r] = D[r] * (base[r] - (T x)[r])   (UPPER = true: a backward sweep)
-// T is the strictly lower or upper part as a compact CSR; row blocks are
-// streamed with coalesced loads through LDS as in the CSR product.
+// T is the strictly lower or upper part as sliced rows (fg_sell.h): one
+// row per lane, every load of the row issued at once, the terms added in the
+// row's entry order.
 template <typename T, bool UPPER>
-__global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nrb, const int *__restrict__ rb,
-                                                      const int *__restrict__ IA,
+__global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nslice, const int *__restrict__ soff,
                                                       const int *__restrict__ JA,
                                                       const T *__restrict__ V,
                                                       const T *__restrict__ x,
                                                       const T *__restrict__ base,
                                                       const T *__restrict__ D,
-                                                      T *__restrict__ out, const int *skip)
+                                                      T *__restrict__ out, int n, const int *skip)
 {
     if (skip && *skip) return;
-    __shared__ T s[TRI_NNZB];
-    __shared__ T ws[FG_TPB / 64];
-    const int b = fg_xcd_remap(blockIdx.x, nrb);
-    const int r0 = rb[b], r1 = rb[b + 1];
-    const int j0 = IA[r0], j1 = IA[r1];
-    // blocks hold at most FG_TPB rows: one row per thread in the sum phase,
-    // its bounds, base and pivot loaded before the barrier
-    const int r = r0 + (int)threadIdx.x;
-    int ra = 0, re = 0;
-    T br = T(0), dr = T(0);
-    if (r < r1) {
-        ra = IA[r];
-        re = IA[r + 1];
-        br = base[r];
-        if (UPPER) dr = D[r];
-    }
-    if (j1 - j0 <= TRI_NNZB) {
-        // the factor's values and columns are read once per sweep
-        // (non-temporal); every thread issues all of its loads of the block
-        // before the gathers of x, so TRI_NPT load chains are in flight
-        // (slots past the block's last entry load that entry again and store
-        // to a dummy LDS word: no branch per load, exact counter waits)
-        constexpr int TRI_NPT = TRI_NNZB / FG_TPB;
-        __shared__ T dummy;
-        if (j1 > j0) {
-            int col[TRI_NPT];
-            T v[TRI_NPT], xv[TRI_NPT];
-#pragma unroll
-            for (int q = 0; q < TRI_NPT; q++) {
-                const int j = j0 + (int)threadIdx.x + q * FG_TPB;
-                const int jc = j < j1 ? j : j1 - 1;
-                col[q] = __builtin_nontemporal_load(&JA[jc]);
-                v[q] = __builtin_nontemporal_load(&V[jc]);
-            }
-#pragma unroll
-            for (int q = 0; q < TRI_NPT; q++) xv[q] = x[col[q]];
-#pragma unroll
-            for (int q = 0; q < TRI_NPT; q++) {
-                const int j = (int)threadIdx.x + q * FG_TPB;
-                *(j < j1 - j0 ? &s[j] : &dummy) = v[q] * xv[q];
-            }
-        }
-        __syncthreads();
-        if (r < r1) {
-            T acc = T(0);
-            for (int j = ra - j0; j < re - j0; j++) acc = acc + s[j];
-            const T y = br - acc;
-            out[r] = UPPER ? dr * y : y;
-        }
-    } else {   // one row longer than a block
-        T acc = T(0);
-        for (int j = j0 + threadIdx.x; j < j1; j += FG_TPB) acc = acc + V[j] * x[JA[j]];
-        acc = fg_block_sum(acc, ws);
-        if (threadIdx.x == 0) {
-            const T y = base[r0] - acc;
-            out[r0] = UPPER ? D[r0] * y : y;
-        }
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
+                  (int)(threadIdx.x / FG_SELL_W);
+    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    if (s < nslice) {
+        const int r = s * FG_SELL_W + lane;
+        const int rc = r < n ? r : n - 1;
+        const T br = base[rc];
+        const T dr = UPPER ? D[rc] : T(0);
+        const int o0 = soff[s];
+        const T acc = fg_sell_row(fg_sell_prod<T>{JA, V, x}, o0, soff[s + 1] - o0, lane);
+        const T y = br - acc;
+        if (r < n) out[r] = UPPER ? dr * y : y;
     }
 }
 
@@ -544,6 +498,12 @@ const double *tri_vals<double>(const fg_ilu::Tri &t) { return t.d_v64; }
 template <>
 const float *tri_vals<float>(const fg_ilu::Tri &t) { return t.d_v32; }
 
+static int sweep_grid(const fg_ilu::Tri &t)
+{
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    return (t.nslice + SPW - 1) / SPW;
+}
+
 template <typename T>
 static int ilu_sweeps_t(fg_ilu *u, fg_ctx *c, const T *D, const T *p, T *z, T *const s[2],
                         const int *skip)
@@ -554,10 +514,10 @@ static int ilu_sweeps_t(fg_ilu *u, fg_ctx *c, const T *D, const T *p, T *z, T *c
     const T *y = p;
     for (int m = 0; m < S; m++) {
         T *out = s[m & 1];
-        if (L.nrb)
-            FG_LAUNCH((k_tri_sweep<T, false>), L.nrb, FG_TPB, 0, c->stream, L.nrb,
-                      (const int *)L.d_rb, (const int *)L.d_IA, (const int *)L.d_JA,
-                      tri_vals<T>(L), y, p, D, out, skip);
+        if (L.nslice)
+            FG_LAUNCH((k_tri_sweep<T, false>), sweep_grid(L), FG_TPB, 0, c->stream, L.nslice,
+                      (const int *)L.d_soff, (const int *)L.d_JA, tri_vals<T>(L), y, p, D, out,
+                      n, skip);
         y = out;
     }
     // backward: x_0..x_S between the free scratch and z, x_S in z (p is no
@@ -566,10 +526,10 @@ static int ilu_sweeps_t(fg_ilu *u, fg_ctx *c, const T *D, const T *p, T *z, T *c
     T *buf[2] = {(S & 1) ? other : z, (S & 1) ? z : other};   // x_m in buf[m & 1]
     FG_LAUNCH(k_dscale<T>, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, D, y, buf[0], skip);
     for (int m = 1; m <= S; m++)
-        if (U.nrb)
-            FG_LAUNCH((k_tri_sweep<T, true>), U.nrb, FG_TPB, 0, c->stream, U.nrb,
-                      (const int *)U.d_rb, (const int *)U.d_IA, (const int *)U.d_JA,
-                      tri_vals<T>(U), (const T *)buf[(m - 1) & 1], y, D, buf[m & 1], skip);
+        if (U.nslice)
+            FG_LAUNCH((k_tri_sweep<T, true>), sweep_grid(U), FG_TPB, 0, c->stream, U.nslice,
+                      (const int *)U.d_soff, (const int *)U.d_JA, tri_vals<T>(U),
+                      (const T *)buf[(m - 1) & 1], y, D, buf[m & 1], n, skip);
     return FG_OK;
 }
 

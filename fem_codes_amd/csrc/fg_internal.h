@@ -138,6 +138,14 @@ struct fg_op {
     int nrb = 0;
     int *d_rb = nullptr;         // [nrb+1] row-block boundaries
     std::vector<int> h_IA, h_JA;
+    // sliced rows (fg_sell.h) of the CSR product and of the EDE half-edge
+    // rows: slice offsets, entry columns (-1: padding), values (EDE: pairs)
+    int nslice = 0;
+    long long nslot = 0;
+    int *d_soff = nullptr, *d_scol = nullptr;
+    long long *d_spos = nullptr; // CSR: slot of each entry (null: row-block product)
+    double *d_sv64 = nullptr;
+    float *d_sv32 = nullptr;
 
     // ---- systems of at most FG_SMALL_MAX equations: lane-coalesced lists for
     // the one-workgroup solver, entry c of row g at c*neq + g (-1: none) ----
@@ -147,14 +155,11 @@ struct fg_op {
     int *d_sm_lmT = nullptr;     // EBE/EDE: lm transposed, [k][nitems]
     // EDE (k = 2): half-edge rows (fg_ede_rows.hip), a second copy of the
     // edge values ordered by equation, kept in step with the blocked copy
-    int edge_rows = 0;           // FG_EDGE_ROWS / _BLOCKED / _AUTO (rows for FP64 only)
+    int edge_rows = 0;           // FG_EDGE_ROWS / _BLOCKED / _AUTO (rows)
     int rows_stale = 1;          // the rows' values lag the blocked values
-    int nrrb = 0;                // row blocks
+    // (sliced: d_soff / d_scol / d_sv64 / d_sv32 below the CSR fields)
     long long nhalf = 0;         // half-edges (rows' entries)
-    int *d_rrb = nullptr, *d_rIA = nullptr, *d_hcol = nullptr, *d_hsrc = nullptr;
-    uint8_t *d_hrow = nullptr;
-    double *d_hv64 = nullptr;
-    float *d_hv32 = nullptr;
+    long long *d_hsrc = nullptr; // [nslot] (blocked value index << 1) | end, -1: padding
 
     // ---- EBE2 (shares the storage of `base` and the blocks of `bd`) ----
     fg_op *base = nullptr;
@@ -282,10 +287,13 @@ int fg_op_apply_any(fg_op *op, int fp32, void *P, void *Q, const int *skip = nul
 // EDE half-edge rows (fg_ede_rows.hip)
 int fg_ede_rows_build(fg_op *op, const int *lm2, const long long *kpos);
 int fg_ede_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
+// the CSR operator's AA / AA32 were written: refresh the sliced copy
+int fg_csr_values_changed(fg_op *op);
 void fg_ede_rows_free(fg_op *op);
 inline bool fg_ede_rows_used(const fg_op *op, int fp32)
 {
-    return op->d_rrb && (op->edge_rows == FG_EDGE_ROWS || (op->edge_rows == FG_EDGE_AUTO && !fp32));
+    (void)fp32;
+    return op->d_hsrc && op->edge_rows != FG_EDGE_BLOCKED;
 }
 int fg_prec_apply_any(fg_prec *pc, int fp32, const void *p, void *z, const int *skip = nullptr);
 int fg_vec_grid(fg_ctx *ctx, long long n);

@@ -62,7 +62,7 @@ extern "C" int fg_op_set_values_dev(fg_op *op, const double *vals)
     if (op->kind == FG_OP_CSR) {
         FG_LAUNCH(k_copy_cast, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, vals, op->d_AA,
                   op->d_AA32);
-        return FG_OK;
+        return fg_csr_values_changed(op);
     }
     return fg_ebe_set_values_dev(op, vals);
 }

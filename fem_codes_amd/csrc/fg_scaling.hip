@@ -84,6 +84,7 @@ extern "C" int fg_scaling_diag(fg_prec *pc, int mode, double *F)
     if (op->kind == FG_OP_CSR) {
         FG_LAUNCH(k_scale_csr, fg_vec_grid(c, neq), FG_TPB, 0, c->stream, neq, mode, op->d_IA,
                   op->d_JA, pc->d_invdiag, op->d_AA, op->d_AA32);
+        FG_CHECK(fg_csr_values_changed(op));
     } else {
         FG_ARG(op->d_lm, "operator without its element lm");
         const long long n = (long long)op->nitems * op->k * op->k;

@@ -1,0 +1,131 @@
+// fg_sell.h -- rows in wavefront slices ("sliced ELL"), shared by the CSR
+// product and the EDE half-edge rows.
+//
+// Rows are cut into slices of 64 consecutive rows, one row per lane of a
+// wavefront.  Slice s owns slot columns soff[s] .. soff[s+1]-1 (its widest
+// row's entry count); entry k of the row at lane l sits at slot
+// (soff[s] + k) * 64 + l, so the k-th entries of the slice's 64 rows are 64
+// consecutive words: every load of a product is one fully coalesced
+// wavefront access, issued without a branch, and each lane adds its row's
+// terms in entry order in registers -- no LDS staging, no barrier, the same
+// per-row summation order as a serial row loop.  Unused slots hold column -1
+// (their loads are issued like the others; the term is masked to nothing).
+#pragma once
+
+#include <vector>
+
+#include "fg_internal.h"
+
+constexpr int FG_SELL_W = 64;       // rows per slice (one wavefront)
+constexpr int FG_SELL_CHUNK = 8;    // entries per unrolled chunk of a wide row
+
+// host: slice layout of rows with entry counts rowptr[r+1]-rowptr[r]
+struct fg_sell_layout {
+    int nslice = 0;
+    long long nslot = 0;
+    std::vector<int> soff;          // [nslice+1] in units of 64 slots
+    // slot of entry k of row r
+    long long slot(int r, int k) const
+    {
+        return ((long long)soff[r / FG_SELL_W] + k) * FG_SELL_W + (r % FG_SELL_W);
+    }
+};
+
+inline fg_sell_layout fg_sell_plan(int nrows, const int *rowptr)
+{
+    fg_sell_layout L;
+    L.nslice = (nrows + FG_SELL_W - 1) / FG_SELL_W;
+    L.soff.assign((size_t)L.nslice + 1, 0);
+    for (int s = 0; s < L.nslice; s++) {
+        int w = 0;
+        const int r1 = (s + 1) * FG_SELL_W < nrows ? (s + 1) * FG_SELL_W : nrows;
+        for (int r = s * FG_SELL_W; r < r1; r++) {
+            const int n = rowptr[r + 1] - rowptr[r];
+            w = n > w ? n : w;
+        }
+        L.soff[s + 1] = L.soff[s] + w;
+    }
+    L.nslot = (long long)L.soff[L.nslice] * FG_SELL_W;
+    return L;
+}
+
+// Issues the loads of L consecutive entries of a row (slot `at` and every
+// 64th after it), gathers, and adds the terms to acc in entry order.  With
+// CLAMP, entries at or beyond `nleft` re-read entry nleft-1 and add nothing
+// (all loads still issued on every path).  Row::load(slot) returns an entry
+// record, Row::gather(rec) issues its dependent loads, Row::term(rec) forms
+// the term (0 for a padding slot).
+template <int L, bool CLAMP, typename Row>
+__device__ __forceinline__ typename Row::T fg_sell_chunk(const Row &row, long long at, int nleft,
+                                                         typename Row::T acc)
+{
+    typename Row::Rec e[L];
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+        const int kk = CLAMP ? (k < nleft ? k : nleft - 1) : k;
+        e[k] = row.load(at + (long long)kk * FG_SELL_W);
+    }
+#pragma unroll
+    for (int k = 0; k < L; k++) row.gather(e[k]);
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+        const typename Row::T t = row.term(e[k]);
+        if (CLAMP) acc = k < nleft ? acc + t : acc;
+        else acc = acc + t;
+    }
+    return acc;
+}
+
+// The row sum of lane `lane` of a slice of width w starting at slot column o0:
+// rows of up to 8 entries fully unrolled (one load burst), wider rows in
+// chunks of 8.
+template <typename Row>
+__device__ __forceinline__ typename Row::T fg_sell_row(const Row &row, int o0, int w, int lane)
+{
+    using T = typename Row::T;
+    const long long at = (long long)o0 * FG_SELL_W + lane;
+    switch (w) {
+    case 0: return T(0);
+    case 1: return fg_sell_chunk<1, false>(row, at, 1, T(0));
+    case 2: return fg_sell_chunk<2, false>(row, at, 2, T(0));
+    case 3: return fg_sell_chunk<3, false>(row, at, 3, T(0));
+    case 4: return fg_sell_chunk<4, false>(row, at, 4, T(0));
+    case 5: return fg_sell_chunk<5, false>(row, at, 5, T(0));
+    case 6: return fg_sell_chunk<6, false>(row, at, 6, T(0));
+    case 7: return fg_sell_chunk<7, false>(row, at, 7, T(0));
+    case 8: return fg_sell_chunk<8, false>(row, at, 8, T(0));
+    default: {
+        T acc = T(0);
+        for (int k0 = 0; k0 < w; k0 += FG_SELL_CHUNK)
+            acc = fg_sell_chunk<FG_SELL_CHUNK, true>(row, at + (long long)k0 * FG_SELL_W, w - k0,
+                                                     acc);
+        return acc;
+    }
+    }
+}
+
+// the entry kind of a sparse product row: a column (-1: padding) and a
+// value, the term value * x[column]
+template <typename TV>
+struct fg_sell_prod {
+    using T = TV;
+    struct Rec {
+        int c;
+        T v, p;
+    };
+    const int *__restrict__ col;
+    const T *__restrict__ val;
+    const T *__restrict__ x;
+    // columns and values are read once per product: non-temporal, so that
+    // they do not evict the x entries neighbouring slices gather again
+    __device__ __forceinline__ Rec load(long long q) const
+    {
+        Rec r;
+        r.c = __builtin_nontemporal_load(&col[q]);
+        r.v = __builtin_nontemporal_load(&val[q]);
+        r.p = T(0);
+        return r;
+    }
+    __device__ __forceinline__ void gather(Rec &r) const { r.p = x[r.c < 0 ? 0 : r.c]; }
+    __device__ __forceinline__ T term(const Rec &r) const { return r.c < 0 ? T(0) : r.v * r.p; }
+};

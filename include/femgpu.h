@@ -144,13 +144,13 @@ int fg_op_set_pipeline(fg_op *op, int wg_per_cu);
 #define FG_ROWS_MFMA 1
 int fg_op_set_row_kernel(fg_op *op, int kind);
 /* EDE product kernel (no reference counterpart: a kernel choice).
- * FG_EDGE_ROWS: the half-edges grouped by equation in edge order, each
- * equation adding its terms A0*p1 + A1*p2 / A2*p1 + A3*p2 in edge order --
- * edemv.c bit for bit (FP64).  FG_EDGE_BLOCKED: the blocked element kernel
- * at k = 2 (halo sums in block order, 1e-14).  FG_EDGE_AUTO (default): rows
- * for FP64 products, blocked for FP32 (the faster of the two for each, as
- * measured on MI355X).  All read the same values (the rows are refreshed
- * from the blocked copy after every change). */
+ * FG_EDGE_ROWS: the half-edges grouped by equation in edge order (64
+ * equations per wavefront slice), each equation adding its terms
+ * A0*p1 + A1*p2 / A2*p1 + A3*p2 in edge order -- edemv.c bit for bit (FP64).
+ * FG_EDGE_BLOCKED: the blocked element kernel at k = 2 (halo sums in block
+ * order, 1e-14).  FG_EDGE_AUTO (default): rows, the faster of the two in
+ * both precisions as measured on MI355X.  All read the same values (the rows
+ * are refreshed from the blocked copy after every change). */
 #define FG_EDGE_ROWS    0
 #define FG_EDGE_BLOCKED 1
 #define FG_EDGE_AUTO    2

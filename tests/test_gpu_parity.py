@@ -68,6 +68,54 @@ def test_matvec_f64(ctx, path):
         assert Q[neq] == 0.0
 
 
+def _csr_case(shape, n, rng):
+    """random CSR patterns of the row shapes the product's two layouts meet"""
+    if shape == "uniform7":
+        lens = np.full(n, 7)
+    elif shape == "wide":           # 15..17 entries: sliced rows in chunks of 8
+        lens = rng.integers(15, 18, n)
+    elif shape == "ragged":         # 0..20 entries: too much padding, row blocks
+        lens = rng.integers(0, 21, n)
+    else:                           # "arrow": row 0 holds every column
+        lens = np.full(n, 3)
+        lens[0] = n
+    IA = np.zeros(n + 1, np.int32)
+    IA[1:] = np.cumsum(lens)
+    JA = np.concatenate([np.sort(rng.choice(n, size=int(m), replace=False)) for m in lens]
+                        ).astype(np.int32)
+    AA = rng.standard_normal(int(IA[-1]))
+    P = rng.standard_normal(n + 1)
+    return IA, JA, AA, P
+
+
+@pytest.mark.parametrize("shape", ["uniform7", "wide", "ragged", "arrow"])
+def test_csr_row_shapes(ctx, shape):
+    """CSR product on patterns of every row shape: FP64 bitwise against the
+    reference's csrmv (oracle), except the arrow's 3000-entry row, which is
+    longer than a row block and is summed as a tree (1e-14 of sum |a p|);
+    FP32 bitwise against a float32 row loop in entry order."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    IA, JA, AA, P = _csr_case(shape, n, rng)
+    op = F.Operator.csr(ctx, n, IA, JA)
+    op.set_values(AA)
+    Q = op.matvec(P)
+    ref = O.Oracle().csrmv(n, AA, JA, IA, P.copy())
+    rows = np.arange(1 if shape == "arrow" else 0, n)
+    assert np.array_equal(Q[rows], ref[rows])
+    assert Q[n] == 0.0
+    if shape == "arrow":
+        bound = np.sum(np.abs(AA[: IA[1]] * P[JA[: IA[1]]]))
+        assert abs(Q[0] - ref[0]) <= 1e-14 * bound
+    A32, P32 = AA.astype(np.float32), P.astype(np.float32)
+    Q32 = op.matvec(P32, np.float32)
+    for r in rows[:: 7]:
+        acc = np.float32(0)
+        for j in range(IA[r], IA[r + 1]):
+            acc = np.float32(acc + np.float32(A32[j] * P32[JA[j]]))
+        assert Q32[r] == acc, r
+
+
 @pytest.mark.parametrize("path", golden_files("ss2d_n16_*.npz"), ids=os.path.basename)
 def test_matvec_f32(ctx, path, orc):
     """FP32 product against the FP64 reference product: the error is bounded

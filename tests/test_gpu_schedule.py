@@ -169,10 +169,11 @@ def test_ede_rows_bitwise(system):
     assert np.array_equal(op.matvec(P), Qo)
     Qr32 = op.matvec(P.astype(np.float32), np.float32)
     assert np.max(np.abs(Qr32[:neq] - Qo[:neq])) <= 1e-6 * np.max(np.abs(Qo[:neq]))
-    # the default (AUTO): rows in FP64, blocked in FP32
+    # the default (AUTO): rows in both precisions
     op.set_edge_kernel(F.EDGE_AUTO)
     assert np.array_equal(op.matvec(P), Qo)
-    assert np.array_equal(op.matvec(P.astype(np.float32), np.float32), Qb32)
+    assert np.array_equal(op.matvec(P.astype(np.float32), np.float32), Qr32)
+    assert np.max(np.abs(Qb32[:neq] - Qo[:neq])) <= 1e-6 * np.max(np.abs(Qo[:neq]))
     op.close()
 
 
