@@ -102,9 +102,15 @@ def leg_traffic(leg, kernel):
     return tj.get("hbm_bytes_per_launch") if tj.get("kernel") == kernel else None
 
 
-def ebe_kernel_name(k, fp32):
+def ebe_kernel_name(k, fp32, op=None):
+    """the dominant kernel of an element product: the sliced rows where the
+    operator uses them, else the blocked kernel at the library's default
+    block size"""
+    t = "float" if fp32 else "double"
+    if op is not None and op.product_sliced():
+        return f"k_ebe_rows<{t}, {k}>"
     ept = 4 if k <= 2 else 2 if k == 3 else 4 if k == 4 else 1   # the library's defaults
-    return f"k_ebe_apply<{k}, {ept}, {'float' if fp32 else 'double'}, 0>"
+    return f"k_ebe_apply<{k}, {ept}, {t}, 0>"
 
 
 def time_cpu(run, budget_s, max_reps):
@@ -233,7 +239,9 @@ def headline_leg(F, ctx, args, rng):
         log("autotune EBE f64 ms by elements/block / ordering / workgroups per CU:",
             {autotune.key_str(k): round(v, 4) for k, v in tune_ms.items()}, "->",
             autotune.key_str((best_epb, best_ord, best_pipe)))
-        if (best_epb, best_ord, best_pipe) == (512, F.ORDER_BFS, 0):
+        if best_pipe == autotune.ROWS or (best_epb, best_ord, best_pipe) == (512, F.ORDER_BFS, 0):
+            default_op.set_product_kernel(F.PRODUCT_AUTO if best_pipe == autotune.ROWS
+                                          else F.PRODUCT_BLOCKED)
             # the tuner chose the default configuration: keep the operator
             # built at setup (allocated before the tuning candidates)
             ebe_tuned.close()
@@ -251,11 +259,16 @@ def headline_leg(F, ctx, args, rng):
     bytes64 = ebe.algorithmic_bytes(False)
     value = bytes64 / (ms * 1e-3) / 1e9
     log(f"EBE f64: {ms:.4f} ms/matvec (block kernel {ms_main:.4f} ms) -> {value:.1f} GB/s")
-    kernel = (f"k_ebe_apply_pipe<3, {best_epb // 256}, double, 0>" if best_pipe
-              else f"k_ebe_apply<3, {best_epb // 256}, double, 0>")
+    if ebe.product_sliced():
+        kernel = "k_ebe_rows<double, 3>"
+        schedule = {"kernel": "sliced rows"}
+    else:
+        kernel = (f"k_ebe_apply_pipe<3, {best_epb // 256}, double, 0>" if best_pipe
+                  else f"k_ebe_apply<3, {best_epb // 256}, double, 0>")
+        schedule = {"kernel": "blocked", "elems_per_block": best_epb, "wg_per_cu": best_pipe,
+                    "ordering": autotune.ORDER_NAMES[best_ord]}
     roof = roofline(ms_main, bytes64, kernel)
-    roof["schedule"] = {"elems_per_block": best_epb, "wg_per_cu": best_pipe,
-                        "ordering": autotune.ORDER_NAMES[best_ord]}
+    roof["schedule"] = schedule
     if n == 2048:
         roof["traffic"] = leg_traffic("headline", kernel)
 
@@ -502,7 +515,7 @@ def batched_leg(F, ctx, args, nprob, seed, kmax=30, tol=1e-8):
     Q64 = ctx.empty(N + 1)
     res["matvec"] = matvec_table(op, P64, Q64, P32, Q32, 20)
     ms_main = op.time_apply(P32, Q32, reps=20, main_only=True)
-    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True), ebe_kernel_name(3, True))
+    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True), ebe_kernel_name(3, True, op))
     res["roofline"]["traffic"] = leg_traffic("batched", res["roofline"]["kernel"])
     Xs = {}
     for precision, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
@@ -684,7 +697,7 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
     if args.no_autotune:
         ebe = F.Operator.ebe(ctx, neq, lm, 4)
         ebe.set_values(ke)
-        ebe_kernel = ebe_kernel_name(4, True)
+        ebe_kernel = ebe_kernel_name(4, True, ebe)
     else:
         # the FP32 product of the inner cycle: elements per block x ordering
         from fem_codes_amd import autotune
@@ -694,7 +707,8 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
                      "ms": {autotune.key_str(k): v for k, v in tms.items()}}
         log("ss3d autotune FP32 product ms:", {autotune.key_str(k): round(v, 4)
                                                for k, v in tms.items()}, "->", autotune.key_str(key))
-        ebe_kernel = f"k_ebe_apply<4, {key[0] // 256}, float, 0>"
+        ebe_kernel = ("k_ebe_rows<float, 4>" if ebe.product_sliced()
+                      else f"k_ebe_apply<4, {key[0] // 256}, float, 0>")
     del cbe, IA, JA
     pc = F.Preconditioner(csr, F.PREC_ILU0)
     levels = pc.ilu_levels()
@@ -849,7 +863,7 @@ def transient_leg(F, ctx, args, n=1024, steps=10):
     Q32 = ctx.empty(neq + 1, np.float32)
     res["matvec"] = matvec_table(op, P64, Q64, P32, Q32, 20)
     ms_main = op.time_apply(P64, Q64, reps=20, main_only=True)
-    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(False), ebe_kernel_name(3, False))
+    res["roofline"] = roofline(ms_main, op.algorithmic_bytes(False), ebe_kernel_name(3, False, op))
     res["roofline"]["traffic"] = leg_traffic("transient", res["roofline"]["kernel"])
     if not args.no_cpu:
         import oracle as O

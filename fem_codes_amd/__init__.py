@@ -29,6 +29,7 @@ PREC_NONE, PREC_DIAG, PREC_JACOBI, PREC_ILU0, PREC_BLOCKDIAG = 0, 1, 2, 3, 4
 ORDER_BFS, ORDER_MESH = 0, 1
 ROWS_VALU, ROWS_MFMA = 0, 1
 EDGE_ROWS, EDGE_BLOCKED, EDGE_AUTO = 0, 1, 2
+PRODUCT_SLICED, PRODUCT_BLOCKED, PRODUCT_AUTO = 0, 1, 2
 EBE2_BLOCKDIAG, EBE2_NOBLOCKDIAG = 0, 1
 P3_DIFCONV, P3_CONST = 0, 1
 GMRES_FP64, GMRES_FP32_IR = 0, 1
@@ -115,6 +116,8 @@ SIGNATURES = {
     "fg_op_set_pipeline": (_i, [_vp, _i]),
     "fg_op_set_row_kernel": (_i, [_vp, _i]),
     "fg_op_set_edge_kernel": (_i, [_vp, _i]),
+    "fg_op_set_product_kernel": (_i, [_vp, _i]),
+    "fg_op_product_kernel": (_i, [_vp, C.POINTER(_i)]),
     "fg_prec_create": (_i, [_vp, _i, C.POINTER(_vp)]),
     "fg_prec_setup": (_i, [_vp, _vp]),
     "fg_prec_apply_f64": (_i, [_vp, _vp, _vp]),
@@ -309,10 +312,11 @@ class Operator:
         self.neq, self.nitems, self.k, self.nnz = neq.value, nit.value, k.value, nnz.value
 
     @classmethod
-    def ebe(cls, ctx, neq, lm, k, elems_per_block=0, ordering=None):
+    def ebe(cls, ctx, neq, lm, k, elems_per_block=0, ordering=None, product=None):
         """ebemv family: lm (nel*k) with sink = neq.  ordering None or
         ORDER_BFS (blocks grown breadth-first) or ORDER_MESH (runs of
-        consecutive elements in lm's order)."""
+        consecutive elements in lm's order); product None (PRODUCT_AUTO),
+        PRODUCT_SLICED or PRODUCT_BLOCKED."""
         lm = np.ascontiguousarray(lm, np.int32)
         h = _vp()
         if ordering is None:
@@ -321,7 +325,10 @@ class Operator:
         else:
             _check(lib.fg_op_ebe_create_ex(ctx.h, neq, lm.size // k, k, lm, elems_per_block,
                                            int(ordering), 0, C.byref(h)), "fg_op_ebe_create_ex")
-        return cls(ctx, h, "EBE")
+        op = cls(ctx, h, "EBE")
+        if product is not None:
+            op.set_product_kernel(product)
+        return op
 
     @classmethod
     def ede(cls, ctx, neq, lm2, edges_per_block=0, ordering=None):
@@ -406,6 +413,17 @@ class Operator:
         """EDGE_AUTO (default: rows in both precisions), EDGE_ROWS or
         EDGE_BLOCKED (EDE operators)"""
         _check(lib.fg_op_set_edge_kernel(self.h, int(kind)), "fg_op_set_edge_kernel")
+
+    def set_product_kernel(self, kind):
+        """PRODUCT_AUTO (default: sliced rows where the operator has them),
+        PRODUCT_SLICED or PRODUCT_BLOCKED (EBE and EDE operators)"""
+        _check(lib.fg_op_set_product_kernel(self.h, int(kind)), "fg_op_set_product_kernel")
+
+    def product_sliced(self) -> bool:
+        """whether the product in use is a sliced-row kernel"""
+        v = _i()
+        _check(lib.fg_op_product_kernel(self.h, C.byref(v)), "fg_op_product_kernel")
+        return bool(v.value)
 
     def close(self):
         if self.h and self.ctx.h:

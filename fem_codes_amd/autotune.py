@@ -29,14 +29,15 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import EDGE_AUTO, EDGE_BLOCKED, EDGE_ROWS, ORDER_BFS, ORDER_MESH, DeviceArray, FemGpuError, Operator
+from . import (ORDER_BFS, ORDER_MESH, PRODUCT_AUTO, PRODUCT_BLOCKED, DeviceArray,
+               FemGpuError, Operator)
 
 CANDIDATES = (256, 512, 1024, 2048)
 PIPELINES = (0, 1, 2, 3, 4, 6)
 ORDERINGS = (ORDER_BFS, ORDER_MESH)
 ORDER_NAMES = {ORDER_BFS: "bfs", ORDER_MESH: "mesh"}
 RESTARTS = (10, 15, 20, 30, 45, 64)
-ROWS = "rows"   # the schedule entry of the half-edge row kernel (edges)
+ROWS = "rows"   # the schedule entry of the sliced-row kernel (edges, k = 3 / 4 elements)
 
 
 def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
@@ -45,8 +46,9 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
     """Return ((epb, ordering, pipeline), {(epb, ordering, pipe): ms}, best_operator).
 
     values: AoS element (or edge) matrices, host array or DeviceArray.
-    edges: build edge-by-edge operators (k must be 2); the half-edge row
-    kernel is timed too, as the candidate (0, ORDER_BFS, ROWS).
+    edges: build edge-by-edge operators (k must be 2).  Where the operator
+    has sliced rows (edges; elements with k = 3 or 4), that kernel is timed
+    too, as the candidate (0, ORDER_BFS, ROWS).
     orderings: the first is tuned over every block size; the others are
     timed at the block size that won."""
     dt = np.float32 if fp32 else np.float64
@@ -63,11 +65,10 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
         except FemGpuError:
             return
         op.set_values(values)
-        if edges:
-            if not ops:
-                op.set_edge_kernel(EDGE_ROWS)
-                times[(0, ORDER_BFS, ROWS)] = op.time_apply(P, Q, reps=reps)
-            op.set_edge_kernel(EDGE_BLOCKED)
+        if not ops and op.product_sliced():
+            times[(0, ORDER_BFS, ROWS)] = op.time_apply(P, Q, reps=reps)
+        if edges or op.product_sliced():
+            op.set_product_kernel(PRODUCT_BLOCKED)
         for pipe in pipelines:
             op.set_pipeline(pipe)
             times[(epb, order, pipe)] = op.time_apply(P, Q, reps=reps)
@@ -89,9 +90,9 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
             op.close()
     op = ops[bbest[:2]]
     op.set_pipeline(bbest[2])
-    if edges:
-        # rows won: both precisions use them (EDGE_AUTO)
-        op.set_edge_kernel(EDGE_AUTO if best[2] == ROWS else EDGE_BLOCKED)
+    # sliced rows won: both precisions use them (PRODUCT_AUTO); else the
+    # blocked schedule that won
+    op.set_product_kernel(PRODUCT_AUTO if best[2] == ROWS else PRODUCT_BLOCKED)
     return best, times, op
 
 

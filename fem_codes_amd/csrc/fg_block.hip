@@ -419,6 +419,7 @@ extern "C" int fg_block_scaling_ebe(fg_prec *pc, double *F)
     FG_LAUNCH(k_block_scale, fg_vec_grid(c, n), FG_TPB, 0, c->stream, b->nel, b->nnodes,
               op->neq, b->d_Id, b->d_vert, b->d_iBD, op->d_kpos, op->d_kstride, op->d_K64,
               op->d_K32);
+    op->rows_stale = 1;
     return FG_OK;
 }
 

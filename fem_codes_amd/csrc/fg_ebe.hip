@@ -442,6 +442,7 @@ int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
 {
     if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, fp32))
         return fg_ede_rows_apply(op, fp32, P, Q, nullptr);
+    if (fg_ebe_rows_used(op)) return fg_ebe_rows_apply(op, fp32, P, Q, nullptr);
     if (op->nblk == 0) return FG_OK;
     if (fp32) {
         switch (op->k) {
@@ -469,11 +470,13 @@ int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
 int fg_ebe_apply_f64(fg_op *op, double *P, double *Q, const int *skip)
 {
     if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, 0)) return fg_ede_rows_apply(op, 0, P, Q, skip);
+    if (fg_ebe_rows_used(op)) return fg_ebe_rows_apply(op, 0, P, Q, skip);
     return ebe_apply<double>(op, op->d_K64, P, Q, op->d_hpart64, skip);
 }
 int fg_ebe_apply_f32(fg_op *op, float *P, float *Q, const int *skip)
 {
     if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, 1)) return fg_ede_rows_apply(op, 1, P, Q, skip);
+    if (fg_ebe_rows_used(op)) return fg_ebe_rows_apply(op, 1, P, Q, skip);
     return ebe_apply<float>(op, op->d_K32, P, Q, op->d_hpart32, skip);
 }
 
@@ -645,6 +648,7 @@ static int ebe_create(fg_ctx *c, int neq, int nitems, int k, const int *lm, int 
     FG_CHECK(upload(ioff, &op->d_inc_off));
     FG_CHECK(upload(idiag, &op->d_inc_diag));
     if (kind == FG_OP_EDE) FG_CHECK(fg_ede_rows_build(op, lm, pl.kpos.data()));
+    else FG_CHECK(fg_ebe_rows_build(op, lm, pl.kpos.data()));
     if (small) {
         std::vector<int> lmT((size_t)k * nitems);
         for (int e = 0; e < nitems; e++)
@@ -691,7 +695,8 @@ int fg_ebe_set_values_dev(fg_op *op, const double *A)
 
 void fg_ebe_free(fg_op *op)
 {
-    fg_ede_rows_free(op);
+    if (op->kind == FG_OP_EDE) fg_ede_rows_free(op);
+    else fg_ebe_rows_free(op);
     hipFree(op->d_eoff); hipFree(op->d_doff); hipFree(op->d_dofs);
     hipFree(op->d_ibase); hipFree(op->d_ioff); hipFree(op->d_inc);
     hipFree(op->d_slots); hipFree(op->d_lml); hipFree(op->d_K64); hipFree(op->d_K32);

@@ -1,0 +1,286 @@
+// fg_ebe_rows.hip -- the element-by-element product as element rows grouped
+// by equation (k = 3 triangles, k = 4 tetrahedra).
+//
+// Reference: MatrixVector_Operations/EBE/ebemv.c:24-40 (k = 3) and
+// EBE/ebemv3Dndof1.c:24-42 (k = 4).  The reference walks the elements in
+// order and, for element I with dofs i_1..i_k and row-major values A[I],
+//     Q[i_r] += A[I][k r] * p_1 + A[I][k r + 1] * p_2 + ... + A[I][k r + k-1] * p_k
+// (the row sum formed left to right, then added to Q), after Q = 0 and
+// P[neq] = 0.  Equation r therefore receives one row sum per element that
+// holds it, in element order.
+//
+// MI355X design: every (element, local row) pair whose dof is free is stored
+// with its equation, in element order, in wavefront slices (fg_sell.h): 64
+// equations per slice, one per lane.  An entry holds the row's k values and
+// the other k-1 columns as offsets from the equation (15-bit signed fields,
+// two in one 32-bit word with the local row index; for k = 4 the third in a
+// 16-bit word), so the index stream costs 4 (k = 3) or 6 (k = 4) bytes per
+// row, against the reference's 4k bytes of lm per element -- the same
+// compulsory bytes for k = 3.  A lane loads its equation's P once, streams
+// its rows (all loads of a chunk issued at once), gathers the other columns'
+// P, forms each row sum left to right exactly as the reference does, and
+// adds the sums in element order starting from 0: Q is the reference's ebemv
+// bit for bit (FMA contraction disabled).  No LDS, no halo pass, no atomics.
+//
+// The values of a slice are stored contiguously (column c of the slice's
+// entry k at ((k_dofs*o0 + c*w + k)*64 + lane) for a slice of width w
+// starting at slot column o0).  The blocked layout (fg_ebe.hip) stays the
+// primary copy of the values; the rows are refreshed from it by one kernel
+// before the first product after any change.  A mesh whose column offsets do
+// not fit the fields keeps the blocked product only.
+#include <vector>
+
+#include "fg_ebe_body.h"
+#include "fg_internal.h"
+#include "fg_sell.h"
+
+namespace {
+
+// A padding slot has both 15-bit fields 0 (and, k = 4, the 16-bit field 0):
+// an element's other columns never equal its own (checked when the rows are
+// built), so no row has that word, and the padding's gathers read the lane's
+// own P, in bounds.
+constexpr unsigned ROW_PAD = 0u;
+constexpr int OFF_SINK = -16384;            // 15-bit field: the Dirichlet sink
+constexpr int OFF_MAX = 16383;
+constexpr int OFF2_SINK = -32768;           // 16-bit field (k = 4, third column)
+
+__device__ __forceinline__ int field_a(unsigned w) { return ((int)(w << 15)) >> 17; }
+__device__ __forceinline__ int field_b(unsigned w) { return ((int)w) >> 17; }
+
+// one element row of an equation's sliced list
+template <typename TV, int K>
+struct ElemRowSlot {
+    using T = TV;
+    struct Rec {
+        unsigned w;
+        int c2;
+        T v[K];
+        T po[K - 1];   // P of the other columns, in column order
+    };
+    const unsigned *__restrict__ idx;
+    const short *__restrict__ idx2;   // k = 4: the third other column
+    const T *__restrict__ V;
+    const T *__restrict__ P;
+    long long vshift;   // value column 0 of slot q at q + vshift
+    long long wstride;  // between value columns: w * 64
+    int r;              // the lane's equation (clamped to the sink for idle lanes)
+    T pown;
+
+    __device__ __forceinline__ Rec load(long long q) const
+    {
+        Rec e;
+        e.w = __builtin_nontemporal_load(&idx[q]);
+        e.c2 = 0;
+        if constexpr (K == 4) e.c2 = (int)__builtin_nontemporal_load(&idx2[q]);
+#pragma unroll
+        for (int c = 0; c < K; c++)
+            e.v[c] = __builtin_nontemporal_load(&V[q + vshift + c * wstride]);
+#pragma unroll
+        for (int c = 0; c < K - 1; c++) e.po[c] = T(0);
+        return e;
+    }
+    __device__ __forceinline__ T col_p(int off, int sink) const
+    {
+        const T t = P[off == sink ? r : r + off];
+        return off == sink ? T(0) : t;
+    }
+    __device__ __forceinline__ void gather(Rec &e) const
+    {
+        e.po[0] = col_p(field_a(e.w), OFF_SINK);
+        e.po[1] = col_p(field_b(e.w), OFF_SINK);
+        if constexpr (K == 4) e.po[K - 2] = col_p(e.c2, OFF2_SINK);
+    }
+    __device__ __forceinline__ T term(const Rec &e) const
+    {
+        const int lr = (int)(e.w & 3u);
+        T s = T(0);
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            // column c: the equation itself at c == lr, else the next other column
+            const T oth = c == 0 ? e.po[0] : (c == K - 1 ? e.po[K - 2] : (c < lr ? e.po[c] : e.po[c - 1]));
+            const T pc = c == lr ? pown : oth;
+            s = c == 0 ? e.v[0] * pc : s + e.v[c] * pc;
+        }
+        return e.w == ROW_PAD ? T(0) : s;
+    }
+};
+
+// four slices (one per wavefront) per workgroup
+template <typename T, int K>
+__global__ __launch_bounds__(FG_TPB) void k_ebe_rows(int nslice, const int *__restrict__ soff,
+                                                     const unsigned *__restrict__ idx,
+                                                     const short *__restrict__ idx2,
+                                                     const T *__restrict__ V,
+                                                     T *__restrict__ P, T *__restrict__ Q,
+                                                     int neq, const int *skip)
+{
+    if (skip && *skip) return;
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
+                  (int)(threadIdx.x / FG_SELL_W);
+    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    if (s < nslice) {
+        const int r = s * FG_SELL_W + lane;
+        // lanes past the last equation use the sink; all their slots are padding
+        const int rc = r < neq ? r : neq;
+        const T pown = P[rc];
+        const int o0 = soff[s], w = soff[s + 1] - o0;
+        const ElemRowSlot<T, K> row{idx, idx2, V, P, (long long)(K - 1) * o0 * FG_SELL_W,
+                                    (long long)w * FG_SELL_W, rc, pown};
+        const T acc = fg_sell_row(row, o0, w, lane);
+        if (r < neq) Q[r] = acc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Q[neq] = T(0);
+        P[neq] = T(0);
+    }
+}
+
+// the rows' values from the blocked values; one thread per (slice, lane)
+template <int K>
+__global__ void k_ebe_rows_mirror(int nslice, const int *__restrict__ soff,
+                                  const int *__restrict__ src, const long long *__restrict__ kpos,
+                                  const int *__restrict__ kstride, const double *__restrict__ K64,
+                                  const float *__restrict__ K32, double *__restrict__ V64,
+                                  float *__restrict__ V32)
+{
+    const long long nt = (long long)nslice * FG_SELL_W;
+    for (long long t = (long long)blockIdx.x * FG_TPB + threadIdx.x; t < nt;
+         t += (long long)gridDim.x * FG_TPB) {
+        const int s = (int)(t / FG_SELL_W), lane = (int)(t % FG_SELL_W);
+        const int o0 = soff[s], w = soff[s + 1] - o0;
+        for (int k = 0; k < w; k++) {
+            const long long q = (long long)(o0 + k) * FG_SELL_W + lane;
+            const int sv = src[q];
+            if (sv < 0) continue;
+            const int e = sv / K, lr = sv - e * K;
+            const long long p = kpos[e];
+            const int st = kstride[e];
+#pragma unroll
+            for (int c = 0; c < K; c++) {
+                const long long vi = fg_ebe_vidx(p, st, lr * K + c, fg_ebe_vec(K));
+                const long long d = ((long long)K * o0 + (long long)c * w + k) * FG_SELL_W + lane;
+                V64[d] = K64[vi];
+                V32[d] = K32[vi];
+            }
+        }
+    }
+}
+
+template <typename T>
+int upload(const std::vector<T> &v, T **d)
+{
+    const size_t n = v.size() ? v.size() : 1;
+    FG_HIP(hipMalloc(d, n * sizeof(T)));
+    if (!v.empty()) FG_HIP(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FG_OK;
+}
+
+}  // namespace
+
+// rows of an element operator with k = 3 or 4; leaves the operator without
+// rows (blocked product only) when a column offset does not fit its field
+int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
+{
+    const int neq = op->neq, nel = op->nitems, K = op->k;
+    if (op->kind != FG_OP_EBE || (K != 3 && K != 4) || neq == 0) return FG_OK;
+    std::vector<int> cnt((size_t)neq + 1, 0);
+    for (int e = 0; e < nel; e++) {
+        if (kpos[e] < 0) continue;
+        for (int a = 0; a < K; a++) {
+            const int r = lm[(size_t)e * K + a];
+            if (r != neq) cnt[r + 1]++;
+        }
+    }
+    for (int r = 0; r < neq; r++) cnt[r + 1] += cnt[r];
+    const fg_sell_layout L = fg_sell_plan(neq, cnt.data());
+    std::vector<unsigned> idx(L.nslot, ROW_PAD);
+    std::vector<short> idx2(K == 4 ? L.nslot : 0, 0);
+    std::vector<int> src(L.nslot, -1);
+    std::vector<int> fill((size_t)neq, 0);
+    for (int e = 0; e < nel; e++) {
+        if (kpos[e] < 0) continue;
+        const int *le = lm + (size_t)e * K;
+        for (int a = 0; a < K; a++) {
+            const int r = le[a];
+            if (r == neq) continue;
+            int off[3] = {0, 0, 0}, m = 0;
+            for (int c = 0; c < K; c++) {
+                if (c == a) continue;
+                const long long d = (long long)le[c] - r;
+                const int lim = m == 2 ? 32767 : OFF_MAX;
+                // an offset beyond its field, or a repeated dof: no rows
+                if (le[c] != neq && (d == 0 || d < -lim || d > lim)) return FG_OK;
+                off[m] = le[c] == neq ? (m == 2 ? OFF2_SINK : OFF_SINK) : (int)d;
+                m++;
+            }
+            const long long q = L.slot(r, fill[r]++);
+            idx[q] = (unsigned)a | ((unsigned)(off[0] & 0x7fff) << 2) |
+                     ((unsigned)(off[1] & 0x7fff) << 17);
+            if (K == 4) idx2[q] = (short)off[2];
+            src[q] = e * K + a;
+        }
+    }
+    op->nslice = L.nslice;
+    op->nslot = L.nslot;
+    FG_CHECK(upload(L.soff, &op->d_soff));
+    FG_CHECK(upload(idx, &op->d_ridx));
+    if (K == 4) FG_CHECK(upload(idx2, &op->d_ridx2));
+    FG_CHECK(upload(src, &op->d_rsrc));
+    const size_t nv = (size_t)K * (L.nslot ? L.nslot : 1);
+    FG_HIP(hipMalloc(&op->d_sv64, sizeof(double) * nv));
+    FG_HIP(hipMalloc(&op->d_sv32, sizeof(float) * nv));
+    FG_HIP(hipMemset(op->d_sv64, 0, sizeof(double) * nv));
+    FG_HIP(hipMemset(op->d_sv32, 0, sizeof(float) * nv));
+    op->rows_stale = 1;
+    return FG_OK;
+}
+
+int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
+{
+    fg_ctx *c = op->ctx;
+    const int K = op->k;
+    if (op->rows_stale) {
+        const long long nt = (long long)op->nslice * FG_SELL_W;
+        if (K == 3)
+            FG_LAUNCH(k_ebe_rows_mirror<3>, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
+                      op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
+                      op->d_sv64, op->d_sv32);
+        else
+            FG_LAUNCH(k_ebe_rows_mirror<4>, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
+                      op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
+                      op->d_sv64, op->d_sv32);
+        op->rows_stale = 0;
+    }
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int g = (op->nslice + SPW - 1) / SPW;
+    if (fp32) {
+        if (K == 3)
+            FG_LAUNCH((k_ebe_rows<float, 3>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                      op->d_ridx, op->d_ridx2, (const float *)op->d_sv32, (float *)P,
+                      (float *)Q, op->neq, skip);
+        else
+            FG_LAUNCH((k_ebe_rows<float, 4>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                      op->d_ridx, op->d_ridx2, (const float *)op->d_sv32, (float *)P,
+                      (float *)Q, op->neq, skip);
+    } else {
+        if (K == 3)
+            FG_LAUNCH((k_ebe_rows<double, 3>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                      op->d_ridx, op->d_ridx2, (const double *)op->d_sv64, (double *)P,
+                      (double *)Q, op->neq, skip);
+        else
+            FG_LAUNCH((k_ebe_rows<double, 4>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                      op->d_ridx, op->d_ridx2, (const double *)op->d_sv64, (double *)P,
+                      (double *)Q, op->neq, skip);
+    }
+    return FG_OK;
+}
+
+void fg_ebe_rows_free(fg_op *op)
+{
+    hipFree(op->d_ridx); hipFree(op->d_ridx2); hipFree(op->d_rsrc);
+    if (op->kind == FG_OP_EBE) {
+        hipFree(op->d_soff); hipFree(op->d_sv64); hipFree(op->d_sv32);
+    }
+}

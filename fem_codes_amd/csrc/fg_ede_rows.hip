@@ -172,7 +172,6 @@ int fg_ede_rows_build(fg_op *op, const int *lm2, const long long *kpos)
     FG_HIP(hipMalloc(&op->d_sv32, sizeof(float) * nv));
     FG_HIP(hipMemset(op->d_sv64, 0, sizeof(double) * nv));
     FG_HIP(hipMemset(op->d_sv32, 0, sizeof(float) * nv));
-    op->edge_rows = FG_EDGE_AUTO;
     op->rows_stale = 1;
     return FG_OK;
 }

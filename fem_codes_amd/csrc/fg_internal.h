@@ -146,6 +146,11 @@ struct fg_op {
     long long *d_spos = nullptr; // CSR: slot of each entry (null: row-block product)
     double *d_sv64 = nullptr;
     float *d_sv32 = nullptr;
+    // EBE k = 3, 4 element rows (fg_ebe_rows.hip): packed column offsets,
+    // the third offset (k = 4), source element * k + row (-1: padding)
+    unsigned *d_ridx = nullptr;
+    short *d_ridx2 = nullptr;
+    int *d_rsrc = nullptr;
 
     // ---- systems of at most FG_SMALL_MAX equations: lane-coalesced lists for
     // the one-workgroup solver, entry c of row g at c*neq + g (-1: none) ----
@@ -155,7 +160,7 @@ struct fg_op {
     int *d_sm_lmT = nullptr;     // EBE/EDE: lm transposed, [k][nitems]
     // EDE (k = 2): half-edge rows (fg_ede_rows.hip), a second copy of the
     // edge values ordered by equation, kept in step with the blocked copy
-    int edge_rows = 0;           // FG_EDGE_ROWS / _BLOCKED / _AUTO (rows)
+    int rows_kind = FG_PRODUCT_AUTO; // FG_PRODUCT_SLICED / _BLOCKED / _AUTO (sliced rows when built)
     int rows_stale = 1;          // the rows' values lag the blocked values
     // (sliced: d_soff / d_scol / d_sv64 / d_sv32 below the CSR fields)
     long long nhalf = 0;         // half-edges (rows' entries)
@@ -293,7 +298,15 @@ void fg_ede_rows_free(fg_op *op);
 inline bool fg_ede_rows_used(const fg_op *op, int fp32)
 {
     (void)fp32;
-    return op->d_hsrc && op->edge_rows != FG_EDGE_BLOCKED;
+    return op->d_hsrc && op->rows_kind != FG_PRODUCT_BLOCKED;
+}
+int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos);
+int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
+void fg_ebe_rows_free(fg_op *op);
+// EBE k = 3, 4: the element-row product in use
+inline bool fg_ebe_rows_used(const fg_op *op)
+{
+    return op->kind == FG_OP_EBE && op->d_ridx && op->rows_kind != FG_PRODUCT_BLOCKED;
 }
 int fg_prec_apply_any(fg_prec *pc, int fp32, const void *p, void *z, const int *skip = nullptr);
 int fg_vec_grid(fg_ctx *ctx, long long n);

@@ -155,6 +155,25 @@ int fg_op_set_row_kernel(fg_op *op, int kind);
 #define FG_EDGE_BLOCKED 1
 #define FG_EDGE_AUTO    2
 int fg_op_set_edge_kernel(fg_op *op, int kind);
+/* Element / edge product kernel (no reference counterpart: a kernel choice).
+ * FG_PRODUCT_SLICED: the sliced rows -- for EBE with k = 3 (ebemv.c:24-40)
+ * and k = 4 (ebemv3Dndof1.c:24-42), every element row stored with its
+ * equation in element order, 64 equations per wavefront slice, each equation
+ * adding its element row sums (formed left to right as the reference does)
+ * in element order: the reference's product bit for bit (FP64); for EDE the
+ * half-edge rows of FG_EDGE_ROWS.  FG_PRODUCT_BLOCKED: the blocked element
+ * kernel (1e-14).  FG_PRODUCT_AUTO (default): sliced rows where the operator
+ * has them.  An element operator gets sliced rows when k is 3 or 4 and every
+ * element's other columns lie within 16383 (k = 4: the third within 32767)
+ * of its own equation; else it keeps the blocked kernel only and
+ * FG_PRODUCT_SLICED is rejected.  fg_op_product_kernel reports whether the
+ * product in use is a sliced-row kernel (CSR: the sliced rows, unless the
+ * pattern would carry more than a quarter of padding). */
+#define FG_PRODUCT_SLICED  0
+#define FG_PRODUCT_BLOCKED 1
+#define FG_PRODUCT_AUTO    2
+int fg_op_set_product_kernel(fg_op *op, int kind);
+int fg_op_product_kernel(const fg_op *op, int *sliced);
 
 /* ---- preconditioners ------------------------------------------------------ */
 #define FG_PREC_NONE    0   /* NO_precond.c / NO_precond_setup.c            */

@@ -73,13 +73,20 @@ def test_csr_k4_structures_and_assembly(ctx, path):
 def test_ebe_k4_matvec(ctx, path):
     d = load_golden(path)
     neq = int(d["neq"])
+    # sliced rows (the default): ebemv3Dndof1 bit for bit
+    op = F.Operator.ebe(ctx, neq, d["lm"], 4)
+    op.set_values(d["ke"])
+    assert op.product_sliced()
+    assert np.array_equal(op.matvec(d["P"])[:neq], d["Q"][:neq])
+    op.close()
     for epb in (0, 256):
-        op = F.Operator.ebe(ctx, neq, d["lm"], 4, epb)
-        op.set_values(d["ke"])
-        assert _rel(op.matvec(d["P"])[:neq], d["Q"][:neq]) <= 1e-14
-        Q32 = op.matvec(d["P"].astype(np.float32), np.float32).astype(np.float64)
-        assert _rel(Q32[:neq], d["Q"][:neq]) <= 1e-5
-        op.close()
+        for product in (F.PRODUCT_SLICED, F.PRODUCT_BLOCKED):
+            op = F.Operator.ebe(ctx, neq, d["lm"], 4, epb, product=product)
+            op.set_values(d["ke"])
+            assert _rel(op.matvec(d["P"])[:neq], d["Q"][:neq]) <= 1e-14
+            Q32 = op.matvec(d["P"].astype(np.float32), np.float32).astype(np.float64)
+            assert _rel(Q32[:neq], d["Q"][:neq]) <= 1e-5
+            op.close()
 
 
 @pytest.mark.parametrize("path", CASES, ids=os.path.basename)

@@ -33,7 +33,7 @@ def system():
 @pytest.mark.parametrize("epb", [256, 512, 1024])
 def test_ebe_pipeline_identical(system, epb):
     ctx, neq, nel, lm, ke, P = system
-    op = F.Operator.ebe(ctx, neq, lm, 3, epb)
+    op = F.Operator.ebe(ctx, neq, lm, 3, epb, product=F.PRODUCT_BLOCKED)
     op.set_values(ke)
     ref64 = op.matvec(P)
     ref32 = op.matvec(P.astype(np.float32), np.float32)
@@ -43,6 +43,75 @@ def test_ebe_pipeline_identical(system, epb):
         assert np.array_equal(op.matvec(P.astype(np.float32), np.float32), ref32)
     Qo = O.Oracle().ebemv(neq, nel, 3, ke.get(), lm, P.copy())
     assert np.max(np.abs(ref64[:neq] - Qo[:neq])) <= 1e-14 * np.max(np.abs(Qo[:neq]))
+    op.close()
+
+
+def _ebemv_f32(neq, lm, ke, P):
+    """ebemv.c:24-40 in float32: each element row summed left to right, the
+    sums added to Q in element order (np.add.at applies them in index order)"""
+    A = np.asarray(ke, np.float32).reshape(-1, 3, 3)
+    L = lm.reshape(-1, 3)
+    p = np.asarray(P, np.float32).copy()
+    p[neq] = 0
+    pe = p[L]
+    s = (A[:, :, 0] * pe[:, None, 0] + A[:, :, 1] * pe[:, None, 1]) + A[:, :, 2] * pe[:, None, 2]
+    Q = np.zeros(neq + 1, np.float32)
+    np.add.at(Q, L.ravel(), s.ravel())
+    Q[neq] = 0
+    return Q
+
+
+def test_ebe_sliced_rows_bitwise(system):
+    """the default element product (sliced rows) is the reference's ebemv bit
+    for bit, in FP64 (oracle) and FP32 (float32 restatement)"""
+    ctx, neq, nel, lm, ke, P = system
+    op = F.Operator.ebe(ctx, neq, lm, 3)
+    op.set_values(ke)
+    assert op.product_sliced()
+    keh = ke.get()
+    Qo = O.Oracle().ebemv(neq, nel, 3, keh, lm, P.copy())
+    assert np.array_equal(op.matvec(P), Qo)
+    P32 = P.astype(np.float32)
+    assert np.array_equal(op.matvec(P32, np.float32), _ebemv_f32(neq, lm, keh, P32))
+    op.close()
+
+
+def test_ebe_sliced_rows_follow_value_changes(system):
+    """new values and a Left/LeftRight scaling reach the row copy before the
+    next product"""
+    ctx, neq, nel, lm, ke, P = system
+    op = F.Operator.ebe(ctx, neq, lm, 3)
+    op.set_values(ke)
+    op.matvec(P)
+    vals = np.random.default_rng(4).uniform(0.5, 1.5, 9 * nel)
+    op.set_values(vals)
+    assert np.array_equal(op.matvec(P), O.Oracle().ebemv(neq, nel, 3, vals, lm, P.copy()))
+    pc = F.Preconditioner(op, F.PREC_DIAG)
+    pc.scaling(F.SCALING_LEFTRIGHT, ctx.to_device(np.ones(neq + 1)))
+    scaled = op.get_values()
+    assert not np.array_equal(scaled, vals)
+    assert np.array_equal(op.matvec(P), O.Oracle().ebemv(neq, nel, 3, scaled, lm, P.copy()))
+    pc.close()
+    op.close()
+
+
+def test_ebe_without_sliced_rows(system):
+    """a numbering whose column offsets exceed the row fields keeps the
+    blocked product: sliced rows absent and refused, products still right"""
+    ctx, neq, nel, lm, ke, P = system
+    perm = np.random.default_rng(2).permutation(neq)
+    relabel = np.append(perm, neq).astype(np.int32)
+    lmx = relabel[lm]
+    Px = np.empty_like(P)
+    Px[relabel[:neq]] = P[:neq]
+    Px[neq] = 0.0
+    op = F.Operator.ebe(ctx, neq, lmx, 3)
+    op.set_values(ke)
+    assert not op.product_sliced()
+    with pytest.raises(F.FemGpuError):
+        op.set_product_kernel(F.PRODUCT_SLICED)
+    Qo = O.Oracle().ebemv(neq, nel, 3, ke.get(), lmx, Px.copy())
+    assert np.max(np.abs(op.matvec(Px)[:neq] - Qo[:neq])) <= 1e-14 * np.max(np.abs(Qo[:neq]))
     op.close()
 
 
@@ -68,7 +137,8 @@ def test_ebe_mesh_ordering(system, epb):
     perm = np.random.default_rng(5).permutation(nel)
     for lmx, kex in ((lm, keh), (lm.reshape(nel, 3)[perm].ravel(), keh[perm])):
         for order in (F.ORDER_BFS, F.ORDER_MESH):
-            op = F.Operator.ebe(ctx, neq, lmx, 3, epb, ordering=order)
+            op = F.Operator.ebe(ctx, neq, lmx, 3, epb, ordering=order,
+                                product=F.PRODUCT_BLOCKED)
             op.set_values(np.ascontiguousarray(kex).ravel())
             for wpc in (0, 2):
                 op.set_pipeline(wpc)
