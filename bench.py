@@ -99,7 +99,8 @@ def leg_traffic(leg, kernel):
         return None
     with open(path) as f:
         tj = json.load(f)
-    return tj.get("hbm_bytes_per_launch") if tj.get("kernel") == kernel else None
+    rec = tj.get("kernels", {}).get(kernel)
+    return rec.get("hbm_bytes_per_launch") if rec else None
 
 
 def ebe_kernel_name(k, fp32, op=None):
@@ -294,6 +295,17 @@ def headline_leg(F, ctx, args, rng):
     for name, op in list(sysd["ops"].items()) + extra:
         matvec[name] = matvec_table(op, P64, Q64, P32, Q32, args.steps)
         log(name, {k: round(v["GBs"], 1) for k, v in matvec[name].items()})
+        if name in ("EBE", "EDE"):
+            # the other product kernel of the same operator, for comparison
+            sliced = op.product_sliced()
+            try:
+                op.set_product_kernel(F.PRODUCT_BLOCKED if sliced else F.PRODUCT_SLICED)
+            except F.FemGpuError:
+                continue           # no sliced rows on this mesh
+            alt = name + ("_blocked" if sliced else "_sliced")
+            matvec[alt] = matvec_table(op, P64, Q64, P32, Q32, args.steps)
+            log(alt, {k: round(v["GBs"], 1) for k, v in matvec[alt].items()})
+            op.set_product_kernel(F.PRODUCT_AUTO if sliced else F.PRODUCT_BLOCKED)
     if ebe_tuned is not default_op:
         default_op.close()
     res = {"value": value, "ms": ms, "wall_ms_per_step": wall * 1e3 / args.steps,

@@ -186,6 +186,16 @@ __global__ __launch_bounds__(FG_TPB) void k_b_cycle_start(Chunks C, BState *st, 
     s->e0 = e0;
     s->rho = e0;
     s->scale = 1.0 / e0;
+    rec[p].e0 = e0;
+    if (!(e0 <= 1.0e300)) {
+        // a non-finite residual (NaN or Inf in B, A or X): breakdown, as the
+        // single-system solver reports it
+        s->fin = FIN_BREAKDOWN;
+        s->done = 1;
+        rec[p].fin = s->fin;
+        rec[p].steps = s->steps;
+        return;
+    }
     double eps = s->eps_final;
     if (inner_tol > 0.0 && inner_tol * e0 > eps) eps = inner_tol * e0;
     s->eps = eps;

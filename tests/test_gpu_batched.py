@@ -162,3 +162,20 @@ def test_batched_cgs_needs_fp32_ir(ctx):
     Bd, X = ctx.to_device(B), ctx.empty(len(B))
     with pytest.raises(F.FemGpuError):
         F.gmres_batched(ctx, op, None, seg, Bd, X, precision=F.GMRES_FP64, orth=F.ORTH_CGS)
+
+
+@pytest.mark.parametrize("precision", [F.GMRES_FP64, F.GMRES_FP32_IR])
+def test_batched_nonfinite_rhs_is_breakdown(ctx, precision):
+    """a NaN in one problem's right-hand side ends that problem as a
+    breakdown (never as converged); the other problems solve as usual"""
+    ds, op, seg, B = _batch(ctx, EBE)
+    B[seg[1] + 3] = np.nan
+    Bd, X = ctx.to_device(B), ctx.empty(len(B))
+    rc, infos = F.gmres_batched(ctx, op, None, seg, Bd, X, kmax=30, lmax=500, tol=1e-8,
+                                precision=precision)
+    assert rc == F.FG_ERR_BREAKDOWN
+    assert infos[1].status == F.FG_ERR_BREAKDOWN
+    for i in range(len(ds)):
+        if i != 1:
+            assert infos[i].status == F.FG_OK
+            assert infos[i].true_res <= infos[i].eps * (1 + 1e-6)
