@@ -733,7 +733,8 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
     rng = np.random.default_rng(1)
     res = {"n": n, "elements": nel, "equations": neq, "nnz": int(nnz),
            "bandwidth_random": int(bw0), "bandwidth_rcm": int(bw1), "ilu_levels": levels[0],
-           "ilu_sweeps": sweeps, "kmax": 50, "tol": 1e-10, "structure_seconds": t_struct}
+           "ilu_sweeps": sweeps, "kmax": 50, "tol": 1e-10, "structure_seconds": t_struct,
+           "fp32_ir_orth": args.orth}
     mv = {}
     P64 = ctx.to_device(rng.standard_normal(neq + 1))
     Q64 = ctx.empty(neq + 1)
@@ -776,7 +777,8 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
         X = ctx.empty(neq + 1)
         ctx.sync()
         w0 = time.perf_counter()
-        info, _, _ = F.gmres(ctx, ebe, pc, B, X, kmax=50, lmax=400, tol=1e-10, precision=prec)
+        info, _, _ = F.gmres(ctx, ebe, pc, B, X, kmax=50, lmax=400, tol=1e-10, precision=prec,
+                             orth=ORTHS[args.orth] if prec == F.GMRES_FP32_IR else F.ORTH_MGS)
         ctx.sync()
         Xs[tag] = X.get()
         res[tag] = solve_record(info, time.perf_counter() - w0, 1e-10)
@@ -966,6 +968,7 @@ def euler_leg(F, ctx, args, n=1024, seed=20260305):
                                "k_ebe_apply_mfma12<float, 0>" if trial["kept"] == "mfma"
                                else ebe_kernel_name(12, True))
     res["roofline"]["traffic"] = leg_traffic("euler", res["roofline"]["kernel"])
+    res["fp32_ir_orth"] = args.orth
     Xs = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         B = ctx.to_device(Rh)
@@ -973,7 +976,8 @@ def euler_leg(F, ctx, args, n=1024, seed=20260305):
         X = ctx.empty(neq + 1)
         ctx.sync()
         w0 = time.perf_counter()
-        info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=30, lmax=2000, tol=1e-8, precision=prec)
+        info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=30, lmax=2000, tol=1e-8, precision=prec,
+                             orth=ORTHS[args.orth] if prec == F.GMRES_FP32_IR else F.ORTH_MGS)
         ctx.sync()
         Xs[tag] = X.get()
         res[tag] = solve_record(info, time.perf_counter() - w0, 1e-8)

@@ -243,10 +243,11 @@ def test_128cube_rcm_ilu0_and_solves(ctx):
     Bh = B.get()
     i64, X64 = _solve(ctx, ebe, None, Bh, neq, F.GMRES_FP64, kmax=50, tol=1e-10, lmax=400, pc=pc)
     assert i64.status == F.FG_OK and _true_rel(i64, 1e-10) <= 1e-10
-    i32, X32 = _solve(ctx, ebe, None, Bh, neq, F.GMRES_FP32_IR, kmax=50, tol=1e-10, lmax=400,
-                      pc=pc)
-    assert i32.status == F.FG_OK and _true_rel(i32, 1e-10) <= 1.05e-10
-    assert _nrel(X32[:neq], X64[:neq]) <= 1e-6
+    for orth in (F.ORTH_MGS, F.ORTH_CGS):
+        i32, X32 = _solve(ctx, ebe, None, Bh, neq, F.GMRES_FP32_IR, kmax=50, tol=1e-10,
+                          lmax=400, pc=pc, orth=orth)
+        assert i32.status == F.FG_OK and _true_rel(i32, 1e-10) <= 1.05e-10
+        assert _nrel(X32[:neq], X64[:neq]) <= 1e-6
     for obj in (pc, pcx, csr, ebe, m):
         obj.close()
 
@@ -343,9 +344,10 @@ def test_euler_1024_element_and_edge_schemes(ctx):
     Bh = B.get()
     i64, X64 = _solve(ctx, op, None, Bh, neq, F.GMRES_FP64, pc=pc, lmax=2000)
     assert i64.status == F.FG_OK and _true_rel(i64, 1e-8) <= 1e-8
-    i32, X32 = _solve(ctx, op, None, Bh, neq, F.GMRES_FP32_IR, pc=pc, lmax=2000)
-    assert i32.status == F.FG_OK and _true_rel(i32, 1e-8) <= 1.05e-8
-    assert _nrel(X32[:neq], X64[:neq]) <= 1e-6
+    for orth in (F.ORTH_MGS, F.ORTH_CGS):
+        i32, X32 = _solve(ctx, op, None, Bh, neq, F.GMRES_FP32_IR, pc=pc, lmax=2000, orth=orth)
+        assert i32.status == F.FG_OK and _true_rel(i32, 1e-8) <= 1.05e-8
+        assert _nrel(X32[:neq], X64[:neq]) <= 1e-6
     pc8 = F.Preconditioner.blockdiag(op8, nn, en, ids32)
     B8 = ctx.to_device(Rh)
     pc8.setup(B8)
