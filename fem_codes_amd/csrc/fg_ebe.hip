@@ -686,6 +686,7 @@ int fg_ebe_set_values_dev(fg_op *op, const double *A)
     fg_ctx *c = op->ctx;
     long long n = (long long)op->nitems * op->k * op->k;
     if (n == 0) return FG_OK;
+    if (op->kind == FG_OP_EBE && op->d_rvpos) return fg_ebe_rows_set_values(op, A);
     int g = fg_vec_grid(c, n);
     FG_LAUNCH(k_ebe_scatter_values, g, FG_TPB, 0, c->stream, n, op->k * op->k, A, op->d_kpos,
               op->d_kstride, op->d_K64, op->d_K32);

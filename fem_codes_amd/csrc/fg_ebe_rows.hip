@@ -168,6 +168,47 @@ __global__ void k_ebe_rows_mirror(int nslice, const int *__restrict__ soff,
     }
 }
 
+// AoS element values -> the blocked layout and the rows in one pass; one
+// thread per (element, local row): the row's k values are read contiguously
+// (consecutive threads, consecutive rows), written to their blocked positions
+// and, when the row's dof is free, to its slot (value c at vpos + c * wst)
+template <int K>
+__global__ void k_ebe_scatter_rows(long long n, const double *__restrict__ A,
+                                   const long long *__restrict__ kpos,
+                                   const int *__restrict__ kstride,
+                                   const long long *__restrict__ vpos,
+                                   const int *__restrict__ wst, double *__restrict__ K64,
+                                   float *__restrict__ K32, double *__restrict__ V64,
+                                   float *__restrict__ V32)
+{
+    for (long long t = (long long)blockIdx.x * FG_TPB + threadIdx.x; t < n;
+         t += (long long)gridDim.x * FG_TPB) {
+        const long long e = t / K;
+        const int lr = (int)(t - e * K);
+        const long long p = kpos[e];
+        if (p < 0) continue;
+        const int st = kstride[e];
+        const long long vp = vpos[t];
+        const int ws = wst[t];
+        double v[K];
+#pragma unroll
+        for (int c = 0; c < K; c++) v[c] = A[t * K + c];
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            const long long d = fg_ebe_vidx(p, st, lr * K + c, fg_ebe_vec(K));
+            K64[d] = v[c];
+            K32[d] = (float)v[c];
+        }
+        if (vp >= 0) {
+#pragma unroll
+            for (int c = 0; c < K; c++) {
+                V64[vp + (long long)c * ws] = v[c];
+                V32[vp + (long long)c * ws] = (float)v[c];
+            }
+        }
+    }
+}
+
 template <typename T>
 int upload(const std::vector<T> &v, T **d)
 {
@@ -198,6 +239,8 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
     std::vector<unsigned> idx(L.nslot, ROW_PAD);
     std::vector<short> idx2(K == 4 ? L.nslot : 0, 0);
     std::vector<int> src(L.nslot, -1);
+    std::vector<long long> vpos((size_t)nel * K, -1);
+    std::vector<int> wst((size_t)nel * K, 0);
     std::vector<int> fill((size_t)neq, 0);
     for (int e = 0; e < nel; e++) {
         if (kpos[e] < 0) continue;
@@ -220,6 +263,9 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
                      ((unsigned)(off[1] & 0x7fff) << 17);
             if (K == 4) idx2[q] = (short)off[2];
             src[q] = e * K + a;
+            const int s0 = r / FG_SELL_W;
+            vpos[(size_t)e * K + a] = q + (long long)(K - 1) * L.soff[s0] * FG_SELL_W;
+            wst[(size_t)e * K + a] = (L.soff[s0 + 1] - L.soff[s0]) * FG_SELL_W;
         }
     }
     op->nslice = L.nslice;
@@ -228,12 +274,31 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
     FG_CHECK(upload(idx, &op->d_ridx));
     if (K == 4) FG_CHECK(upload(idx2, &op->d_ridx2));
     FG_CHECK(upload(src, &op->d_rsrc));
+    FG_CHECK(upload(vpos, &op->d_rvpos));
+    FG_CHECK(upload(wst, &op->d_rwst));
     const size_t nv = (size_t)K * (L.nslot ? L.nslot : 1);
     FG_HIP(hipMalloc(&op->d_sv64, sizeof(double) * nv));
     FG_HIP(hipMalloc(&op->d_sv32, sizeof(float) * nv));
     FG_HIP(hipMemset(op->d_sv64, 0, sizeof(double) * nv));
     FG_HIP(hipMemset(op->d_sv32, 0, sizeof(float) * nv));
     op->rows_stale = 1;
+    return FG_OK;
+}
+
+int fg_ebe_rows_set_values(fg_op *op, const double *A)
+{
+    fg_ctx *c = op->ctx;
+    const long long n = (long long)op->nitems * op->k;
+    if (n == 0) return FG_OK;
+    if (op->k == 3)
+        FG_LAUNCH(k_ebe_scatter_rows<3>, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, A,
+                  op->d_kpos, op->d_kstride, op->d_rvpos, op->d_rwst, op->d_K64, op->d_K32,
+                  op->d_sv64, op->d_sv32);
+    else
+        FG_LAUNCH(k_ebe_scatter_rows<4>, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, A,
+                  op->d_kpos, op->d_kstride, op->d_rvpos, op->d_rwst, op->d_K64, op->d_K32,
+                  op->d_sv64, op->d_sv32);
+    op->rows_stale = 0;
     return FG_OK;
 }
 
@@ -279,7 +344,8 @@ int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 
 void fg_ebe_rows_free(fg_op *op)
 {
-    hipFree(op->d_ridx); hipFree(op->d_ridx2); hipFree(op->d_rsrc);
+    hipFree(op->d_ridx); hipFree(op->d_ridx2); hipFree(op->d_rsrc); hipFree(op->d_rvpos);
+    hipFree(op->d_rwst);
     if (op->kind == FG_OP_EBE) {
         hipFree(op->d_soff); hipFree(op->d_sv64); hipFree(op->d_sv32);
     }

@@ -151,6 +151,8 @@ struct fg_op {
     unsigned *d_ridx = nullptr;
     short *d_ridx2 = nullptr;
     int *d_rsrc = nullptr;
+    long long *d_rvpos = nullptr; // [nel*k] value position of row (e, lr), -1: no row
+    int *d_rwst = nullptr;        // [nel*k] stride between its value columns
 
     // ---- systems of at most FG_SMALL_MAX equations: lane-coalesced lists for
     // the one-workgroup solver, entry c of row g at c*neq + g (-1: none) ----
@@ -302,6 +304,8 @@ inline bool fg_ede_rows_used(const fg_op *op, int fp32)
 }
 int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos);
 int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
+// AoS element values (device) -> blocked layout and rows in one pass
+int fg_ebe_rows_set_values(fg_op *op, const double *A);
 void fg_ebe_rows_free(fg_op *op);
 // EBE k = 3, 4: the element-row product in use
 inline bool fg_ebe_rows_used(const fg_op *op)
