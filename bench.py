@@ -108,7 +108,7 @@ def ebe_kernel_name(k, fp32, op=None):
     operator uses them, else the blocked kernel at the library's default
     block size"""
     t = "float" if fp32 else "double"
-    if op is not None and op.product_sliced():
+    if op is not None and op.product_sliced(fp32):
         return f"k_ebe_rows<{t}, {k}>"
     ept = 4 if k <= 2 else 2 if k == 3 else 4 if k == 4 else 1   # the library's defaults
     return f"k_ebe_apply<{k}, {ept}, {t}, 0>"
@@ -143,7 +143,7 @@ def matvec_table(op, P64, Q64, P32, Q32, reps):
         b = op.algorithmic_bytes(fp32)
         r["f32" if fp32 else "f64"] = dict(ms=t, GBs=b / (t * 1e-3) / 1e9,
                                           pct_peak=100 * b / (t * 1e-3) / 1e9 / PEAK_GBS,
-                                          bytes=b)
+                                          bytes=b, sliced=op.product_sliced(fp32))
     return r
 
 
@@ -233,16 +233,21 @@ def headline_leg(F, ctx, args, rng):
     default_op = sysd["ops"]["EBE"]          # the library default: 512 per block, BFS
     if args.no_autotune:
         best_epb, best_ord, best_pipe, tune_ms = 512, F.ORDER_BFS, 0, {}
+        fp32_ms = None
         ebe_tuned = default_op
     else:
         (best_epb, best_ord, best_pipe), tune_ms, ebe_tuned = autotune.tune_ebe(
             ctx, neq, sysd["lm"], 3, sysd["ke"], orderings=autotune.ORDERINGS)
         log("autotune EBE f64 ms by elements/block / ordering / workgroups per CU:",
             {autotune.key_str(k): round(v, 4) for k, v in tune_ms.items()}, "->",
-            autotune.key_str((best_epb, best_ord, best_pipe)))
+            autotune.key_str((best_epb, best_ord, best_pipe)),
+            "; FP32 products:", getattr(ebe_tuned, "other_precision_ms", None))
+        fp32_ms = getattr(ebe_tuned, "other_precision_ms", {}).get("fp32")
         if best_pipe == autotune.ROWS or (best_epb, best_ord, best_pipe) == (512, F.ORDER_BFS, 0):
-            default_op.set_product_kernel(F.PRODUCT_AUTO if best_pipe == autotune.ROWS
-                                          else F.PRODUCT_BLOCKED)
+            # the same kernel choice per precision on the setup operator
+            for fp32, prec in ((False, F.PRODUCT_FP64), (True, F.PRODUCT_FP32)):
+                default_op.set_product_kernel(F.PRODUCT_AUTO if ebe_tuned.product_sliced(fp32)
+                                              else F.PRODUCT_BLOCKED, prec)
             # the tuner chose the default configuration: keep the operator
             # built at setup (allocated before the tuning candidates)
             ebe_tuned.close()
@@ -296,22 +301,27 @@ def headline_leg(F, ctx, args, rng):
         matvec[name] = matvec_table(op, P64, Q64, P32, Q32, args.steps)
         log(name, {k: round(v["GBs"], 1) for k, v in matvec[name].items()})
         if name in ("EBE", "EDE"):
-            # the other product kernel of the same operator, for comparison
-            sliced = op.product_sliced()
+            # the other product kernel of each precision, for comparison
+            s64, s32 = op.product_sliced(False), op.product_sliced(True)
             try:
-                op.set_product_kernel(F.PRODUCT_BLOCKED if sliced else F.PRODUCT_SLICED)
+                op.set_product_kernel(F.PRODUCT_BLOCKED if s64 else F.PRODUCT_SLICED,
+                                      F.PRODUCT_FP64)
+                op.set_product_kernel(F.PRODUCT_BLOCKED if s32 else F.PRODUCT_SLICED,
+                                      F.PRODUCT_FP32)
             except F.FemGpuError:
                 continue           # no sliced rows on this mesh
-            alt = name + ("_blocked" if sliced else "_sliced")
+            alt = name + "_other"
             matvec[alt] = matvec_table(op, P64, Q64, P32, Q32, args.steps)
-            log(alt, {k: round(v["GBs"], 1) for k, v in matvec[alt].items()})
-            op.set_product_kernel(F.PRODUCT_AUTO if sliced else F.PRODUCT_BLOCKED)
+            log(alt, {k: (v["sliced"], round(v["GBs"], 1)) for k, v in matvec[alt].items()})
+            op.set_product_kernel(F.PRODUCT_AUTO if s64 else F.PRODUCT_BLOCKED, F.PRODUCT_FP64)
+            op.set_product_kernel(F.PRODUCT_AUTO if s32 else F.PRODUCT_BLOCKED, F.PRODUCT_FP32)
     if ebe_tuned is not default_op:
         default_op.close()
     res = {"value": value, "ms": ms, "wall_ms_per_step": wall * 1e3 / args.steps,
            "scale": scale, "kappa": kappa, "theta": theta, "roofline": roof,
            "matvec": matvec, "ebe_autotune_ms": {autotune.key_str(k): v for k, v in tune_ms.items()},
-           "ede_autotune": ede_tune, "neq": neq, "nel": sysd["nel"]}
+           "ede_autotune": ede_tune, "neq": neq, "nel": sysd["nel"],
+           "ebe_fp32_product_ms": fp32_ms}
 
     if args.calibrate_copy:
         m = 64 << 20
@@ -719,7 +729,7 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
                      "ms": {autotune.key_str(k): v for k, v in tms.items()}}
         log("ss3d autotune FP32 product ms:", {autotune.key_str(k): round(v, 4)
                                                for k, v in tms.items()}, "->", autotune.key_str(key))
-        ebe_kernel = ("k_ebe_rows<float, 4>" if ebe.product_sliced()
+        ebe_kernel = ("k_ebe_rows<float, 4>" if ebe.product_sliced(True)
                       else f"k_ebe_apply<4, {key[0] // 256}, float, 0>")
     del cbe, IA, JA
     pc = F.Preconditioner(csr, F.PREC_ILU0)
@@ -1132,6 +1142,7 @@ def main():
         "matvec": head["matvec"],
         "copy_calibration": head.get("copy_calibration"),
         "ebe_autotune_ms": head["ebe_autotune_ms"],
+        "ebe_fp32_product_ms": head.get("ebe_fp32_product_ms"),
         "ede_autotune": head.get("ede_autotune"),
         "solve": head.get("solve"),
         "small": small,

@@ -30,6 +30,7 @@ ORDER_BFS, ORDER_MESH = 0, 1
 ROWS_VALU, ROWS_MFMA = 0, 1
 EDGE_ROWS, EDGE_BLOCKED, EDGE_AUTO = 0, 1, 2
 PRODUCT_SLICED, PRODUCT_BLOCKED, PRODUCT_AUTO = 0, 1, 2
+PRODUCT_FP64, PRODUCT_FP32 = 1, 2
 EBE2_BLOCKDIAG, EBE2_NOBLOCKDIAG = 0, 1
 P3_DIFCONV, P3_CONST = 0, 1
 GMRES_FP64, GMRES_FP32_IR = 0, 1
@@ -116,8 +117,8 @@ SIGNATURES = {
     "fg_op_set_pipeline": (_i, [_vp, _i]),
     "fg_op_set_row_kernel": (_i, [_vp, _i]),
     "fg_op_set_edge_kernel": (_i, [_vp, _i]),
-    "fg_op_set_product_kernel": (_i, [_vp, _i]),
-    "fg_op_product_kernel": (_i, [_vp, C.POINTER(_i)]),
+    "fg_op_set_product_kernel": (_i, [_vp, _i, _i]),
+    "fg_op_product_kernel": (_i, [_vp, _i, C.POINTER(_i)]),
     "fg_prec_create": (_i, [_vp, _i, C.POINTER(_vp)]),
     "fg_prec_setup": (_i, [_vp, _vp]),
     "fg_prec_apply_f64": (_i, [_vp, _vp, _vp]),
@@ -414,15 +415,18 @@ class Operator:
         EDGE_BLOCKED (EDE operators)"""
         _check(lib.fg_op_set_edge_kernel(self.h, int(kind)), "fg_op_set_edge_kernel")
 
-    def set_product_kernel(self, kind):
+    def set_product_kernel(self, kind, precisions=PRODUCT_FP64 | PRODUCT_FP32):
         """PRODUCT_AUTO (default: sliced rows where the operator has them),
-        PRODUCT_SLICED or PRODUCT_BLOCKED (EBE and EDE operators)"""
-        _check(lib.fg_op_set_product_kernel(self.h, int(kind)), "fg_op_set_product_kernel")
+        PRODUCT_SLICED or PRODUCT_BLOCKED (EBE and EDE operators), for the
+        FP64 products, the FP32 products or both"""
+        _check(lib.fg_op_set_product_kernel(self.h, int(precisions), int(kind)),
+               "fg_op_set_product_kernel")
 
-    def product_sliced(self) -> bool:
-        """whether the product in use is a sliced-row kernel"""
+    def product_sliced(self, fp32=False) -> bool:
+        """whether the product in use for that precision is a sliced-row kernel"""
         v = _i()
-        _check(lib.fg_op_product_kernel(self.h, C.byref(v)), "fg_op_product_kernel")
+        _check(lib.fg_op_product_kernel(self.h, int(bool(fp32)), C.byref(v)),
+               "fg_op_product_kernel")
         return bool(v.value)
 
     def close(self):

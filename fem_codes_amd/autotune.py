@@ -29,8 +29,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import (ORDER_BFS, ORDER_MESH, PRODUCT_AUTO, PRODUCT_BLOCKED, DeviceArray,
-               FemGpuError, Operator)
+from . import (ORDER_BFS, ORDER_MESH, PRODUCT_AUTO, PRODUCT_BLOCKED, PRODUCT_FP32, PRODUCT_FP64,
+               DeviceArray, FemGpuError, Operator)
 
 CANDIDATES = (256, 512, 1024, 2048)
 PIPELINES = (0, 1, 2, 3, 4, 6)
@@ -65,9 +65,9 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
         except FemGpuError:
             return
         op.set_values(values)
-        if not ops and op.product_sliced():
+        if not ops and op.product_sliced(fp32):
             times[(0, ORDER_BFS, ROWS)] = op.time_apply(P, Q, reps=reps)
-        if edges or op.product_sliced():
+        if edges or op.product_sliced(fp32):
             op.set_product_kernel(PRODUCT_BLOCKED)
         for pipe in pipelines:
             op.set_pipeline(pipe)
@@ -90,9 +90,22 @@ def tune_ebe(ctx, neq: int, lm: np.ndarray, k: int, values, fp32: bool = False,
             op.close()
     op = ops[bbest[:2]]
     op.set_pipeline(bbest[2])
-    # sliced rows won: both precisions use them (PRODUCT_AUTO); else the
-    # blocked schedule that won
-    op.set_product_kernel(PRODUCT_AUTO if best[2] == ROWS else PRODUCT_BLOCKED)
+    # the tuned precision: sliced rows if they won, else the blocked schedule
+    # that won; the other precision: whichever of the two is faster on the
+    # returned operator (its timings in op.other_precision_ms)
+    mine, other = (PRODUCT_FP32, PRODUCT_FP64) if fp32 else (PRODUCT_FP64, PRODUCT_FP32)
+    op.set_product_kernel(PRODUCT_AUTO if best[2] == ROWS else PRODUCT_BLOCKED, mine)
+    if ROWS in [k[2] for k in times]:
+        odt = np.float64 if fp32 else np.float32
+        oname = "fp64" if fp32 else "fp32"
+        Po = ctx.to_device(np.random.default_rng(1).standard_normal(neq + 1).astype(odt))
+        Qo = DeviceArray(ctx, neq + 1, odt)
+        op.set_product_kernel(PRODUCT_AUTO, other)
+        t_rows = op.time_apply(Po, Qo, reps=reps)
+        op.set_product_kernel(PRODUCT_BLOCKED, other)
+        t_blk = op.time_apply(Po, Qo, reps=reps)
+        op.other_precision_ms = {oname: {"sliced": t_rows, "blocked": t_blk}}
+        op.set_product_kernel(PRODUCT_AUTO if t_rows <= t_blk else PRODUCT_BLOCKED, other)
     return best, times, op
 
 

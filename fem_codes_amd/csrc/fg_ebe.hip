@@ -442,7 +442,7 @@ int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
 {
     if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, fp32))
         return fg_ede_rows_apply(op, fp32, P, Q, nullptr);
-    if (fg_ebe_rows_used(op)) return fg_ebe_rows_apply(op, fp32, P, Q, nullptr);
+    if (fg_ebe_rows_used(op, fp32)) return fg_ebe_rows_apply(op, fp32, P, Q, nullptr);
     if (op->nblk == 0) return FG_OK;
     if (fp32) {
         switch (op->k) {
@@ -470,13 +470,13 @@ int fg_ebe_apply_main(fg_op *op, int fp32, void *P, void *Q)
 int fg_ebe_apply_f64(fg_op *op, double *P, double *Q, const int *skip)
 {
     if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, 0)) return fg_ede_rows_apply(op, 0, P, Q, skip);
-    if (fg_ebe_rows_used(op)) return fg_ebe_rows_apply(op, 0, P, Q, skip);
+    if (fg_ebe_rows_used(op, 0)) return fg_ebe_rows_apply(op, 0, P, Q, skip);
     return ebe_apply<double>(op, op->d_K64, P, Q, op->d_hpart64, skip);
 }
 int fg_ebe_apply_f32(fg_op *op, float *P, float *Q, const int *skip)
 {
     if (op->kind == FG_OP_EDE && fg_ede_rows_used(op, 1)) return fg_ede_rows_apply(op, 1, P, Q, skip);
-    if (fg_ebe_rows_used(op)) return fg_ebe_rows_apply(op, 1, P, Q, skip);
+    if (fg_ebe_rows_used(op, 1)) return fg_ebe_rows_apply(op, 1, P, Q, skip);
     return ebe_apply<float>(op, op->d_K32, P, Q, op->d_hpart32, skip);
 }
 

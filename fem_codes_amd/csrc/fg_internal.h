@@ -162,7 +162,7 @@ struct fg_op {
     int *d_sm_lmT = nullptr;     // EBE/EDE: lm transposed, [k][nitems]
     // EDE (k = 2): half-edge rows (fg_ede_rows.hip), a second copy of the
     // edge values ordered by equation, kept in step with the blocked copy
-    int rows_kind = FG_PRODUCT_AUTO; // FG_PRODUCT_SLICED / _BLOCKED / _AUTO (sliced rows when built)
+    int rows_kind[2] = {FG_PRODUCT_AUTO, FG_PRODUCT_AUTO}; // FP64, FP32 products: FG_PRODUCT_SLICED / _BLOCKED / _AUTO
     int rows_stale = 1;          // the rows' values lag the blocked values
     // (sliced: d_soff / d_scol / d_sv64 / d_sv32 below the CSR fields)
     long long nhalf = 0;         // half-edges (rows' entries)
@@ -299,18 +299,17 @@ int fg_csr_values_changed(fg_op *op);
 void fg_ede_rows_free(fg_op *op);
 inline bool fg_ede_rows_used(const fg_op *op, int fp32)
 {
-    (void)fp32;
-    return op->d_hsrc && op->rows_kind != FG_PRODUCT_BLOCKED;
+    return op->d_hsrc && op->rows_kind[fp32 ? 1 : 0] != FG_PRODUCT_BLOCKED;
 }
 int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos);
 int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
 // AoS element values (device) -> blocked layout and rows in one pass
 int fg_ebe_rows_set_values(fg_op *op, const double *A);
 void fg_ebe_rows_free(fg_op *op);
-// EBE k = 3, 4: the element-row product in use
-inline bool fg_ebe_rows_used(const fg_op *op)
+// EBE k = 3, 4: the element-row product in use for that precision
+inline bool fg_ebe_rows_used(const fg_op *op, int fp32)
 {
-    return op->kind == FG_OP_EBE && op->d_ridx && op->rows_kind != FG_PRODUCT_BLOCKED;
+    return op->kind == FG_OP_EBE && op->d_ridx && op->rows_kind[fp32 ? 1 : 0] != FG_PRODUCT_BLOCKED;
 }
 int fg_prec_apply_any(fg_prec *pc, int fp32, const void *p, void *z, const int *skip = nullptr);
 int fg_vec_grid(fg_ctx *ctx, long long n);

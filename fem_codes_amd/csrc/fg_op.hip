@@ -140,29 +140,32 @@ extern "C" int fg_op_set_edge_kernel(fg_op *op, int kind)
     FG_ARG(op->kind == FG_OP_EDE && op->k == 2, "edge kernels apply to EDE operators");
     FG_ARG(kind == FG_EDGE_ROWS || kind == FG_EDGE_BLOCKED || kind == FG_EDGE_AUTO,
            "kind: FG_EDGE_ROWS, FG_EDGE_BLOCKED or FG_EDGE_AUTO");
-    op->rows_kind = kind;
+    op->rows_kind[0] = op->rows_kind[1] = kind;
     return FG_OK;
 }
 
-extern "C" int fg_op_set_product_kernel(fg_op *op, int kind)
+extern "C" int fg_op_set_product_kernel(fg_op *op, int precisions, int kind)
 {
     FG_ARG(op, "op");
     FG_ARG(op->kind == FG_OP_EBE || op->kind == FG_OP_EDE,
            "the product kernel choice applies to EBE and EDE operators");
+    FG_ARG(precisions >= 1 && precisions <= (FG_PRODUCT_FP64 | FG_PRODUCT_FP32),
+           "precisions: FG_PRODUCT_FP64, FG_PRODUCT_FP32 or both");
     FG_ARG(kind == FG_PRODUCT_SLICED || kind == FG_PRODUCT_BLOCKED || kind == FG_PRODUCT_AUTO,
            "kind: FG_PRODUCT_SLICED, FG_PRODUCT_BLOCKED or FG_PRODUCT_AUTO");
     FG_ARG(kind != FG_PRODUCT_SLICED || op->d_ridx || op->d_hsrc,
            "this operator has no sliced rows (k other than 2, 3, 4, or column offsets "
            "beyond the row fields)");
-    op->rows_kind = kind;
+    if (precisions & FG_PRODUCT_FP64) op->rows_kind[0] = kind;
+    if (precisions & FG_PRODUCT_FP32) op->rows_kind[1] = kind;
     return FG_OK;
 }
 
-extern "C" int fg_op_product_kernel(const fg_op *op, int *sliced)
+extern "C" int fg_op_product_kernel(const fg_op *op, int fp32, int *sliced)
 {
     FG_ARG(op && sliced, "op/sliced");
-    *sliced = (op->kind == FG_OP_EDE && fg_ede_rows_used(op, 0)) || fg_ebe_rows_used(op) ||
-              (op->kind == FG_OP_CSR && op->d_spos);
+    *sliced = (op->kind == FG_OP_EDE && fg_ede_rows_used(op, fp32)) ||
+              fg_ebe_rows_used(op, fp32) || (op->kind == FG_OP_CSR && op->d_spos);
     return FG_OK;
 }
 

@@ -76,6 +76,28 @@ def test_ebe_sliced_rows_bitwise(system):
     op.close()
 
 
+def test_ebe_product_kernel_per_precision(system):
+    """the kernel choice is per precision: blocked FP64 products beside
+    sliced-row FP32 products on one operator"""
+    ctx, neq, nel, lm, ke, P = system
+    op = F.Operator.ebe(ctx, neq, lm, 3)
+    op.set_values(ke)
+    op.set_product_kernel(F.PRODUCT_BLOCKED, F.PRODUCT_FP64)
+    assert not op.product_sliced(False) and op.product_sliced(True)
+    keh = ke.get()
+    Qo = O.Oracle().ebemv(neq, nel, 3, keh, lm, P.copy())
+    assert np.max(np.abs(op.matvec(P)[:neq] - Qo[:neq])) <= 1e-14 * np.max(np.abs(Qo[:neq]))
+    P32 = P.astype(np.float32)
+    assert np.array_equal(op.matvec(P32, np.float32), _ebemv_f32(neq, lm, keh, P32))
+    op.set_product_kernel(F.PRODUCT_AUTO, F.PRODUCT_FP64)
+    op.set_product_kernel(F.PRODUCT_BLOCKED, F.PRODUCT_FP32)
+    assert op.product_sliced(False) and not op.product_sliced(True)
+    assert np.array_equal(op.matvec(P), Qo)
+    with pytest.raises(F.FemGpuError):
+        op.set_product_kernel(F.PRODUCT_AUTO, 0)
+    op.close()
+
+
 def test_ebe_sliced_rows_follow_value_changes(system):
     """new values and a Left/LeftRight scaling reach the row copy before the
     next product"""
