@@ -48,7 +48,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("matvec HBM GB/s (% of peak) + time-to-1e-8 solve, FP32-IR vs FP64 GMRES")
 PEAK_GBS = 8000.0
-ENGINES = {"auto": 0, "kernels": 1, "fused": 2, "small": 3}
+ENGINES = {"auto": 0, "kernels": 1, "fused": 2, "small": 3, "graph": 4}
 ORTHS = {"mgs": 0, "cgs": 1}
 LEGS = ("headline", "small", "batched", "ss3d", "euler", "transient")
 
@@ -744,7 +744,7 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
     res = {"n": n, "elements": nel, "equations": neq, "nnz": int(nnz),
            "bandwidth_random": int(bw0), "bandwidth_rcm": int(bw1), "ilu_levels": levels[0],
            "ilu_sweeps": sweeps, "kmax": 50, "tol": 1e-10, "structure_seconds": t_struct,
-           "fp32_ir_orth": args.orth}
+           "fp32_ir_orth": args.orth, "engine": args.engine}
     mv = {}
     P64 = ctx.to_device(rng.standard_normal(neq + 1))
     Q64 = ctx.empty(neq + 1)
@@ -788,7 +788,8 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
         ctx.sync()
         w0 = time.perf_counter()
         info, _, _ = F.gmres(ctx, ebe, pc, B, X, kmax=50, lmax=400, tol=1e-10, precision=prec,
-                             orth=ORTHS[args.orth] if prec == F.GMRES_FP32_IR else F.ORTH_MGS)
+                             orth=ORTHS[args.orth] if prec == F.GMRES_FP32_IR else F.ORTH_MGS,
+                             engine=ENGINES[args.engine])
         ctx.sync()
         Xs[tag] = X.get()
         res[tag] = solve_record(info, time.perf_counter() - w0, 1e-10)
@@ -857,7 +858,8 @@ def transient_leg(F, ctx, args, n=1024, steps=10):
     u0[idv[free]] = np.exp(-0.5 * ((x[free] - 5.0) ** 2 + (y[free] - 7.5) ** 2))
     dt = 1e-3
     res = {"n": n, "equations": neq, "elements": nel, "steps": steps, "dt": dt,
-           "problem": "CONE", "precond": "Diag", "kmax": 30, "tol": 1e-8}
+           "problem": "CONE", "precond": "Diag", "kmax": 30, "tol": 1e-8,
+           "engine": args.engine}
     U = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         u = ctx.to_device(u0)
@@ -866,7 +868,7 @@ def transient_leg(F, ctx, args, n=1024, steps=10):
         info, hist = F.predictor_old_transport(
             mesh, pb, op, pc, up, u, alpha=0.5, dt=dt, final_time=steps * dt + 0.5 * dt,
             nl_tol=1e-3, nl_maxiter=10, stop_by_norm=True, steady=False, max_steps=steps,
-            kmax=30, lmax=200, tol=1e-8, precision=prec, engine=F.ENGINE_KERNELS,
+            kmax=30, lmax=200, tol=1e-8, precision=prec, engine=ENGINES[args.engine],
             orth=F.ORTH_CGS if prec == F.GMRES_FP32_IR else F.ORTH_MGS)
         wall = time.perf_counter() - t0
         U[tag] = u.get()
@@ -979,6 +981,7 @@ def euler_leg(F, ctx, args, n=1024, seed=20260305):
                                else ebe_kernel_name(12, True))
     res["roofline"]["traffic"] = leg_traffic("euler", res["roofline"]["kernel"])
     res["fp32_ir_orth"] = args.orth
+    res["engine"] = args.engine
     Xs = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         B = ctx.to_device(Rh)
@@ -987,7 +990,8 @@ def euler_leg(F, ctx, args, n=1024, seed=20260305):
         ctx.sync()
         w0 = time.perf_counter()
         info, _, _ = F.gmres(ctx, op, pc, B, X, kmax=30, lmax=2000, tol=1e-8, precision=prec,
-                             orth=ORTHS[args.orth] if prec == F.GMRES_FP32_IR else F.ORTH_MGS)
+                             orth=ORTHS[args.orth] if prec == F.GMRES_FP32_IR else F.ORTH_MGS,
+                             engine=ENGINES[args.engine])
         ctx.sync()
         Xs[tag] = X.get()
         res[tag] = solve_record(info, time.perf_counter() - w0, 1e-8)
@@ -1059,7 +1063,8 @@ def main():
                     help="batched leg: launch every cycle's kernels directly instead of "
                          "replaying a captured hipGraph (kernel-trace profiling runs)")
     ap.add_argument("--engine", default="kernels", choices=sorted(ENGINES),
-                    help="GMRES engine for the 2048^2 time-to-solve runs")
+                    help="GMRES engine for the 2048^2 time-to-solve runs and the single-system "
+                         "solves of the 3D, Euler and transient legs")
     ap.add_argument("--orth", default="cgs", choices=sorted(ORTHS),
                     help="orthogonalisation of the FP32-IR inner cycle (FP64 is always MGS)")
     ap.add_argument("--rot-cycles", type=int, default=40,

@@ -57,7 +57,7 @@ class GmresOpts(C.Structure):
                 ("orth", C.c_int)]
 
 
-ENGINE_AUTO, ENGINE_KERNELS, ENGINE_FUSED, ENGINE_SMALL = 0, 1, 2, 3
+ENGINE_AUTO, ENGINE_KERNELS, ENGINE_FUSED, ENGINE_SMALL, ENGINE_GRAPH = 0, 1, 2, 3, 4
 ORTH_MGS, ORTH_CGS = 0, 1
 
 
@@ -574,8 +574,10 @@ def gmres(ctx: Context, op: Operator, prec: Preconditioner | None, B: DeviceArra
 
     engine: ENGINE_KERNELS (one kernel per vector operation), ENGINE_FUSED
     (the whole solve in one persistent kernel), ENGINE_SMALL (the whole FP64
-    solve in one workgroup, systems of at most 4096 equations) or
-    ENGINE_AUTO (SMALL where it applies, else KERNELS).
+    solve in one workgroup, systems of at most 4096 equations),
+    ENGINE_GRAPH (KERNELS with each restart cycle replayed from one captured
+    hipGraph, kept in the context for later solves) or ENGINE_AUTO (SMALL
+    where it applies, else KERNELS).
     orth: ORTH_MGS or ORTH_CGS for the FP32-IR inner cycle.
     Returns (info, rho_hist, e0_hist)."""
     rho = np.zeros(max(1, hist_cap))

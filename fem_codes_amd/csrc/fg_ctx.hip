@@ -2,7 +2,11 @@
 #include <stdarg.h>
 #include <string.h>
 
+#include <atomic>
+
 #include "fg_internal.h"
+
+static std::atomic<unsigned long long> g_cfg_gen_value{0};
 
 static thread_local char g_err[1024] = "";
 
@@ -56,11 +60,23 @@ extern "C" int fg_ctx_create(int device, fg_ctx **out)
     return FG_OK;
 }
 
+unsigned long long fg_cfg_bump() { return ++g_cfg_gen_value; }
+
+unsigned long long fg_cfg_gen() { return g_cfg_gen_value.load(); }
+
+void fg_cycle_graph_clear(fg_cycle_graph &G)
+{
+    if (G.ge) hipGraphExecDestroy(G.ge);
+    if (G.g) hipGraphDestroy(G.g);
+    G = fg_cycle_graph();
+}
+
 extern "C" int fg_ctx_destroy(fg_ctx *c)
 {
     if (!c) return FG_OK;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    fg_cycle_graph_clear(c->cg);
     hipFree(c->red_part);
     hipFree(c->red_ticket);
     hipFree(c->ws);
@@ -79,6 +95,7 @@ int fg_ctx_workspace(fg_ctx *c, size_t bytes, void **p)
         if (c->ws) FG_HIP(hipFree(c->ws));
         c->ws = nullptr;
         c->ws_bytes = 0;
+        c->ws_gen++;
         FG_HIP(hipMalloc(&c->ws, bytes));
         c->ws_bytes = bytes;
     }
@@ -91,9 +108,11 @@ extern "C" int fg_ctx_trim(fg_ctx *c)
     FG_ARG(c, "ctx");
     FG_HIP(hipSetDevice(c->device));
     FG_HIP(hipStreamSynchronize(c->stream));
+    fg_cycle_graph_clear(c->cg);
     if (c->ws) FG_HIP(hipFree(c->ws));
     c->ws = nullptr;
     c->ws_bytes = 0;
+    c->ws_gen++;
     return FG_OK;
 }
 

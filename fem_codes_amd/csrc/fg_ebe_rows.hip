@@ -302,22 +302,28 @@ int fg_ebe_rows_set_values(fg_op *op, const double *A)
     return FG_OK;
 }
 
+int fg_ebe_rows_refresh(fg_op *op)
+{
+    if (!op->d_ridx || !op->rows_stale) return FG_OK;
+    fg_ctx *c = op->ctx;
+    const long long nt = (long long)op->nslice * FG_SELL_W;
+    if (op->k == 3)
+        FG_LAUNCH(k_ebe_rows_mirror<3>, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
+                  op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
+                  op->d_sv64, op->d_sv32);
+    else
+        FG_LAUNCH(k_ebe_rows_mirror<4>, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
+                  op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
+                  op->d_sv64, op->d_sv32);
+    op->rows_stale = 0;
+    return FG_OK;
+}
+
 int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 {
     fg_ctx *c = op->ctx;
     const int K = op->k;
-    if (op->rows_stale) {
-        const long long nt = (long long)op->nslice * FG_SELL_W;
-        if (K == 3)
-            FG_LAUNCH(k_ebe_rows_mirror<3>, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
-                      op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
-                      op->d_sv64, op->d_sv32);
-        else
-            FG_LAUNCH(k_ebe_rows_mirror<4>, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
-                      op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
-                      op->d_sv64, op->d_sv32);
-        op->rows_stale = 0;
-    }
+    FG_CHECK(fg_ebe_rows_refresh(op));
     constexpr int SPW = FG_TPB / FG_SELL_W;
     const int g = (op->nslice + SPW - 1) / SPW;
     if (fp32) {

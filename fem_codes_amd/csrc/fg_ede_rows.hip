@@ -176,15 +176,21 @@ int fg_ede_rows_build(fg_op *op, const int *lm2, const long long *kpos)
     return FG_OK;
 }
 
+int fg_ede_rows_refresh(fg_op *op)
+{
+    if (!op->d_hsrc || !op->rows_stale) return FG_OK;
+    fg_ctx *c = op->ctx;
+    if (op->nslot > 0)
+        FG_LAUNCH(k_ede_rows_mirror, fg_vec_grid(c, op->nslot), FG_TPB, 0, c->stream,
+                  op->nslot, op->d_hsrc, op->d_K64, op->d_K32, op->d_sv64, op->d_sv32);
+    op->rows_stale = 0;
+    return FG_OK;
+}
+
 int fg_ede_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 {
     fg_ctx *c = op->ctx;
-    if (op->rows_stale) {
-        if (op->nslot > 0)
-            FG_LAUNCH(k_ede_rows_mirror, fg_vec_grid(c, op->nslot), FG_TPB, 0, c->stream,
-                      op->nslot, op->d_hsrc, op->d_K64, op->d_K32, op->d_sv64, op->d_sv32);
-        op->rows_stale = 0;
-    }
+    FG_CHECK(fg_ede_rows_refresh(op));
     if (op->nslice == 0) {
         // no equation: Q[neq] = P[neq] = 0
         if (fp32) {

@@ -628,7 +628,7 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
     FG_ARG(o->kmax >= 1 && o->kmax <= FG_KMAX, "kmax in [1,64]");
     FG_ARG(o->lmax >= 1, "lmax >= 1");
     FG_ARG(o->precision == FG_GMRES_FP64 || o->precision == FG_GMRES_FP32_IR, "precision");
-    FG_ARG(o->engine >= FG_ENGINE_AUTO && o->engine <= FG_ENGINE_SMALL, "engine");
+    FG_ARG(o->engine >= FG_ENGINE_AUTO && o->engine <= FG_ENGINE_GRAPH, "engine");
     if (pc && pc->kind == FG_PREC_NONE) pc = nullptr;
     FG_ARG(!pc || (pc->op->neq == op->neq && pc->op->ctx == c),
            "preconditioner built on an operator of another size or context");
@@ -717,6 +717,11 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
     HostState hs{};
     int cycles = 0, rc = FG_OK;
     const int g = pp.G;
+    // graph replay of the cycle: not for exact level-scheduled ILU(0) solves
+    // (thousands of launches per apply, each level its own grid)
+    const bool use_graph = o->engine == FG_ENGINE_GRAPH && n > 0 &&
+                           !(pc && pc->kind == FG_PREC_ILU0 && fg_ilu_sweep_count(pc->ilu) == 0);
+    if (use_graph) FG_CHECK(fg_op_prepare(op));
     for (;;) {
         FG_CHECK(residual(c, op, pc, B, X, W64, st, inner, pp));
         FG_CHECK(read_state(c, st, hs));
@@ -729,22 +734,69 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         if (ir && hs.conv) break;
         if (!ir && hs.e0 == 0.0) { hs.rho = 0.0; break; }
         int steps_before = hs.steps;
-        if (ir) {
-            FG_LAUNCH(k_scale_to<float>, g, FG_TPB, 0, c->stream, n, W64, (const float *)nullptr,
-                      (float *)U, st, 0);
-            FG_CHECK(cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, kmax, pp,
-                                  cgbuf));
+        // u_0, the kmax Arnoldi steps, the back substitution: every argument
+        // is a workspace or operator pointer, every per-solve quantity lives
+        // in the device state, so the sequence can be captured and replayed
+        auto body = [&]() -> int {
+            if (ir) {
+                FG_LAUNCH(k_scale_to<float>, g, FG_TPB, 0, c->stream, n, W64,
+                          (const float *)nullptr, (float *)U, st, 0);
+                FG_CHECK(cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, kmax, pp,
+                                      cgbuf));
+            } else {
+                FG_LAUNCH(k_scale_to<double>, g, FG_TPB, 0, c->stream, n, W64,
+                          (const double *)nullptr, (double *)U, st, 0);
+                FG_CHECK(cycle<double>(c, op, pc, (double *)U, ld, (double *)W, st, kmax, pp,
+                                       nullptr));
+            }
             FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
-            FG_LAUNCH(k_xupdate<float>, g, FG_TPB, 0, c->stream, n, X, (const float *)U, ld, st);
+            return FG_OK;
+        };
+        if (use_graph) {
+            // the cycle writes a partial buffer before it reads one, so it can
+            // start at either parity: start at 0, as the capture did
+            pp.cur = 0;
+            fg_cycle_graph &G = c->cg;
+            const unsigned long long cfg = fg_cfg_gen();
+            const bool hit = G.ge && G.cfg == cfg && G.ws_gen == c->ws_gen && G.op == op &&
+                             G.pc == pc && G.n == n && G.kmax == kmax && G.ir == (int)ir &&
+                             G.cgs == (int)cgs;
+            if (!hit) {
+                fg_cycle_graph_clear(G);
+                FG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                const int brc = body();
+                hipGraph_t gr = nullptr;
+                const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+                if (brc != FG_OK) {
+                    if (gr) hipGraphDestroy(gr);
+                    return brc;
+                }
+                if (e != hipSuccess) {
+                    fg_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+                    return FG_ERR_HIP;
+                }
+                G.g = gr;
+                FG_HIP(hipGraphInstantiate(&G.ge, G.g, nullptr, nullptr, 0));
+                G.cfg = cfg;
+                G.ws_gen = c->ws_gen;
+                G.op = op;
+                G.pc = pc;
+                G.n = n;
+                G.kmax = kmax;
+                G.ir = ir;
+                G.cgs = cgs;
+                G.pp_after = pp.cur;
+            }
+            FG_HIP(hipGraphLaunch(G.ge, c->stream));
+            pp.cur = G.pp_after;
         } else {
-            FG_LAUNCH(k_scale_to<double>, g, FG_TPB, 0, c->stream, n, W64,
-                      (const double *)nullptr, (double *)U, st, 0);
-            FG_CHECK(cycle<double>(c, op, pc, (double *)U, ld, (double *)W, st, kmax, pp,
-                                   nullptr));
-            FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
+            FG_CHECK(body());
+        }
+        if (ir)
+            FG_LAUNCH(k_xupdate<float>, g, FG_TPB, 0, c->stream, n, X, (const float *)U, ld, st);
+        else
             FG_LAUNCH(k_xupdate<double>, g, FG_TPB, 0, c->stream, n, X, (const double *)U, ld,
                       st);
-        }
         cycles++;
         FG_CHECK(read_state(c, st, hs));
         if (o->rho_hist && o->hist_cap > 0)

@@ -568,3 +568,5 @@ int fg_ilu_get(const fg_ilu *u, fg_ctx *c, double *LU, double *D)
     if (D && u->neq) FG_HIP(hipMemcpy(D, u->d_D, sizeof(double) * u->neq, hipMemcpyDeviceToHost));
     return FG_OK;
 }
+
+int fg_ilu_sweep_count(const fg_ilu *u) { return u->sweeps; }

@@ -60,6 +60,25 @@ int fg_noalloc_init(int device);
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// Configuration generation: bumped whenever an operator or preconditioner is
+// created or destroyed or a kernel knob of one changes.  A captured solver
+// cycle (hipGraph) records the generation and is replayed only while it is
+// unchanged, so a graph never runs with the pointers or launch shapes of an
+// object that is gone or was reconfigured.
+unsigned long long fg_cfg_bump();
+unsigned long long fg_cfg_gen();
+
+// one captured restart cycle of the kernel engine (fg_gmres, FG_ENGINE_GRAPH)
+struct fg_cycle_graph {
+    unsigned long long cfg = 0, ws_gen = 0;
+    const void *op = nullptr, *pc = nullptr;
+    int n = 0, kmax = 0, ir = 0, cgs = 0;
+    int pp_after = 0;                 // parity of the partial buffers after the cycle
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+};
+void fg_cycle_graph_clear(fg_cycle_graph &G);
+
 struct fg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -73,6 +92,8 @@ struct fg_ctx {
     // corrections of a time loop) allocate nothing
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    unsigned long long ws_gen = 0;   // bumped when the workspace moves or is freed
+    fg_cycle_graph cg;               // the last captured GMRES cycle
 };
 
 // the context's solver workspace, at least `bytes` long (256-byte aligned);
@@ -97,6 +118,8 @@ struct fg_carver {
 enum { FG_OP_EBE = 0, FG_OP_EDE = 1, FG_OP_CSR = 2, FG_OP_EBE2 = 3 };
 
 struct fg_op {
+    fg_op() { fg_cfg_bump(); }
+    ~fg_op() { fg_cfg_bump(); }
     fg_ctx *ctx = nullptr;
     int kind = FG_OP_EBE;
     int neq = 0;
@@ -199,6 +222,8 @@ void fg_ede4_sizes(const fg_ede4 *g, fg_ctx **c, int *nel, int *nedge);
 double fg_ebe2_bytes(const fg_op *op, int fp32);
 
 struct fg_prec {
+    fg_prec() { fg_cfg_bump(); }
+    ~fg_prec() { fg_cfg_bump(); }
     fg_op *op = nullptr;
     int kind = FG_PREC_NONE;
     fg_ilu *ilu = nullptr;       // FG_PREC_ILU0
@@ -305,6 +330,13 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos);
 int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
 // AoS element values (device) -> blocked layout and rows in one pass
 int fg_ebe_rows_set_values(fg_op *op, const double *A);
+// refresh row copies that lag the blocked values (element and edge rows)
+int fg_ebe_rows_refresh(fg_op *op);
+int fg_ede_rows_refresh(fg_op *op);
+// run every pending refresh of an operator's derived copies now (before a
+// graph capture or replay, which must not depend on host-side flags)
+int fg_op_prepare(fg_op *op);
+int fg_ilu_sweep_count(const fg_ilu *u);
 void fg_ebe_rows_free(fg_op *op);
 // EBE k = 3, 4: the element-row product in use for that precision
 inline bool fg_ebe_rows_used(const fg_op *op, int fp32)

@@ -120,6 +120,7 @@ extern "C" int fg_op_set_pipeline(fg_op *op, int wg_per_cu)
     FG_ARG(op->kind == FG_OP_EBE || op->kind == FG_OP_EDE, "pipeline applies to EBE/EDE operators");
     FG_ARG(wg_per_cu >= 0 && wg_per_cu <= 16, "wg_per_cu in [0,16]");
     op->pipe_wpc = wg_per_cu;
+    fg_cfg_bump();
     return FG_OK;
 }
 
@@ -131,6 +132,7 @@ extern "C" int fg_op_set_row_kernel(fg_op *op, int kind)
         FG_ARG(op->kind == FG_OP_EBE && op->k == 12 && op->ept == 1,
                "matrix-core rows need a 12-dof element operator with 256 elements per block");
     op->rows_mfma = kind == FG_ROWS_MFMA;
+    fg_cfg_bump();
     return FG_OK;
 }
 
@@ -141,6 +143,7 @@ extern "C" int fg_op_set_edge_kernel(fg_op *op, int kind)
     FG_ARG(kind == FG_EDGE_ROWS || kind == FG_EDGE_BLOCKED || kind == FG_EDGE_AUTO,
            "kind: FG_EDGE_ROWS, FG_EDGE_BLOCKED or FG_EDGE_AUTO");
     op->rows_kind[0] = op->rows_kind[1] = kind;
+    fg_cfg_bump();
     return FG_OK;
 }
 
@@ -158,6 +161,7 @@ extern "C" int fg_op_set_product_kernel(fg_op *op, int precisions, int kind)
            "beyond the row fields)");
     if (precisions & FG_PRODUCT_FP64) op->rows_kind[0] = kind;
     if (precisions & FG_PRODUCT_FP32) op->rows_kind[1] = kind;
+    fg_cfg_bump();
     return FG_OK;
 }
 
@@ -220,5 +224,13 @@ extern "C" int fg_op_destroy(fg_op *op)
     if (op->kind == FG_OP_CSR) fg_csr_free(op);
     else if (op->kind != FG_OP_EBE2) fg_ebe_free(op);
     delete op;
+    return FG_OK;
+}
+
+int fg_op_prepare(fg_op *op)
+{
+    if (op->kind == FG_OP_EBE2) return fg_op_prepare(op->base);
+    if (op->kind == FG_OP_EDE) return fg_ede_rows_refresh(op);
+    if (op->kind == FG_OP_EBE) return fg_ebe_rows_refresh(op);
     return FG_OK;
 }

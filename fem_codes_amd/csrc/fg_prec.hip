@@ -205,6 +205,7 @@ extern "C" int fg_prec_ilu_schedule(fg_prec *pc, int wide_rows)
     FG_ARG(pc && pc->kind == FG_PREC_ILU0, "ILU(0) preconditioner");
     FG_ARG(wide_rows >= 0, "wide_rows >= 0");
     fg_ilu_set_wide(pc->ilu, wide_rows);
+    fg_cfg_bump();
     return FG_OK;
 }
 
@@ -212,6 +213,7 @@ extern "C" int fg_prec_ilu_sweeps(fg_prec *pc, int sweeps)
 {
     FG_ARG(pc && pc->kind == FG_PREC_ILU0, "ILU(0) preconditioner");
     FG_ARG(sweeps >= 0 && sweeps <= 1000, "sweeps in [0,1000]");
+    fg_cfg_bump();
     return fg_ilu_set_sweeps(pc->ilu, sweeps);
 }
 

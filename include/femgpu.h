@@ -485,7 +485,7 @@ typedef struct {
     double *rho_hist;   /* host: rho after every Arnoldi step                  */
     double *e0_hist;    /* host, lmax+1 entries: ||r|| at every restart, or NULL
                            (kernel-per-operation engine only)                  */
-    int    engine;      /* FG_ENGINE_AUTO / _KERNELS / _FUSED / _SMALL         */
+    int    engine;      /* FG_ENGINE_AUTO / _KERNELS / _FUSED / _SMALL / _GRAPH */
     int    orth;        /* FP32-IR inner cycle: FG_ORTH_MGS (0, pgmres.c:107-114)
                            or FG_ORTH_CGS (classical Gram-Schmidt: every h_{j,i}
                            from one pass, one update pass; an option of this
@@ -506,11 +506,22 @@ typedef struct {
  * k <= 4 or CSR, NONE or DIAG preconditioner, neq <= 32 rows per thread of
  * a two-workgroups-per-CU grid); returns FG_ERR_ARG when it does not apply.
  * Its grid-wide barriers cost more than kernel boundaries on a multi-XCD
- * device, so it is kept as an option, not a default. */
+ * device, so it is kept as an option, not a default.
+ * GRAPH: KERNELS with each restart cycle (the scaling of the first basis
+ * vector, the kmax Arnoldi steps, the back substitution) captured once as a
+ * hipGraph and replayed: the same kernels, arguments and results as
+ * KERNELS, without one host launch per kernel.  The captured cycle is kept
+ * in the context and reused by later solves with the same operator,
+ * preconditioner, kmax, precision and orthogonalisation while no operator
+ * or preconditioner has been created, destroyed or reconfigured since (the
+ * time loops' corrections replay one capture); an ILU(0) preconditioner
+ * with exact level-scheduled solves runs as KERNELS.  No reference
+ * counterpart. */
 #define FG_ENGINE_AUTO     0
 #define FG_ENGINE_KERNELS  1
 #define FG_ENGINE_FUSED    2
 #define FG_ENGINE_SMALL    3
+#define FG_ENGINE_GRAPH    4
 
 typedef struct {
     int    steps;       /* Arnoldi steps ("cont", pgmres.c:94)                 */
