@@ -561,8 +561,8 @@ int residual(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, double *X, doub
 
 // one Arnoldi cycle with basis type TB, enqueued without host sync
 template <typename TB>
-int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState *st, int kmax,
-          Parts &pp, double *cgparts)
+int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState *st, int i0,
+          int i1, Parts &pp, double *cgparts)
 {
     int n = op->neq;
     const int g = pp.G;
@@ -578,7 +578,7 @@ int cycle(fg_ctx *c, fg_op *op, fg_prec *pc, TB *U, long long ld, TB *W, GmState
     }
     const int *skip = &st->done;
     const int gc = cgparts ? cgs_grid(n) : 0;
-    for (int i = 0; i < kmax; i++) {
+    for (int i = i0; i < i1; i++) {
         TB *Ui = U + (size_t)i * ld;
         FG_CHECK(fg_op_apply_any(op, fp32, Ui, W, skip));
         if (pc && !diag) FG_CHECK(fg_prec_apply_any(pc, fp32, W, W, skip));
@@ -737,18 +737,24 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         // u_0, the kmax Arnoldi steps, the back substitution: every argument
         // is a workspace or operator pointer, every per-solve quantity lives
         // in the device state, so the sequence can be captured and replayed
-        auto body = [&]() -> int {
-            if (ir) {
+        auto start = [&]() -> int {
+            if (ir)
                 FG_LAUNCH(k_scale_to<float>, g, FG_TPB, 0, c->stream, n, W64,
                           (const float *)nullptr, (float *)U, st, 0);
-                FG_CHECK(cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, kmax, pp,
-                                      cgbuf));
-            } else {
+            else
                 FG_LAUNCH(k_scale_to<double>, g, FG_TPB, 0, c->stream, n, W64,
                           (const double *)nullptr, (double *)U, st, 0);
-                FG_CHECK(cycle<double>(c, op, pc, (double *)U, ld, (double *)W, st, kmax, pp,
-                                       nullptr));
-            }
+            return FG_OK;
+        };
+        auto steps = [&](int i0, int i1) -> int {
+            if (ir)
+                return cycle<float>(c, op, pc, (float *)U, ld, (float *)W, st, i0, i1, pp, cgbuf);
+            return cycle<double>(c, op, pc, (double *)U, ld, (double *)W, st, i0, i1, pp,
+                                 nullptr);
+        };
+        auto body = [&]() -> int {
+            FG_CHECK(start());
+            FG_CHECK(steps(0, kmax));
             FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
             return FG_OK;
         };
@@ -790,15 +796,41 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
             FG_HIP(hipGraphLaunch(G.ge, c->stream));
             pp.cur = G.pp_after;
         } else {
-            FG_CHECK(body());
+            // the steps in chunks: the first as long as this cycle of the
+            // previous solve on the context (the corrections of a time loop
+            // repeat their cycles' lengths), then chunks of CHUNK, reading the
+            // device's `done` flag after each, so that few steps past the end
+            // of the cycle are launched (those return at once, but still cost
+            // a dispatch of every kernel of a step)
+            constexpr int CHUNK = 6;
+            FG_CHECK(start());
+            const int slot = cycles < FG_CYCLE_MEMO ? cycles : FG_CYCLE_MEMO - 1;
+            int first = c->cycle_steps[slot];
+            first = first > 0 ? (first < kmax ? first : kmax) : kmax;
+            int i0 = 0;
+            for (int i1 = first;; i1 = (i0 + CHUNK < kmax ? i0 + CHUNK : kmax)) {
+                FG_CHECK(steps(i0, i1));
+                i0 = i1;
+                if (i0 >= kmax) break;
+                int done = 0;
+                FG_HIP(hipMemcpyAsync(&done, &st->done, sizeof(int), hipMemcpyDeviceToHost,
+                                      c->stream));
+                FG_HIP(hipStreamSynchronize(c->stream));
+                if (done) break;
+            }
+            FG_LAUNCH(k_backsub, 1, 1, 0, c->stream, st);
         }
         if (ir)
             FG_LAUNCH(k_xupdate<float>, g, FG_TPB, 0, c->stream, n, X, (const float *)U, ld, st);
         else
             FG_LAUNCH(k_xupdate<double>, g, FG_TPB, 0, c->stream, n, X, (const double *)U, ld,
                       st);
-        cycles++;
         FG_CHECK(read_state(c, st, hs));
+        if (!use_graph) {
+            const int slot = cycles < FG_CYCLE_MEMO ? cycles : FG_CYCLE_MEMO - 1;
+            c->cycle_steps[slot] = hs.steps - steps_before;
+        }
+        cycles++;
         if (o->rho_hist && o->hist_cap > 0)
             for (int s = steps_before; s < hs.steps && s < o->hist_cap; s++)
                 o->rho_hist[s] = hs.rhoh[s - steps_before];

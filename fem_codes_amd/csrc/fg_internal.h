@@ -94,7 +94,12 @@ struct fg_ctx {
     size_t ws_bytes = 0;
     unsigned long long ws_gen = 0;   // bumped when the workspace moves or is freed
     fg_cycle_graph cg;               // the last captured GMRES cycle
+    // Arnoldi steps of the first FG_CYCLE_MEMO restart cycles of the last
+    // kernel-engine solve (the last slot: every later cycle), the first launch
+    // chunk of the same cycle of the next solve
+    int cycle_steps[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
+constexpr int FG_CYCLE_MEMO = 8;
 
 // the context's solver workspace, at least `bytes` long (256-byte aligned);
 // growing it waits for the stream first.  One user at a time: the solvers
