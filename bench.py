@@ -954,6 +954,25 @@ def euler_leg(F, ctx, args, n=1024, seed=20260305):
     res = {"n": n, "elements": nel, "equations": neq, "element_build_ms": min(bt) * 1e3,
            "element_matrix_bytes": 144 * 8 * nel, "precond": "BlockDiag", "kmax": 30,
            "tol": 1e-8}
+    if not args.no_autotune:
+        # the FP32 12x12 product of the inner cycle: elements per block x
+        # ordering x schedule (blocks of 512 do not fit the LDS at k = 12)
+        from fem_codes_amd import autotune
+        key, tms, op_t = autotune.tune_ebe(ctx, neq, lm, 12, A, fp32=True, reps=10,
+                                           candidates=(256, 512), pipelines=(0, 2, 3),
+                                           orderings=autotune.ORDERINGS)
+        res["autotune"] = {"product": autotune.key_str(key),
+                           "ms": {autotune.key_str(k): v for k, v in tms.items()}}
+        log("euler autotune FP32 12x12 product ms:", {autotune.key_str(k): round(v, 4)
+                                                      for k, v in tms.items()},
+            "->", autotune.key_str(key))
+        if key == (256, F.ORDER_BFS, 0):
+            op_t.close()            # the default: keep the operator built above
+        else:
+            pc.close()
+            op.close()
+            op = op_t
+            pc = F.Preconditioner.blockdiag(op, nn, v, ids32)
     P64 = ctx.to_device(rng.standard_normal(neq + 1))
     Q64 = ctx.empty(neq + 1)
     P32 = ctx.to_device(rng.standard_normal(neq + 1).astype(np.float32))
@@ -976,8 +995,10 @@ def euler_leg(F, ctx, args, n=1024, seed=20260305):
     res["rows_trial"] = trial
     log("euler rows VALU vs MFMA", trial)
     ms_main = op.time_apply(P32, Q32, reps=20, main_only=True)
+    pipe = getattr(op, "pipeline", 0)
     res["roofline"] = roofline(ms_main, op.algorithmic_bytes(True),
                                "k_ebe_apply_mfma12<float, 0>" if trial["kept"] == "mfma"
+                               else "k_ebe_apply_pipe<12, 1, float, 0>" if pipe
                                else ebe_kernel_name(12, True))
     res["roofline"]["traffic"] = leg_traffic("euler", res["roofline"]["kernel"])
     res["fp32_ir_orth"] = args.orth
