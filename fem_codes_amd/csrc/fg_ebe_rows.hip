@@ -106,21 +106,30 @@ struct ElemRowSlot {
     }
 };
 
-// four slices (one per wavefront) per workgroup
+// four slices (one per wavefront) per workgroup.  Optional per-slice mask
+// (the batched solver): slice s belongs to problem sprob[s] (-1: rows of
+// several problems), whose flag pflag[sprob[s] * pstride] != 0 skips it.
 template <typename T, int K>
 __global__ __launch_bounds__(FG_TPB) void k_ebe_rows(int nslice, const int *__restrict__ soff,
                                                      const unsigned *__restrict__ idx,
                                                      const short *__restrict__ idx2,
                                                      const T *__restrict__ V,
                                                      T *__restrict__ P, T *__restrict__ Q,
-                                                     int neq, const int *skip)
+                                                     int neq, const int *skip,
+                                                     const int *__restrict__ sprob,
+                                                     const int *pflag, int pstride)
 {
     if (skip && *skip) return;
     constexpr int SPW = FG_TPB / FG_SELL_W;
     const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
                   (int)(threadIdx.x / FG_SELL_W);
     const int lane = (int)(threadIdx.x % FG_SELL_W);
-    if (s < nslice) {
+    bool run = s < nslice;
+    if (run && sprob) {
+        const int pb = sprob[s];
+        run = pb < 0 || pflag[(long long)pb * pstride] == 0;
+    }
+    if (run) {
         const int r = s * FG_SELL_W + lane;
         // lanes past the last equation use the sink; all their slots are padding
         const int rc = r < neq ? r : neq;
@@ -321,6 +330,12 @@ int fg_ebe_rows_refresh(fg_op *op)
 
 int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 {
+    return fg_ebe_rows_apply_masked(op, fp32, P, Q, skip, nullptr, nullptr, 0);
+}
+
+int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *skip,
+                             const int *sprob, const int *pflag, int pstride)
+{
     fg_ctx *c = op->ctx;
     const int K = op->k;
     FG_CHECK(fg_ebe_rows_refresh(op));
@@ -330,20 +345,20 @@ int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
         if (K == 3)
             FG_LAUNCH((k_ebe_rows<float, 3>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
                       op->d_ridx, op->d_ridx2, (const float *)op->d_sv32, (float *)P,
-                      (float *)Q, op->neq, skip);
+                      (float *)Q, op->neq, skip, sprob, pflag, pstride);
         else
             FG_LAUNCH((k_ebe_rows<float, 4>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
                       op->d_ridx, op->d_ridx2, (const float *)op->d_sv32, (float *)P,
-                      (float *)Q, op->neq, skip);
+                      (float *)Q, op->neq, skip, sprob, pflag, pstride);
     } else {
         if (K == 3)
             FG_LAUNCH((k_ebe_rows<double, 3>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
                       op->d_ridx, op->d_ridx2, (const double *)op->d_sv64, (double *)P,
-                      (double *)Q, op->neq, skip);
+                      (double *)Q, op->neq, skip, sprob, pflag, pstride);
         else
             FG_LAUNCH((k_ebe_rows<double, 4>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
                       op->d_ridx, op->d_ridx2, (const double *)op->d_sv64, (double *)P,
-                      (double *)Q, op->neq, skip);
+                      (double *)Q, op->neq, skip, sprob, pflag, pstride);
     }
     return FG_OK;
 }

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "fg_internal.h"
+#include "fg_sell.h"
 
 namespace {
 
@@ -58,6 +59,9 @@ struct BRec {
     int fin;
     int steps;
 };
+
+// the masked product reads st[p].done with a stride of whole ints
+static_assert(sizeof(BState) % sizeof(int) == 0, "BState size in ints");
 
 __device__ __forceinline__ double &Hij(BState *s, int j, int i) { return s->H[j * LDH + i]; }
 
@@ -558,7 +562,7 @@ struct Parts {
 template <typename TB>
 int b_cycle(fg_ctx *c, fg_op *op, fg_prec *pc, const Chunks &C, int nch, int np, BState *st,
             TB *U, long long ld, TB *W, const double *W64, double *X, int kmax, int lmax, int ir,
-            Parts &pp, double *cgparts)
+            Parts &pp, double *cgparts, const int *sprob)
 {
     const int n = op->neq, fp32 = sizeof(TB) == 4;
     const TB *invD = nullptr;
@@ -573,7 +577,13 @@ int b_cycle(fg_ctx *c, fg_op *op, fg_prec *pc, const Chunks &C, int nch, int np,
     FG_LAUNCH(k_b_scale0<TB>, nch, FG_TPB, 0, c->stream, C, (const BState *)st, W64, U, n);
     for (int i = 0; i < kmax; i++) {
         TB *Ui = U + (size_t)i * ld;
-        FG_CHECK(fg_op_apply_any(op, fp32, Ui, W, nullptr));
+        // the product skips the slices of problems whose cycle has ended
+        // (their rows of W are not read again in this cycle)
+        if (sprob && fg_ebe_rows_used(op, fp32))
+            FG_CHECK(fg_ebe_rows_apply_masked(op, fp32, Ui, W, nullptr, sprob, &st->done,
+                                              (int)(sizeof(BState) / sizeof(int))));
+        else
+            FG_CHECK(fg_op_apply_any(op, fp32, Ui, W, nullptr));
         if (pc && !diag) FG_CHECK(fg_prec_apply_any(pc, fp32, W, W, nullptr));
         if (cgparts) {
             // CGS: all coefficients from one pass, one update pass
@@ -682,7 +692,10 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
             if (g) hipGraphDestroy(g);
         }
     } wk;
-    const int ntab = 3 * (nch ? nch : 1) + 2 * np;
+    // slice -> problem (equations 64s .. 64s+63; -1 when they belong to
+    // several problems), for the masked element-row product
+    const int nsl = (n + FG_SELL_W - 1) / FG_SELL_W;
+    const int ntab = 3 * (nch ? nch : 1) + 2 * np + nsl;
     // buffers carved from the context's solver workspace (kept between calls)
     fg_carver cv;
     const size_t o_st = cv.take(sizeof(BState) * np), o_rec = cv.take(sizeof(BRec) * np);
@@ -709,11 +722,18 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
         std::copy(hcp.begin(), hcp.end(), t + 2 * nch);
         std::copy(hpc0.begin(), hpc0.end(), t + 3 * nch);
         std::copy(hpc1.begin(), hpc1.end(), t + 3 * nch + np);
+        int *sp = t + 3 * nch + 2 * np;
+        for (int q = 0, p = 0; q < nsl; q++) {
+            const int r0 = q * FG_SELL_W, r1 = std::min(n, r0 + FG_SELL_W) - 1;
+            while (p < np && seg[p + 1] <= r0) p++;      // problem of r0
+            sp[q] = seg[p + 1] > r1 ? p : -1;
+        }
         FG_HIP(hipMemcpyAsync(wk.tab, tab.data(), sizeof(int) * ntab, hipMemcpyHostToDevice,
                               c->stream));
         FG_HIP(hipStreamSynchronize(c->stream));
     }
     Chunks C{wk.tab, wk.tab + nch, wk.tab + 2 * nch, wk.tab + 3 * nch, wk.tab + 3 * nch + np};
+    const int *sprob = wk.tab + 3 * nch + 2 * np;
     FG_HIP(hipMemsetAsync(wk.U, 0, sb * ld * (kmax + 1), c->stream));
     FG_HIP(hipMemsetAsync(wk.W, 0, sb * ld, c->stream));
     FG_HIP(hipMemsetAsync(wk.W64, 0, sizeof(double) * ld, c->stream));
@@ -746,10 +766,10 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
                 FG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int rc = ir ? b_cycle<float>(c, op, pc, C, nch, np, wk.st, (float *)wk.U, ld,
                                              (float *)wk.W, wk.W64, X, kmax, o->lmax, 1, pp,
-                                             cgparts)
+                                             cgparts, sprob)
                             : b_cycle<double>(c, op, pc, C, nch, np, wk.st, (double *)wk.U, ld,
                                               (double *)wk.W, wk.W64, X, kmax, o->lmax, 0, pp,
-                                              nullptr);
+                                              nullptr, sprob);
                 hipError_t e = hipStreamEndCapture(c->stream, &wk.g);
                 FG_CHECK(rc);
                 if (e != hipSuccess) {
@@ -761,10 +781,12 @@ extern "C" int fg_gmres_batched(fg_ctx *c, fg_op *op, fg_prec *pc, int nprob, co
             FG_HIP(hipGraphLaunch(wk.ge, c->stream));
         } else if (ir) {
             FG_CHECK(b_cycle<float>(c, op, pc, C, nch, np, wk.st, (float *)wk.U, ld,
-                                    (float *)wk.W, wk.W64, X, kmax, o->lmax, 1, pp, cgparts));
+                                    (float *)wk.W, wk.W64, X, kmax, o->lmax, 1, pp, cgparts,
+                                    sprob));
         } else {
             FG_CHECK(b_cycle<double>(c, op, pc, C, nch, np, wk.st, (double *)wk.U, ld,
-                                     (double *)wk.W, wk.W64, X, kmax, o->lmax, 0, pp, nullptr));
+                                     (double *)wk.W, wk.W64, X, kmax, o->lmax, 0, pp, nullptr,
+                                     sprob));
         }
         iter++;
     }

@@ -333,6 +333,11 @@ inline bool fg_ede_rows_used(const fg_op *op, int fp32)
 }
 int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos);
 int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
+// the element-row product skipping the slices whose problem is flagged:
+// slice s (equations 64s .. 64s+63) belongs to problem sprob[s] (-1: to
+// several), skipped when pflag[sprob[s] * pstride] != 0
+int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *skip,
+                             const int *sprob, const int *pflag, int pstride);
 // AoS element values (device) -> blocked layout and rows in one pass
 int fg_ebe_rows_set_values(fg_op *op, const double *A);
 // refresh row copies that lag the blocked values (element and edge rows)
