@@ -104,6 +104,7 @@ struct ElemRowSlot {
         }
         return e.w == ROW_PAD ? T(0) : s;
     }
+    __device__ __forceinline__ T add(T acc, const Rec &e) const { return acc + term(e); }
 };
 
 // four slices (one per wavefront) per workgroup.  Optional per-slice mask
@@ -218,6 +219,57 @@ __global__ void k_ebe_scatter_rows(long long n, const double *__restrict__ A,
     }
 }
 
+// the diagonal entry of each element row of an equation: value column lr of
+// its slot (padding adds nothing)
+template <int K>
+struct DiagRow {
+    using T = double;
+    struct Rec {
+        unsigned w;
+        long long q;
+        double v;
+    };
+    const unsigned *__restrict__ idx;
+    const double *__restrict__ V;
+    long long vshift, wstride;
+    __device__ __forceinline__ Rec load(long long q) const
+    {
+        Rec r;
+        r.w = idx[q];
+        r.q = q;
+        r.v = 0.0;
+        return r;
+    }
+    __device__ __forceinline__ void gather(Rec &r) const
+    {
+        r.v = V[r.q + vshift + (long long)(r.w & 3u) * wstride];
+    }
+    __device__ __forceinline__ double add(double acc, const Rec &r) const
+    {
+        return r.w == ROW_PAD ? acc : acc + r.v;
+    }
+};
+
+// D[g] = sum over g's element rows, in element order, of their diagonal
+// entries (Diag_precond_EBE_setup.c:18-24), from the rows
+template <int K>
+__global__ __launch_bounds__(FG_TPB) void k_ebe_rows_diag(int nslice, const int *__restrict__ soff,
+                                                          const unsigned *__restrict__ idx,
+                                                          const double *__restrict__ V,
+                                                          double *__restrict__ D, int neq)
+{
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
+                  (int)(threadIdx.x / FG_SELL_W);
+    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    if (s >= nslice) return;
+    const int o0 = soff[s], w = soff[s + 1] - o0;
+    const DiagRow<K> row{idx, V, (long long)(K - 1) * o0 * FG_SELL_W, (long long)w * FG_SELL_W};
+    const double d = fg_sell_row(row, o0, w, lane);
+    const int r = s * FG_SELL_W + lane;
+    if (r < neq) D[r] = d;
+}
+
 template <typename T>
 int upload(const std::vector<T> &v, T **d)
 {
@@ -325,6 +377,21 @@ int fg_ebe_rows_refresh(fg_op *op)
                   op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
                   op->d_sv64, op->d_sv32);
     op->rows_stale = 0;
+    return FG_OK;
+}
+
+int fg_ebe_rows_diag(fg_op *op, double *D)
+{
+    FG_CHECK(fg_ebe_rows_refresh(op));
+    fg_ctx *c = op->ctx;
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int g = (op->nslice + SPW - 1) / SPW;
+    if (op->k == 3)
+        FG_LAUNCH(k_ebe_rows_diag<3>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                  op->d_ridx, (const double *)op->d_sv64, D, op->neq);
+    else
+        FG_LAUNCH(k_ebe_rows_diag<4>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                  op->d_ridx, (const double *)op->d_sv64, D, op->neq);
     return FG_OK;
 }
 

@@ -91,6 +91,7 @@ struct HalfEdgeSlot {
         const T oth = r.v.y * r.p;
         return r.c < 0 ? T(0) : own + oth;
     }
+    __device__ __forceinline__ T add(T acc, const Rec &r) const { return acc + term(r); }
 };
 
 // four slices (one per wavefront) per workgroup

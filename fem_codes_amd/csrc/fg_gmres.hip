@@ -798,15 +798,16 @@ extern "C" int fg_gmres(fg_ctx *c, fg_op *op, fg_prec *pc, const double *B, doub
         } else {
             // the steps in chunks: the first as long as this cycle of the
             // previous solve on the context (the corrections of a time loop
-            // repeat their cycles' lengths), then chunks of CHUNK, reading the
-            // device's `done` flag after each, so that few steps past the end
-            // of the cycle are launched (those return at once, but still cost
-            // a dispatch of every kernel of a step)
-            constexpr int CHUNK = 6;
+            // repeat their cycles' lengths; FIRST0 when there is none), then
+            // chunks of CHUNK, reading the device's `done` flag after each, so
+            // that few steps past the end of the cycle are launched (those
+            // return at once, but still cost a dispatch of every kernel of a
+            // step)
+            constexpr int CHUNK = 6, FIRST0 = 12;
             FG_CHECK(start());
             const int slot = cycles < FG_CYCLE_MEMO ? cycles : FG_CYCLE_MEMO - 1;
-            int first = c->cycle_steps[slot];
-            first = first > 0 ? (first < kmax ? first : kmax) : kmax;
+            int first = c->cycle_steps[slot] > 0 ? c->cycle_steps[slot] : FIRST0;
+            first = first < kmax ? first : kmax;
             int i0 = 0;
             for (int i1 = first;; i1 = (i0 + CHUNK < kmax ? i0 + CHUNK : kmax)) {
                 FG_CHECK(steps(i0, i1));

@@ -340,6 +340,8 @@ int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *s
                              const int *sprob, const int *pflag, int pstride);
 // AoS element values (device) -> blocked layout and rows in one pass
 int fg_ebe_rows_set_values(fg_op *op, const double *A);
+// the sums of each equation's element diagonals (Diag setup) from the rows
+int fg_ebe_rows_diag(fg_op *op, double *D);
 // refresh row copies that lag the blocked values (element and edge rows)
 int fg_ebe_rows_refresh(fg_op *op);
 int fg_ede_rows_refresh(fg_op *op);
@@ -347,6 +349,12 @@ int fg_ede_rows_refresh(fg_op *op);
 // graph capture or replay, which must not depend on host-side flags)
 int fg_op_prepare(fg_op *op);
 int fg_ilu_sweep_count(const fg_ilu *u);
+// fg_supg2d_transient_build with the element matrices written into the
+// k = 3 EBE operator of the mesh (blocked values and element rows) instead of
+// an AoS array
+int fg_supg2d_transient_build_op(fg_mesh2d *m, const fg_supg2d_problem *pb, double alpha,
+                                 double dt, const double *U, const double *dU, fg_op *op,
+                                 double *R);
 void fg_ebe_rows_free(fg_op *op);
 // EBE k = 3, 4: the element-row product in use for that precision
 inline bool fg_ebe_rows_used(const fg_op *op, int fp32)

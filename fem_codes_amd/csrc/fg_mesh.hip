@@ -22,8 +22,10 @@
 #include <algorithm>
 #include <vector>
 
+#include "fg_ebe_body.h"
 #include "fg_host.h"
 #include "fg_internal.h"
+#include "fg_sell.h"
 
 
 // ---------------------------------------------------------------------------
@@ -285,6 +287,9 @@ struct fg_mesh2d {
     double *d_x = nullptr, *d_y = nullptr;
     int *d_vertex = nullptr, *d_id = nullptr, *d_type = nullptr;
     int *d_finc_off = nullptr, *d_finc = nullptr;   // equation -> e*3+r, element order
+    // the same incidences as sliced rows (fg_sell.h) for the residual gather
+    int nslice = 0;
+    int *d_isoff = nullptr, *d_isinc = nullptr;
     double *d_Fe = nullptr;                          // [nel*3]
     double *d_Tr = nullptr;                          // [nel*12] transient residual terms
     std::vector<int> h_lm;
@@ -332,6 +337,15 @@ extern "C" int fg_mesh2d_create(fg_ctx *c, int nnodes, int nel, const double *x,
     FG_CHECK(up(type, nnodes, &m->d_type));
     FG_CHECK(up(off.data(), off.size(), &m->d_finc_off));
     FG_CHECK(up(inc.data(), inc.size(), &m->d_finc));
+    {
+        const fg_sell_layout L = fg_sell_plan(m->neq, off.data());
+        std::vector<int> sinc(L.nslot, -1);
+        for (int g = 0; g < m->neq; g++)
+            for (int q = off[g]; q < off[g + 1]; q++) sinc[L.slot(g, q - off[g])] = inc[q];
+        m->nslice = L.nslice;
+        FG_CHECK(up(L.soff.data(), L.soff.size(), &m->d_isoff));
+        FG_CHECK(up(sinc.data(), sinc.size(), &m->d_isinc));
+    }
     FG_HIP(hipMalloc(&m->d_Fe, sizeof(double) * 3 * (nel ? nel : 1)));
     *out = m;
     return FG_OK;
@@ -388,6 +402,7 @@ extern "C" int fg_mesh2d_destroy(fg_mesh2d *m)
     hipStreamSynchronize(m->ctx->stream);
     hipFree(m->d_x); hipFree(m->d_y); hipFree(m->d_vertex); hipFree(m->d_id);
     hipFree(m->d_type); hipFree(m->d_finc_off); hipFree(m->d_finc); hipFree(m->d_Fe);
+    hipFree(m->d_isoff); hipFree(m->d_isinc);
     hipFree(m->d_Tr);
     delete m;
     return FG_OK;
@@ -566,11 +581,56 @@ extern "C" int fg_supg2d_build(fg_mesh2d *m, const fg_supg2d_problem *pb, const 
 // replays them in element order.  Shock capture: CAU with the residual's time
 // derivative (02 CAU_ShockCapture.c:3-30), h = |sqrt(2 Area)|
 // (02 h_shock_2sqrtArea.c:3-6).
+// where an element's 3x3 matrix goes: the AoS array A[9E..] (the reference's
+// MatrixData->A), or straight into an operator's storage -- its blocked
+// values (FP64 and FP32) and, when it has them, its element rows -- the
+// same places fg_op_set_values_dev puts A[9E..]
+struct ElemSink {
+    double *A;
+    const long long *kpos;
+    const int *kstride;
+    double *K64;
+    float *K32;
+    const long long *vpos;
+    const int *wst;
+    double *V64;
+    float *V32;
+    __device__ __forceinline__ void put(int E, const double (&v)[9]) const
+    {
+        if (A) {
+            double *a = A + 9L * E;
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[q] = v[q];
+            return;
+        }
+        const long long p = kpos[E];
+        if (p < 0) return;   // every dof prescribed: not stored
+        const int st = kstride[E];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const long long d = fg_ebe_vidx(p, st, q, 1);
+            K64[d] = v[q];
+            K32[d] = (float)v[q];
+        }
+        if (!vpos) return;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const long long vp = vpos[3L * E + r];
+            if (vp < 0) continue;
+            const long long ws = wst[3L * E + r];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                V64[vp + c * ws] = v[3 * r + c];
+                V32[vp + c * ws] = (float)v[3 * r + c];
+            }
+        }
+    }
+};
+
 __global__ __launch_bounds__(FG_TPB) void k_supg2d_tr(
     int nel, const double *__restrict__ x, const double *__restrict__ y,
     const int *__restrict__ vertex, const double *__restrict__ U, const double *__restrict__ dU,
-    fg_supg2d_problem pb, double alpha, double dt, double *__restrict__ A,
-    double *__restrict__ Tr)
+    fg_supg2d_problem pb, double alpha, double dt, ElemSink out, double *__restrict__ Tr)
 {
     const double third = 1.0 / 3.0, sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
     double kx, ky, gamma;
@@ -662,9 +722,10 @@ __global__ __launch_bounds__(FG_TPB) void k_supg2d_tr(
         ke[6] = D13 + C11 + Cpg13 + R12 + Rpg31 + Csc13;
         ke[7] = D23 + C12 + Cpg23 + R12 + Rpg31 + Csc23;
         ke[8] = -(D13 + D23) + C13 - (Cpg13 + Cpg23) + R11 + Rpg31 - (Csc13 + Csc23);
-        double *a = A + 9L * E;
+        double ae[9];
 #pragma unroll
-        for (int q = 0; q < 9; q++) a[q] = me[q] + alpha * dt * ke[q];
+        for (int q = 0; q < 9; q++) ae[q] = me[q] + alpha * dt * ke[q];
+        out.put(E, ae);
         const double fs = fe1 + fe2 + fe3;
         double load[3];
         load[0] = Area * twelfth * (2 * fe1 + fe2 + fe3) + sixth * tau * fs * (Be_x * y23 + Be_y * x32);
@@ -699,6 +760,69 @@ __global__ void k_gather_R(int neq, const int *__restrict__ off, const int *__re
     }
 }
 
+// the same sums with the incidences as sliced rows: one equation per lane,
+// its incidences' four terms (32 bytes) gathered together, folded in the
+// order above
+typedef double fg_d4 __attribute__((ext_vector_type(4)));
+struct ResidualRow {
+    using T = double;
+    struct Rec {
+        int c;
+        fg_d4 t;
+    };
+    const int *__restrict__ inc;
+    const fg_d4 *__restrict__ Tr4;
+    __device__ __forceinline__ Rec load(long long q) const
+    {
+        Rec r;
+        r.c = __builtin_nontemporal_load(&inc[q]);
+        r.t = fg_d4{0.0, 0.0, 0.0, 0.0};
+        return r;
+    }
+    __device__ __forceinline__ void gather(Rec &r) const { r.t = Tr4[r.c < 0 ? 0 : r.c]; }
+    __device__ __forceinline__ double add(double acc, const Rec &r) const
+    {
+        double a = acc + r.t.x;
+        a = a - r.t.y;
+        a = a - r.t.z;
+        a = a - r.t.w;
+        return r.c < 0 ? acc : a;
+    }
+};
+
+__global__ __launch_bounds__(FG_TPB) void k_gather_R_sliced(int nslice, const int *__restrict__ soff,
+                                                            const int *__restrict__ sinc,
+                                                            const double *__restrict__ Tr,
+                                                            double *__restrict__ R, int neq)
+{
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
+                  (int)(threadIdx.x / FG_SELL_W);
+    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    if (s < nslice) {
+        const int o0 = soff[s];
+        const ResidualRow row{sinc, reinterpret_cast<const fg_d4 *>(Tr)};
+        const double acc = fg_sell_row(row, o0, soff[s + 1] - o0, lane);
+        const int r = s * FG_SELL_W + lane;
+        if (r < neq) R[r] = acc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) R[neq] = 0.0;
+}
+
+static int gather_R(fg_mesh2d *m, double *R)
+{
+    fg_ctx *c = m->ctx;
+    if (m->nslice == 0) {
+        FG_LAUNCH(k_gather_R, 1, FG_TPB, 0, c->stream, m->neq, m->d_finc_off, m->d_finc,
+                  m->d_Tr, R);
+        return FG_OK;
+    }
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    FG_LAUNCH(k_gather_R_sliced, (m->nslice + SPW - 1) / SPW, FG_TPB, 0, c->stream, m->nslice,
+              m->d_isoff, m->d_isinc, m->d_Tr, R, m->neq);
+    return FG_OK;
+}
+
 extern "C" int fg_supg2d_transient_build(fg_mesh2d *m, const fg_supg2d_problem *pb,
                                          double alpha, double dt, const double *U,
                                          const double *dU, double *A, double *R)
@@ -707,12 +831,33 @@ extern "C" int fg_supg2d_transient_build(fg_mesh2d *m, const fg_supg2d_problem *
     FG_ARG(pb->kind >= FG_P_CONST && pb->kind <= FG_P_ROTATE_ABOUT, "problem kind");
     fg_ctx *c = m->ctx;
     if (!m->d_Tr) FG_HIP(hipMalloc(&m->d_Tr, sizeof(double) * 12 * (size_t)(m->nel ? m->nel : 1)));
+    ElemSink out{A, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (m->nel > 0)
         FG_LAUNCH(k_supg2d_tr, fg_vec_grid(c, m->nel), FG_TPB, 0, c->stream, m->nel, m->d_x,
-                  m->d_y, m->d_vertex, U, dU, *pb, alpha, dt, A, m->d_Tr);
-    FG_LAUNCH(k_gather_R, fg_vec_grid(c, m->neq + 1), FG_TPB, 0, c->stream, m->neq,
-              m->d_finc_off, m->d_finc, m->d_Tr, R);
-    return FG_OK;
+                  m->d_y, m->d_vertex, U, dU, *pb, alpha, dt, out, m->d_Tr);
+    return gather_R(m, R);
+}
+
+// the same build with the element matrices written straight into the k = 3
+// EBE operator of the mesh's lm (its element e is the mesh's element e):
+// the time loop's correction, without the AoS array and its scatter pass
+int fg_supg2d_transient_build_op(fg_mesh2d *m, const fg_supg2d_problem *pb, double alpha,
+                                 double dt, const double *U, const double *dU, fg_op *op,
+                                 double *R)
+{
+    FG_ARG(m && pb && U && dU && op && R, "mesh/problem/U/dU/op/R");
+    FG_ARG(op->kind == FG_OP_EBE && op->k == 3 && op->nitems == m->nel && op->neq == m->neq,
+           "the k = 3 EBE operator of the mesh");
+    fg_ctx *c = m->ctx;
+    if (!m->d_Tr) FG_HIP(hipMalloc(&m->d_Tr, sizeof(double) * 12 * (size_t)(m->nel ? m->nel : 1)));
+    ElemSink out{nullptr, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32, op->d_rvpos,
+                 op->d_rwst, op->d_sv64, op->d_sv32};
+    if (m->nel > 0)
+        FG_LAUNCH(k_supg2d_tr, fg_vec_grid(c, m->nel), FG_TPB, 0, c->stream, m->nel, m->d_x,
+                  m->d_y, m->d_vertex, U, dU, *pb, alpha, dt, out, m->d_Tr);
+    // the rows, when the operator has them, were written with the values
+    op->rows_stale = op->d_rvpos ? 0 : 1;
+    return gather_R(m, R);
 }
 
 // ---------------------------------------------------------------------------

@@ -50,11 +50,12 @@ inline fg_sell_layout fg_sell_plan(int nrows, const int *rowptr)
 }
 
 // Issues the loads of L consecutive entries of a row (slot `at` and every
-// 64th after it), gathers, and adds the terms to acc in entry order.  With
-// CLAMP, entries at or beyond `nleft` re-read entry nleft-1 and add nothing
-// (all loads still issued on every path).  Row::load(slot) returns an entry
-// record, Row::gather(rec) issues its dependent loads, Row::term(rec) forms
-// the term (0 for a padding slot).
+// 64th after it), gathers, and folds the entries into acc in entry order.
+// With CLAMP, entries at or beyond `nleft` re-read entry nleft-1 and add
+// nothing (all loads still issued on every path).  Row::load(slot) returns an
+// entry record, Row::gather(rec) issues its dependent loads, Row::add(acc,
+// rec) folds the entry in (acc + its term for a product row; acc itself for
+// a padding slot).
 template <int L, bool CLAMP, typename Row>
 __device__ __forceinline__ typename Row::T fg_sell_chunk(const Row &row, long long at, int nleft,
                                                          typename Row::T acc)
@@ -69,9 +70,8 @@ __device__ __forceinline__ typename Row::T fg_sell_chunk(const Row &row, long lo
     for (int k = 0; k < L; k++) row.gather(e[k]);
 #pragma unroll
     for (int k = 0; k < L; k++) {
-        const typename Row::T t = row.term(e[k]);
-        if (CLAMP) acc = k < nleft ? acc + t : acc;
-        else acc = acc + t;
+        const typename Row::T a = row.add(acc, e[k]);
+        acc = (!CLAMP || k < nleft) ? a : acc;
     }
     return acc;
 }
@@ -128,4 +128,5 @@ struct fg_sell_prod {
     }
     __device__ __forceinline__ void gather(Rec &r) const { r.p = x[r.c < 0 ? 0 : r.c]; }
     __device__ __forceinline__ T term(const Rec &r) const { return r.c < 0 ? T(0) : r.v * r.p; }
+    __device__ __forceinline__ T add(T acc, const Rec &r) const { return acc + term(r); }
 };

@@ -212,19 +212,19 @@ extern "C" int fg_predictor_old_transport(fg_mesh2d *m, const fg_supg2d_problem 
     fg_ctx *c = fg_mesh2d_ctx(m);
     FG_ARG(A->ctx == c && A->kind == FG_OP_EBE && A->k == 3 && A->neq == neq && A->nitems == nel,
            "A must be the k = 3 EBE operator of the mesh's lm");
-    const size_t ne = (size_t)nel * 9;
     double *work = nullptr;
     double *const *wp = &work;
     const double alpha = p->alpha, dt = p->dt;
     // Build_M_K_R_SUPG.c with eval_U_dU.c of the 02 driver; NO_scaling
     auto build = [&](const double *uu, const double *aa, double *R) -> int {
-        double *U = *wp, *dU = U + nn, *ke = dU + nn;
+        double *U = *wp, *dU = U + nn;
         FG_CHECK(fg_mesh2d_eval_U_dU(m, uu, aa, upresc, U, dU));
-        FG_CHECK(fg_supg2d_transient_build(m, pb, alpha, dt, U, dU, ke, R));
-        return fg_op_set_values_dev(A, ke);
+        // the element matrices go straight into A's storage (the values
+        // fg_op_set_values_dev would store from the AoS array)
+        return fg_supg2d_transient_build_op(m, pb, alpha, dt, U, dU, A, R);
     };
-    return predictor_loop(c, neq, 2 * (size_t)nn + (ne ? ne : 1), A, pc, p, gm, u, corr_hist,
-                          hist_cap, info, &work, build);
+    return predictor_loop(c, neq, 2 * (size_t)nn + 1, A, pc, p, gm, u, corr_hist, hist_cap,
+                          info, &work, build);
 }
 
 // the Euler driver's loop over its element (k = 12) or edge (k = 8, edges !=
