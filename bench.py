@@ -790,6 +790,8 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
     ilu["exact_f64_ms"] = exact_ms
     res["ilu_apply"] = ilu
     log("ss3d ILU(0) apply", ilu)
+    if args.no_solve:
+        return res
     Xs = {}
     for prec, tag in ((F.GMRES_FP64, "fp64"), (F.GMRES_FP32_IR, "fp32_ir")):
         X = ctx.empty(neq + 1)
@@ -1102,7 +1104,8 @@ def main():
     ap.add_argument("--legs", default="all",
                     help="comma-separated subset of " + ",".join(LEGS) + " (default all)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--no-solve", action="store_true", help="headline leg: products only")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="headline and 3D legs: products (and the 3D ILU(0) applies) only")
     ap.add_argument("--no-sequential", action="store_true",
                     help="batched leg: skip the one-by-one comparison")
     ap.add_argument("--no-graph", action="store_true",

@@ -48,7 +48,10 @@ constexpr int OFF2_SINK = -32768;           // 16-bit field (k = 4, third column
 __device__ __forceinline__ int field_a(unsigned w) { return ((int)(w << 15)) >> 17; }
 __device__ __forceinline__ int field_b(unsigned w) { return ((int)w) >> 17; }
 
-// one element row of an equation's sliced list
+// one element row of an equation's sliced list.  The index and value
+// pointers start at the lane-0 slot of the wavefront's slice (wave-uniform
+// bases): a load is a scalar base plus the lane's 32-bit offset, entry k at
+// lane + 64 k.
 template <typename TV, int K>
 struct ElemRowSlot {
     using T = TV;
@@ -60,29 +63,31 @@ struct ElemRowSlot {
     };
     const unsigned *__restrict__ idx;
     const short *__restrict__ idx2;   // k = 4: the third other column
-    const T *__restrict__ V;
+    const T *__restrict__ V[K];       // value column c of the slice
     const T *__restrict__ P;
-    long long vshift;   // value column 0 of slot q at q + vshift
-    long long wstride;  // between value columns: w * 64
     int r;              // the lane's equation (clamped to the sink for idle lanes)
     T pown;
 
-    __device__ __forceinline__ Rec load(long long q) const
+    __device__ __forceinline__ Rec key(unsigned q) const
     {
         Rec e;
         e.w = __builtin_nontemporal_load(&idx[q]);
         e.c2 = 0;
         if constexpr (K == 4) e.c2 = (int)__builtin_nontemporal_load(&idx2[q]);
 #pragma unroll
-        for (int c = 0; c < K; c++)
-            e.v[c] = __builtin_nontemporal_load(&V[q + vshift + c * wstride]);
-#pragma unroll
         for (int c = 0; c < K - 1; c++) e.po[c] = T(0);
         return e;
     }
+    __device__ __forceinline__ void stream(Rec &e, unsigned q) const
+    {
+#pragma unroll
+        for (int c = 0; c < K; c++) e.v[c] = __builtin_nontemporal_load(&V[c][q]);
+    }
     __device__ __forceinline__ T col_p(int off, int sink) const
     {
-        const T t = P[off == sink ? r : r + off];
+        // a 32-bit byte offset from P (the rows are built for neq < 2^28)
+        const unsigned b = (unsigned)(off == sink ? r : r + off) * (unsigned)sizeof(T);
+        const T t = *(const T *)((const char *)P + b);
         return off == sink ? T(0) : t;
     }
     __device__ __forceinline__ void gather(Rec &e) const
@@ -122,23 +127,30 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_rows(int nslice, const int *__re
 {
     if (skip && *skip) return;
     constexpr int SPW = FG_TPB / FG_SELL_W;
-    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
-                  (int)(threadIdx.x / FG_SELL_W);
-    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    const int s = fg_sell_slice(fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW);
+    const unsigned lane = threadIdx.x % FG_SELL_W;
     bool run = s < nslice;
     if (run && sprob) {
         const int pb = sprob[s];
         run = pb < 0 || pflag[(long long)pb * pstride] == 0;
     }
     if (run) {
-        const int r = s * FG_SELL_W + lane;
+        const int r = s * FG_SELL_W + (int)lane;
         // lanes past the last equation use the sink; all their slots are padding
         const int rc = r < neq ? r : neq;
-        const T pown = P[rc];
         const int o0 = soff[s], w = soff[s + 1] - o0;
-        const ElemRowSlot<T, K> row{idx, idx2, V, P, (long long)(K - 1) * o0 * FG_SELL_W,
-                                    (long long)w * FG_SELL_W, rc, pown};
-        const T acc = fg_sell_row(row, o0, w, lane);
+        const long long q0 = (long long)o0 * FG_SELL_W;
+        ElemRowSlot<T, K> row;
+        row.idx = idx + q0;
+        row.idx2 = K == 4 ? idx2 + q0 : idx2;
+#pragma unroll
+        for (int c = 0; c < K; c++) row.V[c] = V + K * q0 + (long long)c * w * FG_SELL_W;
+        row.P = P;
+        row.r = rc;
+        row.pown = P[rc];
+        // tetrahedra: rows of about 24 entries in chunks of 12 (two load
+        // bursts), measured faster than chunks of 8 in both precisions
+        const T acc = fg_sell_run<K == 4 ? 12 : FG_SELL_CHUNK>(row, lane, w);
         if (r < neq) Q[r] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -232,7 +244,7 @@ struct DiagRow {
     const unsigned *__restrict__ idx;
     const double *__restrict__ V;
     long long vshift, wstride;
-    __device__ __forceinline__ Rec load(long long q) const
+    __device__ __forceinline__ Rec key(long long q) const
     {
         Rec r;
         r.w = idx[q];
@@ -240,6 +252,7 @@ struct DiagRow {
         r.v = 0.0;
         return r;
     }
+    __device__ __forceinline__ void stream(Rec &, long long) const {}
     __device__ __forceinline__ void gather(Rec &r) const
     {
         r.v = V[r.q + vshift + (long long)(r.w & 3u) * wstride];
@@ -287,6 +300,8 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
 {
     const int neq = op->neq, nel = op->nitems, K = op->k;
     if (op->kind != FG_OP_EBE || (K != 3 && K != 4) || neq == 0) return FG_OK;
+    // the product addresses P by a 32-bit byte offset
+    if (neq >= (1 << 28)) return FG_OK;
     std::vector<int> cnt((size_t)neq + 1, 0);
     for (int e = 0; e < nel; e++) {
         if (kpos[e] < 0) continue;

@@ -72,13 +72,16 @@ struct HalfEdgeSlot {
     const T *__restrict__ P;
     T pown;
     int neq;
-    __device__ __forceinline__ Rec load(long long q) const
+    __device__ __forceinline__ Rec key(long long q) const
     {
         Rec r;
         r.c = __builtin_nontemporal_load(&col[q]);
-        r.v = __builtin_nontemporal_load(&val[q]);
         r.p = T(0);
         return r;
+    }
+    __device__ __forceinline__ void stream(Rec &r, long long q) const
+    {
+        r.v = __builtin_nontemporal_load(&val[q]);
     }
     __device__ __forceinline__ void gather(Rec &r) const
     {

@@ -772,13 +772,14 @@ struct ResidualRow {
     };
     const int *__restrict__ inc;
     const fg_d4 *__restrict__ Tr4;
-    __device__ __forceinline__ Rec load(long long q) const
+    __device__ __forceinline__ Rec key(long long q) const
     {
         Rec r;
         r.c = __builtin_nontemporal_load(&inc[q]);
         r.t = fg_d4{0.0, 0.0, 0.0, 0.0};
         return r;
     }
+    __device__ __forceinline__ void stream(Rec &, long long) const {}
     __device__ __forceinline__ void gather(Rec &r) const { r.t = Tr4[r.c < 0 ? 0 : r.c]; }
     __device__ __forceinline__ double add(double acc, const Rec &r) const
     {

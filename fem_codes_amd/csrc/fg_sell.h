@@ -17,7 +17,7 @@
 #include "fg_internal.h"
 
 constexpr int FG_SELL_W = 64;       // rows per slice (one wavefront)
-constexpr int FG_SELL_CHUNK = 8;    // entries per unrolled chunk of a wide row
+constexpr int FG_SELL_CHUNK = 8;    // entries per unrolled chunk of a wide row (default)
 
 // host: slice layout of rows with entry counts rowptr[r+1]-rowptr[r]
 struct fg_sell_layout {
@@ -52,20 +52,29 @@ inline fg_sell_layout fg_sell_plan(int nrows, const int *rowptr)
 // Issues the loads of L consecutive entries of a row (slot `at` and every
 // 64th after it), gathers, and folds the entries into acc in entry order.
 // With CLAMP, entries at or beyond `nleft` re-read entry nleft-1 and add
-// nothing (all loads still issued on every path).  Row::load(slot) returns an
-// entry record, Row::gather(rec) issues its dependent loads, Row::add(acc,
-// rec) folds the entry in (acc + its term for a product row; acc itself for
-// a padding slot).
-template <int L, bool CLAMP, typename Row>
-__device__ __forceinline__ typename Row::T fg_sell_chunk(const Row &row, long long at, int nleft,
+// nothing (all loads still issued on every path).  Row::key(slot) returns an
+// entry record holding the entry's index loads (issued first, for every
+// entry), Row::stream(rec, slot) issues the loads that need no index (the
+// values), Row::gather(rec) the loads that do, Row::add(acc, rec) folds the
+// entry in (acc + its term for a product row; acc itself for a padding
+// slot).
+template <int L, bool CLAMP, typename Row, typename I>
+__device__ __forceinline__ typename Row::T fg_sell_chunk(const Row &row, I at, int nleft,
                                                          typename Row::T acc)
 {
     typename Row::Rec e[L];
+    I q[L];
 #pragma unroll
     for (int k = 0; k < L; k++) {
         const int kk = CLAMP ? (k < nleft ? k : nleft - 1) : k;
-        e[k] = row.load(at + (long long)kk * FG_SELL_W);
+        q[k] = at + (I)kk * FG_SELL_W;
+        e[k] = row.key(q[k]);
     }
+#pragma unroll
+    for (int k = 0; k < L; k++) row.stream(e[k], q[k]);
+    // every load of the chunk that needs no index is issued before the first
+    // wait for one (the scheduler would otherwise hold value loads back)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < L; k++) row.gather(e[k]);
 #pragma unroll
@@ -76,14 +85,15 @@ __device__ __forceinline__ typename Row::T fg_sell_chunk(const Row &row, long lo
     return acc;
 }
 
-// The row sum of lane `lane` of a slice of width w starting at slot column o0:
-// rows of up to 8 entries fully unrolled (one load burst), wider rows in
-// chunks of 8.
-template <typename Row>
-__device__ __forceinline__ typename Row::T fg_sell_row(const Row &row, int o0, int w, int lane)
+// The row sum of the lane whose first entry is slot `at` of a slice of
+// width w: rows of up to 8 entries fully unrolled (one load burst), wider
+// rows in chunks of C.  `at` is an absolute slot (long long) or, for a Row
+// whose pointers start at its slice (a wave-uniform base, so that every load
+// is a scalar base plus the lane's 32-bit offset), the lane (unsigned).
+template <int C = FG_SELL_CHUNK, typename Row, typename I>
+__device__ __forceinline__ typename Row::T fg_sell_run(const Row &row, I at, int w)
 {
     using T = typename Row::T;
-    const long long at = (long long)o0 * FG_SELL_W + lane;
     switch (w) {
     case 0: return T(0);
     case 1: return fg_sell_chunk<1, false>(row, at, 1, T(0));
@@ -96,12 +106,26 @@ __device__ __forceinline__ typename Row::T fg_sell_row(const Row &row, int o0, i
     case 8: return fg_sell_chunk<8, false>(row, at, 8, T(0));
     default: {
         T acc = T(0);
-        for (int k0 = 0; k0 < w; k0 += FG_SELL_CHUNK)
-            acc = fg_sell_chunk<FG_SELL_CHUNK, true>(row, at + (long long)k0 * FG_SELL_W, w - k0,
-                                                     acc);
+        for (int k0 = 0; k0 < w; k0 += C)
+            acc = fg_sell_chunk<C, true>(row, at + (I)k0 * FG_SELL_W, w - k0, acc);
         return acc;
     }
     }
+}
+
+// The row sum of lane `lane` of a slice of width w starting at slot column o0
+template <typename Row>
+__device__ __forceinline__ typename Row::T fg_sell_row(const Row &row, int o0, int w, int lane)
+{
+    return fg_sell_run(row, (long long)o0 * FG_SELL_W + lane, w);
+}
+
+// The wavefront's slice: s0 (wave-uniform) plus the wave's index in the
+// workgroup, read back as a scalar so that the slice's offsets and width,
+// and the branch on the width, are scalar.
+__device__ __forceinline__ int fg_sell_slice(int s0)
+{
+    return __builtin_amdgcn_readfirstlane(s0 + (int)(threadIdx.x / FG_SELL_W));
 }
 
 // the entry kind of a sparse product row: a column (-1: padding) and a
@@ -118,13 +142,18 @@ struct fg_sell_prod {
     const T *__restrict__ x;
     // columns and values are read once per product: non-temporal, so that
     // they do not evict the x entries neighbouring slices gather again
-    __device__ __forceinline__ Rec load(long long q) const
+    template <typename I>
+    __device__ __forceinline__ Rec key(I q) const
     {
         Rec r;
         r.c = __builtin_nontemporal_load(&col[q]);
-        r.v = __builtin_nontemporal_load(&val[q]);
         r.p = T(0);
         return r;
+    }
+    template <typename I>
+    __device__ __forceinline__ void stream(Rec &r, I q) const
+    {
+        r.v = __builtin_nontemporal_load(&val[q]);
     }
     __device__ __forceinline__ void gather(Rec &r) const { r.p = x[r.c < 0 ? 0 : r.c]; }
     __device__ __forceinline__ T term(const Rec &r) const { return r.c < 0 ? T(0) : r.v * r.p; }
