@@ -22,12 +22,16 @@
 // adds the sums in element order starting from 0: Q is the reference's ebemv
 // bit for bit (FMA contraction disabled).  No LDS, no halo pass, no atomics.
 //
-// The values of a slice are stored contiguously (column c of the slice's
-// entry k at ((k_dofs*o0 + c*w + k)*64 + lane) for a slice of width w
-// starting at slot column o0).  The blocked layout (fg_ebe.hip) stays the
-// primary copy of the values; the rows are refreshed from it by one kernel
-// before the first product after any change.  A mesh whose column offsets do
-// not fit the fields keeps the blocked product only.
+// The values of a slice are stored contiguously: column c of a slice of
+// width w starting at slot column o0 fills blocks k_dofs*o0 + c*w .. + w-1 of
+// 64 values, entry k of the lane at k*64 + lane, or for FP64 triangles with
+// the entries paired (entries 2j and 2j+1 side by side, one 16-byte load for
+// both; fg_rows_shift): 8% faster for the FP64 triangle product, slower for
+// FP32 and for FP64 tetrahedra (rows of 24 entries in chunks of 12), as
+// measured.  The blocked layout (fg_ebe.hip) stays the primary copy of the
+// values; the rows are refreshed from it by one kernel before the first
+// product after any change.  A mesh whose column offsets do not fit the
+// fields keeps the blocked product only.
 #include <vector>
 
 #include "fg_ebe_body.h"
@@ -66,6 +70,7 @@ struct ElemRowSlot {
     const T *__restrict__ V[K];       // value column c of the slice
     const T *__restrict__ P;
     int r;              // the lane's equation (clamped to the sink for idle lanes)
+    int w;              // the slice width
     T pown;
 
     __device__ __forceinline__ Rec key(unsigned q) const
@@ -80,8 +85,13 @@ struct ElemRowSlot {
     }
     __device__ __forceinline__ void stream(Rec &e, unsigned q) const
     {
+        // FP64: the entry-paired position (consecutive entries of a lane at
+        // consecutive addresses, merged into 16-byte loads)
+        const unsigned p = sizeof(T) == 8 ? q + fg_rows_shift(K, (int)(q / FG_SELL_W),
+                                                              (int)(q % FG_SELL_W), w)
+                                          : q;
 #pragma unroll
-        for (int c = 0; c < K; c++) e.v[c] = __builtin_nontemporal_load(&V[c][q]);
+        for (int c = 0; c < K; c++) e.v[c] = __builtin_nontemporal_load(&V[c][p]);
     }
     __device__ __forceinline__ T col_p(int off, int sink) const
     {
@@ -147,6 +157,7 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_rows(int nslice, const int *__re
         for (int c = 0; c < K; c++) row.V[c] = V + K * q0 + (long long)c * w * FG_SELL_W;
         row.P = P;
         row.r = rc;
+        row.w = w;
         row.pown = P[rc];
         // tetrahedra: rows of about 24 entries in chunks of 12 (two load
         // bursts), measured faster than chunks of 8 in both precisions
@@ -179,11 +190,12 @@ __global__ void k_ebe_rows_mirror(int nslice, const int *__restrict__ soff,
             const int e = sv / K, lr = sv - e * K;
             const long long p = kpos[e];
             const int st = kstride[e];
+            const int sh = fg_rows_shift(K, k, lane, w);
 #pragma unroll
             for (int c = 0; c < K; c++) {
                 const long long vi = fg_ebe_vidx(p, st, lr * K + c, fg_ebe_vec(K));
                 const long long d = ((long long)K * o0 + (long long)c * w + k) * FG_SELL_W + lane;
-                V64[d] = K64[vi];
+                V64[d + sh] = K64[vi];
                 V32[d] = K32[vi];
             }
         }
@@ -193,7 +205,8 @@ __global__ void k_ebe_rows_mirror(int nslice, const int *__restrict__ soff,
 // AoS element values -> the blocked layout and the rows in one pass; one
 // thread per (element, local row): the row's k values are read contiguously
 // (consecutive threads, consecutive rows), written to their blocked positions
-// and, when the row's dof is free, to its slot (value c at vpos + c * wst)
+// and, when the row's dof is free, to its slot (FP32 value c at vpos + c *
+// stride, FP64 shifted to the entry-paired position)
 template <int K>
 __global__ void k_ebe_scatter_rows(long long n, const double *__restrict__ A,
                                    const long long *__restrict__ kpos,
@@ -211,7 +224,8 @@ __global__ void k_ebe_scatter_rows(long long n, const double *__restrict__ A,
         if (p < 0) continue;
         const int st = kstride[e];
         const long long vp = vpos[t];
-        const int ws = wst[t];
+        const long long ws = fg_rows_wst_stride(wst[t]);
+        const int sh = fg_rows_wst_shift(wst[t]);
         double v[K];
 #pragma unroll
         for (int c = 0; c < K; c++) v[c] = A[t * K + c];
@@ -224,27 +238,28 @@ __global__ void k_ebe_scatter_rows(long long n, const double *__restrict__ A,
         if (vp >= 0) {
 #pragma unroll
             for (int c = 0; c < K; c++) {
-                V64[vp + (long long)c * ws] = v[c];
-                V32[vp + (long long)c * ws] = (float)v[c];
+                V64[vp + c * ws + sh] = v[c];
+                V32[vp + c * ws] = (float)v[c];
             }
         }
     }
 }
 
-// the diagonal entry of each element row of an equation: value column lr of
-// its slot (padding adds nothing)
+// the diagonal entry of each element row of an equation: FP64 value column
+// lr of its slot (padding adds nothing); pointers from the slice's lane-0
+// slot, q = k*64 + lane
 template <int K>
 struct DiagRow {
     using T = double;
     struct Rec {
         unsigned w;
-        long long q;
+        unsigned q;
         double v;
     };
     const unsigned *__restrict__ idx;
-    const double *__restrict__ V;
-    long long vshift, wstride;
-    __device__ __forceinline__ Rec key(long long q) const
+    const double *__restrict__ V;   // the slice's column 0
+    int wstride, w;
+    __device__ __forceinline__ Rec key(unsigned q) const
     {
         Rec r;
         r.w = idx[q];
@@ -252,10 +267,11 @@ struct DiagRow {
         r.v = 0.0;
         return r;
     }
-    __device__ __forceinline__ void stream(Rec &, long long) const {}
+    __device__ __forceinline__ void stream(Rec &, unsigned) const {}
     __device__ __forceinline__ void gather(Rec &r) const
     {
-        r.v = V[r.q + vshift + (long long)(r.w & 3u) * wstride];
+        const int sh = fg_rows_shift(K, (int)(r.q / FG_SELL_W), (int)(r.q % FG_SELL_W), w);
+        r.v = V[(long long)r.q + sh + (long long)(r.w & 3u) * wstride];
     }
     __device__ __forceinline__ double add(double acc, const Rec &r) const
     {
@@ -272,14 +288,14 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_rows_diag(int nslice, const int 
                                                           double *__restrict__ D, int neq)
 {
     constexpr int SPW = FG_TPB / FG_SELL_W;
-    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
-                  (int)(threadIdx.x / FG_SELL_W);
-    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    const int s = fg_sell_slice(fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW);
+    const unsigned lane = threadIdx.x % FG_SELL_W;
     if (s >= nslice) return;
     const int o0 = soff[s], w = soff[s + 1] - o0;
-    const DiagRow<K> row{idx, V, (long long)(K - 1) * o0 * FG_SELL_W, (long long)w * FG_SELL_W};
-    const double d = fg_sell_row(row, o0, w, lane);
-    const int r = s * FG_SELL_W + lane;
+    const long long q0 = (long long)o0 * FG_SELL_W;
+    const DiagRow<K> row{idx + q0, V + K * q0, w * FG_SELL_W, w};
+    const double d = fg_sell_run(row, lane, w);
+    const int r = s * FG_SELL_W + (int)lane;
     if (r < neq) D[r] = d;
 }
 
@@ -312,6 +328,9 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
     }
     for (int r = 0; r < neq; r++) cnt[r + 1] += cnt[r];
     const fg_sell_layout L = fg_sell_plan(neq, cnt.data());
+    // fg_rows_wst holds the column stride w*64 in 24 bits
+    for (int s = 0; s < L.nslice; s++)
+        if (L.soff[s + 1] - L.soff[s] >= (1 << 18)) return FG_OK;
     std::vector<unsigned> idx(L.nslot, ROW_PAD);
     std::vector<short> idx2(K == 4 ? L.nslot : 0, 0);
     std::vector<int> src(L.nslot, -1);
@@ -334,14 +353,15 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
                 off[m] = le[c] == neq ? (m == 2 ? OFF2_SINK : OFF_SINK) : (int)d;
                 m++;
             }
-            const long long q = L.slot(r, fill[r]++);
+            const int kr = fill[r]++;
+            const long long q = L.slot(r, kr);
             idx[q] = (unsigned)a | ((unsigned)(off[0] & 0x7fff) << 2) |
                      ((unsigned)(off[1] & 0x7fff) << 17);
             if (K == 4) idx2[q] = (short)off[2];
             src[q] = e * K + a;
-            const int s0 = r / FG_SELL_W;
+            const int s0 = r / FG_SELL_W, w = L.soff[s0 + 1] - L.soff[s0];
             vpos[(size_t)e * K + a] = q + (long long)(K - 1) * L.soff[s0] * FG_SELL_W;
-            wst[(size_t)e * K + a] = (L.soff[s0 + 1] - L.soff[s0]) * FG_SELL_W;
+            wst[(size_t)e * K + a] = fg_rows_wst(w, fg_rows_shift(K, kr, r % FG_SELL_W, w));
         }
     }
     op->nslice = L.nslice;

@@ -122,6 +122,21 @@ struct fg_carver {
 // FG_OP_EBE2: off-block product of a k = 12 EBE operator + node pass
 enum { FG_OP_EBE = 0, FG_OP_EDE = 1, FG_OP_CSR = 2, FG_OP_EBE2 = 3 };
 
+// ---- element rows (fg_ebe_rows.hip): the values of column c of a slice of
+// width w fill w blocks of 64, entry k of lane l at k*64 + l -- except for
+// FP64 triangles (K = 3), whose entries 2j and 2j+1 of a lane sit side by
+// side, at 2j*64 + 2l + (k & 1), so that one 16-byte load reads both (the
+// last entry of an odd width alone, at k*64 + l).  fg_rows_shift is the FP64
+// position minus the FP32 one.
+__host__ __device__ inline int fg_rows_shift(int K, int k, int l, int w)
+{
+    return (K != 3 || ((w & 1) && k == w - 1)) ? 0 : l - (k & 1) * 63;
+}
+// per element row: the column stride w*64 and the FP64 shift in one int
+__host__ __device__ inline int fg_rows_wst(int w, int shift) { return w * 64 * 128 + shift + 64; }
+__host__ __device__ inline long long fg_rows_wst_stride(int wst) { return wst >> 7; }
+__host__ __device__ inline int fg_rows_wst_shift(int wst) { return (wst & 127) - 64; }
+
 struct fg_op {
     fg_op() { fg_cfg_bump(); }
     ~fg_op() { fg_cfg_bump(); }
@@ -179,8 +194,8 @@ struct fg_op {
     unsigned *d_ridx = nullptr;
     short *d_ridx2 = nullptr;
     int *d_rsrc = nullptr;
-    long long *d_rvpos = nullptr; // [nel*k] value position of row (e, lr), -1: no row
-    int *d_rwst = nullptr;        // [nel*k] stride between its value columns
+    long long *d_rvpos = nullptr; // [nel*k] FP32 value position of row (e, lr), -1: no row
+    int *d_rwst = nullptr;        // [nel*k] fg_rows_wst: column stride, FP64 shift
 
     // ---- systems of at most FG_SMALL_MAX equations: lane-coalesced lists for
     // the one-workgroup solver, entry c of row g at c*neq + g (-1: none) ----

@@ -617,10 +617,11 @@ struct ElemSink {
         for (int r = 0; r < 3; r++) {
             const long long vp = vpos[3L * E + r];
             if (vp < 0) continue;
-            const long long ws = wst[3L * E + r];
+            const long long ws = fg_rows_wst_stride(wst[3L * E + r]);
+            const int sh = fg_rows_wst_shift(wst[3L * E + r]);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                V64[vp + c * ws] = v[3 * r + c];
+                V64[vp + c * ws + sh] = v[3 * r + c];
                 V32[vp + c * ws] = (float)v[3 * r + c];
             }
         }

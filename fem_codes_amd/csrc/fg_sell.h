@@ -87,7 +87,7 @@ __device__ __forceinline__ typename Row::T fg_sell_chunk(const Row &row, I at, i
 
 // The row sum of the lane whose first entry is slot `at` of a slice of
 // width w: rows of up to 8 entries fully unrolled (one load burst), wider
-// rows in chunks of C.  `at` is an absolute slot (long long) or, for a Row
+// rows in chunks of C (the last one clamped).  `at` is an absolute slot (long long) or, for a Row
 // whose pointers start at its slice (a wave-uniform base, so that every load
 // is a scalar base plus the lane's 32-bit offset), the lane (unsigned).
 template <int C = FG_SELL_CHUNK, typename Row, typename I>
@@ -105,9 +105,12 @@ __device__ __forceinline__ typename Row::T fg_sell_run(const Row &row, I at, int
     case 7: return fg_sell_chunk<7, false>(row, at, 7, T(0));
     case 8: return fg_sell_chunk<8, false>(row, at, 8, T(0));
     default: {
+        // whole chunks, then the rest (clamped)
         T acc = T(0);
-        for (int k0 = 0; k0 < w; k0 += C)
-            acc = fg_sell_chunk<C, true>(row, at + (I)k0 * FG_SELL_W, w - k0, acc);
+        int k0 = 0;
+        for (; k0 + C <= w; k0 += C)
+            acc = fg_sell_chunk<C, false>(row, at + (I)k0 * FG_SELL_W, C, acc);
+        if (k0 < w) acc = fg_sell_chunk<C, true>(row, at + (I)k0 * FG_SELL_W, w - k0, acc);
         return acc;
     }
     }
