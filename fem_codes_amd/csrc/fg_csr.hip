@@ -3,7 +3,8 @@
 // The product runs on a sliced copy of the rows (fg_sell.h): 64 rows per
 // wavefront slice, lane = row, the k-th entries of a slice's rows stored
 // side by side, so every JA/AA load is one coalesced wavefront access and
-// each lane sums its row in registers.  The per-row summation order is the
+// each lane sums its row in registers (the FP64 values with each lane's
+// entries paired, one 16-byte load for two; fg_pair_shift).  The per-row summation order is the
 // reference's (csrmv.c:15-16: Q[i]=0, then Q[i] += AA[j]*P[JA[j]] for
 // ascending j), so with FMA contraction disabled the FP64 result is bitwise
 // the reference's.  The reference-order arrays IA/JA/AA stay the primary
@@ -91,6 +92,46 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_stream(
     if (blockIdx.x == 0 && threadIdx.x == 0) Q[neq] = T(0);
 }
 
+// one CSR row of a slice: the column and value pointers start at the
+// lane-0 slot of the wavefront's slice (wave-uniform bases, q = k*64 +
+// lane); FP64 values entry-paired (fg_pair_shift), merged into 16-byte loads
+template <typename TV>
+struct CsrRow {
+    using T = TV;
+    struct Rec {
+        int c;
+        T v, p;
+    };
+    const int *__restrict__ col;
+    const T *__restrict__ val;
+    const T *__restrict__ x;
+    int w;
+    __device__ __forceinline__ Rec key(unsigned q) const
+    {
+        Rec r;
+        r.c = __builtin_nontemporal_load(&col[q]);
+        r.p = T(0);
+        return r;
+    }
+    __device__ __forceinline__ void stream(Rec &r, unsigned q) const
+    {
+        const unsigned p = sizeof(T) == 8 ? q + fg_pair_shift((int)(q / FG_SELL_W),
+                                                              (int)(q % FG_SELL_W), w)
+                                          : q;
+        r.v = __builtin_nontemporal_load(&val[p]);
+    }
+    __device__ __forceinline__ void gather(Rec &r) const
+    {
+        // a 32-bit byte offset from x (sliced copies are built for neq < 2^28)
+        const unsigned b = (unsigned)(r.c < 0 ? 0 : r.c) * (unsigned)sizeof(T);
+        r.p = *(const T *)((const char *)x + b);
+    }
+    __device__ __forceinline__ T add(T acc, const Rec &r) const
+    {
+        return acc + (r.c < 0 ? T(0) : r.v * r.p);
+    }
+};
+
 // four slices (one per wavefront) per workgroup
 template <typename T>
 __global__ __launch_bounds__(FG_TPB) void k_csr_sliced(int nslice, const int *__restrict__ soff,
@@ -101,26 +142,36 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_sliced(int nslice, const int *__
 {
     if (skip && *skip) return;
     constexpr int SPW = FG_TPB / FG_SELL_W;
-    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
-                  (int)(threadIdx.x / FG_SELL_W);
-    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    const int s = fg_sell_slice(fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW);
+    const unsigned lane = threadIdx.x % FG_SELL_W;
     if (s < nslice) {
-        const int o0 = soff[s];
-        const T acc = fg_sell_row(fg_sell_prod<T>{scol, sval, P}, o0, soff[s + 1] - o0, lane);
-        const int r = s * FG_SELL_W + lane;
+        const int o0 = soff[s], w = soff[s + 1] - o0;
+        const long long q0 = (long long)o0 * FG_SELL_W;
+        const CsrRow<T> row{scol + q0, sval + q0, P, w};
+        const T acc = fg_sell_run(row, lane, w);
+        const int r = s * FG_SELL_W + (int)lane;
         if (r < neq) Q[r] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) Q[neq] = T(0);
 }
 
-// AA / AA32 (reference order) -> the sliced copies
+// the slot of a CSR entry with its FP64 pair shift in the top byte
+constexpr int FG_CSR_SHIFT_BIT = 56;
+__host__ __device__ inline long long csr_pos_pack(long long q, int shift)
+{
+    return q | ((long long)(shift + 64) << FG_CSR_SHIFT_BIT);
+}
+
+// AA / AA32 (reference order) -> the sliced copies (FP64 entry-paired)
 __global__ void k_csr_mirror(int nnz, const long long *__restrict__ pos,
                              const double *__restrict__ AA, const float *__restrict__ AA32,
                              double *__restrict__ s64, float *__restrict__ s32)
 {
     for (int j = blockIdx.x * FG_TPB + threadIdx.x; j < nnz; j += gridDim.x * FG_TPB) {
-        const long long q = pos[j];
-        s64[q] = AA[j];
+        const long long pk = pos[j];
+        const long long q = pk & ((1LL << FG_CSR_SHIFT_BIT) - 1);
+        const int sh = (int)(pk >> FG_CSR_SHIFT_BIT) - 64;
+        s64[q + sh] = AA[j];
         s32[q] = AA32[j];
     }
 }
@@ -211,15 +262,19 @@ extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, cons
     {
         // the sliced copy, unless its padding exceeds a quarter of the entries
         const fg_sell_layout L = fg_sell_plan(neq, IA);
-        if (nnz > 0 && L.nslot <= (long long)nnz + nnz / 4 + 4096) {
+        // (the product addresses P by a 32-bit byte offset)
+        if (nnz > 0 && neq < (1 << 28) && L.nslot <= (long long)nnz + nnz / 4 + 4096) {
             std::vector<int> scol(L.nslot, -1);
             std::vector<long long> spos(nnz);
-            for (int r = 0; r < neq; r++)
+            for (int r = 0; r < neq; r++) {
+                const int s = r / FG_SELL_W, w = L.soff[s + 1] - L.soff[s];
                 for (int j = IA[r]; j < IA[r + 1]; j++) {
-                    const long long q = L.slot(r, j - IA[r]);
+                    const int k = j - IA[r];
+                    const long long q = L.slot(r, k);
                     scol[q] = JA[j];
-                    spos[j] = q;
+                    spos[j] = csr_pos_pack(q, fg_pair_shift(k, r % FG_SELL_W, w));
                 }
+            }
             op->nslice = L.nslice;
             op->nslot = L.nslot;
             FG_HIP(hipMalloc(&op->d_soff, sizeof(int) * L.soff.size()));

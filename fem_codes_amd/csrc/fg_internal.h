@@ -122,15 +122,22 @@ struct fg_carver {
 // FG_OP_EBE2: off-block product of a k = 12 EBE operator + node pass
 enum { FG_OP_EBE = 0, FG_OP_EDE = 1, FG_OP_CSR = 2, FG_OP_EBE2 = 3 };
 
-// ---- element rows (fg_ebe_rows.hip): the values of column c of a slice of
-// width w fill w blocks of 64, entry k of lane l at k*64 + l -- except for
-// FP64 triangles (K = 3), whose entries 2j and 2j+1 of a lane sit side by
-// side, at 2j*64 + 2l + (k & 1), so that one 16-byte load reads both (the
-// last entry of an odd width alone, at k*64 + l).  fg_rows_shift is the FP64
-// position minus the FP32 one.
+// ---- entry pairs in sliced rows: a value column of a slice of width w fills
+// w blocks of 64, entry k of lane l at k*64 + l; the paired form puts a
+// lane's entries 2j and 2j+1 side by side, at 2j*64 + 2l + (k & 1), so that
+// one 16-byte load reads both FP64 values (the last entry of an odd width
+// alone, at k*64 + l).  Slices of up to 8 entries only (their rows are
+// unrolled whole, so the compiler merges each pair's loads); wider slices
+// stay plain.  fg_pair_shift is the paired position minus the plain one.
+// Paired: the FP64 values of the CSR rows and of the triangle element rows
+// (fg_rows_shift, K = 3; fg_ebe_rows.hip).
+__host__ __device__ inline int fg_pair_shift(int k, int l, int w)
+{
+    return (w > 8 || ((w & 1) && k == w - 1)) ? 0 : l - (k & 1) * 63;
+}
 __host__ __device__ inline int fg_rows_shift(int K, int k, int l, int w)
 {
-    return (K != 3 || ((w & 1) && k == w - 1)) ? 0 : l - (k & 1) * 63;
+    return K != 3 ? 0 : fg_pair_shift(k, l, w);
 }
 // per element row: the column stride w*64 and the FP64 shift in one int
 __host__ __device__ inline int fg_rows_wst(int w, int shift) { return w * 64 * 128 + shift + 64; }
@@ -186,7 +193,8 @@ struct fg_op {
     int nslice = 0;
     long long nslot = 0;
     int *d_soff = nullptr, *d_scol = nullptr;
-    long long *d_spos = nullptr; // CSR: slot of each entry (null: row-block product)
+    long long *d_spos = nullptr; // CSR: slot of each entry, FP64 pair shift in the top
+                                 // byte (null: row-block product)
     double *d_sv64 = nullptr;
     float *d_sv32 = nullptr;
     // EBE k = 3, 4 element rows (fg_ebe_rows.hip): packed column offsets,
