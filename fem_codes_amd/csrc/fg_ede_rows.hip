@@ -72,20 +72,24 @@ struct HalfEdgeSlot {
     const T *__restrict__ P;
     T pown;
     int neq;
-    __device__ __forceinline__ Rec key(long long q) const
+    // col and val start at the lane-0 slot of the wavefront's slice
+    // (wave-uniform bases), q = k*64 + lane
+    __device__ __forceinline__ Rec key(unsigned q) const
     {
         Rec r;
         r.c = __builtin_nontemporal_load(&col[q]);
         r.p = T(0);
         return r;
     }
-    __device__ __forceinline__ void stream(Rec &r, long long q) const
+    __device__ __forceinline__ void stream(Rec &r, unsigned q) const
     {
         r.v = __builtin_nontemporal_load(&val[q]);
     }
     __device__ __forceinline__ void gather(Rec &r) const
     {
-        const T t = P[r.c < 0 ? 0 : r.c];
+        // a 32-bit byte offset from P (rows are built for neq < 2^28)
+        const unsigned b = (unsigned)(r.c < 0 ? 0 : r.c) * (unsigned)sizeof(T);
+        const T t = *(const T *)((const char *)P + b);
         r.p = r.c < neq ? t : T(0);
     }
     __device__ __forceinline__ T term(const Rec &r) const
@@ -108,17 +112,18 @@ __global__ __launch_bounds__(FG_TPB) void k_ede_rows(int nslice, const int *__re
     if (skip && *skip) return;
     using V2 = typename EdePair<T>::type;
     constexpr int SPW = FG_TPB / FG_SELL_W;
-    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
-                  (int)(threadIdx.x / FG_SELL_W);
-    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    const int s = fg_sell_slice(fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW);
+    const unsigned lane = threadIdx.x % FG_SELL_W;
     if (s < nslice) {
-        const int r = s * FG_SELL_W + lane;
+        const int r = s * FG_SELL_W + (int)lane;
         // lanes past the last equation read the sink's P; all their slots
         // are padding
         const T pown = P[r < neq ? r : neq];
         const int o0 = soff[s];
-        const HalfEdgeSlot<T> row{scol, reinterpret_cast<const V2 *>(sval), P, pown, neq};
-        const T acc = fg_sell_row(row, o0, soff[s + 1] - o0, lane);
+        const long long q0 = (long long)o0 * FG_SELL_W;
+        const HalfEdgeSlot<T> row{scol + q0, reinterpret_cast<const V2 *>(sval) + q0, P, pown,
+                                  neq};
+        const T acc = fg_sell_run(row, lane, soff[s + 1] - o0);
         if (r < neq) Q[r] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -139,6 +144,8 @@ static int upload(const std::vector<T> &v, T **d)
 int fg_ede_rows_build(fg_op *op, const int *lm2, const long long *kpos)
 {
     const int neq = op->neq, nedge = op->nitems;
+    // the product addresses P by a 32-bit byte offset
+    if (neq >= (1 << 28)) return FG_OK;
     std::vector<int> rIA((size_t)neq + 1, 0);
     for (int e = 0; e < nedge; e++) {
         if (kpos[e] < 0) continue;
