@@ -108,6 +108,8 @@ def ebe_kernel_name(k, fp32, op=None):
     operator uses them, else the blocked kernel at the library's default
     block size"""
     t = "float" if fp32 else "double"
+    if op is not None and op.product_form(fp32) == 2:
+        return f"k_ebe_nrows<{t}>"
     if op is not None and op.product_sliced(fp32):
         return f"k_ebe_rows<{t}, {k}>"
     ept = 4 if k <= 2 else 2 if k == 3 else 4 if k == 4 else 1   # the library's defaults
@@ -737,7 +739,7 @@ def ss3d_leg(F, ctx, args, n=128, sweeps=4, seed=20260303):
                      "ms": {autotune.key_str(k): v for k, v in tms.items()}}
         log("ss3d autotune FP32 product ms:", {autotune.key_str(k): round(v, 4)
                                                for k, v in tms.items()}, "->", autotune.key_str(key))
-        ebe_kernel = ("k_ebe_rows<float, 4>" if ebe.product_sliced(True)
+        ebe_kernel = (ebe_kernel_name(4, True, ebe) if ebe.product_sliced(True)
                       else f"k_ebe_apply<4, {key[0] // 256}, float, 0>")
     del cbe, IA, JA
     pc = F.Preconditioner(csr, F.PREC_ILU0)

@@ -424,10 +424,15 @@ class Operator:
 
     def product_sliced(self, fp32=False) -> bool:
         """whether the product in use for that precision is a sliced-row kernel"""
+        return self.product_form(fp32) != 0
+
+    def product_form(self, fp32=False) -> int:
+        """0: a blocked kernel; 1: sliced rows; 2: the k = 4 sliced rows with
+        per-equation neighbour tables"""
         v = _i()
         _check(lib.fg_op_product_kernel(self.h, int(bool(fp32)), C.byref(v)),
                "fg_op_product_kernel")
-        return bool(v.value)
+        return int(v.value)
 
     def close(self):
         if self.h and self.ctx.h:

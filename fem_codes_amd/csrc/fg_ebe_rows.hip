@@ -32,6 +32,7 @@
 // values; the rows are refreshed from it by one kernel before the first
 // product after any change.  A mesh whose column offsets do not fit the
 // fields keeps the blocked product only.
+#include <algorithm>
 #include <vector>
 
 #include "fg_ebe_body.h"
@@ -162,6 +163,126 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_rows(int nslice, const int *__re
         // tetrahedra: rows of about 24 entries in chunks of 12 (two load
         // bursts), measured faster than chunks of 8 in both precisions
         const T acc = fg_sell_run<K == 4 ? 12 : FG_SELL_CHUNK>(row, lane, w);
+        if (r < neq) Q[r] = acc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Q[neq] = T(0);
+        P[neq] = T(0);
+    }
+}
+
+// k = 4 rows with per-row neighbour tables.  A tetrahedron row references 3
+// other columns, 72 per equation of 24 rows, but an equation has only about
+// 14 distinct neighbours: each lane gathers its neighbours' P once into its
+// own column of an LDS table, and each entry names its three other columns by
+// 4-bit table indices (2 bytes per row instead of 6).  The row sums and their
+// order are those of the offset rows: the same result bit for bit.
+constexpr int NB_MAX = 15;               // table entries; index 15 holds 0 (the sink)
+constexpr unsigned short NB_PAD = 0xffffu;
+
+template <typename TV>
+struct NeighRow {
+    using T = TV;
+    struct Rec {
+        unsigned cd;
+        T v[4];
+    };
+    const unsigned short *__restrict__ code;   // the slice's lane-0 slot
+    const T *__restrict__ V[4];
+    const T *tab;   // the lane's table (LDS): neighbour j at tab[j * 64]
+    T pown;
+    __device__ __forceinline__ Rec key(unsigned q) const
+    {
+        Rec e;
+        e.cd = __builtin_nontemporal_load(&code[q]);
+        return e;
+    }
+    __device__ __forceinline__ void stream(Rec &e, unsigned q) const
+    {
+#pragma unroll
+        for (int c = 0; c < 4; c++) e.v[c] = __builtin_nontemporal_load(&V[c][q]);
+    }
+    // the table reads wait for little: they are made where the term needs
+    // them, so that a chunk holds no more registers than its loads
+    __device__ __forceinline__ void gather(Rec &) const {}
+    __device__ __forceinline__ T term(const Rec &e) const
+    {
+        const int lr = (int)(e.cd & 3u);
+        T po[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) po[j] = tab[((e.cd >> (2 + 4 * j)) & 15u) * FG_SELL_W];
+        T s = T(0);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const T oth = c == 0 ? po[0] : (c == 3 ? po[2] : (c < lr ? po[c] : po[c - 1]));
+            const T pc = c == lr ? pown : oth;
+            s = c == 0 ? e.v[0] * pc : s + e.v[c] * pc;
+        }
+        return e.cd == NB_PAD ? T(0) : s;
+    }
+    __device__ __forceinline__ T add(T acc, const Rec &e) const { return acc + term(e); }
+};
+
+template <typename T>
+__global__ __launch_bounds__(FG_TPB) void k_ebe_nrows(int nslice, const int *__restrict__ soff,
+                                                      const int *__restrict__ nsoff,
+                                                      const unsigned short *__restrict__ code,
+                                                      const short *__restrict__ noff,
+                                                      const T *__restrict__ V,
+                                                      T *__restrict__ P, T *__restrict__ Q,
+                                                      int neq, const int *skip,
+                                                      const int *__restrict__ sprob,
+                                                      const int *pflag, int pstride)
+{
+    if (skip && *skip) return;
+    constexpr int SPW = FG_TPB / FG_SELL_W;
+    __shared__ T tabs[SPW][(NB_MAX + 1) * FG_SELL_W];
+    const int wid = (int)(threadIdx.x / FG_SELL_W);
+    const int s = fg_sell_slice(fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW);
+    const unsigned lane = threadIdx.x % FG_SELL_W;
+    bool run = s < nslice;
+    if (run && sprob) {
+        const int pb = sprob[s];
+        run = pb < 0 || pflag[(long long)pb * pstride] == 0;
+    }
+    if (run) {
+        const int r = s * FG_SELL_W + (int)lane;
+        const int rc = r < neq ? r : neq;
+        const int o0 = soff[s], w = soff[s + 1] - o0;
+        const int n0 = nsoff[s], m = nsoff[s + 1] - n0;
+        T *tab = &tabs[wid][lane];
+        // the lane's neighbours: all gathers issued together (entries past
+        // the slice's count re-read its last one; padding reads the own P);
+        // only this lane reads its column of the table, so no barrier
+        {
+            const short *no = noff + (long long)n0 * FG_SELL_W;
+            short o[NB_MAX];
+            T t[NB_MAX];
+#pragma unroll
+            for (int j = 0; j < NB_MAX; j++) {
+                const int jj = j < m ? j : m - 1;
+                o[j] = m > 0 ? __builtin_nontemporal_load(&no[(unsigned)jj * FG_SELL_W + lane])
+                             : (short)-32768;
+            }
+#pragma unroll
+            for (int j = 0; j < NB_MAX; j++) {
+                const unsigned b = (unsigned)(o[j] == -32768 ? rc : rc + o[j]) * (unsigned)sizeof(T);
+                t[j] = *(const T *)((const char *)P + b);
+            }
+#pragma unroll
+            for (int j = 0; j < NB_MAX; j++)
+                if (j < m) tab[j * FG_SELL_W] = o[j] == -32768 ? T(0) : t[j];
+            tab[NB_MAX * FG_SELL_W] = T(0);
+        }
+        const long long q0 = (long long)o0 * FG_SELL_W;
+        NeighRow<T> row;
+        row.code = code + q0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) row.V[c] = V + 4 * q0 + (long long)c * w * FG_SELL_W;
+        row.tab = tab;
+        row.pown = P[rc];
+        // (tetrahedron rows: about 24 entries, three whole chunks)
+        const T acc = fg_sell_chunks<8>(row, lane, w);
         if (r < neq) Q[r] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -364,11 +485,72 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
             wst[(size_t)e * K + a] = fg_rows_wst(w, fg_rows_shift(K, kr, r % FG_SELL_W, w));
         }
     }
+    // k = 4: the neighbour form when every row has at most NB_MAX distinct
+    // other columns within 16-bit offsets (neighbours in order of first
+    // appearance), else the offset rows
+    bool neigh = K == 4;
+    std::vector<unsigned short> code;
+    std::vector<int> nsoff;
+    std::vector<short> noff;
+    if (neigh) {
+        std::vector<int> nb((size_t)neq * NB_MAX), nbn((size_t)neq, 0);
+        code.assign(L.nslot, NB_PAD);
+        std::fill(fill.begin(), fill.end(), 0);
+        for (int e = 0; e < nel && neigh; e++) {
+            if (kpos[e] < 0) continue;
+            const int *le = lm + (size_t)e * K;
+            for (int a = 0; a < K && neigh; a++) {
+                const int r = le[a];
+                if (r == neq) continue;
+                unsigned cd = (unsigned)a;
+                int m = 0;
+                for (int c = 0; c < K; c++) {
+                    if (c == a) continue;
+                    int j = NB_MAX;   // the sink
+                    if (le[c] != neq) {
+                        int *L0 = &nb[(size_t)r * NB_MAX];
+                        for (j = 0; j < nbn[r] && L0[j] != le[c]; j++) {}
+                        if (j == nbn[r]) {
+                            const long long d = (long long)le[c] - r;
+                            if (j == NB_MAX || d < -32767 || d > 32767) {
+                                neigh = false;
+                                break;
+                            }
+                            L0[nbn[r]++] = le[c];
+                        }
+                    }
+                    cd |= (unsigned)j << (2 + 4 * m++);
+                }
+                code[L.slot(r, fill[r]++)] = (unsigned short)cd;
+            }
+        }
+        if (neigh) {
+            const int ns = L.nslice;
+            nsoff.assign((size_t)ns + 1, 0);
+            for (int s = 0; s < ns; s++) {
+                int mx = 0;
+                for (int r = s * FG_SELL_W; r < std::min(neq, (s + 1) * FG_SELL_W); r++)
+                    mx = std::max(mx, nbn[r]);
+                nsoff[s + 1] = nsoff[s] + mx;
+            }
+            noff.assign((size_t)nsoff[ns] * FG_SELL_W, (short)-32768);
+            for (int r = 0; r < neq; r++)
+                for (int j = 0; j < nbn[r]; j++)
+                    noff[((size_t)nsoff[r / FG_SELL_W] + j) * FG_SELL_W + r % FG_SELL_W] =
+                        (short)(nb[(size_t)r * NB_MAX + j] - r);
+        }
+    }
     op->nslice = L.nslice;
     op->nslot = L.nslot;
     FG_CHECK(upload(L.soff, &op->d_soff));
     FG_CHECK(upload(idx, &op->d_ridx));
-    if (K == 4) FG_CHECK(upload(idx2, &op->d_ridx2));
+    if (neigh) {
+        FG_CHECK(upload(code, &op->d_rcode));
+        FG_CHECK(upload(nsoff, &op->d_nsoff));
+        FG_CHECK(upload(noff, &op->d_noff));
+    } else if (K == 4) {
+        FG_CHECK(upload(idx2, &op->d_ridx2));
+    }
     FG_CHECK(upload(src, &op->d_rsrc));
     FG_CHECK(upload(vpos, &op->d_rvpos));
     FG_CHECK(upload(wst, &op->d_rwst));
@@ -443,6 +625,17 @@ int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *s
     FG_CHECK(fg_ebe_rows_refresh(op));
     constexpr int SPW = FG_TPB / FG_SELL_W;
     const int g = (op->nslice + SPW - 1) / SPW;
+    if (K == 4 && op->d_rcode) {
+        if (fp32)
+            FG_LAUNCH(k_ebe_nrows<float>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                      op->d_nsoff, op->d_rcode, op->d_noff, (const float *)op->d_sv32,
+                      (float *)P, (float *)Q, op->neq, skip, sprob, pflag, pstride);
+        else
+            FG_LAUNCH(k_ebe_nrows<double>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
+                      op->d_nsoff, op->d_rcode, op->d_noff, (const double *)op->d_sv64,
+                      (double *)P, (double *)Q, op->neq, skip, sprob, pflag, pstride);
+        return FG_OK;
+    }
     if (fp32) {
         if (K == 3)
             FG_LAUNCH((k_ebe_rows<float, 3>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
@@ -468,6 +661,7 @@ int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *s
 void fg_ebe_rows_free(fg_op *op)
 {
     hipFree(op->d_ridx); hipFree(op->d_ridx2); hipFree(op->d_rsrc); hipFree(op->d_rvpos);
+    hipFree(op->d_rcode); hipFree(op->d_nsoff); hipFree(op->d_noff);
     hipFree(op->d_rwst);
     if (op->kind == FG_OP_EBE) {
         hipFree(op->d_soff); hipFree(op->d_sv64); hipFree(op->d_sv32);

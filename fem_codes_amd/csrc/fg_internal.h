@@ -201,6 +201,13 @@ struct fg_op {
     // the third offset (k = 4), source element * k + row (-1: padding)
     unsigned *d_ridx = nullptr;
     short *d_ridx2 = nullptr;
+    // k = 4 neighbour form (the product's index when every row has at most
+    // 15 distinct other columns): per slot lr | 3 x 4-bit neighbour index
+    // (15: the sink; 0xffff: padding), per row its neighbours as offsets
+    // from the equation in slices of their own (-32768: padding)
+    unsigned short *d_rcode = nullptr;
+    int *d_nsoff = nullptr;
+    short *d_noff = nullptr;
     int *d_rsrc = nullptr;
     long long *d_rvpos = nullptr; // [nel*k] FP32 value position of row (e, lr), -1: no row
     int *d_rwst = nullptr;        // [nel*k] fg_rows_wst: column stride, FP64 shift

@@ -170,6 +170,7 @@ extern "C" int fg_op_product_kernel(const fg_op *op, int fp32, int *sliced)
     FG_ARG(op && sliced, "op/sliced");
     *sliced = (op->kind == FG_OP_EDE && fg_ede_rows_used(op, fp32)) ||
               fg_ebe_rows_used(op, fp32) || (op->kind == FG_OP_CSR && op->d_spos);
+    if (*sliced && op->kind == FG_OP_EBE && op->d_rcode) *sliced = 2;
     return FG_OK;
 }
 

@@ -85,6 +85,19 @@ __device__ __forceinline__ typename Row::T fg_sell_chunk(const Row &row, I at, i
     return acc;
 }
 
+// The row sum in whole chunks of C entries, then the rest (clamped): for rows
+// that are almost never short (their registers sized for C entries only)
+template <int C, typename Row, typename I>
+__device__ __forceinline__ typename Row::T fg_sell_chunks(const Row &row, I at, int w)
+{
+    typename Row::T acc = typename Row::T(0);
+    int k0 = 0;
+    for (; k0 + C <= w; k0 += C)
+        acc = fg_sell_chunk<C, false>(row, at + (I)k0 * FG_SELL_W, C, acc);
+    if (k0 < w) acc = fg_sell_chunk<C, true>(row, at + (I)k0 * FG_SELL_W, w - k0, acc);
+    return acc;
+}
+
 // The row sum of the lane whose first entry is slot `at` of a slice of
 // width w: rows of up to 8 entries fully unrolled (one load burst), wider
 // rows in chunks of C (the last one clamped).  `at` is an absolute slot (long long) or, for a Row
@@ -104,15 +117,7 @@ __device__ __forceinline__ typename Row::T fg_sell_run(const Row &row, I at, int
     case 6: return fg_sell_chunk<6, false>(row, at, 6, T(0));
     case 7: return fg_sell_chunk<7, false>(row, at, 7, T(0));
     case 8: return fg_sell_chunk<8, false>(row, at, 8, T(0));
-    default: {
-        // whole chunks, then the rest (clamped)
-        T acc = T(0);
-        int k0 = 0;
-        for (; k0 + C <= w; k0 += C)
-            acc = fg_sell_chunk<C, false>(row, at + (I)k0 * FG_SELL_W, C, acc);
-        if (k0 < w) acc = fg_sell_chunk<C, true>(row, at + (I)k0 * FG_SELL_W, w - k0, acc);
-        return acc;
-    }
+    default: return fg_sell_chunks<C>(row, at, w);
     }
 }
 

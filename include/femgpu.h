@@ -166,12 +166,14 @@ int fg_op_set_edge_kernel(fg_op *op, int kind);
  * has them.  An element operator gets sliced rows when k is 3 or 4 and every
  * element's other columns lie within 16383 (k = 4: the third within 32767)
  * of its own equation; else it keeps the blocked kernel only and
- * FG_PRODUCT_SLICED is rejected.  The choice is made per precision:
+ * FG_PRODUCT_SLICED is rejected.  At k = 4 the rows name their other columns
+ * through per-equation neighbour tables when every equation has at most 15
+ * distinct other columns (same result).  The choice is made per precision:
  * precisions = FG_PRODUCT_FP64, FG_PRODUCT_FP32 or both (the FP32 products of
  * the mixed-precision solver may use one kernel and the FP64 products the
- * other).  fg_op_product_kernel reports whether the product in use for that
- * precision is a sliced-row kernel (CSR: the sliced rows, unless the pattern
- * would carry more than a quarter of padding). */
+ * other).  fg_op_product_kernel sets *sliced to 0 for a blocked kernel, 1
+ * for sliced rows (CSR: the sliced rows, unless the pattern would carry more
+ * than a quarter of padding), 2 for the k = 4 rows with neighbour tables. */
 #define FG_PRODUCT_SLICED  0
 #define FG_PRODUCT_BLOCKED 1
 #define FG_PRODUCT_AUTO    2

@@ -63,6 +63,37 @@ def _synthetic(k, seed):
     return neq, L.astype(np.int32).ravel()
 
 
+def _kuhn(n, seed):
+    """the tetrahedra of the n^3 cube cut into 6 per cell (at most 14
+    neighbours per node) with 30% of them dropped at random (an element list,
+    not a conforming mesh), random Dirichlet nodes; the free nodes numbered by
+    decreasing element count (random among equals), so that the slices take
+    widths from 24 down to a few entries"""
+    rng = np.random.default_rng(seed)
+    x, y, z, t, v = F.structured_mesh3d(n)
+    nn = len(x)
+    cells = np.asarray(v).reshape(-1, 4)
+    cells = cells[rng.random(len(cells)) >= 0.3]
+    fixed = (t <= 0) | (rng.random(nn) < 0.1)
+    free = np.nonzero(~fixed)[0]
+    neq = len(free)
+    cnt = np.bincount(cells.ravel(), minlength=nn)[free]
+    order = np.lexsort((rng.random(neq), -cnt))
+    eq = np.full(nn, neq, np.int64)
+    eq[free[order]] = np.arange(neq)
+    return neq, eq[cells].astype(np.int32).ravel()
+
+
+def _max_neighbours(neq, lm, k):
+    L = lm.reshape(-1, k)
+    nb = [set() for _ in range(neq)]
+    for e in L:
+        for a in range(k):
+            if e[a] < neq:
+                nb[e[a]].update(int(c) for c in e if c < neq and c != e[a])
+    return max(len(s) for s in nb)
+
+
 def _ebemv_f32(neq, lm, A, P, k):
     """ebemv.c:24-40 / ebemv3Dndof1.c:24-42 in float32: each element row
     summed left to right, the sums added to Q in element order"""
@@ -89,10 +120,13 @@ def _slice_widths(neq, lm):
 
 
 @pytest.mark.parametrize("dim,npts,seed", [(2, 6000, 1), (2, 3001, 2), (3, 3000, 3),
-                                           (2, 0, 4), (3, 0, 5)])
+                                           (2, 0, 4), (3, 0, 5), (3, -14, 6), (3, -9, 7)])
 def test_rows_unstructured_bitwise(dim, npts, seed):
+    """(npts > 0: Delaunay of npts points; 0: synthetic element lists;
+    -n: the n^3 Kuhn cube, whose tetrahedron rows use neighbour tables)"""
     k = dim + 1
-    neq, lm = _mesh(dim, npts, seed) if npts else _synthetic(k, seed)
+    neq, lm = (_mesh(dim, npts, seed) if npts > 0 else _synthetic(k, seed) if npts == 0
+               else _kuhn(-npts, seed))
     nel = lm.size // k
     w = _slice_widths(neq, lm)
     # (every interior vertex of a tetrahedralisation lies in an even number
@@ -101,13 +135,16 @@ def test_rows_unstructured_bitwise(dim, npts, seed):
     assert np.any(w > 8) and len(np.unique(w)) > 2
     if not npts:
         assert set(range(1, 14)) <= set(w.tolist())
+    # k = 4: neighbour tables where every equation has at most 15 distinct
+    # other columns, else the offset rows
+    form = 2 if k == 4 and _max_neighbours(neq, lm, k) <= 15 else 1
     rng = np.random.default_rng(seed + 10)
     ctx = F.Context(0)
     orc = O.Oracle()
     op = F.Operator.ebe(ctx, neq, lm, k)
     A = rng.standard_normal(k * k * nel)
     op.set_values(ctx.to_device(A))
-    assert op.product_sliced(False) and op.product_sliced(True)
+    assert op.product_form(False) == form and op.product_form(True) == form
     P = rng.standard_normal(neq + 1)
     P[neq] = 0.0
 
