@@ -108,8 +108,8 @@ def ebe_kernel_name(k, fp32, op=None):
     operator uses them, else the blocked kernel at the library's default
     block size"""
     t = "float" if fp32 else "double"
-    if op is not None and op.product_form(fp32) == 2:
-        return f"k_ebe_nrows<{t}>"
+    if op is not None and op.product_form(fp32) == 2:   # k = 4 neighbour tables
+        return f"k_ebe_nrows<{t}, 4>"
     if op is not None and op.product_sliced(fp32):
         return f"k_ebe_rows<{t}, {k}>"
     ept = 4 if k <= 2 else 2 if k == 3 else 4 if k == 4 else 1   # the library's defaults

@@ -171,26 +171,32 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_rows(int nslice, const int *__re
     }
 }
 
-// k = 4 rows with per-row neighbour tables.  A tetrahedron row references 3
-// other columns, 72 per equation of 24 rows, but an equation has only about
-// 14 distinct neighbours: each lane gathers its neighbours' P once into its
-// own column of an LDS table, and each entry names its three other columns by
-// 4-bit table indices (2 bytes per row instead of 6).  The row sums and their
-// order are those of the offset rows: the same result bit for bit.
-constexpr int NB_MAX = 15;               // table entries; index 15 holds 0 (the sink)
+// Element rows with per-equation neighbour tables (built for tetrahedra).  A
+// tetrahedron row names 3 other columns, 72 scattered P loads per equation of
+// 24 rows, but an equation has only about 14 distinct neighbours: each lane
+// gathers its neighbours' P once into its own column of an LDS table, and
+// each entry names its other columns by 4-bit table indices (2 bytes per row
+// instead of 6).  The row sums and their order are those of the offset rows:
+// the same result bit for bit.
+template <int K>
+struct NbTable {
+    static constexpr int N = K == 3 ? 8 : 16;   // entries; the last holds 0 (the sink)
+    static constexpr int MAXNB = N - 1;          // neighbours per equation
+};
 constexpr unsigned short NB_PAD = 0xffffu;
 
-template <typename TV>
+template <typename TV, int K>
 struct NeighRow {
     using T = TV;
     struct Rec {
         unsigned cd;
-        T v[4];
+        T v[K];
     };
     const unsigned short *__restrict__ code;   // the slice's lane-0 slot
-    const T *__restrict__ V[4];
+    const T *__restrict__ V[K];
     const T *tab;   // the lane's table (LDS): neighbour j at tab[j * 64]
     T pown;
+    int w;
     __device__ __forceinline__ Rec key(unsigned q) const
     {
         Rec e;
@@ -199,8 +205,12 @@ struct NeighRow {
     }
     __device__ __forceinline__ void stream(Rec &e, unsigned q) const
     {
+        // FP64 triangles: the entry-paired positions (fg_rows_shift)
+        const unsigned p = sizeof(T) == 8 ? q + fg_rows_shift(K, (int)(q / FG_SELL_W),
+                                                              (int)(q % FG_SELL_W), w)
+                                          : q;
 #pragma unroll
-        for (int c = 0; c < 4; c++) e.v[c] = __builtin_nontemporal_load(&V[c][q]);
+        for (int c = 0; c < K; c++) e.v[c] = __builtin_nontemporal_load(&V[c][p]);
     }
     // the table reads wait for little: they are made where the term needs
     // them, so that a chunk holds no more registers than its loads
@@ -208,13 +218,13 @@ struct NeighRow {
     __device__ __forceinline__ T term(const Rec &e) const
     {
         const int lr = (int)(e.cd & 3u);
-        T po[3];
+        T po[K - 1];
 #pragma unroll
-        for (int j = 0; j < 3; j++) po[j] = tab[((e.cd >> (2 + 4 * j)) & 15u) * FG_SELL_W];
+        for (int j = 0; j < K - 1; j++) po[j] = tab[((e.cd >> (2 + 4 * j)) & 15u) * FG_SELL_W];
         T s = T(0);
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const T oth = c == 0 ? po[0] : (c == 3 ? po[2] : (c < lr ? po[c] : po[c - 1]));
+        for (int c = 0; c < K; c++) {
+            const T oth = c == 0 ? po[0] : (c == K - 1 ? po[K - 2] : (c < lr ? po[c] : po[c - 1]));
             const T pc = c == lr ? pown : oth;
             s = c == 0 ? e.v[0] * pc : s + e.v[c] * pc;
         }
@@ -223,7 +233,7 @@ struct NeighRow {
     __device__ __forceinline__ T add(T acc, const Rec &e) const { return acc + term(e); }
 };
 
-template <typename T>
+template <typename T, int K>
 __global__ __launch_bounds__(FG_TPB) void k_ebe_nrows(int nslice, const int *__restrict__ soff,
                                                       const int *__restrict__ nsoff,
                                                       const unsigned short *__restrict__ code,
@@ -236,7 +246,8 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_nrows(int nslice, const int *__r
 {
     if (skip && *skip) return;
     constexpr int SPW = FG_TPB / FG_SELL_W;
-    __shared__ T tabs[SPW][(NB_MAX + 1) * FG_SELL_W];
+    constexpr int NB = NbTable<K>::MAXNB;
+    __shared__ T tabs[SPW][NbTable<K>::N * FG_SELL_W];
     const int wid = (int)(threadIdx.x / FG_SELL_W);
     const int s = fg_sell_slice(fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW);
     const unsigned lane = threadIdx.x % FG_SELL_W;
@@ -256,33 +267,35 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_nrows(int nslice, const int *__r
         // only this lane reads its column of the table, so no barrier
         {
             const short *no = noff + (long long)n0 * FG_SELL_W;
-            short o[NB_MAX];
-            T t[NB_MAX];
+            short o[NB];
+            T t[NB];
 #pragma unroll
-            for (int j = 0; j < NB_MAX; j++) {
+            for (int j = 0; j < NB; j++) {
                 const int jj = j < m ? j : m - 1;
                 o[j] = m > 0 ? __builtin_nontemporal_load(&no[(unsigned)jj * FG_SELL_W + lane])
                              : (short)-32768;
             }
 #pragma unroll
-            for (int j = 0; j < NB_MAX; j++) {
+            for (int j = 0; j < NB; j++) {
                 const unsigned b = (unsigned)(o[j] == -32768 ? rc : rc + o[j]) * (unsigned)sizeof(T);
                 t[j] = *(const T *)((const char *)P + b);
             }
 #pragma unroll
-            for (int j = 0; j < NB_MAX; j++)
+            for (int j = 0; j < NB; j++)
                 if (j < m) tab[j * FG_SELL_W] = o[j] == -32768 ? T(0) : t[j];
-            tab[NB_MAX * FG_SELL_W] = T(0);
+            tab[NB * FG_SELL_W] = T(0);
         }
         const long long q0 = (long long)o0 * FG_SELL_W;
-        NeighRow<T> row;
+        NeighRow<T, K> row;
         row.code = code + q0;
 #pragma unroll
-        for (int c = 0; c < 4; c++) row.V[c] = V + 4 * q0 + (long long)c * w * FG_SELL_W;
+        for (int c = 0; c < K; c++) row.V[c] = V + K * q0 + (long long)c * w * FG_SELL_W;
         row.tab = tab;
         row.pown = P[rc];
-        // (tetrahedron rows: about 24 entries, three whole chunks)
-        const T acc = fg_sell_chunks<8>(row, lane, w);
+        row.w = w;
+        // tetrahedron rows (about 24 entries) in whole chunks; triangle rows
+        // (about 6) unrolled whole
+        const T acc = K == 4 ? fg_sell_chunks<8>(row, lane, w) : fg_sell_run<8>(row, lane, w);
         if (r < neq) Q[r] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -485,15 +498,19 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
             wst[(size_t)e * K + a] = fg_rows_wst(w, fg_rows_shift(K, kr, r % FG_SELL_W, w));
         }
     }
-    // k = 4: the neighbour form when every row has at most NB_MAX distinct
-    // other columns within 16-bit offsets (neighbours in order of first
+    // the neighbour form when every equation has at most nbmax distinct other
+    // columns within 16-bit offsets (neighbours in order of first
     // appearance), else the offset rows
+    // (tetrahedra only: for triangle rows the table costs more than the 6
+    // gathers it saves per equation -- 2048^2, measured: 124.7 -> 128.3 us
+    // FP64, 65.3 -> 67.0 us FP32)
+    const int nbmax = NbTable<4>::MAXNB;
     bool neigh = K == 4;
     std::vector<unsigned short> code;
     std::vector<int> nsoff;
     std::vector<short> noff;
     if (neigh) {
-        std::vector<int> nb((size_t)neq * NB_MAX), nbn((size_t)neq, 0);
+        std::vector<int> nb((size_t)neq * nbmax), nbn((size_t)neq, 0);
         code.assign(L.nslot, NB_PAD);
         std::fill(fill.begin(), fill.end(), 0);
         for (int e = 0; e < nel && neigh; e++) {
@@ -506,13 +523,13 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
                 int m = 0;
                 for (int c = 0; c < K; c++) {
                     if (c == a) continue;
-                    int j = NB_MAX;   // the sink
+                    int j = nbmax;   // the sink
                     if (le[c] != neq) {
-                        int *L0 = &nb[(size_t)r * NB_MAX];
+                        int *L0 = &nb[(size_t)r * nbmax];
                         for (j = 0; j < nbn[r] && L0[j] != le[c]; j++) {}
                         if (j == nbn[r]) {
                             const long long d = (long long)le[c] - r;
-                            if (j == NB_MAX || d < -32767 || d > 32767) {
+                            if (j == nbmax || d < -32767 || d > 32767) {
                                 neigh = false;
                                 break;
                             }
@@ -537,7 +554,7 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
             for (int r = 0; r < neq; r++)
                 for (int j = 0; j < nbn[r]; j++)
                     noff[((size_t)nsoff[r / FG_SELL_W] + j) * FG_SELL_W + r % FG_SELL_W] =
-                        (short)(nb[(size_t)r * NB_MAX + j] - r);
+                        (short)(nb[(size_t)r * nbmax + j] - r);
         }
     }
     op->nslice = L.nslice;
@@ -625,15 +642,14 @@ int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *s
     FG_CHECK(fg_ebe_rows_refresh(op));
     constexpr int SPW = FG_TPB / FG_SELL_W;
     const int g = (op->nslice + SPW - 1) / SPW;
-    if (K == 4 && op->d_rcode) {
-        if (fp32)
-            FG_LAUNCH(k_ebe_nrows<float>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
-                      op->d_nsoff, op->d_rcode, op->d_noff, (const float *)op->d_sv32,
-                      (float *)P, (float *)Q, op->neq, skip, sprob, pflag, pstride);
-        else
-            FG_LAUNCH(k_ebe_nrows<double>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
-                      op->d_nsoff, op->d_rcode, op->d_noff, (const double *)op->d_sv64,
-                      (double *)P, (double *)Q, op->neq, skip, sprob, pflag, pstride);
+    if (op->d_rcode) {
+#define FG_NROWS(T, KK, VV)                                                                    \
+    FG_LAUNCH((k_ebe_nrows<T, KK>), g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,          \
+              op->d_nsoff, op->d_rcode, op->d_noff, (const T *)VV, (T *)P, (T *)Q, op->neq,   \
+              skip, sprob, pflag, pstride)
+        if (fp32) FG_NROWS(float, 4, op->d_sv32);
+        else FG_NROWS(double, 4, op->d_sv64);
+#undef FG_NROWS
         return FG_OK;
     }
     if (fp32) {
