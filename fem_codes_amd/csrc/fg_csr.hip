@@ -92,46 +92,6 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_stream(
     if (blockIdx.x == 0 && threadIdx.x == 0) Q[neq] = T(0);
 }
 
-// one CSR row of a slice: the column and value pointers start at the
-// lane-0 slot of the wavefront's slice (wave-uniform bases, q = k*64 +
-// lane); FP64 values entry-paired (fg_pair_shift), merged into 16-byte loads
-template <typename TV>
-struct CsrRow {
-    using T = TV;
-    struct Rec {
-        int c;
-        T v, p;
-    };
-    const int *__restrict__ col;
-    const T *__restrict__ val;
-    const T *__restrict__ x;
-    int w;
-    __device__ __forceinline__ Rec key(unsigned q) const
-    {
-        Rec r;
-        r.c = __builtin_nontemporal_load(&col[q]);
-        r.p = T(0);
-        return r;
-    }
-    __device__ __forceinline__ void stream(Rec &r, unsigned q) const
-    {
-        const unsigned p = sizeof(T) == 8 ? q + fg_pair_shift((int)(q / FG_SELL_W),
-                                                              (int)(q % FG_SELL_W), w)
-                                          : q;
-        r.v = __builtin_nontemporal_load(&val[p]);
-    }
-    __device__ __forceinline__ void gather(Rec &r) const
-    {
-        // a 32-bit byte offset from x (sliced copies are built for neq < 2^28)
-        const unsigned b = (unsigned)(r.c < 0 ? 0 : r.c) * (unsigned)sizeof(T);
-        r.p = *(const T *)((const char *)x + b);
-    }
-    __device__ __forceinline__ T add(T acc, const Rec &r) const
-    {
-        return acc + (r.c < 0 ? T(0) : r.v * r.p);
-    }
-};
-
 // four slices (one per wavefront) per workgroup
 template <typename T>
 __global__ __launch_bounds__(FG_TPB) void k_csr_sliced(int nslice, const int *__restrict__ soff,
@@ -147,19 +107,12 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_sliced(int nslice, const int *__
     if (s < nslice) {
         const int o0 = soff[s], w = soff[s + 1] - o0;
         const long long q0 = (long long)o0 * FG_SELL_W;
-        const CsrRow<T> row{scol + q0, sval + q0, P, w};
+        const fg_sell_csr<T> row{scol + q0, sval + q0, P, w};
         const T acc = fg_sell_run(row, lane, w);
         const int r = s * FG_SELL_W + (int)lane;
         if (r < neq) Q[r] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) Q[neq] = T(0);
-}
-
-// the slot of a CSR entry with its FP64 pair shift in the top byte
-constexpr int FG_CSR_SHIFT_BIT = 56;
-__host__ __device__ inline long long csr_pos_pack(long long q, int shift)
-{
-    return q | ((long long)(shift + 64) << FG_CSR_SHIFT_BIT);
 }
 
 // AA / AA32 (reference order) -> the sliced copies (FP64 entry-paired)
@@ -168,10 +121,8 @@ __global__ void k_csr_mirror(int nnz, const long long *__restrict__ pos,
                              double *__restrict__ s64, float *__restrict__ s32)
 {
     for (int j = blockIdx.x * FG_TPB + threadIdx.x; j < nnz; j += gridDim.x * FG_TPB) {
-        const long long pk = pos[j];
-        const long long q = pk & ((1LL << FG_CSR_SHIFT_BIT) - 1);
-        const int sh = (int)(pk >> FG_CSR_SHIFT_BIT) - 64;
-        s64[q + sh] = AA[j];
+        const long long q = fg_sell_packed_slot(pos[j]);
+        s64[q + fg_sell_packed_shift(pos[j])] = AA[j];
         s32[q] = AA32[j];
     }
 }
@@ -272,7 +223,7 @@ extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, cons
                     const int k = j - IA[r];
                     const long long q = L.slot(r, k);
                     scol[q] = JA[j];
-                    spos[j] = csr_pos_pack(q, fg_pair_shift(k, r % FG_SELL_W, w));
+                    spos[j] = fg_sell_pack(q, fg_pair_shift(k, r % FG_SELL_W, w));
                 }
             }
             op->nslice = L.nslice;

@@ -85,7 +85,8 @@ struct fg_ilu {
         int nslice = 0;
         long long nslot = 0;
         int *d_soff = nullptr, *d_JA = nullptr;
-        long long *d_map = nullptr;   // [nslot] factor slot of each entry, -1: padding
+        long long *d_map = nullptr;   // [nslot] factor slot of each entry (FG_SELL_NOSRC:
+                                      // padding), FP64 pair shift in the top byte
         double *d_v64 = nullptr;
         float *d_v32 = nullptr;
     } lt, ut;
@@ -265,13 +266,22 @@ static int build_tri(int neq, const std::vector<int> &IA, const std::vector<int>
         tIA[i + 1] = tIA[i] + (lower ? lend[i] - IA[i] : IA[i + 1] - ubeg[i]);
     const fg_sell_layout L = fg_sell_plan(neq, tIA.data());
     std::vector<int> col(L.nslot, -1);
-    std::vector<long long> map(L.nslot, -1);
+    // per slot: the factor entry (FG_SELL_NOSRC: padding) and the FP64 pair shift
+    std::vector<long long> map(L.nslot);
+    for (int s = 0; s < L.nslice; s++) {
+        const int w = L.soff[s + 1] - L.soff[s];
+        for (int k = 0; k < w; k++)
+            for (int l = 0; l < FG_SELL_W; l++)
+                map[(long long)(L.soff[s] + k) * FG_SELL_W + l] =
+                    fg_sell_pack(FG_SELL_NOSRC, fg_pair_shift(k, l, w));
+    }
     for (int i = 0; i < neq; i++) {
         const int q0 = lower ? IA[i] : ubeg[i], q1 = lower ? lend[i] : IA[i + 1];
+        const int s = i / FG_SELL_W, w = L.soff[s + 1] - L.soff[s];
         for (int q = q0; q < q1; q++) {
             const long long at = L.slot(i, q - q0);
             col[at] = JA[q];
-            map[at] = q;
+            map[at] = fg_sell_pack(q, fg_pair_shift(q - q0, i % FG_SELL_W, w));
         }
     }
     t.nslice = L.nslice;
@@ -303,9 +313,9 @@ __global__ void k_tri_gather(long long n, const long long *__restrict__ map,
 {
     for (long long t = (long long)blockIdx.x * FG_TPB + threadIdx.x; t < n;
          t += (long long)gridDim.x * FG_TPB) {
-        const long long q = map[t];
-        const double a = q < 0 ? 0.0 : LU[q];
-        v64[t] = a;
+        const long long q = fg_sell_packed_slot(map[t]);
+        const double a = q == FG_SELL_NOSRC ? 0.0 : LU[q];
+        v64[t + fg_sell_packed_shift(map[t])] = a;   // FP64 entry-paired
         v32[t] = (float)a;
     }
 }
@@ -466,16 +476,16 @@ __global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nslice, const int *__r
 {
     if (skip && *skip) return;
     constexpr int SPW = FG_TPB / FG_SELL_W;
-    const int s = fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW +
-                  (int)(threadIdx.x / FG_SELL_W);
-    const int lane = (int)(threadIdx.x % FG_SELL_W);
+    const int s = fg_sell_slice(fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW);
+    const unsigned lane = threadIdx.x % FG_SELL_W;
     if (s < nslice) {
-        const int r = s * FG_SELL_W + lane;
+        const int r = s * FG_SELL_W + (int)lane;
         const int rc = r < n ? r : n - 1;
         const T br = base[rc];
         const T dr = UPPER ? D[rc] : T(0);
-        const int o0 = soff[s];
-        const T acc = fg_sell_row(fg_sell_prod<T>{JA, V, x}, o0, soff[s + 1] - o0, lane);
+        const int o0 = soff[s], w = soff[s + 1] - o0;
+        const long long q0 = (long long)o0 * FG_SELL_W;
+        const T acc = fg_sell_run(fg_sell_csr<T>{JA + q0, V + q0, x, w}, lane, w);
         const T y = br - acc;
         if (r < n) out[r] = UPPER ? dr * y : y;
     }
@@ -535,6 +545,11 @@ static int ilu_sweeps_t(fg_ilu *u, fg_ctx *c, const T *D, const T *p, T *z, T *c
 
 int fg_ilu_set_sweeps(fg_ilu *u, int sweeps)
 {
+    // the sweeps address vectors by 32-bit byte offsets
+    if (sweeps > 0 && u->neq >= (1 << 28)) {
+        fg_set_error("ILU(0) sweeps need fewer than 2^28 equations");
+        return FG_ERR_ARG;
+    }
     u->sweeps = sweeps;
     if (sweeps > 0 && !u->d_s64[0]) {
         const size_t n = (size_t)u->neq + 1;

@@ -136,10 +136,14 @@ __device__ __forceinline__ int fg_sell_slice(int s0)
     return __builtin_amdgcn_readfirstlane(s0 + (int)(threadIdx.x / FG_SELL_W));
 }
 
-// the entry kind of a sparse product row: a column (-1: padding) and a
-// value, the term value * x[column]
+// a sparse product row of a slice (CSR rows, the ILU(0) triangular parts):
+// the column and value pointers start at the lane-0 slot of the wavefront's
+// slice (wave-uniform bases, q = k*64 + lane; column -1: padding); FP64
+// values entry-paired (fg_pair_shift), merged into 16-byte loads.  Columns
+// and values are read once per product: non-temporal, so that they do not
+// evict the x entries neighbouring slices gather again.
 template <typename TV>
-struct fg_sell_prod {
+struct fg_sell_csr {
     using T = TV;
     struct Rec {
         int c;
@@ -148,22 +152,46 @@ struct fg_sell_prod {
     const int *__restrict__ col;
     const T *__restrict__ val;
     const T *__restrict__ x;
-    // columns and values are read once per product: non-temporal, so that
-    // they do not evict the x entries neighbouring slices gather again
-    template <typename I>
-    __device__ __forceinline__ Rec key(I q) const
+    int w;
+    __device__ __forceinline__ Rec key(unsigned q) const
     {
         Rec r;
         r.c = __builtin_nontemporal_load(&col[q]);
         r.p = T(0);
         return r;
     }
-    template <typename I>
-    __device__ __forceinline__ void stream(Rec &r, I q) const
+    __device__ __forceinline__ void stream(Rec &r, unsigned q) const
     {
-        r.v = __builtin_nontemporal_load(&val[q]);
+        const unsigned p = sizeof(T) == 8 ? q + fg_pair_shift((int)(q / FG_SELL_W),
+                                                              (int)(q % FG_SELL_W), w)
+                                          : q;
+        r.v = __builtin_nontemporal_load(&val[p]);
     }
-    __device__ __forceinline__ void gather(Rec &r) const { r.p = x[r.c < 0 ? 0 : r.c]; }
-    __device__ __forceinline__ T term(const Rec &r) const { return r.c < 0 ? T(0) : r.v * r.p; }
-    __device__ __forceinline__ T add(T acc, const Rec &r) const { return acc + term(r); }
+    __device__ __forceinline__ void gather(Rec &r) const
+    {
+        // a 32-bit byte offset from x (sliced copies are built for n < 2^28)
+        const unsigned b = (unsigned)(r.c < 0 ? 0 : r.c) * (unsigned)sizeof(T);
+        r.p = *(const T *)((const char *)x + b);
+    }
+    __device__ __forceinline__ T add(T acc, const Rec &r) const
+    {
+        return acc + (r.c < 0 ? T(0) : r.v * r.p);
+    }
 };
+
+// the slot of an entry with its FP64 pair shift in the top byte (the
+// writers of paired values); FG_SELL_NOSRC in the low bits: no source entry
+constexpr int FG_SELL_SHIFT_BIT = 56;
+constexpr long long FG_SELL_NOSRC = (1LL << FG_SELL_SHIFT_BIT) - 1;
+__host__ __device__ inline long long fg_sell_pack(long long q, int shift)
+{
+    return q | ((long long)(shift + 64) << FG_SELL_SHIFT_BIT);
+}
+__host__ __device__ inline long long fg_sell_packed_slot(long long pk)
+{
+    return pk & FG_SELL_NOSRC;
+}
+__host__ __device__ inline int fg_sell_packed_shift(long long pk)
+{
+    return (int)(pk >> FG_SELL_SHIFT_BIT) - 64;
+}
