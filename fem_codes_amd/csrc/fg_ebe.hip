@@ -725,7 +725,7 @@ int fg_ebe_diag(fg_op *op, double *D)
     fg_ctx *c = op->ctx;
     if (op->neq == 0) return FG_OK;
     // element rows: one equation per lane, the same values in the same order
-    if (op->kind == FG_OP_EBE && op->d_ridx) return fg_ebe_rows_diag(op, D);
+    if (op->kind == FG_OP_EBE && op->d_ridx && op->k <= 4) return fg_ebe_rows_diag(op, D);
     FG_LAUNCH(k_ebe_diag, fg_vec_grid(c, op->neq), FG_TPB, 0, c->stream, op->neq,
               op->d_inc_off, op->d_inc_diag, op->d_K64, D);
     return FG_OK;

@@ -449,6 +449,7 @@ int upload(const std::vector<T> &v, T **d)
 int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
 {
     const int neq = op->neq, nel = op->nitems, K = op->k;
+    if (op->kind == FG_OP_EBE && K == 12) return fg_ebe_node_build(op, lm, kpos);
     if (op->kind != FG_OP_EBE || (K != 3 && K != 4) || neq == 0) return FG_OK;
     // the product addresses P by a 32-bit byte offset
     if (neq >= (1 << 28)) return FG_OK;
@@ -582,6 +583,7 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
 
 int fg_ebe_rows_set_values(fg_op *op, const double *A)
 {
+    if (op->k == 12) return fg_ebe_node_set_values(op, A);
     fg_ctx *c = op->ctx;
     const long long n = (long long)op->nitems * op->k;
     if (n == 0) return FG_OK;
@@ -599,6 +601,7 @@ int fg_ebe_rows_set_values(fg_op *op, const double *A)
 
 int fg_ebe_rows_refresh(fg_op *op)
 {
+    if (op->k == 12) return fg_ebe_node_refresh(op);
     if (!op->d_ridx || !op->rows_stale) return FG_OK;
     fg_ctx *c = op->ctx;
     const long long nt = (long long)op->nslice * FG_SELL_W;
@@ -637,6 +640,8 @@ int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *skip,
                              const int *sprob, const int *pflag, int pstride)
 {
+    // node rows (k = 12): every slice (the mask is an optimisation only)
+    if (op->k == 12) return fg_ebe_node_apply(op, fp32, P, Q, skip);
     fg_ctx *c = op->ctx;
     const int K = op->k;
     FG_CHECK(fg_ebe_rows_refresh(op));
@@ -677,7 +682,7 @@ int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *s
 void fg_ebe_rows_free(fg_op *op)
 {
     hipFree(op->d_ridx); hipFree(op->d_ridx2); hipFree(op->d_rsrc); hipFree(op->d_rvpos);
-    hipFree(op->d_rcode); hipFree(op->d_nsoff); hipFree(op->d_noff);
+    hipFree(op->d_rcode); hipFree(op->d_nsoff); hipFree(op->d_noff); hipFree(op->d_nbase);
     hipFree(op->d_rwst);
     if (op->kind == FG_OP_EBE) {
         hipFree(op->d_soff); hipFree(op->d_sv64); hipFree(op->d_sv32);

@@ -208,6 +208,12 @@ struct fg_op {
     unsigned short *d_rcode = nullptr;
     int *d_nsoff = nullptr;
     short *d_noff = nullptr;
+    // k = 12 node rows (fg_ebe_node.hip): nodes (four consecutive equations
+    // from nbase) per lane; d_ridx then holds one word per node entry,
+    // d_rsrc its element * 3 + node position, d_rvpos per (element, node) the
+    // entry's value block plus the lane (-1: a prescribed node)
+    int nnode = 0;
+    int *d_nbase = nullptr;
     int *d_rsrc = nullptr;
     long long *d_rvpos = nullptr; // [nel*k] FP32 value position of row (e, lr), -1: no row
     int *d_rwst = nullptr;        // [nel*k] fg_rows_wst: column stride, FP64 shift
@@ -386,7 +392,12 @@ int fg_supg2d_transient_build_op(fg_mesh2d *m, const fg_supg2d_problem *pb, doub
                                  double dt, const double *U, const double *dU, fg_op *op,
                                  double *R);
 void fg_ebe_rows_free(fg_op *op);
-// EBE k = 3, 4: the element-row product in use for that precision
+int fg_ebe_node_build(fg_op *op, const int *lm, const long long *kpos);
+int fg_ebe_node_set_values(fg_op *op, const double *A);
+int fg_ebe_node_refresh(fg_op *op);
+int fg_ebe_node_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
+// EBE k = 3, 4 (element rows) and 12 (node rows): the row product in use for
+// that precision
 inline bool fg_ebe_rows_used(const fg_op *op, int fp32)
 {
     return op->kind == FG_OP_EBE && op->d_ridx && op->rows_kind[fp32 ? 1 : 0] != FG_PRODUCT_BLOCKED;

@@ -253,6 +253,7 @@ def test_mfma_rows_fixtures(ctx, path):
     d = load_golden(path)
     neq = int(d["neq"])
     op, pc = _setup(ctx, d)
+    op.set_product_kernel(F.PRODUCT_BLOCKED)   # the matrix cores serve the blocked kernel
     op.set_row_kernel(F.ROWS_MFMA)
     Q = op.matvec(d["P"])
     assert _rel(Q[:neq], d["Q"][:neq]) <= 1e-14 and Q[neq] == 0.0
@@ -291,6 +292,7 @@ def test_mfma_rows_large_mesh(ctx):
     Qo = O.Oracle().ebemv(neq, nel, 12, A, lm, P.copy())
     op = F.Operator.ebe(ctx, neq, lm, 12)
     op.set_values(A)
+    op.set_product_kernel(F.PRODUCT_BLOCKED)
     Qv = op.matvec(P)
     op.set_row_kernel(F.ROWS_MFMA)
     Qm = op.matvec(P)
