@@ -5,10 +5,11 @@ Block_scaling_EBE) and the oracle.
 
 Tolerances: the node blocks, their inverses, the block apply and the block
 scaling are per node / per element in the reference's operation order with
-no FMA contraction: bitwise.  The element product adds a dof's element
-contributions in the blocked (breadth-first) element order and the halo
-partials in block order, not in mesh element order: 1e-14 relative (max
-norm); the ebe2 forms inherit that.  FP32 products: 2e-6 relative.
+no FMA contraction: bitwise.  The element product: the node rows (where
+built) are bitwise (tests/test_gpu_node_rows.py); the blocked kernel adds a
+dof's element contributions in the blocked (breadth-first) element order and
+the halo partials in block order, not in mesh element order: 1e-14 relative
+(max norm), the tolerance kept here for both; the ebe2 forms inherit it.  FP32 products: 2e-6 relative.
 GMRES: the solution within 1e-8 relative of a dense solve of the same
 (block-preconditioned) system."""
 import os
