@@ -110,8 +110,8 @@ def ebe_kernel_name(k, fp32, op=None):
     t = "float" if fp32 else "double"
     if op is not None and op.product_form(fp32) == 2:   # k = 4 neighbour tables
         return f"k_ebe_nrows<{t}, 4>"
-    if op is not None and k == 12 and op.product_sliced(fp32):
-        return f"k_ebe_node_rows<{t}>"
+    if op is not None and k in (8, 12) and op.product_sliced(fp32):
+        return f"k_ebe_node_rows<{t}, {k}>"
     if op is not None and op.product_sliced(fp32):
         return f"k_ebe_rows<{t}, {k}>"
     ept = 4 if k <= 2 else 2 if k == 3 else 4 if k == 4 else 1   # the library's defaults

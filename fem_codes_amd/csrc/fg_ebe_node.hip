@@ -1,25 +1,25 @@
-// fg_ebe_node.hip -- the 4-dof-per-node element product (k = 12) as node
-// rows.
+// fg_ebe_node.hip -- the 4-dof-per-node item products (k = 12: triangles,
+// k = 8: edges) as node rows.
 //
-// Reference: MatrixVector_Operations/EBE/ebemvNDOF4.c:5-112.  For element I
-// with dofs i_1..i_12 (three nodes, four consecutive dofs each) and
-// row-major values A[I], Q[i_r] += A[I][12 r] * p_1 + ... + A[I][12 r + 11] *
-// p_12 (the row sum formed left to right, then added to Q), after Q = 0 and
-// P[neq] = 0: equation r receives one row sum per element that holds it, in
-// element order.
+// Reference: MatrixVector_Operations/EBE/ebemvNDOF4.c:5-112 (k = 12) and
+// EDE/edemvNDOF4.c:4-72 (k = 8).  For item I with dofs i_1..i_k (k/4 nodes,
+// four consecutive dofs each) and row-major values A[I], Q[i_r] += A[I][k r]
+// * p_1 + ... + A[I][k r + k-1] * p_k (the row sum formed left to right, then
+// added to Q), after Q = 0 and P[neq] = 0: equation r receives one row sum
+// per item that holds it, in item order.
 //
 // MI355X design: the four equations of a node move together, so a node is
-// one lane of a wavefront slice (fg_sell.h): its entries are the elements
-// that hold it, in element order, each carrying the node's 4 element rows (48
-// values) and one 32-bit word -- the node's position in the element (2 bits)
-// and the first equations of the element's two other nodes as 15-bit offsets
-// from the lane's own (the Dirichlet sink: -16384).  A lane loads its own
-// node's 4 P values once, streams each entry's 48 values with 16-byte loads,
-// gathers the 8 P values of the other two nodes, forms the 4 row sums left to
-// right in column order and adds them in element order: the reference's
-// ebemvNDOF4 bit for bit (FMA contraction disabled).  No LDS, no halo pass.
+// one lane of a wavefront slice (fg_sell.h): its entries are the items that
+// hold it, in item order, each carrying the node's 4 item rows (4k values)
+// and one 32-bit word -- the node's position in the item (2 bits) and the
+// first equations of the item's other nodes as 15-bit offsets from the
+// lane's own (the Dirichlet sink: -16384).  A lane loads its own node's 4 P
+// values once, streams each entry's values with 16-byte loads, gathers the 4
+// P values of each other node, forms the 4 row sums left to right in column
+// order and adds them in item order: the reference's product bit for bit (FMA
+// contraction disabled).  No LDS, no halo pass.
 //
-// Built when every element's node groups are either four consecutive
+// Built when every item's node groups are either four consecutive
 // equations or all prescribed, the groups do not overlap, every equation lies
 // in one, and every offset fits its field (the Euler drivers number the four
 // variables of a free node consecutively); else the operator keeps the
@@ -33,23 +33,22 @@
 
 namespace {
 
-constexpr int NK = 12;                 // dofs per element
-constexpr int NV = 48;                 // values per entry: 4 rows x 12 columns
-constexpr unsigned NODE_PAD = 0u;      // padding (no entry has two zero offsets)
+// k dofs per item, k/4 nodes, 4k values per entry (4 rows x k columns)
+constexpr unsigned NODE_PAD = 0u;      // padding (no entry has a zero first offset)
 constexpr int NOFF_SINK = -16384;
 constexpr int NOFF_MAX = 16383;
 
 __device__ __forceinline__ int nfield_a(unsigned w) { return ((int)(w << 15)) >> 17; }
 __device__ __forceinline__ int nfield_b(unsigned w) { return ((int)w) >> 17; }
 
-// value c48 = v*12 + c (row v of the node, column c) of an entry whose block
-// of NV*64 values starts at E, for lane l: 16-byte groups, coalesced over the
+// value j = v*k + c (row v of the node, column c) of an entry whose block of
+// 4k*64 values starts at E, for lane l: 16-byte groups, coalesced over the
 // lanes (4 FP32 or 2 FP64 values per group)
 template <typename T>
-__host__ __device__ inline long long node_vpos(long long E, int l, int c48)
+__host__ __device__ inline long long node_vpos(long long E, int l, int j)
 {
     constexpr int VW = 16 / (int)sizeof(T);
-    return E + ((long long)(c48 / VW) * FG_SELL_W + l) * VW + c48 % VW;
+    return E + ((long long)(j / VW) * FG_SELL_W + l) * VW + j % VW;
 }
 
 template <typename T>
@@ -63,7 +62,7 @@ struct VecOf<double> {
     typedef double type __attribute__((ext_vector_type(2)));
 };
 
-template <typename T>
+template <typename T, int NK>
 __global__ __launch_bounds__(FG_TPB) void k_ebe_node_rows(int nslice, int nnode,
                                                           const int *__restrict__ soff,
                                                           const int *__restrict__ nbase,
@@ -75,6 +74,7 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_node_rows(int nslice, int nnode,
     if (skip && *skip) return;
     using TV = typename VecOf<T>::type;
     constexpr int VW = 16 / (int)sizeof(T);
+    constexpr int NV = 4 * NK;
     constexpr int SPW = FG_TPB / FG_SELL_W;
     const int s = fg_sell_slice(fg_xcd_remap(blockIdx.x, (nslice + SPW - 1) / SPW) * SPW);
     const unsigned lane = threadIdx.x % FG_SELL_W;
@@ -97,15 +97,19 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_node_rows(int nslice, int nnode,
 #pragma unroll
                 for (int j = 0; j < VW; j++) a[g * VW + j] = x[j];
             }
-            // the other two nodes' P (0 for the sink); padding reads the own node
-            const int oa = nfield_a(wd), ob = nfield_b(wd);
+            // the other nodes' P (0 for the sink); padding reads the own node
+            const int oa = nfield_a(wd), ob = NK == 12 ? nfield_b(wd) : NOFF_SINK;
             T pa[4], pb[4];
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const T ta = P[(oa == NOFF_SINK ? b : b + oa) + v];
-                const T tb = P[(ob == NOFF_SINK ? b : b + ob) + v];
                 pa[v] = oa == NOFF_SINK ? T(0) : ta;
-                pb[v] = ob == NOFF_SINK ? T(0) : tb;
+                if constexpr (NK == 12) {
+                    const T tb = P[(ob == NOFF_SINK ? b : b + ob) + v];
+                    pb[v] = ob == NOFF_SINK ? T(0) : tb;
+                } else {
+                    pb[v] = T(0);
+                }
             }
             const int a0 = (int)(wd & 3u);
 #pragma unroll
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_node_rows(int nslice, int nnode,
                 T sum = T(0);
 #pragma unroll
                 for (int c = 0; c < NK; c++) {
-                    // column c belongs to the element's node g = c/4: this
+                    // column c belongs to the item's node g = c/4: this
                     // lane's node, else the first or second other node
                     const int g = c / 4;
                     const T pc = g == a0 ? pown[c % 4]
@@ -135,6 +139,7 @@ __global__ __launch_bounds__(FG_TPB) void k_ebe_node_rows(int nslice, int nnode,
 }
 
 // the node rows' values from the blocked values; one thread per (slice, lane)
+template <int NK>
 __global__ void k_ebe_node_mirror(int nslice, const int *__restrict__ soff,
                                   const int *__restrict__ src, const long long *__restrict__ kpos,
                                   const int *__restrict__ kstride, const double *__restrict__ K64,
@@ -150,24 +155,26 @@ __global__ void k_ebe_node_mirror(int nslice, const int *__restrict__ soff,
             const long long q = (long long)(o0 + k) * FG_SELL_W + lane;
             const int sv = src[q];
             if (sv < 0) continue;
-            const int e = sv / 3, a0 = sv - 3 * e;
+            constexpr int NN = NK / 4;
+            const int e = sv / NN, a0 = sv - NN * e;
             const long long p = kpos[e];
             const int st = kstride[e];
-            const long long E = (long long)(o0 + k) * NV * FG_SELL_W;
-            for (int c48 = 0; c48 < NV; c48++) {
-                const int r = 4 * a0 + c48 / NK, c = c48 % NK;
+            const long long E = (long long)(o0 + k) * 4 * NK * FG_SELL_W;
+            for (int j = 0; j < 4 * NK; j++) {
+                const int r = 4 * a0 + j / NK, c = j % NK;
                 const long long vi = fg_ebe_vidx(p, st, r * NK + c, fg_ebe_vec(NK));
-                V64[node_vpos<double>(E, lane, c48)] = K64[vi];
-                V32[node_vpos<float>(E, lane, c48)] = K32[vi];
+                V64[node_vpos<double>(E, lane, j)] = K64[vi];
+                V32[node_vpos<float>(E, lane, j)] = K32[vi];
             }
         }
     }
 }
 
-// AoS element values -> the blocked layout and the node rows; one thread per
-// (element, row): the row's 12 values, written to their blocked positions and
-// (a free node) to the entry of its node (npos: the entry's value block plus
-// the lane)
+// AoS item values -> the blocked layout and the node rows; one thread per
+// (item, row): the row's k values, written to their blocked positions and (a
+// free node) to the entry of its node (npos: the entry's value block plus the
+// lane)
+template <int NK>
 __global__ void k_ebe_node_scatter(long long n, const double *__restrict__ A,
                                    const long long *__restrict__ kpos,
                                    const int *__restrict__ kstride,
@@ -191,15 +198,15 @@ __global__ void k_ebe_node_scatter(long long n, const double *__restrict__ A,
             K64[d] = v[c];
             K32[d] = (float)v[c];
         }
-        const long long np = npos[3 * e + r / 4];
+        const long long np = npos[(NK / 4) * e + r / 4];
         if (np >= 0) {
             const int lane = (int)(np % FG_SELL_W);
             const long long E = np - lane;
 #pragma unroll
             for (int c = 0; c < NK; c++) {
-                const int c48 = (r % 4) * NK + c;
-                V64[node_vpos<double>(E, lane, c48)] = v[c];
-                V32[node_vpos<float>(E, lane, c48)] = (float)v[c];
+                const int j = (r % 4) * NK + c;
+                V64[node_vpos<double>(E, lane, j)] = v[c];
+                V32[node_vpos<float>(E, lane, j)] = (float)v[c];
             }
         }
     }
@@ -218,14 +225,15 @@ int upload(const std::vector<T> &v, T **d)
 
 int fg_ebe_node_build(fg_op *op, const int *lm, const long long *kpos)
 {
-    const int neq = op->neq, nel = op->nitems;
-    if (op->kind != FG_OP_EBE || op->k != NK || neq == 0 || neq >= (1 << 28)) return FG_OK;
+    const int neq = op->neq, nel = op->nitems, NK = op->k, NN = NK / 4, NV = 4 * NK;
+    if (op->kind != FG_OP_EBE || (NK != 8 && NK != 12) || neq == 0 || neq >= (1 << 28))
+        return FG_OK;
     // node groups: four consecutive equations or all prescribed; every
     // equation in exactly one group
     std::vector<int> owner((size_t)neq, -1);
     for (int e = 0; e < nel; e++) {
         if (kpos[e] < 0) continue;
-        for (int a = 0; a < 3; a++) {
+        for (int a = 0; a < NN; a++) {
             const int *g = lm + (size_t)e * NK + 4 * a;
             if (g[0] == neq && g[1] == neq && g[2] == neq && g[3] == neq) continue;
             if (g[0] < 0 || g[0] > neq - 4) return FG_OK;
@@ -249,7 +257,7 @@ int fg_ebe_node_build(fg_op *op, const int *lm, const long long *kpos)
     std::vector<int> cnt((size_t)nnode + 1, 0);
     for (int e = 0; e < nel; e++) {
         if (kpos[e] < 0) continue;
-        for (int a = 0; a < 3; a++) {
+        for (int a = 0; a < NN; a++) {
             const int b = lm[(size_t)e * NK + 4 * a];
             if (b != neq) cnt[nidx[b] + 1]++;
         }
@@ -258,16 +266,16 @@ int fg_ebe_node_build(fg_op *op, const int *lm, const long long *kpos)
     const fg_sell_layout L = fg_sell_plan(nnode, cnt.data());
     std::vector<unsigned> word(L.nslot, NODE_PAD);
     std::vector<int> src(L.nslot, -1);
-    std::vector<long long> npos((size_t)nel * 3, -1);
+    std::vector<long long> npos((size_t)nel * NN, -1);
     std::vector<int> fill((size_t)nnode, 0);
     for (int e = 0; e < nel; e++) {
         if (kpos[e] < 0) continue;
         const int *le = lm + (size_t)e * NK;
-        for (int a = 0; a < 3; a++) {
+        for (int a = 0; a < NN; a++) {
             const int b = le[4 * a];
             if (b == neq) continue;
-            int off[2], m = 0;
-            for (int a2 = 0; a2 < 3; a2++) {
+            int off[2] = {0, 0}, m = 0;
+            for (int a2 = 0; a2 < NN; a2++) {
                 if (a2 == a) continue;
                 const int b2 = le[4 * a2];
                 const long long d = (long long)b2 - b;
@@ -278,8 +286,8 @@ int fg_ebe_node_build(fg_op *op, const int *lm, const long long *kpos)
             const long long q = L.slot(i, k);
             word[q] = (unsigned)a | ((unsigned)(off[0] & 0x7fff) << 2) |
                       ((unsigned)(off[1] & 0x7fff) << 17);
-            src[q] = 3 * e + a;
-            npos[(size_t)e * 3 + a] = (q - i % FG_SELL_W) * NV + i % FG_SELL_W;
+            src[q] = NN * e + a;
+            npos[(size_t)e * NN + a] = (q - i % FG_SELL_W) * NV + i % FG_SELL_W;
         }
     }
     op->nslice = L.nslice;
@@ -302,10 +310,16 @@ int fg_ebe_node_build(fg_op *op, const int *lm, const long long *kpos)
 int fg_ebe_node_set_values(fg_op *op, const double *A)
 {
     fg_ctx *c = op->ctx;
-    const long long n = (long long)op->nitems * NK;
+    const long long n = (long long)op->nitems * op->k;
     if (n == 0) return FG_OK;
-    FG_LAUNCH(k_ebe_node_scatter, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, A, op->d_kpos,
-              op->d_kstride, op->d_rvpos, op->d_K64, op->d_K32, op->d_sv64, op->d_sv32);
+    if (op->k == 12)
+        FG_LAUNCH(k_ebe_node_scatter<12>, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, A,
+                  op->d_kpos, op->d_kstride, op->d_rvpos, op->d_K64, op->d_K32, op->d_sv64,
+                  op->d_sv32);
+    else
+        FG_LAUNCH(k_ebe_node_scatter<8>, fg_vec_grid(c, n), FG_TPB, 0, c->stream, n, A,
+                  op->d_kpos, op->d_kstride, op->d_rvpos, op->d_K64, op->d_K32, op->d_sv64,
+                  op->d_sv32);
     op->rows_stale = 0;
     return FG_OK;
 }
@@ -315,9 +329,14 @@ int fg_ebe_node_refresh(fg_op *op)
     if (!op->d_ridx || !op->rows_stale) return FG_OK;
     fg_ctx *c = op->ctx;
     const long long nt = (long long)op->nslice * FG_SELL_W;
-    FG_LAUNCH(k_ebe_node_mirror, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
-              op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
-              op->d_sv64, op->d_sv32);
+    if (op->k == 12)
+        FG_LAUNCH(k_ebe_node_mirror<12>, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
+                  op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
+                  op->d_sv64, op->d_sv32);
+    else
+        FG_LAUNCH(k_ebe_node_mirror<8>, fg_vec_grid(c, nt), FG_TPB, 0, c->stream, op->nslice,
+                  op->d_soff, op->d_rsrc, op->d_kpos, op->d_kstride, op->d_K64, op->d_K32,
+                  op->d_sv64, op->d_sv32);
     op->rows_stale = 0;
     return FG_OK;
 }
@@ -328,13 +347,17 @@ int fg_ebe_node_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
     FG_CHECK(fg_ebe_node_refresh(op));
     constexpr int SPW = FG_TPB / FG_SELL_W;
     const int g = (op->nslice + SPW - 1) / SPW;
-    if (fp32)
-        FG_LAUNCH(k_ebe_node_rows<float>, g, FG_TPB, 0, c->stream, op->nslice, op->nnode,
-                  op->d_soff, op->d_nbase, op->d_ridx, (const float *)op->d_sv32, (float *)P,
-                  (float *)Q, op->neq, skip);
-    else
-        FG_LAUNCH(k_ebe_node_rows<double>, g, FG_TPB, 0, c->stream, op->nslice, op->nnode,
-                  op->d_soff, op->d_nbase, op->d_ridx, (const double *)op->d_sv64, (double *)P,
-                  (double *)Q, op->neq, skip);
+#define FG_NODE(T, KK, VV)                                                                     \
+    FG_LAUNCH((k_ebe_node_rows<T, KK>), g, FG_TPB, 0, c->stream, op->nslice, op->nnode,       \
+              op->d_soff, op->d_nbase, op->d_ridx, (const T *)VV, (T *)P, (T *)Q, op->neq,    \
+              skip)
+    if (op->k == 12) {
+        if (fp32) FG_NODE(float, 12, op->d_sv32);
+        else FG_NODE(double, 12, op->d_sv64);
+    } else {
+        if (fp32) FG_NODE(float, 8, op->d_sv32);
+        else FG_NODE(double, 8, op->d_sv64);
+    }
+#undef FG_NODE
     return FG_OK;
 }

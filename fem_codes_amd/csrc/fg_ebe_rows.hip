@@ -449,7 +449,7 @@ int upload(const std::vector<T> &v, T **d)
 int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
 {
     const int neq = op->neq, nel = op->nitems, K = op->k;
-    if (op->kind == FG_OP_EBE && K == 12) return fg_ebe_node_build(op, lm, kpos);
+    if (op->kind == FG_OP_EBE && (K == 8 || K == 12)) return fg_ebe_node_build(op, lm, kpos);
     if (op->kind != FG_OP_EBE || (K != 3 && K != 4) || neq == 0) return FG_OK;
     // the product addresses P by a 32-bit byte offset
     if (neq >= (1 << 28)) return FG_OK;
@@ -583,7 +583,7 @@ int fg_ebe_rows_build(fg_op *op, const int *lm, const long long *kpos)
 
 int fg_ebe_rows_set_values(fg_op *op, const double *A)
 {
-    if (op->k == 12) return fg_ebe_node_set_values(op, A);
+    if (op->k == 8 || op->k == 12) return fg_ebe_node_set_values(op, A);
     fg_ctx *c = op->ctx;
     const long long n = (long long)op->nitems * op->k;
     if (n == 0) return FG_OK;
@@ -601,7 +601,7 @@ int fg_ebe_rows_set_values(fg_op *op, const double *A)
 
 int fg_ebe_rows_refresh(fg_op *op)
 {
-    if (op->k == 12) return fg_ebe_node_refresh(op);
+    if (op->k == 8 || op->k == 12) return fg_ebe_node_refresh(op);
     if (!op->d_ridx || !op->rows_stale) return FG_OK;
     fg_ctx *c = op->ctx;
     const long long nt = (long long)op->nslice * FG_SELL_W;
@@ -640,8 +640,8 @@ int fg_ebe_rows_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip)
 int fg_ebe_rows_apply_masked(fg_op *op, int fp32, void *P, void *Q, const int *skip,
                              const int *sprob, const int *pflag, int pstride)
 {
-    // node rows (k = 12): every slice (the mask is an optimisation only)
-    if (op->k == 12) return fg_ebe_node_apply(op, fp32, P, Q, skip);
+    // node rows (k = 8, 12): every slice (the mask is an optimisation only)
+    if (op->k == 8 || op->k == 12) return fg_ebe_node_apply(op, fp32, P, Q, skip);
     fg_ctx *c = op->ctx;
     const int K = op->k;
     FG_CHECK(fg_ebe_rows_refresh(op));

@@ -208,10 +208,10 @@ struct fg_op {
     unsigned short *d_rcode = nullptr;
     int *d_nsoff = nullptr;
     short *d_noff = nullptr;
-    // k = 12 node rows (fg_ebe_node.hip): nodes (four consecutive equations
-    // from nbase) per lane; d_ridx then holds one word per node entry,
-    // d_rsrc its element * 3 + node position, d_rvpos per (element, node) the
-    // entry's value block plus the lane (-1: a prescribed node)
+    // k = 8, 12 node rows (fg_ebe_node.hip): nodes (four consecutive
+    // equations from nbase) per lane; d_ridx then holds one word per node
+    // entry, d_rsrc its item * (k/4) + node position, d_rvpos per (item,
+    // node) the entry's value block plus the lane (-1: a prescribed node)
     int nnode = 0;
     int *d_nbase = nullptr;
     int *d_rsrc = nullptr;
@@ -396,8 +396,8 @@ int fg_ebe_node_build(fg_op *op, const int *lm, const long long *kpos);
 int fg_ebe_node_set_values(fg_op *op, const double *A);
 int fg_ebe_node_refresh(fg_op *op);
 int fg_ebe_node_apply(fg_op *op, int fp32, void *P, void *Q, const int *skip);
-// EBE k = 3, 4 (element rows) and 12 (node rows): the row product in use for
-// that precision
+// EBE k = 3, 4 (element rows) and 8, 12 (node rows): the row product in use
+// for that precision
 inline bool fg_ebe_rows_used(const fg_op *op, int fp32)
 {
     return op->kind == FG_OP_EBE && op->d_ridx && op->rows_kind[fp32 ? 1 : 0] != FG_PRODUCT_BLOCKED;

@@ -1,12 +1,12 @@
-"""Node rows of the 4-dof-per-node element product (k = 12, ebemvNDOF4):
-structured and Delaunay meshes whose free nodes carry four consecutive
-equations (numbered in mesh order or at random), whole nodes prescribed on
-the inflow side and at random, random element values.  The node rows must be
-built and give the reference's ebemvNDOF4 bit for bit in FP64 (the oracle)
-and FP32 (a float32 restatement), after new values (AoS scatter), after
-Block_scaling_EBE (the rows refreshed from the blocked copy), and under the
-batched solver's masked product call; a mesh with a partly prescribed node
-keeps the blocked kernel."""
+"""Node rows of the 4-dof-per-node item products (k = 12 triangles,
+ebemvNDOF4; k = 8 edges, edemvNDOF4): structured and Delaunay meshes whose
+free nodes carry four consecutive equations (numbered in mesh order or at
+random), whole nodes prescribed on the inflow side and at random, random item
+values.  The node rows must be built and give the reference's product bit for
+bit in FP64 (the oracle's row-sum product) and FP32 (a float32 restatement),
+after new values (AoS scatter) and after Block_scaling_EBE (the rows
+refreshed from the blocked copy); a mesh with a partly prescribed node keeps
+the blocked kernel."""
 import numpy as np
 import pytest
 from scipy.spatial import Delaunay
@@ -72,40 +72,56 @@ def _mesh(kind, seed):
     return tri, nn, ids, neq
 
 
+def _edges(tri):
+    """the unique edges of a triangulation, in order of first appearance"""
+    e = np.stack([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]], 1).reshape(-1, 2)
+    key = np.minimum(e[:, 0], e[:, 1]) * (tri.max() + 1) + np.maximum(e[:, 0], e[:, 1])
+    _, first = np.unique(key, return_index=True)
+    return e[np.sort(first)]
+
+
+@pytest.mark.parametrize("k", [12, 8])
 @pytest.mark.parametrize("kind,seed", [("structured", 1), ("delaunay", 2)])
-def test_node_rows_bitwise(ctx, kind, seed):
+def test_node_rows_bitwise(ctx, kind, seed, k):
     tri, nn, ids, neq = _mesh(kind, seed)
-    lm = _lm(tri, ids, neq)
-    nel = len(tri)
+    items = tri if k == 12 else _edges(tri)
+    lm = _lm(items, ids, neq)
+    nel = len(items)
     rng = np.random.default_rng(seed + 10)
     orc = O.Oracle()
-    op = F.Operator.ebe(ctx, neq, lm, 12)
-    A = rng.standard_normal(144 * nel)
+    op = F.Operator.ebe(ctx, neq, lm, k)
+    A = rng.standard_normal(k * k * nel)
     op.set_values(ctx.to_device(A))
     assert op.product_sliced(False) and op.product_sliced(True)
     P = rng.standard_normal(neq + 1)
     P[neq] = 0.0
 
     def check(vals):
-        assert np.array_equal(op.matvec(P), orc.ebemv(neq, nel, 12, vals, lm, P.copy()))
+        assert np.array_equal(op.matvec(P), orc.ebemv(neq, nel, k, vals, lm, P.copy()))
         P32 = P.astype(np.float32)
-        assert np.array_equal(op.matvec(P32, np.float32), _ebemv_f32(neq, lm, vals, P32))
+        assert np.array_equal(op.matvec(P32, np.float32), _ebemv_f32(neq, lm, vals, P32, k))
 
     check(A)
-    A2 = rng.uniform(0.5, 1.5, 144 * nel)
+    A2 = rng.uniform(0.5, 1.5, k * k * nel)
     op.set_values(ctx.to_device(A2))
     check(A2)
-    # Block_scaling_EBE rewrites the blocked values; the node rows follow them
-    ids32 = ids.reshape(-1).astype(np.int32)
-    pc = F.Preconditioner.blockdiag(op, nn, tri.reshape(-1).astype(np.int32), ids32)
-    pc.block_scaling(ctx.to_device(rng.standard_normal(neq + 1)))
+    # a scaling rewrites the blocked values; the node rows follow them
+    # (Block_scaling_EBE for the element operator, which the reference scales
+    # that way only; the Left/Right Diag scaling for the edge operator)
+    if k == 12:
+        ids32 = ids.reshape(-1).astype(np.int32)
+        pc = F.Preconditioner.blockdiag(op, nn, items.reshape(-1).astype(np.int32), ids32)
+        pc.block_scaling(ctx.to_device(rng.standard_normal(neq + 1)))
+    else:
+        pc = F.Preconditioner(op, F.PREC_DIAG)
+        pc.scaling(F.SCALING_LEFTRIGHT, ctx.to_device(np.ones(neq + 1)))
     scaled = op.get_values()
     assert not np.array_equal(scaled, A2)
     check(scaled)
     # the blocked kernel on the same operator: the same product to 1e-14
     op.set_product_kernel(F.PRODUCT_BLOCKED)
     Qb = op.matvec(P)
-    Qo = orc.ebemv(neq, nel, 12, scaled, lm, P.copy())
+    Qo = orc.ebemv(neq, nel, k, scaled, lm, P.copy())
     assert np.max(np.abs(Qb - Qo)) <= 1e-14 * np.max(np.abs(Qo))
     pc.close()
     op.close()
