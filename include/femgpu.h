@@ -160,8 +160,11 @@ int fg_op_set_edge_kernel(fg_op *op, int kind);
  * and k = 4 (ebemv3Dndof1.c:24-42), every element row stored with its
  * equation in element order, 64 equations per wavefront slice, each equation
  * adding its element row sums (formed left to right as the reference does)
- * in element order: the reference's product bit for bit (FP64); for EDE the
- * half-edge rows of FG_EDGE_ROWS.  FG_PRODUCT_BLOCKED: the blocked element
+ * in element order: the reference's product bit for bit (FP64); for k = 12
+ * (ebemvNDOF4.c:5-112) and k = 8 (edemvNDOF4.c:4-72) node rows, one node's
+ * four equations per lane, where every node's dofs are four consecutive
+ * equations or all prescribed; for EDE the half-edge rows of FG_EDGE_ROWS.
+ * FG_PRODUCT_BLOCKED: the blocked element
  * kernel (1e-14).  FG_PRODUCT_AUTO (default): sliced rows where the operator
  * has them.  An element operator gets sliced rows when k is 3 or 4 and every
  * element's other columns lie within 16383 (k = 4: the third within 32767)
