@@ -633,9 +633,19 @@ def batched_tuned(F, ctx, args, N, lm, ke_all, seg, B, X64, tol, default_s, defa
                                     precision=F.GMRES_FP32_IR, graph=not args.no_graph,
                                     orth=ORTHS[args.orth])
         ctx.sync()
+        if rc != F.FG_OK:
+            st = [f.status for f in infos]
+            log(f"batched tuned kmax {km}: status {rc}, problem statuses",
+                {s: st.count(s) for s in set(st)}, "max steps", max(f.steps for f in infos))
         return time.perf_counter() - w0, rc == F.FG_OK, infos
 
-    best_km, km_s = autotune.tune_restart(lambda km: run(km)[:2])
+    try:
+        best_km, km_s = autotune.tune_restart(lambda km: run(km)[:2])
+    except F.FemGpuError as e:
+        log("batched restart tuning:", e)
+        op_t.close()
+        pc_t.close()
+        return {"product": autotune.key_str(key), "error": str(e)}
     log("batched autotune FP32-IR seconds by restart length:", km_s, "->", best_km)
     X = ctx.empty(N + 1)
     sec, ok, infos = run(best_km, X)
