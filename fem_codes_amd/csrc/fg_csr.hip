@@ -96,6 +96,7 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_stream(
 template <typename T>
 __global__ __launch_bounds__(FG_TPB) void k_csr_sliced(int nslice, const int *__restrict__ soff,
                                                        const int *__restrict__ scol,
+                                                       const short *__restrict__ soff16,
                                                        const T *__restrict__ sval,
                                                        const T *__restrict__ P,
                                                        T *__restrict__ Q, int neq, const int *skip)
@@ -107,9 +108,13 @@ __global__ __launch_bounds__(FG_TPB) void k_csr_sliced(int nslice, const int *__
     if (s < nslice) {
         const int o0 = soff[s], w = soff[s + 1] - o0;
         const long long q0 = (long long)o0 * FG_SELL_W;
-        const fg_sell_csr<T> row{scol + q0, sval + q0, P, w};
-        const T acc = fg_sell_run(row, lane, w);
         const int r = s * FG_SELL_W + (int)lane;
+        // columns as 16-bit offsets from the row where they fit, else indices
+        const T acc = soff16
+                          ? fg_sell_run(fg_sell_csr16<T>{soff16 + q0, sval + q0, P,
+                                                         r < neq ? r : neq - 1, w},
+                                        lane, w)
+                          : fg_sell_run(fg_sell_csr<T>{scol + q0, sval + q0, P, w}, lane, w);
         if (r < neq) Q[r] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) Q[neq] = T(0);
@@ -149,12 +154,12 @@ int fg_csr_apply(fg_op *op, int fp32, const void *P, void *Q, const int *skip)
         const int g = (op->nslice + SPW - 1) / SPW;
         if (fp32)
             FG_LAUNCH(k_csr_sliced<float>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
-                      op->d_scol, (const float *)op->d_sv32, (const float *)P, (float *)Q,
-                      op->neq, skip);
+                      op->d_scol, op->d_soff16, (const float *)op->d_sv32, (const float *)P,
+                      (float *)Q, op->neq, skip);
         else
             FG_LAUNCH(k_csr_sliced<double>, g, FG_TPB, 0, c->stream, op->nslice, op->d_soff,
-                      op->d_scol, (const double *)op->d_sv64, (const double *)P, (double *)Q,
-                      op->neq, skip);
+                      op->d_scol, op->d_soff16, (const double *)op->d_sv64, (const double *)P,
+                      (double *)Q, op->neq, skip);
         return FG_OK;
     }
     if (fp32)
@@ -216,6 +221,8 @@ extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, cons
         // (the product addresses P by a 32-bit byte offset)
         if (nnz > 0 && neq < (1 << 28) && L.nslot <= (long long)nnz + nnz / 4 + 4096) {
             std::vector<int> scol(L.nslot, -1);
+            std::vector<short> soff16(L.nslot, FG_SELL_OFF_PAD);
+            bool fits16 = true;
             std::vector<long long> spos(nnz);
             for (int r = 0; r < neq; r++) {
                 const int s = r / FG_SELL_W, w = L.soff[s + 1] - L.soff[s];
@@ -223,6 +230,9 @@ extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, cons
                     const int k = j - IA[r];
                     const long long q = L.slot(r, k);
                     scol[q] = JA[j];
+                    const int d = JA[j] - r;
+                    if (d < -32767 || d > 32767) fits16 = false;
+                    else soff16[q] = (short)d;
                     spos[j] = fg_sell_pack(q, fg_pair_shift(k, r % FG_SELL_W, w));
                 }
             }
@@ -234,6 +244,11 @@ extern "C" int fg_op_csr_create(fg_ctx *c, int neq, int nnz, const int *IA, cons
             FG_HIP(hipMalloc(&op->d_scol, sizeof(int) * L.nslot));
             FG_HIP(hipMemcpy(op->d_scol, scol.data(), sizeof(int) * L.nslot,
                              hipMemcpyHostToDevice));
+            if (fits16) {
+                FG_HIP(hipMalloc(&op->d_soff16, sizeof(short) * L.nslot));
+                FG_HIP(hipMemcpy(op->d_soff16, soff16.data(), sizeof(short) * L.nslot,
+                                 hipMemcpyHostToDevice));
+            }
             FG_HIP(hipMalloc(&op->d_spos, sizeof(long long) * nnz));
             FG_HIP(hipMemcpy(op->d_spos, spos.data(), sizeof(long long) * nnz,
                              hipMemcpyHostToDevice));
@@ -264,7 +279,8 @@ void fg_csr_free(fg_op *op)
 {
     hipFree(op->d_rb); hipFree(op->d_IA); hipFree(op->d_JA); hipFree(op->d_AA); hipFree(op->d_AA32);
     hipFree(op->d_sm_iT);
-    hipFree(op->d_soff); hipFree(op->d_scol); hipFree(op->d_spos); hipFree(op->d_sv64);
+    hipFree(op->d_soff); hipFree(op->d_scol); hipFree(op->d_soff16); hipFree(op->d_spos);
+    hipFree(op->d_sv64);
     hipFree(op->d_sv32);
 }
 

@@ -85,6 +85,7 @@ struct fg_ilu {
         int nslice = 0;
         long long nslot = 0;
         int *d_soff = nullptr, *d_JA = nullptr;
+        short *d_off16 = nullptr;   // columns as 16-bit offsets from the row (else d_JA)
         long long *d_map = nullptr;   // [nslot] factor slot of each entry (FG_SELL_NOSRC:
                                       // padding), FP64 pair shift in the top byte
         double *d_v64 = nullptr;
@@ -266,6 +267,8 @@ static int build_tri(int neq, const std::vector<int> &IA, const std::vector<int>
         tIA[i + 1] = tIA[i] + (lower ? lend[i] - IA[i] : IA[i + 1] - ubeg[i]);
     const fg_sell_layout L = fg_sell_plan(neq, tIA.data());
     std::vector<int> col(L.nslot, -1);
+    std::vector<short> off16(L.nslot, FG_SELL_OFF_PAD);
+    bool fits16 = true;
     // per slot: the factor entry (FG_SELL_NOSRC: padding) and the FP64 pair shift
     std::vector<long long> map(L.nslot);
     for (int s = 0; s < L.nslice; s++) {
@@ -281,6 +284,9 @@ static int build_tri(int neq, const std::vector<int> &IA, const std::vector<int>
         for (int q = q0; q < q1; q++) {
             const long long at = L.slot(i, q - q0);
             col[at] = JA[q];
+            const int d = JA[q] - i;
+            if (d < -32767 || d > 32767) fits16 = false;
+            else off16[at] = (short)d;
             map[at] = fg_sell_pack(q, fg_pair_shift(q - q0, i % FG_SELL_W, w));
         }
     }
@@ -294,6 +300,11 @@ static int build_tri(int neq, const std::vector<int> &IA, const std::vector<int>
     FG_HIP(hipMalloc(&t.d_v32, sizeof(float) * ns));
     FG_HIP(hipMemcpy(t.d_soff, L.soff.data(), sizeof(int) * L.soff.size(),
                      hipMemcpyHostToDevice));
+    if (t.nslot && fits16) {
+        FG_HIP(hipMalloc(&t.d_off16, sizeof(short) * ns));
+        FG_HIP(hipMemcpy(t.d_off16, off16.data(), sizeof(short) * t.nslot,
+                         hipMemcpyHostToDevice));
+    }
     if (t.nslot) {
         FG_HIP(hipMemcpy(t.d_JA, col.data(), sizeof(int) * t.nslot, hipMemcpyHostToDevice));
         FG_HIP(hipMemcpy(t.d_map, map.data(), sizeof(long long) * t.nslot,
@@ -304,7 +315,8 @@ static int build_tri(int neq, const std::vector<int> &IA, const std::vector<int>
 
 static void free_tri(fg_ilu::Tri &t)
 {
-    hipFree(t.d_soff); hipFree(t.d_JA); hipFree(t.d_map); hipFree(t.d_v64); hipFree(t.d_v32);
+    hipFree(t.d_soff); hipFree(t.d_JA); hipFree(t.d_off16); hipFree(t.d_map); hipFree(t.d_v64);
+    hipFree(t.d_v32);
 }
 
 __global__ void k_tri_gather(long long n, const long long *__restrict__ map,
@@ -468,6 +480,7 @@ static int ilu_apply_t(fg_ilu *u, fg_ctx *c, const T *LU, const T *D, const T *p
 template <typename T, bool UPPER>
 __global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nslice, const int *__restrict__ soff,
                                                       const int *__restrict__ JA,
+                                                      const short *__restrict__ off16,
                                                       const T *__restrict__ V,
                                                       const T *__restrict__ x,
                                                       const T *__restrict__ base,
@@ -485,7 +498,9 @@ __global__ __launch_bounds__(FG_TPB) void k_tri_sweep(int nslice, const int *__r
         const T dr = UPPER ? D[rc] : T(0);
         const int o0 = soff[s], w = soff[s + 1] - o0;
         const long long q0 = (long long)o0 * FG_SELL_W;
-        const T acc = fg_sell_run(fg_sell_csr<T>{JA + q0, V + q0, x, w}, lane, w);
+        // columns as 16-bit offsets from the row where they fit, else indices
+        const T acc = off16 ? fg_sell_run(fg_sell_csr16<T>{off16 + q0, V + q0, x, rc, w}, lane, w)
+                            : fg_sell_run(fg_sell_csr<T>{JA + q0, V + q0, x, w}, lane, w);
         const T y = br - acc;
         if (r < n) out[r] = UPPER ? dr * y : y;
     }
@@ -526,8 +541,8 @@ static int ilu_sweeps_t(fg_ilu *u, fg_ctx *c, const T *D, const T *p, T *z, T *c
         T *out = s[m & 1];
         if (L.nslice)
             FG_LAUNCH((k_tri_sweep<T, false>), sweep_grid(L), FG_TPB, 0, c->stream, L.nslice,
-                      (const int *)L.d_soff, (const int *)L.d_JA, tri_vals<T>(L), y, p, D, out,
-                      n, skip);
+                      (const int *)L.d_soff, (const int *)L.d_JA, (const short *)L.d_off16,
+                      tri_vals<T>(L), y, p, D, out, n, skip);
         y = out;
     }
     // backward: x_0..x_S between the free scratch and z, x_S in z (p is no
@@ -538,8 +553,8 @@ static int ilu_sweeps_t(fg_ilu *u, fg_ctx *c, const T *D, const T *p, T *z, T *c
     for (int m = 1; m <= S; m++)
         if (U.nslice)
             FG_LAUNCH((k_tri_sweep<T, true>), sweep_grid(U), FG_TPB, 0, c->stream, U.nslice,
-                      (const int *)U.d_soff, (const int *)U.d_JA, tri_vals<T>(U),
-                      (const T *)buf[(m - 1) & 1], y, D, buf[m & 1], n, skip);
+                      (const int *)U.d_soff, (const int *)U.d_JA, (const short *)U.d_off16,
+                      tri_vals<T>(U), (const T *)buf[(m - 1) & 1], y, D, buf[m & 1], n, skip);
     return FG_OK;
 }
 

@@ -193,6 +193,7 @@ struct fg_op {
     int nslice = 0;
     long long nslot = 0;
     int *d_soff = nullptr, *d_scol = nullptr;
+    short *d_soff16 = nullptr;   // CSR: columns as 16-bit offsets from the row (when all fit)
     long long *d_spos = nullptr; // CSR: slot of each entry, FP64 pair shift in the top
                                  // byte (null: row-block product)
     double *d_sv64 = nullptr;

@@ -180,8 +180,11 @@ extern "C" double fg_op_algorithmic_bytes(const fg_op *op, int fp32)
     double s = fp32 ? 4.0 : 8.0;
     double neq = op->neq;
     if (op->kind == FG_OP_EBE2) return fg_ebe2_bytes(op, fp32);
-    if (op->kind == FG_OP_CSR)
+    if (op->kind == FG_OP_CSR) {
+        // the 16-bit column form moves 2 index bytes per entry and no IA
+        if (op->d_spos && op->d_soff16) return (double)op->nnz * (s + 2.0) + 2.0 * neq * s;
         return (double)op->nnz * (s + 4.0) + 4.0 * (neq + 1.0) + 2.0 * neq * s;
+    }
     double n = op->nitems, k = op->k;
     if (op->kind == FG_OP_EDE) return n * (4.0 * s + 8.0) + 2.0 * neq * s;
     return n * (k * k * s + 4.0 * k) + 2.0 * neq * s;

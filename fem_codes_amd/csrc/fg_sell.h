@@ -179,6 +179,47 @@ struct fg_sell_csr {
     }
 };
 
+// the same row with each column stored as a 16-bit offset from the lane's
+// row (FG_SELL_OFF_PAD: padding): 2 bytes of index per entry instead of 4,
+// for patterns whose columns all lie within 32767 of their row
+constexpr short FG_SELL_OFF_PAD = -32768;
+template <typename TV>
+struct fg_sell_csr16 {
+    using T = TV;
+    struct Rec {
+        int o;
+        T v, p;
+    };
+    const short *__restrict__ off;
+    const T *__restrict__ val;
+    const T *__restrict__ x;
+    int r;   // the lane's row (any row of the slice for idle lanes)
+    int w;
+    __device__ __forceinline__ Rec key(unsigned q) const
+    {
+        Rec e;
+        e.o = __builtin_nontemporal_load(&off[q]);
+        e.p = T(0);
+        return e;
+    }
+    __device__ __forceinline__ void stream(Rec &e, unsigned q) const
+    {
+        const unsigned p = sizeof(T) == 8 ? q + fg_pair_shift((int)(q / FG_SELL_W),
+                                                              (int)(q % FG_SELL_W), w)
+                                          : q;
+        e.v = __builtin_nontemporal_load(&val[p]);
+    }
+    __device__ __forceinline__ void gather(Rec &e) const
+    {
+        const unsigned b = (unsigned)(e.o == FG_SELL_OFF_PAD ? r : r + e.o) * (unsigned)sizeof(T);
+        e.p = *(const T *)((const char *)x + b);
+    }
+    __device__ __forceinline__ T add(T acc, const Rec &e) const
+    {
+        return acc + (e.o == FG_SELL_OFF_PAD ? T(0) : e.v * e.p);
+    }
+};
+
 // the slot of an entry with its FP64 pair shift in the top byte (the
 // writers of paired values); FG_SELL_NOSRC in the low bits: no source entry
 constexpr int FG_SELL_SHIFT_BIT = 56;

@@ -117,7 +117,10 @@ int fg_op_apply_f32(fg_op *op, float *P, float *Q);
 int fg_op_info(const fg_op *op, int *neq, int *nitems, int *k, long long *nnz);
 /* bytes one matvec must move (compulsory traffic, see metric definition):
  * EBE nel(k^2 s + 4k) + 2 neq s ; EDE nedge(4s+8) + 2 neq s ;
- * CSR nnz(s+4) + 4(neq+1) + 2 neq s ; s = 8 (f64) or 4 (f32) */
+ * CSR nnz(s+4) + 4(neq+1) + 2 neq s, or nnz(s+2) + 2 neq s where the
+ * operator stores its columns as 16-bit offsets from the row (every column
+ * within 32767 of its row; the sliced product then reads 2 bytes of index
+ * per entry and no row pointers) ; s = 8 (f64) or 4 (f32) */
 double fg_op_algorithmic_bytes(const fg_op *op, int fp32);
 /* time `reps` back-to-back applies with HIP events on the context stream;
  * returns the mean milliseconds per apply in *ms.  main_only = 1 times the

@@ -216,11 +216,17 @@ def test_gmres_ebe_k4_with_csr_ilu0(ctx, precision):
     assert np.linalg.norm(X.get()[:neq] - Xd) <= 1e-8 * np.linalg.norm(Xd)
 
 
-def test_ilu0_sweeps_reach_the_exact_solve(ctx):
+@pytest.mark.parametrize("n", [8, 34])
+def test_ilu0_sweeps_reach_the_exact_solve(ctx, n):
     """s Jacobi sweeps per triangle equal the exact triangular solves once s
     reaches the number of dependency levels (the sweep iteration matrix is
-    nilpotent of that order); fewer sweeps differ"""
-    m, lm, ke, Fv, csr, IA, JA = _system3d(ctx, 8, perm_seed=5)
+    nilpotent of that order); fewer sweeps differ.  n = 34 (35,937 equations,
+    randomly numbered): columns beyond 32767 of their row, so the sweeps read
+    full column indices instead of 16-bit offsets"""
+    m, lm, ke, Fv, csr, IA, JA = _system3d(ctx, n, perm_seed=5)
+    if n == 34:
+        rows = np.repeat(np.arange(m.neq), np.diff(IA))
+        assert np.max(np.abs(JA - rows)) > 32767
     neq = m.neq
     pc = F.Preconditioner(csr, F.PREC_ILU0)
     pc.setup(None)

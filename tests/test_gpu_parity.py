@@ -125,6 +125,34 @@ def test_csr_row_shapes(ctx, shape):
         assert Q32[r] == acc, r
 
 
+def test_csr_wide_band(ctx):
+    """columns beyond 32767 of their row: the sliced rows read full column
+    indices instead of 16-bit offsets; FP64 bitwise against csrmv, FP32
+    against a float32 row loop"""
+    rng = np.random.default_rng(12)
+    n = 70000
+    lens = np.full(n, 7)
+    IA = np.zeros(n + 1, np.int32)
+    IA[1:] = np.cumsum(lens)
+    JA = np.concatenate([np.sort(rng.choice(n, size=7, replace=False)) for _ in range(n)]
+                        ).astype(np.int32)
+    rows = np.repeat(np.arange(n), lens)
+    assert np.max(np.abs(JA - rows)) > 32767
+    AA = rng.standard_normal(int(IA[-1]))
+    P = rng.standard_normal(n + 1)
+    op = F.Operator.csr(ctx, n, IA, JA)
+    op.set_values(AA)
+    assert np.array_equal(op.matvec(P)[:n], O.Oracle().csrmv(n, AA, JA, IA, P.copy())[:n])
+    A32, P32 = AA.astype(np.float32), P.astype(np.float32)
+    Q32 = op.matvec(P32, np.float32)
+    for r in range(0, n, 997):
+        acc = np.float32(0)
+        for j in range(IA[r], IA[r + 1]):
+            acc = np.float32(acc + np.float32(A32[j] * P32[JA[j]]))
+        assert Q32[r] == acc, r
+    op.close()
+
+
 @pytest.mark.parametrize("path", golden_files("ss2d_n16_*.npz"), ids=os.path.basename)
 def test_matvec_f32(ctx, path, orc):
     """FP32 product against the FP64 reference product: the error is bounded
